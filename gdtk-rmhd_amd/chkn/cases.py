@@ -1,0 +1,209 @@
+"""Synthetic chicken workloads (all FP64, ideal air gamma=1.4, R=287.1).
+
+* ``sod_box``        -- 3D Sod-type shock tube in a slip-walled box.
+* ``ramp``           -- Mach 5 flow over a 15-degree compression ramp with a seeded 1%
+                        density perturbation (sum of 8 Fourier modes).
+* ``ramp_array``     -- the ramp split into an array of tensor-product uneven blocks with a
+                        seeded exchange/wall connection graph.
+* ``mixed_bc_array`` -- a small block array exercising every built boundary condition.
+
+Each function describes the job through chkn.prep and returns ``JobData``.
+"""
+import math
+
+import numpy as np
+
+from . import prep
+from .geom import StructuredGrid, TFIVolume, Vector3, cell_centres
+from .job import JobData
+
+GAMMA, R_GAS = 1.4, 287.1
+CV = R_GAS / (GAMMA - 1.0)
+
+
+def _state_arrays(p, T, velx, vely=0.0, velz=0.0, rho=None):
+    """Per-cell flow arrays with the prep's gas-state formulas (chkn_prep.py:328-334)."""
+    p = np.asarray(p, dtype=float)
+    T = np.asarray(T, dtype=float)
+    shape = np.broadcast(p, T, np.asarray(velx)).shape
+    out = {"p": np.broadcast_to(p, shape), "T": np.broadcast_to(T, shape)}
+    out["e"] = CV * out["T"] + 0.0 * 0.0
+    out["rho"] = out["p"] / (R_GAS * out["T"]) if rho is None else np.broadcast_to(rho, shape)
+    out["a"] = np.sqrt(GAMMA * R_GAS * out["T"])
+    out["YB"] = np.zeros(shape)
+    out["velx"] = np.broadcast_to(np.asarray(velx, dtype=float), shape)
+    out["vely"] = np.broadcast_to(np.asarray(vely, dtype=float), shape)
+    out["velz"] = np.broadcast_to(np.asarray(velz, dtype=float), shape)
+    return {k: np.ascontiguousarray(v, dtype=float) for k, v in out.items()}
+
+
+def _common(max_step, cfl=0.5, dt_init=1.0e-6, max_time=1.0, title=""):
+    prep.reset()
+    c = prep.config
+    c.title = title
+    c.max_step = max_step
+    c.max_time = max_time
+    c.dt_init = dt_init
+    c.cfl_count = 10
+    c.x_order = 2
+    c.flux_calc = "ausmdv"
+    prep.add_cfl_value(0.0, cfl)
+    prep.add_dt_plot(0.0, max_time, max_time)
+
+
+def box_grid(nic, njc, nkc, L=(1.0, 0.5, 0.5), origin=(0.0, 0.0, 0.0)):
+    x0, y0, z0 = origin
+    Lx, Ly, Lz = L
+    vol = TFIVolume(p000=Vector3(x0, y0, z0), p100=Vector3(x0 + Lx, y0, z0),
+                    p110=Vector3(x0 + Lx, y0 + Ly, z0), p010=Vector3(x0, y0 + Ly, z0),
+                    p001=Vector3(x0, y0, z0 + Lz), p101=Vector3(x0 + Lx, y0, z0 + Lz),
+                    p111=Vector3(x0 + Lx, y0 + Ly, z0 + Lz), p011=Vector3(x0, y0 + Ly, z0 + Lz))
+    return StructuredGrid(pvolume=vol, niv=nic + 1, njv=njc + 1, nkv=nkc + 1)
+
+
+def sod_box(nic=64, njc=32, nkc=32, max_step=200, axis=0, title="3D Sod shock tube"):
+    """Sod-type tube along `axis`: x<0.5 p=1e5 Pa T=348 K, else p=1e4 Pa T=278 K, at rest,
+    slip walls on all six faces."""
+    _common(max_step, title=title)
+    dims = [nic, njc, nkc]
+    L = [0.5, 0.5, 0.5]
+    L[axis] = 1.0
+    grid = box_grid(*dims, L=tuple(L))
+    cc = cell_centres(grid)
+    coord = (cc.x, cc.y, cc.z)[axis]
+    high = coord < 0.5
+    st = _state_arrays(np.where(high, 1.0e5, 1.0e4), np.where(high, 348.0, 278.0), 0.0)
+    prep.FluidBlock(i=0, j=0, k=0, grid=grid, initialState=st)
+    return JobData.from_prep()
+
+
+def ramp_vertices(NI, NJ, NK, length=1.0, corner=0.25, angle_deg=15.0, top=0.5, span=0.25):
+    """Vertices of the ramp domain: uniform in x and y; in z uniform between the wall
+    z_w(x) = max(0, (x - corner) tan(angle)) and the top."""
+    x = np.linspace(0.0, length, NI + 1)
+    y = np.linspace(0.0, span, NJ + 1)
+    t = np.linspace(0.0, 1.0, NK + 1)
+    zw = np.maximum(0.0, (x - corner) * math.tan(math.radians(angle_deg)))
+    X = np.broadcast_to(x[:, None, None], (NI + 1, NJ + 1, NK + 1))
+    Y = np.broadcast_to(y[None, :, None], (NI + 1, NJ + 1, NK + 1))
+    Z = zw[:, None, None] + t[None, None, :] * (top - zw)[:, None, None]
+    Z = np.broadcast_to(Z, (NI + 1, NJ + 1, NK + 1))
+    return StructuredGrid(x=X.copy(), y=Y.copy(), z=Z.copy())
+
+
+def freestream(mach=5.0, p=2.0e3, T=220.0):
+    a = math.sqrt(GAMMA * R_GAS * T)
+    return prep.FlowState(p=p, T=T, velx=mach * a)
+
+
+def perturbed_state(grid, fs, seed=1, amplitude=0.01, nmodes=8):
+    """Freestream with rho = rho_inf (1 + amplitude * S / sum|a_m|), p held fixed."""
+    rng = np.random.default_rng(seed)
+    kvec = rng.integers(1, 5, size=(nmodes, 3))
+    amp = rng.uniform(0.5, 1.0, size=nmodes)
+    phase = rng.uniform(0.0, 2.0 * math.pi, size=nmodes)
+    cc = cell_centres(grid)
+    S = np.zeros_like(cc.x)
+    for m in range(nmodes):
+        S += amp[m] * np.sin(2.0 * math.pi * (kvec[m, 0] * cc.x + kvec[m, 1] * cc.y + kvec[m, 2] * cc.z)
+                             + phase[m])
+    rho_inf = fs.gas.p / (R_GAS * fs.gas.T)
+    rho = rho_inf * (1.0 + amplitude * S / amp.sum())
+    T = fs.gas.p / (R_GAS * rho)
+    return _state_arrays(fs.gas.p, T, fs.vel.x, rho=rho)
+
+
+def ramp(nic=256, njc=128, nkc=128, max_step=500, seed=1, mach=5.0, p=2.0e3, T=220.0,
+         title="Mach 5 flow over a 15-degree ramp"):
+    _common(max_step, title=title + " (seed %d)" % seed)
+    grid = ramp_vertices(nic, njc, nkc)
+    fs = freestream(mach, p, T)
+    st = perturbed_state(grid, fs, seed=seed)
+    bcs = {"iminus": prep.InflowBC(fs), "iplus": prep.OutflowBC(), "kminus": prep.WallWithSlipBC(),
+           "kplus": prep.OutflowBC(), "jminus": prep.WallWithSlipBC(), "jplus": prep.WallWithSlipBC()}
+    prep.FluidBlock(i=0, j=0, k=0, grid=grid, initialState=st, bcs=bcs)
+    return JobData.from_prep()
+
+
+def uneven_sizes(seed, nib=5, njb=4, nkb=2, lo=16, hi=96):
+    rng = np.random.default_rng(seed)
+    return (rng.integers(lo, hi + 1, size=nib), rng.integers(lo, hi + 1, size=njb),
+            rng.integers(lo, hi + 1, size=nkb))
+
+
+def ramp_array(seed=7, nib=5, njb=4, nkb=2, lo=16, hi=96, max_step=300, p_exchange=0.8,
+               sizes=None, title="Mach 5 ramp on uneven blocks"):
+    """Ramp flow on an nib x njb x nkb array of blocks whose slab sizes are seeded in
+    [lo, hi]; each internal interface is an exchange pair with probability p_exchange,
+    else a pair of slip walls."""
+    _common(max_step, title=title + " (seed %d)" % seed)
+    nics, njcs, nkcs = sizes if sizes is not None else uneven_sizes(seed, nib, njb, nkb, lo, hi)
+    nib, njb, nkb = len(nics), len(njcs), len(nkcs)
+    rng = np.random.default_rng(seed + 1000)
+    NI, NJ, NK = int(sum(nics)), int(sum(njcs)), int(sum(nkcs))
+    grid = ramp_vertices(NI, NJ, NK)
+    fs = freestream()
+    st = perturbed_state(grid, fs, seed=seed)
+    # Interface kinds: True = exchange.  Index [dir][i][j][k] for the plus face of block ijk.
+    link = {d: rng.random((nib, njb, nkb)) < p_exchange for d in range(3)}
+    io = np.concatenate([[0], np.cumsum(nics)])
+    jo = np.concatenate([[0], np.cumsum(njcs)])
+    ko = np.concatenate([[0], np.cumsum(nkcs)])
+    for k in range(nkb):
+        for j in range(njb):
+            for i in range(nib):
+                bcs = {}
+                bcs["iminus"] = prep.InflowBC(fs) if i == 0 else (
+                    prep.ExchangeBC() if link[0][i - 1, j, k] else prep.WallWithSlipBC())
+                bcs["iplus"] = prep.OutflowBC() if i == nib - 1 else (
+                    prep.ExchangeBC() if link[0][i, j, k] else prep.WallWithSlipBC())
+                bcs["jminus"] = prep.WallWithSlipBC() if j == 0 else (
+                    prep.ExchangeBC() if link[1][i, j - 1, k] else prep.WallWithSlipBC())
+                bcs["jplus"] = prep.WallWithSlipBC() if j == njb - 1 else (
+                    prep.ExchangeBC() if link[1][i, j, k] else prep.WallWithSlipBC())
+                bcs["kminus"] = prep.WallWithSlipBC() if k == 0 else (
+                    prep.ExchangeBC() if link[2][i, j, k - 1] else prep.WallWithSlipBC())
+                bcs["kplus"] = prep.OutflowBC() if k == nkb - 1 else (
+                    prep.ExchangeBC() if link[2][i, j, k] else prep.WallWithSlipBC())
+                sub = grid.subgrid(i0=io[i], j0=jo[j], k0=ko[k], niv=nics[i] + 1, njv=njcs[j] + 1,
+                                   nkv=nkcs[k] + 1)
+                init = {key: v[io[i]:io[i + 1], jo[j]:jo[j + 1], ko[k]:ko[k + 1]] if np.ndim(v) == 3 else v
+                        for key, v in st.items()}
+                prep.FluidBlock(i=i, j=j, k=k, grid=sub, initialState=init, bcs=bcs)
+    return JobData.from_prep()
+
+
+def mixed_bc_array(max_step=20, seed=3, title="mixed boundary conditions"):
+    """A 3x2x2 array of small warped blocks using inflow, outflow, slip walls, exchange
+    (including a periodic wrap in j) and an inactive block."""
+    _common(max_step, title=title, cfl=0.4)
+    nics, njcs, nkcs = [6, 5, 4], [4, 6], [5, 3]
+    NI, NJ, NK = sum(nics), sum(njcs), sum(nkcs)
+    rng = np.random.default_rng(seed)
+    g = ramp_vertices(NI, NJ, NK, angle_deg=10.0)
+    # Warp the interior vertices a little so that faces are not planar.
+    jit = 0.15 / max(NI, NJ, NK)
+    for a in (g.vertices.x, g.vertices.y, g.vertices.z):
+        a[1:-1, 1:-1, 1:-1] += jit * rng.uniform(-1, 1, size=a[1:-1, 1:-1, 1:-1].shape)
+    fs = freestream(mach=2.5, p=5.0e3, T=260.0)
+    st = perturbed_state(g, fs, seed=seed, amplitude=0.05)
+    st["vely"] = st["velx"] * 0.1
+    st["velz"] = -st["velx"] * 0.05
+    io, jo, ko = np.cumsum([0] + nics), np.cumsum([0] + njcs), np.cumsum([0] + nkcs)
+    for k in range(2):
+        for j in range(2):
+            for i in range(3):
+                bcs = {"iminus": prep.InflowBC(fs) if i == 0 else prep.ExchangeBC(),
+                       "iplus": prep.OutflowBC() if i == 2 else prep.ExchangeBC(),
+                       "jminus": prep.ExchangeBC(), "jplus": prep.ExchangeBC(),
+                       "kminus": prep.WallWithSlipBC() if k == 0 else prep.ExchangeBC(),
+                       "kplus": prep.OutflowBC() if k == 1 else prep.ExchangeBC()}
+                active = not (i == 2 and j == 1 and k == 1)
+                if not active:
+                    bcs = {}
+                # Blocks next to the inactive one see it through exchange (its state is frozen).
+                sub = g.subgrid(i0=io[i], j0=jo[j], k0=ko[k], niv=nics[i] + 1, njv=njcs[j] + 1,
+                                nkv=nkcs[k] + 1)
+                init = {key: v[io[i]:io[i + 1], jo[j]:jo[j + 1], ko[k]:ko[k + 1]] for key, v in st.items()}
+                prep.FluidBlock(i=i, j=j, k=k, grid=sub, initialState=init, bcs=bcs, active=active)
+    return JobData.from_prep()

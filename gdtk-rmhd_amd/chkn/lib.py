@@ -1,0 +1,218 @@
+"""ctypes binding of libchkn.so (include/chkn.h), the HIP/gfx950 chicken step.
+
+There is no fallback: if the shared library is missing this module raises at import of
+``load()``, so nothing silently runs on the CPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libchkn.so")
+
+_I = ctypes.POINTER(ctypes.c_int)
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+class ChknDesc(ctypes.Structure):
+    _fields_ = [("nblocks", ctypes.c_int), ("nib", ctypes.c_int), ("njb", ctypes.c_int),
+                ("nkb", ctypes.c_int), ("blk_ids", _I), ("blk_ijk", _I), ("blk_dims", _I),
+                ("active", _I), ("bc_codes", _I), ("bc_fs", _I), ("n_flow_states", ctypes.c_int),
+                ("flow_states", _D), ("vertices", _D), ("iovars", _D), ("x_order", ctypes.c_int),
+                ("flux_calc", ctypes.c_int), ("t_order", ctypes.c_int), ("cfl_count", ctypes.c_int),
+                ("max_step", ctypes.c_int), ("print_count", ctypes.c_int),
+                ("max_time", ctypes.c_double), ("dt_init", ctypes.c_double),
+                ("n_cfl", ctypes.c_int), ("cfl_times", _D), ("cfl_values", _D),
+                ("n_dt_plot", ctypes.c_int), ("dt_plot_times", _D), ("dt_plot_values", _D),
+                ("t0", ctypes.c_double), ("start_tindx", ctypes.c_int), ("reacting", ctypes.c_int),
+                ("viscous", ctypes.c_int)]
+
+
+class ChknReport(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_int), ("t", ctypes.c_double), ("dt", ctypes.c_double),
+                ("bad_stage", ctypes.c_int), ("bad_count", ctypes.c_int),
+                ("steps_done", ctypes.c_int)]
+
+
+class ChknLaunchOpts(ctypes.Structure):
+    _fields_ = [("tile_i", ctypes.c_int), ("tile_j", ctypes.c_int), ("tile_k", ctypes.c_int),
+                ("per_block_launch", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("persistent", ctypes.c_int)]
+
+
+EXPORTS = {
+    "chkn_create": (ctypes.c_int, [ctypes.POINTER(ChknDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "chkn_load_job": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_void_p)]),
+    "chkn_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ChknReport)]),
+    "chkn_march": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ChknReport)]),
+    "chkn_finalize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "chkn_write_flow": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "chkn_get_iovars": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    "chkn_get_conserved": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    "chkn_set_iovars": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    "chkn_set_launch_opts": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ChknLaunchOpts)]),
+    "chkn_get_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ChknReport)]),
+    "chkn_num_blocks": (ctypes.c_int, [ctypes.c_void_p]),
+    "chkn_block_dims": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I]),
+    "chkn_active_cells": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "chkn_last_timing": (ctypes.c_int, [ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_longlong)]),
+    "chkn_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "chkn_destroy": (None, [ctypes.c_void_p]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libchkn.so and declare its C-ABI; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libchkn.so is not built (expected at %s); run "
+                               "__graft_entry__.build() or make -C gdtk-rmhd_amd" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class ChknError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+class BadCellError(ChknError):
+    """A stage produced bad cells: the reference's "Stage k bad cell count: n"."""
+
+    def __init__(self, stage, count, msg):
+        super().__init__(count, msg)
+        self.stage = stage
+        self.count = count
+
+
+def _ptr(a, typ):
+    return a.ctypes.data_as(typ)
+
+
+class Sim:
+    """One simulation handle (chkn_sim)."""
+
+    def __init__(self, job=None, job_dir=None, tindx=0, binary=True):
+        lib = load()
+        h = ctypes.c_void_p()
+        if job_dir is not None:
+            rc = lib.chkn_load_job(os.fsencode(job_dir), tindx, 1 if binary else 0, ctypes.byref(h))
+        else:
+            f = job.flat()
+            self._keep = f
+            d = ChknDesc()
+            d.nblocks = len(job.blocks)
+            d.nib, d.njb, d.nkb = job.nib, job.njb, job.nkb
+            for key in ("blk_ids", "blk_ijk", "blk_dims", "active", "bc_codes", "bc_fs"):
+                setattr(d, key, _ptr(f[key], _I))
+            d.n_flow_states = len(job.flow_states)
+            for key in ("flow_states", "vertices", "iovars", "cfl_times", "cfl_values",
+                        "dt_plot_times", "dt_plot_values"):
+                setattr(d, key, _ptr(f[key], _D))
+            d.x_order, d.flux_calc, d.t_order = job.x_order, job.flux_calc, job.t_order
+            d.cfl_count, d.max_step, d.print_count = job.cfl_count, job.max_step, job.print_count
+            d.max_time, d.dt_init = job.max_time, job.dt_init
+            d.n_cfl = len(job.cfl_times)
+            d.n_dt_plot = len(job.dt_plot_times)
+            d.t0, d.start_tindx = job.t0, job.start_tindx
+            d.reacting, d.viscous = int(job.reacting), int(job.viscous)
+            rc = lib.chkn_create(ctypes.byref(d), ctypes.byref(h))
+        if rc != 0:
+            raise ChknError(rc, lib.chkn_last_error(None).decode())
+        self._lib = lib
+        self.h = h
+        self.nblocks = lib.chkn_num_blocks(h)
+
+    def _check(self, rc, rep=None):
+        if rc > 0 and rep is not None and rep.bad_stage:
+            raise BadCellError(rep.bad_stage, rep.bad_count, self._lib.chkn_last_error(self.h).decode())
+        if rc != 0:
+            raise ChknError(rc, self._lib.chkn_last_error(self.h).decode())
+
+    def step(self, nsteps=1):
+        rep = ChknReport()
+        self._check(self._lib.chkn_step(self.h, int(nsteps), ctypes.byref(rep)), rep)
+        return rep
+
+    def march(self, binary=True):
+        rep = ChknReport()
+        self._check(self._lib.chkn_march(self.h, 1 if binary else 0, ctypes.byref(rep)), rep)
+        return rep
+
+    def finalize(self, binary=True):
+        self._check(self._lib.chkn_finalize(self.h, 1 if binary else 0))
+
+    def write_flow(self, tindx, binary=True):
+        self._check(self._lib.chkn_write_flow(self.h, tindx, 1 if binary else 0))
+
+    def dims(self, blk):
+        d = np.zeros(3, dtype=np.int32)
+        self._check(self._lib.chkn_block_dims(self.h, blk, _ptr(d, _I)))
+        return tuple(int(v) for v in d)
+
+    def ncells(self, blk):
+        a, b, c = self.dims(blk)
+        return a * b * c
+
+    def iovars(self, blk):
+        out = np.empty((13, self.ncells(blk)))
+        self._check(self._lib.chkn_get_iovars(self.h, blk, _ptr(out, _D)))
+        return out
+
+    def conserved(self, blk):
+        out = np.empty((6, self.ncells(blk)))
+        self._check(self._lib.chkn_get_conserved(self.h, blk, _ptr(out, _D)))
+        return out
+
+    def set_iovars(self, blk, io):
+        io = np.ascontiguousarray(io, dtype=np.float64)
+        self._check(self._lib.chkn_set_iovars(self.h, blk, _ptr(io, _D)))
+
+    def set_launch(self, tile_i=0, tile_j=0, tile_k=0, per_block_launch=False, use_graph=False,
+                   persistent=False):
+        o = ChknLaunchOpts(tile_i, tile_j, tile_k, int(per_block_launch), int(use_graph),
+                           int(persistent))
+        self._check(self._lib.chkn_set_launch_opts(self.h, ctypes.byref(o)))
+
+    def report(self):
+        rep = ChknReport()
+        self._check(self._lib.chkn_get_report(self.h, ctypes.byref(rep)))
+        return rep
+
+    def last_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._check(self._lib.chkn_last_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    @property
+    def active_cells(self):
+        return int(self._lib.chkn_active_cells(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.chkn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,48 @@
+// chkn_io.h -- chicken job-directory formats: config.json, grid and flow files, times.data.
+//
+// Formats follow the reference reader/writers: read_config_file (config.cu:347-525),
+// Block::readGrid / readFlow / writeFlow (block.cu:785-976), and the times.data handling of
+// initialize_simulation / write_flow_data (simulate.cu:134-149, :158-179).
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace chkn {
+
+struct JobBlock {
+    int i = 0, j = 0, k = 0;
+    int nic = 0, njc = 0, nkc = 0;
+    bool active = true;
+    int bc[6] = {0, 0, 0, 0, 0, 0};
+    int bc_fs[6] = {0, 0, 0, 0, 0, 0};
+    std::string label;
+};
+
+struct JobConfig {
+    std::string title;
+    int nib = 0, njb = 0, nkb = 0;
+    std::vector<int> blk_ids;  // (i*njb + j)*nkb + k
+    std::vector<JobBlock> blocks;
+    std::vector<double> flow_states;  // 6 per state: p, T, YB, vx, vy, vz
+    int x_order = 2, t_order = 2, flux_calc = 0, cfl_count = 10, max_step = 100, print_count = 20;
+    double max_time = 1.0e-3, dt_init = 1.0e-6;
+    std::vector<double> cfl_times, cfl_values, dt_plot_times, dt_plot_values;
+    bool reacting = false, viscous = false;
+    int source_terms = 0;
+};
+
+// Throw std::runtime_error with the reference's message text on failure.
+JobConfig read_config_json(const std::string& path);
+// Vertices (nic+1)(njc+1)(nkc+1) xyz triples, i fastest.
+std::vector<double> read_grid(const std::string& path, bool binary, int nic, int njc, int nkc);
+// 13*ncells IO variables, variable-major.
+std::vector<double> read_flow(const std::string& path, bool binary, int ncells);
+void write_flow(const std::string& path, bool binary, const double* iovars, int ncells);
+// Start time and next plot index for time index tindx from times.data (t = 0 if absent).
+void read_times(const std::string& job, int tindx, double& t, int& next_plot_indx);
+void append_times(const std::string& job, int tindx, double t);
+std::string block_file(const std::string& job, const char* kind, int tindx, int i, int j, int k,
+                       bool binary);
+
+}  // namespace chkn

@@ -1,0 +1,213 @@
+// chkn_physics.h -- pointwise FP64 physics of the chicken step, for gfx950 device code and
+// the host geometry pre-pass.
+//
+// Every expression keeps the operand order and association of the reference so that, with
+// floating-point contraction disabled (-ffp-contract=off), the GPU reproduces the
+// reference CPU build bit for bit.  References are to src/chicken/ in the reference tree.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define CHKN_HD __host__ __device__ __forceinline__
+
+namespace chkn {
+
+// Ideal air, gas.cu:86-93 (the IDEAL_AIR build).
+constexpr double kGamma = 1.4;
+constexpr double kR = 287.1;
+constexpr double kQ = 0.0;
+constexpr double kCv = kR / (kGamma - 1.0);
+constexpr double kOneSixth = 1.0 / 6.0;   // number.cu:34
+constexpr double kFourSixth = 4.0 / 6.0;  // number.cu:35
+
+// Primitive state carried between stages: the six values the reconstruction reads
+// (face.cu:359-368).  p, T and a follow from rho and e through the equation of state.
+struct Prim {
+    double rho, e, YB, vx, vy, vz;
+};
+
+struct Frame {
+    double nx, ny, nz, t1x, t1y, t1z, t2x, t2y, t2z;
+};
+
+// gas.cu:128-135 (update_from_rhoe): T, then p = rho*R*T, a = sqrt(g*R*T).
+CHKN_HD double eos_T(double e, double YB) { return (e + YB * kQ) / kCv; }
+CHKN_HD double eos_p(double rho, double T) { return rho * kR * T; }
+CHKN_HD double eos_a(double T) { return sqrt(kGamma * kR * T); }
+
+// vector3.cu:97-109 (global -> local) and :111-119 (local -> global).
+CHKN_HD void to_local(const Frame& f, double x, double y, double z, double& u, double& v,
+                      double& w)
+{
+    u = x * f.nx + y * f.ny + z * f.nz;
+    v = x * f.t1x + y * f.t1y + z * f.t1z;
+    w = x * f.t2x + y * f.t2y + z * f.t2z;
+}
+CHKN_HD void to_global(const Frame& f, double u, double v, double w, double& x, double& y,
+                       double& z)
+{
+    x = u * f.nx + v * f.t1x + w * f.t2x;
+    y = u * f.ny + v * f.t1y + w * f.t2y;
+    z = u * f.nz + v * f.t1z + w * f.t2z;
+}
+
+// Slip-wall ghost state: interior state with the face-normal velocity negated in the face
+// frame (bcs.cu:26-37).
+CHKN_HD Prim reflect(const Prim& s, const Frame& f)
+{
+    double u, v, w;
+    to_local(f, s.vx, s.vy, s.vz, u, v, w);
+    u = -(u);
+    Prim r = s;
+    to_global(f, u, v, w, r.vx, r.vy, r.vz);
+    return r;
+}
+
+// face.cu:72-79
+CHKN_HD double van_albada(double a, double b)
+{
+    const double eps = 1.0e-12;
+    return (a * b + fabs(a * b) + eps) / (a * a + b * b + eps);
+}
+
+// face.cu:81-96
+CHKN_HD void muscl(double qL1, double qL0, double qR0, double qR1, double& qL, double& qR)
+{
+    double dm = (qL0 - qL1);
+    double d = (qR0 - qL0);
+    double dp = (qR1 - qR0);
+    double sL = van_albada(dm, d);
+    double sR = van_albada(d, dp);
+    qL = qL0 + sL * 0.125 * (3.0 * d + dm);
+    qR = qR0 - sR * 0.125 * (dp + 3.0 * d);
+}
+
+// AUSMDV flux of Wada & Liou (1994) on reconstructed face states, face.cu:163-265.
+// Inputs are the gas states (rho, e, p, a, YB) and global-frame velocities on each side.
+CHKN_HD void ausmdv(double rhoL, double eL, double pL, double aL, double YBL, double vxL,
+                    double vyL, double vzL, double rhoR, double eR, double pR, double aR,
+                    double YBR, double vxR, double vyR, double vzR, const Frame& f,
+                    double F[6])
+{
+    double velxL, velyL, velzL, velxR, velyR, velzR;
+    to_local(f, vxL, vyL, vzL, velxL, velyL, velzL);
+    to_local(f, vxR, vyR, vzR, velxR, velyR, velzR);
+    double pLrL = pL / rhoL;
+    double keL = 0.5 * (velxL * velxL + velyL * velyL + velzL * velzL);
+    double HL = eL + pLrL + keL;
+    double pRrR = pR / rhoR;
+    double keR = 0.5 * (velxR * velxR + velyR * velyR + velzR * velzR);
+    double HR = eR + pR / rhoR + keR;
+    double alphaL = 2.0 * pLrL / (pLrL + pRrR);
+    double alphaR = 2.0 * pRrR / (pLrL + pRrR);
+    double am = fmax(aL, aR);
+    double ML = velxL / am;
+    double MR = velxR / am;
+    double pLplus, velxLplus;
+    double dvelxL = 0.5 * (velxL + fabs(velxL));
+    if (fabs(ML) <= 1.0) {
+        pLplus = pL * (ML + 1.0) * (ML + 1.0) * (2.0 - ML) * 0.25;
+        velxLplus = alphaL * ((velxL + am) * (velxL + am) / (4.0 * am) - dvelxL) + dvelxL;
+    } else {
+        pLplus = pL * dvelxL / velxL;
+        velxLplus = dvelxL;
+    }
+    double pRminus, velxRminus;
+    double dvelxR = 0.5 * (velxR - fabs(velxR));
+    if (fabs(MR) <= 1.0) {
+        pRminus = pR * (MR - 1.0) * (MR - 1.0) * (2.0 + MR) * 0.25;
+        velxRminus = alphaR * (-(velxR - am) * (velxR - am) / (4.0 * am) - dvelxR) + dvelxR;
+    } else {
+        pRminus = pR * dvelxR / velxR;
+        velxRminus = dvelxR;
+    }
+    double massL = velxLplus * rhoL;
+    double massR = velxRminus * rhoR;
+    double mass_half = massL + massR;
+    double p_half = pLplus + pRminus;
+    double dp = pL - pR;
+    dp = 10.0 * fabs(dp) / fmin(pL, pR);
+    double s = 0.5 * fmin(1.0, dp);
+    double rv2_V = massL * velxL + massR * velxR;
+    double rv2_D = 0.5 * (mass_half * (velxL + velxR) - fabs(mass_half) * (velxR - velxL));
+    double rv2 = (0.5 + s) * rv2_V + (0.5 - s) * rv2_D;
+    bool up = (mass_half >= 0.0);
+    double vely = up ? velyL : velyR;
+    double velz = up ? velzL : velzR;
+    F[0] = mass_half;
+    to_global(f, rv2 + p_half, mass_half * vely, mass_half * velz, F[1], F[2], F[3]);
+    F[4] = mass_half * (up ? HL : HR);
+    F[5] = mass_half * (up ? YBL : YBR);
+}
+
+// face.cu:347-371 for flux_calc = ausmdv.  XORDER 2 reconstructs rho, e, YB, then applies
+// the equation of state, then reconstructs the velocity; XORDER 1 uses the adjacent cells.
+template <int XORDER>
+CHKN_HD void face_flux(const Prim& L1, const Prim& L0, const Prim& R0, const Prim& R1,
+                       const Frame& f, double F[6])
+{
+    double rhoL, rhoR, eL, eR, YBL, YBR, vxL, vxR, vyL, vyR, vzL, vzR;
+    if (XORDER > 1) {
+        muscl(L1.rho, L0.rho, R0.rho, R1.rho, rhoL, rhoR);
+        muscl(L1.e, L0.e, R0.e, R1.e, eL, eR);
+        muscl(L1.YB, L0.YB, R0.YB, R1.YB, YBL, YBR);
+        muscl(L1.vx, L0.vx, R0.vx, R1.vx, vxL, vxR);
+        muscl(L1.vy, L0.vy, R0.vy, R1.vy, vyL, vyR);
+        muscl(L1.vz, L0.vz, R0.vz, R1.vz, vzL, vzR);
+    } else {
+        rhoL = L0.rho; eL = L0.e; YBL = L0.YB; vxL = L0.vx; vyL = L0.vy; vzL = L0.vz;
+        rhoR = R0.rho; eR = R0.e; YBR = R0.YB; vxR = R0.vx; vyR = R0.vy; vzR = R0.vz;
+    }
+    double TL = eos_T(eL, YBL), TR = eos_T(eR, YBR);
+    double pL = eos_p(rhoL, TL), pR = eos_p(rhoR, TR);
+    double aL = eos_a(TL), aR = eos_a(TR);
+    ausmdv(rhoL, eL, pL, aL, YBL, vxL, vyL, vzL, rhoR, eR, pR, aR, YBR, vxR, vyR, vzR, f, F);
+}
+
+// flow.cu:227-243
+CHKN_HD void encode(const Prim& s, double U[6])
+{
+    double rho = s.rho;
+    U[0] = rho;
+    U[1] = rho * s.vx;
+    U[2] = rho * s.vy;
+    U[3] = rho * s.vz;
+    double ke = 0.5 * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+    U[4] = rho * (s.e + ke);
+    U[5] = rho * s.YB;
+}
+
+// flow.cu:245-268 without the equation-of-state tail.  Returns 1 for a bad cell.
+CHKN_HD int decode(const double U[6], Prim& s)
+{
+    bool nan = isnan(U[0]) || isnan(U[1]) || isnan(U[2]) || isnan(U[3]) || isnan(U[4]) || isnan(U[5]);
+    if (nan) return 1;
+    double rho = U[0];
+    if (rho <= 0.0) return 1;
+    double dinv = 1.0 / rho;
+    s.vx = U[1] * dinv;
+    s.vy = U[2] * dinv;
+    s.vz = U[3] * dinv;
+    double e = U[4] * dinv;
+    double ke = 0.5 * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+    e -= ke;
+    s.rho = rho;
+    s.e = e;
+    s.YB = U[5] * dinv;
+    return 0;
+}
+
+// Signal-speed time-step estimate without the cfl factor, cell.cu:155-164.  The cfl
+// multiply is applied after the minimum: rounding is monotone, so
+// min(fl(cfl*x_i)) == fl(cfl*min(x_i)) for cfl > 0.
+CHKN_HD double local_signal(const Prim& s, double a, double iL, double jL, double kL,
+                            const Frame& fi, const Frame& fj, const Frame& fk)
+{
+    double is = iL / (fabs(s.vx * fi.nx + s.vy * fi.ny + s.vz * fi.nz) + a);
+    double js = jL / (fabs(s.vx * fj.nx + s.vy * fj.ny + s.vz * fj.nz) + a);
+    double ks = kL / (fabs(s.vx * fk.nx + s.vy * fk.ny + s.vz * fk.nz) + a);
+    return fmin(fmin(is, js), ks);
+}
+
+}  // namespace chkn

@@ -1,0 +1,849 @@
+// chkn_sim.cpp -- host runtime of libchkn: job set-up, geometry pre-pass, device memory,
+// step orchestration (segments of steps between CFL updates, optionally replayed from a
+// captured hipGraph) and the C-ABI of include/chkn.h.
+//
+// The march loop restates march_in_time_using_cpu_only (simulate.cu:182-342).  The time step
+// only changes every cfl_count steps, so the host knows dt for a whole segment of steps:
+// it writes dt into the device control block, issues the segment's 3*K stage launches
+// without synchronising, and reads the control block back once at the end of the segment
+// (bad-cell counts per stage and, when the segment ends on a CFL step, the minimum signal
+// time reduced inside the last stage-3 launch).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/chkn.h"
+#include "chkn_geometry.h"
+#include "chkn_internal.h"
+#include "chkn_io.h"
+#include "chkn_physics.h"
+
+using namespace chkn;
+
+namespace {
+
+constexpr int kMaxSeg = 64;  // steps per segment (device control slots)
+
+thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw HipError(std::string("HIP error ") + hipGetErrorString(e_) + " at " #expr);   \
+    } while (0)
+
+struct HostBlock {
+    int bi, bj, bk, nic, njc, nkc;
+    bool active;
+    int bc[6], bc_fs[6];
+    int n;
+    std::vector<double> io0;  // IO variables as given at start (flow part used while fresh)
+    std::vector<double> pos;  // 3*n computed centroids
+    std::vector<double> vol;  // n computed volumes
+    // device
+    double* W = nullptr;  // 3 buffers (active) or 1 (inactive)
+    double* U = nullptr;
+    double* L = nullptr;
+    double* gi = nullptr;
+    double* gj = nullptr;
+    double* gk = nullptr;
+    double* volinv = nullptr;
+    double* len = nullptr;
+    int tile_first = 0, tile_count = 0;
+};
+
+struct GraphKey {
+    int K;
+    bool dtred;
+    bool operator<(const GraphKey& o) const { return K != o.K ? K < o.K : dtred < o.dtred; }
+};
+
+}  // namespace
+
+struct chkn_sim {
+    // configuration
+    int nib = 0, njb = 0, nkb = 0;
+    std::vector<int> blk_ids;
+    std::vector<HostBlock> blocks;
+    std::vector<double> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz
+    int x_order = 2, flux_calc = 0, cfl_count = 10, max_step = 100, print_count = 20;
+    double max_time = 1e-3, dt_init = 1e-6;
+    std::vector<double> cfl_t, cfl_v, plot_t, plot_v;
+    std::string job;  // empty when created from arrays
+    // state (SimState of simulate.cu:32-39)
+    double t = 0.0, dt = 0.0, t_plot = 0.0;
+    int step = 0, next_plot_indx = 1, steps_since_last_plot = 0;
+    bool fresh = true;   // no step taken since the state was set from IO variables
+    bool failed = false;
+    bool have_sig = false;
+    double sigmin = 0.0;
+    // device
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    BlockDev* d_blocks = nullptr;
+    Tile* d_tiles = nullptr;
+    double* d_inflow = nullptr;
+    StepCtl* d_ctl = nullptr;
+    StepCtl* h_ctl = nullptr;
+    int ntiles = 0;
+    // launch options
+    chkn_launch_opts opts{32, 8, 0, 0, 0, 0};
+    int kc = 8;
+    int grid_limit = 0;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    // timing of the last chkn_step call
+    double last_ms = 0.0;
+    long long last_launches = 0;
+    std::string err;
+};
+
+namespace {
+
+int set_error(chkn_sim* s, int code, const std::string& msg)
+{
+    if (s) s->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+double schedule_get(const std::vector<double>& ts, const std::vector<double>& vs, double t)
+{
+    // config.cu:263-269
+    int i = (int)ts.size() - 1;
+    while (i > 0 && t < ts[i]) i--;
+    return vs[i];
+}
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+void free_device(chkn_sim* s)
+{
+    for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+    s->graphs.clear();
+    for (auto& b : s->blocks) {
+        dfree(b.W); dfree(b.U); dfree(b.L); dfree(b.gi); dfree(b.gj); dfree(b.gk);
+        dfree(b.volinv); dfree(b.len);
+        b.W = b.U = b.L = b.gi = b.gj = b.gk = b.volinv = b.len = nullptr;
+    }
+    dfree(s->d_blocks); dfree(s->d_tiles); dfree(s->d_inflow); dfree(s->d_ctl);
+    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->d_blocks = nullptr; s->d_tiles = nullptr; s->d_inflow = nullptr; s->d_ctl = nullptr;
+    s->h_ctl = nullptr; s->ev0 = s->ev1 = nullptr; s->stream = nullptr;
+}
+
+// Geometry pre-pass of one block (block.cu:633-652) into device-layout host arrays.
+void block_geometry(HostBlock& b, const double* vtx, std::vector<double>& gi, std::vector<double>& gj,
+                    std::vector<double>& gk, std::vector<double>& volinv, std::vector<double>& len)
+{
+    const int nic = b.nic, njc = b.njc, nkc = b.nkc;
+    auto V = [&](int i, int j, int k) {
+        size_t q = (size_t)k * (nic + 1) * (njc + 1) + (size_t)j * (nic + 1) + i;
+        return geom::V3{vtx[3 * q], vtx[3 * q + 1], vtx[3 * q + 2]};
+    };
+    const size_t n = (size_t)b.n;
+    b.pos.assign(3 * n, 0.0);
+    b.vol.assign(n, 0.0);
+    volinv.assign(n, 0.0);
+    len.assign(3 * n, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < njc; ++j)
+            for (int i = 0; i < nic; ++i) {
+                geom::V3 p[8] = {V(i, j, k),         V(i + 1, j, k),     V(i + 1, j + 1, k),
+                                 V(i, j + 1, k),     V(i, j, k + 1),     V(i + 1, j, k + 1),
+                                 V(i + 1, j + 1, k + 1), V(i, j + 1, k + 1)};
+                geom::Hex h = geom::hex(p);
+                size_t c = (size_t)i + (size_t)nic * (j + (size_t)njc * k);
+                b.pos[c] = h.c.x; b.pos[n + c] = h.c.y; b.pos[2 * n + c] = h.c.z;
+                b.vol[c] = h.vol;
+                volinv[c] = 1.0 / h.vol;  // cell.cu:190
+                len[c] = h.iL; len[n + c] = h.jL; len[2 * n + c] = h.kL;
+            }
+    auto put = [](std::vector<double>& g, size_t nf, size_t f, const geom::Quad& q) {
+        g[f] = q.area;
+        g[nf + f] = q.n.x; g[2 * nf + f] = q.n.y; g[3 * nf + f] = q.n.z;
+        g[4 * nf + f] = q.t1.x; g[5 * nf + f] = q.t1.y; g[6 * nf + f] = q.t1.z;
+        g[7 * nf + f] = q.t2.x; g[8 * nf + f] = q.t2.y; g[9 * nf + f] = q.t2.z;
+    };
+    size_t nfi = (size_t)(nic + 1) * njc * nkc, nfj = (size_t)nic * (njc + 1) * nkc,
+           nfk = (size_t)nic * njc * (nkc + 1);
+    gi.assign(kFrameN * nfi, 0.0);
+    gj.assign(kFrameN * nfj, 0.0);
+    gk.assign(kFrameN * nfk, 0.0);
+    // Face vertex orders: i-faces block.cu:267-270, j-faces :368-371, k-faces :469-472.
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < njc; ++j)
+            for (int i = 0; i <= nic; ++i)
+                put(gi, nfi, (size_t)i + (size_t)(nic + 1) * (j + (size_t)njc * k),
+                    geom::quad(V(i, j, k), V(i, j + 1, k), V(i, j + 1, k + 1), V(i, j, k + 1)));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j <= njc; ++j)
+            for (int i = 0; i < nic; ++i)
+                put(gj, nfj, (size_t)i + (size_t)nic * (j + (size_t)(njc + 1) * k),
+                    geom::quad(V(i + 1, j, k), V(i, j, k), V(i, j, k + 1), V(i + 1, j, k + 1)));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k <= nkc; ++k)
+        for (int j = 0; j < njc; ++j)
+            for (int i = 0; i < nic; ++i)
+                put(gk, nfk, (size_t)i + (size_t)nic * (j + (size_t)njc * k),
+                    geom::quad(V(i, j, k), V(i + 1, j, k), V(i + 1, j + 1, k), V(i, j + 1, k)));
+}
+
+// Primitive stage state from IO variables (rho, e, YB, vel from the flow file).
+void prim_from_io(const double* io, int n, std::vector<double>& w)
+{
+    w.assign((size_t)6 * n, 0.0);
+    for (int c = 0; c < n; ++c) {
+        w[c] = io[6 * (size_t)n + c];              // rho
+        w[(size_t)n + c] = io[7 * (size_t)n + c];  // e
+        w[2 * (size_t)n + c] = io[8 * (size_t)n + c];  // YB
+        w[3 * (size_t)n + c] = io[10 * (size_t)n + c];
+        w[4 * (size_t)n + c] = io[11 * (size_t)n + c];
+        w[5 * (size_t)n + c] = io[12 * (size_t)n + c];
+    }
+}
+
+void conserved_from_prim(const std::vector<double>& w, int n, std::vector<double>& u)
+{
+    u.assign((size_t)6 * n, 0.0);
+    for (int c = 0; c < n; ++c) {
+        Prim p{w[c], w[(size_t)n + c], w[2 * (size_t)n + c], w[3 * (size_t)n + c],
+               w[4 * (size_t)n + c], w[5 * (size_t)n + c]};
+        double U[6];
+        encode(p, U);  // simulate.cu:196-203 via flow.cu:227-243
+        for (int q = 0; q < 6; ++q) u[(size_t)q * n + c] = U[q];
+    }
+}
+
+void upload_state(chkn_sim* s, HostBlock& b, const double* io)
+{
+    std::vector<double> w, u;
+    prim_from_io(io, b.n, w);
+    size_t bytes = sizeof(double) * 6 * (size_t)b.n;
+    int nbuf = b.active ? 3 : 1;
+    for (int r = 0; r < nbuf; ++r) HIPCHK(hipMemcpy(b.W + (size_t)r * 6 * b.n, w.data(), bytes, hipMemcpyHostToDevice));
+    if (b.active) {
+        conserved_from_prim(w, b.n, u);
+        HIPCHK(hipMemcpy(b.U, u.data(), bytes, hipMemcpyHostToDevice));
+    }
+}
+
+void build_tiles(chkn_sim* s)
+{
+    std::vector<Tile> tiles;
+    const int TI = s->opts.tile_i, TJ = s->opts.tile_j;
+    for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
+        HostBlock& b = s->blocks[ib];
+        b.tile_first = (int)tiles.size();
+        if (b.active) {
+            int nti = (b.nic + TI - 1) / TI, ntj = (b.njc + TJ - 1) / TJ;
+            int kc = s->opts.tile_k > 0 ? s->opts.tile_k : s->kc;
+            if (s->opts.tile_k <= 0) {
+                // Default k-chunk: aim for about 2048 work-groups over the whole launch.
+                long long cols = 0;
+                for (auto& o : s->blocks)
+                    if (o.active) cols += (long long)((o.nic + TI - 1) / TI) * ((o.njc + TJ - 1) / TJ) * o.nkc;
+                long long want = std::max(1LL, cols / 2048);
+                kc = (int)std::min<long long>(std::max<long long>(want, 2), 32);
+            }
+            for (int k0 = 0; k0 < b.nkc; k0 += kc)
+                for (int tj = 0; tj < ntj; ++tj)
+                    for (int ti = 0; ti < nti; ++ti)
+                        tiles.push_back(Tile{(int)ib, ti * TI, tj * TJ, k0, std::min(kc, b.nkc - k0)});
+        }
+        b.tile_count = (int)tiles.size() - b.tile_first;
+    }
+    dfree(s->d_tiles);
+    s->d_tiles = nullptr;
+    s->ntiles = (int)tiles.size();
+    if (!tiles.empty()) {
+        HIPCHK(hipMalloc(&s->d_tiles, sizeof(Tile) * tiles.size()));
+        HIPCHK(hipMemcpy(s->d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+    }
+    for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+    s->graphs.clear();
+}
+
+void setup(chkn_sim* s, const chkn_desc* d)
+{
+    if (d->nblocks <= 0) throw std::invalid_argument("no blocks");
+    if (d->reacting) throw std::invalid_argument("the reacting-gas model is not built");
+    if (d->viscous) throw std::invalid_argument("viscous fluxes are not built");
+    if (d->flux_calc != 0) throw std::invalid_argument("flux_calc sbp_asf is not built");
+    if (d->x_order != 1 && d->x_order != 2) throw std::invalid_argument("x_order must be 1 or 2");
+    if (d->n_cfl <= 0) throw std::invalid_argument("need at least one cfl value");
+    if (d->cfl_count <= 0) throw std::invalid_argument("cfl_count must be positive");
+    s->nib = d->nib; s->njb = d->njb; s->nkb = d->nkb;
+    s->blk_ids.assign(d->blk_ids, d->blk_ids + (size_t)d->nib * d->njb * d->nkb);
+    s->x_order = d->x_order; s->flux_calc = d->flux_calc; s->cfl_count = d->cfl_count;
+    s->max_step = d->max_step; s->print_count = d->print_count;
+    s->max_time = d->max_time; s->dt_init = d->dt_init;
+    s->cfl_t.assign(d->cfl_times, d->cfl_times + d->n_cfl);
+    s->cfl_v.assign(d->cfl_values, d->cfl_values + d->n_cfl);
+    if (d->n_dt_plot > 0) {
+        s->plot_t.assign(d->dt_plot_times, d->dt_plot_times + d->n_dt_plot);
+        s->plot_v.assign(d->dt_plot_values, d->dt_plot_values + d->n_dt_plot);
+    } else {
+        s->plot_t.assign(1, 0.0);
+        s->plot_v.assign(1, d->max_time);
+    }
+    // Inflow states: update_from_pT of the configured p, T, YB (config.cu:398-406).
+    for (int q = 0; q < d->n_flow_states; ++q) {
+        const double* f = d->flow_states + 6 * q;
+        double p = f[0], T = f[1], YB = f[2];
+        double e = T * kCv - YB * kQ;   // gas.cu:123
+        double rho = p / (T * kR);      // gas.cu:124
+        double w[6] = {rho, e, YB, f[3], f[4], f[5]};
+        s->inflow_w.insert(s->inflow_w.end(), w, w + 6);
+    }
+    if (s->inflow_w.empty()) s->inflow_w.assign(6, 0.0);
+    // Blocks.
+    size_t voff = 0, ooff = 0;
+    s->blocks.resize(d->nblocks);
+    for (int ib = 0; ib < d->nblocks; ++ib) {
+        HostBlock& b = s->blocks[ib];
+        b.bi = d->blk_ijk[3 * ib]; b.bj = d->blk_ijk[3 * ib + 1]; b.bk = d->blk_ijk[3 * ib + 2];
+        b.nic = d->blk_dims[3 * ib]; b.njc = d->blk_dims[3 * ib + 1]; b.nkc = d->blk_dims[3 * ib + 2];
+        b.active = d->active[ib] != 0;
+        for (int f = 0; f < 6; ++f) { b.bc[f] = d->bc_codes[6 * ib + f]; b.bc_fs[f] = d->bc_fs[6 * ib + f]; }
+        // block.cu:282-284 and its j/k twins.
+        if (b.nic < 2) throw std::invalid_argument("Too few cells in the i-index direction.");
+        if (b.njc < 2) throw std::invalid_argument("Too few cells in the j-index direction.");
+        if (b.nkc < 2) throw std::invalid_argument("Too few cells in the k-index direction.");
+        long long n = (long long)b.nic * b.njc * b.nkc;
+        if (n > (1LL << 30)) throw std::invalid_argument("block too large for 32-bit cell indexing");
+        b.n = (int)n;
+        if (b.bi < 0 || b.bi >= s->nib || b.bj < 0 || b.bj >= s->njb || b.bk < 0 || b.bk >= s->nkb)
+            throw std::invalid_argument("block position outside the block array");
+        if (s->blk_ids[((size_t)b.bi * s->njb + b.bj) * s->nkb + b.bk] != ib)
+            throw std::invalid_argument("blk_id=" + std::to_string(ib) + " is inconsistent");
+        for (int f = 0; f < 6; ++f) {
+            if (b.active && b.bc[f] == CHKN_BC_INFLOW_FUNCTION)
+                throw std::invalid_argument("inflow_function boundaries are not built");
+            if (b.active && b.bc[f] == CHKN_BC_INFLOW && (b.bc_fs[f] < 0 || b.bc_fs[f] >= d->n_flow_states))
+                throw std::invalid_argument("inflow flow_state_index out of range");
+        }
+        b.io0.assign(d->iovars + ooff, d->iovars + ooff + (size_t)13 * b.n);
+        ooff += (size_t)13 * b.n;
+        voff += 0;
+    }
+    // Exchange partners (bcs.cu:145-252, periodic wrap at the array edges).
+    std::vector<std::array<int, 6>> partner(d->nblocks);
+    for (int ib = 0; ib < d->nblocks; ++ib) {
+        HostBlock& b = s->blocks[ib];
+        for (int f = 0; f < 6; ++f) {
+            partner[ib][f] = -1;
+            if (!b.active || b.bc[f] != CHKN_BC_EXCHANGE) continue;
+            int oi = b.bi + (f == 1) - (f == 0), oj = b.bj + (f == 3) - (f == 2), ok = b.bk + (f == 5) - (f == 4);
+            oi = (oi + s->nib) % s->nib; oj = (oj + s->njb) % s->njb; ok = (ok + s->nkb) % s->nkb;
+            int id = s->blk_ids[((size_t)oi * s->njb + oj) * s->nkb + ok];
+            if (id < 0 || id >= d->nblocks) throw std::invalid_argument("exchange boundary without a neighbour block");
+            HostBlock& o = s->blocks[id];
+            bool okd = (f < 2) ? (o.njc == b.njc && o.nkc == b.nkc)
+                       : (f < 4) ? (o.nic == b.nic && o.nkc == b.nkc) : (o.nic == b.nic && o.njc == b.njc);
+            if (!okd) throw std::invalid_argument("exchange neighbour has mismatched face dimensions");
+            partner[ib][f] = id;
+        }
+    }
+    // Device.
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&s->ev0));
+    HIPCHK(hipEventCreate(&s->ev1));
+    std::vector<BlockDev> bdev(d->nblocks);
+    voff = 0;
+    for (int ib = 0; ib < d->nblocks; ++ib) {
+        HostBlock& b = s->blocks[ib];
+        std::vector<double> gi, gj, gk, volinv, len;
+        block_geometry(b, d->vertices + voff, gi, gj, gk, volinv, len);
+        voff += (size_t)3 * (b.nic + 1) * (b.njc + 1) * (b.nkc + 1);
+        size_t n = (size_t)b.n, sz6 = sizeof(double) * 6 * n;
+        HIPCHK(hipMalloc(&b.W, sz6 * (b.active ? 3 : 1)));
+        BlockDev& bd = bdev[ib];
+        std::memset(&bd, 0, sizeof bd);
+        bd.nic = b.nic; bd.njc = b.njc; bd.nkc = b.nkc; bd.active = b.active ? 1 : 0;
+        bd.n = b.n;
+        bd.nfi = (long long)(b.nic + 1) * b.njc * b.nkc;
+        bd.nfj = (long long)b.nic * (b.njc + 1) * b.nkc;
+        bd.nfk = (long long)b.nic * b.njc * (b.nkc + 1);
+        for (int r = 0; r < 3; ++r) bd.W[r] = b.W + (b.active ? (size_t)r * 6 * n : 0);
+        if (b.active) {
+            HIPCHK(hipMalloc(&b.U, sz6));
+            HIPCHK(hipMalloc(&b.L, sz6));
+            HIPCHK(hipMemset(b.L, 0, sz6));
+            HIPCHK(hipMalloc(&b.gi, sizeof(double) * gi.size()));
+            HIPCHK(hipMalloc(&b.gj, sizeof(double) * gj.size()));
+            HIPCHK(hipMalloc(&b.gk, sizeof(double) * gk.size()));
+            HIPCHK(hipMalloc(&b.volinv, sizeof(double) * n));
+            HIPCHK(hipMalloc(&b.len, sizeof(double) * 3 * n));
+            HIPCHK(hipMemcpy(b.gi, gi.data(), sizeof(double) * gi.size(), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b.gj, gj.data(), sizeof(double) * gj.size(), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b.gk, gk.data(), sizeof(double) * gk.size(), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b.volinv, volinv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b.len, len.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+            bd.U = b.U; bd.L = b.L; bd.gi = b.gi; bd.gj = b.gj; bd.gk = b.gk;
+            bd.volinv = b.volinv; bd.len = b.len;
+        }
+        for (int f = 0; f < 6; ++f) {
+            bd.bc[f] = b.bc[f];
+            bd.bc_arg[f] = (b.bc[f] == CHKN_BC_EXCHANGE) ? partner[ib][f] : b.bc_fs[f];
+        }
+        upload_state(s, b, b.io0.data());
+    }
+    HIPCHK(hipMalloc(&s->d_blocks, sizeof(BlockDev) * bdev.size()));
+    HIPCHK(hipMemcpy(s->d_blocks, bdev.data(), sizeof(BlockDev) * bdev.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&s->d_inflow, sizeof(double) * s->inflow_w.size()));
+    HIPCHK(hipMemcpy(s->d_inflow, s->inflow_w.data(), sizeof(double) * s->inflow_w.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&s->d_ctl, sizeof(StepCtl) * kMaxSeg));
+    HIPCHK(hipHostMalloc(&s->h_ctl, sizeof(StepCtl) * kMaxSeg, hipHostMallocDefault));
+    build_tiles(s);
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    s->grid_limit = ncu * 2;
+    // SimState at start (simulate.cu:132-150, :190-191).
+    s->t = d->t0;
+    s->dt = d->dt_init;
+    s->step = 0;
+    s->next_plot_indx = d->start_tindx + 1;
+    s->t_plot = s->t + schedule_get(s->plot_t, s->plot_v, s->t);
+    s->steps_since_last_plot = 0;
+    s->fresh = true;
+}
+
+// Launch the 3*K stage kernels of a segment (optionally per block).
+void launch_segment(chkn_sim* s, int K, bool dtred_last)
+{
+    StageArgs a;
+    a.blocks = s->d_blocks;
+    a.tiles = s->d_tiles;
+    a.inflow = s->d_inflow;
+    for (int q = 0; q < K; ++q) {
+        a.ctl = s->d_ctl + q;
+        for (int stage = 1; stage <= 3; ++stage) {
+            a.win = stage - 1;
+            a.wout = stage % 3;
+            bool dtred = (stage == 3) && dtred_last && (q == K - 1);
+            if (s->opts.per_block_launch) {
+                for (auto& b : s->blocks) {
+                    if (!b.active || b.tile_count == 0) continue;
+                    a.tile_base = b.tile_first;
+                    a.ntiles = b.tile_count;
+                    HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, dtred,
+                                        s->opts.persistent != 0, s->grid_limit, a, s->stream));
+                    s->last_launches++;
+                }
+            } else {
+                a.tile_base = 0;
+                a.ntiles = s->ntiles;
+                HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, dtred,
+                                    s->opts.persistent != 0, s->grid_limit, a, s->stream));
+                s->last_launches++;
+            }
+        }
+    }
+}
+
+void run_segment(chkn_sim* s, int K, bool dtred_last)
+{
+    for (int q = 0; q < K; ++q) {
+        s->h_ctl[q].dt = s->dt;
+        s->h_ctl[q].sigmin_bits = ~0ull;
+        s->h_ctl[q].bad[0] = s->h_ctl[q].bad[1] = s->h_ctl[q].bad[2] = 0;
+        s->h_ctl[q].pad = 0;
+    }
+    HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(StepCtl) * K, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipEventRecord(s->ev0, s->stream));
+    if (s->opts.use_graph) {
+        GraphKey key{K, dtred_last};
+        auto it = s->graphs.find(key);
+        if (it == s->graphs.end()) {
+            hipGraph_t g;
+            long long saved = s->last_launches;
+            HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            launch_segment(s, K, dtred_last);
+            HIPCHK(hipStreamEndCapture(s->stream, &g));
+            hipGraphExec_t ge;
+            HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            it = s->graphs.emplace(key, ge).first;
+            s->last_launches = saved;
+        }
+        HIPCHK(hipGraphLaunch(it->second, s->stream));
+        long long per_stage = 1;
+        if (s->opts.per_block_launch) {
+            per_stage = 0;
+            for (auto& b : s->blocks) if (b.active && b.tile_count) per_stage++;
+        }
+        s->last_launches += 3LL * K * per_stage;
+    } else {
+        launch_segment(s, K, dtred_last);
+    }
+    HIPCHK(hipEventRecord(s->ev1, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(StepCtl) * K, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->last_ms += ms;
+}
+
+// Minimum signal time of the current state when no stage-3 launch reduced it (first CFL
+// step after the state was replaced).
+void signal_reduction(chkn_sim* s)
+{
+    s->h_ctl[0].sigmin_bits = ~0ull;
+    HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(StepCtl), hipMemcpyHostToDevice, s->stream));
+    for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
+        HostBlock& b = s->blocks[ib];
+        if (!b.active) continue;
+        HIPCHK(launch_signal_min(s->d_blocks + ib, b.n, s->d_ctl, s->stream));
+    }
+    HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    unsigned long long bits = s->h_ctl[0].sigmin_bits;
+    double m;
+    std::memcpy(&m, &bits, sizeof m);
+    s->sigmin = (bits == ~0ull) ? DBL_MAX : m;
+    s->have_sig = true;
+}
+
+// Advance up to nsteps steps; when plot_writes, stop segments at snapshot times and write.
+int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report* rep)
+{
+    if (s->failed) return set_error(s, CHKN_ERR_STATE, "simulation is in a failed state");
+    s->last_ms = 0.0;
+    s->last_launches = 0;
+    int done = 0;
+    int rc = CHKN_OK;
+    while (done < nsteps && s->step < s->max_step && s->t < s->max_time) {
+        if (s->step > 0 && (s->step % s->cfl_count) == 0) {  // simulate.cu:208-221
+            if (!s->have_sig) signal_reduction(s);
+            double cfl = schedule_get(s->cfl_t, s->cfl_v, s->t);
+            double smallest = cfl * s->sigmin;
+            s->dt = std::fmin(1.5 * s->dt, smallest);
+            s->have_sig = false;
+        }
+        // Segment: constant dt until the next CFL step, bounded by the run limits.
+        int to_cfl = s->cfl_count - (s->step % s->cfl_count);
+        int K = std::min({to_cfl, nsteps - done, s->max_step - s->step, kMaxSeg});
+        {
+            double tt = s->t;
+            int q = 0;
+            while (q < K && tt < s->max_time) {
+                tt += s->dt;
+                ++q;
+                if (plot_writes && tt >= s->t_plot) break;
+            }
+            K = q;
+        }
+        bool dtred = ((s->step + K) % s->cfl_count) == 0;
+        run_segment(s, K, dtred);
+        for (int q = 0; q < K; ++q) {
+            const StepCtl& c = s->h_ctl[q];
+            for (int st = 0; st < 3; ++st)
+                if (c.bad[st] > 0) {
+                    s->failed = true;
+                    s->step += q;
+                    if (rep) { rep->bad_stage = st + 1; rep->bad_count = c.bad[st]; }
+                    char msg[128];
+                    std::snprintf(msg, sizeof msg, "Stage %d bad cell count: %d", st + 1, c.bad[st]);
+                    set_error(s, c.bad[st], msg);
+                    return c.bad[st];
+                }
+        }
+        if (dtred) {
+            unsigned long long bits = s->h_ctl[K - 1].sigmin_bits;
+            double m;
+            std::memcpy(&m, &bits, sizeof m);
+            if (bits == ~0ull) m = DBL_MAX;  // no active cell reported
+            s->sigmin = m;
+            s->have_sig = true;
+        }
+        for (int q = 0; q < K; ++q) {  // simulate.cu:306-308
+            s->t += s->dt;
+            s->step += 1;
+            s->steps_since_last_plot += 1;
+        }
+        s->fresh = false;
+        done += K;
+        if (plot_writes && s->t >= s->t_plot) {  // simulate.cu:329-334
+            rc = chkn_write_flow(s, s->next_plot_indx, binary);
+            if (rc != CHKN_OK) return rc;
+            s->steps_since_last_plot = 0;
+            s->next_plot_indx += 1;
+            s->t_plot = s->t + schedule_get(s->plot_t, s->plot_v, s->t);
+        }
+    }
+    if (rep) {
+        rep->step = s->step; rep->t = s->t; rep->dt = s->dt;
+        rep->bad_stage = 0; rep->bad_count = 0; rep->steps_done = done;
+    }
+    return rc;
+}
+
+void block_iovars(chkn_sim* s, int ib, double* out)
+{
+    HostBlock& b = s->blocks[ib];
+    size_t n = (size_t)b.n;
+    std::memcpy(out, b.pos.data(), sizeof(double) * 3 * n);
+    std::memcpy(out + 3 * n, b.vol.data(), sizeof(double) * n);
+    if (!b.active || s->fresh) {
+        std::memcpy(out + 4 * n, b.io0.data() + 4 * n, sizeof(double) * 9 * n);
+        return;
+    }
+    std::vector<double> w(6 * n);
+    HIPCHK(hipMemcpy(w.data(), b.W, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+    for (size_t c = 0; c < n; ++c) {
+        double rho = w[c], e = w[n + c], YB = w[2 * n + c];
+        double T = eos_T(e, YB), p = eos_p(rho, T), a = eos_a(T);  // gas.cu:128-135
+        out[4 * n + c] = p; out[5 * n + c] = T; out[6 * n + c] = rho; out[7 * n + c] = e;
+        out[8 * n + c] = YB; out[9 * n + c] = a;
+        out[10 * n + c] = w[3 * n + c]; out[11 * n + c] = w[4 * n + c]; out[12 * n + c] = w[5 * n + c];
+    }
+}
+
+template <typename F>
+int guarded(chkn_sim* s, F&& fn)
+{
+    try {
+        return fn();
+    } catch (const HipError& e) {
+        return set_error(s, CHKN_ERR_HIP, e.what());
+    } catch (const std::invalid_argument& e) {
+        return set_error(s, CHKN_ERR_ARG, e.what());
+    } catch (const std::exception& e) {
+        return set_error(s, CHKN_ERR_IO, e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int chkn_create(const chkn_desc* desc, chkn_sim** out)
+{
+    if (!desc || !out) return set_error(nullptr, CHKN_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto s = std::make_unique<chkn_sim>();
+    try {
+        setup(s.get(), desc);
+    } catch (const HipError& e) {
+        free_device(s.get());
+        return set_error(nullptr, CHKN_ERR_HIP, e.what());
+    } catch (const std::invalid_argument& e) {
+        free_device(s.get());
+        return set_error(nullptr, CHKN_ERR_ARG, e.what());
+    } catch (const std::exception& e) {
+        free_device(s.get());
+        return set_error(nullptr, CHKN_ERR_CONFIG, e.what());
+    }
+    *out = s.release();
+    return CHKN_OK;
+}
+
+int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
+{
+    if (!job_dir || !out) return set_error(nullptr, CHKN_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::string job(job_dir);
+    JobConfig c;
+    std::vector<double> verts, io;
+    double t0 = 0.0;
+    int next_plot = 1;
+    try {
+        if (!std::filesystem::is_directory(job))
+            return set_error(nullptr, CHKN_ERR_IO, "Job directory is not present in current directory.");
+        c = read_config_json(job + "/config.json");
+        for (auto& b : c.blocks) {
+            std::vector<double> v = read_grid(block_file(job, "grid", 0, b.i, b.j, b.k, binary != 0), binary != 0, b.nic, b.njc, b.nkc);
+            verts.insert(verts.end(), v.begin(), v.end());
+            std::vector<double> f = read_flow(block_file(job, "flow", tindx, b.i, b.j, b.k, binary != 0), binary != 0, b.nic * b.njc * b.nkc);
+            io.insert(io.end(), f.begin(), f.end());
+        }
+        read_times(job, tindx, t0, next_plot);
+    } catch (const std::runtime_error& e) {
+        std::string m = e.what();
+        return set_error(nullptr, m == "Incorrect gas model data" ? CHKN_ERR_CONFIG : CHKN_ERR_IO, m);
+    }
+    if (c.reacting) return set_error(nullptr, CHKN_ERR_ARG, "the reacting-gas model is not built");
+    std::vector<int> ijk, dims, active, bcc, bcf;
+    for (auto& b : c.blocks) {
+        ijk.insert(ijk.end(), {b.i, b.j, b.k});
+        dims.insert(dims.end(), {b.nic, b.njc, b.nkc});
+        active.push_back(b.active ? 1 : 0);
+        bcc.insert(bcc.end(), b.bc, b.bc + 6);
+        bcf.insert(bcf.end(), b.bc_fs, b.bc_fs + 6);
+    }
+    chkn_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.nblocks = (int)c.blocks.size(); d.nib = c.nib; d.njb = c.njb; d.nkb = c.nkb;
+    d.blk_ids = c.blk_ids.data(); d.blk_ijk = ijk.data(); d.blk_dims = dims.data();
+    d.active = active.data(); d.bc_codes = bcc.data(); d.bc_fs = bcf.data();
+    d.n_flow_states = (int)c.flow_states.size() / 6; d.flow_states = c.flow_states.data();
+    d.vertices = verts.data(); d.iovars = io.data();
+    d.x_order = c.x_order; d.flux_calc = c.flux_calc; d.t_order = c.t_order;
+    d.cfl_count = c.cfl_count; d.max_step = c.max_step; d.print_count = c.print_count;
+    d.max_time = c.max_time; d.dt_init = c.dt_init;
+    d.n_cfl = (int)c.cfl_times.size(); d.cfl_times = c.cfl_times.data(); d.cfl_values = c.cfl_values.data();
+    d.n_dt_plot = (int)c.dt_plot_times.size(); d.dt_plot_times = c.dt_plot_times.data();
+    d.dt_plot_values = c.dt_plot_values.data();
+    d.t0 = t0; d.start_tindx = next_plot - 1;
+    d.reacting = c.reacting; d.viscous = c.viscous;
+    int rc = chkn_create(&d, out);
+    if (rc == CHKN_OK) (*out)->job = job;
+    return rc;
+}
+
+int chkn_step(chkn_sim* sim, int nsteps, chkn_report* rep)
+{
+    if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
+    return guarded(sim, [&] { return do_steps(sim, nsteps, false, true, rep); });
+}
+
+int chkn_march(chkn_sim* sim, int binary, chkn_report* rep)
+{
+    if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
+    if (sim->job.empty()) return set_error(sim, CHKN_ERR_STATE, "chkn_march needs a job directory");
+    return guarded(sim, [&] { return do_steps(sim, 0x7fffffff, true, binary != 0, rep); });
+}
+
+int chkn_finalize(chkn_sim* sim, int binary)
+{
+    if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
+    if (sim->failed) return CHKN_OK;  // the reference skips finalize after a thrown error
+    if (sim->steps_since_last_plot > 0) return chkn_write_flow(sim, sim->next_plot_indx, binary);
+    return CHKN_OK;
+}
+
+int chkn_write_flow(chkn_sim* sim, int tindx, int binary)
+{
+    if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
+    if (sim->job.empty()) return set_error(sim, CHKN_ERR_STATE, "no job directory to write into");
+    return guarded(sim, [&] {
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "/flow/t%04d", tindx);
+        std::filesystem::create_directories(sim->job + buf);
+        for (size_t ib = 0; ib < sim->blocks.size(); ++ib) {
+            HostBlock& b = sim->blocks[ib];
+            std::vector<double> io((size_t)13 * b.n);
+            block_iovars(sim, (int)ib, io.data());
+            write_flow(block_file(sim->job, "flow", tindx, b.bi, b.bj, b.bk, binary != 0), binary != 0, io.data(), b.n);
+        }
+        append_times(sim->job, tindx, sim->t);
+        return CHKN_OK;
+    });
+}
+
+int chkn_get_iovars(chkn_sim* sim, int blk, double* out)
+{
+    if (!sim || !out || blk < 0 || blk >= (int)sim->blocks.size()) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] { block_iovars(sim, blk, out); return CHKN_OK; });
+}
+
+int chkn_get_conserved(chkn_sim* sim, int blk, double* out)
+{
+    if (!sim || !out || blk < 0 || blk >= (int)sim->blocks.size()) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] {
+        HostBlock& b = sim->blocks[blk];
+        if (!b.active) { std::memset(out, 0, sizeof(double) * 6 * (size_t)b.n); return CHKN_OK; }
+        HIPCHK(hipMemcpy(out, b.U, sizeof(double) * 6 * (size_t)b.n, hipMemcpyDeviceToHost));
+        return CHKN_OK;
+    });
+}
+
+int chkn_set_iovars(chkn_sim* sim, int blk, const double* iovars)
+{
+    if (!sim || !iovars || blk < 0 || blk >= (int)sim->blocks.size()) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] {
+        HostBlock& b = sim->blocks[blk];
+        std::memcpy(b.io0.data() + 4 * (size_t)b.n, iovars + 4 * (size_t)b.n, sizeof(double) * 9 * (size_t)b.n);
+        upload_state(sim, b, b.io0.data());
+        sim->fresh = true;
+        sim->failed = false;
+        sim->have_sig = false;
+        return CHKN_OK;
+    });
+}
+
+int chkn_set_launch_opts(chkn_sim* sim, const chkn_launch_opts* o)
+{
+    if (!sim || !o) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    chkn_launch_opts n = *o;
+    if (n.tile_i == 0) n.tile_i = 32;
+    if (n.tile_j == 0) n.tile_j = 8;
+    if (!tile_shape_supported(n.tile_i, n.tile_j))
+        return set_error(sim, CHKN_ERR_ARG, "unsupported tile shape");
+    if (n.tile_k < 0) return set_error(sim, CHKN_ERR_ARG, "tile_k must be >= 0");
+    return guarded(sim, [&] {
+        sim->opts = n;
+        build_tiles(sim);
+        return CHKN_OK;
+    });
+}
+
+int chkn_get_report(chkn_sim* sim, chkn_report* rep)
+{
+    if (!sim || !rep) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    rep->step = sim->step; rep->t = sim->t; rep->dt = sim->dt;
+    rep->bad_stage = 0; rep->bad_count = 0; rep->steps_done = 0;
+    return CHKN_OK;
+}
+
+int chkn_num_blocks(chkn_sim* sim) { return sim ? (int)sim->blocks.size() : -1; }
+
+int chkn_block_dims(chkn_sim* sim, int blk, int* dims3)
+{
+    if (!sim || !dims3 || blk < 0 || blk >= (int)sim->blocks.size()) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    dims3[0] = sim->blocks[blk].nic; dims3[1] = sim->blocks[blk].njc; dims3[2] = sim->blocks[blk].nkc;
+    return CHKN_OK;
+}
+
+long long chkn_active_cells(chkn_sim* sim)
+{
+    if (!sim) return -1;
+    long long n = 0;
+    for (auto& b : sim->blocks) if (b.active) n += b.n;
+    return n;
+}
+
+int chkn_last_timing(chkn_sim* sim, double* ms, long long* launches)
+{
+    if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
+    if (ms) *ms = sim->last_ms;
+    if (launches) *launches = sim->last_launches;
+    return CHKN_OK;
+}
+
+const char* chkn_last_error(chkn_sim* sim) { return sim ? sim->err.c_str() : g_create_error.c_str(); }
+
+void chkn_destroy(chkn_sim* sim)
+{
+    if (!sim) return;
+    free_device(sim);
+    delete sim;
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+/* chkn.h -- C-ABI of libchkn, the MI355X (gfx950) explicit finite-volume step of chicken.
+ *
+ * The reference has no plugin or FFI surface for its time step: the step is chosen at
+ * compile time between march_in_time_using_cpu_only and march_in_time_using_gpu
+ * (src/chicken/main.cu:71-80) and runs between initialize_simulation and
+ * finalize_simulation on process-global state (src/chicken/simulate.cu:32-46).  The entry
+ * points below are that seam as a library: each cites the reference function it replaces.
+ * Plain pointers and sizes only; all device memory is owned by the handle.
+ *
+ * Return codes: 0 = success; > 0 = number of bad cells found in the first failing stage
+ * (the reference throws "Stage k bad cell count: n", simulate.cu:243-245, :264-266,
+ * :285-287; the same text is available from chkn_last_error); < 0 = error class below.
+ */
+#ifndef CHKN_H
+#define CHKN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHKN_OK 0
+#define CHKN_ERR_ARG (-1)     /* invalid argument or unsupported configuration */
+#define CHKN_ERR_IO (-2)      /* job file missing or malformed */
+#define CHKN_ERR_CONFIG (-3)  /* inconsistent configuration, e.g. gas-model mismatch */
+#define CHKN_ERR_HIP (-4)     /* HIP runtime error */
+#define CHKN_ERR_STATE (-5)   /* call not valid in the handle's current state */
+
+/* Boundary-condition codes, identical to src/chicken/face.cu:26-32. */
+#define CHKN_BC_WALL_WITH_SLIP 0
+#define CHKN_BC_WALL_NO_SLIP_ADIABATIC 1
+#define CHKN_BC_WALL_NO_SLIP_FIXED_T 2
+#define CHKN_BC_EXCHANGE 3
+#define CHKN_BC_INFLOW 4
+#define CHKN_BC_OUTFLOW 5
+#define CHKN_BC_INFLOW_FUNCTION 6
+
+/* Number of IO variables per cell in flow files, src/chicken/cell.cu:66-84. */
+#define CHKN_N_IOVARS 13
+
+typedef struct chkn_sim chkn_sim;
+
+/* Array form of a job: what config.json plus the grid and flow files hold
+ * (src/chicken/config.cu:347-525, block.cu:785-930).  Block data are concatenated in block
+ * order.  flow_states holds (p, T, YB, velx, vely, velz) per state; rho, e and a follow
+ * from update_from_pT exactly as in config.cu:398-406. */
+typedef struct {
+    int nblocks, nib, njb, nkb;
+    const int* blk_ids;      /* nib*njb*nkb, index (i*njb + j)*nkb + k; -1 = no block */
+    const int* blk_ijk;      /* 3 per block */
+    const int* blk_dims;     /* 3 per block: nic, njc, nkc */
+    const int* active;       /* 1 per block */
+    const int* bc_codes;     /* 6 per block, face order iminus, iplus, jminus, jplus, kminus, kplus */
+    const int* bc_fs;        /* 6 per block: inflow flow-state index */
+    int n_flow_states;
+    const double* flow_states;
+    const double* vertices;  /* per block (nic+1)(njc+1)(nkc+1) xyz triples, i fastest */
+    const double* iovars;    /* per block 13*ncells, flow-file order */
+    int x_order;             /* 1 or 2 */
+    int flux_calc;           /* 0 = ausmdv (only ausmdv is built) */
+    int t_order;             /* parsed and ignored, as config.cu:491 */
+    int cfl_count, max_step, print_count;
+    double max_time, dt_init;
+    int n_cfl;
+    const double* cfl_times;
+    const double* cfl_values;
+    int n_dt_plot;
+    const double* dt_plot_times;
+    const double* dt_plot_values;
+    double t0;               /* start time (times.data entry for the start index) */
+    int start_tindx;
+    int reacting, viscous;   /* must be 0: not built */
+} chkn_desc;
+
+/* Progress of the march, filled by chkn_step and chkn_march. */
+typedef struct {
+    int step;          /* steps taken since creation (SimState::step) */
+    double t;          /* simulation time */
+    double dt;         /* time step used by the last step */
+    int bad_stage;     /* 1..3 if a stage produced bad cells, else 0 */
+    int bad_count;
+    int steps_done;    /* steps taken by this call */
+} chkn_report;
+
+/* Launch options (performance only; results do not depend on them). */
+typedef struct {
+    int tile_i, tile_j;   /* cell tile per work-group in i and j; 0 = library default */
+    int tile_k;           /* cells marched in k per work-group; 0 = default */
+    int per_block_launch; /* 1 = one launch per block per stage (the reference's launch
+                             structure, simulate.cu:605-625); 0 = one launch per stage */
+    int use_graph;        /* 1 = replay steps from a captured hipGraph */
+    int persistent;       /* 1 = persistent work-groups loop over the tile table */
+} chkn_launch_opts;
+
+/* Replaces initialize_simulation (simulate.cu:55-155), array form. */
+int chkn_create(const chkn_desc* desc, chkn_sim** out);
+
+/* Replaces initialize_simulation (simulate.cu:55-155) reading the job directory's
+ * config.json, grid and flow files (binary = 1: .bin, 0: .gz) at time index tindx. */
+int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out);
+
+/* Advance up to nsteps reference time steps (simulate.cu:205-308, without snapshot
+ * writes), stopping at max_step or max_time.  rep may be NULL. */
+int chkn_step(chkn_sim* sim, int nsteps, chkn_report* rep);
+
+/* Replaces march_in_time_using_cpu_only/_gpu (simulate.cu:182-342): march to max_step or
+ * max_time, writing snapshots on the dt_plot schedule.  Needs a job directory. */
+int chkn_march(chkn_sim* sim, int binary, chkn_report* rep);
+
+/* Replaces finalize_simulation (simulate.cu:768-792): final snapshot if steps are
+ * unwritten. */
+int chkn_finalize(chkn_sim* sim, int binary);
+
+/* Replaces write_flow_data (simulate.cu:158-179). */
+int chkn_write_flow(chkn_sim* sim, int tindx, int binary);
+
+/* The block's 13 IO variables (Block::writeFlow order, block.cu:943-954), 13*ncells. */
+int chkn_get_iovars(chkn_sim* sim, int blk, double* out);
+
+/* Conserved quantities of the block, 6*ncells, quantity-major (U0 of the next step). */
+int chkn_get_conserved(chkn_sim* sim, int blk, double* out);
+
+/* Replace the block's state by primitive IO variables (13*ncells, only p..velz are read);
+ * the conserved state is re-encoded as at start-up (simulate.cu:196-203). */
+int chkn_set_iovars(chkn_sim* sim, int blk, const double* iovars);
+
+int chkn_set_launch_opts(chkn_sim* sim, const chkn_launch_opts* opts);
+int chkn_get_report(chkn_sim* sim, chkn_report* rep);
+int chkn_num_blocks(chkn_sim* sim);
+int chkn_block_dims(chkn_sim* sim, int blk, int* dims3);
+long long chkn_active_cells(chkn_sim* sim);
+
+/* Device time of the last chkn_step call's kernels, measured with HIP events on the
+ * handle's stream (ms), and the number of stage launches it issued. */
+int chkn_last_timing(chkn_sim* sim, double* ms, long long* launches);
+
+/* Error text of the last failing call on this handle (or of the last failed create/load
+ * when sim is NULL). */
+const char* chkn_last_error(chkn_sim* sim);
+
+void chkn_destroy(chkn_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1012 @@
+/* chkn_oracle.c -- CPU restatement of chicken's explicit FV time step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see chkn_oracle.h).  The product never links this file.
+ *
+ * Plain C99, FP64, built with -ffp-contract=off so that every a*b+c is two roundings, as in
+ * the reference's gcc -O2 x86-64 build.  Every expression keeps the reference's operand
+ * order and association; comments give the reference file:line each piece restates
+ * (paths relative to the reference's src/chicken/).
+ *
+ * Pinning (what checks this code, see tests/test_oracle_*.py):
+ *   - geometry, EOS, encode/decode, limiter, MUSCL, AUSMDV: closed forms and invariants
+ *     (axis-aligned boxes, consistency F(W,W) = Euler flux, supersonic upwinding,
+ *     mirror-state zero mass flux, s(a,a)=1, linear data reproduced exactly);
+ *   - flux on general states, hex volumes on warped cells, whole multi-step trajectories:
+ *     golden vectors written by the reference's own sources (tests/golden/), bit for bit.
+ */
+#include "chkn_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------------------ */
+/* Constants: number.cu:29-45, gas.cu:21-49 (ideal air, the IDEAL_AIR build).            */
+
+static const double zero = 0.0, half = 0.5, one_quarter = 0.25;
+static const double one_eighth = 0.125, one = 1.0, two = 2.0, three = 3.0, four = 4.0;
+static const double six = 6.0, seven = 7.0, eight = 8.0;
+#define ONE_SIXTH (1.0 / 6.0)
+#define FOUR_SIXTH (4.0 / 6.0)
+#define ONE_TWELFTH (1.0 / 12.0)
+
+static const double GAS_G = 1.4;
+static const double GAS_R = 287.1;
+static const double GAS_Q = 0.0;
+#define GAS_CV (GAS_R / (GAS_G - 1.0))
+
+static char g_err[512];
+const char* orc_last_error(void) { return g_err; }
+static void set_err(const char* m) { snprintf(g_err, sizeof g_err, "%s", m); }
+
+/* ------------------------------------------------------------------------------------ */
+/* Types.                                                                                 */
+
+typedef struct { double x, y, z; } V3;
+typedef struct { double rho, e, p, T, YB, a; V3 vel; } FS; /* FlowState, flow.cu:38-40 */
+
+typedef struct {
+    V3 pos; double vol, iL, jL, kL;
+    FS fs;
+    int face[6];
+} Cell;
+
+typedef struct {
+    V3 pos, n, t1, t2; double area;
+    double F[6];
+    int L1, L0, R0, R1;
+    FS fs; /* face-average state, face.cu:373 (kept for viscous use) */
+} Face;
+
+typedef struct {
+    int bi, bj, bk, nic, njc, nkc, active;
+    int ncells, n0c[6], n1c[6], nghost[6], first_ghost[6], ntot;
+    int nif, njf, nkf, nfaces;
+    int bc[6], bcfs[6];
+    Cell* cells;
+    Face* faces;
+    V3* vtx;
+    double* Q;  /* 2 levels x ncells x 6, cell-major (Q[level*ncells+i][j]) */
+    double* dQ; /* 3 levels */
+} Blk;
+
+struct orc_sim {
+    int nblocks, nib, njb, nkb;
+    int* blk_ids;
+    Blk* b;
+    int nfs;
+    FS* inflow;
+    int x_order, flux_calc, cfl_count, max_step;
+    double max_time, dt_init;
+    int n_cfl;
+    double* cfl_t;
+    double* cfl_v;
+    double t, dt;
+    int step;
+};
+
+/* ------------------------------------------------------------------------------------ */
+/* Vector3 (vector3.cu).                                                                  */
+
+static double dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; } /* :72-75 */
+
+/* vector3.cu:97-109 */
+static V3 to_local(V3 v, V3 n, V3 t1, V3 t2)
+{
+    V3 r; r.x = dot3(v, n); r.y = dot3(v, t1); r.z = dot3(v, t2); return r;
+}
+
+/* vector3.cu:111-119 */
+static V3 to_global(V3 v, V3 n, V3 t1, V3 t2)
+{
+    V3 r;
+    r.x = v.x * n.x + v.y * t1.x + v.z * t2.x;
+    r.y = v.x * n.y + v.y * t1.y + v.z * t2.y;
+    r.z = v.x * n.z + v.y * t1.z + v.z * t2.z;
+    return r;
+}
+
+/* vector3.cu:77-93 */
+static V3 normalize(V3 v)
+{
+    double m = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    if (m > zero) { v.x /= m; v.y /= m; v.z /= m; }
+    else { v.x = v.y = v.z = zero; }
+    const double small = 1.0e-30;
+    if (fabs(v.x) < small) v.x = zero;
+    if (fabs(v.y) < small) v.y = zero;
+    if (fabs(v.z) < small) v.z = zero;
+    return v;
+}
+
+/* vector3.cu:150-157, cross(v3, v1, v2) */
+static V3 cross_v(V3 v1, V3 v2)
+{
+    V3 r;
+    r.x = v1.y * v2.z - v2.y * v1.z;
+    r.y = v2.x * v1.z - v1.x * v2.z;
+    r.z = v1.x * v2.y - v2.x * v1.y;
+    return r;
+}
+
+/* vector3.cu:179-187, cross_product(a, b, c) */
+static void cross_c(double ax, double ay, double az, double bx, double by, double bz,
+                    double* cx, double* cy, double* cz)
+{
+    *cx = ay * bz - az * by;
+    *cy = az * bx - ax * bz;
+    *cz = ax * by - ay * bx;
+}
+
+/* vector3.cu:196-245 */
+static void quad_properties(V3 p0, V3 p1, V3 p2, V3 p3, V3* c, V3* n, V3* t1, V3* t2,
+                            double* area)
+{
+    c->x = one_quarter * (p0.x + p1.x + p2.x + p3.x);
+    c->y = one_quarter * (p0.y + p1.y + p2.y + p3.y);
+    c->z = one_quarter * (p0.z + p1.z + p2.z + p3.z);
+    double p01x = p1.x - p0.x + p2.x - p3.x;
+    double p01y = p1.y - p0.y + p2.y - p3.y;
+    double p01z = p1.z - p0.z + p2.z - p3.z;
+    double p03x = p3.x - p0.x + p2.x - p1.x;
+    double p03y = p3.y - p0.y + p2.y - p1.y;
+    double p03z = p3.z - p0.z + p2.z - p1.z;
+    double vx, vy, vz;
+    cross_c(p01x, p01y, p01z, p03x, p03y, p03z, &vx, &vy, &vz);
+    vx *= one_quarter; vy *= one_quarter; vz *= one_quarter;
+    *area = sqrt(vx * vx + vy * vy + vz * vz);
+    if (*area > 1.0e-20) {
+        n->x = vx / *area; n->y = vy / *area; n->z = vz / *area;
+        V3 t = {p01x, p01y, p01z};
+        *t1 = normalize(t);
+        *t2 = normalize(cross_v(*n, *t1));
+    } else {
+        n->x = one; n->y = zero; n->z = zero;
+        t1->x = zero; t1->y = one; t1->z = zero;
+        t2->x = zero; t2->y = zero; t2->z = one;
+    }
+}
+
+/* vector3.cu:259-270 */
+static double tet_volume(V3 p0, V3 p1, V3 p2, V3 p3)
+{
+    double dx01 = p1.x - p0.x, dy01 = p1.y - p0.y, dz01 = p1.z - p0.z;
+    double dx02 = p2.x - p0.x, dy02 = p2.y - p0.y, dz02 = p2.z - p0.z;
+    double cx, cy, cz;
+    cross_c(dx01, dy01, dz01, dx02, dy02, dz02, &cx, &cy, &cz);
+    double dx03 = p3.x - p0.x, dy03 = p3.y - p0.y, dz03 = p3.z - p0.z;
+    return (dx03 * cx + dy03 * cy + dz03 * cz) / six;
+}
+
+/* vector3.cu:309-348 with true_centroid = true, as called from hex_cell_properties */
+static void pyramid_props(V3 p0, V3 p1, V3 p2, V3 p3, V3 p4, V3* cen, double* vol)
+{
+    V3 pmB;
+    pmB.x = one_quarter * (p0.x + p1.x + p2.x + p3.x);
+    pmB.y = one_quarter * (p0.y + p1.y + p2.y + p3.y);
+    pmB.z = one_quarter * (p0.z + p1.z + p2.z + p3.z);
+    V3 q[4][4] = {{p0, p1, pmB, p4}, {p1, p2, pmB, p4}, {p2, p3, pmB, p4}, {p3, p0, pmB, p4}};
+    double v = zero; V3 m = {zero, zero, zero};
+    for (int t = 0; t < 4; ++t) {
+        /* tetrahedron_properties, vector3.cu:272-281 */
+        V3 tc;
+        tc.x = one_quarter * (q[t][0].x + q[t][1].x + q[t][2].x + q[t][3].x);
+        tc.y = one_quarter * (q[t][0].y + q[t][1].y + q[t][2].y + q[t][3].y);
+        tc.z = one_quarter * (q[t][0].z + q[t][1].z + q[t][2].z + q[t][3].z);
+        double tv = tet_volume(q[t][0], q[t][1], q[t][2], q[t][3]);
+        v += tv;
+        tc.x *= tv; tc.y *= tv; tc.z *= tv;
+        m.x += tc.x; m.y += tc.y; m.z += tc.z;
+    }
+    if (fabs(v) > zero) { m.x /= v; m.y /= v; m.z /= v; }
+    *cen = m; *vol = v;
+}
+
+/* vector3.cu:350-433 with true_centroid = false (block.cu:642-646) */
+static void hex_props(const V3 p[8], V3* cen, double* vol, double* iL, double* jL,
+                      double* kL)
+{
+    cen->x = one_eighth * (p[0].x + p[1].x + p[2].x + p[3].x + p[4].x + p[5].x + p[6].x + p[7].x);
+    cen->y = one_eighth * (p[0].y + p[1].y + p[2].y + p[3].y + p[4].y + p[5].y + p[6].y + p[7].y);
+    cen->z = one_eighth * (p[0].z + p[1].z + p[2].z + p[3].z + p[4].z + p[5].z + p[6].z + p[7].z);
+#define MID4(a, b, c, d, r) do { \
+        (r).x = one_quarter * (p[a].x + p[b].x + p[c].x + p[d].x); \
+        (r).y = one_quarter * (p[a].y + p[b].y + p[c].y + p[d].y); \
+        (r).z = one_quarter * (p[a].z + p[b].z + p[c].z + p[d].z); } while (0)
+    V3 pmN, pmE, pmS, pmW, pmT, pmB;
+    MID4(3, 2, 6, 7, pmN);
+    MID4(1, 2, 6, 5, pmE);
+    MID4(0, 1, 5, 4, pmS);
+    MID4(0, 3, 7, 4, pmW);
+    MID4(4, 5, 6, 7, pmT);
+    MID4(0, 1, 2, 3, pmB);
+#undef MID4
+    double dx, dy, dz;
+    dx = pmE.x - pmW.x; dy = pmE.y - pmW.y; dz = pmE.z - pmW.z;
+    *iL = sqrt(dx * dx + dy * dy + dz * dz);
+    dx = pmN.x - pmS.x; dy = pmN.y - pmS.y; dz = pmN.z - pmS.z;
+    *jL = sqrt(dx * dx + dy * dy + dz * dz);
+    dx = pmT.x - pmB.x; dy = pmT.y - pmB.y; dz = pmT.z - pmB.z;
+    *kL = sqrt(dx * dx + dy * dy + dz * dz);
+    static const int pyr[6][4] = {{6, 7, 3, 2}, {5, 6, 2, 1}, {4, 5, 1, 0},
+                                  {7, 4, 0, 3}, {7, 6, 5, 4}, {0, 1, 2, 3}};
+    double v = zero; V3 m = {zero, zero, zero};
+    for (int q = 0; q < 6; ++q) {
+        V3 sc; double sv;
+        pyramid_props(p[pyr[q][0]], p[pyr[q][1]], p[pyr[q][2]], p[pyr[q][3]], *cen, &sc, &sv);
+        v += sv;
+        sc.x *= sv; sc.y *= sv; sc.z *= sv;
+        m.x += sc.x; m.y += sc.y; m.z += sc.z;
+    }
+    if ((v < zero && fabs(v) < 1.0e-12) || (v >= zero && v < 1.0e-20)) v = zero;
+    *vol = v;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Gas and flow state (gas.cu, flow.cu).                                                  */
+
+static void update_from_pT(FS* s) /* gas.cu:119-126 */
+{
+    s->e = s->T * GAS_CV - s->YB * GAS_Q;
+    s->rho = s->p / (s->T * GAS_R);
+    s->a = sqrt(GAS_G * GAS_R * s->T);
+}
+
+static void update_from_rhoe(FS* s) /* gas.cu:128-135 */
+{
+    s->T = (s->e + s->YB * GAS_Q) / GAS_CV;
+    s->p = s->rho * GAS_R * s->T;
+    s->a = sqrt(GAS_G * GAS_R * s->T);
+}
+
+static void encode(const FS* s, double* U) /* flow.cu:227-243 */
+{
+    double rho = s->rho;
+    U[0] = rho;
+    U[1] = rho * s->vel.x;
+    U[2] = rho * s->vel.y;
+    U[3] = rho * s->vel.z;
+    double ke = half * (s->vel.x * s->vel.x + s->vel.y * s->vel.y + s->vel.z * s->vel.z);
+    U[4] = rho * (s->e + ke);
+    U[5] = rho * s->YB;
+}
+
+static int decode(FS* s, const double* U) /* flow.cu:245-268 */
+{
+    for (int i = 0; i < 6; ++i) if (isnan(U[i])) return 1;
+    double rho = U[0];
+    if (rho <= zero) return 1;
+    double dinv = one / rho;
+    s->vel.x = U[1] * dinv; s->vel.y = U[2] * dinv; s->vel.z = U[3] * dinv;
+    double e = U[4] * dinv;
+    double ke = half * (s->vel.x * s->vel.x + s->vel.y * s->vel.y + s->vel.z * s->vel.z);
+    e -= ke;
+    s->rho = rho;
+    s->e = e;
+    s->YB = U[5] * dinv;
+    update_from_rhoe(s);
+    return 0;
+}
+
+static void set_as_average(FS* s, const FS* a, const FS* b) /* flow.cu:220-225 */
+{
+    s->rho = half * (a->rho + b->rho);
+    s->e = half * (a->e + b->e);
+    s->YB = half * (a->YB + b->YB);
+    update_from_rhoe(s);
+    s->vel.x = half * (a->vel.x + b->vel.x);
+    s->vel.y = half * (a->vel.y + b->vel.y);
+    s->vel.z = half * (a->vel.z + b->vel.z);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Face flux (face.cu).                                                                   */
+
+static double van_albada(double a, double b) /* face.cu:72-79 */
+{
+    const double eps = 1.0e-12;
+    return (a * b + fabs(a * b) + eps) / (a * a + b * b + eps);
+}
+
+/* face.cu:81-96 */
+static void interp_l2r2(double qL1, double qL0, double qR0, double qR1, double* qL, double* qR)
+{
+    double delLminus = (qL0 - qL1);
+    double del = (qR0 - qL0);
+    double delRplus = (qR1 - qR0);
+    double sL = van_albada(delLminus, del);
+    double sR = van_albada(del, delRplus);
+    *qL = qL0 + sL * one_eighth * (three * del + delLminus);
+    *qR = qR0 - sR * one_eighth * (delRplus + three * del);
+}
+
+/* face.cu:163-265 */
+static void ausmdv(const FS* fsL, const FS* fsR, V3 n, V3 t1, V3 t2, double* F)
+{
+    V3 velL = to_local(fsL->vel, n, t1, t2);
+    V3 velR = to_local(fsR->vel, n, t1, t2);
+    double rhoL = fsL->rho, pL = fsL->p, pLrL = pL / rhoL;
+    double velxL = velL.x, velyL = velL.y, velzL = velL.z;
+    double uL = fsL->e, aL = fsL->a;
+    double keL = half * (velxL * velxL + velyL * velyL + velzL * velzL);
+    double HL = uL + pLrL + keL;
+    double YBL = fsL->YB;
+    double rhoR = fsR->rho, pR = fsR->p, pRrR = pR / rhoR;
+    double velxR = velR.x, velyR = velR.y, velzR = velR.z;
+    double uR = fsR->e, aR = fsR->a;
+    double keR = half * (velxR * velxR + velyR * velyR + velzR * velzR);
+    double HR = uR + pR / rhoR + keR;
+    double YBR = fsR->YB;
+    double alphaL = two * pLrL / (pLrL + pRrR);
+    double alphaR = two * pRrR / (pLrL + pRrR);
+    double am = fmax(aL, aR);
+    double ML = velxL / am;
+    double MR = velxR / am;
+    double pLplus, velxLplus;
+    double dvelxL = half * (velxL + fabs(velxL));
+    if (fabs(ML) <= one) {
+        pLplus = pL * (ML + one) * (ML + one) * (two - ML) * one_quarter;
+        velxLplus = alphaL * ((velxL + am) * (velxL + am) / (four * am) - dvelxL) + dvelxL;
+    } else {
+        pLplus = pL * dvelxL / velxL;
+        velxLplus = dvelxL;
+    }
+    double pRminus, velxRminus;
+    double dvelxR = half * (velxR - fabs(velxR));
+    if (fabs(MR) <= one) {
+        pRminus = pR * (MR - one) * (MR - one) * (two + MR) * one_quarter;
+        velxRminus = alphaR * (-(velxR - am) * (velxR - am) / (four * am) - dvelxR) + dvelxR;
+    } else {
+        pRminus = pR * dvelxR / velxR;
+        velxRminus = dvelxR;
+    }
+    double massL = velxLplus * rhoL;
+    double massR = velxRminus * rhoR;
+    double mass_half = massL + massR;
+    double p_half = pLplus + pRminus;
+    double dp = pL - pR;
+    const double K_SWITCH = 10.0;
+    dp = K_SWITCH * fabs(dp) / fmin(pL, pR);
+    double s = half * fmin(one, dp);
+    double rvel2_AUSMV = massL * velxL + massR * velxR;
+    double rvel2_AUSMD = half * (mass_half * (velxL + velxR) - fabs(mass_half) * (velxR - velxL));
+    double rvel2_half = (half + s) * rvel2_AUSMV + (half - s) * rvel2_AUSMD;
+    F[0] = mass_half;
+    double vely = (mass_half >= zero) ? velyL : velyR;
+    double velz = (mass_half >= zero) ? velzL : velzR;
+    V3 mom = {rvel2_half + p_half, mass_half * vely, mass_half * velz};
+    mom = to_global(mom, n, t1, t2);
+    F[1] = mom.x; F[2] = mom.y; F[3] = mom.z;
+    double H = (mass_half >= zero) ? HL : HR;
+    F[4] = mass_half * H;
+    double YB = (mass_half >= zero) ? YBL : YBR;
+    F[5] = mass_half * YB;
+}
+
+/* face.cu:268-342 */
+static void sbp_asf(const FS* s0, const FS* s1, const FS* s2, const FS* s3, V3 n, V3 t1, V3 t2,
+                    double* F)
+{
+    const FS* g[4] = {s0, s1, s2, s3};
+    V3 vel[4];
+    for (int i = 0; i < 4; ++i) vel[i] = to_local(g[i]->vel, n, t1, t2);
+    double v[10][4], w[10][4];
+    for (int i = 0; i < 4; ++i) {
+        double rho = g[i]->rho, p = g[i]->p, e = g[i]->e;
+        double velx = vel[i].x, vely = vel[i].y, velz = vel[i].z;
+        v[0][i] = rho;             w[0][i] = velx;
+        v[1][i] = velx * rho;      w[1][i] = velx;
+        v[2][i] = vely * rho;      w[2][i] = velx;
+        v[3][i] = velz * rho;      w[3][i] = velx;
+        v[4][i] = e * rho;         w[4][i] = velx;
+        v[5][i] = velx * velx * rho; w[5][i] = velx;
+        v[6][i] = vely * vely * rho; w[6][i] = velx;
+        v[7][i] = velz * velz * rho; w[7][i] = velx;
+        v[8][i] = p;               w[8][i] = velx;
+        v[9][i] = p;               w[9][i] = 1;
+    }
+    double f_c[10], f_e[10];
+    for (int j = 0; j < 10; ++j) {
+        f_c[j] = ONE_TWELFTH * (-v[j][0] * w[j][0] + seven * v[j][1] * w[j][1]
+                                + seven * v[j][2] * w[j][2] - v[j][3] * w[j][3]);
+        f_e[j] = ONE_TWELFTH * (-v[j][0] * w[j][2] - v[j][2] * w[j][0] + eight * v[j][1] * w[j][2]
+                                + eight * v[j][2] * w[j][1] - v[j][1] * w[j][3]
+                                - v[j][3] * w[j][1]);
+    }
+    const double alpha_mass = 1.0, alpha_mom = 0.5, alpha_ie = 0.5, alpha_ke = 0.0, alpha_p = 0.0;
+    double mass_flux = alpha_mass * f_c[0] + (one - alpha_mass) * f_e[0];
+    F[0] = mass_flux;
+    double mom_x = alpha_mom * f_c[1] + (one - alpha_mom) * f_e[1]
+                   + (alpha_p * f_c[9] + (one - alpha_p) * f_e[9]);
+    double mom_y = alpha_mom * f_c[2] + (one - alpha_mom) * f_e[2];
+    double mom_z = alpha_mom * f_c[3] + (one - alpha_mom) * f_e[3];
+    V3 mom = {mom_x, mom_y, mom_z};
+    mom = to_global(mom, n, t1, t2);
+    F[1] = mom.x; F[2] = mom.y; F[3] = mom.z;
+    F[4] = alpha_ie * f_c[4] + (one - alpha_ie) * f_e[4]
+           + half * (alpha_ke * f_c[5] + (one - alpha_ke) * f_e[5]
+                     + alpha_ke * f_c[6] + (one - alpha_ke) * f_e[6]
+                     + alpha_ke * f_c[7] + (one - alpha_ke) * f_e[7])
+           + alpha_p * f_c[8] + (one - alpha_p) * f_e[8];
+    F[5] = mass_flux * ((mass_flux >= zero) ? g[1]->YB : g[2]->YB);
+}
+
+/* face.cu:347-378 */
+static void face_flux(Face* f, const FS* L1, const FS* L0, const FS* R0, const FS* R1,
+                      int flux_calc, int x_order)
+{
+    if (flux_calc == 0) {
+        FS L = *L0, R = *R0;
+        if (x_order > 1) {
+            interp_l2r2(L1->rho, L0->rho, R0->rho, R1->rho, &L.rho, &R.rho);
+            interp_l2r2(L1->e, L0->e, R0->e, R1->e, &L.e, &R.e);
+            interp_l2r2(L1->YB, L0->YB, R0->YB, R1->YB, &L.YB, &R.YB);
+            update_from_rhoe(&L);
+            update_from_rhoe(&R);
+            interp_l2r2(L1->vel.x, L0->vel.x, R0->vel.x, R1->vel.x, &L.vel.x, &R.vel.x);
+            interp_l2r2(L1->vel.y, L0->vel.y, R0->vel.y, R1->vel.y, &L.vel.y, &R.vel.y);
+            interp_l2r2(L1->vel.z, L0->vel.z, R0->vel.z, R1->vel.z, &L.vel.z, &R.vel.z);
+        }
+        ausmdv(&L, &R, f->n, f->t1, f->t2, f->F);
+        set_as_average(&f->fs, &L, &R);
+    } else if (flux_calc == 1) {
+        sbp_asf(L1, L0, R0, R1, f->n, f->t1, f->t2, f->F);
+        set_as_average(&f->fs, L0, R0);
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Block indexing and configuration (config.cu:152-225, block.cu:194-564).               */
+
+enum { IM = 0, IP = 1, JM = 2, JP = 3, KM = 4, KP = 5 };
+
+static int cidx(const Blk* b, int i, int j, int k) { return k * b->nic * b->njc + j * b->nic + i; }
+static int ghost(const Blk* b, int f, int i0, int i1, int depth) /* config.cu:195-201 */
+{
+    return b->first_ghost[f] + b->n0c[f] * b->n1c[f] * depth + i1 * b->n0c[f] + i0;
+}
+static int ifi(const Blk* b, int i, int j, int k) { return i * b->njc * b->nkc + k * b->njc + j; }
+static int jfi(const Blk* b, int i, int j, int k) { return b->nif + j * b->nic * b->nkc + k * b->nic + i; }
+static int kfi(const Blk* b, int i, int j, int k) { return b->nif + b->njf + k * b->nic * b->njc + j * b->nic + i; }
+static int vidx(const Blk* b, int i, int j, int k) { return k * (b->nic + 1) * (b->njc + 1) + j * (b->nic + 1) + i; }
+
+/* Stencil cell for position m along a line of n cells: interior index, or ghost of the
+ * minus/plus face at depth -m-1 / m-n.  This is the rule the explicit cases of
+ * block.cu:272-316 (i), :373-417 (j), :474-518 (k) encode, including n == 2. */
+static int line_cell(const Blk* b, int dir, int m, int a, int c)
+{
+    int n = (dir == 0) ? b->nic : (dir == 1) ? b->njc : b->nkc;
+    int fm = 2 * dir, fp = 2 * dir + 1;
+    /* (a, c) are the transverse indices in the order ghostCellIndex expects (i0, i1). */
+    if (m < 0) return ghost(b, fm, a, c, -m - 1);
+    if (m >= n) return ghost(b, fp, a, c, m - n);
+    if (dir == 0) return cidx(b, m, a, c);
+    if (dir == 1) return cidx(b, a, m, c);
+    return cidx(b, a, c, m);
+}
+
+static int configure(Blk* b) /* config.cu:152-185, block.cu:194-564 */
+{
+    int nic = b->nic, njc = b->njc, nkc = b->nkc;
+    if (nic < 2) { set_err("Too few cells in the i-index direction."); return -1; }
+    if (njc < 2) { set_err("Too few cells in the j-index direction."); return -1; }
+    if (nkc < 2) { set_err("Too few cells in the k-index direction."); return -1; }
+    b->ncells = nic * njc * nkc;
+    b->n0c[IM] = njc; b->n1c[IM] = nkc; b->n0c[IP] = njc; b->n1c[IP] = nkc;
+    b->n0c[JM] = nic; b->n1c[JM] = nkc; b->n0c[JP] = nic; b->n1c[JP] = nkc;
+    b->n0c[KM] = nic; b->n1c[KM] = njc; b->n0c[KP] = nic; b->n1c[KP] = njc;
+    int ng = 0;
+    for (int f = 0; f < 6; ++f) {
+        b->nghost[f] = b->active ? 2 * b->n0c[f] * b->n1c[f] : 0;
+        b->first_ghost[f] = (f > 0) ? b->first_ghost[f - 1] + b->nghost[f - 1] : b->ncells;
+        ng += b->nghost[f];
+    }
+    b->ntot = b->ncells + ng;
+    b->nif = (nic + 1) * njc * nkc;
+    b->njf = nic * (njc + 1) * nkc;
+    b->nkf = nic * njc * (nkc + 1);
+    b->nfaces = b->nif + b->njf + b->nkf;
+    b->cells = (Cell*)calloc((size_t)b->ntot, sizeof(Cell));
+    b->faces = (Face*)calloc((size_t)b->nfaces, sizeof(Face));
+    b->vtx = (V3*)calloc((size_t)(nic + 1) * (njc + 1) * (nkc + 1), sizeof(V3));
+    if (b->active) {
+        b->Q = (double*)calloc((size_t)b->ncells * 2 * 6, sizeof(double));
+        b->dQ = (double*)calloc((size_t)b->ncells * 3 * 6, sizeof(double));
+    }
+    if (!b->cells || !b->faces || !b->vtx || (b->active && (!b->Q || !b->dQ))) {
+        set_err("allocation failed"); return -1;
+    }
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < njc; ++j)
+            for (int i = 0; i < nic; ++i) {
+                Cell* c = &b->cells[cidx(b, i, j, k)];
+                c->face[IM] = ifi(b, i, j, k); c->face[IP] = ifi(b, i + 1, j, k);
+                c->face[JM] = jfi(b, i, j, k); c->face[JP] = jfi(b, i, j + 1, k);
+                c->face[KM] = kfi(b, i, j, k); c->face[KP] = kfi(b, i, j, k + 1);
+            }
+    /* Only active blocks have ghost cells; an inactive block never evaluates fluxes. */
+    if (!b->active) return 0;
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < njc; ++j)
+            for (int i = 0; i <= nic; ++i) {
+                Face* f = &b->faces[ifi(b, i, j, k)];
+                f->L1 = line_cell(b, 0, i - 2, j, k); f->L0 = line_cell(b, 0, i - 1, j, k);
+                f->R0 = line_cell(b, 0, i, j, k);     f->R1 = line_cell(b, 0, i + 1, j, k);
+            }
+    for (int k = 0; k < nkc; ++k)
+        for (int i = 0; i < nic; ++i)
+            for (int j = 0; j <= njc; ++j) {
+                Face* f = &b->faces[jfi(b, i, j, k)];
+                f->L1 = line_cell(b, 1, j - 2, i, k); f->L0 = line_cell(b, 1, j - 1, i, k);
+                f->R0 = line_cell(b, 1, j, i, k);     f->R1 = line_cell(b, 1, j + 1, i, k);
+            }
+    for (int j = 0; j < njc; ++j)
+        for (int i = 0; i < nic; ++i)
+            for (int k = 0; k <= nkc; ++k) {
+                Face* f = &b->faces[kfi(b, i, j, k)];
+                f->L1 = line_cell(b, 2, k - 2, i, j); f->L0 = line_cell(b, 2, k - 1, i, j);
+                f->R0 = line_cell(b, 2, k, i, j);     f->R1 = line_cell(b, 2, k + 1, i, j);
+            }
+    return 0;
+}
+
+/* block.cu:633-783 */
+static void compute_geometry(Blk* b)
+{
+    int nic = b->nic, njc = b->njc, nkc = b->nkc;
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < njc; ++j)
+            for (int i = 0; i < nic; ++i) {
+                V3 p[8];
+                p[0] = b->vtx[vidx(b, i, j, k)];         p[1] = b->vtx[vidx(b, i + 1, j, k)];
+                p[2] = b->vtx[vidx(b, i + 1, j + 1, k)]; p[3] = b->vtx[vidx(b, i, j + 1, k)];
+                p[4] = b->vtx[vidx(b, i, j, k + 1)];     p[5] = b->vtx[vidx(b, i + 1, j, k + 1)];
+                p[6] = b->vtx[vidx(b, i + 1, j + 1, k + 1)]; p[7] = b->vtx[vidx(b, i, j + 1, k + 1)];
+                Cell* c = &b->cells[cidx(b, i, j, k)];
+                hex_props(p, &c->pos, &c->vol, &c->iL, &c->jL, &c->kL);
+            }
+    /* Face vertex orders, block.cu:267-270 (i), :368-371 (j), :469-472 (k). */
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < njc; ++j)
+            for (int i = 0; i <= nic; ++i) {
+                Face* f = &b->faces[ifi(b, i, j, k)];
+                quad_properties(b->vtx[vidx(b, i, j, k)], b->vtx[vidx(b, i, j + 1, k)],
+                                b->vtx[vidx(b, i, j + 1, k + 1)], b->vtx[vidx(b, i, j, k + 1)],
+                                &f->pos, &f->n, &f->t1, &f->t2, &f->area);
+            }
+    for (int k = 0; k < nkc; ++k)
+        for (int i = 0; i < nic; ++i)
+            for (int j = 0; j <= njc; ++j) {
+                Face* f = &b->faces[jfi(b, i, j, k)];
+                quad_properties(b->vtx[vidx(b, i + 1, j, k)], b->vtx[vidx(b, i, j, k)],
+                                b->vtx[vidx(b, i, j, k + 1)], b->vtx[vidx(b, i + 1, j, k + 1)],
+                                &f->pos, &f->n, &f->t1, &f->t2, &f->area);
+            }
+    for (int j = 0; j < njc; ++j)
+        for (int i = 0; i < nic; ++i)
+            for (int k = 0; k <= nkc; ++k) {
+                Face* f = &b->faces[kfi(b, i, j, k)];
+                quad_properties(b->vtx[vidx(b, i, j, k)], b->vtx[vidx(b, i + 1, j, k)],
+                                b->vtx[vidx(b, i + 1, j + 1, k)], b->vtx[vidx(b, i, j + 1, k)],
+                                &f->pos, &f->n, &f->t1, &f->t2, &f->area);
+            }
+    if (!b->active) return;
+    /* Ghost positions and lengths, block.cu:656-780: g0 = f + d, g1 = f + 3d, d = f - c0. */
+    for (int dir = 0; dir < 3; ++dir)
+        for (int side = 0; side < 2; ++side) {
+            int na = (dir == 0) ? njc : nic, nc = (dir == 2) ? njc : nkc;
+            for (int c2 = 0; c2 < nc; ++c2)
+                for (int a = 0; a < na; ++a) {
+                    int fi;
+                    if (dir == 0) fi = ifi(b, side ? nic : 0, a, c2);
+                    else if (dir == 1) fi = jfi(b, a, side ? njc : 0, c2);
+                    else fi = kfi(b, a, c2, side ? nkc : 0);
+                    Face* f = &b->faces[fi];
+                    Cell* c0 = &b->cells[side ? f->L0 : f->R0];
+                    Cell* g0 = &b->cells[side ? f->R0 : f->L0];
+                    Cell* g1 = &b->cells[side ? f->R1 : f->L1];
+                    g0->iL = c0->iL; g0->jL = c0->jL; g0->kL = c0->kL;
+                    V3 d = {f->pos.x - c0->pos.x, f->pos.y - c0->pos.y, f->pos.z - c0->pos.z};
+                    g0->pos.x = f->pos.x + d.x; g0->pos.y = f->pos.y + d.y; g0->pos.z = f->pos.z + d.z;
+                    g1->iL = c0->iL; g1->jL = c0->jL; g1->kL = c0->kL;
+                    d.x *= 3.0; d.y *= 3.0; d.z *= 3.0;
+                    g1->pos.x = f->pos.x + d.x; g1->pos.y = f->pos.y + d.y; g1->pos.z = f->pos.z + d.z;
+                }
+        }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Boundary conditions (bcs.cu:16-544).                                                   */
+
+static int boundary_face(const Blk* b, int bf, int a, int c)
+{
+    switch (bf) {
+    case IM: return ifi(b, 0, a, c);
+    case IP: return ifi(b, b->nic, a, c);
+    case JM: return jfi(b, a, 0, c);
+    case JP: return jfi(b, a, b->njc, c);
+    case KM: return kfi(b, a, c, 0);
+    default: return kfi(b, a, c, b->nkc);
+    }
+}
+
+static void apply_bc(orc_sim* s, int ib, int bf)
+{
+    Blk* b = &s->b[ib];
+    int plus = bf & 1;
+    int na = (bf < 2) ? b->njc : b->nic;
+    int nc = (bf < 4) ? b->nkc : b->njc;
+    const Blk* o = NULL;
+    if (b->bc[bf] == ORC_EXCHANGE) { /* bcs.cu:135-254, with periodic wrap :147 etc. */
+        int oi = b->bi, oj = b->bj, ok = b->bk;
+        if (bf == IM) { oi = b->bi - 1; if (oi < 0) oi = s->nib - 1; }
+        if (bf == IP) { oi = b->bi + 1; if (oi >= s->nib) oi = 0; }
+        if (bf == JM) { oj = b->bj - 1; if (oj < 0) oj = s->njb - 1; }
+        if (bf == JP) { oj = b->bj + 1; if (oj >= s->njb) oj = 0; }
+        if (bf == KM) { ok = b->bk - 1; if (ok < 0) ok = s->nkb - 1; }
+        if (bf == KP) { ok = b->bk + 1; if (ok >= s->nkb) ok = 0; }
+        o = &s->b[s->blk_ids[(oi * s->njb + oj) * s->nkb + ok]];
+    }
+    for (int c = 0; c < nc; ++c)
+        for (int a = 0; a < na; ++a) {
+            Face* f = &b->faces[boundary_face(b, bf, a, c)];
+            int g0 = plus ? f->R0 : f->L0, g1 = plus ? f->R1 : f->L1;
+            const FS* in = &b->cells[plus ? f->L0 : f->R0].fs;
+            switch (b->bc[bf]) {
+            case ORC_WALL_SLIP: case ORC_WALL_NOSLIP_ADIABATIC: case ORC_WALL_NOSLIP_FIXED_T: {
+                FS r = *in; /* bcs.cu:26-37 */
+                V3 v = to_local(r.vel, f->n, f->t1, f->t2);
+                v.x = -(v.x);
+                r.vel = to_global(v, f->n, f->t1, f->t2);
+                b->cells[g0].fs = r;
+                b->cells[g1].fs = r;
+                break;
+            }
+            case ORC_EXCHANGE: {
+                /* Ghost depth d = the other block's interior cell at depth d from the shared
+                 * face (other face's left/right cells, bcs.cu:155-158). */
+                int m0, m1;
+                if (bf == IM) { m0 = cidx(o, o->nic - 1, a, c); m1 = cidx(o, o->nic - 2, a, c); }
+                else if (bf == IP) { m0 = cidx(o, 0, a, c); m1 = cidx(o, 1, a, c); }
+                else if (bf == JM) { m0 = cidx(o, a, o->njc - 1, c); m1 = cidx(o, a, o->njc - 2, c); }
+                else if (bf == JP) { m0 = cidx(o, a, 0, c); m1 = cidx(o, a, 1, c); }
+                else if (bf == KM) { m0 = cidx(o, a, c, o->nkc - 1); m1 = cidx(o, a, c, o->nkc - 2); }
+                else { m0 = cidx(o, a, c, 0); m1 = cidx(o, a, c, 1); }
+                b->cells[g0].fs = o->cells[m0].fs;
+                b->cells[g1].fs = o->cells[m1].fs;
+                break;
+            }
+            case ORC_INFLOW: /* bcs.cu:257-319 */
+                b->cells[g0].fs = s->inflow[b->bcfs[bf]];
+                b->cells[g1].fs = s->inflow[b->bcfs[bf]];
+                break;
+            case ORC_OUTFLOW: /* bcs.cu:435-503 */
+                b->cells[g0].fs = *in;
+                b->cells[g1].fs = *in;
+                break;
+            default:
+                break;
+            }
+        }
+}
+
+/* bcs.cu:506-544 */
+static void apply_all_bcs(orc_sim* s)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int ib = 0; ib < s->nblocks; ++ib) {
+        if (!s->b[ib].active) continue;
+        for (int bf = 0; bf < 6; ++bf) apply_bc(s, ib, bf);
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Cell-level operations (cell.cu, block.cu:978-1172).                                    */
+
+/* cell.cu:155-164 */
+static double local_dt(const Blk* b, const Cell* c, double cfl)
+{
+    V3 in = b->faces[c->face[IM]].n, jn = b->faces[c->face[JM]].n, kn = b->faces[c->face[KM]].n;
+    double is = c->iL / (fabs(dot3(c->fs.vel, in)) + c->fs.a);
+    double js = c->jL / (fabs(dot3(c->fs.vel, jn)) + c->fs.a);
+    double ks = c->kL / (fabs(dot3(c->fs.vel, kn)) + c->fs.a);
+    return cfl * fmin(fmin(is, js), ks);
+}
+
+/* cell.cu:186-217 */
+static void eval_dudt(const Blk* b, const Cell* c, double* d)
+{
+    double vol_inv = one / c->vol;
+    const Face* fim = &b->faces[c->face[IM]]; const Face* fip = &b->faces[c->face[IP]];
+    const Face* fjm = &b->faces[c->face[JM]]; const Face* fjp = &b->faces[c->face[JP]];
+    const Face* fkm = &b->faces[c->face[KM]]; const Face* fkp = &b->faces[c->face[KP]];
+    for (int i = 0; i < 6; ++i) {
+        double si = fim->area * fim->F[i] - fip->area * fip->F[i]
+                    + fjm->area * fjm->F[i] - fjp->area * fjp->F[i]
+                    + fkm->area * fkm->F[i] - fkp->area * fkp->F[i];
+        d[i] = vol_inv * si;
+    }
+}
+
+/* block.cu:1018-1028 */
+static void block_fluxes(orc_sim* s, Blk* b)
+{
+    for (int fi = 0; fi < b->nfaces; ++fi) {
+        Face* f = &b->faces[fi];
+        face_flux(f, &b->cells[f->L1].fs, &b->cells[f->L0].fs, &b->cells[f->R0].fs,
+                  &b->cells[f->R1].fs, s->flux_calc, s->x_order);
+    }
+}
+
+/* block.cu:1090-1160 */
+static int block_update(Blk* b, int stage, double dt)
+{
+    int bad = 0, n = b->ncells;
+    for (int i = 0; i < n; ++i) {
+        Cell* c = &b->cells[i];
+        double* d0 = &b->dQ[(size_t)i * 6];
+        double* d1 = &b->dQ[((size_t)n + i) * 6];
+        double* d2 = &b->dQ[((size_t)2 * n + i) * 6];
+        double* U0 = &b->Q[(size_t)i * 6];
+        double* U1 = &b->Q[((size_t)n + i) * 6];
+        if (stage == 1) {
+            eval_dudt(b, c, d0);
+            for (int j = 0; j < 6; ++j) U1[j] = U0[j] + dt * d0[j];
+        } else if (stage == 2) {
+            eval_dudt(b, c, d1);
+            for (int j = 0; j < 6; ++j) U1[j] = U0[j] + one_quarter * dt * (d0[j] + d1[j]);
+        } else {
+            eval_dudt(b, c, d2);
+            for (int j = 0; j < 6; ++j)
+                U1[j] = U0[j] + dt * (ONE_SIXTH * d0[j] + ONE_SIXTH * d1[j] + FOUR_SIXTH * d2[j]);
+        }
+        bad += decode(&c->fs, U1);
+    }
+    return bad;
+}
+
+/* config.cu:263-269 */
+static double schedule_value(const orc_sim* s, double t)
+{
+    int i = s->n_cfl - 1;
+    while (i > 0 && t < s->cfl_t[i]) i--;
+    return s->cfl_v[i];
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Public API.                                                                            */
+
+orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
+                    const int* blk_ijk, const int* blk_dims, const int* active,
+                    const int* bc_codes, const int* bc_fs, int n_flow_states,
+                    const double* flow_states, const double* vertices, const double* iovars,
+                    const orc_params* p)
+{
+    orc_sim* s = (orc_sim*)calloc(1, sizeof(orc_sim));
+    if (!s) { set_err("allocation failed"); return NULL; }
+    s->nblocks = nblocks; s->nib = nib; s->njb = njb; s->nkb = nkb;
+    s->blk_ids = (int*)malloc(sizeof(int) * (size_t)nib * njb * nkb);
+    memcpy(s->blk_ids, blk_ids, sizeof(int) * (size_t)nib * njb * nkb);
+    s->nfs = n_flow_states;
+    s->inflow = (FS*)calloc((size_t)(n_flow_states > 0 ? n_flow_states : 1), sizeof(FS));
+    for (int i = 0; i < n_flow_states; ++i) { /* config.cu:398-406 */
+        FS* f = &s->inflow[i];
+        f->p = flow_states[6 * i + 0]; f->T = flow_states[6 * i + 1]; f->YB = flow_states[6 * i + 2];
+        update_from_pT(f);
+        f->vel.x = flow_states[6 * i + 3]; f->vel.y = flow_states[6 * i + 4]; f->vel.z = flow_states[6 * i + 5];
+    }
+    s->x_order = p->x_order; s->flux_calc = p->flux_calc; s->cfl_count = p->cfl_count;
+    s->max_step = p->max_step; s->max_time = p->max_time; s->dt_init = p->dt_init;
+    s->n_cfl = p->n_cfl;
+    s->cfl_t = (double*)malloc(sizeof(double) * (size_t)p->n_cfl);
+    s->cfl_v = (double*)malloc(sizeof(double) * (size_t)p->n_cfl);
+    memcpy(s->cfl_t, p->cfl_times, sizeof(double) * (size_t)p->n_cfl);
+    memcpy(s->cfl_v, p->cfl_values, sizeof(double) * (size_t)p->n_cfl);
+    s->b = (Blk*)calloc((size_t)nblocks, sizeof(Blk));
+    size_t voff = 0, ooff = 0;
+    for (int ib = 0; ib < nblocks; ++ib) {
+        Blk* b = &s->b[ib];
+        b->bi = blk_ijk[3 * ib]; b->bj = blk_ijk[3 * ib + 1]; b->bk = blk_ijk[3 * ib + 2];
+        b->nic = blk_dims[3 * ib]; b->njc = blk_dims[3 * ib + 1]; b->nkc = blk_dims[3 * ib + 2];
+        b->active = active[ib];
+        for (int f = 0; f < 6; ++f) { b->bc[f] = bc_codes[6 * ib + f]; b->bcfs[f] = bc_fs[6 * ib + f]; }
+        if (configure(b)) { orc_destroy(s); return NULL; }
+        size_t nv = (size_t)(b->nic + 1) * (b->njc + 1) * (b->nkc + 1);
+        for (size_t v = 0; v < nv; ++v) {
+            b->vtx[v].x = vertices[voff + 3 * v]; b->vtx[v].y = vertices[voff + 3 * v + 1];
+            b->vtx[v].z = vertices[voff + 3 * v + 2];
+        }
+        voff += 3 * nv;
+        /* readFlow, block.cu:882-907: 13 IO variables, cell order i fastest. */
+        int n = b->ncells;
+        for (int c = 0; c < n; ++c) {
+            Cell* cl = &b->cells[c];
+            const double* io = iovars + ooff;
+            cl->pos.x = io[0 * n + c]; cl->pos.y = io[1 * n + c]; cl->pos.z = io[2 * n + c];
+            cl->vol = io[3 * n + c];
+            cl->fs.p = io[4 * n + c]; cl->fs.T = io[5 * n + c]; cl->fs.rho = io[6 * n + c];
+            cl->fs.e = io[7 * n + c]; cl->fs.YB = io[8 * n + c]; cl->fs.a = io[9 * n + c];
+            cl->fs.vel.x = io[10 * n + c]; cl->fs.vel.y = io[11 * n + c]; cl->fs.vel.z = io[12 * n + c];
+        }
+        ooff += 13 * (size_t)n;
+        compute_geometry(b);
+    }
+    /* Exchange partners must exist (a missing block would be index -1 in the reference). */
+    for (int ib = 0; ib < nblocks; ++ib)
+        for (int f = 0; f < 6; ++f)
+            if (s->b[ib].active && s->b[ib].bc[f] == ORC_EXCHANGE) {
+                Blk* b = &s->b[ib];
+                int oi = b->bi + (f == IP) - (f == IM), oj = b->bj + (f == JP) - (f == JM);
+                int ok = b->bk + (f == KP) - (f == KM);
+                oi = (oi + nib) % nib; oj = (oj + njb) % njb; ok = (ok + nkb) % nkb;
+                int id = s->blk_ids[(oi * njb + oj) * nkb + ok];
+                if (id < 0 || id >= nblocks) { set_err("exchange boundary without a neighbour block"); orc_destroy(s); return NULL; }
+                Blk* o = &s->b[id];
+                int ok_dims = (f < 2) ? (o->njc == b->njc && o->nkc == b->nkc)
+                              : (f < 4) ? (o->nic == b->nic && o->nkc == b->nkc)
+                                        : (o->nic == b->nic && o->njc == b->njc);
+                if (!ok_dims) { set_err("exchange neighbour has mismatched face dimensions"); orc_destroy(s); return NULL; }
+            }
+    s->t = p->t0; s->dt = p->dt_init; s->step = 0;
+    /* simulate.cu:196-203: encode level 0 from the file's flow states. */
+    for (int ib = 0; ib < nblocks; ++ib) {
+        Blk* b = &s->b[ib];
+        if (!b->active) continue;
+        for (int c = 0; c < b->ncells; ++c) encode(&b->cells[c].fs, &b->Q[(size_t)c * 6]);
+    }
+    return s;
+}
+
+void orc_destroy(orc_sim* s)
+{
+    if (!s) return;
+    if (s->b)
+        for (int ib = 0; ib < s->nblocks; ++ib) {
+            free(s->b[ib].cells); free(s->b[ib].faces); free(s->b[ib].vtx);
+            free(s->b[ib].Q); free(s->b[ib].dQ);
+        }
+    free(s->b); free(s->blk_ids); free(s->inflow); free(s->cfl_t); free(s->cfl_v);
+    free(s);
+}
+
+/* simulate.cu:205-335 (without plotting; the caller writes snapshots). */
+int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
+{
+    int done = 0;
+    while (done < nsteps && s->step < s->max_step && s->t < s->max_time) {
+        if (s->step > 0 && (s->step % s->cfl_count) == 0) { /* :208-221 */
+            double smallest = DBL_MAX;
+            double cfl = schedule_value(s, s->t);
+            for (int ib = 0; ib < s->nblocks; ++ib) {
+                Blk* b = &s->b[ib];
+                if (!b->active) continue;
+                double bmin = DBL_MAX; /* block.cu:978-990 */
+                for (int c = 0; c < b->ncells; ++c) bmin = fmin(bmin, local_dt(b, &b->cells[c], cfl));
+                smallest = fmin(smallest, bmin);
+            }
+            s->dt = fmin(1.5 * s->dt, smallest);
+        }
+        for (int stage = 1; stage <= 3; ++stage) { /* :223-287 */
+            apply_all_bcs(s);
+            int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : bad)
+            for (int ib = 0; ib < s->nblocks; ++ib) {
+                Blk* b = &s->b[ib];
+                if (!b->active) continue;
+                block_fluxes(s, b);
+                bad += block_update(b, stage, s->dt);
+            }
+            if (bad > 0) {
+                if (bad_count) *bad_count = bad;
+                return 100 * stage + (bad > 99 ? 99 : bad);
+            }
+        }
+        for (int ib = 0; ib < s->nblocks; ++ib) { /* :288-294, copy level 1 -> 0 */
+            Blk* b = &s->b[ib];
+            if (!b->active) continue;
+            memcpy(b->Q, b->Q + (size_t)b->ncells * 6, sizeof(double) * (size_t)b->ncells * 6);
+        }
+        if (dt_log) dt_log[done] = s->dt;
+        s->t += s->dt;
+        s->step += 1;
+        done += 1;
+    }
+    if (bad_count) *bad_count = 0;
+    return 0;
+}
+
+int orc_step_count(const orc_sim* s) { return s->step; }
+double orc_time(const orc_sim* s) { return s->t; }
+double orc_dt(const orc_sim* s) { return s->dt; }
+
+/* cell.cu:133-153 */
+void orc_get_iovars(const orc_sim* s, int blk, double* out)
+{
+    const Blk* b = &s->b[blk];
+    int n = b->ncells;
+    for (int c = 0; c < n; ++c) {
+        const Cell* cl = &b->cells[c];
+        out[0 * n + c] = cl->pos.x; out[1 * n + c] = cl->pos.y; out[2 * n + c] = cl->pos.z;
+        out[3 * n + c] = cl->vol;
+        out[4 * n + c] = cl->fs.p; out[5 * n + c] = cl->fs.T; out[6 * n + c] = cl->fs.rho;
+        out[7 * n + c] = cl->fs.e; out[8 * n + c] = cl->fs.YB; out[9 * n + c] = cl->fs.a;
+        out[10 * n + c] = cl->fs.vel.x; out[11 * n + c] = cl->fs.vel.y; out[12 * n + c] = cl->fs.vel.z;
+    }
+}
+
+void orc_get_conserved(const orc_sim* s, int blk, double* out)
+{
+    const Blk* b = &s->b[blk];
+    int n = b->ncells;
+    if (!b->active) { memset(out, 0, sizeof(double) * 6 * (size_t)n); return; }
+    for (int c = 0; c < n; ++c)
+        for (int j = 0; j < 6; ++j) out[(size_t)j * n + c] = b->Q[(size_t)c * 6 + j];
+}
+
+void orc_get_dudt(const orc_sim* s, int blk, int level, double* out)
+{
+    const Blk* b = &s->b[blk];
+    int n = b->ncells;
+    if (!b->active) { memset(out, 0, sizeof(double) * 6 * (size_t)n); return; }
+    for (int c = 0; c < n; ++c)
+        for (int j = 0; j < 6; ++j) out[(size_t)j * n + c] = b->dQ[((size_t)level * n + c) * 6 + j];
+}
+
+static FS fs_from9(const double* w)
+{
+    FS f;
+    f.rho = w[0]; f.e = w[1]; f.p = w[2]; f.T = w[3]; f.YB = w[4]; f.a = w[5];
+    f.vel.x = w[6]; f.vel.y = w[7]; f.vel.z = w[8];
+    return f;
+}
+
+void orc_face_flux(const double* states, const double* frame, int x_order, int flux_calc, double* F)
+{
+    Face f;
+    memset(&f, 0, sizeof f);
+    f.n.x = frame[0]; f.n.y = frame[1]; f.n.z = frame[2];
+    f.t1.x = frame[3]; f.t1.y = frame[4]; f.t1.z = frame[5];
+    f.t2.x = frame[6]; f.t2.y = frame[7]; f.t2.z = frame[8];
+    FS s0 = fs_from9(states), s1 = fs_from9(states + 9), s2 = fs_from9(states + 18),
+       s3 = fs_from9(states + 27);
+    face_flux(&f, &s0, &s1, &s2, &s3, flux_calc, x_order);
+    for (int i = 0; i < 6; ++i) F[i] = f.F[i];
+}
+
+void orc_hex_props(const double* p, double* out)
+{
+    V3 v[8];
+    for (int i = 0; i < 8; ++i) { v[i].x = p[3 * i]; v[i].y = p[3 * i + 1]; v[i].z = p[3 * i + 2]; }
+    V3 c; double vol, iL, jL, kL;
+    hex_props(v, &c, &vol, &iL, &jL, &kL);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = vol; out[4] = iL; out[5] = jL; out[6] = kL;
+}
+
+void orc_quad_props(const double* p, double* out)
+{
+    V3 v[4];
+    for (int i = 0; i < 4; ++i) { v[i].x = p[3 * i]; v[i].y = p[3 * i + 1]; v[i].z = p[3 * i + 2]; }
+    V3 c, n, t1, t2; double area;
+    quad_properties(v[0], v[1], v[2], v[3], &c, &n, &t1, &t2, &area);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = n.x; out[4] = n.y; out[5] = n.z;
+    out[6] = t1.x; out[7] = t1.y; out[8] = t1.z; out[9] = t2.x; out[10] = t2.y; out[11] = t2.z;
+    out[12] = area;
+}
+
+int orc_decode(const double* U, double* W)
+{
+    FS f;
+    memset(&f, 0, sizeof f);
+    int bad = decode(&f, U);
+    W[0] = f.rho; W[1] = f.e; W[2] = f.p; W[3] = f.T; W[4] = f.YB; W[5] = f.a;
+    W[6] = f.vel.x; W[7] = f.vel.y; W[8] = f.vel.z;
+    return bad;
+}
+
+void orc_encode(const double* W, double* U)
+{
+    FS f = fs_from9(W);
+    encode(&f, U);
+}

@@ -1,0 +1,160 @@
+"""HIP step (libchkn, through the C-ABI) against the CPU oracle, on the GPU.
+
+Tolerances (FP64 throughout): after N steps the IO fields agree to 1e-9 relative, and a
+single step from an identical state agrees to 1e-12 relative.  The kernels are built
+without FMA contraction and keep the reference's operation order, so the results are
+expected to be bit-identical; the tests report the observed maximum difference.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from chkn import cases, prep
+from chkn.job import JobData
+from chkn.lib import BadCellError, Sim
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1.0e-9
+STEP_TOL = 1.0e-12
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def compare(job, steps, launch=None, tol=FIELD_TOL):
+    sim = Sim(job)
+    if launch:
+        sim.set_launch(**launch)
+    rep = sim.step(steps)
+    orc = ob.Oracle(job)
+    rc, _, dts = orc.march(steps)
+    assert rc == 0
+    assert rep.step == orc.step
+    assert rep.t == orc.t, (rep.t, orc.t)
+    worst = 0.0
+    for b in range(len(job.blocks)):
+        worst = max(worst, rel(sim.iovars(b), orc.iovars(b)))
+        if job.blocks[b].active:
+            worst = max(worst, rel(sim.conserved(b), orc.conserved(b)))
+    sim.close()
+    print("max rel diff %g" % worst)
+    assert worst <= tol
+    return worst
+
+
+def _cfl_ramp():
+    cases.ramp(20, 6, 8, max_step=40)
+    prep.config.dt_init = 2.0e-5
+    return JobData.from_prep()
+
+
+def _first_order():
+    cases.mixed_bc_array(max_step=15)
+    prep.config.x_order = 1
+    return JobData.from_prep()
+
+
+JOBS = {
+    "sod": (lambda: cases.sod_box(16, 8, 8, max_step=30), 30),
+    "mixed_bc": (lambda: cases.mixed_bc_array(max_step=25), 25),
+    "ramp_cfl": (_cfl_ramp, 40),
+    "x_order_1": (_first_order, 15),
+    "two_cell_block": (lambda: cases.ramp(2, 3, 2, max_step=20), 20),
+    "uneven_array": (lambda: cases.ramp_array(seed=3, lo=4, hi=9, max_step=22), 22),
+}
+
+
+@pytest.mark.parametrize("name", sorted(JOBS))
+def test_fields_match_oracle(name):
+    builder, steps = JOBS[name]
+    compare(builder(), steps)
+
+
+@pytest.mark.parametrize("launch", [
+    dict(per_block_launch=True),
+    dict(use_graph=True),
+    dict(persistent=True),
+    dict(tile_i=64, tile_j=4, tile_k=3),
+    dict(tile_i=16, tile_j=16, tile_k=1),
+    dict(tile_i=64, tile_j=8, use_graph=True, per_block_launch=True),
+])
+def test_launch_options_do_not_change_results(launch):
+    compare(cases.ramp_array(seed=5, lo=4, hi=12, max_step=25), 25, launch=launch)
+
+
+def test_single_step_residual():
+    # From an identical developed state (the oracle's state after 12 steps), one step of
+    # the HIP path agrees with one oracle step to 1e-12 relative in U.
+    job = cases.ramp(24, 8, 10, max_step=30)
+    orc = ob.Oracle(job)
+    orc.march(12)
+    job.blocks[0].iovars = orc.iovars(0)
+    sim = Sim(job)
+    sim.step(1)
+    orc2 = ob.Oracle(job)
+    orc2.march(1)
+    assert rel(sim.conserved(0), orc2.conserved(0)) <= STEP_TOL
+    sim.close()
+
+
+def test_bad_cell_reports_stage_and_count():
+    job = cases.sod_box(8, 4, 4, max_step=5)
+    io = job.blocks[0].iovars.copy()
+    io[6, 5] = -1.0  # negative density in one cell
+    io[6, 17] = -2.0
+    job.blocks[0].iovars = io
+    sim = Sim(job)
+    with pytest.raises(BadCellError) as ei:
+        sim.step(5)
+    orc = ob.Oracle(job)
+    rc, bad, _ = orc.march(5)
+    assert rc // 100 == ei.value.stage
+    assert bad == ei.value.count
+    assert "Stage %d bad cell count: %d" % (ei.value.stage, bad) in str(ei.value)
+    sim.close()
+
+
+def test_restart_continues_after_chkn_step_calls():
+    # Several chkn_step calls (splitting CFL segments) equal one long call.
+    job = _cfl_ramp()
+    a = Sim(job)
+    for n in (3, 7, 1, 9, 20):
+        a.step(n)
+    b = Sim(job)
+    b.step(40)
+    assert np.array_equal(a.conserved(0), b.conserved(0))
+    a.close()
+    b.close()
+
+
+def test_large_sod_conserves_mass_and_energy():
+    # All-wall box: total mass and energy are conserved to round-off (size-independent check).
+    job = cases.sod_box(128, 64, 64, max_step=40)
+    sim = Sim(job)
+    vol = sim.iovars(0)[3]
+    U0 = sim.conserved(0)
+    sim.step(40)
+    U1 = sim.conserved(0)
+    for q in (0, 4):
+        m0, m1 = float(np.dot(U0[q], vol)), float(np.dot(U1[q], vol))
+        assert abs(m1 - m0) <= 1e-11 * abs(m0)
+    sim.close()
+
+
+def test_sod_direction_symmetry():
+    # The same tube along i, j and k gives the same profile (sod.py / sod_j.py / sod_k.py).
+    profs = []
+    for axis in range(3):
+        dims = [8, 8, 8]
+        dims[axis] = 48
+        job = cases.sod_box(*dims, max_step=30, axis=axis)
+        sim = Sim(job)
+        sim.step(30)
+        p = sim.iovars(0)[4].reshape(dims[2], dims[1], dims[0])
+        line = [p[4, 4, :], p[4, :, 4], p[:, 4, 4]][axis]
+        profs.append(line)
+        sim.close()
+    assert np.allclose(profs[0], profs[1], rtol=1e-12)
+    assert np.allclose(profs[0], profs[2], rtol=1e-12)

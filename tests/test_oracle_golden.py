@@ -1,0 +1,97 @@
+"""The C oracle against golden vectors written by the reference's own sources.
+
+The vectors come from tests/golden/make_golden.py, which drives the reference's
+face/cell/block/bcs code (compiled from the reference tree into oracle/_ref).  Every
+comparison is bitwise: the oracle restates the reference's arithmetic operation by
+operation.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from chkn import cases, prep
+from chkn.job import JobData
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.int64)
+
+
+def test_flux_vectors_bitwise():
+    d = np.load(os.path.join(GOLD, "flux_vectors.npz"))
+    rec, F = d["rec"], d["F"]
+    assert len(rec) >= 1000
+    got = np.array([ob.face_flux(r[:36], r[36:45], int(r[45]), int(r[46])) for r in rec])
+    assert np.array_equal(bits(got), bits(F)), np.max(np.abs(got - F))
+
+
+def test_geometry_vectors_bitwise():
+    d = np.load(os.path.join(GOLD, "geom_vectors.npz"))
+    for p, out in zip(d["p"], d["out"]):
+        hexo = ob.hex_props(p)
+        assert np.array_equal(bits(hexo), bits(out[:7]))
+        quad = ob.quad_props(p[:12])
+        assert np.array_equal(bits(quad), bits(out[7:20]))
+
+
+def _cfl_ramp():
+    cases.ramp(20, 6, 8, max_step=40)
+    prep.config.dt_init = 2.0e-5
+    return JobData.from_prep()
+
+
+def _first_order():
+    cases.mixed_bc_array(max_step=15)
+    prep.config.x_order = 1
+    return JobData.from_prep()
+
+
+def _two_cell():
+    cases.ramp(2, 3, 2, max_step=20)
+    return JobData.from_prep()
+
+
+STEP_JOBS = {
+    "sod": (lambda: cases.sod_box(16, 8, 8, max_step=30), 30),
+    "mixed": (lambda: cases.mixed_bc_array(max_step=25), 25),
+    "ramp_cfl": (_cfl_ramp, 40),
+    "xorder1": (_first_order, 15),
+    "tiny_n2": (_two_cell, 20),
+}
+
+
+def job_digest(job):
+    import hashlib
+    f = job.flat()
+    h = hashlib.sha256()
+    for key in sorted(f):
+        h.update(key.encode())
+        h.update(np.ascontiguousarray(f[key]).tobytes())
+    for key in ("x_order", "cfl_count", "max_step", "max_time", "dt_init"):
+        h.update(repr(getattr(job, key)).encode())
+    return h.hexdigest()
+
+
+def load_step_fixture(name):
+    return np.load(os.path.join(GOLD, "step_%s.npz" % name))
+
+
+@pytest.mark.parametrize("name", sorted(STEP_JOBS))
+def test_march_matches_reference_bitwise(name, tmp_path):
+    builder, steps = STEP_JOBS[name]
+    job = builder()
+    # The golden run read the job back from binary files; do the same.
+    prep.write_initial_files(True, str(tmp_path))
+    jd = JobData.from_job_dir(str(tmp_path))
+    gold = load_step_fixture(name)
+    assert str(gold["digest"]) == job_digest(jd), "job generator changed; regenerate golden"
+    o = ob.Oracle(jd)
+    rc, bad, dts = o.march(steps)
+    assert rc == 0
+    assert np.array_equal(bits(dts[:steps]), bits(gold["dt"]))
+    for b in range(len(jd.blocks)):
+        assert np.array_equal(bits(o.iovars(b)), bits(gold["io%d" % b])), "block %d" % b
