@@ -121,16 +121,102 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
     return v;
 }
 
+// LDS layout of a tile: the k-plane of primitive states with a 2-cell halo in i and j
+// (rectangle of (TI+4) x (TJ+4) positions; only the cross-shaped part is filled), then the
+// area-weighted fluxes of the plane's i-faces and j-faces.
+template <int TI, int TJ>
+struct TileLds {
+    static constexpr int PW = TI + 4;
+    static constexpr int NP = PW * (TJ + 4);
+    static constexpr int NFI = (TI + 1) * TJ;
+    static constexpr int NFJ = TI * (TJ + 1);
+    static constexpr int NLOAD = (TI + 4) * TJ + 4 * TI;  // filled plane positions
+    static constexpr int DOUBLES = 6 * NP + 6 * NFI + 6 * NFJ;
+};
+
+// Load one plane position (ii, jj relative to the tile origin) for plane k into LDS:
+// an interior cell, a ghost cell across an i- or j-boundary, or nothing if no face of the
+// tile can reference it.
+template <int TI, int TJ>
+__device__ __forceinline__ void fill_plane_position(const StageArgs& A, const BlockDev* B,
+                                                    const double* __restrict__ Win, int i0,
+                                                    int j0, int k, int p, double* Wp)
+{
+    using L = TileLds<TI, TJ>;
+    int ii, jj;
+    if (p < (TI + 4) * TJ) {
+        jj = p / (TI + 4);
+        ii = p % (TI + 4) - 2;
+    } else {
+        const int q = p - (TI + 4) * TJ;
+        const int r = q / TI;
+        jj = (r < 2) ? r - 2 : TJ + r - 2;
+        ii = q % TI;
+    }
+    const int gi = i0 + ii, gj = j0 + jj;
+    const int nic = B->nic, njc = B->njc;
+    const bool in_i = (gi >= 0) && (gi < nic), in_j = (gj >= 0) && (gj < njc);
+    Prim w;
+    if (in_i && in_j) {
+        w = ld_prim(Win, (int)B->n, gi + nic * (gj + njc * k));
+    } else if (in_j && gi >= -2 && gi <= nic + 1) {
+        w = ghost_prim(A, B, 0, gi, gj, k);
+    } else if (in_i && gj >= -2 && gj <= njc + 1) {
+        w = ghost_prim(A, B, 1, gj, gi, k);
+    } else {
+        return;
+    }
+    const int o = (jj + 2) * L::PW + (ii + 2);
+    Wp[o] = w.rho;
+    Wp[L::NP + o] = w.e;
+    Wp[2 * L::NP + o] = w.YB;
+    Wp[3 * L::NP + o] = w.vx;
+    Wp[4 * L::NP + o] = w.vy;
+    Wp[5 * L::NP + o] = w.vz;
+}
+
+__device__ __forceinline__ Prim lds_prim(const double* Wp, int np, int o)
+{
+    Prim p;
+    p.rho = Wp[o];
+    p.e = Wp[np + o];
+    p.YB = Wp[2 * np + o];
+    p.vx = Wp[3 * np + o];
+    p.vy = Wp[4 * np + o];
+    p.vz = Wp[5 * np + o];
+    return p;
+}
+
+// Area-weighted flux of the k-face (i, j, kf) between cells kf-1 and kf.
+template <int XORDER>
+__device__ __forceinline__ void k_face_af(const StageArgs& A, const BlockDev* B,
+                                          const double* __restrict__ Win, int i, int j, int kf,
+                                          double AF[6])
+{
+    const int nic = B->nic, njc = B->njc, nkc = B->nkc;
+    const int base = i + nic * j, stride = nic * njc;
+    const Prim s0 = stencil_prim(A, B, Win, nkc, base, stride, 2, kf - 2, i, j);
+    const Prim s1 = stencil_prim(A, B, Win, nkc, base, stride, 2, kf - 1, i, j);
+    const Prim s2 = stencil_prim(A, B, Win, nkc, base, stride, 2, kf, i, j);
+    const Prim s3 = stencil_prim(A, B, Win, nkc, base, stride, 2, kf + 1, i, j);
+    const int fi = i + nic * (j + njc * kf);
+    const Frame fr = ld_frame(B->gk, (int)B->nfk, fi);
+    const double area = B->gk[fi];
+    double F[6];
+    face_flux<XORDER>(s0, s1, s2, s3, fr, F);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) AF[q] = area * F[q];
+}
+
 template <int STAGE, int TI, int TJ, int XORDER, bool DTRED>
 __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile, double* lds)
 {
+    using L = TileLds<TI, TJ>;
     constexpr int NT = TI * TJ;
-    constexpr int NFI = (TI + 1) * TJ;
-    constexpr int NFJ = TI * (TJ + 1);
-    constexpr int NFK = TI * TJ;
-    double* AFi = lds;                      // [6][TJ][TI+1]
-    double* AFj = AFi + 6 * NFI;            // [6][TJ+1][TI]
-    double* AFk = AFj + 6 * NFJ;            // [2][6][TJ][TI]
+    constexpr int NFI = L::NFI, NFJ = L::NFJ, NP = L::NP, PW = L::PW;
+    double* Wp = lds;               // [6][TJ+4][TI+4]
+    double* AFi = Wp + 6 * NP;      // [6][TJ][TI+1]
+    double* AFj = AFi + 6 * NFI;    // [6][TJ+1][TI]
 
     const BlockDev* B = A.blocks + tile.blk;
     const int nic = B->nic, njc = B->njc;
@@ -145,62 +231,60 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
     int bad = 0;
     unsigned long long sigmin = ~0ull;
 
-    for (int kk = kbeg - 1; kk < kend; ++kk) {
-        const bool plane = (kk >= kbeg);
-        const int nfaces = (plane ? NFI + NFJ : 0) + NFK;
-        const int fofs = plane ? 0 : NFI + NFJ;
-        double* AFkp = AFk + ((kk + 1) & 1) * 6 * NFK;
-        for (int f = t; f < nfaces; f += NT) {
-            const int fg = f + fofs;
-            int dir, m0, a, c, base, stride, n_line, nf, fi;
+    double AFkm[6] = {0, 0, 0, 0, 0, 0};
+    if (own) k_face_af<XORDER>(A, B, Win, i, j, kbeg, AFkm);
+
+    for (int k = kbeg; k < kend; ++k) {
+        // Stage the plane (with halo and ghost cells) in LDS; the thread's own k-face one
+        // plane up is evaluated from global memory meanwhile.
+        for (int p = t; p < L::NLOAD; p += NT)
+            fill_plane_position<TI, TJ>(A, B, Win, tile.i0, tile.j0, k, p, Wp);
+        double AFkp[6] = {0, 0, 0, 0, 0, 0};
+        if (own) k_face_af<XORDER>(A, B, Win, i, j, k + 1, AFkp);
+        __syncthreads();
+        // i- and j-faces of the plane, one flattened list, stencils read from LDS.
+        for (int f = t; f < NFI + NFJ; f += NT) {
+            int base, stride, fa, fb, fi, nf;
             const double* g;
             double* dst;
-            int lds_stride;
+            int lstride;
             bool valid;
-            if (fg < NFI) {  // i-face (gi, gj, kk) between cells gi-1 and gi
-                const int fa = fg % (TI + 1), fb = fg / (TI + 1);
+            if (f < NFI) {
+                fa = f % (TI + 1);
+                fb = f / (TI + 1);
                 const int gi = tile.i0 + fa, gj = tile.j0 + fb;
                 valid = (gi <= nic) && (gj < njc);
-                dir = 0; m0 = gi; a = gj; c = kk;
-                base = nic * (gj + njc * kk); stride = 1; n_line = nic;
-                g = B->gi; nf = (int)B->nfi; fi = gi + (nic + 1) * (gj + njc * kk);
-                dst = AFi + fb * (TI + 1) + fa; lds_stride = NFI;
-            } else if (fg < NFI + NFJ) {  // j-face (gi, gj, kk)
-                const int h = fg - NFI;
-                const int fa = h % TI, fb = h / TI;
+                base = (fb + 2) * PW + fa;  // cells fa-2 .. fa+1 of row fb
+                stride = 1;
+                g = B->gi; nf = (int)B->nfi; fi = gi + (nic + 1) * (gj + njc * k);
+                dst = AFi + fb * (TI + 1) + fa; lstride = NFI;
+            } else {
+                const int h = f - NFI;
+                fa = h % TI;
+                fb = h / TI;
                 const int gi = tile.i0 + fa, gj = tile.j0 + fb;
                 valid = (gi < nic) && (gj <= njc);
-                dir = 1; m0 = gj; a = gi; c = kk;
-                base = gi + nic * njc * kk; stride = nic; n_line = njc;
-                g = B->gj; nf = (int)B->nfj; fi = gi + nic * (gj + (njc + 1) * kk);
-                dst = AFj + fb * TI + fa; lds_stride = NFJ;
-            } else {  // k-face (gi, gj, kk+1) of the thread-column layout
-                const int h = fg - NFI - NFJ;
-                const int fa = h % TI, fb = h / TI;
-                const int gi = tile.i0 + fa, gj = tile.j0 + fb;
-                valid = (gi < nic) && (gj < njc);
-                dir = 2; m0 = kk + 1; a = gi; c = gj;
-                base = gi + nic * gj; stride = nic * njc; n_line = B->nkc;
-                g = B->gk; nf = (int)B->nfk; fi = gi + nic * (gj + njc * (kk + 1));
-                dst = AFkp + fb * TI + fa; lds_stride = NFK;
+                base = fb * PW + fa + 2;    // rows fb-2 .. fb+1 of column fa
+                stride = PW;
+                g = B->gj; nf = (int)B->nfj; fi = gi + nic * (gj + (njc + 1) * k);
+                dst = AFj + fb * TI + fa; lstride = NFJ;
             }
             if (valid) {
-                Prim s0 = stencil_prim(A, B, Win, n_line, base, stride, dir, m0 - 2, a, c);
-                Prim s1 = stencil_prim(A, B, Win, n_line, base, stride, dir, m0 - 1, a, c);
-                Prim s2 = stencil_prim(A, B, Win, n_line, base, stride, dir, m0, a, c);
-                Prim s3 = stencil_prim(A, B, Win, n_line, base, stride, dir, m0 + 1, a, c);
                 const Frame fr = ld_frame(g, nf, fi);
                 const double area = g[fi];
+                const Prim s0 = lds_prim(Wp, NP, base);
+                const Prim s1 = lds_prim(Wp, NP, base + stride);
+                const Prim s2 = lds_prim(Wp, NP, base + 2 * stride);
+                const Prim s3 = lds_prim(Wp, NP, base + 3 * stride);
                 double F[6];
                 face_flux<XORDER>(s0, s1, s2, s3, fr, F);
 #pragma unroll
-                for (int q = 0; q < 6; ++q) dst[q * lds_stride] = area * F[q];
+                for (int q = 0; q < 6; ++q) dst[q * lstride] = area * F[q];
             }
         }
         __syncthreads();
-        if (plane && own) {
-            const double* AFkm = AFk + (kk & 1) * 6 * NFK;
-            const int cidx = i + nic * (j + njc * kk);
+        if (own) {
+            const int cidx = i + nic * (j + njc * k);
             const double vinv = B->volinv[cidx];
             double d[6];
 #pragma unroll
@@ -208,30 +292,30 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
                 double s = AFi[q * NFI + tj * (TI + 1) + ti] - AFi[q * NFI + tj * (TI + 1) + ti + 1];
                 s = s + AFj[q * NFJ + tj * TI + ti];
                 s = s - AFj[q * NFJ + (tj + 1) * TI + ti];
-                s = s + AFkm[q * NFK + tj * TI + ti];
-                s = s - AFkp[q * NFK + tj * TI + ti];
+                s = s + AFkm[q];
+                s = s - AFkp[q];
                 d[q] = vinv * s;
             }
             double* __restrict__ U = B->U;
-            double* __restrict__ L = B->L;
+            double* __restrict__ Lr = B->L;
             double Un[6];
             if (STAGE == 1) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
                     Un[q] = U[q * n + cidx] + dt * d[q];
-                    L[q * n + cidx] = d[q];
+                    Lr[q * n + cidx] = d[q];
                 }
             } else if (STAGE == 2) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
-                    const double l0 = L[q * n + cidx];
+                    const double l0 = Lr[q * n + cidx];
                     Un[q] = U[q * n + cidx] + 0.25 * dt * (l0 + d[q]);
-                    L[q * n + cidx] = kOneSixth * l0 + kOneSixth * d[q];
+                    Lr[q * n + cidx] = kOneSixth * l0 + kOneSixth * d[q];
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
-                    Un[q] = U[q * n + cidx] + dt * (L[q * n + cidx] + kFourSixth * d[q]);
+                    Un[q] = U[q * n + cidx] + dt * (Lr[q * n + cidx] + kFourSixth * d[q]);
                     U[q * n + cidx] = Un[q];
                 }
             }
@@ -248,14 +332,16 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
             if (DTRED && !flag) {
                 const double T = eos_T(w.e, w.YB);
                 const double sa = eos_a(T);
-                const Frame fi_ = ld_frame(B->gi, (int)B->nfi, i + (nic + 1) * (j + njc * kk));
-                const Frame fj_ = ld_frame(B->gj, (int)B->nfj, i + nic * (j + (njc + 1) * kk));
+                const Frame fi_ = ld_frame(B->gi, (int)B->nfi, i + (nic + 1) * (j + njc * k));
+                const Frame fj_ = ld_frame(B->gj, (int)B->nfj, i + nic * (j + (njc + 1) * k));
                 const Frame fk_ = ld_frame(B->gk, (int)B->nfk, cidx);
                 const double sig = local_signal(w, sa, B->len[cidx], B->len[n + cidx],
                                                 B->len[2 * n + cidx], fi_, fj_, fk_);
                 const unsigned long long bits = (unsigned long long)__double_as_longlong(sig);
                 sigmin = bits < sigmin ? bits : sigmin;
             }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) AFkm[q] = AFkp[q];
         }
         __syncthreads();
     }
@@ -274,8 +360,7 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
 template <int STAGE, int TI, int TJ, int XORDER, bool DTRED>
 __global__ __launch_bounds__(TI* TJ) void stage_kernel(StageArgs A, int persistent)
 {
-    constexpr int LDS_DOUBLES = 6 * ((TI + 1) * TJ + TI * (TJ + 1) + 2 * TI * TJ);
-    __shared__ double lds[LDS_DOUBLES];
+    __shared__ double lds[TileLds<TI, TJ>::DOUBLES];
     // XCD-aware tile mapping: work-group b runs on XCD b % 8, and each XCD takes a
     // contiguous range of the (spatially ordered) tile table, so halo cells shared by
     // neighbouring tiles are re-read from the same L2.
