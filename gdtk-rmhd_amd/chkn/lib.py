@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libchkn.so")
+# CHKN_LIB selects another build of the library (for A/B measurements of kernel variants).
+LIB_PATH = os.environ.get("CHKN_LIB") or os.path.join(_HERE, "libchkn.so")
 
 _I = ctypes.POINTER(ctypes.c_int)
 _D = ctypes.POINTER(ctypes.c_double)

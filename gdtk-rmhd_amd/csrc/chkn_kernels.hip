@@ -73,9 +73,11 @@ __device__ __forceinline__ void boundary_face(const BlockDev* B, int dir, int si
     }
 }
 
-// Ghost-cell state for a stencil position outside the block (m < 0 or m >= n).
+// Ghost-cell state for a stencil position outside the block (m < 0 or m >= n).  Inlined:
+// an out-of-line call would make every caller preserve its live registers across the
+// call ABI, which costs about 30 VGPRs and one wave per SIMD of occupancy.
 // Depth d = -m-1 on the minus side and m-n on the plus side (block.cu:272-316).
-__device__ __noinline__ Prim ghost_prim(const StageArgs& A, const BlockDev* B, int dir, int m,
+__device__ __forceinline__ Prim ghost_prim(const StageArgs& A, const BlockDev* B, int dir, int m,
                                         int a, int c)
 {
     int n = dir == 0 ? B->nic : (dir == 1 ? B->njc : B->nkc);
@@ -440,7 +442,7 @@ hipError_t launch_signal_min(const BlockDev* block, int ncells, StepCtl* ctl, hi
 bool tile_shape_supported(int ti, int tj)
 {
     return (ti == 32 && tj == 8) || (ti == 64 && tj == 4) || (ti == 16 && tj == 16) ||
-           (ti == 64 && tj == 8);
+           (ti == 64 && tj == 8) || (ti == 16 && tj == 4) || (ti == 8 && tj == 8);
 }
 
 hipError_t launch_stage(int stage, int x_order, int ti, int tj, bool dt_reduce, bool persistent,
@@ -451,6 +453,8 @@ hipError_t launch_stage(int stage, int x_order, int ti, int tj, bool dt_reduce, 
     if (ti == 64 && tj == 4) return launch_order<64, 4>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
     if (ti == 16 && tj == 16) return launch_order<16, 16>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
     if (ti == 64 && tj == 8) return launch_order<64, 8>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
+    if (ti == 16 && tj == 4) return launch_order<16, 4>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
+    if (ti == 8 && tj == 8) return launch_order<8, 8>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
     return hipErrorInvalidValue;
 }
 
