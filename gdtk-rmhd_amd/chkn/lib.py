@@ -44,6 +44,12 @@ class ChknLaunchOpts(ctypes.Structure):
 
 EXPORTS = {
     "chkn_create": (ctypes.c_int, [ctypes.POINTER(ChknDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "chkn_create_batch": (ctypes.c_int, [ctypes.POINTER(ChknDesc), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "chkn_num_instances": (ctypes.c_int, [ctypes.c_void_p]),
+    "chkn_instance_blocks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I, _I]),
+    "chkn_get_instance_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                                ctypes.POINTER(ChknReport)]),
     "chkn_load_job": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_void_p)]),
     "chkn_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ChknReport)]),
@@ -101,39 +107,52 @@ def _ptr(a, typ):
     return a.ctypes.data_as(typ)
 
 
-class Sim:
-    """One simulation handle (chkn_sim)."""
+def _desc(job, keep):
+    """Fill a ChknDesc from a JobData; the flat arrays are appended to keep."""
+    f = job.flat()
+    keep.append(f)
+    d = ChknDesc()
+    d.nblocks = len(job.blocks)
+    d.nib, d.njb, d.nkb = job.nib, job.njb, job.nkb
+    for key in ("blk_ids", "blk_ijk", "blk_dims", "active", "bc_codes", "bc_fs"):
+        setattr(d, key, _ptr(f[key], _I))
+    d.n_flow_states = len(job.flow_states)
+    for key in ("flow_states", "vertices", "iovars", "cfl_times", "cfl_values",
+                "dt_plot_times", "dt_plot_values"):
+        setattr(d, key, _ptr(f[key], _D))
+    d.x_order, d.flux_calc, d.t_order = job.x_order, job.flux_calc, job.t_order
+    d.cfl_count, d.max_step, d.print_count = job.cfl_count, job.max_step, job.print_count
+    d.max_time, d.dt_init = job.max_time, job.dt_init
+    d.n_cfl = len(job.cfl_times)
+    d.n_dt_plot = len(job.dt_plot_times)
+    d.t0, d.start_tindx = job.t0, job.start_tindx
+    d.reacting, d.viscous = int(job.reacting), int(job.viscous)
+    return d
 
-    def __init__(self, job=None, job_dir=None, tindx=0, binary=True):
+
+class Sim:
+    """One simulation handle (chkn_sim): a single job (``job`` or ``job_dir``) or an ensemble
+    of independent jobs stepped together (``jobs``)."""
+
+    def __init__(self, job=None, job_dir=None, tindx=0, binary=True, jobs=None):
         lib = load()
         h = ctypes.c_void_p()
+        self._keep = []
         if job_dir is not None:
             rc = lib.chkn_load_job(os.fsencode(job_dir), tindx, 1 if binary else 0, ctypes.byref(h))
+        elif jobs is not None:
+            arr = (ChknDesc * len(jobs))(*[_desc(j, self._keep) for j in jobs])
+            self._keep.append(arr)
+            rc = lib.chkn_create_batch(arr, len(jobs), ctypes.byref(h))
         else:
-            f = job.flat()
-            self._keep = f
-            d = ChknDesc()
-            d.nblocks = len(job.blocks)
-            d.nib, d.njb, d.nkb = job.nib, job.njb, job.nkb
-            for key in ("blk_ids", "blk_ijk", "blk_dims", "active", "bc_codes", "bc_fs"):
-                setattr(d, key, _ptr(f[key], _I))
-            d.n_flow_states = len(job.flow_states)
-            for key in ("flow_states", "vertices", "iovars", "cfl_times", "cfl_values",
-                        "dt_plot_times", "dt_plot_values"):
-                setattr(d, key, _ptr(f[key], _D))
-            d.x_order, d.flux_calc, d.t_order = job.x_order, job.flux_calc, job.t_order
-            d.cfl_count, d.max_step, d.print_count = job.cfl_count, job.max_step, job.print_count
-            d.max_time, d.dt_init = job.max_time, job.dt_init
-            d.n_cfl = len(job.cfl_times)
-            d.n_dt_plot = len(job.dt_plot_times)
-            d.t0, d.start_tindx = job.t0, job.start_tindx
-            d.reacting, d.viscous = int(job.reacting), int(job.viscous)
+            d = _desc(job, self._keep)
             rc = lib.chkn_create(ctypes.byref(d), ctypes.byref(h))
         if rc != 0:
             raise ChknError(rc, lib.chkn_last_error(None).decode())
         self._lib = lib
         self.h = h
         self.nblocks = lib.chkn_num_blocks(h)
+        self.ninstances = lib.chkn_num_instances(h)
 
     def _check(self, rc, rep=None):
         if rc > 0 and rep is not None and rep.bad_stage:
@@ -189,6 +208,17 @@ class Sim:
     def report(self):
         rep = ChknReport()
         self._check(self._lib.chkn_get_report(self.h, ctypes.byref(rep)))
+        return rep
+
+    def instance_blocks(self, inst):
+        """(first, count) global block numbers of instance inst."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.chkn_instance_blocks(self.h, inst, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def instance_report(self, inst):
+        rep = ChknReport()
+        self._check(self._lib.chkn_get_instance_report(self.h, inst, ctypes.byref(rep)))
         return rep
 
     def last_timing(self):

@@ -24,6 +24,7 @@ constexpr int kFrameN = 10; // area + n + t1 + t2
 
 struct BlockDev {
     int nic, njc, nkc, active;
+    int inst;        // index of the instance (independent job) the block belongs to
     int bc[6];       // face.cu:26-32 codes
     int bc_arg[6];   // inflow flow-state index, or neighbour block id for exchange
     long long n;     // cells
@@ -44,12 +45,14 @@ struct Tile {
     int nk;
 };
 
-// Per-launch control data in device memory (host writes it before a step segment).
+// Per-step control data in device memory, written by the host before a segment of steps.
+// One slot per instance for dt, run flag and signal-time minimum; one bad-cell counter per
+// stage.
 struct StepCtl {
-    double dt;
-    unsigned long long sigmin_bits; // min over cells of the cfl-free signal time
-    int bad[3];
-    int pad;
+    const double* dt;                 // [ninst] time step of each instance
+    const int* run;                   // [ninst] 1 if the instance takes this step
+    unsigned long long* sigmin_bits;  // [ninst] min over cells of the cfl-free signal time
+    int* bad;                         // [3] bad cells per stage
 };
 
 struct StageArgs {
@@ -58,7 +61,7 @@ struct StageArgs {
     int ntiles;
     int tile_base;         // offset into the tile table (per-block launches)
     const double* inflow;  // 6 doubles per flow state: rho, e, YB, vx, vy, vz
-    StepCtl* ctl;
+    StepCtl ctl;
     int win, wout;         // W ring indices read / written by this stage
 };
 
@@ -69,7 +72,8 @@ hipError_t launch_stage(int stage, int x_order, int ti, int tj, bool dt_reduce, 
 bool tile_shape_supported(int ti, int tj);
 
 // Minimum over the block's cells of the cfl-free signal time of its W[0] state, folded into
-// ctl->sigmin_bits (used when no stage-3 launch has reduced the current state).
-hipError_t launch_signal_min(const BlockDev* block, int ncells, StepCtl* ctl, hipStream_t s);
+// *sigmin_bits (used when no stage-3 launch has reduced the current state).
+hipError_t launch_signal_min(const BlockDev* block, int ncells, unsigned long long* sigmin_bits,
+                             hipStream_t s);
 
 }  // namespace chkn

@@ -229,7 +229,7 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
     const int i = tile.i0 + ti, j = tile.j0 + tj;
     const bool own = (i < nic) && (j < njc);
     const int kbeg = tile.k0, kend = tile.k0 + tile.nk;
-    const double dt = A.ctl->dt;
+    const double dt = A.ctl.dt[B->inst];
     int bad = 0;
     unsigned long long sigmin = ~0ull;
 
@@ -351,11 +351,11 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
     if (__ballot(bad != 0)) {
         int cnt = bad;
         for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&A.ctl->bad[STAGE - 1], cnt);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&A.ctl.bad[STAGE - 1], cnt);
     }
     if (DTRED) {
         const unsigned long long wm = wave_min_u64(sigmin);
-        if ((threadIdx.x & 63) == 0 && wm != ~0ull) atomicMin(&A.ctl->sigmin_bits, wm);
+        if ((threadIdx.x & 63) == 0 && wm != ~0ull) atomicMin(&A.ctl.sigmin_bits[B->inst], wm);
     }
 }
 
@@ -376,7 +376,8 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(StageArgs A, int persiste
     if (t1 > nt) t1 = nt;
     for (int tt = t0 + local; tt < t1; tt += per_xcd) {
         const Tile tile = A.tiles[A.tile_base + tt];
-        process_tile<STAGE, TI, TJ, XORDER, DTRED>(A, tile, lds);
+        // Instances that have reached their end time skip the step (uniform per tile).
+        if (A.ctl.run[A.blocks[tile.blk].inst]) process_tile<STAGE, TI, TJ, XORDER, DTRED>(A, tile, lds);
         if (!persistent) break;
     }
 }
@@ -410,7 +411,8 @@ hipError_t launch_order(int stage, int x_order, bool dtred, bool persistent, int
     return launch_shape<TI, TJ, 1>(stage, dtred, persistent, grid_limit, a, s);
 }
 
-__global__ __launch_bounds__(256) void signal_min_kernel(const BlockDev* B, StepCtl* ctl)
+__global__ __launch_bounds__(256) void signal_min_kernel(const BlockDev* B,
+                                                        unsigned long long* sigmin_bits)
 {
     const int n = (int)B->n;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,15 +429,17 @@ __global__ __launch_bounds__(256) void signal_min_kernel(const BlockDev* B, Step
         v = (unsigned long long)__double_as_longlong(sig);
     }
     v = wave_min_u64(v);
-    if ((threadIdx.x & 63) == 0 && v != ~0ull) atomicMin(&ctl->sigmin_bits, v);
+    if ((threadIdx.x & 63) == 0 && v != ~0ull) atomicMin(sigmin_bits, v);
 }
 
 }  // namespace
 
-hipError_t launch_signal_min(const BlockDev* block, int ncells, StepCtl* ctl, hipStream_t s)
+hipError_t launch_signal_min(const BlockDev* block, int ncells, unsigned long long* sigmin_bits,
+                             hipStream_t s)
 {
     if (ncells <= 0) return hipSuccess;
-    hipLaunchKernelGGL(signal_min_kernel, dim3((ncells + 255) / 256), dim3(256), 0, s, block, ctl);
+    hipLaunchKernelGGL(signal_min_kernel, dim3((ncells + 255) / 256), dim3(256), 0, s, block,
+                       sigmin_bits);
     return hipGetLastError();
 }
 

@@ -8,6 +8,11 @@
 // without synchronising, and reads the control block back once at the end of the segment
 // (bad-cell counts per stage and, when the segment ends on a CFL step, the minimum signal
 // time reduced inside the last stage-3 launch).
+//
+// A handle may hold several instances: independent jobs, each with its own block array,
+// inflow states, time, time step, CFL and plot schedules.  All instances advance in
+// lockstep, one launch per stage for every block of every instance; an instance that has
+// reached its max_step or max_time skips the remaining steps (its run flag is 0).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,6 +54,7 @@ struct HipError : std::runtime_error {
     } while (0)
 
 struct HostBlock {
+    int inst = 0;
     int bi, bj, bk, nic, njc, nkc;
     bool active;
     int bc[6], bc_fs[6];
@@ -68,6 +74,21 @@ struct HostBlock {
     int tile_first = 0, tile_count = 0;
 };
 
+// Time control of one instance (SimState of simulate.cu:32-39 plus its schedules).
+struct Instance {
+    int first_block = 0, nblocks = 0;
+    int nib = 0, njb = 0, nkb = 0;
+    std::vector<int> blk_ids;  // global block ids
+    int cfl_count = 10, max_step = 100, print_count = 20;
+    double max_time = 1e-3, dt_init = 1e-6;
+    std::vector<double> cfl_t, cfl_v, plot_t, plot_v;
+    std::string job;
+    double t = 0.0, dt = 0.0, t_plot = 0.0, sigmin = 0.0;
+    int step = 0, next_plot_indx = 1, steps_since_last_plot = 0;
+    bool have_sig = false;
+    bool running() const { return step < max_step && t < max_time; }
+};
+
 struct GraphKey {
     int K;
     bool dtred;
@@ -78,29 +99,23 @@ struct GraphKey {
 
 struct chkn_sim {
     // configuration
-    int nib = 0, njb = 0, nkb = 0;
-    std::vector<int> blk_ids;
     std::vector<HostBlock> blocks;
-    std::vector<double> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz
-    int x_order = 2, flux_calc = 0, cfl_count = 10, max_step = 100, print_count = 20;
-    double max_time = 1e-3, dt_init = 1e-6;
-    std::vector<double> cfl_t, cfl_v, plot_t, plot_v;
-    std::string job;  // empty when created from arrays
-    // state (SimState of simulate.cu:32-39)
-    double t = 0.0, dt = 0.0, t_plot = 0.0;
-    int step = 0, next_plot_indx = 1, steps_since_last_plot = 0;
+    std::vector<Instance> inst;
+    std::vector<double> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz (all instances)
+    int x_order = 2, flux_calc = 0;
+    int gstep = 0;       // lockstep step counter shared by the instances
     bool fresh = true;   // no step taken since the state was set from IO variables
     bool failed = false;
-    bool have_sig = false;
-    double sigmin = 0.0;
     // device
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     BlockDev* d_blocks = nullptr;
     Tile* d_tiles = nullptr;
     double* d_inflow = nullptr;
-    StepCtl* d_ctl = nullptr;
-    StepCtl* h_ctl = nullptr;
+    // Control area, kMaxSeg step slots: dt[S][N], sigmin[S][N], run[S][N], bad[S][3].
+    char* d_ctl = nullptr;
+    char* h_ctl = nullptr;
+    size_t ctl_bytes = 0;
     int ntiles = 0;
     // launch options
     chkn_launch_opts opts{32, 8, 0, 0, 0, 0};
@@ -288,7 +303,22 @@ void build_tiles(chkn_sim* s)
     s->graphs.clear();
 }
 
-void setup(chkn_sim* s, const chkn_desc* d)
+// Control-area views for segment slot q.
+double* ctl_dt(char* base, int q, int N) { return reinterpret_cast<double*>(base) + (size_t)q * N; }
+unsigned long long* ctl_sig(char* base, int q, int N)
+{
+    return reinterpret_cast<unsigned long long*>(base + sizeof(double) * kMaxSeg * N) + (size_t)q * N;
+}
+int* ctl_run(char* base, int q, int N)
+{
+    return reinterpret_cast<int*>(base + 2 * sizeof(double) * kMaxSeg * N) + (size_t)q * N;
+}
+int* ctl_bad(char* base, int q, int N)
+{
+    return reinterpret_cast<int*>(base + 2 * sizeof(double) * kMaxSeg * N + sizeof(int) * kMaxSeg * N) + 3 * q;
+}
+
+void check_desc(const chkn_desc* d)
 {
     if (d->nblocks <= 0) throw std::invalid_argument("no blocks");
     if (d->reacting) throw std::invalid_argument("the reacting-gas model is not built");
@@ -297,71 +327,105 @@ void setup(chkn_sim* s, const chkn_desc* d)
     if (d->x_order != 1 && d->x_order != 2) throw std::invalid_argument("x_order must be 1 or 2");
     if (d->n_cfl <= 0) throw std::invalid_argument("need at least one cfl value");
     if (d->cfl_count <= 0) throw std::invalid_argument("cfl_count must be positive");
-    s->nib = d->nib; s->njb = d->njb; s->nkb = d->nkb;
-    s->blk_ids.assign(d->blk_ids, d->blk_ids + (size_t)d->nib * d->njb * d->nkb);
-    s->x_order = d->x_order; s->flux_calc = d->flux_calc; s->cfl_count = d->cfl_count;
-    s->max_step = d->max_step; s->print_count = d->print_count;
-    s->max_time = d->max_time; s->dt_init = d->dt_init;
-    s->cfl_t.assign(d->cfl_times, d->cfl_times + d->n_cfl);
-    s->cfl_v.assign(d->cfl_values, d->cfl_values + d->n_cfl);
-    if (d->n_dt_plot > 0) {
-        s->plot_t.assign(d->dt_plot_times, d->dt_plot_times + d->n_dt_plot);
-        s->plot_v.assign(d->dt_plot_values, d->dt_plot_values + d->n_dt_plot);
-    } else {
-        s->plot_t.assign(1, 0.0);
-        s->plot_v.assign(1, d->max_time);
+}
+
+// Set up instances from job descriptions: validation, inflow states, geometry pre-pass,
+// device buffers (initialize_simulation, simulate.cu:55-155, for each instance).
+void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
+{
+    if (ninst <= 0) throw std::invalid_argument("no instances");
+    for (int q = 0; q < ninst; ++q) check_desc(&descs[q]);
+    s->x_order = descs[0].x_order;
+    s->flux_calc = descs[0].flux_calc;
+    for (int q = 1; q < ninst; ++q) {
+        if (descs[q].x_order != s->x_order) throw std::invalid_argument("instances must share x_order");
+        if (descs[q].cfl_count != descs[0].cfl_count) throw std::invalid_argument("instances must share cfl_count");
     }
-    // Inflow states: update_from_pT of the configured p, T, YB (config.cu:398-406).
-    for (int q = 0; q < d->n_flow_states; ++q) {
-        const double* f = d->flow_states + 6 * q;
-        double p = f[0], T = f[1], YB = f[2];
-        double e = T * kCv - YB * kQ;   // gas.cu:123
-        double rho = p / (T * kR);      // gas.cu:124
-        double w[6] = {rho, e, YB, f[3], f[4], f[5]};
-        s->inflow_w.insert(s->inflow_w.end(), w, w + 6);
+    s->inst.resize(ninst);
+    int nblocks_total = 0;
+    for (int q = 0; q < ninst; ++q) nblocks_total += descs[q].nblocks;
+    s->blocks.resize(nblocks_total);
+    int fs_base = 0, blk_base = 0;
+    for (int qi = 0; qi < ninst; ++qi) {
+        const chkn_desc* d = &descs[qi];
+        Instance& I = s->inst[qi];
+        I.first_block = blk_base;
+        I.nblocks = d->nblocks;
+        I.nib = d->nib; I.njb = d->njb; I.nkb = d->nkb;
+        I.blk_ids.assign(d->blk_ids, d->blk_ids + (size_t)d->nib * d->njb * d->nkb);
+        I.cfl_count = d->cfl_count; I.max_step = d->max_step; I.print_count = d->print_count;
+        I.max_time = d->max_time; I.dt_init = d->dt_init;
+        I.cfl_t.assign(d->cfl_times, d->cfl_times + d->n_cfl);
+        I.cfl_v.assign(d->cfl_values, d->cfl_values + d->n_cfl);
+        if (d->n_dt_plot > 0) {
+            I.plot_t.assign(d->dt_plot_times, d->dt_plot_times + d->n_dt_plot);
+            I.plot_v.assign(d->dt_plot_values, d->dt_plot_values + d->n_dt_plot);
+        } else {
+            I.plot_t.assign(1, 0.0);
+            I.plot_v.assign(1, d->max_time);
+        }
+        // Inflow states: update_from_pT of the configured p, T, YB (config.cu:398-406).
+        for (int q = 0; q < d->n_flow_states; ++q) {
+            const double* f = d->flow_states + 6 * q;
+            double p = f[0], T = f[1], YB = f[2];
+            double e = T * kCv - YB * kQ;   // gas.cu:123
+            double rho = p / (T * kR);      // gas.cu:124
+            double w[6] = {rho, e, YB, f[3], f[4], f[5]};
+            s->inflow_w.insert(s->inflow_w.end(), w, w + 6);
+        }
+        size_t ooff = 0;
+        for (int lb = 0; lb < d->nblocks; ++lb) {
+            HostBlock& b = s->blocks[blk_base + lb];
+            b.inst = qi;
+            b.bi = d->blk_ijk[3 * lb]; b.bj = d->blk_ijk[3 * lb + 1]; b.bk = d->blk_ijk[3 * lb + 2];
+            b.nic = d->blk_dims[3 * lb]; b.njc = d->blk_dims[3 * lb + 1]; b.nkc = d->blk_dims[3 * lb + 2];
+            b.active = d->active[lb] != 0;
+            for (int f = 0; f < 6; ++f) {
+                b.bc[f] = d->bc_codes[6 * lb + f];
+                b.bc_fs[f] = d->bc_fs[6 * lb + f] + fs_base;
+            }
+            // block.cu:282-284 and its j/k twins.
+            if (b.nic < 2) throw std::invalid_argument("Too few cells in the i-index direction.");
+            if (b.njc < 2) throw std::invalid_argument("Too few cells in the j-index direction.");
+            if (b.nkc < 2) throw std::invalid_argument("Too few cells in the k-index direction.");
+            long long n = (long long)b.nic * b.njc * b.nkc;
+            if (n > (1LL << 30)) throw std::invalid_argument("block too large for 32-bit cell indexing");
+            b.n = (int)n;
+            if (b.bi < 0 || b.bi >= d->nib || b.bj < 0 || b.bj >= d->njb || b.bk < 0 || b.bk >= d->nkb)
+                throw std::invalid_argument("block position outside the block array");
+            if (I.blk_ids[((size_t)b.bi * d->njb + b.bj) * d->nkb + b.bk] != lb)
+                throw std::invalid_argument("blk_id=" + std::to_string(lb) + " is inconsistent");
+            for (int f = 0; f < 6; ++f) {
+                if (b.active && b.bc[f] == CHKN_BC_INFLOW_FUNCTION)
+                    throw std::invalid_argument("inflow_function boundaries are not built");
+                if (b.active && b.bc[f] == CHKN_BC_INFLOW &&
+                    (d->bc_fs[6 * lb + f] < 0 || d->bc_fs[6 * lb + f] >= d->n_flow_states))
+                    throw std::invalid_argument("inflow flow_state_index out of range");
+            }
+            b.io0.assign(d->iovars + ooff, d->iovars + ooff + (size_t)13 * b.n);
+            ooff += (size_t)13 * b.n;
+        }
+        for (auto& id : I.blk_ids) {
+            if (id >= d->nblocks) throw std::invalid_argument("blk_ids entry beyond nblocks");
+            if (id >= 0) id += blk_base;
+        }
+        fs_base += d->n_flow_states;
+        blk_base += d->nblocks;
     }
     if (s->inflow_w.empty()) s->inflow_w.assign(6, 0.0);
-    // Blocks.
-    size_t voff = 0, ooff = 0;
-    s->blocks.resize(d->nblocks);
-    for (int ib = 0; ib < d->nblocks; ++ib) {
-        HostBlock& b = s->blocks[ib];
-        b.bi = d->blk_ijk[3 * ib]; b.bj = d->blk_ijk[3 * ib + 1]; b.bk = d->blk_ijk[3 * ib + 2];
-        b.nic = d->blk_dims[3 * ib]; b.njc = d->blk_dims[3 * ib + 1]; b.nkc = d->blk_dims[3 * ib + 2];
-        b.active = d->active[ib] != 0;
-        for (int f = 0; f < 6; ++f) { b.bc[f] = d->bc_codes[6 * ib + f]; b.bc_fs[f] = d->bc_fs[6 * ib + f]; }
-        // block.cu:282-284 and its j/k twins.
-        if (b.nic < 2) throw std::invalid_argument("Too few cells in the i-index direction.");
-        if (b.njc < 2) throw std::invalid_argument("Too few cells in the j-index direction.");
-        if (b.nkc < 2) throw std::invalid_argument("Too few cells in the k-index direction.");
-        long long n = (long long)b.nic * b.njc * b.nkc;
-        if (n > (1LL << 30)) throw std::invalid_argument("block too large for 32-bit cell indexing");
-        b.n = (int)n;
-        if (b.bi < 0 || b.bi >= s->nib || b.bj < 0 || b.bj >= s->njb || b.bk < 0 || b.bk >= s->nkb)
-            throw std::invalid_argument("block position outside the block array");
-        if (s->blk_ids[((size_t)b.bi * s->njb + b.bj) * s->nkb + b.bk] != ib)
-            throw std::invalid_argument("blk_id=" + std::to_string(ib) + " is inconsistent");
-        for (int f = 0; f < 6; ++f) {
-            if (b.active && b.bc[f] == CHKN_BC_INFLOW_FUNCTION)
-                throw std::invalid_argument("inflow_function boundaries are not built");
-            if (b.active && b.bc[f] == CHKN_BC_INFLOW && (b.bc_fs[f] < 0 || b.bc_fs[f] >= d->n_flow_states))
-                throw std::invalid_argument("inflow flow_state_index out of range");
-        }
-        b.io0.assign(d->iovars + ooff, d->iovars + ooff + (size_t)13 * b.n);
-        ooff += (size_t)13 * b.n;
-        voff += 0;
-    }
     // Exchange partners (bcs.cu:145-252, periodic wrap at the array edges).
-    std::vector<std::array<int, 6>> partner(d->nblocks);
-    for (int ib = 0; ib < d->nblocks; ++ib) {
+    std::vector<std::array<int, 6>> partner(s->blocks.size());
+    for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
         HostBlock& b = s->blocks[ib];
+        const Instance& I = s->inst[b.inst];
         for (int f = 0; f < 6; ++f) {
             partner[ib][f] = -1;
             if (!b.active || b.bc[f] != CHKN_BC_EXCHANGE) continue;
             int oi = b.bi + (f == 1) - (f == 0), oj = b.bj + (f == 3) - (f == 2), ok = b.bk + (f == 5) - (f == 4);
-            oi = (oi + s->nib) % s->nib; oj = (oj + s->njb) % s->njb; ok = (ok + s->nkb) % s->nkb;
-            int id = s->blk_ids[((size_t)oi * s->njb + oj) * s->nkb + ok];
-            if (id < 0 || id >= d->nblocks) throw std::invalid_argument("exchange boundary without a neighbour block");
+            oi = (oi + I.nib) % I.nib; oj = (oj + I.njb) % I.njb; ok = (ok + I.nkb) % I.nkb;
+            int id = I.blk_ids[((size_t)oi * I.njb + oj) * I.nkb + ok];
+            if (id < 0 || id >= I.first_block + I.nblocks)
+                throw std::invalid_argument("exchange boundary without a neighbour block");
             HostBlock& o = s->blocks[id];
             bool okd = (f < 2) ? (o.njc == b.njc && o.nkc == b.nkc)
                        : (f < 4) ? (o.nic == b.nic && o.nkc == b.nkc) : (o.nic == b.nic && o.njc == b.njc);
@@ -373,76 +437,102 @@ void setup(chkn_sim* s, const chkn_desc* d)
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&s->ev0));
     HIPCHK(hipEventCreate(&s->ev1));
-    std::vector<BlockDev> bdev(d->nblocks);
-    voff = 0;
-    for (int ib = 0; ib < d->nblocks; ++ib) {
-        HostBlock& b = s->blocks[ib];
-        std::vector<double> gi, gj, gk, volinv, len;
-        block_geometry(b, d->vertices + voff, gi, gj, gk, volinv, len);
-        voff += (size_t)3 * (b.nic + 1) * (b.njc + 1) * (b.nkc + 1);
-        size_t n = (size_t)b.n, sz6 = sizeof(double) * 6 * n;
-        HIPCHK(hipMalloc(&b.W, sz6 * (b.active ? 3 : 1)));
-        BlockDev& bd = bdev[ib];
-        std::memset(&bd, 0, sizeof bd);
-        bd.nic = b.nic; bd.njc = b.njc; bd.nkc = b.nkc; bd.active = b.active ? 1 : 0;
-        bd.n = b.n;
-        bd.nfi = (long long)(b.nic + 1) * b.njc * b.nkc;
-        bd.nfj = (long long)b.nic * (b.njc + 1) * b.nkc;
-        bd.nfk = (long long)b.nic * b.njc * (b.nkc + 1);
-        for (int r = 0; r < 3; ++r) bd.W[r] = b.W + (b.active ? (size_t)r * 6 * n : 0);
-        if (b.active) {
-            HIPCHK(hipMalloc(&b.U, sz6));
-            HIPCHK(hipMalloc(&b.L, sz6));
-            HIPCHK(hipMemset(b.L, 0, sz6));
-            HIPCHK(hipMalloc(&b.gi, sizeof(double) * gi.size()));
-            HIPCHK(hipMalloc(&b.gj, sizeof(double) * gj.size()));
-            HIPCHK(hipMalloc(&b.gk, sizeof(double) * gk.size()));
-            HIPCHK(hipMalloc(&b.volinv, sizeof(double) * n));
-            HIPCHK(hipMalloc(&b.len, sizeof(double) * 3 * n));
-            HIPCHK(hipMemcpy(b.gi, gi.data(), sizeof(double) * gi.size(), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(b.gj, gj.data(), sizeof(double) * gj.size(), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(b.gk, gk.data(), sizeof(double) * gk.size(), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(b.volinv, volinv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(b.len, len.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-            bd.U = b.U; bd.L = b.L; bd.gi = b.gi; bd.gj = b.gj; bd.gk = b.gk;
-            bd.volinv = b.volinv; bd.len = b.len;
+    std::vector<BlockDev> bdev(s->blocks.size());
+    for (int qi = 0; qi < ninst; ++qi) {
+        const chkn_desc* d = &descs[qi];
+        size_t voff = 0;
+        for (int lb = 0; lb < d->nblocks; ++lb) {
+            int ib = s->inst[qi].first_block + lb;
+            HostBlock& b = s->blocks[ib];
+            std::vector<double> gi, gj, gk, volinv, len;
+            block_geometry(b, d->vertices + voff, gi, gj, gk, volinv, len);
+            voff += (size_t)3 * (b.nic + 1) * (b.njc + 1) * (b.nkc + 1);
+            size_t n = (size_t)b.n, sz6 = sizeof(double) * 6 * n;
+            HIPCHK(hipMalloc(&b.W, sz6 * (b.active ? 3 : 1)));
+            BlockDev& bd = bdev[ib];
+            std::memset(&bd, 0, sizeof bd);
+            bd.nic = b.nic; bd.njc = b.njc; bd.nkc = b.nkc; bd.active = b.active ? 1 : 0;
+            bd.inst = b.inst;
+            bd.n = b.n;
+            bd.nfi = (long long)(b.nic + 1) * b.njc * b.nkc;
+            bd.nfj = (long long)b.nic * (b.njc + 1) * b.nkc;
+            bd.nfk = (long long)b.nic * b.njc * (b.nkc + 1);
+            for (int r = 0; r < 3; ++r) bd.W[r] = b.W + (b.active ? (size_t)r * 6 * n : 0);
+            if (b.active) {
+                HIPCHK(hipMalloc(&b.U, sz6));
+                HIPCHK(hipMalloc(&b.L, sz6));
+                HIPCHK(hipMemset(b.L, 0, sz6));
+                HIPCHK(hipMalloc(&b.gi, sizeof(double) * gi.size()));
+                HIPCHK(hipMalloc(&b.gj, sizeof(double) * gj.size()));
+                HIPCHK(hipMalloc(&b.gk, sizeof(double) * gk.size()));
+                HIPCHK(hipMalloc(&b.volinv, sizeof(double) * n));
+                HIPCHK(hipMalloc(&b.len, sizeof(double) * 3 * n));
+                HIPCHK(hipMemcpy(b.gi, gi.data(), sizeof(double) * gi.size(), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(b.gj, gj.data(), sizeof(double) * gj.size(), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(b.gk, gk.data(), sizeof(double) * gk.size(), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(b.volinv, volinv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(b.len, len.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+                bd.U = b.U; bd.L = b.L; bd.gi = b.gi; bd.gj = b.gj; bd.gk = b.gk;
+                bd.volinv = b.volinv; bd.len = b.len;
+            }
+            for (int f = 0; f < 6; ++f) {
+                bd.bc[f] = b.bc[f];
+                bd.bc_arg[f] = (b.bc[f] == CHKN_BC_EXCHANGE) ? partner[ib][f] : b.bc_fs[f];
+            }
+            upload_state(s, b, b.io0.data());
         }
-        for (int f = 0; f < 6; ++f) {
-            bd.bc[f] = b.bc[f];
-            bd.bc_arg[f] = (b.bc[f] == CHKN_BC_EXCHANGE) ? partner[ib][f] : b.bc_fs[f];
-        }
-        upload_state(s, b, b.io0.data());
     }
     HIPCHK(hipMalloc(&s->d_blocks, sizeof(BlockDev) * bdev.size()));
     HIPCHK(hipMemcpy(s->d_blocks, bdev.data(), sizeof(BlockDev) * bdev.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&s->d_inflow, sizeof(double) * s->inflow_w.size()));
     HIPCHK(hipMemcpy(s->d_inflow, s->inflow_w.data(), sizeof(double) * s->inflow_w.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&s->d_ctl, sizeof(StepCtl) * kMaxSeg));
-    HIPCHK(hipHostMalloc(&s->h_ctl, sizeof(StepCtl) * kMaxSeg, hipHostMallocDefault));
+    const int N = ninst;
+    s->ctl_bytes = (2 * sizeof(double) + sizeof(int)) * (size_t)kMaxSeg * N + sizeof(int) * 3 * kMaxSeg;
+    HIPCHK(hipMalloc(&s->d_ctl, s->ctl_bytes));
+    HIPCHK(hipHostMalloc(&s->h_ctl, s->ctl_bytes, hipHostMallocDefault));
+    std::memset(s->h_ctl, 0, s->ctl_bytes);
     build_tiles(s);
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     s->grid_limit = ncu * 2;
     // SimState at start (simulate.cu:132-150, :190-191).
-    s->t = d->t0;
-    s->dt = d->dt_init;
-    s->step = 0;
-    s->next_plot_indx = d->start_tindx + 1;
-    s->t_plot = s->t + schedule_get(s->plot_t, s->plot_v, s->t);
-    s->steps_since_last_plot = 0;
+    for (int qi = 0; qi < ninst; ++qi) {
+        Instance& I = s->inst[qi];
+        I.t = descs[qi].t0;
+        I.dt = descs[qi].dt_init;
+        I.step = 0;
+        I.next_plot_indx = descs[qi].start_tindx + 1;
+        I.t_plot = I.t + schedule_get(I.plot_t, I.plot_v, I.t);
+        I.steps_since_last_plot = 0;
+        I.have_sig = false;
+    }
+    s->gstep = 0;
     s->fresh = true;
+}
+
+StageArgs stage_args(chkn_sim* s, int q)
+{
+    const int N = (int)s->inst.size();
+    StageArgs a;
+    a.blocks = s->d_blocks;
+    a.tiles = s->d_tiles;
+    a.inflow = s->d_inflow;
+    a.ctl.dt = ctl_dt(s->d_ctl, q, N);
+    a.ctl.sigmin_bits = ctl_sig(s->d_ctl, q, N);
+    a.ctl.run = ctl_run(s->d_ctl, q, N);
+    a.ctl.bad = ctl_bad(s->d_ctl, q, N);
+    a.tile_base = 0;
+    a.ntiles = 0;
+    a.win = a.wout = 0;
+    return a;
 }
 
 // Launch the 3*K stage kernels of a segment (optionally per block).
 void launch_segment(chkn_sim* s, int K, bool dtred_last)
 {
-    StageArgs a;
-    a.blocks = s->d_blocks;
-    a.tiles = s->d_tiles;
-    a.inflow = s->d_inflow;
     for (int q = 0; q < K; ++q) {
-        a.ctl = s->d_ctl + q;
+        StageArgs a = stage_args(s, q);
         for (int stage = 1; stage <= 3; ++stage) {
             a.win = stage - 1;
             a.wout = stage % 3;
@@ -454,35 +544,44 @@ void launch_segment(chkn_sim* s, int K, bool dtred_last)
                     a.ntiles = b.tile_count;
                     HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, dtred,
                                         s->opts.persistent != 0, s->grid_limit, a, s->stream));
-                    s->last_launches++;
                 }
             } else {
                 a.tile_base = 0;
                 a.ntiles = s->ntiles;
                 HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, dtred,
                                     s->opts.persistent != 0, s->grid_limit, a, s->stream));
-                s->last_launches++;
             }
         }
     }
 }
 
+long long launches_per_segment(chkn_sim* s, int K)
+{
+    long long per_stage = 1;
+    if (s->opts.per_block_launch) {
+        per_stage = 0;
+        for (auto& b : s->blocks) if (b.active && b.tile_count) per_stage++;
+    }
+    return 3LL * K * per_stage;
+}
+
+// run[q][inst] and dt[q][inst] must already be in h_ctl.
 void run_segment(chkn_sim* s, int K, bool dtred_last)
 {
+    const int N = (int)s->inst.size();
     for (int q = 0; q < K; ++q) {
-        s->h_ctl[q].dt = s->dt;
-        s->h_ctl[q].sigmin_bits = ~0ull;
-        s->h_ctl[q].bad[0] = s->h_ctl[q].bad[1] = s->h_ctl[q].bad[2] = 0;
-        s->h_ctl[q].pad = 0;
+        unsigned long long* sg = ctl_sig(s->h_ctl, q, N);
+        for (int qi = 0; qi < N; ++qi) sg[qi] = ~0ull;
+        int* bd = ctl_bad(s->h_ctl, q, N);
+        bd[0] = bd[1] = bd[2] = 0;
     }
-    HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(StepCtl) * K, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, s->ctl_bytes, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipEventRecord(s->ev0, s->stream));
     if (s->opts.use_graph) {
         GraphKey key{K, dtred_last};
         auto it = s->graphs.find(key);
         if (it == s->graphs.end()) {
             hipGraph_t g;
-            long long saved = s->last_launches;
             HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
             launch_segment(s, K, dtred_last);
             HIPCHK(hipStreamEndCapture(s->stream, &g));
@@ -490,118 +589,169 @@ void run_segment(chkn_sim* s, int K, bool dtred_last)
             HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
             (void)hipGraphDestroy(g);
             it = s->graphs.emplace(key, ge).first;
-            s->last_launches = saved;
         }
         HIPCHK(hipGraphLaunch(it->second, s->stream));
-        long long per_stage = 1;
-        if (s->opts.per_block_launch) {
-            per_stage = 0;
-            for (auto& b : s->blocks) if (b.active && b.tile_count) per_stage++;
-        }
-        s->last_launches += 3LL * K * per_stage;
     } else {
         launch_segment(s, K, dtred_last);
     }
+    s->last_launches += launches_per_segment(s, K);
     HIPCHK(hipEventRecord(s->ev1, s->stream));
-    HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(StepCtl) * K, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, s->ctl_bytes, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     s->last_ms += ms;
 }
 
-// Minimum signal time of the current state when no stage-3 launch reduced it (first CFL
-// step after the state was replaced).
+// Minimum signal time of each instance's current state when no stage-3 launch reduced it
+// (first CFL step after the state was replaced).
 void signal_reduction(chkn_sim* s)
 {
-    s->h_ctl[0].sigmin_bits = ~0ull;
-    HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(StepCtl), hipMemcpyHostToDevice, s->stream));
+    const int N = (int)s->inst.size();
+    unsigned long long* sg = ctl_sig(s->h_ctl, 0, N);
+    for (int qi = 0; qi < N; ++qi) sg[qi] = ~0ull;
+    HIPCHK(hipMemcpyAsync(ctl_sig(s->d_ctl, 0, N), sg, sizeof(unsigned long long) * N,
+                          hipMemcpyHostToDevice, s->stream));
     for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
         HostBlock& b = s->blocks[ib];
         if (!b.active) continue;
-        HIPCHK(launch_signal_min(s->d_blocks + ib, b.n, s->d_ctl, s->stream));
+        HIPCHK(launch_signal_min(s->d_blocks + ib, b.n, ctl_sig(s->d_ctl, 0, N) + b.inst, s->stream));
     }
-    HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(sg, ctl_sig(s->d_ctl, 0, N), sizeof(unsigned long long) * N,
+                          hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    unsigned long long bits = s->h_ctl[0].sigmin_bits;
-    double m;
-    std::memcpy(&m, &bits, sizeof m);
-    s->sigmin = (bits == ~0ull) ? DBL_MAX : m;
-    s->have_sig = true;
+    for (int qi = 0; qi < N; ++qi) {
+        double m;
+        std::memcpy(&m, &sg[qi], sizeof m);
+        s->inst[qi].sigmin = (sg[qi] == ~0ull) ? DBL_MAX : m;
+        s->inst[qi].have_sig = true;
+    }
 }
 
-// Advance up to nsteps steps; when plot_writes, stop segments at snapshot times and write.
+int write_instance_flow(chkn_sim* s, int qi, int tindx, bool binary);
+
+void fill_report(chkn_sim* s, int qi, chkn_report* rep, int done)
+{
+    if (!rep) return;
+    const Instance& I = s->inst[qi];
+    rep->step = I.step; rep->t = I.t; rep->dt = I.dt;
+    rep->bad_stage = 0; rep->bad_count = 0; rep->steps_done = done;
+}
+
+// Advance up to nsteps lockstep steps; when plot_writes, stop segments at snapshot times and
+// write.  Restates the loop of simulate.cu:205-335 for every instance.
 int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report* rep)
 {
     if (s->failed) return set_error(s, CHKN_ERR_STATE, "simulation is in a failed state");
     s->last_ms = 0.0;
     s->last_launches = 0;
+    const int N = (int)s->inst.size();
+    const int cfl_count = s->inst[0].cfl_count;
     int done = 0;
-    int rc = CHKN_OK;
-    while (done < nsteps && s->step < s->max_step && s->t < s->max_time) {
-        if (s->step > 0 && (s->step % s->cfl_count) == 0) {  // simulate.cu:208-221
-            if (!s->have_sig) signal_reduction(s);
-            double cfl = schedule_get(s->cfl_t, s->cfl_v, s->t);
-            double smallest = cfl * s->sigmin;
-            s->dt = std::fmin(1.5 * s->dt, smallest);
-            s->have_sig = false;
+    auto any_running = [&] {
+        for (auto& I : s->inst) if (I.running()) return true;
+        return false;
+    };
+    while (done < nsteps && any_running()) {
+        if (s->gstep > 0 && (s->gstep % cfl_count) == 0) {  // simulate.cu:208-221
+            bool need = false;
+            for (auto& I : s->inst) if (I.running() && !I.have_sig) need = true;
+            if (need) signal_reduction(s);
+            for (auto& I : s->inst) {
+                if (!I.running()) continue;
+                double cfl = schedule_get(I.cfl_t, I.cfl_v, I.t);
+                double smallest = cfl * I.sigmin;
+                I.dt = std::fmin(1.5 * I.dt, smallest);
+                I.have_sig = false;
+            }
         }
         // Segment: constant dt until the next CFL step, bounded by the run limits.
-        int to_cfl = s->cfl_count - (s->step % s->cfl_count);
-        int K = std::min({to_cfl, nsteps - done, s->max_step - s->step, kMaxSeg});
-        {
-            double tt = s->t;
-            int q = 0;
-            while (q < K && tt < s->max_time) {
-                tt += s->dt;
+        int to_cfl = cfl_count - (s->gstep % cfl_count);
+        int K = std::min({to_cfl, nsteps - done, kMaxSeg});
+        std::vector<int> nrun(N, 0);
+        int Kuse = 0;
+        for (int qi = 0; qi < N; ++qi) {
+            Instance& I = s->inst[qi];
+            double tt = I.t;
+            int st = I.step, q = 0;
+            while (q < K && st < I.max_step && tt < I.max_time) {
+                tt += I.dt;
+                ++st;
                 ++q;
-                if (plot_writes && tt >= s->t_plot) break;
+                if (plot_writes && tt >= I.t_plot) break;
             }
-            K = q;
+            nrun[qi] = q;
+            Kuse = std::max(Kuse, q);
         }
-        bool dtred = ((s->step + K) % s->cfl_count) == 0;
+        if (plot_writes)  // a snapshot of any instance ends the segment for all
+            for (int qi = 0; qi < N; ++qi) {
+                Instance& I = s->inst[qi];
+                double tt = I.t;
+                for (int q = 0; q < nrun[qi]; ++q) {
+                    tt += I.dt;
+                    if (tt >= I.t_plot) { Kuse = std::min(Kuse, q + 1); break; }
+                }
+            }
+        K = Kuse;
+        for (int q = 0; q < K; ++q) {
+            double* dtv = ctl_dt(s->h_ctl, q, N);
+            int* run = ctl_run(s->h_ctl, q, N);
+            for (int qi = 0; qi < N; ++qi) {
+                dtv[qi] = s->inst[qi].dt;
+                run[qi] = (q < nrun[qi]) ? 1 : 0;
+            }
+        }
+        bool dtred = ((s->gstep + K) % cfl_count) == 0;
         run_segment(s, K, dtred);
         for (int q = 0; q < K; ++q) {
-            const StepCtl& c = s->h_ctl[q];
+            const int* bad = ctl_bad(s->h_ctl, q, N);
             for (int st = 0; st < 3; ++st)
-                if (c.bad[st] > 0) {
+                if (bad[st] > 0) {
                     s->failed = true;
-                    s->step += q;
-                    if (rep) { rep->bad_stage = st + 1; rep->bad_count = c.bad[st]; }
+                    for (int qi = 0; qi < N; ++qi) s->inst[qi].step += std::min(q, nrun[qi]);
+                    s->gstep += q;
+                    if (rep) { rep->bad_stage = st + 1; rep->bad_count = bad[st]; }
                     char msg[128];
-                    std::snprintf(msg, sizeof msg, "Stage %d bad cell count: %d", st + 1, c.bad[st]);
-                    set_error(s, c.bad[st], msg);
-                    return c.bad[st];
+                    std::snprintf(msg, sizeof msg, "Stage %d bad cell count: %d", st + 1, bad[st]);
+                    set_error(s, bad[st], msg);
+                    return bad[st];
                 }
         }
         if (dtred) {
-            unsigned long long bits = s->h_ctl[K - 1].sigmin_bits;
-            double m;
-            std::memcpy(&m, &bits, sizeof m);
-            if (bits == ~0ull) m = DBL_MAX;  // no active cell reported
-            s->sigmin = m;
-            s->have_sig = true;
+            const unsigned long long* sg = ctl_sig(s->h_ctl, K - 1, N);
+            for (int qi = 0; qi < N; ++qi) {
+                if (nrun[qi] < K) continue;
+                double m;
+                std::memcpy(&m, &sg[qi], sizeof m);
+                s->inst[qi].sigmin = (sg[qi] == ~0ull) ? DBL_MAX : m;
+                s->inst[qi].have_sig = true;
+            }
         }
-        for (int q = 0; q < K; ++q) {  // simulate.cu:306-308
-            s->t += s->dt;
-            s->step += 1;
-            s->steps_since_last_plot += 1;
+        for (int qi = 0; qi < N; ++qi) {  // simulate.cu:306-308
+            Instance& I = s->inst[qi];
+            for (int q = 0; q < nrun[qi]; ++q) {
+                I.t += I.dt;
+                I.step += 1;
+                I.steps_since_last_plot += 1;
+            }
         }
+        s->gstep += K;
         s->fresh = false;
         done += K;
-        if (plot_writes && s->t >= s->t_plot) {  // simulate.cu:329-334
-            rc = chkn_write_flow(s, s->next_plot_indx, binary);
-            if (rc != CHKN_OK) return rc;
-            s->steps_since_last_plot = 0;
-            s->next_plot_indx += 1;
-            s->t_plot = s->t + schedule_get(s->plot_t, s->plot_v, s->t);
-        }
+        if (plot_writes)
+            for (int qi = 0; qi < N; ++qi) {  // simulate.cu:329-334
+                Instance& I = s->inst[qi];
+                if (nrun[qi] > 0 && I.t >= I.t_plot) {
+                    int rc = write_instance_flow(s, qi, I.next_plot_indx, binary);
+                    if (rc != CHKN_OK) return rc;
+                    I.steps_since_last_plot = 0;
+                    I.next_plot_indx += 1;
+                    I.t_plot = I.t + schedule_get(I.plot_t, I.plot_v, I.t);
+                }
+            }
     }
-    if (rep) {
-        rep->step = s->step; rep->t = s->t; rep->dt = s->dt;
-        rep->bad_stage = 0; rep->bad_count = 0; rep->steps_done = done;
-    }
-    return rc;
+    fill_report(s, 0, rep, done);
+    return CHKN_OK;
 }
 
 void block_iovars(chkn_sim* s, int ib, double* out)
@@ -625,6 +775,24 @@ void block_iovars(chkn_sim* s, int ib, double* out)
     }
 }
 
+// write_flow_data (simulate.cu:158-179) for one instance.
+int write_instance_flow(chkn_sim* s, int qi, int tindx, bool binary)
+{
+    Instance& I = s->inst[qi];
+    if (I.job.empty()) return set_error(s, CHKN_ERR_STATE, "no job directory to write into");
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "/flow/t%04d", tindx);
+    std::filesystem::create_directories(I.job + buf);
+    for (int lb = 0; lb < I.nblocks; ++lb) {
+        HostBlock& b = s->blocks[I.first_block + lb];
+        std::vector<double> io((size_t)13 * b.n);
+        block_iovars(s, I.first_block + lb, io.data());
+        write_flow(block_file(I.job, "flow", tindx, b.bi, b.bj, b.bk, binary), binary, io.data(), b.n);
+    }
+    append_times(I.job, tindx, I.t);
+    return CHKN_OK;
+}
+
 template <typename F>
 int guarded(chkn_sim* s, F&& fn)
 {
@@ -639,17 +807,13 @@ int guarded(chkn_sim* s, F&& fn)
     }
 }
 
-}  // namespace
-
-extern "C" {
-
-int chkn_create(const chkn_desc* desc, chkn_sim** out)
+int create_from(const chkn_desc* descs, int n, chkn_sim** out)
 {
-    if (!desc || !out) return set_error(nullptr, CHKN_ERR_ARG, "null argument");
+    if (!descs || !out || n <= 0) return set_error(nullptr, CHKN_ERR_ARG, "null argument");
     *out = nullptr;
     auto s = std::make_unique<chkn_sim>();
     try {
-        setup(s.get(), desc);
+        setup(s.get(), descs, n);
     } catch (const HipError& e) {
         free_device(s.get());
         return set_error(nullptr, CHKN_ERR_HIP, e.what());
@@ -662,6 +826,17 @@ int chkn_create(const chkn_desc* desc, chkn_sim** out)
     }
     *out = s.release();
     return CHKN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chkn_create(const chkn_desc* desc, chkn_sim** out) { return create_from(desc, 1, out); }
+
+int chkn_create_batch(const chkn_desc* descs, int ninst, chkn_sim** out)
+{
+    return create_from(descs, ninst, out);
 }
 
 int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
@@ -688,7 +863,6 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
         std::string m = e.what();
         return set_error(nullptr, m == "Incorrect gas model data" ? CHKN_ERR_CONFIG : CHKN_ERR_IO, m);
     }
-    if (c.reacting) return set_error(nullptr, CHKN_ERR_ARG, "the reacting-gas model is not built");
     std::vector<int> ijk, dims, active, bcc, bcf;
     for (auto& b : c.blocks) {
         ijk.insert(ijk.end(), {b.i, b.j, b.k});
@@ -713,7 +887,7 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
     d.t0 = t0; d.start_tindx = next_plot - 1;
     d.reacting = c.reacting; d.viscous = c.viscous;
     int rc = chkn_create(&d, out);
-    if (rc == CHKN_OK) (*out)->job = job;
+    if (rc == CHKN_OK) (*out)->inst[0].job = job;
     return rc;
 }
 
@@ -726,7 +900,8 @@ int chkn_step(chkn_sim* sim, int nsteps, chkn_report* rep)
 int chkn_march(chkn_sim* sim, int binary, chkn_report* rep)
 {
     if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
-    if (sim->job.empty()) return set_error(sim, CHKN_ERR_STATE, "chkn_march needs a job directory");
+    for (auto& I : sim->inst)
+        if (I.job.empty()) return set_error(sim, CHKN_ERR_STATE, "chkn_march needs a job directory");
     return guarded(sim, [&] { return do_steps(sim, 0x7fffffff, true, binary != 0, rep); });
 }
 
@@ -734,25 +909,26 @@ int chkn_finalize(chkn_sim* sim, int binary)
 {
     if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
     if (sim->failed) return CHKN_OK;  // the reference skips finalize after a thrown error
-    if (sim->steps_since_last_plot > 0) return chkn_write_flow(sim, sim->next_plot_indx, binary);
-    return CHKN_OK;
+    return guarded(sim, [&] {
+        for (int qi = 0; qi < (int)sim->inst.size(); ++qi) {
+            Instance& I = sim->inst[qi];
+            if (I.steps_since_last_plot > 0 && !I.job.empty()) {
+                int rc = write_instance_flow(sim, qi, I.next_plot_indx, binary != 0);
+                if (rc != CHKN_OK) return rc;
+            }
+        }
+        return CHKN_OK;
+    });
 }
 
 int chkn_write_flow(chkn_sim* sim, int tindx, int binary)
 {
     if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
-    if (sim->job.empty()) return set_error(sim, CHKN_ERR_STATE, "no job directory to write into");
     return guarded(sim, [&] {
-        char buf[64];
-        std::snprintf(buf, sizeof buf, "/flow/t%04d", tindx);
-        std::filesystem::create_directories(sim->job + buf);
-        for (size_t ib = 0; ib < sim->blocks.size(); ++ib) {
-            HostBlock& b = sim->blocks[ib];
-            std::vector<double> io((size_t)13 * b.n);
-            block_iovars(sim, (int)ib, io.data());
-            write_flow(block_file(sim->job, "flow", tindx, b.bi, b.bj, b.bk, binary != 0), binary != 0, io.data(), b.n);
+        for (int qi = 0; qi < (int)sim->inst.size(); ++qi) {
+            int rc = write_instance_flow(sim, qi, tindx, binary != 0);
+            if (rc != CHKN_OK) return rc;
         }
-        append_times(sim->job, tindx, sim->t);
         return CHKN_OK;
     });
 }
@@ -783,7 +959,7 @@ int chkn_set_iovars(chkn_sim* sim, int blk, const double* iovars)
         upload_state(sim, b, b.io0.data());
         sim->fresh = true;
         sim->failed = false;
-        sim->have_sig = false;
+        for (auto& I : sim->inst) I.have_sig = false;
         return CHKN_OK;
     });
 }
@@ -807,8 +983,24 @@ int chkn_set_launch_opts(chkn_sim* sim, const chkn_launch_opts* o)
 int chkn_get_report(chkn_sim* sim, chkn_report* rep)
 {
     if (!sim || !rep) return set_error(sim, CHKN_ERR_ARG, "bad argument");
-    rep->step = sim->step; rep->t = sim->t; rep->dt = sim->dt;
-    rep->bad_stage = 0; rep->bad_count = 0; rep->steps_done = 0;
+    fill_report(sim, 0, rep, 0);
+    return CHKN_OK;
+}
+
+int chkn_num_instances(chkn_sim* sim) { return sim ? (int)sim->inst.size() : -1; }
+
+int chkn_instance_blocks(chkn_sim* sim, int inst, int* first, int* count)
+{
+    if (!sim || inst < 0 || inst >= (int)sim->inst.size()) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    if (first) *first = sim->inst[inst].first_block;
+    if (count) *count = sim->inst[inst].nblocks;
+    return CHKN_OK;
+}
+
+int chkn_get_instance_report(chkn_sim* sim, int inst, chkn_report* rep)
+{
+    if (!sim || !rep || inst < 0 || inst >= (int)sim->inst.size()) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    fill_report(sim, inst, rep, 0);
     return CHKN_OK;
 }
 

@@ -94,6 +94,20 @@ typedef struct {
 /* Replaces initialize_simulation (simulate.cu:55-155), array form. */
 int chkn_create(const chkn_desc* desc, chkn_sim** out);
 
+/* Ensemble form of initialize_simulation (simulate.cu:55-155): ninst independent jobs in one
+ * handle.  Each instance keeps its own block array, inflow states, time, time step, CFL and
+ * plot schedules; all instances must share x_order and cfl_count.  Every stage of every
+ * instance is computed by one launch over all blocks (the reference runs one job per
+ * process and one launch per block, simulate.cu:605-625).  Blocks are numbered globally:
+ * instance q owns blocks [first, first+count) of chkn_instance_blocks.  chkn_step advances
+ * all instances in lockstep; an instance that reached its max_step or max_time idles.  A bad
+ * cell in any instance stops the handle; the reported count sums the instances. */
+int chkn_create_batch(const chkn_desc* descs, int ninst, chkn_sim** out);
+int chkn_num_instances(chkn_sim* sim);
+int chkn_instance_blocks(chkn_sim* sim, int inst, int* first, int* count);
+/* Step, time and time step of one instance (chkn_get_report reports instance 0). */
+int chkn_get_instance_report(chkn_sim* sim, int inst, chkn_report* rep);
+
 /* Replaces initialize_simulation (simulate.cu:55-155) reading the job directory's
  * config.json, grid and flow files (binary = 1: .bin, 0: .gz) at time index tindx. */
 int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out);
