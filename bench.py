@@ -82,6 +82,19 @@ def cpu_baseline(job, steps):
                       % (steps, cells, nblocks, threads)}
 
 
+def copy_calibration(torch):
+    """Three copies of a 2 GiB FP64 buffer: each dispatch reads and writes exactly 2 GiB, so
+    the HBM byte counters of these dispatches give the counters' scale factor."""
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +106,8 @@ def main():
     ap.add_argument("--per-block", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 2 GiB device copy (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one block array "
@@ -100,6 +115,9 @@ def main():
         raise SystemExit(2)
 
     import torch
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     from chkn.lib import Sim
 
     job, desc = build_workload(args.workload, args.warmup + args.steps)
@@ -125,10 +143,12 @@ def main():
     value = cells * args.steps / wall
     achieved = alg_bytes / (kernel_ms * 1e-3)
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    tfile = os.path.join(ROOT, "profiles", "stage_kernel_traffic_%s.txt" % args.workload)
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            for ln in f:
+                if ln.startswith("hbm_bytes_per_launch:"):
+                    traffic = float(ln.split(":")[1])
     line = {
         "metric": "cell-updates/s",
         "value": value,
