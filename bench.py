@@ -6,7 +6,9 @@ the CFL time-step reduction every cfl_count steps.  Default workload: the 256x12
 Mach 5 compression-ramp block (FP64, 1% seeded density perturbation), inputs resident in
 HBM before timing.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload scale|plain|batched]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload scale|plain|batched|ensemble]
+
+The launch shape is autotuned first (chkn.tune) unless --tile is given.
 """
 import argparse
 import json
@@ -25,32 +27,61 @@ BYTES_PER_CELL_STEP = 864  # algorithmic FP64 bytes per cell-update (3 stages)
 BYTES_PER_CELL_DT = 144    # extra bytes per cell per CFL reduction
 
 
+WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300}
+
+
 def build_workload(name, steps_total):
+    """(list of JobData, description) of a named workload."""
     from chkn import cases
     if name == "scale":
-        job = cases.ramp(256, 128, 128, max_step=steps_total)
+        jobs = [cases.ramp(256, 128, 128, max_step=steps_total)]
         desc = {"workload": "plain-at-scale: one 256x128x128 block, Mach 5 flow over a 15-degree "
-                            "ramp, p=2e3 Pa, T=220 K, 1% seeded density perturbation, FP64",
-                "cells": 256 * 128 * 128, "blocks": 1}
+                            "ramp, p=2e3 Pa, T=220 K, 1% seeded density perturbation, FP64"}
     elif name == "plain":
-        job = cases.sod_box(64, 32, 32, max_step=steps_total)
-        desc = {"workload": "plain: one 64x32x32 block, 3D Sod shock tube, slip walls, FP64",
-                "cells": 64 * 32 * 32, "blocks": 1}
+        jobs = [cases.sod_box(64, 32, 32, max_step=steps_total)]
+        desc = {"workload": "plain: one 64x32x32 block, 3D Sod shock tube, slip walls, FP64"}
     elif name == "batched":
-        job = cases.ramp_array(seed=7, max_step=steps_total)
-        desc = {"workload": "batched: 5x4x2 array of tensor-product uneven blocks (seed 7), "
-                            "Mach 5 ramp, seeded exchange/wall connection graph, FP64",
-                "cells": job.active_cells, "blocks": len(job.blocks)}
+        jobs = [cases.ramp_array(seed=7, max_step=steps_total)]
+        desc = {"workload": "batched: 5x4x2 array of 40 uneven blocks (slab sizes seeded in "
+                            "[16,96], seed 7), Mach 5 ramp, seeded exchange/wall connection graph, FP64"}
+    elif name == "ensemble":
+        jobs = cases.ramp_ensemble(seed=11, n=64, max_step=steps_total)
+        desc = {"workload": "ensemble: 64 independent single-block Mach 3-8 ramp jobs (seed 11), "
+                            "grids 32-96 x 32-64 x 32-64, per-instance dt, FP64"}
     else:
         raise SystemExit("unknown workload " + name)
-    desc.update({"scheme": "MUSCL/van Albada + AUSMDV, TVD-RK3 (3 stages)", "cfl": 0.5,
-                 "cfl_count": job.cfl_count})
-    return job, desc
+    desc.update({"cells": sum(j.active_cells for j in jobs),
+                 "blocks": sum(len(j.blocks) for j in jobs), "instances": len(jobs),
+                 "scheme": "MUSCL/van Albada + AUSMDV, TVD-RK3 (3 stages)", "cfl": 0.5,
+                 "cfl_count": jobs[0].cfl_count})
+    return jobs, desc
+
+
+def run_timed(jobs, opts, steps, warmup):
+    """Fresh handle, warmup steps, then `steps` timed steps.  Returns (wall_s, kernel_ms,
+    launches, report)."""
+    import torch
+    from chkn.lib import Sim
+    sim = Sim(jobs=jobs) if len(jobs) > 1 else Sim(jobs[0])
+    sim.set_launch(**opts)
+    if warmup:
+        sim.step(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = sim.step(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms, launches = sim.last_timing()
+    cells = sim.active_cells
+    sim.close()
+    return wall, kernel_ms, launches, rep, cells
 
 
 def cpu_baseline(job, steps):
-    """Time the reference's own sources (oracle/_ref) on the first `steps` steps of the same
-    job; fall back to the C oracle when the reference build is absent."""
+    """Time the reference's own sources (oracle/_ref) on the first `steps` steps of `job`,
+    written out through chkn.prep (which holds the job built last); fall back to the C
+    oracle when the reference build is absent.  OpenMP threads: one per block, at most 16
+    (the host share of one GPU)."""
     ref = os.path.join(ROOT, "oracle", "_ref", "chkn_ref")
     cells = job.active_cells
     nblocks = sum(1 for b in job.blocks if b.active)
@@ -59,7 +90,7 @@ def cpu_baseline(job, steps):
         d = tempfile.mkdtemp(prefix="chkn_cpu_")
         try:
             prep.write_initial_files(True, d)
-            threads = min(nblocks, os.cpu_count() or 1)
+            threads = min(nblocks, 16, os.cpu_count() or 1)
             env = dict(os.environ, OMP_NUM_THREADS=str(threads))
             out = subprocess.run([ref, "run", d, "0", str(steps), "-"], env=env, check=True,
                                  capture_output=True, text=True).stdout
@@ -98,12 +129,17 @@ def copy_calibration(torch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="scale", choices=["scale", "plain", "batched"])
-    ap.add_argument("--tile", default="", help="TIxTJxTK launch tile, e.g. 32x8x8")
+    ap.add_argument("--workload", default="scale", choices=sorted(WORKLOAD_STEPS))
+    ap.add_argument("--tile", default="", help="TIxTJxTK launch tile, e.g. 32x8x8 (skips tuning)")
+    ap.add_argument("--order", type=int, default=0, help="tile order with --tile")
+    ap.add_argument("--persistent", type=int, default=0, help="persistent work-groups with --tile")
+    ap.add_argument("--tune", type=int, default=1, help="autotune the launch shape first")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--per-block", type=int, default=0)
+    ap.add_argument("--compare-per-block", type=int, default=-1,
+                    help="also time one launch per block per stage (default: batched only)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -118,29 +154,28 @@ def main():
     if args.copy_calibration:
         copy_calibration(torch)
         return
-    from chkn.lib import Sim
+    from chkn import tune
 
-    job, desc = build_workload(args.workload, args.warmup + args.steps)
-    sim = Sim(job)
-    opts = {"use_graph": bool(args.graph), "per_block_launch": bool(args.per_block)}
+    steps = args.steps or WORKLOAD_STEPS[args.workload]
+    jobs, desc = build_workload(args.workload, args.warmup + steps)
+    tuned = None
     if args.tile:
         ti, tj, tk = (int(v) for v in args.tile.split("x"))
-        opts.update(tile_i=ti, tile_j=tj, tile_k=tk)
-    sim.set_launch(**opts)
-    if args.warmup:
-        sim.step(args.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rep = sim.step(args.steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms, launches = sim.last_timing()
-    cells = sim.active_cells
+        opts = dict(tile_i=ti, tile_j=tj, tile_k=tk, tile_order=args.order,
+                    persistent=bool(args.persistent))
+    elif args.tune:
+        t0 = time.perf_counter()
+        opts, table = tune.autotune(jobs=jobs, per_block_launch=bool(args.per_block))
+        tuned = {"candidates": len(table), "seconds": round(time.perf_counter() - t0, 1),
+                 "best_ms_per_step": table[0][0], "worst_ms_per_step": table[-1][0]}
+    else:
+        opts = {}
+    opts = dict(opts, use_graph=bool(args.graph), per_block_launch=bool(args.per_block))
+    wall, kernel_ms, launches, rep, cells = run_timed(jobs, opts, steps, args.warmup)
     first = args.warmup
-    reductions = sum(1 for s in range(first + 1, first + args.steps + 1)
-                     if s % job.cfl_count == 0)
-    alg_bytes = cells * (BYTES_PER_CELL_STEP * args.steps + BYTES_PER_CELL_DT * reductions)
-    value = cells * args.steps / wall
+    reductions = sum(1 for s in range(first + 1, first + steps + 1) if s % jobs[0].cfl_count == 0)
+    alg_bytes = cells * (BYTES_PER_CELL_STEP * steps + BYTES_PER_CELL_DT * reductions)
+    value = cells * steps / wall
     achieved = alg_bytes / (kernel_ms * 1e-3)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "stage_kernel_traffic_%s.txt" % args.workload)
@@ -149,20 +184,31 @@ def main():
             for ln in f:
                 if ln.startswith("hbm_bytes_per_launch:"):
                     traffic = float(ln.split(":")[1])
+    config = dict(desc, launch=opts, steps_completed=rep.step)
+    if tuned:
+        config["autotune"] = tuned
+    compare = args.compare_per_block if args.compare_per_block >= 0 else int(args.workload == "batched")
+    if compare and not args.per_block:
+        pb_opts, _ = tune.autotune(jobs=jobs, per_block_launch=True)
+        pb_opts = dict(pb_opts, use_graph=bool(args.graph), per_block_launch=True)
+        pw, _, pl, _, _ = run_timed(jobs, pb_opts, steps, args.warmup)
+        config["per_block_comparator"] = {
+            "value": cells * steps / pw, "launches": pl, "launch": pb_opts,
+            "fused_speedup": pw / wall}
     line = {
         "metric": "cell-updates/s",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": wall * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": dict(desc, launch=opts, steps_completed=rep.step),
+        "config": config,
         "hbm_pct_of_peak": 100.0 * achieved / HBM_PEAK,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
@@ -170,9 +216,10 @@ def main():
                      "avg_launch_us": kernel_ms * 1e3 / max(launches, 1),
                      "alg_bytes_per_launch": alg_bytes / max(launches, 1)},
     }
-    sim.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(job, args.cpu_steps)
+        # chkn.prep holds the job built last: for an ensemble, its last member (the reference
+        # runs one job per process).
+        line["cpu_baseline"] = cpu_baseline(jobs[-1], args.cpu_steps)
     print(json.dumps(line))
 
 

@@ -132,10 +132,13 @@ def uneven_sizes(seed, nib=5, njb=4, nkb=2, lo=16, hi=96):
 
 
 def ramp_array(seed=7, nib=5, njb=4, nkb=2, lo=16, hi=96, max_step=300, p_exchange=0.8,
-               sizes=None, title="Mach 5 ramp on uneven blocks"):
+               sizes=None, i_walls=False, title="Mach 5 ramp on uneven blocks"):
     """Ramp flow on an nib x njb x nkb array of blocks whose slab sizes are seeded in
-    [lo, hi]; each internal interface is an exchange pair with probability p_exchange,
-    else a pair of slip walls."""
+    [lo, hi]; each internal j- and k-interface is an exchange pair with probability
+    p_exchange, else a pair of slip walls (plates along the stream).  Internal i-interfaces
+    are exchange pairs unless i_walls: a wall across the Mach 5 stream leaves the block
+    behind it expanding into vacuum, which ends the run with a bad cell within a few hundred
+    steps."""
     _common(max_step, title=title + " (seed %d)" % seed)
     nics, njcs, nkcs = sizes if sizes is not None else uneven_sizes(seed, nib, njb, nkb, lo, hi)
     nib, njb, nkb = len(nics), len(njcs), len(nkcs)
@@ -146,6 +149,8 @@ def ramp_array(seed=7, nib=5, njb=4, nkb=2, lo=16, hi=96, max_step=300, p_exchan
     st = perturbed_state(grid, fs, seed=seed)
     # Interface kinds: True = exchange.  Index [dir][i][j][k] for the plus face of block ijk.
     link = {d: rng.random((nib, njb, nkb)) < p_exchange for d in range(3)}
+    if not i_walls:
+        link[0][:] = True
     io = np.concatenate([[0], np.cumsum(nics)])
     jo = np.concatenate([[0], np.cumsum(njcs)])
     ko = np.concatenate([[0], np.cumsum(nkcs)])
@@ -207,3 +212,20 @@ def mixed_bc_array(max_step=20, seed=3, title="mixed boundary conditions"):
                 init = {key: v[io[i]:io[i + 1], jo[j]:jo[j + 1], ko[k]:ko[k + 1]] for key, v in st.items()}
                 prep.FluidBlock(i=i, j=j, k=k, grid=sub, initialState=init, bcs=bcs, active=active)
     return JobData.from_prep()
+
+
+def ramp_ensemble(seed=11, n=64, max_step=300, nic=(32, 96), njk=(32, 64)):
+    """n independent single-block ramp jobs with seeded inflow Mach in [3, 8], p in
+    [1e3, 1e4] Pa, T in [200, 300] K and grids of nic in nic-range by njc, nkc in njk-range."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for q in range(n):
+        mach = float(rng.uniform(3.0, 8.0))
+        p = float(rng.uniform(1.0e3, 1.0e4))
+        T = float(rng.uniform(200.0, 300.0))
+        ni = int(rng.integers(nic[0], nic[1] + 1))
+        nj = int(rng.integers(njk[0], njk[1] + 1))
+        nk = int(rng.integers(njk[0], njk[1] + 1))
+        jobs.append(ramp(ni, nj, nk, max_step=max_step, seed=seed * 1000 + q, mach=mach, p=p, T=T,
+                         title="ensemble member %d: Mach %.3f, p %.1f Pa, T %.2f K" % (q, mach, p, T)))
+    return jobs

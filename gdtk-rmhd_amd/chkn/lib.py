@@ -39,7 +39,7 @@ class ChknReport(ctypes.Structure):
 class ChknLaunchOpts(ctypes.Structure):
     _fields_ = [("tile_i", ctypes.c_int), ("tile_j", ctypes.c_int), ("tile_k", ctypes.c_int),
                 ("per_block_launch", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("persistent", ctypes.c_int)]
+                ("persistent", ctypes.c_int), ("tile_order", ctypes.c_int)]
 
 
 EXPORTS = {
@@ -50,6 +50,16 @@ EXPORTS = {
     "chkn_instance_blocks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I, _I]),
     "chkn_get_instance_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                                 ctypes.POINTER(ChknReport)]),
+    "chkn_state_size": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "chkn_num_flow_states": (ctypes.c_int, [ctypes.c_void_p]),
+    "chkn_read_conserved_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "chkn_write_conserved_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "chkn_set_inflow_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    "chkn_get_inflow_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    "chkn_boundary_face_areas": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]),
+    "chkn_grad_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
+    "chkn_grad_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _D]),
+    "chkn_flux_vjp_points": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D]),
     "chkn_load_job": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_void_p)]),
     "chkn_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ChknReport)]),
@@ -105,6 +115,26 @@ class BadCellError(ChknError):
 
 def _ptr(a, typ):
     return a.ctypes.data_as(typ)
+
+
+def _check_dev(t, n):
+    import torch
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == n):
+        raise ValueError("expected a contiguous float64 device tensor of %d elements" % n)
+
+
+def flux_vjp_points(prims, frames, fbar, x_order=2):
+    """Face-flux vector-Jacobian products on the device (test hook of the adjoint kernels).
+    prims (n, 4, 6), frames (n, 9), fbar (n, 6) -> (n, 4, 6)."""
+    lib = load()
+    n = len(prims)
+    inp = np.ascontiguousarray(np.concatenate([np.reshape(prims, (n, 24)), np.reshape(frames, (n, 9)),
+                                               np.reshape(fbar, (n, 6))], axis=1), dtype=np.float64)
+    out = np.zeros((n, 24))
+    rc = lib.chkn_flux_vjp_points(n, x_order, _ptr(inp, _D), _ptr(out, _D))
+    if rc != 0:
+        raise ChknError(rc, lib.chkn_last_error(None).decode())
+    return out.reshape(n, 4, 6)
 
 
 def _desc(job, keep):
@@ -200,9 +230,9 @@ class Sim:
         self._check(self._lib.chkn_set_iovars(self.h, blk, _ptr(io, _D)))
 
     def set_launch(self, tile_i=0, tile_j=0, tile_k=0, per_block_launch=False, use_graph=False,
-                   persistent=False):
+                   persistent=False, tile_order=0):
         o = ChknLaunchOpts(tile_i, tile_j, tile_k, int(per_block_launch), int(use_graph),
-                           int(persistent))
+                           int(persistent), int(tile_order))
         self._check(self._lib.chkn_set_launch_opts(self.h, ctypes.byref(o)))
 
     def report(self):
@@ -220,6 +250,65 @@ class Sim:
         rep = ChknReport()
         self._check(self._lib.chkn_get_instance_report(self.h, inst, ctypes.byref(rep)))
         return rep
+
+    # ---------------------------------------------------------------- gradients
+    @property
+    def state_size(self):
+        """Doubles in a flat conserved-state array (every block's 6*ncells, block order)."""
+        return int(self._lib.chkn_state_size(self.h))
+
+    @property
+    def num_flow_states(self):
+        return int(self._lib.chkn_num_flow_states(self.h))
+
+    def read_conserved(self, out):
+        """Copy the conserved state into `out`, a contiguous float64 device tensor."""
+        _check_dev(out, self.state_size)
+        self._check(self._lib.chkn_read_conserved_dev(self.h, ctypes.c_void_p(out.data_ptr())))
+
+    def write_conserved(self, src):
+        """Set the conserved state (and W = decode(U)) from a float64 device tensor."""
+        _check_dev(src, self.state_size)
+        self._check(self._lib.chkn_write_conserved_dev(self.h, ctypes.c_void_p(src.data_ptr())))
+
+    def set_inflow_state(self, state, w6):
+        w = np.ascontiguousarray(w6, dtype=np.float64).reshape(6)
+        self._check(self._lib.chkn_set_inflow_state(self.h, int(state), _ptr(w, _D)))
+
+    def get_inflow_state(self, state):
+        w = np.zeros(6)
+        self._check(self._lib.chkn_get_inflow_state(self.h, int(state), _ptr(w, _D)))
+        return w
+
+    def boundary_face_areas(self, blk, side):
+        a, b, c = self.dims(blk)
+        d = side // 2
+        n = (b * c) if d == 0 else ((a * c) if d == 1 else (a * b))
+        out = np.zeros(n)
+        self._check(self._lib.chkn_boundary_face_areas(self.h, int(blk), int(side), _ptr(out, _D)))
+        return out
+
+    def grad_forward(self, nsteps, checkpoint_every, dts=None):
+        """Advance nsteps steps recording checkpoints; returns the dts used, (nsteps, ninst)."""
+        out = np.zeros((nsteps, self.ninstances))
+        dptr = None
+        if dts is not None:
+            dts = np.ascontiguousarray(dts, dtype=np.float64).reshape(nsteps, self.ninstances)
+            dptr = _ptr(dts, _D)
+        self._check(self._lib.chkn_grad_forward(self.h, int(nsteps), int(checkpoint_every), dptr,
+                                                _ptr(out, _D)))
+        return out
+
+    def grad_backward(self, lam, grad_u0):
+        """Reverse the last grad_forward: lam, grad_u0 are flat float64 device tensors.
+        Returns dLoss/d(rho, e, YB, vx, vy, vz) of each inflow state, (nstates, 6)."""
+        _check_dev(lam, self.state_size)
+        _check_dev(grad_u0, self.state_size)
+        gin = np.zeros((max(self.num_flow_states, 1), 6))
+        self._check(self._lib.chkn_grad_backward(self.h, ctypes.c_void_p(lam.data_ptr()),
+                                                 ctypes.c_void_p(grad_u0.data_ptr()),
+                                                 _ptr(gin, _D)))
+        return gin[:self.num_flow_states]
 
     def last_timing(self):
         ms = ctypes.c_double()

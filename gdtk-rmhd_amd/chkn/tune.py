@@ -1,0 +1,112 @@
+"""Launch-shape autotuner for libchkn.
+
+The stage kernel's speed depends on the tile shape (TI x TJ cells per work-group), the
+number of k-planes a work-group marches (tile_k), the order of the tile table and whether
+work-groups are persistent.  None of these changes the results (the parity tests check
+that), so the tuner times each candidate on a scratch handle built from the same job(s),
+on the real block sizes, and returns the fastest.  Results are cached per problem shape.
+
+    from chkn import tune
+    best, table = tune.autotune(job)            # or tune.autotune(jobs=[...])
+    sim.set_launch(**best)
+"""
+import copy
+import hashlib
+import json
+import os
+
+from .lib import BadCellError, Sim
+
+TILES = [(32, 8), (64, 4), (16, 16), (64, 8), (16, 4), (8, 8)]
+TILE_K = [0, 4, 8, 16]
+ORDERS = [0, 1]
+PERSISTENT = [0, 1]
+
+
+def candidates(tiles=TILES, tile_k=TILE_K, orders=ORDERS, persistent=PERSISTENT):
+    """Every launch option combination to try (graph replay is always on)."""
+    out = []
+    for ti, tj in tiles:
+        for tk in tile_k:
+            for o in orders:
+                for p in persistent:
+                    out.append(dict(tile_i=ti, tile_j=tj, tile_k=tk, tile_order=o,
+                                    persistent=bool(p)))
+    return out
+
+
+def shape_key(jobs):
+    """Cache key: block sizes, activity, x_order and the GPU's name."""
+    h = hashlib.sha1()
+    for j in jobs:
+        h.update(repr((j.x_order, [(b.nic, b.njc, b.nkc, b.active) for b in j.blocks])).encode())
+    try:
+        import torch
+        h.update(torch.cuda.get_device_name(0).encode())
+    except Exception:
+        pass
+    return h.hexdigest()
+
+
+def _scratch(jobs):
+    out = []
+    for j in jobs:
+        c = copy.copy(j)
+        c.max_step = 1 << 30
+        c.max_time = 1.0e30
+        out.append(c)
+    return out
+
+
+def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=None,
+             per_block_launch=False, verbose=False):
+    """Time every candidate for `steps` steps (after `warmup` untimed steps) on a scratch
+    handle and return (best_options, [(ms_per_step, options), ...] sorted fastest first).
+
+    Timing is the device time of the step kernels (HIP events around the launches).  The
+    scratch handle's state advances during the sweep; a candidate that meets a bad cell
+    rebuilds the handle from the initial state and is timed again."""
+    jobs = list(jobs) if jobs is not None else [job]
+    cands = cands if cands is not None else candidates()
+    key = shape_key(jobs) + ("-pb" if per_block_launch else "")
+    cache = {}
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            cache = json.load(f)
+        if key in cache:
+            return cache[key]["best"], [tuple(r) for r in cache[key]["table"]]
+    scratch = _scratch(jobs)
+
+    def fresh():
+        return Sim(jobs=scratch) if len(scratch) > 1 else Sim(scratch[0])
+
+    sim = fresh()
+    table = []
+    for c in cands:
+        opts = dict(c, use_graph=True, per_block_launch=per_block_launch)
+        for attempt in range(2):
+            try:
+                sim.set_launch(**opts)
+                if warmup:
+                    sim.step(warmup)
+                sim.step(steps)
+                break
+            except BadCellError:
+                sim.close()
+                sim = fresh()
+                if attempt == 1:
+                    raise
+        ms, _ = sim.last_timing()
+        table.append((ms / steps, opts))
+        if verbose:
+            print("%8.3f ms/step  %s" % (ms / steps, opts), flush=True)
+    sim.close()
+    table.sort(key=lambda r: r[0])
+    best = table[0][1]
+    if cache_path:
+        cache[key] = {"best": best, "table": table}
+        d = os.path.dirname(os.path.abspath(cache_path))
+        os.makedirs(d, exist_ok=True)
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1)
+    return best, table

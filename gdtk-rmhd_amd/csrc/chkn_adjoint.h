@@ -1,0 +1,291 @@
+// chkn_adjoint.h -- hand-written reverse-mode derivatives (vector-Jacobian products) of the
+// pointwise physics in chkn_physics.h: MUSCL/van Albada reconstruction, the ideal-gas
+// equation of state, the AUSMDV flux (face.cu:163-265, :347-371) and the conserved ->
+// primitive decode (flow.cu:245-268).
+//
+// Each *_vjp function recomputes the forward intermediates it needs and accumulates
+// adjoints into its outputs.  At the non-differentiable points of the scheme (|x| at 0,
+// min/max ties, the sub/supersonic switch at |M| = 1, the K_SWITCH clip) the derivative of
+// the branch the forward evaluation takes is used.  Host and device code.
+#pragma once
+
+#include "chkn_physics.h"
+
+namespace chkn {
+
+CHKN_HD double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+
+// Partial derivatives of van_albada(a, b) = (ab + |ab| + eps) / (a^2 + b^2 + eps).
+CHKN_HD void van_albada_grad(double a, double b, double& da, double& db)
+{
+    const double eps = 1.0e-12;
+    const double ab = a * b;
+    const double N = ab + fabs(ab) + eps;
+    const double D = a * a + b * b + eps;
+    const double k = 1.0 + sgn(ab);
+    const double D2 = D * D;
+    da = (b * k * D - N * 2.0 * a) / D2;
+    db = (a * k * D - N * 2.0 * b) / D2;
+}
+
+// Adjoint of muscl(qL1, qL0, qR0, qR1) -> (qL, qR): accumulates into g[0..3].
+CHKN_HD void muscl_vjp(double qL1, double qL0, double qR0, double qR1, double bqL, double bqR,
+                       double g[4])
+{
+    const double c = 0.125;
+    const double dm = qL0 - qL1, d = qR0 - qL0, dp = qR1 - qR0;
+    const double sL = van_albada(dm, d), sR = van_albada(d, dp);
+    double bdm = 0.0, bd = 0.0, bdp = 0.0;
+    // qL = qL0 + sL*c*(3d + dm)
+    g[1] += bqL;
+    const double bsL = bqL * c * (3.0 * d + dm);
+    bd += bqL * sL * c * 3.0;
+    bdm += bqL * sL * c;
+    // qR = qR0 - sR*c*(dp + 3d)
+    g[2] += bqR;
+    const double bsR = -bqR * c * (dp + 3.0 * d);
+    bdp -= bqR * sR * c;
+    bd -= bqR * sR * c * 3.0;
+    double ga, gb;
+    van_albada_grad(dm, d, ga, gb);
+    bdm += bsL * ga;
+    bd += bsL * gb;
+    van_albada_grad(d, dp, ga, gb);
+    bd += bsR * ga;
+    bdp += bsR * gb;
+    g[1] += bdm; g[0] -= bdm;  // dm = qL0 - qL1
+    g[2] += bd;  g[1] -= bd;   // d = qR0 - qL0
+    g[3] += bdp; g[2] -= bdp;  // dp = qR1 - qR0
+}
+
+// Adjoint of the AUSMDV flux with respect to its face states.  Inputs as ausmdv(); outputs
+// accumulate adjoints of (rho, e, p, a, YB, vx, vy, vz) on each side: bL[8], bR[8].
+CHKN_HD void ausmdv_vjp(double rhoL, double eL, double pL, double aL, double YBL, double vxL,
+                        double vyL, double vzL, double rhoR, double eR, double pR, double aR,
+                        double YBR, double vxR, double vyR, double vzR, const Frame& f,
+                        const double Fb[6], double bL[8], double bR[8])
+{
+    // ---- forward (same expressions as ausmdv)
+    double uL, vL, wL, uR, vR, wR;
+    to_local(f, vxL, vyL, vzL, uL, vL, wL);
+    to_local(f, vxR, vyR, vzR, uR, vR, wR);
+    const double pLrL = pL / rhoL, pRrR = pR / rhoR;
+    const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+    const double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+    const double HL = eL + pLrL + keL, HR = eR + pRrR + keR;
+    const double S = pLrL + pRrR;
+    const double alphaL = 2.0 * pLrL / S, alphaR = 2.0 * pRrR / S;
+    const double am = fmax(aL, aR);
+    const double ML = uL / am, MR = uR / am;
+    const double dvL = 0.5 * (uL + fabs(uL));
+    const bool subL = fabs(ML) <= 1.0;
+    const double uLplus = subL ? alphaL * ((uL + am) * (uL + am) / (4.0 * am) - dvL) + dvL : dvL;
+    const double dvR = 0.5 * (uR - fabs(uR));
+    const bool subR = fabs(MR) <= 1.0;
+    const double uRminus = subR ? alphaR * (-(uR - am) * (uR - am) / (4.0 * am) - dvR) + dvR : dvR;
+    const double massL = uLplus * rhoL, massR = uRminus * rhoR;
+    const double m = massL + massR;
+    const double g = pL - pR;
+    const double pmin = fmin(pL, pR);
+    const double dp = 10.0 * fabs(g) / pmin;
+    const double s = 0.5 * fmin(1.0, dp);
+    const double rvV = massL * uL + massR * uR;
+    const double rvD = 0.5 * (m * (uL + uR) - fabs(m) * (uR - uL));
+    const bool up = (m >= 0.0);
+    const double vt = up ? vL : vR, wt = up ? wL : wR;
+    const double Hu = up ? HL : HR, Yu = up ? YBL : YBR;
+
+    // ---- reverse
+    double X1, X2, X3;  // adjoint of the local-frame flux (transpose of to_global)
+    to_local(f, Fb[1], Fb[2], Fb[3], X1, X2, X3);
+    const double brv2 = X1, bph = X1;
+    double bm = Fb[0] + X2 * vt + X3 * wt + Fb[4] * Hu + Fb[5] * Yu;
+    const double bvt = X2 * m, bwt = X3 * m, bHu = Fb[4] * m, bYu = Fb[5] * m;
+    double buL = 0.0, bvL = 0.0, bwL = 0.0, buR = 0.0, bvR = 0.0, bwR = 0.0;
+    double bHL = 0.0, bHR = 0.0;
+    if (up) { bvL += bvt; bwL += bwt; bHL += bHu; bL[4] += bYu; }
+    else    { bvR += bvt; bwR += bwt; bHR += bHu; bR[4] += bYu; }
+    // rv2 = (0.5 + s) rvV + (0.5 - s) rvD
+    const double bs = brv2 * (rvV - rvD);
+    const double brvV = brv2 * (0.5 + s), brvD = brv2 * (0.5 - s);
+    // rvD = 0.5 (m (uL + uR) - |m| (uR - uL))
+    bm += brvD * 0.5 * ((uL + uR) - sgn(m) * (uR - uL));
+    buL += brvD * 0.5 * (m + fabs(m));
+    buR += brvD * 0.5 * (m - fabs(m));
+    // rvV = massL uL + massR uR
+    double bmassL = brvV * uL, bmassR = brvV * uR;
+    buL += brvV * massL;
+    buR += brvV * massR;
+    // s = 0.5 min(1, dp); dp = 10 |g| / pmin
+    if (dp < 1.0) {
+        const double bdp = 0.5 * bs;
+        const double bg = bdp * 10.0 * sgn(g) / pmin;
+        const double bpmin = -bdp * dp / pmin;
+        bL[2] += bg;
+        bR[2] -= bg;
+        if (pL <= pR) bL[2] += bpmin;
+        else bR[2] += bpmin;
+    }
+    // p_half = pLplus + pRminus; m = massL + massR
+    const double bpLplus = bph, bpRminus = bph;
+    bmassL += bm;
+    bmassR += bm;
+    const double buLplus = bmassL * rhoL, buRminus = bmassR * rhoR;
+    bL[0] += bmassL * uLplus;
+    bR[0] += bmassR * uRminus;
+    double bam = 0.0, balphaL = 0.0, balphaR = 0.0, bML = 0.0, bMR = 0.0;
+    double bdvL = 0.0, bdvR = 0.0;
+    if (subL) {
+        const double q1 = (ML + 1.0) * (ML + 1.0);
+        bL[2] += bpLplus * q1 * (2.0 - ML) * 0.25;
+        bML += bpLplus * pL * 0.25 * (2.0 * (ML + 1.0) * (2.0 - ML) - q1);
+        const double Q = (uL + am) * (uL + am) / (4.0 * am);
+        balphaL += buLplus * (Q - dvL);
+        const double bQ = buLplus * alphaL;
+        bdvL += buLplus * (1.0 - alphaL);
+        buL += bQ * 2.0 * (uL + am) / (4.0 * am);
+        bam += bQ * (2.0 * (uL + am) / (4.0 * am) - (uL + am) * (uL + am) / (4.0 * am * am));
+    } else {
+        bL[2] += bpLplus * dvL / uL;
+        bdvL += bpLplus * pL / uL;
+        buL -= bpLplus * pL * dvL / (uL * uL);
+        bdvL += buLplus;
+    }
+    buL += bdvL * 0.5 * (1.0 + sgn(uL));
+    if (subR) {
+        const double q1 = (MR - 1.0) * (MR - 1.0);
+        bR[2] += bpRminus * q1 * (2.0 + MR) * 0.25;
+        bMR += bpRminus * pR * 0.25 * (2.0 * (MR - 1.0) * (2.0 + MR) + q1);
+        const double Q = -(uR - am) * (uR - am) / (4.0 * am);
+        balphaR += buRminus * (Q - dvR);
+        const double bQ = buRminus * alphaR;
+        bdvR += buRminus * (1.0 - alphaR);
+        buR -= bQ * 2.0 * (uR - am) / (4.0 * am);
+        bam += bQ * (2.0 * (uR - am) / (4.0 * am) + (uR - am) * (uR - am) / (4.0 * am * am));
+    } else {
+        bR[2] += bpRminus * dvR / uR;
+        bdvR += bpRminus * pR / uR;
+        buR -= bpRminus * pR * dvR / (uR * uR);
+        bdvR += buRminus;
+    }
+    buR += bdvR * 0.5 * (1.0 - sgn(uR));
+    // ML = uL / am, MR = uR / am
+    buL += bML / am;
+    buR += bMR / am;
+    bam -= (bML * ML + bMR * MR) / am;
+    if (aL >= aR) bL[3] += bam;
+    else bR[3] += bam;
+    // alpha = 2 prho / S, S = pLrL + pRrR
+    double bpLrL = balphaL * 2.0 / S, bpRrR = balphaR * 2.0 / S;
+    const double bS = -(balphaL * alphaL + balphaR * alphaR) / S;
+    bpLrL += bS;
+    bpRrR += bS;
+    // H = e + p/rho + ke
+    bL[1] += bHL; bpLrL += bHL;
+    bR[1] += bHR; bpRrR += bHR;
+    buL += bHL * uL; bvL += bHL * vL; bwL += bHL * wL;
+    buR += bHR * uR; bvR += bHR * vR; bwR += bHR * wR;
+    // p/rho
+    bL[2] += bpLrL / rhoL;
+    bL[0] -= bpLrL * pLrL / rhoL;
+    bR[2] += bpRrR / rhoR;
+    bR[0] -= bpRrR * pRrR / rhoR;
+    // velocities back to the global frame (transpose of to_local)
+    double gx, gy, gz;
+    to_global(f, buL, bvL, bwL, gx, gy, gz);
+    bL[5] += gx; bL[6] += gy; bL[7] += gz;
+    to_global(f, buR, bvR, bwR, gx, gy, gz);
+    bR[5] += gx; bR[6] += gy; bR[7] += gz;
+}
+
+// Adjoint of face_flux<XORDER>: accumulates the adjoints of the four stencil states
+// (rho, e, YB, vx, vy, vz each) into b[0..3][6] (L1, L0, R0, R1).
+template <int XORDER>
+CHKN_HD void face_flux_vjp(const Prim& L1, const Prim& L0, const Prim& R0, const Prim& R1,
+                           const Frame& f, const double Fb[6], double b[4][6])
+{
+    double rhoL, rhoR, eL, eR, YBL, YBR, vxL, vxR, vyL, vyR, vzL, vzR;
+    if (XORDER > 1) {
+        muscl(L1.rho, L0.rho, R0.rho, R1.rho, rhoL, rhoR);
+        muscl(L1.e, L0.e, R0.e, R1.e, eL, eR);
+        muscl(L1.YB, L0.YB, R0.YB, R1.YB, YBL, YBR);
+        muscl(L1.vx, L0.vx, R0.vx, R1.vx, vxL, vxR);
+        muscl(L1.vy, L0.vy, R0.vy, R1.vy, vyL, vyR);
+        muscl(L1.vz, L0.vz, R0.vz, R1.vz, vzL, vzR);
+    } else {
+        rhoL = L0.rho; eL = L0.e; YBL = L0.YB; vxL = L0.vx; vyL = L0.vy; vzL = L0.vz;
+        rhoR = R0.rho; eR = R0.e; YBR = R0.YB; vxR = R0.vx; vyR = R0.vy; vzR = R0.vz;
+    }
+    const double TL = eos_T(eL, YBL), TR = eos_T(eR, YBR);
+    const double pL = eos_p(rhoL, TL), pR = eos_p(rhoR, TR);
+    const double aL = eos_a(TL), aR = eos_a(TR);
+    double bL[8] = {0, 0, 0, 0, 0, 0, 0, 0}, bR[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    ausmdv_vjp(rhoL, eL, pL, aL, YBL, vxL, vyL, vzL, rhoR, eR, pR, aR, YBR, vxR, vyR, vzR, f, Fb,
+               bL, bR);
+    // Equation of state: T = (e + YB Q)/Cv, p = rho R T, a = sqrt(gamma R T).
+    double side[2][6];  // adjoints of the reconstructed (rho, e, YB, vx, vy, vz)
+    const double* bs[2] = {bL, bR};
+    const double rho[2] = {rhoL, rhoR}, T[2] = {TL, TR}, a[2] = {aL, aR};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double* bb = bs[q];
+        const double bT = bb[2] * rho[q] * kR + bb[3] * kGamma * kR / (2.0 * a[q]);
+        side[q][0] = bb[0] + bb[2] * kR * T[q];
+        side[q][1] = bb[1] + bT / kCv;
+        side[q][2] = bb[4] + bT * kQ / kCv;
+        side[q][3] = bb[5];
+        side[q][4] = bb[6];
+        side[q][5] = bb[7];
+    }
+    if (XORDER > 1) {
+        double g[4];
+#define CHKN_MUSCL_VJP(Q, FIELD)                                                      \
+        g[0] = g[1] = g[2] = g[3] = 0.0;                                              \
+        muscl_vjp(L1.FIELD, L0.FIELD, R0.FIELD, R1.FIELD, side[0][Q], side[1][Q], g); \
+        b[0][Q] += g[0]; b[1][Q] += g[1]; b[2][Q] += g[2]; b[3][Q] += g[3];
+        CHKN_MUSCL_VJP(0, rho)
+        CHKN_MUSCL_VJP(1, e)
+        CHKN_MUSCL_VJP(2, YB)
+        CHKN_MUSCL_VJP(3, vx)
+        CHKN_MUSCL_VJP(4, vy)
+        CHKN_MUSCL_VJP(5, vz)
+#undef CHKN_MUSCL_VJP
+    } else {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            b[1][q] += side[0][q];
+            b[2][q] += side[1][q];
+        }
+    }
+}
+
+// Adjoint of decode(U) -> (rho, e, YB, vx, vy, vz) at the primitive state w:
+// Ub[q] = sum_p Wb[p] dW_p/dU_q.
+CHKN_HD void decode_vjp(const Prim& w, const double Wb[6], double Ub[6])
+{
+    const double rho = w.rho;
+    const double dinv = 1.0 / rho;
+    const double bx = Wb[3] - Wb[1] * w.vx;  // velocity adjoints including e = U4/rho - ke
+    const double by = Wb[4] - Wb[1] * w.vy;
+    const double bz = Wb[5] - Wb[1] * w.vz;
+    const double ke = 0.5 * (w.vx * w.vx + w.vy * w.vy + w.vz * w.vz);
+    Ub[1] = bx * dinv;
+    Ub[2] = by * dinv;
+    Ub[3] = bz * dinv;
+    Ub[4] = Wb[1] * dinv;
+    Ub[5] = Wb[2] * dinv;
+    Ub[0] = Wb[0] - (bx * w.vx + by * w.vy + bz * w.vz + Wb[1] * (w.e + ke) + Wb[2] * w.YB) * dinv;
+}
+
+// Adjoint of the wall ghost state reflect(s, f): the map is v -> F^T diag(-1,1,1) F v,
+// which is symmetric, so its transpose applied to the ghost's velocity adjoint is the same
+// reflection; rho, e and YB pass through.
+CHKN_HD void reflect_vjp(const double Gb[6], const Frame& f, double Sb[6])
+{
+    Prim g;
+    g.rho = Gb[0]; g.e = Gb[1]; g.YB = Gb[2]; g.vx = Gb[3]; g.vy = Gb[4]; g.vz = Gb[5];
+    const Prim r = reflect(g, f);
+    Sb[0] += r.rho; Sb[1] += r.e; Sb[2] += r.YB; Sb[3] += r.vx; Sb[4] += r.vy; Sb[5] += r.vz;
+}
+
+}  // namespace chkn

@@ -1,0 +1,530 @@
+// chkn_grad.cpp -- gradients of an unrolled chicken march (reverse mode), C-ABI part.
+//
+// chkn_grad_forward runs n steps from the handle's current state, keeping a checkpoint (U
+// and W) every C steps and the time step of every step.  chkn_grad_backward takes the
+// adjoint of the final conserved state and walks the checkpoint segments backwards: each
+// segment is recomputed from its checkpoint with the recorded time steps, writing the
+// 3C+1 stage states of the segment into a tape (the block table's W is re-pointed at the
+// tape, so the stage kernels are the forward ones), then the steps of the segment are
+// reversed with the adjoint kernels.  The time-step schedule is held fixed (the gradient of
+// the CFL min-reduction is not taken), so the gradient is that of the map with the
+// recorded dt sequence.  The recomputation repeats the forward arithmetic exactly, so the
+// gradient does not depend on C.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/chkn.h"
+#include "chkn_internal.h"
+#include "chkn_runtime.h"
+
+using namespace chkn;
+using namespace chkn::rt;
+
+namespace {
+
+// Work tables of the adjoint kernels (faces, cells, inflow lines), built once per handle.
+void build_adj_tables(chkn_sim* s)
+{
+    GradState& g = *s->grad;
+    if (g.d_face_chunks) return;
+    std::vector<FaceChunk> fc;
+    std::vector<CellChunk> cc;
+    std::vector<InflowCol> ic;
+    for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
+        const HostBlock& b = s->blocks[ib];
+        if (!b.active) continue;
+        const long long nf[3] = {(long long)(b.nic + 1) * b.njc * b.nkc,
+                                 (long long)b.nic * (b.njc + 1) * b.nkc,
+                                 (long long)b.nic * b.njc * (b.nkc + 1)};
+        for (int d = 0; d < 3; ++d)
+            for (long long f0 = 0; f0 < nf[d]; f0 += 256) fc.push_back(FaceChunk{(int)ib, d, (int)f0});
+        for (int c0 = 0; c0 < b.n; c0 += 256) cc.push_back(CellChunk{(int)ib, c0});
+        for (int side = 0; side < 6; ++side) {
+            if (b.bc[side] != CHKN_BC_INFLOW) continue;
+            const int d = side >> 1;
+            const int nt = d == 0 ? b.njc * b.nkc : (d == 1 ? b.nic * b.nkc : b.nic * b.njc);
+            for (int t = 0; t < nt; ++t) {
+                ic.push_back(InflowCol{(int)ib, side, t});
+                g.inflow_col_state.push_back(b.bc_fs[side]);
+            }
+        }
+    }
+    g.nface_chunks = (int)fc.size();
+    g.ncell_chunks = (int)cc.size();
+    g.ninflow_cols = (int)ic.size();
+    HIPCHK(hipMalloc(&g.d_face_chunks, sizeof(FaceChunk) * std::max<size_t>(fc.size(), 1)));
+    HIPCHK(hipMalloc(&g.d_cell_chunks, sizeof(CellChunk) * std::max<size_t>(cc.size(), 1)));
+    HIPCHK(hipMalloc(&g.d_inflow_cols, sizeof(InflowCol) * std::max<size_t>(ic.size(), 1)));
+    HIPCHK(hipMalloc(&g.d_inflow_acc, sizeof(double) * 6 * std::max<size_t>(ic.size(), 1)));
+    if (!fc.empty())
+        HIPCHK(hipMemcpy(g.d_face_chunks, fc.data(), sizeof(FaceChunk) * fc.size(), hipMemcpyHostToDevice));
+    if (!cc.empty())
+        HIPCHK(hipMemcpy(g.d_cell_chunks, cc.data(), sizeof(CellChunk) * cc.size(), hipMemcpyHostToDevice));
+    if (!ic.empty())
+        HIPCHK(hipMemcpy(g.d_inflow_cols, ic.data(), sizeof(InflowCol) * ic.size(), hipMemcpyHostToDevice));
+}
+
+GradState& grad_state(chkn_sim* s)
+{
+    if (!s->grad) s->grad.reset(new GradState());
+    return *s->grad;
+}
+
+// Checkpoint slot j of every active block: U, then W (slot 0 of the ring).
+void save_checkpoint(chkn_sim* s, int j)
+{
+    for (auto& b : s->blocks) {
+        if (!b.active) continue;
+        const size_t n6 = 6 * (size_t)b.n;
+        double* dst = b.ckpt + (size_t)j * 2 * n6;
+        HIPCHK(hipMemcpyAsync(dst, b.U, sizeof(double) * n6, hipMemcpyDeviceToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(dst + n6, b.W, sizeof(double) * n6, hipMemcpyDeviceToDevice, s->stream));
+    }
+}
+
+// Restores U from checkpoint j and writes its W into `wdst(block)`.
+template <typename F>
+void load_checkpoint(chkn_sim* s, int j, F wdst)
+{
+    for (auto& b : s->blocks) {
+        if (!b.active) continue;
+        const size_t n6 = 6 * (size_t)b.n;
+        const double* src = b.ckpt + (size_t)j * 2 * n6;
+        HIPCHK(hipMemcpyAsync(b.U, src, sizeof(double) * n6, hipMemcpyDeviceToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(wdst(b), src + n6, sizeof(double) * n6, hipMemcpyDeviceToDevice, s->stream));
+    }
+}
+
+void alloc_checkpoints(chkn_sim* s, int nckpt)
+{
+    GradState& g = *s->grad;
+    if (g.nckpt_alloc >= nckpt) return;
+    for (auto& b : s->blocks) {
+        if (!b.active) continue;
+        dfree(b.ckpt);
+        b.ckpt = nullptr;
+        HIPCHK(hipMalloc(&b.ckpt, sizeof(double) * 12 * (size_t)b.n * nckpt));
+    }
+    g.nckpt_alloc = nckpt;
+}
+
+void alloc_adjoint(chkn_sim* s, int C)
+{
+    GradState& g = *s->grad;
+    for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
+        HostBlock& b = s->blocks[ib];
+        if (!b.active) continue;
+        if (g.tape_C < C) {
+            dfree(b.tape);
+            b.tape = nullptr;
+            HIPCHK(hipMalloc(&b.tape, sizeof(double) * 6 * (size_t)b.n * (3 * (size_t)C + 1)));
+        }
+        if (!b.adj) {
+            const size_t nf = (size_t)(b.nic + 1) * b.njc * b.nkc + (size_t)b.nic * (b.njc + 1) * b.nkc +
+                              (size_t)b.nic * b.njc * (b.nkc + 1);
+            HIPCHK(hipMalloc(&b.adj, sizeof(double) * 18 * (size_t)b.n));
+            HIPCHK(hipMalloc(&b.G, sizeof(double) * 24 * nf));
+            s->bdev[ib].adj = b.adj;
+            s->bdev[ib].G = b.G;
+        }
+    }
+    g.tape_C = std::max(g.tape_C, C);
+}
+
+// Points the block table's W at the tapes (tape = true) or back at the stepping ring.
+void point_w(chkn_sim* s, bool tape)
+{
+    for (size_t ib = 0; ib < s->blocks.size(); ++ib)
+        if (s->blocks[ib].active) s->bdev[ib].W = tape ? s->blocks[ib].tape : s->blocks[ib].W;
+    upload_blocks(s);
+}
+
+// K steps with the time steps dts[q*N + inst] from the handle's current state.  With
+// tape = true, step q reads tape slot 3q and writes slots 3q+1 .. 3q+3; otherwise the
+// stepping ring is used.  Returns the first stage's bad-cell count (0 if none).
+int fixed_dt_steps(chkn_sim* s, int K, const double* dts, bool tape)
+{
+    const int N = (int)s->inst.size();
+    for (int q = 0; q < K; ++q) {
+        double* dtv = ctl_dt(s->h_ctl, q, N);
+        int* run = ctl_run(s->h_ctl, q, N);
+        unsigned long long* sg = ctl_sig(s->h_ctl, q, N);
+        int* bad = ctl_bad(s->h_ctl, q, N);
+        for (int qi = 0; qi < N; ++qi) {
+            dtv[qi] = dts[(size_t)q * N + qi];
+            run[qi] = 1;
+            sg[qi] = ~0ull;
+        }
+        bad[0] = bad[1] = bad[2] = 0;
+    }
+    if (!tape) {
+        run_segment(s, K, false);
+    } else {
+        HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, s->ctl_bytes, hipMemcpyHostToDevice, s->stream));
+        for (int q = 0; q < K; ++q) {
+            StageArgs a = stage_args(s, q);
+            a.tile_base = 0;
+            a.ntiles = s->ntiles;
+            for (int stage = 1; stage <= 3; ++stage) {
+                a.win = 3 * q + stage - 1;
+                a.wout = 3 * q + stage;
+                HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, false,
+                                    s->opts.persistent != 0, s->grid_limit, a, s->stream));
+            }
+        }
+        HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, s->ctl_bytes, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    for (int q = 0; q < K; ++q) {
+        const int* bad = ctl_bad(s->h_ctl, q, N);
+        for (int st = 0; st < 3; ++st)
+            if (bad[st] > 0) return bad[st];
+    }
+    return 0;
+}
+
+// The unroll length is explicit: the instances' max_step and max_time do not stop it.
+struct LiftLimits {
+    chkn_sim* s;
+    std::vector<std::pair<int, double>> saved;
+    explicit LiftLimits(chkn_sim* sim) : s(sim)
+    {
+        for (auto& I : s->inst) {
+            saved.emplace_back(I.max_step, I.max_time);
+            I.max_step = 0x7fffffff;
+            I.max_time = HUGE_VAL;
+        }
+    }
+    ~LiftLimits()
+    {
+        for (size_t q = 0; q < s->inst.size(); ++q) {
+            s->inst[q].max_step = saved[q].first;
+            s->inst[q].max_time = saved[q].second;
+        }
+    }
+};
+
+int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt_out)
+{
+    if (s->failed) return set_error(s, CHKN_ERR_STATE, "simulation is in a failed state");
+    if (nsteps <= 0) return set_error(s, CHKN_ERR_ARG, "nsteps must be positive");
+    if (C <= 0 || C > kMaxSeg) return set_error(s, CHKN_ERR_ARG, "checkpoint interval must be in 1..64");
+    LiftLimits lift(s);
+    GradState& g = grad_state(s);
+    const int N = (int)s->inst.size();
+    const int nck = (nsteps + C - 1) / C;
+    alloc_checkpoints(s, nck + 1);  // + 1 slot to park the final state during the backward pass
+    g.valid = false;
+    g.C = C;
+    g.nsteps = nsteps;
+    g.dts.assign((size_t)nsteps * N, 0.0);
+    s->last_ms = 0.0;
+    s->last_launches = 0;
+    double ms_total = 0.0;
+    long long launches = 0;
+    for (int j = 0; j < nck; ++j) {
+        const int K = std::min(C, nsteps - j * C);
+        save_checkpoint(s, j);
+        if (dt_in) {
+            const double* dts = dt_in + (size_t)j * C * N;
+            const int bad = fixed_dt_steps(s, K, dts, false);
+            ms_total += s->last_ms;
+            launches += s->last_launches;
+            s->last_ms = 0.0;
+            s->last_launches = 0;
+            if (bad > 0) {
+                s->failed = true;
+                return set_error(s, bad, "bad cells in the gradient forward pass");
+            }
+            for (int qi = 0; qi < N; ++qi) {
+                Instance& I = s->inst[qi];
+                for (int q = 0; q < K; ++q) {
+                    I.dt = dts[(size_t)q * N + qi];
+                    I.t += I.dt;
+                }
+                I.step += K;
+                I.steps_since_last_plot += K;
+                I.have_sig = false;
+            }
+            s->gstep += K;
+            s->fresh = false;
+            std::copy(dts, dts + (size_t)K * N, g.dts.begin() + (size_t)j * C * N);
+        } else {
+            std::vector<double> log;
+            s->dt_log = &log;
+            const int rc = do_steps(s, K, false, true, nullptr);
+            s->dt_log = nullptr;
+            ms_total += s->last_ms;
+            launches += s->last_launches;
+            if (rc != CHKN_OK) return rc;
+            if ((int)log.size() != K * N) return set_error(s, CHKN_ERR_STATE, "an instance stopped during the unroll");
+            for (double v : log)
+                if (v == 0.0) return set_error(s, CHKN_ERR_STATE, "an instance stopped during the unroll");
+            std::copy(log.begin(), log.end(), g.dts.begin() + (size_t)j * C * N);
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->last_ms = ms_total;
+    s->last_launches = launches;
+    if (dt_out) std::copy(g.dts.begin(), g.dts.end(), dt_out);
+    g.valid = true;
+    return CHKN_OK;
+}
+
+size_t flat_size(chkn_sim* s)
+{
+    size_t n = 0;
+    for (auto& b : s->blocks) n += 6 * (size_t)b.n;
+    return n;
+}
+
+int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_inflow)
+{
+    GradState& g = grad_state(s);
+    if (!g.valid) return set_error(s, CHKN_ERR_STATE, "chkn_grad_backward needs a chkn_grad_forward first");
+    const int N = (int)s->inst.size();
+    const int C = g.C, nsteps = g.nsteps;
+    const int nck = (nsteps + C - 1) / C;
+    alloc_adjoint(s, C);
+    build_adj_tables(s);
+    hipEvent_t e0 = s->ev0, e1 = s->ev1;
+    HIPCHK(hipEventRecord(e0, s->stream));
+    // Park the final state; seed A0 with the final adjoint.
+    save_checkpoint(s, nck);
+    size_t off = 0;
+    for (auto& b : s->blocks) {
+        const size_t n6 = 6 * (size_t)b.n;
+        if (b.active)
+            HIPCHK(hipMemcpyAsync(b.adj, lam + off, sizeof(double) * n6, hipMemcpyDeviceToDevice, s->stream));
+        off += n6;
+    }
+    if (g.ninflow_cols > 0)
+        HIPCHK(hipMemsetAsync(g.d_inflow_acc, 0, sizeof(double) * 6 * g.ninflow_cols, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    point_w(s, true);
+    AdjArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.blocks = s->d_blocks;
+    A.inflow = s->d_inflow;
+    A.faces = (const FaceChunk*)g.d_face_chunks;
+    A.cells = (const CellChunk*)g.d_cell_chunks;
+    A.icols = (const InflowCol*)g.d_inflow_cols;
+    A.inflow_acc = g.d_inflow_acc;
+    A.nicols = g.ninflow_cols;
+    long long launches = 0;
+    int rc = CHKN_OK;
+    for (int j = nck - 1; j >= 0 && rc == CHKN_OK; --j) {
+        const int K = std::min(C, nsteps - j * C);
+        // Recompute the segment into the tape (its dt values land in control slots 0..K-1).
+        load_checkpoint(s, j, [](HostBlock& b) { return b.tape; });
+        const int bad = fixed_dt_steps(s, K, g.dts.data() + (size_t)j * C * N, true);
+        launches += 3LL * K;
+        if (bad > 0) {
+            rc = set_error(s, CHKN_ERR_STATE, "bad cells while recomputing a checkpoint segment");
+            break;
+        }
+        for (int q = K - 1; q >= 0; --q) {
+            A.dt = ctl_dt(s->d_ctl, q, N);
+            HIPCHK(launch_adj_seed(A, g.ncell_chunks, s->stream));
+            for (int stage = 3; stage >= 1; --stage) {
+                A.slot = 3 * q + stage - 1;
+                HIPCHK(launch_adj_faces(A, g.nface_chunks, s->x_order, s->stream));
+                HIPCHK(launch_adj_inflow(A, s->stream));
+                HIPCHK(launch_adj_gather(A, g.ncell_chunks, stage, s->stream));
+                launches += 3;
+            }
+            launches += 1;
+        }
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    // Restore the final state into the stepping ring.
+    point_w(s, false);
+    load_checkpoint(s, nck, [](HostBlock& b) { return b.W; });
+    if (rc == CHKN_OK) {
+        off = 0;
+        for (auto& b : s->blocks) {
+            const size_t n6 = 6 * (size_t)b.n;
+            if (b.active)
+                HIPCHK(hipMemcpyAsync(grad_u0 + off, b.adj, sizeof(double) * n6, hipMemcpyDeviceToDevice, s->stream));
+            else
+                HIPCHK(hipMemsetAsync(grad_u0 + off, 0, sizeof(double) * n6, s->stream));
+            off += n6;
+        }
+    }
+    HIPCHK(hipEventRecord(e1, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    s->last_ms = ms;
+    s->last_launches = launches;
+    if (rc != CHKN_OK) return rc;
+    if (grad_inflow) {
+        const int ns = (int)s->inflow_w.size() / 6;
+        std::fill(grad_inflow, grad_inflow + 6 * (size_t)ns, 0.0);
+        std::vector<double> acc(6 * (size_t)g.ninflow_cols);
+        if (g.ninflow_cols > 0)
+            HIPCHK(hipMemcpy(acc.data(), g.d_inflow_acc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost));
+        for (int c = 0; c < g.ninflow_cols; ++c)
+            for (int q = 0; q < 6; ++q)
+                grad_inflow[6 * (size_t)g.inflow_col_state[c] + q] += acc[(size_t)q * g.ninflow_cols + c];
+    }
+    return CHKN_OK;
+}
+
+}  // namespace
+
+namespace chkn {
+namespace rt {
+
+void free_grad(chkn_sim* s)
+{
+    for (auto& b : s->blocks) {
+        dfree(b.tape); dfree(b.ckpt); dfree(b.adj); dfree(b.G);
+        b.tape = b.ckpt = b.adj = b.G = nullptr;
+    }
+    if (s->grad) {
+        dfree(s->grad->d_face_chunks); dfree(s->grad->d_cell_chunks);
+        dfree(s->grad->d_inflow_cols); dfree(s->grad->d_inflow_acc);
+        s->grad.reset();
+    }
+}
+
+}  // namespace rt
+}  // namespace chkn
+
+extern "C" {
+
+int chkn_grad_forward(chkn_sim* sim, int nsteps, int checkpoint_every, const double* dt_in,
+                      double* dt_out)
+{
+    if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
+    return guarded(sim, [&] { return grad_forward(sim, nsteps, checkpoint_every, dt_in, dt_out); });
+}
+
+int chkn_grad_backward(chkn_sim* sim, const double* lambda_dev, double* grad_u0_dev,
+                       double* grad_inflow)
+{
+    if (!sim || !lambda_dev || !grad_u0_dev) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] { return grad_backward(sim, lambda_dev, grad_u0_dev, grad_inflow); });
+}
+
+long long chkn_state_size(chkn_sim* sim) { return sim ? (long long)flat_size(sim) : -1; }
+
+int chkn_num_flow_states(chkn_sim* sim) { return sim ? (int)sim->inflow_w.size() / 6 : -1; }
+
+int chkn_read_conserved_dev(chkn_sim* sim, double* dst_dev)
+{
+    if (!sim || !dst_dev) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] {
+        size_t off = 0;
+        for (auto& b : sim->blocks) {
+            const size_t n6 = 6 * (size_t)b.n;
+            if (b.active)
+                HIPCHK(hipMemcpyAsync(dst_dev + off, b.U, sizeof(double) * n6, hipMemcpyDeviceToDevice, sim->stream));
+            else
+                HIPCHK(hipMemsetAsync(dst_dev + off, 0, sizeof(double) * n6, sim->stream));
+            off += n6;
+        }
+        HIPCHK(hipStreamSynchronize(sim->stream));
+        return CHKN_OK;
+    });
+}
+
+int chkn_write_conserved_dev(chkn_sim* sim, const double* src_dev)
+{
+    if (!sim || !src_dev) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] {
+        GradState& g = grad_state(sim);
+        build_adj_tables(sim);
+        size_t off = 0;
+        for (auto& b : sim->blocks) {
+            const size_t n6 = 6 * (size_t)b.n;
+            if (b.active)
+                HIPCHK(hipMemcpyAsync(b.U, src_dev + off, sizeof(double) * n6, hipMemcpyDeviceToDevice, sim->stream));
+            off += n6;
+        }
+        HIPCHK(launch_decode_all(sim->d_blocks, (const CellChunk*)g.d_cell_chunks, g.ncell_chunks, sim->stream));
+        HIPCHK(hipStreamSynchronize(sim->stream));
+        sim->fresh = false;
+        sim->failed = false;
+        for (auto& I : sim->inst) I.have_sig = false;
+        g.valid = false;
+        return CHKN_OK;
+    });
+}
+
+int chkn_set_inflow_state(chkn_sim* sim, int state, const double* w6)
+{
+    if (!sim || !w6 || state < 0 || state >= (int)sim->inflow_w.size() / 6)
+        return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] {
+        std::copy(w6, w6 + 6, sim->inflow_w.begin() + 6 * (size_t)state);
+        HIPCHK(hipMemcpy(sim->d_inflow + 6 * (size_t)state, w6, sizeof(double) * 6, hipMemcpyHostToDevice));
+        if (sim->grad) sim->grad->valid = false;
+        return CHKN_OK;
+    });
+}
+
+int chkn_get_inflow_state(chkn_sim* sim, int state, double* w6)
+{
+    if (!sim || !w6 || state < 0 || state >= (int)sim->inflow_w.size() / 6)
+        return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    std::copy(sim->inflow_w.begin() + 6 * (size_t)state, sim->inflow_w.begin() + 6 * (size_t)state + 6, w6);
+    return CHKN_OK;
+}
+
+int chkn_boundary_face_areas(chkn_sim* sim, int blk, int side, double* out)
+{
+    if (!sim || !out || blk < 0 || blk >= (int)sim->blocks.size() || side < 0 || side > 5)
+        return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] {
+        HostBlock& b = sim->blocks[blk];
+        if (!b.active) return set_error(sim, CHKN_ERR_ARG, "inactive block");
+        const int d = side >> 1, s = side & 1;
+        const int nic = b.nic, njc = b.njc, nkc = b.nkc;
+        const double* g = d == 0 ? b.gi : (d == 1 ? b.gj : b.gk);
+        const size_t nf = d == 0 ? (size_t)(nic + 1) * njc * nkc
+                                 : (d == 1 ? (size_t)nic * (njc + 1) * nkc : (size_t)nic * njc * (nkc + 1));
+        std::vector<double> area(nf);
+        HIPCHK(hipMemcpy(area.data(), g, sizeof(double) * nf, hipMemcpyDeviceToHost));
+        size_t o = 0;
+        if (d == 0) {
+            for (int k = 0; k < nkc; ++k)
+                for (int j = 0; j < njc; ++j) out[o++] = area[(s ? nic : 0) + (size_t)(nic + 1) * (j + (size_t)njc * k)];
+        } else if (d == 1) {
+            for (int k = 0; k < nkc; ++k)
+                for (int i = 0; i < nic; ++i) out[o++] = area[i + (size_t)nic * ((s ? njc : 0) + (size_t)(njc + 1) * k)];
+        } else {
+            for (int j = 0; j < njc; ++j)
+                for (int i = 0; i < nic; ++i) out[o++] = area[i + (size_t)nic * (j + (size_t)njc * (s ? nkc : 0))];
+        }
+        return CHKN_OK;
+    });
+}
+
+int chkn_flux_vjp_points(int npts, int x_order, const double* in, double* out)
+{
+    if (npts <= 0 || !in || !out) return set_error(nullptr, CHKN_ERR_ARG, "bad argument");
+    double *din = nullptr, *dout = nullptr;
+    try {
+        HIPCHK(hipMalloc(&din, sizeof(double) * 39 * (size_t)npts));
+        HIPCHK(hipMalloc(&dout, sizeof(double) * 24 * (size_t)npts));
+        HIPCHK(hipMemcpy(din, in, sizeof(double) * 39 * (size_t)npts, hipMemcpyHostToDevice));
+        HIPCHK(launch_flux_vjp_points(din, dout, npts, x_order, nullptr));
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(out, dout, sizeof(double) * 24 * (size_t)npts, hipMemcpyDeviceToHost));
+    } catch (const HipError& e) {
+        dfree(din);
+        dfree(dout);
+        return set_error(nullptr, CHKN_ERR_HIP, e.what());
+    }
+    dfree(din);
+    dfree(dout);
+    return CHKN_OK;
+}
+
+}  // extern "C"

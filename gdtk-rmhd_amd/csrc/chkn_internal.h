@@ -2,7 +2,9 @@
 // runtime (chkn_sim.cpp) and the gfx950 kernels (chkn_kernels.hip).
 //
 // HBM layout (per active block of n = nic*njc*nkc cells, structure of arrays, FP64):
-//   W[3][6][n]   primitive stage states (rho, e, YB, vx, vy, vz), a 3-buffer ring
+//   W[slots][6][n] primitive stage states (rho, e, YB, vx, vy, vz): a 3-slot ring while
+//                stepping, or the 3C+1 stage states of a C-step segment while recording a
+//                tape for the adjoint; slot r starts at W + r*wstride
 //   U[6][n]      conserved quantities at the start of the step (updated in place, stage 3)
 //   L[6][n]      stage-1 residual dU/dt, then (L0/6 + L1/6) after stage 2
 //   face geometry per direction, 10 components x nfaces: area, n(3), t1(3), t2(3)
@@ -10,8 +12,8 @@
 //     j-faces: nic x (njc+1) x nkc, index i + nic*(j + (njc+1)*k)
 //     k-faces: nic x njc x (nkc+1), index i + nic*(j + njc*k)
 //   volinv[n] = 1/volume, lengths[3][n] (iLength, jLength, kLength)
-// Inactive blocks hold only W (all three buffers equal to the file state) and geometry
-// needed by exchange neighbours.
+// Inactive blocks hold one W slot (wstride 0: every slot is the file state) and the
+// geometry needed by exchange neighbours.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,7 +31,8 @@ struct BlockDev {
     int bc_arg[6];   // inflow flow-state index, or neighbour block id for exchange
     long long n;     // cells
     long long nfi, nfj, nfk;
-    double* W[3];
+    double* W;             // slot 0 of the primitive stage states
+    long long wstride;     // doubles between slots: 6n (active) or 0 (inactive)
     double* U;
     double* L;
     const double* gi;
@@ -37,6 +40,10 @@ struct BlockDev {
     const double* gk;
     const double* volinv;
     const double* len;
+    // adjoint
+    int rp[6];             // block whose opposite side exchanges into this side, or -1
+    double* G;             // face stencil adjoints: [24][nfi], then [24][nfj], then [24][nfk]
+    double* adj;           // A0 (adjoint of U at the step boundary), P, D: [3][6][n]
 };
 
 struct Tile {
@@ -62,8 +69,46 @@ struct StageArgs {
     int tile_base;         // offset into the tile table (per-block launches)
     const double* inflow;  // 6 doubles per flow state: rho, e, YB, vx, vy, vz
     StepCtl ctl;
-    int win, wout;         // W ring indices read / written by this stage
+    int win, wout;         // W slots read / written by this stage
 };
+
+// Work items of the adjoint kernels: 256 faces of one direction of one block, 256 cells of
+// one block, one boundary column (line of ghost cells) of an inflow boundary.
+struct FaceChunk {
+    int blk, dir, f0;
+};
+struct CellChunk {
+    int blk, c0;
+};
+struct InflowCol {
+    int blk, side, t;  // t: transverse index a + na*c of the line
+};
+
+struct AdjArgs {
+    const BlockDev* blocks;   // W must point at the tape: slot = stage state being reversed
+    const double* inflow;
+    const FaceChunk* faces;
+    const CellChunk* cells;
+    const InflowCol* icols;
+    double* inflow_acc;       // [6][nicols]
+    int nicols;
+    int slot;                 // W slot of the stage state W_s
+    const double* dt;         // [ninst] time step of the step being reversed
+};
+
+// Adjoint of one stage's residual: face pass (stencil adjoints of every face into G), then
+// the inflow columns, then the per-cell gather with the stage's RK combination (stage 3, 2,
+// 1 in reverse order).  seed: P = dt/6 A0, D = dt 4/6 A0 (start of a reversed step).
+hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s);
+hipError_t launch_adj_faces(const AdjArgs& a, int nface_chunks, int x_order, hipStream_t s);
+hipError_t launch_adj_inflow(const AdjArgs& a, hipStream_t s);
+hipError_t launch_adj_gather(const AdjArgs& a, int ncell_chunks, int stage, hipStream_t s);
+// U and W = decode(U) of every active block from U (after the conserved state was replaced).
+hipError_t launch_decode_all(const BlockDev* blocks, const CellChunk* cells, int ncell_chunks,
+                             hipStream_t s);
+// Point evaluation of face_flux_vjp (test hook): in = per point 24 prims, 9 frame, 6 Fbar;
+// out = 24 stencil adjoints.
+hipError_t launch_flux_vjp_points(const double* in, double* out, int n, int x_order, hipStream_t s);
 
 // Launch one stage over tiles [tile_base, tile_base + ntiles) of the table.
 // stage in 1..3; x_order 1 or 2; tile shape (ti, tj) must be a built variant.

@@ -1,0 +1,177 @@
+// chkn_runtime.h -- host-side state of a libchkn handle, shared by the stepping runtime
+// (chkn_sim.cpp) and the adjoint runtime (chkn_grad.cpp).  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/chkn.h"
+#include "chkn_internal.h"
+
+namespace chkn {
+namespace rt {
+
+constexpr int kMaxSeg = 64;  // steps per segment (device control slots)
+
+// Error text of the last failed create/load on this thread.
+extern thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw HipError(std::string("HIP error ") + hipGetErrorString(e_) + " at " #expr);   \
+    } while (0)
+
+struct HostBlock {
+    int inst = 0;
+    int bi, bj, bk, nic, njc, nkc;
+    bool active;
+    int bc[6], bc_fs[6];
+    int n;
+    std::vector<double> io0;  // IO variables as given at start (flow part used while fresh)
+    std::vector<double> pos;  // 3*n computed centroids
+    std::vector<double> vol;  // n computed volumes
+    // device
+    double* W = nullptr;  // 3 buffers (active) or 1 (inactive)
+    double* U = nullptr;
+    double* L = nullptr;
+    double* gi = nullptr;
+    double* gj = nullptr;
+    double* gk = nullptr;
+    double* volinv = nullptr;
+    double* len = nullptr;
+    int tile_first = 0, tile_count = 0;
+    // adjoint (allocated by the first gradient call)
+    double* tape = nullptr;    // (3C+1) W slots of the segment being reversed
+    double* ckpt = nullptr;    // checkpoints: U then W, 12n doubles each
+    double* adj = nullptr;     // A0, P, D: 18n doubles
+    double* G = nullptr;       // face stencil adjoints: 24 x (nfi + nfj + nfk)
+};
+
+// Time control of one instance (SimState of simulate.cu:32-39 plus its schedules).
+struct Instance {
+    int first_block = 0, nblocks = 0;
+    int nib = 0, njb = 0, nkb = 0;
+    std::vector<int> blk_ids;  // global block ids
+    int cfl_count = 10, max_step = 100, print_count = 20;
+    double max_time = 1e-3, dt_init = 1e-6;
+    std::vector<double> cfl_t, cfl_v, plot_t, plot_v;
+    std::string job;
+    double t = 0.0, dt = 0.0, t_plot = 0.0, sigmin = 0.0;
+    int step = 0, next_plot_indx = 1, steps_since_last_plot = 0;
+    bool have_sig = false;
+    bool running() const { return step < max_step && t < max_time; }
+};
+
+// Tape of the last chkn_grad_forward call: checkpoints every C steps, the dt of every step.
+struct GradState {
+    int C = 0;                    // checkpoint interval (steps)
+    int nsteps = 0;               // steps recorded
+    int nckpt_alloc = 0;          // checkpoint slots allocated per block
+    int tape_C = 0;               // segment length the tapes are sized for
+    std::vector<double> dts;      // [nsteps][ninst]
+    std::vector<double> t0;       // time of each instance at the start
+    int step0 = 0, gstep0 = 0;    // counters at the start
+    bool valid = false;
+    // face/cell work tables and inflow patches (device)
+    int nface_chunks = 0, ncell_chunks = 0, ninflow_cols = 0;
+    void* d_face_chunks = nullptr;
+    void* d_cell_chunks = nullptr;
+    void* d_inflow_cols = nullptr;
+    double* d_inflow_acc = nullptr;  // 6 x ninflow_cols
+    std::vector<int> inflow_col_state;  // flow-state index of each inflow column
+};
+
+struct GraphKey {
+    int K;
+    bool dtred;
+    bool operator<(const GraphKey& o) const { return K != o.K ? K < o.K : dtred < o.dtred; }
+};
+
+}  // namespace rt
+}  // namespace chkn
+
+struct chkn_sim {
+    using HostBlock = chkn::rt::HostBlock;
+    using Instance = chkn::rt::Instance;
+    using GraphKey = chkn::rt::GraphKey;
+    using BlockDev = chkn::BlockDev;
+    using Tile = chkn::Tile;
+    // configuration
+    std::vector<HostBlock> blocks;
+    std::vector<Instance> inst;
+    std::vector<double> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz (all instances)
+    int x_order = 2, flux_calc = 0;
+    int gstep = 0;       // lockstep step counter shared by the instances
+    bool fresh = true;   // no step taken since the state was set from IO variables
+    bool failed = false;
+    // device
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    BlockDev* d_blocks = nullptr;
+    Tile* d_tiles = nullptr;
+    double* d_inflow = nullptr;
+    // Control area, kMaxSeg step slots: dt[S][N], sigmin[S][N], run[S][N], bad[S][3].
+    char* d_ctl = nullptr;
+    char* h_ctl = nullptr;
+    size_t ctl_bytes = 0;
+    int ntiles = 0;
+    // launch options
+    chkn_launch_opts opts{32, 8, 0, 0, 0, 0, 0};
+    int kc = 8;
+    int grid_limit = 0;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    // timing of the last chkn_step call
+    double last_ms = 0.0;
+    long long last_launches = 0;
+    std::string err;
+    // host copy of the device block table (the adjoint re-points W at its tape)
+    std::vector<chkn::BlockDev> bdev;
+    std::unique_ptr<chkn::rt::GradState> grad;
+    std::vector<double>* dt_log = nullptr;  // when set, do_steps appends [step][inst] dts
+};
+
+
+namespace chkn {
+namespace rt {
+
+int set_error(chkn_sim* s, int code, const std::string& msg);
+void dfree(void* p);
+// Views of the control area for step slot q (N instances).
+double* ctl_dt(char* base, int q, int N);
+unsigned long long* ctl_sig(char* base, int q, int N);
+int* ctl_run(char* base, int q, int N);
+int* ctl_bad(char* base, int q, int N);
+StageArgs stage_args(chkn_sim* s, int q);
+// Runs K steps whose dt and run flags are already in the host control area.
+void run_segment(chkn_sim* s, int K, bool dtred_last);
+int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report* rep);
+void upload_blocks(chkn_sim* s);
+void free_grad(chkn_sim* s);
+
+template <typename F>
+int guarded(chkn_sim* s, F&& fn)
+{
+    try {
+        return fn();
+    } catch (const HipError& e) {
+        return set_error(s, CHKN_ERR_HIP, e.what());
+    } catch (const std::invalid_argument& e) {
+        return set_error(s, CHKN_ERR_ARG, e.what());
+    } catch (const std::exception& e) {
+        return set_error(s, CHKN_ERR_IO, e.what());
+    }
+}
+
+}  // namespace rt
+}  // namespace chkn

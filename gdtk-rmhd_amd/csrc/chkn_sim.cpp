@@ -36,99 +36,12 @@
 
 using namespace chkn;
 
-namespace {
+#include "chkn_runtime.h"
 
-constexpr int kMaxSeg = 64;  // steps per segment (device control slots)
+namespace chkn {
+namespace rt {
 
 thread_local std::string g_create_error;
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-#define HIPCHK(expr)                                                                            \
-    do {                                                                                        \
-        hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess)                                                                   \
-            throw HipError(std::string("HIP error ") + hipGetErrorString(e_) + " at " #expr);   \
-    } while (0)
-
-struct HostBlock {
-    int inst = 0;
-    int bi, bj, bk, nic, njc, nkc;
-    bool active;
-    int bc[6], bc_fs[6];
-    int n;
-    std::vector<double> io0;  // IO variables as given at start (flow part used while fresh)
-    std::vector<double> pos;  // 3*n computed centroids
-    std::vector<double> vol;  // n computed volumes
-    // device
-    double* W = nullptr;  // 3 buffers (active) or 1 (inactive)
-    double* U = nullptr;
-    double* L = nullptr;
-    double* gi = nullptr;
-    double* gj = nullptr;
-    double* gk = nullptr;
-    double* volinv = nullptr;
-    double* len = nullptr;
-    int tile_first = 0, tile_count = 0;
-};
-
-// Time control of one instance (SimState of simulate.cu:32-39 plus its schedules).
-struct Instance {
-    int first_block = 0, nblocks = 0;
-    int nib = 0, njb = 0, nkb = 0;
-    std::vector<int> blk_ids;  // global block ids
-    int cfl_count = 10, max_step = 100, print_count = 20;
-    double max_time = 1e-3, dt_init = 1e-6;
-    std::vector<double> cfl_t, cfl_v, plot_t, plot_v;
-    std::string job;
-    double t = 0.0, dt = 0.0, t_plot = 0.0, sigmin = 0.0;
-    int step = 0, next_plot_indx = 1, steps_since_last_plot = 0;
-    bool have_sig = false;
-    bool running() const { return step < max_step && t < max_time; }
-};
-
-struct GraphKey {
-    int K;
-    bool dtred;
-    bool operator<(const GraphKey& o) const { return K != o.K ? K < o.K : dtred < o.dtred; }
-};
-
-}  // namespace
-
-struct chkn_sim {
-    // configuration
-    std::vector<HostBlock> blocks;
-    std::vector<Instance> inst;
-    std::vector<double> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz (all instances)
-    int x_order = 2, flux_calc = 0;
-    int gstep = 0;       // lockstep step counter shared by the instances
-    bool fresh = true;   // no step taken since the state was set from IO variables
-    bool failed = false;
-    // device
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    BlockDev* d_blocks = nullptr;
-    Tile* d_tiles = nullptr;
-    double* d_inflow = nullptr;
-    // Control area, kMaxSeg step slots: dt[S][N], sigmin[S][N], run[S][N], bad[S][3].
-    char* d_ctl = nullptr;
-    char* h_ctl = nullptr;
-    size_t ctl_bytes = 0;
-    int ntiles = 0;
-    // launch options
-    chkn_launch_opts opts{32, 8, 0, 0, 0, 0};
-    int kc = 8;
-    int grid_limit = 0;
-    std::map<GraphKey, hipGraphExec_t> graphs;
-    // timing of the last chkn_step call
-    double last_ms = 0.0;
-    long long last_launches = 0;
-    std::string err;
-};
-
-namespace {
 
 int set_error(chkn_sim* s, int code, const std::string& msg)
 {
@@ -152,6 +65,7 @@ void dfree(void* p)
 
 void free_device(chkn_sim* s)
 {
+    free_grad(s);
     for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
     s->graphs.clear();
     for (auto& b : s->blocks) {
@@ -166,6 +80,12 @@ void free_device(chkn_sim* s)
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->d_blocks = nullptr; s->d_tiles = nullptr; s->d_inflow = nullptr; s->d_ctl = nullptr;
     s->h_ctl = nullptr; s->ev0 = s->ev1 = nullptr; s->stream = nullptr;
+}
+
+void upload_blocks(chkn_sim* s)
+{
+    HIPCHK(hipMemcpy(s->d_blocks, s->bdev.data(), sizeof(BlockDev) * s->bdev.size(),
+                     hipMemcpyHostToDevice));
 }
 
 // Geometry pre-pass of one block (block.cu:633-652) into device-layout host arrays.
@@ -285,10 +205,17 @@ void build_tiles(chkn_sim* s)
                 long long want = std::max(1LL, cols / 2048);
                 kc = (int)std::min<long long>(std::max<long long>(want, 2), 32);
             }
-            for (int k0 = 0; k0 < b.nkc; k0 += kc)
+            if (s->opts.tile_order == 1) {
                 for (int tj = 0; tj < ntj; ++tj)
                     for (int ti = 0; ti < nti; ++ti)
-                        tiles.push_back(Tile{(int)ib, ti * TI, tj * TJ, k0, std::min(kc, b.nkc - k0)});
+                        for (int k0 = 0; k0 < b.nkc; k0 += kc)
+                            tiles.push_back(Tile{(int)ib, ti * TI, tj * TJ, k0, std::min(kc, b.nkc - k0)});
+            } else {
+                for (int k0 = 0; k0 < b.nkc; k0 += kc)
+                    for (int tj = 0; tj < ntj; ++tj)
+                        for (int ti = 0; ti < nti; ++ti)
+                            tiles.push_back(Tile{(int)ib, ti * TI, tj * TJ, k0, std::min(kc, b.nkc - k0)});
+            }
         }
         b.tile_count = (int)tiles.size() - b.tile_first;
     }
@@ -433,6 +360,13 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             partner[ib][f] = id;
         }
     }
+    // Reverse partners: the block whose side 2d+1 (2d) exchanges into side 2d (2d+1).
+    std::vector<std::array<int, 6>> rpart(s->blocks.size());
+    for (auto& r : rpart) r.fill(-1);
+    for (size_t ib = 0; ib < s->blocks.size(); ++ib)
+        for (int f = 0; f < 6; ++f)
+            if (partner[ib][f] >= 0 && s->blocks[partner[ib][f]].active)
+                rpart[partner[ib][f]][f ^ 1] = (int)ib;
     // Device.
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&s->ev0));
@@ -457,7 +391,8 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             bd.nfi = (long long)(b.nic + 1) * b.njc * b.nkc;
             bd.nfj = (long long)b.nic * (b.njc + 1) * b.nkc;
             bd.nfk = (long long)b.nic * b.njc * (b.nkc + 1);
-            for (int r = 0; r < 3; ++r) bd.W[r] = b.W + (b.active ? (size_t)r * 6 * n : 0);
+            bd.W = b.W;
+            bd.wstride = b.active ? (long long)(6 * n) : 0;
             if (b.active) {
                 HIPCHK(hipMalloc(&b.U, sz6));
                 HIPCHK(hipMalloc(&b.L, sz6));
@@ -476,14 +411,16 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
                 bd.volinv = b.volinv; bd.len = b.len;
             }
             for (int f = 0; f < 6; ++f) {
+                bd.rp[f] = rpart[ib][f];
                 bd.bc[f] = b.bc[f];
                 bd.bc_arg[f] = (b.bc[f] == CHKN_BC_EXCHANGE) ? partner[ib][f] : b.bc_fs[f];
             }
             upload_state(s, b, b.io0.data());
         }
     }
+    s->bdev = bdev;
     HIPCHK(hipMalloc(&s->d_blocks, sizeof(BlockDev) * bdev.size()));
-    HIPCHK(hipMemcpy(s->d_blocks, bdev.data(), sizeof(BlockDev) * bdev.size(), hipMemcpyHostToDevice));
+    upload_blocks(s);
     HIPCHK(hipMalloc(&s->d_inflow, sizeof(double) * s->inflow_w.size()));
     HIPCHK(hipMemcpy(s->d_inflow, s->inflow_w.data(), sizeof(double) * s->inflow_w.size(), hipMemcpyHostToDevice));
     const int N = ninst;
@@ -727,6 +664,9 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
                 s->inst[qi].have_sig = true;
             }
         }
+        if (s->dt_log)
+            for (int q = 0; q < K; ++q)
+                for (int qi = 0; qi < N; ++qi) s->dt_log->push_back(q < nrun[qi] ? s->inst[qi].dt : 0.0);
         for (int qi = 0; qi < N; ++qi) {  // simulate.cu:306-308
             Instance& I = s->inst[qi];
             for (int q = 0; q < nrun[qi]; ++q) {
@@ -793,20 +733,6 @@ int write_instance_flow(chkn_sim* s, int qi, int tindx, bool binary)
     return CHKN_OK;
 }
 
-template <typename F>
-int guarded(chkn_sim* s, F&& fn)
-{
-    try {
-        return fn();
-    } catch (const HipError& e) {
-        return set_error(s, CHKN_ERR_HIP, e.what());
-    } catch (const std::invalid_argument& e) {
-        return set_error(s, CHKN_ERR_ARG, e.what());
-    } catch (const std::exception& e) {
-        return set_error(s, CHKN_ERR_IO, e.what());
-    }
-}
-
 int create_from(const chkn_desc* descs, int n, chkn_sim** out)
 {
     if (!descs || !out || n <= 0) return set_error(nullptr, CHKN_ERR_ARG, "null argument");
@@ -828,7 +754,10 @@ int create_from(const chkn_desc* descs, int n, chkn_sim** out)
     return CHKN_OK;
 }
 
-}  // namespace
+}  // namespace rt
+}  // namespace chkn
+
+using namespace chkn::rt;
 
 extern "C" {
 
@@ -973,6 +902,7 @@ int chkn_set_launch_opts(chkn_sim* sim, const chkn_launch_opts* o)
     if (!tile_shape_supported(n.tile_i, n.tile_j))
         return set_error(sim, CHKN_ERR_ARG, "unsupported tile shape");
     if (n.tile_k < 0) return set_error(sim, CHKN_ERR_ARG, "tile_k must be >= 0");
+    if (n.tile_order != 0 && n.tile_order != 1) return set_error(sim, CHKN_ERR_ARG, "tile_order must be 0 or 1");
     return guarded(sim, [&] {
         sim->opts = n;
         build_tiles(sim);

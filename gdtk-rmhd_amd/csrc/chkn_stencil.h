@@ -1,0 +1,96 @@
+// chkn_stencil.h -- device-side access to block data shared by the stage kernels and the
+// adjoint kernels: SoA loads of primitive states and face frames, line indexing, and the
+// ghost-cell states of the boundary conditions.
+#pragma once
+
+#include "chkn_internal.h"
+#include "chkn_physics.h"
+
+namespace chkn {
+
+__device__ __forceinline__ Prim ld_prim(const double* __restrict__ W, int n, int c)
+{
+    Prim p;
+    p.rho = W[c];
+    p.e = W[n + c];
+    p.YB = W[2 * n + c];
+    p.vx = W[3 * n + c];
+    p.vy = W[4 * n + c];
+    p.vz = W[5 * n + c];
+    return p;
+}
+
+__device__ __forceinline__ Frame ld_frame(const double* __restrict__ g, int nf, int f)
+{
+    Frame r;
+    r.nx = g[1 * nf + f];
+    r.ny = g[2 * nf + f];
+    r.nz = g[3 * nf + f];
+    r.t1x = g[4 * nf + f];
+    r.t1y = g[5 * nf + f];
+    r.t1z = g[6 * nf + f];
+    r.t2x = g[7 * nf + f];
+    r.t2y = g[8 * nf + f];
+    r.t2z = g[9 * nf + f];
+    return r;
+}
+
+// Linear cell index of line position m in direction dir with transverse indices (a, c):
+// dir 0 -> (m, a, c), dir 1 -> (a, m, c), dir 2 -> (a, c, m).
+__device__ __forceinline__ int cell_index(int nic, int njc, int dir, int m, int a, int c)
+{
+    if (dir == 0) return m + nic * (a + njc * c);
+    if (dir == 1) return a + nic * (m + njc * c);
+    return a + nic * (c + njc * m);
+}
+
+// Boundary face (side 0 = minus, 1 = plus) of the line through (a, c) in direction dir.
+__device__ __forceinline__ void boundary_face(const BlockDev* B, int dir, int side, int a, int c,
+                                              const double*& g, int& nf, int& fi)
+{
+    if (dir == 0) {
+        g = B->gi; nf = (int)B->nfi;
+        fi = (side ? B->nic : 0) + (B->nic + 1) * (a + B->njc * c);
+    } else if (dir == 1) {
+        g = B->gj; nf = (int)B->nfj;
+        fi = a + B->nic * ((side ? B->njc : 0) + (B->njc + 1) * c);
+    } else {
+        g = B->gk; nf = (int)B->nfk;
+        fi = a + B->nic * (c + B->njc * (side ? B->nkc : 0));
+    }
+}
+
+// Ghost-cell state for a stencil position outside the block (m < 0 or m >= n).  Inlined:
+// an out-of-line call would make every caller preserve its live registers across the
+// call ABI, which costs about 30 VGPRs and one wave per SIMD of occupancy.
+// Depth d = -m-1 on the minus side and m-n on the plus side (block.cu:272-316).
+__device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const double* inflow, int win,
+                                           const BlockDev* B, int dir, int m, int a, int c)
+{
+    int n = dir == 0 ? B->nic : (dir == 1 ? B->njc : B->nkc);
+    int side = (m < 0) ? 0 : 1;
+    int depth = side ? m - n : -m - 1;
+    int bf = 2 * dir + side;
+    int code = B->bc[bf];
+    if (code == 3) {  // exchange: neighbour's interior cell at the same depth (bcs.cu:135-254)
+        const BlockDev* O = blocks + B->bc_arg[bf];
+        int no = dir == 0 ? O->nic : (dir == 1 ? O->njc : O->nkc);
+        int mo = side ? depth : no - 1 - depth;
+        return ld_prim(O->W + win * O->wstride, (int)O->n, cell_index(O->nic, O->njc, dir, mo, a, c));
+    }
+    if (code == 4) {  // inflow: the configured flow state (bcs.cu:257-319)
+        const double* s = inflow + 6 * B->bc_arg[bf];
+        Prim p;
+        p.rho = s[0]; p.e = s[1]; p.YB = s[2]; p.vx = s[3]; p.vy = s[4]; p.vz = s[5];
+        return p;
+    }
+    Prim in = ld_prim(B->W + win * B->wstride, (int)B->n,
+                      cell_index(B->nic, B->njc, dir, side ? n - 1 : 0, a, c));
+    if (code == 5) return in;  // outflow copies the adjacent interior cell (bcs.cu:435-503)
+    // Walls: both ghost layers reflect the adjacent interior cell (bcs.cu:16-132).
+    const double* g; int nf, fi;
+    boundary_face(B, dir, side, a, c, g, nf, fi);
+    return reflect(in, ld_frame(g, nf, fi));
+}
+
+}  // namespace chkn

@@ -89,6 +89,8 @@ typedef struct {
                              structure, simulate.cu:605-625); 0 = one launch per stage */
     int use_graph;        /* 1 = replay steps from a captured hipGraph */
     int persistent;       /* 1 = persistent work-groups loop over the tile table */
+    int tile_order;       /* 0 = k-chunk layers outermost (plane-major tile table);
+                             1 = the k-chunks of one (i, j) column consecutive */
 } chkn_launch_opts;
 
 /* Replaces initialize_simulation (simulate.cu:55-155), array form. */
@@ -146,6 +148,44 @@ long long chkn_active_cells(chkn_sim* sim);
 /* Device time of the last chkn_step call's kernels, measured with HIP events on the
  * handle's stream (ms), and the number of stage launches it issued. */
 int chkn_last_timing(chkn_sim* sim, double* ms, long long* launches);
+
+/* ---- Gradients of an unrolled march (no counterpart in the reference, which has no
+ * adjoint; these serve differentiable pipelines built on the step).  Flat state arrays
+ * concatenate every block's 6*ncells conserved quantities (quantity-major per block, block
+ * order; inactive blocks hold zeros) and live in device memory. */
+
+/* Doubles in a flat state array. */
+long long chkn_state_size(chkn_sim* sim);
+/* Copy the conserved state of every block into / out of a flat device array.  Writing sets
+ * each active block's primitive state to decode(U) (flow.cu:245-268). */
+int chkn_read_conserved_dev(chkn_sim* sim, double* dst_dev);
+int chkn_write_conserved_dev(chkn_sim* sim, const double* src_dev);
+/* Number of inflow flow states over all instances (state indices are global: instance q's
+ * states follow those of instances 0..q-1). */
+int chkn_num_flow_states(chkn_sim* sim);
+/* Inflow flow state `state` (global index over instances) as (rho, e, YB, vx, vy, vz); the
+ * reference derives rho and e from p and T (config.cu:398-406, gas.cu:119-126). */
+int chkn_set_inflow_state(chkn_sim* sim, int state, const double* w6);
+int chkn_get_inflow_state(chkn_sim* sim, int state, double* w6);
+/* Areas of a block's boundary faces on side 0..5 (iminus .. kplus), ordered with the
+ * first transverse index fastest (j,k for i-sides; i,k for j-sides; i,j for k-sides). */
+int chkn_boundary_face_areas(chkn_sim* sim, int blk, int side, double* out);
+/* Advance nsteps steps keeping a checkpoint every checkpoint_every (1..64) steps (max_step
+ * and max_time do not stop it).  The time steps follow the CFL schedule (dt_in NULL) or are
+ * given, dt_in[step*ninst + inst]; those used are written to dt_out when it is not NULL. */
+int chkn_grad_forward(chkn_sim* sim, int nsteps, int checkpoint_every, const double* dt_in,
+                      double* dt_out);
+/* Reverse the last chkn_grad_forward with its time steps held fixed: lambda_dev = dLoss/dU
+ * of the final state (flat, device); grad_u0_dev receives dLoss/dU of the state the forward
+ * started from (flat, device); grad_inflow (host, 6 per inflow state, may be NULL)
+ * receives dLoss/d(rho, e, YB, vx, vy, vz) of every inflow state.  The handle's state is
+ * left at the final state. */
+int chkn_grad_backward(chkn_sim* sim, const double* lambda_dev, double* grad_u0_dev,
+                       double* grad_inflow);
+/* Test hook: vector-Jacobian products of the face flux at npts points on the device.
+ * in: per point 4 x 6 stencil primitives (L1, L0, R0, R1; rho, e, YB, vx, vy, vz), the 9
+ * frame components (n, t1, t2) and 6 flux adjoints; out: 24 stencil adjoints per point. */
+int chkn_flux_vjp_points(int npts, int x_order, const double* in, double* out);
 
 /* Error text of the last failing call on this handle (or of the last failed create/load
  * when sim is NULL). */

@@ -87,6 +87,12 @@ void orc_quad_props(const double* p, double* out);
 int orc_decode(const double* U, double* W);
 void orc_encode(const double* W, double* U);
 
+/* Complex-step face flux (chkn_oracle_cs.c).  prims: 4 states (L1, L0, R0, R1) of
+ * (rho, e, YB, vx, vy, vz); frame: n, t1, t2; dir: 24 doubles.  Fre = Re F(x + i h dir),
+ * jv = Im F(x + i h dir) / h, the directional derivative of the flux. */
+void orc_face_flux_cs(const double* prims, const double* frame, int x_order, const double* dir,
+                      double h, double* Fre, double* jv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,7 @@ def load():
             getattr(lib, name).argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
         lib.orc_get_dudt.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         lib.orc_face_flux.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, _D]
+        lib.orc_face_flux_cs.argtypes = [_D, _D, ctypes.c_int, _D, ctypes.c_double, _D, _D]
         lib.orc_hex_props.argtypes = [_D, _D]
         lib.orc_quad_props.argtypes = [_D, _D]
         lib.orc_decode.argtypes = [_D, _D]
@@ -135,6 +136,18 @@ def face_flux(states, frame, x_order=2, flux_calc=0):
     F = np.zeros(6)
     lib.orc_face_flux(_p(s, _D), _p(fr, _D), x_order, flux_calc, _p(F, _D))
     return F
+
+
+def face_flux_cs(prims, frame, direction, x_order=2, h=1.0e-30):
+    """(Re F, J.direction) of the face flux at prims (4 x 6: rho, e, YB, vx, vy, vz) by the
+    complex step."""
+    lib = load()
+    s = np.ascontiguousarray(prims, dtype=float).reshape(24)
+    fr = np.ascontiguousarray(frame, dtype=float).reshape(9)
+    d = np.ascontiguousarray(direction, dtype=float).reshape(24)
+    Fre, jv = np.zeros(6), np.zeros(6)
+    lib.orc_face_flux_cs(_p(s, _D), _p(fr, _D), x_order, _p(d, _D), h, _p(Fre, _D), _p(jv, _D))
+    return Fre, jv
 
 
 def hex_props(p8):

@@ -43,3 +43,12 @@ def test_gas_model_mismatch_is_a_config_error(tmp_path):
     rc = so.chkn_load_job(str(tmp_path).encode(), 0, 1, ctypes.byref(h))
     assert rc == -3
     assert so.chkn_last_error(None) == b"Incorrect gas model data"
+
+
+def test_tuner_candidates_are_valid_launch_options():
+    from chkn import tune
+    c = tune.candidates()
+    assert len(c) == len(tune.TILES) * len(tune.TILE_K) * len(tune.ORDERS) * len(tune.PERSISTENT)
+    for o in c:
+        assert set(o) == {"tile_i", "tile_j", "tile_k", "tile_order", "persistent"}
+        assert o["tile_i"] * o["tile_j"] <= 1024

@@ -1,0 +1,223 @@
+"""Adjoint (reverse-mode) gradients of the unrolled step, on the GPU.
+
+* flux level: the device vector-Jacobian product of the face flux against the complex-step
+  derivative of the oracle's flux: <J^T Fbar, v> = <Fbar, J v> to 1e-12 relative;
+* unroll level: d(wall-pressure loss)/d(inflow Mach, p, T) against central finite
+  differences (relative step 1e-7, time-step sequence frozen) to 1e-5 relative, and the
+  derivative along a seeded random smooth direction of U0 to 1e-3 at step 1e-7 and 1e-5 at
+  step 1e-9 (see _fd_rows_u0 for why the two);
+* the gradient is deterministic and does not depend on the checkpoint interval (bit for
+  bit), and an ensemble member's gradient equals that of the job run alone (bit for bit).
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from chkn import cases, prep
+from chkn.job import JobData
+from chkn.lib import Sim, flux_vjp_points
+
+torch = pytest.importorskip("torch")
+from chkn import adjoint  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+FD_REL = 1.0e-7
+FD_TOL = 1.0e-5
+
+
+def _points(rng, n, speed):
+    prims = np.empty((n, 4, 6))
+    frames = np.empty((n, 9))
+    for t in range(n):
+        base_rho, base_e = rng.uniform(0.1, 1.0), rng.uniform(1.5e5, 3.0e5)
+        for c in range(4):
+            prims[t, c] = [base_rho * (1 + 0.2 * rng.uniform(-1, 1)), base_e * (1 + 0.2 * rng.uniform(-1, 1)),
+                           0.0, speed * rng.uniform(-1, 1), speed * rng.uniform(-1, 1), speed * rng.uniform(-1, 1)]
+        nrm = rng.normal(size=3)
+        nrm /= np.linalg.norm(nrm)
+        t1 = rng.normal(size=3)
+        t1 -= t1.dot(nrm) * nrm
+        t1 /= np.linalg.norm(t1)
+        frames[t] = np.concatenate([nrm, t1, np.cross(nrm, t1)])
+    return prims, frames
+
+
+@pytest.mark.parametrize("x_order", [1, 2])
+def test_flux_vjp_matches_complex_step(x_order):
+    rng = np.random.default_rng(10 + x_order)
+    worst = 0.0
+    for speed in (80.0, 400.0, 1500.0):
+        prims, frames = _points(rng, 64, speed)
+        fbar = rng.normal(size=(64, 6))
+        b = flux_vjp_points(prims, frames, fbar, x_order=x_order)
+        for t in range(64):
+            v = rng.normal(size=(4, 6)) * (np.abs(prims[t]) + 1.0)
+            v[:, 2] = 0.0  # YB = 0: the limiter's kink at zero differences
+            _, jv = ob.face_flux_cs(prims[t], frames[t], v, x_order=x_order)
+            lhs = float(np.sum(b[t] * v))
+            rhs = float(np.dot(fbar[t], jv))
+            scale = float(np.sum(np.abs(b[t] * v))) + 1e-300
+            worst = max(worst, abs(lhs - rhs) / scale)
+    assert worst < 1e-12, worst
+
+
+def _vary_state(job, seed, scale):
+    """Give the initial velocity and internal energy (hence pressure) smooth variations.  On
+    a uniform field the scheme sits on its kinks -- |ab| = 0 in every limiter, |pL - pR| = 0
+    in AUSMDV's switch -- where central differences of step 1e-7 average two one-sided
+    slopes instead of measuring the derivative."""
+    for b in job.blocks:
+        io = b.iovars
+        x, y, z = io[0], io[1], io[2]
+        u = np.full_like(x, scale)
+        io[7] = io[7] * (1.0 + 0.02 * np.sin(4.0 * x + 9.0 * y + 5.0 * z + 0.7 * seed))
+        io[10] = io[10] + 0.02 * u * np.sin(7.0 * x + 3.0 * z + 0.3 * seed)
+        io[11] = io[11] + 0.01 * u * np.sin(5.0 * x + 11.0 * y)
+        io[12] = io[12] + 0.01 * u * np.cos(3.0 * x + 13.0 * z)
+    return job
+
+
+def _small_ramp(max_step=60, seed=2):
+    cases.ramp(12, 4, 6, max_step=max_step, seed=seed)
+    prep.config.dt_init = 2.0e-6
+    return _vary_state(JobData.from_prep(), seed, 1486.8)
+
+
+def _smooth_direction(prob, seed):
+    """A seeded random direction for U0 that varies smoothly in space (random Fourier modes
+    per conserved quantity), scaled by |U0| cell by cell."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for job in prob.jobs:
+        for b in job.blocks:
+            x, y, z = (np.asarray(b.iovars[r]) for r in range(3))
+            for q in range(6):
+                f = np.zeros_like(x)
+                for _ in range(4):
+                    k = rng.integers(1, 6, size=3)
+                    f += rng.normal() * np.sin(2 * np.pi * (k[0] * x + k[1] * y + k[2] * z) + rng.uniform(0, 6.3))
+                parts.append(f)
+    v = torch.as_tensor(np.concatenate(parts), device="cuda")
+    return v * prob.u0.abs()
+
+
+class Problem:
+    """A job with its loss as a function of (U0, inflow (Mach, p, T)) via the adjoint."""
+
+    def __init__(self, jobs, nsteps, C, mpt):
+        self.jobs = jobs
+        self.sim = Sim(jobs=jobs) if len(jobs) > 1 else Sim(jobs[0])
+        self.u0 = adjoint.initial_state(self.sim)
+        self.loss_fn = adjoint.WallPressure(self.sim, jobs)
+        self.nsteps, self.C = nsteps, C
+        self.mpt = [np.asarray(m, dtype=np.float64) for m in mpt]
+        self.dts = None
+        # Each inflow flow state takes the (Mach, p, T) of its instance.
+        self.state_inst = [q for q, j in enumerate(jobs) for _ in j.flow_states]
+
+    def inflow(self, mpts):
+        return torch.stack([adjoint.inflow_from_mpt(mpts[q]) for q in self.state_inst])
+
+    def loss(self, u0, mpts, C=None):
+        uN = adjoint.unroll(self.sim, u0, self.inflow(mpts), self.nsteps, C or self.C, self.dts)
+        if self.dts is None:
+            self.dts = uN.grad_fn.dts
+        return self.loss_fn(uN), uN
+
+    def gradient(self, C=None):
+        u0 = self.u0.clone().requires_grad_(True)
+        mpts = [torch.tensor(m, dtype=torch.float64, requires_grad=True) for m in self.mpt]
+        L, _ = self.loss(u0, mpts, C)
+        L.backward()
+        return float(L.detach()), u0.grad.clone(), [m.grad.clone() for m in mpts]
+
+    def value(self, u0=None, mpts=None):
+        with torch.no_grad():
+            mp = [torch.tensor(m, dtype=torch.float64) for m in (mpts or self.mpt)]
+            L, _ = self.loss(self.u0 if u0 is None else u0, mp)
+        return float(L)
+
+
+def _fd_rows_inflow(prob, grad_mpt, inst=0):
+    rows = []
+    for k in range(3):
+        h = FD_REL * abs(prob.mpt[inst][k])
+        mp = [m.copy() for m in prob.mpt]
+        mm = [m.copy() for m in prob.mpt]
+        mp[inst][k] += h
+        mm[inst][k] -= h
+        fd = (prob.value(mpts=mp) - prob.value(mpts=mm)) / (2 * h)
+        rows.append(("dL/d" + "MpT"[k], fd, float(grad_mpt[inst][k]), FD_TOL))
+    return rows
+
+
+def _fd_rows_u0(prob, grad_u0):
+    """Central differences along a smooth direction of U0.  The limited scheme has kinks
+    (|ab| in each van Albada limiter, |pL - pR| and the upwind switch in AUSMDV) at which
+    its derivative jumps; a perturbation of every cell by 1e-7 relative crosses a few of
+    them, which costs central differences up to about 1e-4 of accuracy, while the step 1e-9
+    does not (and roundoff still allows 1e-5 there).  Both are checked."""
+    v = _smooth_direction(prob, 7)
+    g = float(torch.dot(grad_u0, v))
+    rows = []
+    for h, tol in ((FD_REL, 1e-3), (1e-9, FD_TOL)):
+        fd = (prob.value(u0=prob.u0 + h * v) - prob.value(u0=prob.u0 - h * v)) / (2 * h)
+        rows.append(("dL/dU0.v (step %.0e)" % h, fd, g, tol))
+    return rows
+
+
+def _assert_rows(rows):
+    for name, fd, g, tol in rows:
+        print("%-22s fd %.12e adjoint %.12e rel %.2e" % (name, fd, g, abs(fd - g) / max(abs(g), abs(fd))))
+    for name, fd, g, tol in rows:
+        assert abs(fd - g) <= tol * max(abs(g), abs(fd)), (name, fd, g)
+
+
+# The U0 check uses a short unroll (4 steps), where the kinks are few.
+
+
+def test_unroll_gradient_matches_finite_differences():
+    job = _small_ramp()
+    prob = Problem([job], nsteps=40, C=7, mpt=[[5.0, 2.0e3, 220.0]])
+    L, gu0, gm = prob.gradient()
+    assert L > 0 and np.all(np.isfinite(gm[0].numpy()))
+    assert abs(prob.value() - L) <= 1e-15 * abs(L)  # replay with the recorded dts
+    rows = _fd_rows_inflow(prob, gm)
+    short = Problem([job], nsteps=4, C=3, mpt=[[5.0, 2.0e3, 220.0]])
+    _, gu0s, _ = short.gradient()
+    _assert_rows(rows + _fd_rows_u0(short, gu0s))
+
+
+def test_gradient_through_exchange_walls_outflow_and_inactive_block():
+    cases.mixed_bc_array(max_step=40)
+    job = _vary_state(JobData.from_prep(), 3, 808.0)
+    prob = Problem([job], nsteps=12, C=5, mpt=[[2.5, 5.0e3, 260.0]])
+    _, _, gm = prob.gradient()
+    rows = _fd_rows_inflow(prob, gm)
+    short = Problem([job], nsteps=4, C=3, mpt=[[2.5, 5.0e3, 260.0]])
+    _, gu0s, _ = short.gradient()
+    _assert_rows(rows + _fd_rows_u0(short, gu0s))
+
+
+def test_gradient_is_deterministic_and_independent_of_checkpoint_interval():
+    job = _small_ramp()
+    prob = Problem([job], nsteps=25, C=4, mpt=[[5.0, 2.0e3, 220.0]])
+    _, g1, m1 = prob.gradient()
+    _, g2, m2 = prob.gradient()
+    _, g3, m3 = prob.gradient(C=25)
+    assert torch.equal(g1, g2) and torch.equal(m1[0], m2[0])
+    assert torch.equal(g1, g3) and torch.equal(m1[0], m3[0])
+
+
+def test_ensemble_member_gradient_equals_single_job():
+    jobs = [_small_ramp(seed=2), _small_ramp(seed=5)]
+    mpt = [[5.0, 2.0e3, 220.0], [5.0, 2.0e3, 220.0]]
+    ens = Problem(jobs, nsteps=20, C=6, mpt=mpt)
+    _, gu_e, gm_e = ens.gradient()
+    n0 = 6 * 12 * 4 * 6
+    for q in range(2):
+        one = Problem([jobs[q]], nsteps=20, C=6, mpt=[mpt[q]])
+        _, gu, gm = one.gradient()
+        assert torch.equal(gm[0], gm_e[q])
+        assert torch.equal(gu, gu_e[q * n0:(q + 1) * n0])

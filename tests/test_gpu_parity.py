@@ -62,7 +62,7 @@ JOBS = {
     "ramp_cfl": (_cfl_ramp, 40),
     "x_order_1": (_first_order, 15),
     "two_cell_block": (lambda: cases.ramp(2, 3, 2, max_step=20), 20),
-    "uneven_array": (lambda: cases.ramp_array(seed=3, lo=4, hi=9, max_step=22), 22),
+    "uneven_array": (lambda: cases.ramp_array(seed=3, lo=4, hi=9, max_step=22, i_walls=True), 22),
 }
 
 
