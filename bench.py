@@ -27,7 +27,8 @@ BYTES_PER_CELL_STEP = 864  # algorithmic FP64 bytes per cell-update (3 stages)
 BYTES_PER_CELL_DT = 144    # extra bytes per cell per CFL reduction
 
 
-WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300}
+WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300, "diff": 500,
+                  "ensemble-diff": 300}
 
 
 def build_workload(name, steps_total):
@@ -48,6 +49,16 @@ def build_workload(name, steps_total):
         jobs = cases.ramp_ensemble(seed=11, n=64, max_step=steps_total)
         desc = {"workload": "ensemble: 64 independent single-block Mach 3-8 ramp jobs (seed 11), "
                             "grids 32-96 x 32-64 x 32-64, per-instance dt, FP64"}
+    elif name == "diff":
+        jobs = [cases.ramp(128, 64, 64, max_step=1 << 30)]
+        desc = {"workload": "differentiation: one 128x64x64 block, Mach 5 ramp; forward + adjoint "
+                            "of the wall-pressure loss w.r.t. U0 and inflow (Mach, p, T), "
+                            "checkpoint every 50 steps with recomputation, dt sequence frozen, FP64"}
+    elif name == "ensemble-diff":
+        jobs = cases.ramp_ensemble(seed=11, n=64, max_step=1 << 30)
+        desc = {"workload": "batched differentiation: 64 independent Mach 3-8 ramp jobs (seed 11), "
+                            "forward + adjoint of each wall-pressure loss w.r.t. its inflow "
+                            "(Mach, p, T), checkpoint every 50 steps, FP64"}
     else:
         raise SystemExit("unknown workload " + name)
     desc.update({"cells": sum(j.active_cells for j in jobs),
@@ -75,6 +86,56 @@ def run_timed(jobs, opts, steps, warmup):
     cells = sim.active_cells
     sim.close()
     return wall, kernel_ms, launches, rep, cells
+
+
+def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
+    """Forward (taped) + adjoint of the wall-pressure loss against the plain forward of the
+    same steps.  Returns a dict of timings."""
+    import torch
+    from chkn import adjoint
+    from chkn.lib import Sim
+    sim = Sim(jobs=jobs) if len(jobs) > 1 else Sim(jobs[0])
+    sim.set_launch(**opts)
+    loss_fn = adjoint.WallPressure(sim, jobs)
+    u0 = adjoint.initial_state(sim)
+    mpts = []
+    for j in jobs:
+        p, T, _, vx = (float(v) for v in j.flow_states[0][:4])
+        mpts.append([vx / (1.4 * 287.1 * T) ** 0.5, p, T])
+    state_inst = [q for q, j in enumerate(jobs) for _ in j.flow_states]
+
+    def gradient(n):
+        mp = [torch.tensor(m, dtype=torch.float64, requires_grad=True) for m in mpts]
+        inflow = torch.stack([adjoint.inflow_from_mpt(mp[q]) for q in state_inst])
+        u = u0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        uN = adjoint.unroll(sim, u, inflow, n, checkpoint_every)
+        L = loss_fn(uN)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, float(L.detach()), [m.grad for m in mp], u.grad
+
+    gradient(min(warmup, steps) or 1)  # allocates tapes and checkpoints
+    t_fwd, t_adj, L, gm, gu = gradient(steps)
+    # Plain forward of the same steps from the same state (stepping ring, graph replay).
+    sim.write_conserved(u0)
+    sim.step(1)
+    sim.write_conserved(u0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    cells = sim.active_cells
+    sim.close()
+    return {"seconds_forward_taped": t_fwd, "seconds_adjoint": t_adj, "seconds_plain_forward": t_plain,
+            "fwd_adj_over_fwd": (t_fwd + t_adj) / t_plain, "loss": L,
+            "dL_dMpT_first": [float(v) for v in gm[0]], "grad_u0_norm": float(gu.norm()),
+            "cells": cells}
 
 
 def cpu_baseline(job, steps):
@@ -171,6 +232,19 @@ def main():
     else:
         opts = {}
     opts = dict(opts, use_graph=bool(args.graph), per_block_launch=bool(args.per_block))
+    if args.workload in ("diff", "ensemble-diff"):
+        r = run_diff(jobs, opts, steps, args.warmup)
+        wall = r["seconds_forward_taped"] + r["seconds_adjoint"]
+        line = {"metric": "cell-updates/s (forward + adjoint)",
+                "value": r["cells"] * steps / wall, "unit": "cell-updates/s", "n_gpus": 1,
+                "steps": steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / steps,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                "data": "synthetic",
+                "config": dict(desc, launch=opts, autotune=tuned, checkpoint_every=50,
+                               plain_forward_value=r["cells"] * steps / r["seconds_plain_forward"],
+                               timings=r)}
+        print(json.dumps(line))
+        return
     wall, kernel_ms, launches, rep, cells = run_timed(jobs, opts, steps, args.warmup)
     first = args.warmup
     reductions = sum(1 for s in range(first + 1, first + steps + 1) if s % jobs[0].cfl_count == 0)

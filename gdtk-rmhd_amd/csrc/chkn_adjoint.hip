@@ -14,6 +14,8 @@
 //      adjoint to the conserved one (decode_vjp) and applies the transpose of the TVD-RK3
 //      stage combination (block.cu:1090-1160).
 // No atomics: every sum has one writer and a fixed order, so the gradient is deterministic.
+// Gradients need no bit-exact arithmetic, so this file is built with floating-point
+// contraction and reciprocal arithmetic allowed (the stage kernels are not).
 //
 // Reversed step with A0 = adjoint of U^{n+1} on entry (seed: P = dt/6 A0, D = 4dt/6 A0):
 //   stage 3 (W2): A0 += U2bar; D = P + dt/4 U2bar; P = D
@@ -66,14 +68,22 @@ __device__ __forceinline__ void add_slot(const BlockDev* B, int d, int fi, int p
     for (int q = 0; q < 6; ++q) acc[q] += g[(long long)(6 * p + q) * nf + fi];
 }
 
+constexpr int kFaceThreads = 128;
+
+// One thread per face.  The four stencil states are staged in LDS (one column per thread,
+// read back only by that thread) so that they need not stay in registers across the
+// flux adjoint: reconstruction reads them, AUSMDV's adjoint runs on the reconstructed
+// states alone, and the limiter adjoints read them again and write G directly.
 template <int XORDER>
-__global__ __launch_bounds__(256) void adj_face_kernel(AdjArgs A)
+__global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
 {
+    __shared__ double st[24][kFaceThreads];
+    const int t = (int)threadIdx.x;
     const FaceChunk ch = A.faces[blockIdx.x];
     const BlockDev* B = A.blocks + ch.blk;
     const int d = ch.dir;
     const long long nf = nfaces(B, d);
-    const int fi = ch.f0 + (int)threadIdx.x;
+    const int fi = ch.f0 + t;
     if (fi >= nf) return;
     const int nic = B->nic, njc = B->njc;
     int m, a, c;
@@ -110,28 +120,70 @@ __global__ __launch_bounds__(256) void adj_face_kernel(AdjArgs A)
         Fb[q] = area * (vr * dr - vl * dl);
         any = any || (Fb[q] != 0.0);
     }
-    double b[4][6];
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) b[p][q] = 0.0;
-    if (any) {
-        const double* W = B->W + A.slot * B->wstride;
-        Prim s[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const int pos = m - 2 + p;
-            s[p] = (pos >= 0 && pos < nline) ? ld_prim(W, n, cell_index(nic, njc, d, pos, a, c))
-                                              : ghost_prim(A.blocks, A.inflow, A.slot, B, d, pos, a, c);
-        }
-        const Frame fr = ld_frame(g, (int)nf, fi);
-        face_flux_vjp<XORDER>(s[0], s[1], s[2], s[3], fr, Fb, b);
-    }
     double* G = gface(B, d);
+    if (!any) {
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+        for (int k = 0; k < 24; ++k) G[(long long)k * nf + fi] = 0.0;
+        return;
+    }
+    const double* W = B->W + A.slot * B->wstride;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) G[(long long)(6 * p + q) * nf + fi] = b[p][q];
+    for (int p = 0; p < 4; ++p) {
+        const int pos = m - 2 + p;
+        const Prim sp = (pos >= 0 && pos < nline) ? ld_prim(W, n, cell_index(nic, njc, d, pos, a, c))
+                                                  : ghost_prim(A.blocks, A.inflow, A.slot, B, d, pos, a, c);
+        st[6 * p + 0][t] = sp.rho;
+        st[6 * p + 1][t] = sp.e;
+        st[6 * p + 2][t] = sp.YB;
+        st[6 * p + 3][t] = sp.vx;
+        st[6 * p + 4][t] = sp.vy;
+        st[6 * p + 5][t] = sp.vz;
+    }
+    // Reconstructed face states (face.cu:347-371).
+    double qL[6], qR[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        if (XORDER > 1) {
+            muscl(st[q][t], st[6 + q][t], st[12 + q][t], st[18 + q][t], qL[q], qR[q]);
+        } else {
+            qL[q] = st[6 + q][t];
+            qR[q] = st[12 + q][t];
+        }
+    }
+    const Frame fr = ld_frame(g, (int)nf, fi);
+    const double TL = eos_T(qL[1], qL[2]), TR = eos_T(qR[1], qR[2]);
+    const double pL = eos_p(qL[0], TL), pR = eos_p(qR[0], TR);
+    const double aL = eos_a(TL), aR = eos_a(TR);
+    double bL[8] = {0, 0, 0, 0, 0, 0, 0, 0}, bR[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    ausmdv_vjp(qL[0], qL[1], pL, aL, qL[2], qL[3], qL[4], qL[5], qR[0], qR[1], pR, aR, qR[2], qR[3],
+               qR[4], qR[5], fr, Fb, bL, bR);
+    double side[2][6];
+    {
+        const double bT = bL[2] * qL[0] * kR + bL[3] * kGamma * kR / (2.0 * aL);
+        side[0][0] = bL[0] + bL[2] * kR * TL;
+        side[0][1] = bL[1] + bT / kCv;
+        side[0][2] = bL[4] + bT * kQ / kCv;
+        side[0][3] = bL[5]; side[0][4] = bL[6]; side[0][5] = bL[7];
+    }
+    {
+        const double bT = bR[2] * qR[0] * kR + bR[3] * kGamma * kR / (2.0 * aR);
+        side[1][0] = bR[0] + bR[2] * kR * TR;
+        side[1][1] = bR[1] + bT / kCv;
+        side[1][2] = bR[4] + bT * kQ / kCv;
+        side[1][3] = bR[5]; side[1][4] = bR[6]; side[1][5] = bR[7];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        double gq[4] = {0.0, 0.0, 0.0, 0.0};
+        if (XORDER > 1) {
+            muscl_vjp(st[q][t], st[6 + q][t], st[12 + q][t], st[18 + q][t], side[0][q], side[1][q], gq);
+        } else {
+            gq[1] = side[0][q];
+            gq[2] = side[1][q];
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) G[(long long)(6 * p + q) * nf + fi] = gq[p];
+    }
 }
 
 // Adjoint reaching the ghost line of boundary side `side` at transverse (a, c): the sum over
@@ -271,27 +323,6 @@ __global__ __launch_bounds__(256) void adj_seed_kernel(AdjArgs A)
     }
 }
 
-__global__ __launch_bounds__(256) void decode_all_kernel(const BlockDev* blocks, const CellChunk* cells)
-{
-    const CellChunk ch = cells[blockIdx.x];
-    const BlockDev* B = blocks + ch.blk;
-    const int n = (int)B->n;
-    const int cidx = ch.c0 + (int)threadIdx.x;
-    if (cidx >= n) return;
-    double U[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) U[q] = B->U[(long long)q * n + cidx];
-    Prim w{0, 0, 0, 0, 0, 0};
-    decode(U, w);
-    double* W = B->W;
-    W[cidx] = w.rho;
-    W[n + cidx] = w.e;
-    W[2 * n + cidx] = w.YB;
-    W[3 * n + cidx] = w.vx;
-    W[4 * n + cidx] = w.vy;
-    W[5 * n + cidx] = w.vz;
-}
-
 __global__ __launch_bounds__(64) void flux_vjp_points_kernel(const double* in, double* out, int npts,
                                                             int x_order)
 {
@@ -321,8 +352,8 @@ hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s)
 hipError_t launch_adj_faces(const AdjArgs& a, int nface_chunks, int x_order, hipStream_t s)
 {
     if (nface_chunks <= 0) return hipSuccess;
-    if (x_order > 1) hipLaunchKernelGGL(adj_face_kernel<2>, dim3(nface_chunks), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(adj_face_kernel<1>, dim3(nface_chunks), dim3(256), 0, s, a);
+    if (x_order > 1) hipLaunchKernelGGL(adj_face_kernel<2>, dim3(nface_chunks), dim3(kFaceThreads), 0, s, a);
+    else hipLaunchKernelGGL(adj_face_kernel<1>, dim3(nface_chunks), dim3(kFaceThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -339,14 +370,6 @@ hipError_t launch_adj_gather(const AdjArgs& a, int ncell_chunks, int stage, hipS
     if (stage == 3) hipLaunchKernelGGL(adj_gather_kernel<3>, dim3(ncell_chunks), dim3(256), 0, s, a);
     else if (stage == 2) hipLaunchKernelGGL(adj_gather_kernel<2>, dim3(ncell_chunks), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(adj_gather_kernel<1>, dim3(ncell_chunks), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_decode_all(const BlockDev* blocks, const CellChunk* cells, int ncell_chunks,
-                             hipStream_t s)
-{
-    if (ncell_chunks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(decode_all_kernel, dim3(ncell_chunks), dim3(256), 0, s, blocks, cells);
     return hipGetLastError();
 }
 

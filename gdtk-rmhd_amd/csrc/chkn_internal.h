@@ -72,7 +72,7 @@ struct StageArgs {
     int win, wout;         // W slots read / written by this stage
 };
 
-// Work items of the adjoint kernels: 256 faces of one direction of one block, 256 cells of
+// Work items of the adjoint kernels: 128 faces of one direction of one block, 256 cells of
 // one block, one boundary column (line of ghost cells) of an inflow boundary.
 struct FaceChunk {
     int blk, dir, f0;

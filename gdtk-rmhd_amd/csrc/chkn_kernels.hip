@@ -348,7 +348,37 @@ __global__ __launch_bounds__(256) void signal_min_kernel(const BlockDev* B,
     if ((threadIdx.x & 63) == 0 && v != ~0ull) atomicMin(sigmin_bits, v);
 }
 
+// W = decode(U) for every active block (after its conserved state was replaced).
+__global__ __launch_bounds__(256) void decode_all_kernel(const BlockDev* blocks, const CellChunk* cells)
+{
+    const CellChunk ch = cells[blockIdx.x];
+    const BlockDev* B = blocks + ch.blk;
+    const int n = (int)B->n;
+    const int cidx = ch.c0 + (int)threadIdx.x;
+    if (cidx >= n) return;
+    double U[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) U[q] = B->U[(long long)q * n + cidx];
+    Prim w{0, 0, 0, 0, 0, 0};
+    decode(U, w);
+    double* W = B->W;
+    W[cidx] = w.rho;
+    W[n + cidx] = w.e;
+    W[2 * n + cidx] = w.YB;
+    W[3 * n + cidx] = w.vx;
+    W[4 * n + cidx] = w.vy;
+    W[5 * n + cidx] = w.vz;
+}
+
 }  // namespace
+
+hipError_t launch_decode_all(const BlockDev* blocks, const CellChunk* cells, int ncell_chunks,
+                             hipStream_t s)
+{
+    if (ncell_chunks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(decode_all_kernel, dim3(ncell_chunks), dim3(256), 0, s, blocks, cells);
+    return hipGetLastError();
+}
 
 hipError_t launch_signal_min(const BlockDev* block, int ncells, unsigned long long* sigmin_bits,
                              hipStream_t s)
