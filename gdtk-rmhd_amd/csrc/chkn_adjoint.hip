@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
     const int d = ch.dir;
     const long long nf = nfaces(B, d);
     const int fi = ch.f0 + t;
-    if (fi >= nf) return;
+    if (fi >= ch.f1) return;
     const int nic = B->nic, njc = B->njc;
     int m, a, c;
     if (d == 0) {
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
     const BlockDev* B = A.blocks + ch.blk;
     const int n = (int)B->n;
     const int cidx = ch.c0 + (int)threadIdx.x;
-    if (cidx >= n) return;
+    if (cidx >= ch.c1) return;
     const int nic = B->nic, njc = B->njc;
     const int ijk[3] = {cidx % nic, (cidx / nic) % njc, cidx / (nic * njc)};
     double Wb[6] = {0, 0, 0, 0, 0, 0};
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256) void adj_seed_kernel(AdjArgs A)
     const BlockDev* B = A.blocks + ch.blk;
     const int n = (int)B->n;
     const int cidx = ch.c0 + (int)threadIdx.x;
-    if (cidx >= n) return;
+    if (cidx >= ch.c1) return;
     const double dt = A.dt[B->inst];
     const double* A0 = B->adj;
     double* Pp = B->adj + 6 * (long long)n;

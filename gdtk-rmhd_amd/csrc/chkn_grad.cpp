@@ -41,8 +41,9 @@ void build_adj_tables(chkn_sim* s)
                                  (long long)b.nic * (b.njc + 1) * b.nkc,
                                  (long long)b.nic * b.njc * (b.nkc + 1)};
         for (int d = 0; d < 3; ++d)
-            for (long long f0 = 0; f0 < nf[d]; f0 += 128) fc.push_back(FaceChunk{(int)ib, d, (int)f0});
-        for (int c0 = 0; c0 < b.n; c0 += 256) cc.push_back(CellChunk{(int)ib, c0});
+            for (long long f0 = 0; f0 < nf[d]; f0 += 128)
+                fc.push_back(FaceChunk{(int)ib, d, (int)f0, (int)std::min(f0 + 128, nf[d])});
+        for (int c0 = 0; c0 < b.n; c0 += 256) cc.push_back(CellChunk{(int)ib, c0, std::min(c0 + 256, b.n)});
         for (int side = 0; side < 6; ++side) {
             if (b.bc[side] != CHKN_BC_INFLOW) continue;
             const int d = side >> 1;

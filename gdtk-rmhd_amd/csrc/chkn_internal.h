@@ -72,13 +72,13 @@ struct StageArgs {
     int win, wout;         // W slots read / written by this stage
 };
 
-// Work items of the adjoint kernels: 128 faces of one direction of one block, 256 cells of
-// one block, one boundary column (line of ghost cells) of an inflow boundary.
+// Work items of the adjoint kernels: up to 128 faces of one direction of one block, up to 256
+// cells of one block, one boundary column (line of ghost cells) of an inflow boundary.
 struct FaceChunk {
-    int blk, dir, f0;
+    int blk, dir, f0, f1;  // faces [f0, f1) of direction dir, at most 128
 };
 struct CellChunk {
-    int blk, c0;
+    int blk, c0, c1;       // cells [c0, c1), at most 256
 };
 struct InflowCol {
     int blk, side, t;  // t: transverse index a + na*c of the line

@@ -355,7 +355,7 @@ __global__ __launch_bounds__(256) void decode_all_kernel(const BlockDev* blocks,
     const BlockDev* B = blocks + ch.blk;
     const int n = (int)B->n;
     const int cidx = ch.c0 + (int)threadIdx.x;
-    if (cidx >= n) return;
+    if (cidx >= ch.c1) return;
     double U[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) U[q] = B->U[(long long)q * n + cidx];
