@@ -6,13 +6,13 @@
 //      its two adjacent cells (ghost sides contribute nothing) and the vector-Jacobian
 //      product of its flux (face_flux_vjp) with respect to its four stencil states, stored
 //      per face in G (24 values: L1, L0, R0, R1 x rho, e, YB, vx, vy, vz);
-//   2. inflow pass: the stencil adjoints that land on inflow ghost cells are added into one
-//      accumulator per boundary line (the gradient of the inflow state);
-//   3. gather: every cell sums, in a fixed order, the stencil adjoints of the faces whose
+//   2. gather: every cell sums, in a fixed order, the stencil adjoints of the faces whose
 //      stencils contain it -- its own block's faces, the wall and outflow ghosts copied from
 //      it, and the exchange ghosts other blocks read from it -- then maps the primitive
 //      adjoint to the conserved one (decode_vjp) and applies the transpose of the TVD-RK3
-//      stage combination (block.cu:1090-1160).
+//      stage combination (block.cu:1090-1160).  A cell next to an inflow boundary also adds
+//      the adjoints that land on its line of inflow ghosts into that line's accumulator (the
+//      gradient of the inflow state).
 // No atomics: every sum has one writer and a fixed order, so the gradient is deterministic.
 // Gradients need no bit-exact arithmetic, so this file is built with floating-point
 // contraction and reciprocal arithmetic allowed (the stage kernels are not).
@@ -20,7 +20,7 @@
 // Reversed step with A0 = adjoint of U^{n+1} on entry (seed: P = dt/6 A0, D = 4dt/6 A0):
 //   stage 3 (W2): A0 += U2bar; D = P + dt/4 U2bar; P = D
 //   stage 2 (W1): A0 += U1bar; D = P + dt U1bar
-//   stage 1 (W0): A0 += U0bar        -> A0 = adjoint of U^n
+//   stage 1 (W0): A0 += U0bar        -> A0 = adjoint of U^n; seed of the next reversed step
 #include "chkn_adjoint.h"
 #include "chkn_internal.h"
 #include "chkn_physics.h"
@@ -206,20 +206,6 @@ __device__ __forceinline__ void ghost_line_adjoint(const BlockDev* B, int side, 
     }
 }
 
-__global__ __launch_bounds__(256) void adj_inflow_kernel(AdjArgs A)
-{
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= A.nicols) return;
-    const InflowCol ic = A.icols[col];
-    const BlockDev* B = A.blocks + ic.blk;
-    const int d = ic.side >> 1;
-    const int na = d == 0 ? B->njc : B->nic;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    ghost_line_adjoint(B, ic.side, ic.t % na, ic.t / na, acc);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) A.inflow_acc[(long long)q * A.nicols + col] += acc[q];
-}
-
 template <int STAGE>
 __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
 {
@@ -248,10 +234,16 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
         for (int side = 0; side < 2; ++side) {
             if (m != (side ? nline - 1 : 0)) continue;
             const int code = B->bc[2 * d + side];
-            if (code == 3 || code == 4 || code == 6) continue;
+            if (code == 3 || code == 6) continue;
             double gb[6] = {0, 0, 0, 0, 0, 0};
             ghost_line_adjoint(B, 2 * d + side, a, c, gb);
-            if (code == 5) {
+            if (code == 4) {
+                // Inflow ghosts hold the inflow state: this cell owns its line's accumulator.
+                const int na = d == 0 ? njc : nic;
+                const long long col = B->icol + B->icol_side[2 * d + side] + a + na * c;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) A.inflow_acc[(long long)q * A.nicols + col] += gb[q];
+            } else if (code == 5) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) Wb[q] += gb[q];
             } else {
@@ -300,6 +292,10 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
             Pp[o] = v;
         } else if (STAGE == 2) {
             D[o] = Pp[o] + dt * Ub[q];
+        } else if (A.dt_next) {
+            const double dtn = A.dt_next[B->inst];
+            Pp[o] = kOneSixth * dtn * A0[o];
+            D[o] = kFourSixth * dtn * A0[o];
         }
     }
 }
@@ -354,13 +350,6 @@ hipError_t launch_adj_faces(const AdjArgs& a, int nface_chunks, int x_order, hip
     if (nface_chunks <= 0) return hipSuccess;
     if (x_order > 1) hipLaunchKernelGGL(adj_face_kernel<2>, dim3(nface_chunks), dim3(kFaceThreads), 0, s, a);
     else hipLaunchKernelGGL(adj_face_kernel<1>, dim3(nface_chunks), dim3(kFaceThreads), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_adj_inflow(const AdjArgs& a, hipStream_t s)
-{
-    if (a.nicols <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_inflow_kernel, dim3((a.nicols + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -44,8 +44,12 @@ void build_adj_tables(chkn_sim* s)
             for (long long f0 = 0; f0 < nf[d]; f0 += 128)
                 fc.push_back(FaceChunk{(int)ib, d, (int)f0, (int)std::min(f0 + 128, nf[d])});
         for (int c0 = 0; c0 < b.n; c0 += 256) cc.push_back(CellChunk{(int)ib, c0, std::min(c0 + 256, b.n)});
+        BlockDev& bd = s->bdev[ib];
+        bd.icol = (int)ic.size();
         for (int side = 0; side < 6; ++side) {
+            bd.icol_side[side] = -1;
             if (b.bc[side] != CHKN_BC_INFLOW) continue;
+            bd.icol_side[side] = (int)ic.size() - bd.icol;
             const int d = side >> 1;
             const int nt = d == 0 ? b.njc * b.nkc : (d == 1 ? b.nic * b.nkc : b.nic * b.njc);
             for (int t = 0; t < nt; ++t) {
@@ -292,6 +296,10 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     const int nck = (nsteps + C - 1) / C;
     alloc_adjoint(s, C);
     build_adj_tables(s);
+    dfree(g.d_dts);
+    g.d_dts = nullptr;
+    HIPCHK(hipMalloc(&g.d_dts, sizeof(double) * g.dts.size()));
+    HIPCHK(hipMemcpy(g.d_dts, g.dts.data(), sizeof(double) * g.dts.size(), hipMemcpyHostToDevice));
     hipEvent_t e0 = s->ev0, e1 = s->ev1;
     HIPCHK(hipEventRecord(e0, s->stream));
     // Park the final state; seed A0 with the final adjoint.
@@ -329,16 +337,19 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
             break;
         }
         for (int q = K - 1; q >= 0; --q) {
-            A.dt = ctl_dt(s->d_ctl, q, N);
-            HIPCHK(launch_adj_seed(A, g.ncell_chunks, s->stream));
+            const int step = j * C + q;
+            A.dt = g.d_dts + (size_t)step * N;
+            A.dt_next = step > 0 ? g.d_dts + (size_t)(step - 1) * N : nullptr;
+            if (step == nsteps - 1) {
+                HIPCHK(launch_adj_seed(A, g.ncell_chunks, s->stream));
+                launches += 1;
+            }
             for (int stage = 3; stage >= 1; --stage) {
                 A.slot = 3 * q + stage - 1;
                 HIPCHK(launch_adj_faces(A, g.nface_chunks, s->x_order, s->stream));
-                HIPCHK(launch_adj_inflow(A, s->stream));
                 HIPCHK(launch_adj_gather(A, g.ncell_chunks, stage, s->stream));
-                launches += 3;
+                launches += 2;
             }
-            launches += 1;
         }
         HIPCHK(hipStreamSynchronize(s->stream));
     }
@@ -389,7 +400,7 @@ void free_grad(chkn_sim* s)
     }
     if (s->grad) {
         dfree(s->grad->d_face_chunks); dfree(s->grad->d_cell_chunks);
-        dfree(s->grad->d_inflow_cols); dfree(s->grad->d_inflow_acc);
+        dfree(s->grad->d_inflow_cols); dfree(s->grad->d_inflow_acc); dfree(s->grad->d_dts);
         s->grad.reset();
     }
 }

@@ -42,6 +42,8 @@ struct BlockDev {
     const double* len;
     // adjoint
     int rp[6];             // block whose opposite side exchanges into this side, or -1
+    int icol;              // first inflow line of this block in the inflow accumulator
+    int icol_side[6];      // offset of each inflow side's lines from icol, or -1
     double* G;             // face stencil adjoints: [24][nfi], then [24][nfj], then [24][nfk]
     double* adj;           // A0 (adjoint of U at the step boundary), P, D: [3][6][n]
 };
@@ -94,14 +96,15 @@ struct AdjArgs {
     int nicols;
     int slot;                 // W slot of the stage state W_s
     const double* dt;         // [ninst] time step of the step being reversed
+    const double* dt_next;    // [ninst] time step of the step reversed next, or null (stage 1)
 };
 
 // Adjoint of one stage's residual: face pass (stencil adjoints of every face into G), then
-// the inflow columns, then the per-cell gather with the stage's RK combination (stage 3, 2,
-// 1 in reverse order).  seed: P = dt/6 A0, D = dt 4/6 A0 (start of a reversed step).
+// the per-cell gather with the stage's RK combination (stage 3, 2, 1 in reverse order) and
+// the inflow lines; the stage-1 gather also seeds the next reversed step when dt_next is
+// given.  seed: P = dt/6 A0, D = dt 4/6 A0 (start of the first reversed step).
 hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s);
 hipError_t launch_adj_faces(const AdjArgs& a, int nface_chunks, int x_order, hipStream_t s);
-hipError_t launch_adj_inflow(const AdjArgs& a, hipStream_t s);
 hipError_t launch_adj_gather(const AdjArgs& a, int ncell_chunks, int stage, hipStream_t s);
 // U and W = decode(U) of every active block from U (after the conserved state was replaced).
 hipError_t launch_decode_all(const BlockDev* blocks, const CellChunk* cells, int ncell_chunks,

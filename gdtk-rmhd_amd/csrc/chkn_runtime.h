@@ -89,6 +89,7 @@ struct GradState {
     void* d_cell_chunks = nullptr;
     void* d_inflow_cols = nullptr;
     double* d_inflow_acc = nullptr;  // 6 x ninflow_cols
+    double* d_dts = nullptr;         // dts on the device, [nsteps][ninst]
     std::vector<int> inflow_col_state;  // flow-state index of each inflow column
 };
 
