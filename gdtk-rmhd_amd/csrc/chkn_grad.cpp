@@ -91,16 +91,16 @@ void save_checkpoint(chkn_sim* s, int j)
     }
 }
 
-// Restores U from checkpoint j and writes its W into `wdst(block)`.
+// Restores U from checkpoint j and writes its W into `wdst(block)`, on stream st.
 template <typename F>
-void load_checkpoint(chkn_sim* s, int j, F wdst)
+void load_checkpoint(chkn_sim* s, int j, F wdst, hipStream_t st)
 {
     for (auto& b : s->blocks) {
         if (!b.active) continue;
         const size_t n6 = 6 * (size_t)b.n;
         const double* src = b.ckpt + (size_t)j * 2 * n6;
-        HIPCHK(hipMemcpyAsync(b.U, src, sizeof(double) * n6, hipMemcpyDeviceToDevice, s->stream));
-        HIPCHK(hipMemcpyAsync(wdst(b), src + n6, sizeof(double) * n6, hipMemcpyDeviceToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(b.U, src, sizeof(double) * n6, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(wdst(b), src + n6, sizeof(double) * n6, hipMemcpyDeviceToDevice, st));
     }
 }
 
@@ -125,7 +125,8 @@ void alloc_adjoint(chkn_sim* s, int C)
         if (!b.active) continue;
         if (g.tape_C < C) {
             dfree(b.tape);
-            b.tape = nullptr;
+            dfree(b.tape2);
+            b.tape = b.tape2 = nullptr;
             HIPCHK(hipMalloc(&b.tape, sizeof(double) * 6 * (size_t)b.n * (3 * (size_t)C + 1)));
         }
         if (!b.adj) {
@@ -137,21 +138,98 @@ void alloc_adjoint(chkn_sim* s, int C)
             s->bdev[ib].G = b.G;
         }
     }
+    if (g.tape_C < C) {
+        // The second tape is optional: without it the segments are recomputed and reversed
+        // one after the other.
+        g.two_tapes = true;
+        for (auto& b : s->blocks) {
+            if (!b.active) continue;
+            if (hipMalloc(&b.tape2, sizeof(double) * 6 * (size_t)b.n * (3 * (size_t)C + 1)) != hipSuccess) {
+                (void)hipGetLastError();
+                b.tape2 = nullptr;
+                g.two_tapes = false;
+                break;
+            }
+        }
+        if (!g.two_tapes)
+            for (auto& b : s->blocks) {
+                dfree(b.tape2);
+                b.tape2 = nullptr;
+            }
+    }
     g.tape_C = std::max(g.tape_C, C);
+    if (!g.rec) {
+        HIPCHK(hipStreamCreateWithFlags(&g.rec, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&g.ev_rev[0], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&g.ev_rev[1], hipEventDisableTiming));
+        HIPCHK(hipMalloc(&g.d_tab[0], sizeof(BlockDev) * s->bdev.size()));
+        HIPCHK(hipMalloc(&g.d_tab[1], sizeof(BlockDev) * s->bdev.size()));
+    }
 }
 
-// Points the block table's W at the tapes (tape = true) or back at the stepping ring.
-void point_w(chkn_sim* s, bool tape)
+// Block tables with W at tape (t = 0) and tape2 (t = 1), with the current adjoint pointers.
+void upload_tape_tables(chkn_sim* s)
 {
-    for (size_t ib = 0; ib < s->blocks.size(); ++ib)
-        if (s->blocks[ib].active) s->bdev[ib].W = tape ? s->blocks[ib].tape : s->blocks[ib].W;
-    upload_blocks(s);
+    GradState& g = *s->grad;
+    for (int t = 0; t < 2; ++t) {
+        std::vector<BlockDev> tab = s->bdev;
+        for (size_t ib = 0; ib < s->blocks.size(); ++ib)
+            if (s->blocks[ib].active) tab[ib].W = t == 0 ? s->blocks[ib].tape : s->blocks[ib].tape2;
+        HIPCHK(hipMemcpy(g.d_tab[t], tab.data(), sizeof(BlockDev) * tab.size(), hipMemcpyHostToDevice));
+    }
 }
 
-// K steps with the time steps dts[q*N + inst] from the handle's current state.  With
-// tape = true, step q reads tape slot 3q and writes slots 3q+1 .. 3q+3; otherwise the
-// stepping ring is used.  Returns the first stage's bad-cell count (0 if none).
-int fixed_dt_steps(chkn_sim* s, int K, const double* dts, bool tape)
+// Queues the recomputation of checkpoint segment j (K steps) into tape t on stream st:
+// checkpoint load, control area, 3K stage launches, control area read-back.  The caller
+// synchronises st before reading the bad-cell counters (recompute_bad).
+void queue_recompute(chkn_sim* s, int j, int K, int t, hipStream_t st)
+{
+    GradState& g = *s->grad;
+    const int N = (int)s->inst.size();
+    load_checkpoint(s, j, [t](HostBlock& b) { return t == 0 ? b.tape : b.tape2; }, st);
+    const double* dts = g.dts.data() + (size_t)j * g.C * N;
+    for (int q = 0; q < K; ++q) {
+        double* dtv = ctl_dt(s->h_ctl, q, N);
+        int* run = ctl_run(s->h_ctl, q, N);
+        unsigned long long* sg = ctl_sig(s->h_ctl, q, N);
+        int* bad = ctl_bad(s->h_ctl, q, N);
+        for (int qi = 0; qi < N; ++qi) {
+            dtv[qi] = dts[(size_t)q * N + qi];
+            run[qi] = 1;
+            sg[qi] = ~0ull;
+        }
+        bad[0] = bad[1] = bad[2] = 0;
+    }
+    HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, s->ctl_bytes, hipMemcpyHostToDevice, st));
+    for (int q = 0; q < K; ++q) {
+        StageArgs a = stage_args(s, q);
+        a.blocks = g.d_tab[t];
+        a.tile_base = 0;
+        a.ntiles = s->ntiles;
+        for (int stage = 1; stage <= 3; ++stage) {
+            a.win = 3 * q + stage - 1;
+            a.wout = 3 * q + stage;
+            HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, false,
+                                s->opts.persistent != 0, s->grid_limit, a, st));
+        }
+    }
+    HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, s->ctl_bytes, hipMemcpyDeviceToHost, st));
+}
+
+int recompute_bad(chkn_sim* s, int K)
+{
+    const int N = (int)s->inst.size();
+    for (int q = 0; q < K; ++q) {
+        const int* bad = ctl_bad(s->h_ctl, q, N);
+        for (int st = 0; st < 3; ++st)
+            if (bad[st] > 0) return bad[st];
+    }
+    return 0;
+}
+
+// K steps with the time steps dts[q*N + inst] from the handle's current state, on the
+// stepping ring.  Returns the first stage's bad-cell count (0 if none).
+int fixed_dt_steps(chkn_sim* s, int K, const double* dts)
 {
     const int N = (int)s->inst.size();
     for (int q = 0; q < K; ++q) {
@@ -166,24 +244,7 @@ int fixed_dt_steps(chkn_sim* s, int K, const double* dts, bool tape)
         }
         bad[0] = bad[1] = bad[2] = 0;
     }
-    if (!tape) {
-        run_segment(s, K, false);
-    } else {
-        HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, s->ctl_bytes, hipMemcpyHostToDevice, s->stream));
-        for (int q = 0; q < K; ++q) {
-            StageArgs a = stage_args(s, q);
-            a.tile_base = 0;
-            a.ntiles = s->ntiles;
-            for (int stage = 1; stage <= 3; ++stage) {
-                a.win = 3 * q + stage - 1;
-                a.wout = 3 * q + stage;
-                HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, false,
-                                    s->opts.persistent != 0, s->grid_limit, a, s->stream));
-            }
-        }
-        HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, s->ctl_bytes, hipMemcpyDeviceToHost, s->stream));
-        HIPCHK(hipStreamSynchronize(s->stream));
-    }
+    run_segment(s, K, false);
     for (int q = 0; q < K; ++q) {
         const int* bad = ctl_bad(s->h_ctl, q, N);
         for (int st = 0; st < 3; ++st)
@@ -236,7 +297,7 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
         save_checkpoint(s, j);
         if (dt_in) {
             const double* dts = dt_in + (size_t)j * C * N;
-            const int bad = fixed_dt_steps(s, K, dts, false);
+            const int bad = fixed_dt_steps(s, K, dts);
             ms_total += s->last_ms;
             launches += s->last_launches;
             s->last_ms = 0.0;
@@ -314,10 +375,9 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     if (g.ninflow_cols > 0)
         HIPCHK(hipMemsetAsync(g.d_inflow_acc, 0, sizeof(double) * 6 * g.ninflow_cols, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    point_w(s, true);
+    upload_tape_tables(s);
     AdjArgs A;
     std::memset(&A, 0, sizeof A);
-    A.blocks = s->d_blocks;
     A.inflow = s->d_inflow;
     A.faces = (const FaceChunk*)g.d_face_chunks;
     A.cells = (const CellChunk*)g.d_cell_chunks;
@@ -326,16 +386,22 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     A.nicols = g.ninflow_cols;
     long long launches = 0;
     int rc = CHKN_OK;
-    for (int j = nck - 1; j >= 0 && rc == CHKN_OK; --j) {
-        const int K = std::min(C, nsteps - j * C);
-        // Recompute the segment into the tape (its dt values land in control slots 0..K-1).
-        load_checkpoint(s, j, [](HostBlock& b) { return b.tape; });
-        const int bad = fixed_dt_steps(s, K, g.dts.data() + (size_t)j * C * N, true);
+    // With two tapes, segment j-1 is recomputed on the second stream while segment j is
+    // reversed; the recompute into a tape waits for the reversal that last read it.
+    const bool two = g.two_tapes;
+    hipStream_t rs = two ? g.rec : s->stream;
+    auto seg_len = [&](int j) { return std::min(C, nsteps - j * C); };
+    queue_recompute(s, nck - 1, seg_len(nck - 1), (nck - 1) & (two ? 1 : 0), rs);
+    for (int j = nck - 1; j >= 0; --j) {
+        const int K = seg_len(j);
+        const int t = two ? (j & 1) : 0;
+        HIPCHK(hipStreamSynchronize(rs));
         launches += 3LL * K;
-        if (bad > 0) {
+        if (recompute_bad(s, K) > 0) {
             rc = set_error(s, CHKN_ERR_STATE, "bad cells while recomputing a checkpoint segment");
             break;
         }
+        A.blocks = g.d_tab[t];
         for (int q = K - 1; q >= 0; --q) {
             const int step = j * C + q;
             A.dt = g.d_dts + (size_t)step * N;
@@ -351,11 +417,19 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
                 launches += 2;
             }
         }
-        HIPCHK(hipStreamSynchronize(s->stream));
+        if (j > 0) {
+            if (two) {
+                HIPCHK(hipEventRecord(g.ev_rev[t], s->stream));
+                // Tape t^1 was last read by the reversal of segment j+1.
+                if (j + 1 <= nck - 1) HIPCHK(hipStreamWaitEvent(rs, g.ev_rev[t ^ 1], 0));
+            }
+            queue_recompute(s, j - 1, seg_len(j - 1), two ? (t ^ 1) : 0, rs);
+        }
     }
+    HIPCHK(hipStreamSynchronize(rs));
+    HIPCHK(hipStreamSynchronize(s->stream));
     // Restore the final state into the stepping ring.
-    point_w(s, false);
-    load_checkpoint(s, nck, [](HostBlock& b) { return b.W; });
+    load_checkpoint(s, nck, [](HostBlock& b) { return b.W; }, s->stream);
     if (rc == CHKN_OK) {
         off = 0;
         for (auto& b : s->blocks) {
@@ -395,12 +469,16 @@ namespace rt {
 void free_grad(chkn_sim* s)
 {
     for (auto& b : s->blocks) {
-        dfree(b.tape); dfree(b.ckpt); dfree(b.adj); dfree(b.G);
-        b.tape = b.ckpt = b.adj = b.G = nullptr;
+        dfree(b.tape); dfree(b.tape2); dfree(b.ckpt); dfree(b.adj); dfree(b.G);
+        b.tape = b.tape2 = b.ckpt = b.adj = b.G = nullptr;
     }
     if (s->grad) {
         dfree(s->grad->d_face_chunks); dfree(s->grad->d_cell_chunks);
         dfree(s->grad->d_inflow_cols); dfree(s->grad->d_inflow_acc); dfree(s->grad->d_dts);
+        dfree(s->grad->d_tab[0]); dfree(s->grad->d_tab[1]);
+        if (s->grad->ev_rev[0]) (void)hipEventDestroy(s->grad->ev_rev[0]);
+        if (s->grad->ev_rev[1]) (void)hipEventDestroy(s->grad->ev_rev[1]);
+        if (s->grad->rec) (void)hipStreamDestroy(s->grad->rec);
         s->grad.reset();
     }
 }

@@ -53,6 +53,7 @@ struct HostBlock {
     int tile_first = 0, tile_count = 0;
     // adjoint (allocated by the first gradient call)
     double* tape = nullptr;    // (3C+1) W slots of the segment being reversed
+    double* tape2 = nullptr;   // second tape: the next segment is recomputed into it meanwhile
     double* ckpt = nullptr;    // checkpoints: U then W, 12n doubles each
     double* adj = nullptr;     // A0, P, D: 18n doubles
     double* G = nullptr;       // face stencil adjoints: 24 x (nfi + nfj + nfk)
@@ -90,6 +91,12 @@ struct GradState {
     void* d_inflow_cols = nullptr;
     double* d_inflow_acc = nullptr;  // 6 x ninflow_cols
     double* d_dts = nullptr;         // dts on the device, [nsteps][ninst]
+    // Two tapes: segment j-1 is recomputed on stream rec into one while segment j is reversed
+    // from the other on the handle's stream.
+    bool two_tapes = false;
+    hipStream_t rec = nullptr;
+    hipEvent_t ev_rev[2] = {nullptr, nullptr};
+    BlockDev* d_tab[2] = {nullptr, nullptr};  // block tables with W at tape / tape2
     std::vector<int> inflow_col_state;  // flow-state index of each inflow column
 };
 
