@@ -201,7 +201,7 @@ def main():
     ap.add_argument("--per-block", type=int, default=0)
     ap.add_argument("--compare-per-block", type=int, default=-1,
                     help="also time one launch per block per stage (default: batched only)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device copy (scale for HBM byte counters)")

@@ -221,3 +221,15 @@ def test_ensemble_member_gradient_equals_single_job():
         _, gu, gm = one.gradient()
         assert torch.equal(gm[0], gm_e[q])
         assert torch.equal(gu, gu_e[q * n0:(q + 1) * n0])
+
+
+def test_first_order_gradient_matches_finite_differences():
+    cases.mixed_bc_array(max_step=40)
+    prep.config.x_order = 1
+    job = _vary_state(JobData.from_prep(), 5, 808.0)
+    prob = Problem([job], nsteps=10, C=4, mpt=[[2.5, 5.0e3, 260.0]])
+    _, _, gm = prob.gradient()
+    rows = _fd_rows_inflow(prob, gm)
+    short = Problem([job], nsteps=4, C=3, mpt=[[2.5, 5.0e3, 260.0]])
+    _, gu0s, _ = short.gradient()
+    _assert_rows(rows + _fd_rows_u0(short, gu0s))
