@@ -229,3 +229,45 @@ def ramp_ensemble(seed=11, n=64, max_step=300, nic=(32, 96), njk=(32, 64)):
         jobs.append(ramp(ni, nj, nk, max_step=max_step, seed=seed * 1000 + q, mach=mach, p=p, T=T,
                          title="ensemble member %d: Mach %.3f, p %.1f Pa, T %.2f K" % (q, mach, p, T)))
     return jobs
+
+
+def viscous_plate(nics=(10, 8), njc=4, nkcs=(8, 6), max_step=30, seed=5, x_order=2, TWall=350.0,
+                  title="viscous flow over a flat plate"):
+    """Mach 2 air over a plate on a 2x1x2 array of warped blocks: inflow and outflow in i,
+    a periodic exchange in j, the kminus plate a no-slip adiabatic wall (first block) and a
+    no-slip wall at TWall (second block), outflow on top; exchange on the internal
+    interfaces.  The initial state varies smoothly in density, energy and velocity."""
+    _common(max_step, title=title + " (seed %d)" % seed, cfl=0.4, dt_init=1.0e-8)
+    prep.config.viscous = True
+    prep.config.x_order = x_order
+    NI, NK = sum(nics), sum(nkcs)
+    g = box_grid(NI, njc, NK, L=(0.02, 0.004, 0.01))
+    rng = np.random.default_rng(seed)
+    jit = 0.1 / max(NI, njc, NK)
+    v = g.vertices
+    for a, Lx in ((v.x, 0.02), (v.y, 0.004), (v.z, 0.01)):
+        a[1:-1, 1:-1, 1:-1] += Lx * jit * rng.uniform(-1, 1, size=a[1:-1, 1:-1, 1:-1].shape)
+    fs = freestream(mach=2.0, p=1.0e4, T=300.0)
+    cc = cell_centres(g)
+    X, Y, Z = cc.x / 0.02, cc.y / 0.004, cc.z / 0.01
+    rho0 = fs.gas.p / (R_GAS * fs.gas.T)
+    rho = rho0 * (1.0 + 0.03 * np.sin(2 * np.pi * (X + 0.5 * Z) + 0.3 * seed))
+    T = fs.gas.T * (1.0 + 0.02 * np.cos(2 * np.pi * (Y + Z)))
+    st = _state_arrays(rho * R_GAS * T, T, fs.vel.x * (1.0 + 0.05 * np.sin(2 * np.pi * Z)), rho=rho)
+    st["vely"] = 0.02 * fs.vel.x * np.sin(2 * np.pi * (X + Y))
+    st["velz"] = 0.01 * fs.vel.x * np.cos(2 * np.pi * (X - Z))
+    io = np.cumsum([0] + list(nics))
+    ko = np.cumsum([0] + list(nkcs))
+    for k in range(2):
+        for i in range(2):
+            bcs = {"iminus": prep.InflowBC(fs) if i == 0 else prep.ExchangeBC(),
+                   "iplus": prep.OutflowBC() if i == 1 else prep.ExchangeBC(),
+                   "jminus": prep.ExchangeBC(), "jplus": prep.ExchangeBC(),
+                   "kminus": prep.ExchangeBC() if k == 1 else (
+                       prep.WallNoSlipAdiabaticBC() if i == 0 else prep.WallNoSlipFixedTBC(TWall)),
+                   "kplus": prep.OutflowBC() if k == 1 else prep.ExchangeBC()}
+            sub = g.subgrid(i0=io[i], j0=0, k0=ko[k], niv=nics[i] + 1, njv=njc + 1, nkv=nkcs[k] + 1)
+            init = {key: val[io[i]:io[i + 1], :, ko[k]:ko[k + 1]] if np.ndim(val) == 3 else val
+                    for key, val in st.items()}
+            prep.FluidBlock(i=i, j=0, k=k, grid=sub, initialState=init, bcs=bcs)
+    return JobData.from_prep()

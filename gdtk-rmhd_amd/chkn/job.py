@@ -23,8 +23,8 @@ def bc_code(tag):
 
 
 class JobBlock:
-    __slots__ = ("i", "j", "k", "nic", "njc", "nkc", "active", "bc", "bc_fs", "vertices", "iovars",
-                 "label")
+    __slots__ = ("i", "j", "k", "nic", "njc", "nkc", "active", "bc", "bc_fs", "bc_twall", "vertices",
+                 "iovars", "label")
 
     @property
     def ncells(self):
@@ -87,6 +87,9 @@ class JobData:
             jb.bc = [bc_code(b["bcs"][f]["tag"]) for f in FACE_NAMES]
             jb.bc_fs = [b["bcs"][f].get("flow_state_index", 0) if jb.bc[q] == 4 else 0
                         for q, f in enumerate(FACE_NAMES)]
+            # config.cu:463-467; NaN marks a TWall given as a function (not built).
+            jb.bc_twall = [(float("nan") if b["bcs"][f].get("form") == "fun" else float(b["bcs"][f].get("value", 0.0)))
+                           if jb.bc[q] == 2 else 0.0 for q, f in enumerate(FACE_NAMES)]
             jb.label = b.get("label", "")
             blocks.append(jb)
         self.blocks = blocks
@@ -139,6 +142,7 @@ class JobData:
             "cfl_values": dbl(self.cfl_values),
             "dt_plot_times": dbl(self.dt_plot_times),
             "dt_plot_values": dbl(self.dt_plot_values),
+            "bc_twall": dbl([getattr(b, "bc_twall", [0.0] * 6) for b in self.blocks]).ravel(),
         }
 
     @property

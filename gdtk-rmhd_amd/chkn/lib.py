@@ -27,7 +27,7 @@ class ChknDesc(ctypes.Structure):
                 ("n_cfl", ctypes.c_int), ("cfl_times", _D), ("cfl_values", _D),
                 ("n_dt_plot", ctypes.c_int), ("dt_plot_times", _D), ("dt_plot_values", _D),
                 ("t0", ctypes.c_double), ("start_tindx", ctypes.c_int), ("reacting", ctypes.c_int),
-                ("viscous", ctypes.c_int)]
+                ("viscous", ctypes.c_int), ("bc_twall", _D)]
 
 
 class ChknReport(ctypes.Structure):
@@ -148,7 +148,7 @@ def _desc(job, keep):
         setattr(d, key, _ptr(f[key], _I))
     d.n_flow_states = len(job.flow_states)
     for key in ("flow_states", "vertices", "iovars", "cfl_times", "cfl_values",
-                "dt_plot_times", "dt_plot_values"):
+                "dt_plot_times", "dt_plot_values", "bc_twall"):
         setattr(d, key, _ptr(f[key], _D))
     d.x_order, d.flux_calc, d.t_order = job.x_order, job.flux_calc, job.t_order
     d.cfl_count, d.max_step, d.print_count = job.cfl_count, job.max_step, job.print_count

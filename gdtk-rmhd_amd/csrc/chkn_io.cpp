@@ -117,6 +117,11 @@ JobConfig read_config_json(const std::string& path)
             blk.bc[f] = bc_code(bc["tag"].as_str());
             blk.bc_fs[f] = 0;
             if (blk.bc[f] == 4) blk.bc_fs[f] = bc["flow_state_index"].as_int();
+            blk.bc_twall[f] = 0.0;
+            if (blk.bc[f] == 2) {  // config.cu:463-467
+                blk.bc_twall[f] = bc["value"].as_num();
+                if (bc["form"].as_str() == "fun") blk.bc_twall[f] = std::nan("");
+            }
         }
         if (blk.i < 0 || blk.i >= c.nib || blk.j < 0 || blk.j >= c.njb || blk.k < 0 || blk.k >= c.nkb)
             throw std::runtime_error("block position outside the block array");

@@ -16,6 +16,7 @@ struct JobBlock {
     bool active = true;
     int bc[6] = {0, 0, 0, 0, 0, 0};
     int bc_fs[6] = {0, 0, 0, 0, 0, 0};
+    double bc_twall[6] = {0, 0, 0, 0, 0, 0};  // fixed-T walls; NaN when TWall is a function
     std::string label;
 };
 

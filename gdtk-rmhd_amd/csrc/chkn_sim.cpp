@@ -793,12 +793,14 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
         return set_error(nullptr, m == "Incorrect gas model data" ? CHKN_ERR_CONFIG : CHKN_ERR_IO, m);
     }
     std::vector<int> ijk, dims, active, bcc, bcf;
+    std::vector<double> twall;
     for (auto& b : c.blocks) {
         ijk.insert(ijk.end(), {b.i, b.j, b.k});
         dims.insert(dims.end(), {b.nic, b.njc, b.nkc});
         active.push_back(b.active ? 1 : 0);
         bcc.insert(bcc.end(), b.bc, b.bc + 6);
         bcf.insert(bcf.end(), b.bc_fs, b.bc_fs + 6);
+        twall.insert(twall.end(), b.bc_twall, b.bc_twall + 6);
     }
     chkn_desc d;
     std::memset(&d, 0, sizeof d);
@@ -815,6 +817,7 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
     d.dt_plot_values = c.dt_plot_values.data();
     d.t0 = t0; d.start_tindx = next_plot - 1;
     d.reacting = c.reacting; d.viscous = c.viscous;
+    d.bc_twall = twall.data();
     int rc = chkn_create(&d, out);
     if (rc == CHKN_OK) (*out)->inst[0].job = job;
     return rc;

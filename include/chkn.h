@@ -68,7 +68,10 @@ typedef struct {
     const double* dt_plot_values;
     double t0;               /* start time (times.data entry for the start index) */
     int start_tindx;
-    int reacting, viscous;   /* must be 0: not built */
+    int reacting;            /* must be 0: not built */
+    int viscous;             /* 1 = add viscous fluxes (block.cu:88-162) */
+    const double* bc_twall;  /* 6 per block: TWall of wall_no_slip_fixed_T sides (NaN: TWall
+                                from a function, not built); may be NULL without such walls */
 } chkn_desc;
 
 /* Progress of the march, filled by chkn_step and chkn_march. */

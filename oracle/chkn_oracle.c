@@ -57,7 +57,10 @@ typedef struct {
     V3 pos, n, t1, t2; double area;
     double F[6];
     int L1, L0, R0, R1;
-    FS fs; /* face-average state, face.cu:373 (kept for viscous use) */
+    FS fs; /* face-average state, face.cu:373 */
+    /* viscous: boundary code (-1 interior), cloud of cells and faces, LSQ weights */
+    int bcode, nc, nf, cc[2], fcl[8];
+    double TWall, wx[10], wy[10], wz[10];
 } Face;
 
 typedef struct {
@@ -65,6 +68,7 @@ typedef struct {
     int ncells, n0c[6], n1c[6], nghost[6], first_ghost[6], ntot;
     int nif, njf, nkf, nfaces;
     int bc[6], bcfs[6];
+    double twall[6];
     Cell* cells;
     Face* faces;
     V3* vtx;
@@ -78,7 +82,7 @@ struct orc_sim {
     Blk* b;
     int nfs;
     FS* inflow;
-    int x_order, flux_calc, cfl_count, max_step;
+    int x_order, flux_calc, cfl_count, max_step, viscous;
     double max_time, dt_init;
     int n_cfl;
     double* cfl_t;
@@ -487,6 +491,90 @@ static int line_cell(const Blk* b, int dir, int m, int a, int c)
     return cidx(b, a, c, m);
 }
 
+static int is_wall(int code)
+{
+    return code == ORC_WALL_SLIP || code == ORC_WALL_NOSLIP_ADIABATIC || code == ORC_WALL_NOSLIP_FIXED_T;
+}
+
+/* Clouds of cells and faces for the viscous gradients and the face's boundary code:
+ * block.cu:317-358 (i-faces), :418-459 (j-faces), :519-560 (k-faces).  A wall face uses
+ * only its interior cell; another boundary face also its first ghost cell. */
+static void set_cloud(Blk* b, Face* f, int side_bc, int c_minus, int c_plus, const int* fl, int nfl)
+{
+    f->bcode = side_bc;
+    f->TWall = 0.0;  /* the caller sets it on boundary faces */
+    f->nc = 0;
+    if (c_minus >= 0) f->cc[f->nc++] = c_minus;
+    if (c_plus >= 0) f->cc[f->nc++] = c_plus;
+    f->nf = nfl;
+    for (int q = 0; q < nfl; ++q) f->fcl[q] = fl[q];
+}
+
+static void configure_clouds(Blk* b)
+{
+    int nic = b->nic, njc = b->njc, nkc = b->nkc;
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < njc; ++j)
+            for (int i = 0; i <= nic; ++i) {
+                Face* f = &b->faces[ifi(b, i, j, k)];
+                if (i == 0) {
+                    int fl[4] = {jfi(b, i, j, k), jfi(b, i, j + 1, k), kfi(b, i, j, k), kfi(b, i, j, k + 1)};
+                    int g = is_wall(b->bc[IM]) ? -1 : ghost(b, IM, j, k, 0);
+                    set_cloud(b, f, b->bc[IM], g, cidx(b, i, j, k), fl, 4);
+                    f->TWall = b->twall[IM];
+                } else if (i == nic) {
+                    int fl[4] = {jfi(b, i - 1, j, k), jfi(b, i - 1, j + 1, k), kfi(b, i - 1, j, k), kfi(b, i - 1, j, k + 1)};
+                    int g = is_wall(b->bc[IP]) ? -1 : ghost(b, IP, j, k, 0);
+                    set_cloud(b, f, b->bc[IP], cidx(b, i - 1, j, k), g, fl, 4);
+                    f->TWall = b->twall[IP];
+                } else {
+                    int fl[8] = {jfi(b, i - 1, j, k), jfi(b, i - 1, j + 1, k), kfi(b, i - 1, j, k), kfi(b, i - 1, j, k + 1),
+                                 jfi(b, i, j, k), jfi(b, i, j + 1, k), kfi(b, i, j, k), kfi(b, i, j, k + 1)};
+                    set_cloud(b, f, -1, cidx(b, i - 1, j, k), cidx(b, i, j, k), fl, 8);
+                }
+            }
+    for (int k = 0; k < nkc; ++k)
+        for (int i = 0; i < nic; ++i)
+            for (int j = 0; j <= njc; ++j) {
+                Face* f = &b->faces[jfi(b, i, j, k)];
+                if (j == 0) {
+                    int fl[4] = {ifi(b, i, j, k), ifi(b, i + 1, j, k), kfi(b, i, j, k), kfi(b, i, j, k + 1)};
+                    int g = is_wall(b->bc[JM]) ? -1 : ghost(b, JM, i, k, 0);
+                    set_cloud(b, f, b->bc[JM], g, cidx(b, i, j, k), fl, 4);
+                    f->TWall = b->twall[JM];
+                } else if (j == njc) {
+                    int fl[4] = {ifi(b, i, j - 1, k), ifi(b, i + 1, j - 1, k), kfi(b, i, j - 1, k), kfi(b, i, j - 1, k + 1)};
+                    int g = is_wall(b->bc[JP]) ? -1 : ghost(b, JP, i, k, 0);
+                    set_cloud(b, f, b->bc[JP], cidx(b, i, j - 1, k), g, fl, 4);
+                    f->TWall = b->twall[JP];
+                } else {
+                    int fl[8] = {ifi(b, i, j - 1, k), ifi(b, i + 1, j - 1, k), kfi(b, i, j - 1, k), kfi(b, i, j - 1, k + 1),
+                                 ifi(b, i, j, k), ifi(b, i + 1, j, k), kfi(b, i, j, k), kfi(b, i, j, k + 1)};
+                    set_cloud(b, f, -1, cidx(b, i, j - 1, k), cidx(b, i, j, k), fl, 8);
+                }
+            }
+    for (int j = 0; j < njc; ++j)
+        for (int i = 0; i < nic; ++i)
+            for (int k = 0; k <= nkc; ++k) {
+                Face* f = &b->faces[kfi(b, i, j, k)];
+                if (k == 0) {
+                    int fl[4] = {ifi(b, i, j, k), ifi(b, i + 1, j, k), jfi(b, i, j, k), jfi(b, i, j + 1, k)};
+                    int g = is_wall(b->bc[KM]) ? -1 : ghost(b, KM, i, j, 0);
+                    set_cloud(b, f, b->bc[KM], g, cidx(b, i, j, k), fl, 4);
+                    f->TWall = b->twall[KM];
+                } else if (k == nkc) {
+                    int fl[4] = {ifi(b, i, j, k - 1), ifi(b, i + 1, j, k - 1), jfi(b, i, j, k - 1), jfi(b, i, j + 1, k - 1)};
+                    int g = is_wall(b->bc[KP]) ? -1 : ghost(b, KP, i, j, 0);
+                    set_cloud(b, f, b->bc[KP], cidx(b, i, j, k - 1), g, fl, 4);
+                    f->TWall = b->twall[KP];
+                } else {
+                    int fl[8] = {ifi(b, i, j, k - 1), ifi(b, i + 1, j, k - 1), jfi(b, i, j, k - 1), jfi(b, i, j + 1, k - 1),
+                                 ifi(b, i, j, k), ifi(b, i + 1, j, k), jfi(b, i, j, k), jfi(b, i, j + 1, k)};
+                    set_cloud(b, f, -1, cidx(b, i, j, k - 1), cidx(b, i, j, k), fl, 8);
+                }
+            }
+}
+
 static int configure(Blk* b) /* config.cu:152-185, block.cu:194-564 */
 {
     int nic = b->nic, njc = b->njc, nkc = b->nkc;
@@ -549,6 +637,7 @@ static int configure(Blk* b) /* config.cu:152-185, block.cu:194-564 */
                 f->L1 = line_cell(b, 2, k - 2, i, j); f->L0 = line_cell(b, 2, k - 1, i, j);
                 f->R0 = line_cell(b, 2, k, i, j);     f->R1 = line_cell(b, 2, k + 1, i, j);
             }
+    configure_clouds(b);
     return 0;
 }
 
@@ -767,6 +856,122 @@ static int block_update(Blk* b, int stage, double dt)
     return bad;
 }
 
+/* rsla.cu:53-72: 3x3 inverse by cofactors; returns -1 when |det| <= very_small_value. */
+static int m_inverse3(double c[3][3], double ci[3][3], double very_small_value)
+{
+    double det = c[0][0] * (c[1][1] * c[2][2] - c[1][2] * c[2][1])
+                 - c[0][1] * (c[1][0] * c[2][2] - c[1][2] * c[2][0])
+                 + c[0][2] * (c[1][0] * c[2][1] - c[1][1] * c[2][0]);
+    if (fabs(det) <= very_small_value) return -1;
+    double one_over_det = one / det;
+    ci[0][0] = (c[1][1] * c[2][2] - c[1][2] * c[2][1]) * one_over_det;
+    ci[0][1] = (c[0][2] * c[2][1] - c[0][1] * c[2][2]) * one_over_det;
+    ci[0][2] = (c[0][1] * c[1][2] - c[0][2] * c[1][1]) * one_over_det;
+    ci[1][0] = (c[1][2] * c[2][0] - c[1][0] * c[2][2]) * one_over_det;
+    ci[1][1] = (c[0][0] * c[2][2] - c[0][2] * c[2][0]) * one_over_det;
+    ci[1][2] = (c[0][2] * c[1][0] - c[0][0] * c[1][2]) * one_over_det;
+    ci[2][0] = (c[1][0] * c[2][1] - c[1][1] * c[2][0]) * one_over_det;
+    ci[2][1] = (c[0][1] * c[2][0] - c[0][0] * c[2][1]) * one_over_det;
+    ci[2][2] = (c[0][0] * c[1][1] - c[0][1] * c[1][0]) * one_over_det;
+    return 0;
+}
+
+/* block.cu:28-86: least-squares weights of the face's cloud, face centre as reference.
+ * A singular normal matrix leaves the weights at zero (the reference reports it on stderr
+ * and carries on: its failure counter is never incremented). */
+static void setup_lsq(Blk* b, Face* f)
+{
+    V3 pos[10];
+    int n = 0;
+    for (int q = 0; q < f->nc; ++q) pos[n++] = b->cells[f->cc[q]].pos;
+    for (int q = 0; q < f->nf; ++q) pos[n++] = b->faces[f->fcl[q]].pos;
+    double w2[10], dx[10], dy[10], dz[10];
+    double x0 = f->pos.x, y0 = f->pos.y, z0 = f->pos.z;
+    for (int i = 0; i < n; ++i) {
+        dx[i] = pos[i].x - x0;
+        dy[i] = pos[i].y - y0;
+        dz[i] = pos[i].z - z0;
+        w2[i] = 1.0 / (dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i]);
+    }
+    double xx = zero, xy = zero, xz = zero, yy = zero, yz = zero, zz = zero;
+    for (int i = 0; i < n; ++i) {
+        xx += w2[i] * dx[i] * dx[i];
+        xy += w2[i] * dx[i] * dy[i];
+        xz += w2[i] * dx[i] * dz[i];
+        yy += w2[i] * dy[i] * dy[i];
+        yz += w2[i] * dy[i] * dz[i];
+        zz += w2[i] * dz[i] * dz[i];
+    }
+    double m[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}}, mi[3][3];
+    for (int i = 0; i < 10; ++i) f->wx[i] = f->wy[i] = f->wz[i] = zero;
+    if (m_inverse3(m, mi, 1.0e-16)) return;
+    for (int i = 0; i < n; ++i) {
+        f->wx[i] = w2[i] * (mi[0][0] * dx[i] + mi[0][1] * dy[i] + mi[0][2] * dz[i]);
+        f->wy[i] = w2[i] * (mi[1][0] * dx[i] + mi[1][1] * dy[i] + mi[1][2] * dz[i]);
+        f->wz[i] = w2[i] * (mi[2][0] * dx[i] + mi[2][1] * dy[i] + mi[2][2] * dz[i]);
+    }
+}
+
+/* bcs.cu:777-797 (the inflow-function case is not built). */
+static void viscous_bc(Face* f)
+{
+    if (f->bcode == ORC_WALL_NOSLIP_ADIABATIC) {
+        f->fs.vel.x = 0.0; f->fs.vel.y = 0.0; f->fs.vel.z = 0.0;
+    } else if (f->bcode == ORC_WALL_NOSLIP_FIXED_T) {
+        f->fs.vel.x = 0.0; f->fs.vel.y = 0.0; f->fs.vel.z = 0.0;
+        f->fs.T = f->TWall;
+    }
+}
+
+/* gas.cu:95-100, Sutherland's law for air. */
+static void trans_coeffs(double T, double* mu, double* k)
+{
+    const double mu_ref = 1.716e-5, T_ref = 273.0, S = 111.0, Prandtl = 0.72;
+    const double Cp = GAS_CV + GAS_R;
+    *mu = mu_ref * sqrt(T / T_ref) * (T / T_ref) * (T_ref + S) / (T + S);
+    *k = Cp * *mu / Prandtl;
+}
+
+/* block.cu:88-162 */
+static void add_viscous(Blk* b, Face* f)
+{
+    FS cl[10];
+    int n = 0;
+    for (int q = 0; q < f->nc; ++q) cl[n++] = b->cells[f->cc[q]].fs;
+    for (int q = 0; q < f->nf; ++q) cl[n++] = b->faces[f->fcl[q]].fs;
+    FS fs0 = f->fs;
+    V3 gT = {zero, zero, zero}, gx = {zero, zero, zero}, gy = {zero, zero, zero}, gz = {zero, zero, zero};
+    double dq = zero;
+    for (int i = 0; i < n; ++i) {
+        double wx = f->wx[i], wy = f->wy[i], wz = f->wz[i];
+        dq = cl[i].T - fs0.T;
+        gT.x += wx * dq; gT.y += wy * dq; gT.z += wz * dq;
+        dq = cl[i].vel.x - fs0.vel.x;
+        gx.x += wx * dq; gx.y += wy * dq; gx.z += wz * dq;
+        dq = cl[i].vel.y - fs0.vel.y;
+        gy.x += wx * dq; gy.y += wy * dq; gy.z += wz * dq;
+        dq = cl[i].vel.z - fs0.vel.z;
+        gz.x += wx * dq; gz.y += wy * dq; gz.z += wz * dq;
+    }
+    double mu, k;
+    trans_coeffs(fs0.T, &mu, &k);
+    double lmbda = -two / three * mu;
+    double tau_xx = two * mu * gx.x + lmbda * (gx.x + gy.y + gz.z);
+    double tau_yy = two * mu * gy.y + lmbda * (gx.x + gy.y + gz.z);
+    double tau_zz = two * mu * gz.z + lmbda * (gx.x + gy.y + gz.z);
+    double tau_xy = mu * (gx.y + gy.x);
+    double tau_xz = mu * (gx.z + gz.x);
+    double tau_yz = mu * (gy.z + gz.y);
+    double qx = k * gT.x, qy = k * gT.y, qz = k * gT.z;
+    V3 nn = f->n;
+    f->F[1] -= tau_xx * nn.x + tau_xy * nn.y + tau_xz * nn.z;
+    f->F[2] -= tau_xy * nn.x + tau_yy * nn.y + tau_yz * nn.z;
+    f->F[3] -= tau_xz * nn.x + tau_yz * nn.y + tau_zz * nn.z;
+    f->F[4] -= (tau_xx * fs0.vel.x + tau_xy * fs0.vel.y + tau_xz * fs0.vel.z + qx) * nn.x
+               + (tau_xy * fs0.vel.x + tau_yy * fs0.vel.y + tau_yz * fs0.vel.z + qy) * nn.y
+               + (tau_xz * fs0.vel.x + tau_yz * fs0.vel.y + tau_zz * fs0.vel.z + qz) * nn.z;
+}
+
 /* config.cu:263-269 */
 static double schedule_value(const orc_sim* s, double t)
 {
@@ -798,6 +1003,7 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
         f->vel.x = flow_states[6 * i + 3]; f->vel.y = flow_states[6 * i + 4]; f->vel.z = flow_states[6 * i + 5];
     }
     s->x_order = p->x_order; s->flux_calc = p->flux_calc; s->cfl_count = p->cfl_count;
+    s->viscous = p->viscous;
     s->max_step = p->max_step; s->max_time = p->max_time; s->dt_init = p->dt_init;
     s->n_cfl = p->n_cfl;
     s->cfl_t = (double*)malloc(sizeof(double) * (size_t)p->n_cfl);
@@ -811,7 +1017,10 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
         b->bi = blk_ijk[3 * ib]; b->bj = blk_ijk[3 * ib + 1]; b->bk = blk_ijk[3 * ib + 2];
         b->nic = blk_dims[3 * ib]; b->njc = blk_dims[3 * ib + 1]; b->nkc = blk_dims[3 * ib + 2];
         b->active = active[ib];
-        for (int f = 0; f < 6; ++f) { b->bc[f] = bc_codes[6 * ib + f]; b->bcfs[f] = bc_fs[6 * ib + f]; }
+        for (int f = 0; f < 6; ++f) {
+            b->bc[f] = bc_codes[6 * ib + f]; b->bcfs[f] = bc_fs[6 * ib + f];
+            b->twall[f] = p->bc_twall ? p->bc_twall[6 * ib + f] : 0.0;
+        }
         if (configure(b)) { orc_destroy(s); return NULL; }
         size_t nv = (size_t)(b->nic + 1) * (b->njc + 1) * (b->nkc + 1);
         for (size_t v = 0; v < nv; ++v) {
@@ -832,6 +1041,8 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
         }
         ooff += 13 * (size_t)n;
         compute_geometry(b);
+        if (s->viscous && b->active) /* simulate.cu:201 */
+            for (int fi = 0; fi < b->nfaces; ++fi) setup_lsq(b, &b->faces[fi]);
     }
     /* Exchange partners must exist (a missing block would be index -1 in the reference). */
     for (int ib = 0; ib < nblocks; ++ib)
@@ -896,6 +1107,10 @@ int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
                 Blk* b = &s->b[ib];
                 if (!b->active) continue;
                 block_fluxes(s, b);
+                if (s->viscous) { /* simulate.cu:235-238 */
+                    for (int fi = 0; fi < b->nfaces; ++fi) viscous_bc(&b->faces[fi]);
+                    for (int fi = 0; fi < b->nfaces; ++fi) add_viscous(b, &b->faces[fi]);
+                }
                 bad += block_update(b, stage, s->dt);
             }
             if (bad > 0) {

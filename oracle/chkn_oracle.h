@@ -32,6 +32,8 @@ typedef struct {
     const double* cfl_times;
     const double* cfl_values;
     double t0;            /* starting time (times.data) */
+    int viscous;          /* add viscous fluxes, block.cu:88-162 */
+    const double* bc_twall; /* 6 per block, TWall of fixed-T walls (may be NULL) */
 } orc_params;
 
 typedef struct orc_sim orc_sim;

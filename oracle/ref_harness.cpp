@@ -99,6 +99,10 @@ static int stage(int which)
         Block& blk = fluidBlocks[ib];
         if (!cfg.active) continue;
         blk.calculate_convective_fluxes(Config::flux_calc, Config::x_order);
+        if (Config::viscous) { // simulate.cu:235-238
+            apply_viscous_boundary_conditions(blk, cfg);
+            blk.add_viscous_fluxes();
+        }
         if (which == 1) bad[ib] = blk.update_stage_1(cfg, H::dt);
         else if (which == 2) bad[ib] = blk.update_stage_2(cfg, H::dt);
         else bad[ib] = blk.update_stage_3(cfg, H::dt);
@@ -117,7 +121,10 @@ static int run(const char* job, int tindx, int nsteps, const char* dtlog)
     int nb = Config::nFluidBlocks;
 #pragma omp parallel for
     for (int ib = 0; ib < nb; ++ib)
-        if (blk_configs[ib].active) fluidBlocks[ib].encodeConserved(blk_configs[ib], 0);
+        if (blk_configs[ib].active) {
+            fluidBlocks[ib].encodeConserved(blk_configs[ib], 0);
+            if (Config::viscous) fluidBlocks[ib].setup_LSQ_arrays(); // simulate.cu:201
+        }
     FILE* fdt = (dtlog && std::string(dtlog) != "-") ? std::fopen(dtlog, "w") : nullptr;
     auto t_start = std::chrono::steady_clock::now();
     int done = 0;

@@ -10,7 +10,7 @@ that script) in this container only.  Outputs, all data:
   step_<name>.npz    small jobs marched by the reference driver: dt per step and the final
                      IO variables of every block, plus a digest of the job's inputs
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [step fixture name ...]   (default: all)
 """
 import hashlib
 import os
@@ -110,7 +110,7 @@ def job_digest(job):
     for key in sorted(f):
         h.update(key.encode())
         h.update(np.ascontiguousarray(f[key]).tobytes())
-    for key in ("x_order", "cfl_count", "max_step", "max_time", "dt_init"):
+    for key in ("x_order", "cfl_count", "max_step", "max_time", "dt_init", "viscous"):
         h.update(repr(getattr(job, key)).encode())
     return h.hexdigest()
 
@@ -122,6 +122,8 @@ STEP_JOBS = {
     "ramp_cfl": (lambda: _cfl_limited_ramp(), 40),
     "xorder1": (lambda: _first_order(), 15),
     "tiny_n2": (lambda: _two_cell_block(), 20),
+    "viscous": (lambda: cases.viscous_plate(max_step=30), 30),
+    "viscous_x1": (lambda: cases.viscous_plate(max_step=20, x_order=1, seed=8), 20),
 }
 
 
@@ -156,9 +158,13 @@ def step_fixture(name, builder, steps):
                         digest=job_digest(jd), **{"io%d" % b: f for b, f in enumerate(fields)})
 
 
-def main():
+def main(names):
     if not os.path.exists(REF):
         sys.exit("build oracle/_ref first (oracle/build_ref.sh)")
+    if names:  # only the named step fixtures
+        for name in names:
+            step_fixture(name, *STEP_JOBS[name])
+        return
     rng = np.random.default_rng(20240601)
     rec = flux_vectors(rng)
     F = run_ref("flux", rec).reshape(-1, 6)
@@ -172,4 +178,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -19,7 +19,8 @@ class OrcParams(ctypes.Structure):
     _fields_ = [("x_order", ctypes.c_int), ("flux_calc", ctypes.c_int), ("cfl_count", ctypes.c_int),
                 ("max_step", ctypes.c_int), ("max_time", ctypes.c_double),
                 ("dt_init", ctypes.c_double), ("n_cfl", ctypes.c_int), ("cfl_times", _D),
-                ("cfl_values", _D), ("t0", ctypes.c_double)]
+                ("cfl_values", _D), ("t0", ctypes.c_double), ("viscous", ctypes.c_int),
+                ("bc_twall", _D)]
 
 
 _lib = None
@@ -70,7 +71,7 @@ class Oracle:
         self._keep = f
         prm = OrcParams(job.x_order, job.flux_calc, job.cfl_count, job.max_step, job.max_time,
                         job.dt_init, len(job.cfl_times), _p(f["cfl_times"], _D),
-                        _p(f["cfl_values"], _D), job.t0)
+                        _p(f["cfl_values"], _D), job.t0, int(job.viscous), _p(f["bc_twall"], _D))
         self._prm = prm
         h = lib.orc_create(len(job.blocks), job.nib, job.njb, job.nkb, _p(f["blk_ids"], _I),
                            _p(f["blk_ijk"], _I), _p(f["blk_dims"], _I), _p(f["active"], _I),

@@ -61,6 +61,8 @@ STEP_JOBS = {
     "ramp_cfl": (_cfl_ramp, 40),
     "xorder1": (_first_order, 15),
     "tiny_n2": (_two_cell, 20),
+    "viscous": (lambda: cases.viscous_plate(max_step=30), 30),
+    "viscous_x1": (lambda: cases.viscous_plate(max_step=20, x_order=1, seed=8), 20),
 }
 
 
@@ -71,7 +73,7 @@ def job_digest(job):
     for key in sorted(f):
         h.update(key.encode())
         h.update(np.ascontiguousarray(f[key]).tobytes())
-    for key in ("x_order", "cfl_count", "max_step", "max_time", "dt_init"):
+    for key in ("x_order", "cfl_count", "max_step", "max_time", "dt_init", "viscous"):
         h.update(repr(getattr(job, key)).encode())
     return h.hexdigest()
 
