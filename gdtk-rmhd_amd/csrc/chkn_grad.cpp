@@ -209,8 +209,7 @@ void queue_recompute(chkn_sim* s, int j, int K, int t, hipStream_t st)
         for (int stage = 1; stage <= 3; ++stage) {
             a.win = 3 * q + stage - 1;
             a.wout = 3 * q + stage;
-            HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, false,
-                                s->opts.persistent != 0, s->grid_limit, a, st));
+            launch_stage_form(s, a, stage, false, false, st);
         }
     }
     HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, s->ctl_bytes, hipMemcpyDeviceToHost, st));
@@ -279,6 +278,10 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
     if (s->failed) return set_error(s, CHKN_ERR_STATE, "simulation is in a failed state");
     if (nsteps <= 0) return set_error(s, CHKN_ERR_ARG, "nsteps must be positive");
     if (C <= 0 || C > kMaxSeg) return set_error(s, CHKN_ERR_ARG, "checkpoint interval must be in 1..64");
+    if (s->viscous) return set_error(s, CHKN_ERR_ARG, "the adjoint of the viscous fluxes is not built");
+    // The unroll is a function of the conserved state: gas states follow from it, also in
+    // the first stage (the recomputation from checkpoints does the same).
+    s->fresh = false;
     LiftLimits lift(s);
     GradState& g = grad_state(s);
     const int N = (int)s->inst.size();
@@ -554,6 +557,7 @@ int chkn_set_inflow_state(chkn_sim* sim, int state, const double* w6)
     return guarded(sim, [&] {
         std::copy(w6, w6 + 6, sim->inflow_w.begin() + 6 * (size_t)state);
         HIPCHK(hipMemcpy(sim->d_inflow + 6 * (size_t)state, w6, sizeof(double) * 6, hipMemcpyHostToDevice));
+        split_set_inflow(sim, state);
         if (sim->grad) sim->grad->valid = false;
         return CHKN_OK;
     });

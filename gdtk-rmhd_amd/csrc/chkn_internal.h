@@ -12,6 +12,13 @@
 //     j-faces: nic x (njc+1) x nkc, index i + nic*(j + (njc+1)*k)
 //     k-faces: nic x njc x (nkc+1), index i + nic*(j + njc*k)
 //   volinv[n] = 1/volume, lengths[3][n] (iLength, jLength, kLength)
+// Face-list stage (viscous jobs; first-order jobs whose file or inflow gas states are not
+// reproduced by the equation of state), faces numbered i-faces, then j-faces (+nfi), then
+// k-faces (+nfi+nfj), nft faces in all:
+//   F[6][nft]    face fluxes (not area-weighted), viscous part subtracted in place
+//   FS[4][nft]   face T, vx, vy, vz after the viscous boundary conditions
+//   lsq[3][10][nft] least-squares gradient weights (x, y, z) of the face's cloud
+//   gas[3][n]    p, T, a as read from the flow file
 // Inactive blocks hold one W slot (wstride 0: every slot is the file state) and the
 // geometry needed by exchange neighbours.
 #pragma once
@@ -46,6 +53,12 @@ struct BlockDev {
     int icol_side[6];      // offset of each inflow side's lines from icol, or -1
     double* G;             // face stencil adjoints: [24][nfi], then [24][nfj], then [24][nfk]
     double* adj;           // A0 (adjoint of U at the step boundary), P, D: [3][6][n]
+    // face-list stage
+    double* F;
+    double* FS;
+    const double* lsq;
+    const double* gas;     // null when the block's file state is not needed
+    double twall[6];       // TWall of each wall_no_slip_fixed_T side
 };
 
 struct Tile {
@@ -112,6 +125,28 @@ hipError_t launch_decode_all(const BlockDev* blocks, const CellChunk* cells, int
 // Point evaluation of face_flux_vjp (test hook): in = per point 24 prims, 9 frame, 6 Fbar;
 // out = 24 stencil adjoints.
 hipError_t launch_flux_vjp_points(const double* in, double* out, int n, int x_order, hipStream_t s);
+
+// Face-list form of one stage: every face of every active block in one pass (convective
+// flux, and for viscous jobs the face state), then the viscous pass (gradients and viscous
+// flux, viscous jobs only), then a per-cell pass (flux divergence, RK stage, decode).  Cell
+// gas states (p, T, a) come from the flow file for inactive blocks, and for active blocks
+// while file_gas is set (first stage after the state was read); inflow ghost cells carry the
+// configured p, T, a (inflow_gas, 3 per flow state).
+struct SplitArgs {
+    const BlockDev* blocks;
+    const double* inflow;
+    const double* inflow_gas;
+    const FaceChunk* faces;  // at most 256 faces each
+    const CellChunk* cells;  // at most 256 cells each
+    StepCtl ctl;
+    int win, wout;
+    int file_gas;
+};
+hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order, bool viscous,
+                              hipStream_t s);
+hipError_t launch_split_viscous(const SplitArgs& a, int nface_chunks, hipStream_t s);
+hipError_t launch_split_update(const SplitArgs& a, int ncell_chunks, int stage, bool dt_reduce,
+                               hipStream_t s);
 
 // Launch one stage over tiles [tile_base, tile_base + ntiles) of the table.
 // stage in 1..3; x_order 1 or 2; tile shape (ti, tj) must be a built variant.

@@ -30,15 +30,6 @@ __device__ __forceinline__ Prim stencil_prim(const StageArgs& A, const BlockDev*
     return ghost_prim(A.blocks, A.inflow, A.win, B, dir, m, a, c);
 }
 
-__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
-{
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(v, off, 64);
-        v = (o < v) ? o : v;
-    }
-    return v;
-}
-
 // LDS layout of a tile: the k-plane of primitive states with a 2-cell halo in i and j
 // (rectangle of (TI+4) x (TJ+4) positions; only the cross-shaped part is filled), then the
 // area-weighted fluxes of the plane's i-faces and j-faces.
@@ -214,48 +205,11 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
                 s = s - AFkp[q];
                 d[q] = vinv * s;
             }
-            double* __restrict__ U = B->U;
-            double* __restrict__ Lr = B->L;
-            double Un[6];
-            if (STAGE == 1) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    Un[q] = U[q * n + cidx] + dt * d[q];
-                    Lr[q * n + cidx] = d[q];
-                }
-            } else if (STAGE == 2) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const double l0 = Lr[q * n + cidx];
-                    Un[q] = U[q * n + cidx] + 0.25 * dt * (l0 + d[q]);
-                    Lr[q * n + cidx] = kOneSixth * l0 + kOneSixth * d[q];
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    Un[q] = U[q * n + cidx] + dt * (Lr[q * n + cidx] + kFourSixth * d[q]);
-                    U[q * n + cidx] = Un[q];
-                }
-            }
             Prim w;
-            const int flag = decode(Un, w);
+            const int flag = stage_update<STAGE>(B, n, cidx, dt, d, A.wout, w);
             bad += flag;
-            double* __restrict__ Wout = B->W + A.wout * B->wstride;
-            Wout[cidx] = w.rho;
-            Wout[n + cidx] = w.e;
-            Wout[2 * n + cidx] = w.YB;
-            Wout[3 * n + cidx] = w.vx;
-            Wout[4 * n + cidx] = w.vy;
-            Wout[5 * n + cidx] = w.vz;
             if (DTRED && !flag) {
-                const double T = eos_T(w.e, w.YB);
-                const double sa = eos_a(T);
-                const Frame fi_ = ld_frame(B->gi, (int)B->nfi, i + (nic + 1) * (j + njc * k));
-                const Frame fj_ = ld_frame(B->gj, (int)B->nfj, i + nic * (j + (njc + 1) * k));
-                const Frame fk_ = ld_frame(B->gk, (int)B->nfk, cidx);
-                const double sig = local_signal(w, sa, B->len[cidx], B->len[n + cidx],
-                                                B->len[2 * n + cidx], fi_, fj_, fk_);
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(sig);
+                const unsigned long long bits = signal_bits(B, n, i, j, k, cidx, w);
                 sigmin = bits < sigmin ? bits : sigmin;
             }
 #pragma unroll
@@ -336,13 +290,7 @@ __global__ __launch_bounds__(256) void signal_min_kernel(const BlockDev* B,
     if (c < n) {
         const int nic = B->nic, njc = B->njc;
         const int i = c % nic, j = (c / nic) % njc, k = c / (nic * njc);
-        const Prim w = ld_prim(B->W, n, c);
-        const double sa = eos_a(eos_T(w.e, w.YB));
-        const Frame fi_ = ld_frame(B->gi, (int)B->nfi, i + (nic + 1) * (j + njc * k));
-        const Frame fj_ = ld_frame(B->gj, (int)B->nfj, i + nic * (j + (njc + 1) * k));
-        const Frame fk_ = ld_frame(B->gk, (int)B->nfk, c);
-        const double sig = local_signal(w, sa, B->len[c], B->len[n + c], B->len[2 * n + c], fi_, fj_, fk_);
-        v = (unsigned long long)__double_as_longlong(sig);
+        v = signal_bits(B, n, i, j, k, c, ld_prim(B->W, n, c));
     }
     v = wave_min_u64(v);
     if ((threadIdx.x & 63) == 0 && v != ~0ull) atomicMin(sigmin_bits, v);

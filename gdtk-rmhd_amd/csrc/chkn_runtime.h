@@ -51,6 +51,13 @@ struct HostBlock {
     double* volinv = nullptr;
     double* len = nullptr;
     int tile_first = 0, tile_count = 0;
+    // face-list stage (viscous jobs; first-order jobs with verbatim gas states)
+    double twall[6] = {0, 0, 0, 0, 0, 0};
+    bool gas_differs = false;  // some cell's file p, T, a differ from the equation of state
+    double* F = nullptr;       // 6 x nft face fluxes
+    double* FS = nullptr;      // 4 x nft face T, vx, vy, vz (viscous)
+    double* lsq = nullptr;     // 30 x nft least-squares weights (viscous)
+    double* gas = nullptr;     // 3 x n file p, T, a (only when gas_differs)
     // adjoint (allocated by the first gradient call)
     double* tape = nullptr;    // (3C+1) W slots of the segment being reversed
     double* tape2 = nullptr;   // second tape: the next segment is recomputed into it meanwhile
@@ -103,7 +110,13 @@ struct GradState {
 struct GraphKey {
     int K;
     bool dtred;
-    bool operator<(const GraphKey& o) const { return K != o.K ? K < o.K : dtred < o.dtred; }
+    bool fresh;  // the first stage reads the file gas states
+    bool operator<(const GraphKey& o) const
+    {
+        if (K != o.K) return K < o.K;
+        if (dtred != o.dtred) return dtred < o.dtred;
+        return fresh < o.fresh;
+    }
 };
 
 }  // namespace rt
@@ -120,6 +133,11 @@ struct chkn_sim {
     std::vector<Instance> inst;
     std::vector<double> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz (all instances)
     int x_order = 2, flux_calc = 0;
+    bool viscous = false;
+    // Gas states (p, T, a) of the inflow states as configured (update_from_pT), and whether
+    // the equation of state of their (rho, e) gives other values.
+    std::vector<double> inflow_gas;
+    std::vector<char> inflow_differs;
     int gstep = 0;       // lockstep step counter shared by the instances
     bool fresh = true;   // no step taken since the state was set from IO variables
     bool failed = false;
@@ -129,6 +147,11 @@ struct chkn_sim {
     BlockDev* d_blocks = nullptr;
     Tile* d_tiles = nullptr;
     double* d_inflow = nullptr;
+    double* d_inflow_gas = nullptr;
+    // face-list stage work tables
+    int nsplit_faces = 0, nsplit_cells = 0;
+    chkn::FaceChunk* d_split_faces = nullptr;
+    chkn::CellChunk* d_split_cells = nullptr;
     // Control area, kMaxSeg step slots: dt[S][N], sigmin[S][N], run[S][N], bad[S][3].
     char* d_ctl = nullptr;
     char* h_ctl = nullptr;
@@ -166,6 +189,27 @@ void run_segment(chkn_sim* s, int K, bool dtred_last);
 int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report* rep);
 void upload_blocks(chkn_sim* s);
 void free_grad(chkn_sim* s);
+// One stage of every active block: fused tile kernel, or the face-list form where the job
+// needs it (file_gas: the first stage after the state was read from IO variables).
+void launch_stage_form(chkn_sim* s, const StageArgs& a, int stage, bool dtred, bool file_gas,
+                       hipStream_t st);
+long long launches_per_stage(chkn_sim* s, bool file_gas);
+// Whether the first stage after the state was read must use the file gas states.
+bool fresh_needs_file_gas(const chkn_sim* s);
+
+// Face-list stage set-up (chkn_split.cpp).
+// Per block after its geometry: face fluxes, and for viscous jobs face states and the
+// least-squares weights from cell centres and face centres fpos ([3][nft]).
+void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vector<double>& fpos);
+// The file gas states of block b from its IO variables (re-run when they are replaced).
+void split_block_gas(chkn_sim* s, HostBlock& b, BlockDev& bd);
+// Inflow gas states from the configured p, T, YB of every flow state.
+void split_inflow_gas(chkn_sim* s, const std::vector<double>& pTYB);
+// Work tables over every active block, after all blocks are set up.
+void split_tables(chkn_sim* s);
+void split_free(chkn_sim* s);
+// An inflow state replaced by (rho, e, YB, v): its gas state follows from the equation of state.
+void split_set_inflow(chkn_sim* s, int state);
 
 template <typename F>
 int guarded(chkn_sim* s, F&& fn)

@@ -66,6 +66,7 @@ void dfree(void* p)
 void free_device(chkn_sim* s)
 {
     free_grad(s);
+    split_free(s);
     for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
     s->graphs.clear();
     for (auto& b : s->blocks) {
@@ -89,8 +90,10 @@ void upload_blocks(chkn_sim* s)
 }
 
 // Geometry pre-pass of one block (block.cu:633-652) into device-layout host arrays.
+// fpos, when given, receives the face centres [3][nft] (i-, then j-, then k-faces).
 void block_geometry(HostBlock& b, const double* vtx, std::vector<double>& gi, std::vector<double>& gj,
-                    std::vector<double>& gk, std::vector<double>& volinv, std::vector<double>& len)
+                    std::vector<double>& gk, std::vector<double>& volinv, std::vector<double>& len,
+                    std::vector<double>* fpos)
 {
     const int nic = b.nic, njc = b.njc, nkc = b.nkc;
     auto V = [&](int i, int j, int k) {
@@ -124,6 +127,12 @@ void block_geometry(HostBlock& b, const double* vtx, std::vector<double>& gi, st
     };
     size_t nfi = (size_t)(nic + 1) * njc * nkc, nfj = (size_t)nic * (njc + 1) * nkc,
            nfk = (size_t)nic * njc * (nkc + 1);
+    const size_t nft = nfi + nfj + nfk;
+    if (fpos) fpos->assign(3 * nft, 0.0);
+    auto centre = [&](size_t ff, const geom::Quad& q) {
+        if (!fpos) return;
+        (*fpos)[ff] = q.c.x; (*fpos)[nft + ff] = q.c.y; (*fpos)[2 * nft + ff] = q.c.z;
+    };
     gi.assign(kFrameN * nfi, 0.0);
     gj.assign(kFrameN * nfj, 0.0);
     gk.assign(kFrameN * nfk, 0.0);
@@ -131,21 +140,30 @@ void block_geometry(HostBlock& b, const double* vtx, std::vector<double>& gi, st
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < nkc; ++k)
         for (int j = 0; j < njc; ++j)
-            for (int i = 0; i <= nic; ++i)
-                put(gi, nfi, (size_t)i + (size_t)(nic + 1) * (j + (size_t)njc * k),
-                    geom::quad(V(i, j, k), V(i, j + 1, k), V(i, j + 1, k + 1), V(i, j, k + 1)));
+            for (int i = 0; i <= nic; ++i) {
+                const size_t f = (size_t)i + (size_t)(nic + 1) * (j + (size_t)njc * k);
+                const geom::Quad q = geom::quad(V(i, j, k), V(i, j + 1, k), V(i, j + 1, k + 1), V(i, j, k + 1));
+                put(gi, nfi, f, q);
+                centre(f, q);
+            }
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < nkc; ++k)
         for (int j = 0; j <= njc; ++j)
-            for (int i = 0; i < nic; ++i)
-                put(gj, nfj, (size_t)i + (size_t)nic * (j + (size_t)(njc + 1) * k),
-                    geom::quad(V(i + 1, j, k), V(i, j, k), V(i, j, k + 1), V(i + 1, j, k + 1)));
+            for (int i = 0; i < nic; ++i) {
+                const size_t f = (size_t)i + (size_t)nic * (j + (size_t)(njc + 1) * k);
+                const geom::Quad q = geom::quad(V(i + 1, j, k), V(i, j, k), V(i, j, k + 1), V(i + 1, j, k + 1));
+                put(gj, nfj, f, q);
+                centre(nfi + f, q);
+            }
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k <= nkc; ++k)
         for (int j = 0; j < njc; ++j)
-            for (int i = 0; i < nic; ++i)
-                put(gk, nfk, (size_t)i + (size_t)nic * (j + (size_t)njc * k),
-                    geom::quad(V(i, j, k), V(i + 1, j, k), V(i + 1, j + 1, k), V(i, j + 1, k)));
+            for (int i = 0; i < nic; ++i) {
+                const size_t f = (size_t)i + (size_t)nic * (j + (size_t)njc * k);
+                const geom::Quad q = geom::quad(V(i, j, k), V(i + 1, j, k), V(i + 1, j + 1, k), V(i, j + 1, k));
+                put(gk, nfk, f, q);
+                centre(nfi + nfj + f, q);
+            }
 }
 
 // Primitive stage state from IO variables (rho, e, YB, vel from the flow file).
@@ -249,7 +267,6 @@ void check_desc(const chkn_desc* d)
 {
     if (d->nblocks <= 0) throw std::invalid_argument("no blocks");
     if (d->reacting) throw std::invalid_argument("the reacting-gas model is not built");
-    if (d->viscous) throw std::invalid_argument("viscous fluxes are not built");
     if (d->flux_calc != 0) throw std::invalid_argument("flux_calc sbp_asf is not built");
     if (d->x_order != 1 && d->x_order != 2) throw std::invalid_argument("x_order must be 1 or 2");
     if (d->n_cfl <= 0) throw std::invalid_argument("need at least one cfl value");
@@ -264,8 +281,10 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     for (int q = 0; q < ninst; ++q) check_desc(&descs[q]);
     s->x_order = descs[0].x_order;
     s->flux_calc = descs[0].flux_calc;
+    s->viscous = descs[0].viscous != 0;
     for (int q = 1; q < ninst; ++q) {
         if (descs[q].x_order != s->x_order) throw std::invalid_argument("instances must share x_order");
+        if ((descs[q].viscous != 0) != s->viscous) throw std::invalid_argument("instances must share viscous");
         if (descs[q].cfl_count != descs[0].cfl_count) throw std::invalid_argument("instances must share cfl_count");
     }
     s->inst.resize(ninst);
@@ -273,6 +292,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     for (int q = 0; q < ninst; ++q) nblocks_total += descs[q].nblocks;
     s->blocks.resize(nblocks_total);
     int fs_base = 0, blk_base = 0;
+    std::vector<double> inflow_pTYB;  // configured p, T, YB of every flow state
     for (int qi = 0; qi < ninst; ++qi) {
         const chkn_desc* d = &descs[qi];
         Instance& I = s->inst[qi];
@@ -299,6 +319,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             double rho = p / (T * kR);      // gas.cu:124
             double w[6] = {rho, e, YB, f[3], f[4], f[5]};
             s->inflow_w.insert(s->inflow_w.end(), w, w + 6);
+            inflow_pTYB.insert(inflow_pTYB.end(), f, f + 3);
         }
         size_t ooff = 0;
         for (int lb = 0; lb < d->nblocks; ++lb) {
@@ -310,6 +331,9 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             for (int f = 0; f < 6; ++f) {
                 b.bc[f] = d->bc_codes[6 * lb + f];
                 b.bc_fs[f] = d->bc_fs[6 * lb + f] + fs_base;
+                b.twall[f] = d->bc_twall ? d->bc_twall[6 * lb + f] : 0.0;
+                if (s->viscous && b.active && b.bc[f] == CHKN_BC_WALL_NO_SLIP_FIXED_T && !std::isfinite(b.twall[f]))
+                    throw std::invalid_argument("TWall given as a function is not built");
             }
             // block.cu:282-284 and its j/k twins.
             if (b.nic < 2) throw std::invalid_argument("Too few cells in the i-index direction.");
@@ -378,8 +402,9 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
         for (int lb = 0; lb < d->nblocks; ++lb) {
             int ib = s->inst[qi].first_block + lb;
             HostBlock& b = s->blocks[ib];
-            std::vector<double> gi, gj, gk, volinv, len;
-            block_geometry(b, d->vertices + voff, gi, gj, gk, volinv, len);
+            std::vector<double> gi, gj, gk, volinv, len, fpos;
+            const bool want_fpos = s->viscous && b.active;
+            block_geometry(b, d->vertices + voff, gi, gj, gk, volinv, len, want_fpos ? &fpos : nullptr);
             voff += (size_t)3 * (b.nic + 1) * (b.njc + 1) * (b.nkc + 1);
             size_t n = (size_t)b.n, sz6 = sizeof(double) * 6 * n;
             HIPCHK(hipMalloc(&b.W, sz6 * (b.active ? 3 : 1)));
@@ -416,6 +441,8 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
                 bd.bc_arg[f] = (b.bc[f] == CHKN_BC_EXCHANGE) ? partner[ib][f] : b.bc_fs[f];
             }
             upload_state(s, b, b.io0.data());
+            split_block_setup(s, b, bd, fpos);
+            split_block_gas(s, b, bd);
         }
     }
     s->bdev = bdev;
@@ -423,6 +450,8 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     upload_blocks(s);
     HIPCHK(hipMalloc(&s->d_inflow, sizeof(double) * s->inflow_w.size()));
     HIPCHK(hipMemcpy(s->d_inflow, s->inflow_w.data(), sizeof(double) * s->inflow_w.size(), hipMemcpyHostToDevice));
+    split_inflow_gas(s, inflow_pTYB);
+    split_tables(s);
     const int N = ninst;
     s->ctl_bytes = (2 * sizeof(double) + sizeof(int)) * (size_t)kMaxSeg * N + sizeof(int) * 3 * kMaxSeg;
     HIPCHK(hipMalloc(&s->d_ctl, s->ctl_bytes));
@@ -465,8 +494,9 @@ StageArgs stage_args(chkn_sim* s, int q)
     return a;
 }
 
-// Launch the 3*K stage kernels of a segment (optionally per block).
-void launch_segment(chkn_sim* s, int K, bool dtred_last)
+// Launch the 3*K stages of a segment; fresh: the segment starts from the state read from
+// IO variables, whose first stage reads the file gas states.
+void launch_segment(chkn_sim* s, int K, bool dtred_last, bool fresh)
 {
     for (int q = 0; q < K; ++q) {
         StageArgs a = stage_args(s, q);
@@ -474,32 +504,17 @@ void launch_segment(chkn_sim* s, int K, bool dtred_last)
             a.win = stage - 1;
             a.wout = stage % 3;
             bool dtred = (stage == 3) && dtred_last && (q == K - 1);
-            if (s->opts.per_block_launch) {
-                for (auto& b : s->blocks) {
-                    if (!b.active || b.tile_count == 0) continue;
-                    a.tile_base = b.tile_first;
-                    a.ntiles = b.tile_count;
-                    HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, dtred,
-                                        s->opts.persistent != 0, s->grid_limit, a, s->stream));
-                }
-            } else {
-                a.tile_base = 0;
-                a.ntiles = s->ntiles;
-                HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, dtred,
-                                    s->opts.persistent != 0, s->grid_limit, a, s->stream));
-            }
+            launch_stage_form(s, a, stage, dtred, fresh && q == 0 && stage == 1, s->stream);
         }
     }
 }
 
-long long launches_per_segment(chkn_sim* s, int K)
+long long launches_per_segment(chkn_sim* s, int K, bool fresh)
 {
-    long long per_stage = 1;
-    if (s->opts.per_block_launch) {
-        per_stage = 0;
-        for (auto& b : s->blocks) if (b.active && b.tile_count) per_stage++;
-    }
-    return 3LL * K * per_stage;
+    const long long per_stage = launches_per_stage(s, false);
+    long long n = 3LL * K * per_stage;
+    if (fresh) n += launches_per_stage(s, true) - per_stage;  // first stage in face-list form
+    return n;
 }
 
 // run[q][inst] and dt[q][inst] must already be in h_ctl.
@@ -514,13 +529,14 @@ void run_segment(chkn_sim* s, int K, bool dtred_last)
     }
     HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, s->ctl_bytes, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipEventRecord(s->ev0, s->stream));
+    const bool fresh = fresh_needs_file_gas(s);
     if (s->opts.use_graph) {
-        GraphKey key{K, dtred_last};
+        GraphKey key{K, dtred_last, fresh};
         auto it = s->graphs.find(key);
         if (it == s->graphs.end()) {
             hipGraph_t g;
             HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            launch_segment(s, K, dtred_last);
+            launch_segment(s, K, dtred_last, fresh);
             HIPCHK(hipStreamEndCapture(s->stream, &g));
             hipGraphExec_t ge;
             HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -529,9 +545,9 @@ void run_segment(chkn_sim* s, int K, bool dtred_last)
         }
         HIPCHK(hipGraphLaunch(it->second, s->stream));
     } else {
-        launch_segment(s, K, dtred_last);
+        launch_segment(s, K, dtred_last, fresh);
     }
-    s->last_launches += launches_per_segment(s, K);
+    s->last_launches += launches_per_segment(s, K, fresh);
     HIPCHK(hipEventRecord(s->ev1, s->stream));
     HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, s->ctl_bytes, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -889,6 +905,10 @@ int chkn_set_iovars(chkn_sim* sim, int blk, const double* iovars)
         HostBlock& b = sim->blocks[blk];
         std::memcpy(b.io0.data() + 4 * (size_t)b.n, iovars + 4 * (size_t)b.n, sizeof(double) * 9 * (size_t)b.n);
         upload_state(sim, b, b.io0.data());
+        split_block_gas(sim, b, sim->bdev[blk]);
+        upload_blocks(sim);
+        for (auto& kv : sim->graphs) (void)hipGraphExecDestroy(kv.second);
+        sim->graphs.clear();
         sim->fresh = true;
         sim->failed = false;
         for (auto& I : sim->inst) I.have_sig = false;

@@ -1,0 +1,384 @@
+// chkn_split.cpp -- host side of the face-list form of a stage: which stages need it, the
+// least-squares weights of the viscous gradients, the gas states the reference keeps
+// verbatim, work tables and launches.
+//
+// The reference holds a full gas state (p, T, a besides rho and e) in every cell.  After
+// a cell's first update it follows from (rho, e) through update_from_rhoe (gas.cu:77-83);
+// before it, and forever in inactive blocks, it is what the flow file held
+// (cell.cu:112-131); inflow ghost cells hold the configured state after update_from_pT
+// (config.cu:398-406).  Only the first-order flux (p and a of the cells beside a face,
+// face.cu:356-371) and the viscous gradients (T of the cloud cells, block.cu:109-131) read
+// these values; the fused kernel derives them from (rho, e), so a job runs its stages in the
+// face-list form when the two differ somewhere it matters.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "chkn_internal.h"
+#include "chkn_physics.h"
+
+using namespace chkn;
+
+#include "chkn_runtime.h"
+
+namespace chkn {
+namespace rt {
+
+namespace {
+
+constexpr int kChunk = 256;  // faces or cells per work-group
+
+bool is_wall(int code) { return code >= 0 && code <= 2; }
+
+// The equation of state reproduces (p, T, a) from (rho, e, YB) bit for bit.
+bool eos_agrees(double rho, double e, double YB, double p, double T, double a)
+{
+    const double T2 = eos_T(e, YB);
+    return T2 == T && eos_p(rho, T2) == p && eos_a(T2) == a;
+}
+
+bool need_gas(const chkn_sim* s) { return s->x_order == 1 || s->viscous; }
+
+struct P3 {
+    double x, y, z;
+};
+
+// block.cu:28-86 with rsla.cu:53-72: least-squares weights of one face's cloud (n points,
+// face centre f as origin); a singular normal matrix (|det| <= 1e-16) leaves them zero, as
+// the reference reports the failure and carries on.
+void face_weights(const P3& f, const P3* pts, int n, double* wx, double* wy, double* wz)
+{
+    double w2[10], dx[10], dy[10], dz[10];
+    for (int i = 0; i < n; ++i) {
+        dx[i] = pts[i].x - f.x;
+        dy[i] = pts[i].y - f.y;
+        dz[i] = pts[i].z - f.z;
+        w2[i] = 1.0 / (dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i]);
+    }
+    double xx = 0.0, xy = 0.0, xz = 0.0, yy = 0.0, yz = 0.0, zz = 0.0;
+    for (int i = 0; i < n; ++i) {
+        xx += w2[i] * dx[i] * dx[i];
+        xy += w2[i] * dx[i] * dy[i];
+        xz += w2[i] * dx[i] * dz[i];
+        yy += w2[i] * dy[i] * dy[i];
+        yz += w2[i] * dy[i] * dz[i];
+        zz += w2[i] * dz[i] * dz[i];
+    }
+    const double c[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
+    for (int i = 0; i < 10; ++i) wx[i] = wy[i] = wz[i] = 0.0;
+    const double det = c[0][0] * (c[1][1] * c[2][2] - c[1][2] * c[2][1]) -
+                       c[0][1] * (c[1][0] * c[2][2] - c[1][2] * c[2][0]) +
+                       c[0][2] * (c[1][0] * c[2][1] - c[1][1] * c[2][0]);
+    if (std::fabs(det) <= 1.0e-16) return;
+    const double r = 1.0 / det;
+    double m[3][3];
+    m[0][0] = (c[1][1] * c[2][2] - c[1][2] * c[2][1]) * r;
+    m[0][1] = (c[0][2] * c[2][1] - c[0][1] * c[2][2]) * r;
+    m[0][2] = (c[0][1] * c[1][2] - c[0][2] * c[1][1]) * r;
+    m[1][0] = (c[1][2] * c[2][0] - c[1][0] * c[2][2]) * r;
+    m[1][1] = (c[0][0] * c[2][2] - c[0][2] * c[2][0]) * r;
+    m[1][2] = (c[0][2] * c[1][0] - c[0][0] * c[1][2]) * r;
+    m[2][0] = (c[1][0] * c[2][1] - c[1][1] * c[2][0]) * r;
+    m[2][1] = (c[0][1] * c[2][0] - c[0][0] * c[2][1]) * r;
+    m[2][2] = (c[0][0] * c[1][1] - c[0][1] * c[1][0]) * r;
+    for (int i = 0; i < n; ++i) {
+        wx[i] = w2[i] * (m[0][0] * dx[i] + m[0][1] * dy[i] + m[0][2] * dz[i]);
+        wy[i] = w2[i] * (m[1][0] * dx[i] + m[1][1] * dy[i] + m[1][2] * dz[i]);
+        wz[i] = w2[i] * (m[2][0] * dx[i] + m[2][1] * dy[i] + m[2][2] * dz[i]);
+    }
+}
+
+// Least-squares weights of every face of an active block, [3][10][nft], clouds in the order
+// the viscous kernel visits them: the cells beside the face (minus side first; on a
+// boundary the interior cell and, unless the side is a wall, the first ghost cell, placed
+// at f + (f - c) as block.cu:656-780 places it), then each interior cell's faces in the two
+// other directions, lower direction first, low side first.
+void block_weights(const HostBlock& b, const std::vector<double>& fpos, std::vector<double>& lsq)
+{
+    const int nic = b.nic, njc = b.njc, nkc = b.nkc;
+    const size_t n = (size_t)b.n;
+    const size_t nfi = (size_t)(nic + 1) * njc * nkc, nfj = (size_t)nic * (njc + 1) * nkc;
+    const size_t nft = nfi + nfj + (size_t)nic * njc * (nkc + 1);
+    lsq.assign(30 * nft, 0.0);
+    auto cell = [&](int i, int j, int k) {
+        size_t c = (size_t)i + (size_t)nic * (j + (size_t)njc * k);
+        return P3{b.pos[c], b.pos[n + c], b.pos[2 * n + c]};
+    };
+    auto face = [&](size_t ff) { return P3{fpos[ff], fpos[nft + ff], fpos[2 * nft + ff]}; };
+    auto cell_face = [&](int d, int hi, int i, int j, int k) -> size_t {
+        if (d == 0) return (size_t)(i + hi) + (size_t)(nic + 1) * (j + (size_t)njc * k);
+        if (d == 1) return nfi + (size_t)i + (size_t)nic * ((j + hi) + (size_t)(njc + 1) * k);
+        return nfi + nfj + (size_t)i + (size_t)nic * (j + (size_t)njc * (k + hi));
+    };
+    const int nl[3] = {nic, njc, nkc};
+    for (int dir = 0; dir < 3; ++dir) {
+        const int na = dir == 0 ? njc : nic, nc = dir == 2 ? njc : nkc;
+        const size_t base = dir == 0 ? 0 : (dir == 1 ? nfi : nfi + nfj);
+        const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int c = 0; c < nc; ++c)
+            for (int a = 0; a < na; ++a)
+                for (int m = 0; m <= nl[dir]; ++m) {
+                    auto ijk = [&](int mm, int& i, int& j, int& k) {
+                        if (dir == 0) { i = mm; j = a; k = c; }
+                        else if (dir == 1) { i = a; j = mm; k = c; }
+                        else { i = a; j = c; k = mm; }
+                    };
+                    size_t f;
+                    {
+                        int i, j, k;
+                        ijk(m, i, j, k);
+                        if (dir == 0) f = (size_t)i + (size_t)(nic + 1) * (j + (size_t)njc * k);
+                        else if (dir == 1) f = (size_t)i + (size_t)nic * (j + (size_t)(njc + 1) * k);
+                        else f = (size_t)i + (size_t)nic * (j + (size_t)njc * k);
+                    }
+                    const size_t ff = base + f;
+                    const P3 fc = face(ff);
+                    P3 pts[10];
+                    int np = 0;
+                    auto interior = [&](int mm) {
+                        int i, j, k;
+                        ijk(mm, i, j, k);
+                        return cell(i, j, k);
+                    };
+                    auto ghost = [&](int mm) {  // ghost across this face from interior mm
+                        const P3 c0 = interior(mm);
+                        const double dx = fc.x - c0.x, dy = fc.y - c0.y, dz = fc.z - c0.z;
+                        return P3{fc.x + dx, fc.y + dy, fc.z + dz};
+                    };
+                    if (m == 0) {
+                        if (!is_wall(b.bc[2 * dir])) pts[np++] = ghost(0);
+                        pts[np++] = interior(0);
+                    } else if (m == nl[dir]) {
+                        pts[np++] = interior(m - 1);
+                        if (!is_wall(b.bc[2 * dir + 1])) pts[np++] = ghost(m - 1);
+                    } else {
+                        pts[np++] = interior(m - 1);
+                        pts[np++] = interior(m);
+                    }
+                    for (int mc = m - 1; mc <= m; ++mc) {
+                        if (mc < 0 || mc >= nl[dir]) continue;
+                        int i, j, k;
+                        ijk(mc, i, j, k);
+                        pts[np++] = face(cell_face(d1, 0, i, j, k));
+                        pts[np++] = face(cell_face(d1, 1, i, j, k));
+                        pts[np++] = face(cell_face(d2, 0, i, j, k));
+                        pts[np++] = face(cell_face(d2, 1, i, j, k));
+                    }
+                    double wx[10], wy[10], wz[10];
+                    face_weights(fc, pts, np, wx, wy, wz);
+                    for (int q = 0; q < 10; ++q) {
+                        lsq[(size_t)q * nft + ff] = wx[q];
+                        lsq[(size_t)(10 + q) * nft + ff] = wy[q];
+                        lsq[(size_t)(20 + q) * nft + ff] = wz[q];
+                    }
+                }
+    }
+}
+
+size_t face_count(const HostBlock& b)
+{
+    return (size_t)(b.nic + 1) * b.njc * b.nkc + (size_t)b.nic * (b.njc + 1) * b.nkc +
+           (size_t)b.nic * b.njc * (b.nkc + 1);
+}
+
+// The face-list form runs every stage: viscous jobs, and first-order jobs in which an inflow
+// state used by an active block, or an inactive block that an active block exchanges with,
+// holds gas states the equation of state does not reproduce.
+bool split_always(const chkn_sim* s)
+{
+    if (s->viscous) return true;
+    if (s->x_order != 1) return false;
+    const int nb = (int)s->blocks.size();
+    for (int ib = 0; ib < nb; ++ib) {
+        const HostBlock& b = s->blocks[ib];
+        if (!b.active) continue;
+        for (int f = 0; f < 6; ++f) {
+            if (b.bc[f] == CHKN_BC_INFLOW && s->inflow_differs[b.bc_fs[f]]) return true;
+            if (b.bc[f] == CHKN_BC_EXCHANGE) {
+                const int o = s->bdev[ib].bc_arg[f];
+                if (!s->blocks[o].active && s->blocks[o].gas_differs) return true;
+            }
+        }
+    }
+    return false;
+}
+
+}  // namespace
+
+bool fresh_needs_file_gas(const chkn_sim* s)
+{
+    if (!s->fresh || !need_gas(s)) return false;
+    for (const auto& b : s->blocks)
+        if (b.active && b.gas_differs) return true;
+    return false;
+}
+
+void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vector<double>& fpos)
+{
+    for (int f = 0; f < 6; ++f) bd.twall[f] = b.twall[f];
+    if (!b.active || !need_gas(s)) return;
+    const size_t nft = face_count(b);
+    HIPCHK(hipMalloc(&b.F, sizeof(double) * 6 * nft));
+    bd.F = b.F;
+    if (!s->viscous) return;
+    HIPCHK(hipMalloc(&b.FS, sizeof(double) * 4 * nft));
+    std::vector<double> lsq;
+    block_weights(b, fpos, lsq);
+    HIPCHK(hipMalloc(&b.lsq, sizeof(double) * lsq.size()));
+    HIPCHK(hipMemcpy(b.lsq, lsq.data(), sizeof(double) * lsq.size(), hipMemcpyHostToDevice));
+    bd.FS = b.FS;
+    bd.lsq = b.lsq;
+}
+
+void split_block_gas(chkn_sim* s, HostBlock& b, BlockDev& bd)
+{
+    if (!need_gas(s)) return;
+    const size_t n = (size_t)b.n;
+    const double* io = b.io0.data();
+    std::vector<double> g(3 * n);
+    bool differs = false;
+    for (size_t c = 0; c < n; ++c) {
+        const double p = io[4 * n + c], T = io[5 * n + c], rho = io[6 * n + c], e = io[7 * n + c];
+        const double YB = io[8 * n + c], a = io[9 * n + c];
+        g[c] = p; g[n + c] = T; g[2 * n + c] = a;
+        if (!eos_agrees(rho, e, YB, p, T, a)) differs = true;
+    }
+    b.gas_differs = differs;
+    if (differs) {
+        if (!b.gas) HIPCHK(hipMalloc(&b.gas, sizeof(double) * 3 * n));
+        HIPCHK(hipMemcpy(b.gas, g.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    }
+    bd.gas = differs ? b.gas : nullptr;
+}
+
+void split_inflow_gas(chkn_sim* s, const std::vector<double>& pTYB)
+{
+    const size_t ns = pTYB.size() / 3;
+    s->inflow_gas.assign(3 * std::max<size_t>(ns, 1), 0.0);
+    s->inflow_differs.assign(std::max<size_t>(ns, 1), 0);
+    for (size_t q = 0; q < ns; ++q) {
+        const double p = pTYB[3 * q], T = pTYB[3 * q + 1], YB = pTYB[3 * q + 2];
+        const double* w = s->inflow_w.data() + 6 * q;  // rho, e from update_from_pT
+        const double a = eos_a(T);                      // gas.cu:72
+        s->inflow_gas[3 * q] = p;
+        s->inflow_gas[3 * q + 1] = T;
+        s->inflow_gas[3 * q + 2] = a;
+        s->inflow_differs[q] = eos_agrees(w[0], w[1], YB, p, T, a) ? 0 : 1;
+    }
+    dfree(s->d_inflow_gas);
+    s->d_inflow_gas = nullptr;
+    HIPCHK(hipMalloc(&s->d_inflow_gas, sizeof(double) * s->inflow_gas.size()));
+    HIPCHK(hipMemcpy(s->d_inflow_gas, s->inflow_gas.data(), sizeof(double) * s->inflow_gas.size(),
+                     hipMemcpyHostToDevice));
+}
+
+void split_set_inflow(chkn_sim* s, int q)
+{
+    const double* w = s->inflow_w.data() + 6 * (size_t)q;
+    const double T = eos_T(w[1], w[2]);
+    const double g[3] = {eos_p(w[0], T), T, eos_a(T)};
+    std::copy(g, g + 3, s->inflow_gas.begin() + 3 * (size_t)q);
+    s->inflow_differs[q] = 0;
+    HIPCHK(hipMemcpy(s->d_inflow_gas + 3 * (size_t)q, g, sizeof g, hipMemcpyHostToDevice));
+    for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+    s->graphs.clear();
+}
+
+void split_tables(chkn_sim* s)
+{
+    dfree(s->d_split_faces);
+    dfree(s->d_split_cells);
+    s->d_split_faces = nullptr;
+    s->d_split_cells = nullptr;
+    s->nsplit_faces = s->nsplit_cells = 0;
+    if (!need_gas(s)) return;
+    std::vector<FaceChunk> fc;
+    std::vector<CellChunk> cc;
+    for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
+        const HostBlock& b = s->blocks[ib];
+        if (!b.active) continue;
+        const long long nf[3] = {(long long)(b.nic + 1) * b.njc * b.nkc,
+                                 (long long)b.nic * (b.njc + 1) * b.nkc,
+                                 (long long)b.nic * b.njc * (b.nkc + 1)};
+        for (int d = 0; d < 3; ++d)
+            for (long long f0 = 0; f0 < nf[d]; f0 += kChunk)
+                fc.push_back(FaceChunk{(int)ib, d, (int)f0, (int)std::min<long long>(f0 + kChunk, nf[d])});
+        for (int c0 = 0; c0 < b.n; c0 += kChunk)
+            cc.push_back(CellChunk{(int)ib, c0, std::min(c0 + kChunk, b.n)});
+    }
+    s->nsplit_faces = (int)fc.size();
+    s->nsplit_cells = (int)cc.size();
+    if (!fc.empty()) {
+        HIPCHK(hipMalloc(&s->d_split_faces, sizeof(FaceChunk) * fc.size()));
+        HIPCHK(hipMemcpy(s->d_split_faces, fc.data(), sizeof(FaceChunk) * fc.size(), hipMemcpyHostToDevice));
+    }
+    if (!cc.empty()) {
+        HIPCHK(hipMalloc(&s->d_split_cells, sizeof(CellChunk) * cc.size()));
+        HIPCHK(hipMemcpy(s->d_split_cells, cc.data(), sizeof(CellChunk) * cc.size(), hipMemcpyHostToDevice));
+    }
+}
+
+void split_free(chkn_sim* s)
+{
+    for (auto& b : s->blocks) {
+        dfree(b.F); dfree(b.FS); dfree(b.lsq); dfree(b.gas);
+        b.F = b.FS = b.lsq = b.gas = nullptr;
+    }
+    dfree(s->d_split_faces); dfree(s->d_split_cells); dfree(s->d_inflow_gas);
+    s->d_split_faces = nullptr;
+    s->d_split_cells = nullptr;
+    s->d_inflow_gas = nullptr;
+}
+
+long long launches_per_stage(chkn_sim* s, bool file_gas)
+{
+    if (split_always(s) || file_gas) return s->viscous ? 3 : 2;
+    if (!s->opts.per_block_launch) return 1;
+    long long n = 0;
+    for (auto& b : s->blocks) if (b.active && b.tile_count) n++;
+    return n;
+}
+
+void launch_stage_form(chkn_sim* s, const StageArgs& a, int stage, bool dtred, bool file_gas,
+                       hipStream_t st)
+{
+    if (split_always(s) || file_gas) {
+        SplitArgs sa;
+        sa.blocks = a.blocks;
+        sa.inflow = a.inflow;
+        sa.inflow_gas = s->d_inflow_gas;
+        sa.faces = s->d_split_faces;
+        sa.cells = s->d_split_cells;
+        sa.ctl = a.ctl;
+        sa.win = a.win;
+        sa.wout = a.wout;
+        sa.file_gas = file_gas ? 1 : 0;
+        HIPCHK(launch_split_faces(sa, s->nsplit_faces, s->x_order, s->viscous, st));
+        if (s->viscous) HIPCHK(launch_split_viscous(sa, s->nsplit_faces, st));
+        HIPCHK(launch_split_update(sa, s->nsplit_cells, stage, dtred, st));
+        return;
+    }
+    StageArgs b = a;
+    if (s->opts.per_block_launch) {
+        for (auto& hb : s->blocks) {
+            if (!hb.active || hb.tile_count == 0) continue;
+            b.tile_base = hb.tile_first;
+            b.ntiles = hb.tile_count;
+            HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, dtred,
+                                s->opts.persistent != 0, s->grid_limit, b, st));
+        }
+    } else {
+        b.tile_base = 0;
+        b.ntiles = s->ntiles;
+        HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, dtred,
+                            s->opts.persistent != 0, s->grid_limit, b, st));
+    }
+}
+
+}  // namespace rt
+}  // namespace chkn

@@ -1,0 +1,413 @@
+// chkn_split.hip -- gfx950 kernels of the face-list form of a stage (FP64).
+//
+// The fused stage kernel forms each cell's flux divergence from face fluxes it evaluates on
+// the fly.  Two cases need the face fluxes as a stored list instead:
+//  * viscous jobs: the viscous flux of a face (block.cu:88-162) takes least-squares
+//    gradients over a cloud made of the cells beside the face and the transverse faces of
+//    those cells, so it needs the face states of its neighbours;
+//  * first-order jobs whose cells or inflow ghost cells hold gas states (p, T, a) that the
+//    equation of state does not reproduce from (rho, e): the reference's first-order flux
+//    reads p and a of the two cells beside the face verbatim (face.cu:356-371), as read from
+//    the flow file (cell.cu:112-131) or configured for the inflow (config.cu:398-406).
+// Three launches per stage, every active block of every instance in each:
+//   faces:   one thread per face: convective flux and, for viscous jobs, the face state
+//            set_as_average(L, R) (flow.cu:220-225) with the viscous boundary conditions
+//            (bcs.cu:777-797);
+//   viscous: one thread per face: gradients of T and velocity over the face's cloud, then
+//            the viscous stress and heat flux subtracted from the face flux;
+//   update:  one thread per cell: flux divergence in the reference's order (cell.cu:206-213),
+//            TVD-RK3 stage, decode, next stage state, bad-cell count, signal-time minimum.
+//
+// Build with -ffp-contract=off: every product and sum rounds exactly as in the reference.
+#include "chkn_internal.h"
+#include "chkn_physics.h"
+#include "chkn_stencil.h"
+
+namespace chkn {
+
+namespace {
+
+constexpr int kChunk = 256;
+
+// gas.cu:36-48: Cp and Sutherland's law for air.
+constexpr double kCp = kCv + kR;
+constexpr double kMuRef = 1.716e-5;
+constexpr double kTRef = 273.0;
+constexpr double kSuth = 111.0;
+constexpr double kPrandtl = 0.72;
+
+struct Gas {
+    double p, T, a;
+};
+
+__device__ __forceinline__ Gas eos_gas(const Prim& w)  // gas.cu:77-83
+{
+    Gas g;
+    g.T = eos_T(w.e, w.YB);
+    g.p = eos_p(w.rho, g.T);
+    g.a = eos_a(g.T);
+    return g;
+}
+
+// Gas state of cell c of block B in state w: the flow file's p, T, a where the reference
+// still holds them (cells of inactive blocks; active cells before their first update),
+// otherwise the equation of state of (rho, e).  A block whose file values agree with the
+// equation of state carries no gas array.
+__device__ __forceinline__ Gas cell_gas(const SplitArgs& A, const BlockDev* B, const Prim& w,
+                                        int c)
+{
+    if (B->gas && (A.file_gas || !B->active)) {
+        const int n = (int)B->n;
+        return Gas{B->gas[c], B->gas[n + c], B->gas[2 * n + c]};
+    }
+    return eos_gas(w);
+}
+
+struct Point {
+    Prim w;
+    Gas g;
+};
+
+// Stencil position m of the line (dir, a, c): an interior cell, or the ghost cell of the
+// side's boundary condition (bcs.cu:16-503) with the gas state the reference copies into it.
+template <bool GAS>
+__device__ __forceinline__ Point point(const SplitArgs& A, const BlockDev* B, int dir, int m,
+                                       int a, int c)
+{
+    const int nl = dir == 0 ? B->nic : (dir == 1 ? B->njc : B->nkc);
+    Point r;
+    if (m >= 0 && m < nl) {
+        const int idx = cell_index(B->nic, B->njc, dir, m, a, c);
+        r.w = ld_prim(B->W + A.win * B->wstride, (int)B->n, idx);
+        if (GAS) r.g = cell_gas(A, B, r.w, idx);
+        return r;
+    }
+    const int side = m < 0 ? 0 : 1;
+    const int depth = side ? m - nl : -m - 1;
+    const int bf = 2 * dir + side;
+    const int code = B->bc[bf];
+    if (code == 3) {  // exchange: the neighbour's cell at the same depth
+        const BlockDev* O = A.blocks + B->bc_arg[bf];
+        const int no = dir == 0 ? O->nic : (dir == 1 ? O->njc : O->nkc);
+        const int idx = cell_index(O->nic, O->njc, dir, side ? depth : no - 1 - depth, a, c);
+        r.w = ld_prim(O->W + A.win * O->wstride, (int)O->n, idx);
+        if (GAS) r.g = cell_gas(A, O, r.w, idx);
+        return r;
+    }
+    if (code == 4) {  // inflow: the configured flow state
+        const double* s = A.inflow + 6 * B->bc_arg[bf];
+        r.w = Prim{s[0], s[1], s[2], s[3], s[4], s[5]};
+        if (GAS) {
+            const double* g = A.inflow_gas + 3 * B->bc_arg[bf];
+            r.g = Gas{g[0], g[1], g[2]};
+        }
+        return r;
+    }
+    const int idx = cell_index(B->nic, B->njc, dir, side ? nl - 1 : 0, a, c);
+    r.w = ld_prim(B->W + A.win * B->wstride, (int)B->n, idx);
+    if (GAS) r.g = cell_gas(A, B, r.w, idx);
+    if (code == 5) return r;  // outflow copies the interior cell
+    const double* g;          // walls reflect it; p, T, a are copied unchanged
+    int nf, fi;
+    boundary_face(B, dir, side, a, c, g, nf, fi);
+    r.w = reflect(r.w, ld_frame(g, nf, fi));
+    return r;
+}
+
+// Face f of direction dir: line position m (between cells m-1 and m), transverse indices
+// (a, c) of its line, and its index in the block's face list.
+struct FacePos {
+    int m, a, c, ff;
+};
+
+__device__ __forceinline__ FacePos face_pos(const BlockDev* B, int dir, int f)
+{
+    const int nic = B->nic, njc = B->njc;
+    FacePos p;
+    if (dir == 0) {  // i + (nic+1)*(j + njc*k)
+        p.m = f % (nic + 1);
+        const int r = f / (nic + 1);
+        p.a = r % njc;
+        p.c = r / njc;
+        p.ff = f;
+    } else if (dir == 1) {  // i + nic*(j + (njc+1)*k)
+        p.a = f % nic;
+        const int r = f / nic;
+        p.m = r % (njc + 1);
+        p.c = r / (njc + 1);
+        p.ff = (int)B->nfi + f;
+    } else {  // i + nic*(j + njc*k)
+        p.a = f % nic;
+        const int r = f / nic;
+        p.c = r % njc;
+        p.m = r / njc;
+        p.ff = (int)(B->nfi + B->nfj) + f;
+    }
+    return p;
+}
+
+__device__ __forceinline__ int line_length(const BlockDev* B, int dir)
+{
+    return dir == 0 ? B->nic : (dir == 1 ? B->njc : B->nkc);
+}
+
+__device__ __forceinline__ bool is_wall(int code) { return code >= 0 && code <= 2; }
+
+template <int XORDER, bool VISC>
+__global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
+{
+    const FaceChunk ch = A.faces[blockIdx.x];
+    const BlockDev* B = A.blocks + ch.blk;
+    const int f = ch.f0 + (int)threadIdx.x;
+    if (f >= ch.f1 || !A.ctl.run[B->inst]) return;
+    const int dir = ch.dir;
+    const FacePos p = face_pos(B, dir, f);
+    const double* g = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
+    const int nf = (int)(dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk));
+    const Frame fr = ld_frame(g, nf, f);
+    double rhoL, rhoR, eL, eR, YBL, YBR, vxL, vxR, vyL, vyR, vzL, vzR, pL, pR, aL, aR;
+    if (XORDER > 1) {
+        // face.cu:359-368: rho, e, YB, then the equation of state, then the velocity.
+        const Prim s0 = point<false>(A, B, dir, p.m - 2, p.a, p.c).w;
+        const Prim s1 = point<false>(A, B, dir, p.m - 1, p.a, p.c).w;
+        const Prim s2 = point<false>(A, B, dir, p.m, p.a, p.c).w;
+        const Prim s3 = point<false>(A, B, dir, p.m + 1, p.a, p.c).w;
+        muscl(s0.rho, s1.rho, s2.rho, s3.rho, rhoL, rhoR);
+        muscl(s0.e, s1.e, s2.e, s3.e, eL, eR);
+        muscl(s0.YB, s1.YB, s2.YB, s3.YB, YBL, YBR);
+        muscl(s0.vx, s1.vx, s2.vx, s3.vx, vxL, vxR);
+        muscl(s0.vy, s1.vy, s2.vy, s3.vy, vyL, vyR);
+        muscl(s0.vz, s1.vz, s2.vz, s3.vz, vzL, vzR);
+        const double TL = eos_T(eL, YBL), TR = eos_T(eR, YBR);
+        pL = eos_p(rhoL, TL);
+        pR = eos_p(rhoR, TR);
+        aL = eos_a(TL);
+        aR = eos_a(TR);
+    } else {
+        const Point L = point<true>(A, B, dir, p.m - 1, p.a, p.c);
+        const Point R = point<true>(A, B, dir, p.m, p.a, p.c);
+        rhoL = L.w.rho; eL = L.w.e; YBL = L.w.YB; vxL = L.w.vx; vyL = L.w.vy; vzL = L.w.vz;
+        rhoR = R.w.rho; eR = R.w.e; YBR = R.w.YB; vxR = R.w.vx; vyR = R.w.vy; vzR = R.w.vz;
+        pL = L.g.p; aL = L.g.a;
+        pR = R.g.p; aR = R.g.a;
+    }
+    double F[6];
+    ausmdv(rhoL, eL, pL, aL, YBL, vxL, vyL, vzL, rhoR, eR, pR, aR, YBR, vxR, vyR, vzR, fr, F);
+    const long long nft = B->nfi + B->nfj + B->nfk;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) B->F[q * nft + p.ff] = F[q];
+    if (VISC) {
+        // Face state: average of the two reconstructed states, T from the mean e and YB.
+        double T = eos_T(0.5 * (eL + eR), 0.5 * (YBL + YBR));
+        double vx = 0.5 * (vxL + vxR), vy = 0.5 * (vyL + vyR), vz = 0.5 * (vzL + vzR);
+        const int nl = line_length(B, dir);
+        const int side = (p.m == 0) ? 0 : ((p.m == nl) ? 1 : -1);
+        if (side >= 0) {
+            const int code = B->bc[2 * dir + side];
+            if (code == 1 || code == 2) {  // no-slip walls; fixed-T walls also set T
+                vx = 0.0;
+                vy = 0.0;
+                vz = 0.0;
+                if (code == 2) T = B->twall[2 * dir + side];
+            }
+        }
+        B->FS[p.ff] = T;
+        B->FS[nft + p.ff] = vx;
+        B->FS[2 * nft + p.ff] = vy;
+        B->FS[3 * nft + p.ff] = vz;
+    }
+}
+
+// Index in the face list of the face on side hi (0 or 1) in direction d of cell (i, j, k).
+__device__ __forceinline__ int cell_face(const BlockDev* B, int d, int hi, int i, int j, int k)
+{
+    const int nic = B->nic, njc = B->njc;
+    if (d == 0) return (i + hi) + (nic + 1) * (j + njc * k);
+    if (d == 1) return (int)B->nfi + i + nic * ((j + hi) + (njc + 1) * k);
+    return (int)(B->nfi + B->nfj) + i + nic * (j + njc * (k + hi));
+}
+
+struct Grad {
+    double T[3], x[3], y[3], z[3];  // gradients of T, vx, vy, vz
+};
+
+// One cloud point of weight index q: accumulate w_q * (value - face value) in the
+// reference's order (block.cu:109-131).
+__device__ __forceinline__ void cloud_add(Grad& G, const double* __restrict__ lsq, long long nft,
+                                          int ff, int q, double T0, double vx0, double vy0,
+                                          double vz0, double T, double vx, double vy, double vz)
+{
+    const double wx = lsq[(long long)q * nft + ff];
+    const double wy = lsq[(long long)(10 + q) * nft + ff];
+    const double wz = lsq[(long long)(20 + q) * nft + ff];
+    double dq = T - T0;
+    G.T[0] += wx * dq; G.T[1] += wy * dq; G.T[2] += wz * dq;
+    dq = vx - vx0;
+    G.x[0] += wx * dq; G.x[1] += wy * dq; G.x[2] += wz * dq;
+    dq = vy - vy0;
+    G.y[0] += wx * dq; G.y[1] += wy * dq; G.y[2] += wz * dq;
+    dq = vz - vz0;
+    G.z[0] += wx * dq; G.z[1] += wy * dq; G.z[2] += wz * dq;
+}
+
+// Cloud (block.cu:317-358, :418-459, :519-560): the cells beside the face, minus side
+// first -- on a boundary the interior cell and, unless the side is a wall, the first ghost
+// cell -- then for each of those interior cells its faces in the two other directions,
+// lower direction first, low side before high side.
+__global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A)
+{
+    const FaceChunk ch = A.faces[blockIdx.x];
+    const BlockDev* B = A.blocks + ch.blk;
+    const int f = ch.f0 + (int)threadIdx.x;
+    if (f >= ch.f1 || !A.ctl.run[B->inst]) return;
+    const int dir = ch.dir;
+    const FacePos p = face_pos(B, dir, f);
+    const long long nft = B->nfi + B->nfj + B->nfk;
+    const double* __restrict__ FS = B->FS;
+    const double* __restrict__ lsq = B->lsq;
+    const double T0 = FS[p.ff], vx0 = FS[nft + p.ff], vy0 = FS[2 * nft + p.ff],
+                 vz0 = FS[3 * nft + p.ff];
+    Grad G;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) G.T[r] = G.x[r] = G.y[r] = G.z[r] = 0.0;
+    const int nl = line_length(B, dir);
+    int q = 0;
+    auto add_cell = [&](int m) {
+        const Point pt = point<true>(A, B, dir, m, p.a, p.c);
+        cloud_add(G, lsq, nft, p.ff, q++, T0, vx0, vy0, vz0, pt.g.T, pt.w.vx, pt.w.vy, pt.w.vz);
+    };
+    if (p.m == 0) {
+        if (!is_wall(B->bc[2 * dir])) add_cell(-1);
+        add_cell(0);
+    } else if (p.m == nl) {
+        add_cell(nl - 1);
+        if (!is_wall(B->bc[2 * dir + 1])) add_cell(nl);
+    } else {
+        add_cell(p.m - 1);
+        add_cell(p.m);
+    }
+    const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+    for (int mc = p.m - 1; mc <= p.m; ++mc) {
+        if (mc < 0 || mc >= nl) continue;
+        int i, j, k;
+        if (dir == 0) { i = mc; j = p.a; k = p.c; }
+        else if (dir == 1) { i = p.a; j = mc; k = p.c; }
+        else { i = p.a; j = p.c; k = mc; }
+        const int fl[4] = {cell_face(B, d1, 0, i, j, k), cell_face(B, d1, 1, i, j, k),
+                           cell_face(B, d2, 0, i, j, k), cell_face(B, d2, 1, i, j, k)};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int o = fl[r];
+            cloud_add(G, lsq, nft, p.ff, q++, T0, vx0, vy0, vz0, FS[o], FS[nft + o],
+                      FS[2 * nft + o], FS[3 * nft + o]);
+        }
+    }
+    // Transport coefficients at the face temperature (gas.cu:95-100), stresses and heat
+    // flux (block.cu:134-161).
+    const double mu = kMuRef * sqrt(T0 / kTRef) * (T0 / kTRef) * (kTRef + kSuth) / (T0 + kSuth);
+    const double kcond = kCp * mu / kPrandtl;
+    const double lmbda = -2.0 / 3.0 * mu;
+    const double tau_xx = 2.0 * mu * G.x[0] + lmbda * (G.x[0] + G.y[1] + G.z[2]);
+    const double tau_yy = 2.0 * mu * G.y[1] + lmbda * (G.x[0] + G.y[1] + G.z[2]);
+    const double tau_zz = 2.0 * mu * G.z[2] + lmbda * (G.x[0] + G.y[1] + G.z[2]);
+    const double tau_xy = mu * (G.x[1] + G.y[0]);
+    const double tau_xz = mu * (G.x[2] + G.z[0]);
+    const double tau_yz = mu * (G.y[2] + G.z[1]);
+    const double qx = kcond * G.T[0], qy = kcond * G.T[1], qz = kcond * G.T[2];
+    const double* gf = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
+    const int nf = (int)(dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk));
+    const double nx = gf[nf + f], ny = gf[2 * nf + f], nz = gf[3 * nf + f];
+    double* __restrict__ F = B->F;
+    F[nft + p.ff] -= tau_xx * nx + tau_xy * ny + tau_xz * nz;
+    F[2 * nft + p.ff] -= tau_xy * nx + tau_yy * ny + tau_yz * nz;
+    F[3 * nft + p.ff] -= tau_xz * nx + tau_yz * ny + tau_zz * nz;
+    F[4 * nft + p.ff] -= (tau_xx * vx0 + tau_xy * vy0 + tau_xz * vz0 + qx) * nx +
+                         (tau_xy * vx0 + tau_yy * vy0 + tau_yz * vz0 + qy) * ny +
+                         (tau_xz * vx0 + tau_yz * vy0 + tau_zz * vz0 + qz) * nz;
+}
+
+template <int STAGE, bool DTRED>
+__global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A)
+{
+    const CellChunk ch = A.cells[blockIdx.x];
+    const BlockDev* B = A.blocks + ch.blk;
+    if (!A.ctl.run[B->inst]) return;  // uniform over the work-group
+    const int cidx = ch.c0 + (int)threadIdx.x;
+    int bad = 0;
+    unsigned long long sigmin = ~0ull;
+    if (cidx < ch.c1) {
+        const int nic = B->nic, njc = B->njc;
+        const int n = (int)B->n;
+        const int i = cidx % nic, j = (cidx / nic) % njc, k = cidx / (nic * njc);
+        const long long nfi = B->nfi, nfj = B->nfj, nft = nfi + B->nfj + B->nfk;
+        const int fim = i + (nic + 1) * (j + njc * k);
+        const int fjm = i + nic * (j + (njc + 1) * k);
+        const int fkm = cidx;
+        const double aim = B->gi[fim], aip = B->gi[fim + 1];
+        const double ajm = B->gj[fjm], ajp = B->gj[fjm + nic];
+        const double akm = B->gk[fkm], akp = B->gk[fkm + nic * njc];
+        const double vinv = B->volinv[cidx];
+        double d[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double* Fq = B->F + q * nft;
+            double s = aim * Fq[fim] - aip * Fq[fim + 1];
+            s = s + ajm * Fq[nfi + fjm];
+            s = s - ajp * Fq[nfi + fjm + nic];
+            s = s + akm * Fq[nfi + nfj + fkm];
+            s = s - akp * Fq[nfi + nfj + fkm + nic * njc];
+            d[q] = vinv * s;
+        }
+        Prim w;
+        const int flag = stage_update<STAGE>(B, n, cidx, A.ctl.dt[B->inst], d, A.wout, w);
+        bad = flag;
+        if (DTRED && !flag) sigmin = signal_bits(B, n, i, j, k, cidx, w);
+    }
+    if (__ballot(bad != 0)) {
+        int cnt = bad;
+        for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&A.ctl.bad[STAGE - 1], cnt);
+    }
+    if (DTRED) {
+        const unsigned long long wm = wave_min_u64(sigmin);
+        if ((threadIdx.x & 63) == 0 && wm != ~0ull) atomicMin(&A.ctl.sigmin_bits[B->inst], wm);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order, bool viscous,
+                              hipStream_t s)
+{
+    if (nface_chunks <= 0) return hipSuccess;
+    const dim3 grid(nface_chunks), blk(kChunk);
+    if (x_order > 1) {
+        if (viscous) hipLaunchKernelGGL((split_face_kernel<2, true>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((split_face_kernel<2, false>), grid, blk, 0, s, a);
+    } else {
+        if (viscous) hipLaunchKernelGGL((split_face_kernel<1, true>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((split_face_kernel<1, false>), grid, blk, 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_split_viscous(const SplitArgs& a, int nface_chunks, hipStream_t s)
+{
+    if (nface_chunks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(split_viscous_kernel, dim3(nface_chunks), dim3(kChunk), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_update(const SplitArgs& a, int ncell_chunks, int stage, bool dt_reduce,
+                               hipStream_t s)
+{
+    if (ncell_chunks <= 0) return hipSuccess;
+    const dim3 grid(ncell_chunks), blk(kChunk);
+    if (stage == 1) hipLaunchKernelGGL((split_update_kernel<1, false>), grid, blk, 0, s, a);
+    else if (stage == 2) hipLaunchKernelGGL((split_update_kernel<2, false>), grid, blk, 0, s, a);
+    else if (dt_reduce) hipLaunchKernelGGL((split_update_kernel<3, true>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((split_update_kernel<3, false>), grid, blk, 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace chkn

@@ -93,4 +93,70 @@ __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const double*
     return reflect(in, ld_frame(g, nf, fi));
 }
 
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off, 64);
+        v = (o < v) ? o : v;
+    }
+    return v;
+}
+
+// TVD-RK3 stage of cell cidx from its flux divergence d (block.cu:1090-1160): stage 1 keeps
+// d in L, stage 2 replaces L by L/6 + d/6, stage 3 writes the new U.  Decodes the stage
+// result (flow.cu:245-268) into W slot wout and returns the decode's bad-cell flag.
+template <int STAGE>
+__device__ __forceinline__ int stage_update(const BlockDev* B, int n, int cidx, double dt,
+                                            const double d[6], int wout, Prim& w)
+{
+    double* __restrict__ U = B->U;
+    double* __restrict__ Lr = B->L;
+    double Un[6];
+    if (STAGE == 1) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            Un[q] = U[q * n + cidx] + dt * d[q];
+            Lr[q * n + cidx] = d[q];
+        }
+    } else if (STAGE == 2) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double l0 = Lr[q * n + cidx];
+            Un[q] = U[q * n + cidx] + 0.25 * dt * (l0 + d[q]);
+            Lr[q * n + cidx] = kOneSixth * l0 + kOneSixth * d[q];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            Un[q] = U[q * n + cidx] + dt * (Lr[q * n + cidx] + kFourSixth * d[q]);
+            U[q * n + cidx] = Un[q];
+        }
+    }
+    const int flag = decode(Un, w);
+    double* __restrict__ Wout = B->W + wout * B->wstride;
+    Wout[cidx] = w.rho;
+    Wout[n + cidx] = w.e;
+    Wout[2 * n + cidx] = w.YB;
+    Wout[3 * n + cidx] = w.vx;
+    Wout[4 * n + cidx] = w.vy;
+    Wout[5 * n + cidx] = w.vz;
+    return flag;
+}
+
+// Bits of the cfl-free signal time of cell (i, j, k) = cidx in state w (cell.cu:155-164);
+// positive doubles order as their bit patterns, so integer minima give the minimum time.
+__device__ __forceinline__ unsigned long long signal_bits(const BlockDev* B, int n, int i, int j,
+                                                         int k, int cidx, const Prim& w)
+{
+    const int nic = B->nic, njc = B->njc;
+    const double T = eos_T(w.e, w.YB);
+    const double sa = eos_a(T);
+    const Frame fi_ = ld_frame(B->gi, (int)B->nfi, i + (nic + 1) * (j + njc * k));
+    const Frame fj_ = ld_frame(B->gj, (int)B->nfj, i + nic * (j + (njc + 1) * k));
+    const Frame fk_ = ld_frame(B->gk, (int)B->nfk, cidx);
+    const double sig = local_signal(w, sa, B->len[cidx], B->len[n + cidx], B->len[2 * n + cidx],
+                                    fi_, fj_, fk_);
+    return (unsigned long long)__double_as_longlong(sig);
+}
+
 }  // namespace chkn

@@ -1,0 +1,99 @@
+"""Viscous fluxes and the face-list stage form on the GPU (through the C-ABI).
+
+The viscous plate (inflow, outflow, periodic exchange, a no-slip adiabatic wall and a
+no-slip fixed-T wall, warped grid, varied initial state) against the oracle at x_order 2 and
+1, to the step tolerances of test_gpu_parity (1e-9 relative after N steps); launch options
+and ensembles must not change the result (bit for bit).  First-order jobs whose file and
+inflow gas states differ from the equation of state run in the same face-list form.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from chkn import cases, prep
+from chkn.job import JobData
+from chkn.lib import ChknError, Sim
+from test_gpu_parity import FIELD_TOL, compare
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("x_order,steps", [(2, 30), (1, 20)])
+def test_viscous_plate_matches_oracle(x_order, steps):
+    compare(cases.viscous_plate(max_step=steps, x_order=x_order, seed=3 + x_order), steps)
+
+
+def test_viscous_plate_single_step_residual():
+    # One step from the oracle's developed state (no file gas states left) to 1e-12 in U.
+    job = cases.viscous_plate(max_step=40)
+    orc = ob.Oracle(job)
+    orc.march(15)
+    for b in range(len(job.blocks)):
+        job.blocks[b].iovars = orc.iovars(b)
+    sim = Sim(job)
+    sim.step(1)
+    orc2 = ob.Oracle(job)
+    orc2.march(1)
+    for b in range(len(job.blocks)):
+        a, r = sim.conserved(b), orc2.conserved(b)
+        assert float(np.max(np.abs(a - r) / np.maximum(np.abs(r), 1e-300))) <= 1e-12
+    sim.close()
+
+
+def _run(job_or_jobs, steps, launch=None, splits=None):
+    sim = Sim(jobs=job_or_jobs) if isinstance(job_or_jobs, list) else Sim(job_or_jobs)
+    if launch:
+        sim.set_launch(**launch)
+    for n in (splits or [steps]):
+        sim.step(n)
+    out = [sim.conserved(b) for b in range(sim.nblocks)]
+    sim.close()
+    return out
+
+
+@pytest.mark.parametrize("launch", [dict(use_graph=True), dict(per_block_launch=True),
+                                    dict(persistent=True, tile_i=16, tile_j=4)])
+def test_viscous_launch_options_and_split_calls(launch):
+    job = cases.viscous_plate(max_step=25)
+    ref = _run(job, 25)
+    got = _run(job, 25, launch=launch, splits=[1, 6, 18])
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+
+
+def test_viscous_ensemble_member_equals_single():
+    jobs = [cases.viscous_plate(max_step=20, seed=5), cases.viscous_plate(nics=(6, 9), nkcs=(5, 7),
+                                                                            max_step=14, seed=9)]
+    ens = _run(jobs, 20)
+    off = 0
+    for job in jobs:
+        single = _run(job, 20)
+        for b in range(len(job.blocks)):
+            assert np.array_equal(single[b], ens[off + b])
+        off += len(job.blocks)
+
+
+def test_first_order_with_file_and_inflow_gas_states():
+    # mixed_bc_array: inflow at p = 5e3, T = 260 (rho, e do not give back p), an inactive
+    # block read from the file, cells whose file T differs from e/Cv.
+    cases.mixed_bc_array(max_step=15)
+    prep.config.x_order = 1
+    job = JobData.from_prep()
+    worst = compare(job, 15)
+    assert worst <= FIELD_TOL
+
+
+def test_twall_function_is_rejected():
+    job = cases.viscous_plate(max_step=5)
+    job.blocks[1].bc_twall = [0.0, 0.0, 0.0, 0.0, math.nan, 0.0]
+    with pytest.raises(ChknError, match="TWall given as a function"):
+        Sim(job)
+
+
+def test_viscous_gradient_is_rejected():
+    sim = Sim(cases.viscous_plate(max_step=5))
+    with pytest.raises(ChknError, match="adjoint of the viscous fluxes"):
+        sim.grad_forward(2, 2)
+    sim.close()
