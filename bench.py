@@ -25,10 +25,12 @@ sys.path.insert(0, os.path.join(ROOT, "gdtk-rmhd_amd"))
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BYTES_PER_CELL_STEP = 864  # algorithmic FP64 bytes per cell-update (3 stages)
 BYTES_PER_CELL_DT = 144    # extra bytes per cell per CFL reduction
+# Viscous step: also the cell centre and the three face centres a cell's clouds need, per stage.
+BYTES_PER_CELL_STEP_VISCOUS = BYTES_PER_CELL_STEP + 3 * 96
 
 
 WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300, "diff": 500,
-                  "ensemble-diff": 300}
+                  "ensemble-diff": 300, "viscous": 300}
 
 
 def build_workload(name, steps_total):
@@ -38,6 +40,11 @@ def build_workload(name, steps_total):
         jobs = [cases.ramp(256, 128, 128, max_step=steps_total)]
         desc = {"workload": "plain-at-scale: one 256x128x128 block, Mach 5 flow over a 15-degree "
                             "ramp, p=2e3 Pa, T=220 K, 1% seeded density perturbation, FP64"}
+    elif name == "viscous":
+        jobs = [cases.ramp(256, 128, 128, max_step=steps_total, viscous=True)]
+        desc = {"workload": "viscous: one 256x128x128 block, Mach 5 flow over a 15-degree ramp "
+                            "with viscous fluxes (least-squares face gradients, Sutherland air), "
+                            "no-slip ramp wall at 300 K, 1% seeded density perturbation, FP64"}
     elif name == "plain":
         jobs = [cases.sod_box(64, 32, 32, max_step=steps_total)]
         desc = {"workload": "plain: one 64x32x32 block, 3D Sod shock tube, slip walls, FP64"}
@@ -63,7 +70,8 @@ def build_workload(name, steps_total):
         raise SystemExit("unknown workload " + name)
     desc.update({"cells": sum(j.active_cells for j in jobs),
                  "blocks": sum(len(j.blocks) for j in jobs), "instances": len(jobs),
-                 "scheme": "MUSCL/van Albada + AUSMDV, TVD-RK3 (3 stages)", "cfl": 0.5,
+                 "scheme": "MUSCL/van Albada + AUSMDV, TVD-RK3 (3 stages)"
+                           + (", viscous fluxes" if jobs[0].viscous else ""), "cfl": 0.5,
                  "cfl_count": jobs[0].cfl_count})
     return jobs, desc
 
@@ -224,7 +232,7 @@ def main():
         ti, tj, tk = (int(v) for v in args.tile.split("x"))
         opts = dict(tile_i=ti, tile_j=tj, tile_k=tk, tile_order=args.order,
                     persistent=bool(args.persistent))
-    elif args.tune:
+    elif args.tune and not jobs[0].viscous:  # the face-list stage has no tile shape
         t0 = time.perf_counter()
         opts, table = tune.autotune(jobs=jobs, per_block_launch=bool(args.per_block))
         tuned = {"candidates": len(table), "seconds": round(time.perf_counter() - t0, 1),
@@ -248,7 +256,9 @@ def main():
     wall, kernel_ms, launches, rep, cells = run_timed(jobs, opts, steps, args.warmup)
     first = args.warmup
     reductions = sum(1 for s in range(first + 1, first + steps + 1) if s % jobs[0].cfl_count == 0)
-    alg_bytes = cells * (BYTES_PER_CELL_STEP * steps + BYTES_PER_CELL_DT * reductions)
+    viscous = bool(jobs[0].viscous)
+    per_step = BYTES_PER_CELL_STEP_VISCOUS if viscous else BYTES_PER_CELL_STEP
+    alg_bytes = cells * (per_step * steps + BYTES_PER_CELL_DT * reductions)
     value = cells * steps / wall
     achieved = alg_bytes / (kernel_ms * 1e-3)
     traffic = None
@@ -286,7 +296,8 @@ def main():
         "hbm_pct_of_peak": 100.0 * achieved / HBM_PEAK,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "stage_kernel (3 launches per step)",
+                     "kernel": ("split_face_kernel + split_viscous_kernel + split_update_kernel "
+                                "(9 launches per step)") if viscous else "stage_kernel (3 launches per step)",
                      "avg_launch_us": kernel_ms * 1e3 / max(launches, 1),
                      "alg_bytes_per_launch": alg_bytes / max(launches, 1)},
     }

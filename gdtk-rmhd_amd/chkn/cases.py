@@ -114,12 +114,16 @@ def perturbed_state(grid, fs, seed=1, amplitude=0.01, nmodes=8):
 
 
 def ramp(nic=256, njc=128, nkc=128, max_step=500, seed=1, mach=5.0, p=2.0e3, T=220.0,
-         title="Mach 5 flow over a 15-degree ramp"):
+         title="Mach 5 flow over a 15-degree ramp", viscous=False, TWall=300.0):
+    """Ramp flow on one block.  viscous: viscous fluxes on, and the ramp surface a no-slip
+    wall held at TWall (the side walls stay slip walls)."""
     _common(max_step, title=title + " (seed %d)" % seed)
     grid = ramp_vertices(nic, njc, nkc)
     fs = freestream(mach, p, T)
     st = perturbed_state(grid, fs, seed=seed)
-    bcs = {"iminus": prep.InflowBC(fs), "iplus": prep.OutflowBC(), "kminus": prep.WallWithSlipBC(),
+    wall = prep.WallNoSlipFixedTBC(TWall) if viscous else prep.WallWithSlipBC()
+    prep.config.viscous = bool(viscous)
+    bcs = {"iminus": prep.InflowBC(fs), "iplus": prep.OutflowBC(), "kminus": wall,
            "kplus": prep.OutflowBC(), "jminus": prep.WallWithSlipBC(), "jplus": prep.WallWithSlipBC()}
     prep.FluidBlock(i=0, j=0, k=0, grid=grid, initialState=st, bcs=bcs)
     return JobData.from_prep()

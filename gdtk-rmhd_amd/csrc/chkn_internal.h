@@ -16,8 +16,8 @@
 // reproduced by the equation of state), faces numbered i-faces, then j-faces (+nfi), then
 // k-faces (+nfi+nfj), nft faces in all:
 //   F[6][nft]    face fluxes (not area-weighted), viscous part subtracted in place
-//   FS[4][nft]   face T, vx, vy, vz after the viscous boundary conditions
-//   lsq[3][10][nft] least-squares gradient weights (x, y, z) of the face's cloud
+//   FS[nft][4]   face T, vx, vy, vz after the viscous boundary conditions (32-byte records)
+//   fpos[nft][4], cpos[n][4] face and cell centres x, y, z (+1 unused) for the clouds
 //   gas[3][n]    p, T, a as read from the flow file
 // Inactive blocks hold one W slot (wstride 0: every slot is the file state) and the
 // geometry needed by exchange neighbours.
@@ -56,7 +56,8 @@ struct BlockDev {
     // face-list stage
     double* F;
     double* FS;
-    const double* lsq;
+    const double* fpos;
+    const double* cpos;
     const double* gas;     // null when the block's file state is not needed
     double twall[6];       // TWall of each wall_no_slip_fixed_T side
 };

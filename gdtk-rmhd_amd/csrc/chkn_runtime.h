@@ -55,8 +55,9 @@ struct HostBlock {
     double twall[6] = {0, 0, 0, 0, 0, 0};
     bool gas_differs = false;  // some cell's file p, T, a differ from the equation of state
     double* F = nullptr;       // 6 x nft face fluxes
-    double* FS = nullptr;      // 4 x nft face T, vx, vy, vz (viscous)
-    double* lsq = nullptr;     // 30 x nft least-squares weights (viscous)
+    double* FS = nullptr;      // nft x 4 face T, vx, vy, vz (viscous)
+    double* fpos = nullptr;    // nft x 4 face centres (viscous)
+    double* cpos = nullptr;    // n x 4 cell centres (viscous)
     double* gas = nullptr;     // 3 x n file p, T, a (only when gas_differs)
     // adjoint (allocated by the first gradient call)
     double* tape = nullptr;    // (3C+1) W slots of the segment being reversed
@@ -199,7 +200,7 @@ bool fresh_needs_file_gas(const chkn_sim* s);
 
 // Face-list stage set-up (chkn_split.cpp).
 // Per block after its geometry: face fluxes, and for viscous jobs face states and the
-// least-squares weights from cell centres and face centres fpos ([3][nft]).
+// cell and face centres (fpos, [3][nft]) the least-squares gradients are taken over.
 void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vector<double>& fpos);
 // The file gas states of block b from its IO variables (re-run when they are replaced).
 void split_block_gas(chkn_sim* s, HostBlock& b, BlockDev& bd);

@@ -1,6 +1,6 @@
 // chkn_split.cpp -- host side of the face-list form of a stage: which stages need it, the
-// least-squares weights of the viscous gradients, the gas states the reference keeps
-// verbatim, work tables and launches.
+// device data of the viscous gradients, the gas states the reference keeps verbatim, work
+// tables and launches.
 //
 // The reference holds a full gas state (p, T, a besides rho and e) in every cell.  After
 // a cell's first update it follows from (rho, e) through update_from_rhoe (gas.cu:77-83);
@@ -13,7 +13,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -32,8 +31,6 @@ namespace {
 
 constexpr int kChunk = 256;  // faces or cells per work-group
 
-bool is_wall(int code) { return code >= 0 && code <= 2; }
-
 // The equation of state reproduces (p, T, a) from (rho, e, YB) bit for bit.
 bool eos_agrees(double rho, double e, double YB, double p, double T, double a)
 {
@@ -42,143 +39,6 @@ bool eos_agrees(double rho, double e, double YB, double p, double T, double a)
 }
 
 bool need_gas(const chkn_sim* s) { return s->x_order == 1 || s->viscous; }
-
-struct P3 {
-    double x, y, z;
-};
-
-// block.cu:28-86 with rsla.cu:53-72: least-squares weights of one face's cloud (n points,
-// face centre f as origin); a singular normal matrix (|det| <= 1e-16) leaves them zero, as
-// the reference reports the failure and carries on.
-void face_weights(const P3& f, const P3* pts, int n, double* wx, double* wy, double* wz)
-{
-    double w2[10], dx[10], dy[10], dz[10];
-    for (int i = 0; i < n; ++i) {
-        dx[i] = pts[i].x - f.x;
-        dy[i] = pts[i].y - f.y;
-        dz[i] = pts[i].z - f.z;
-        w2[i] = 1.0 / (dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i]);
-    }
-    double xx = 0.0, xy = 0.0, xz = 0.0, yy = 0.0, yz = 0.0, zz = 0.0;
-    for (int i = 0; i < n; ++i) {
-        xx += w2[i] * dx[i] * dx[i];
-        xy += w2[i] * dx[i] * dy[i];
-        xz += w2[i] * dx[i] * dz[i];
-        yy += w2[i] * dy[i] * dy[i];
-        yz += w2[i] * dy[i] * dz[i];
-        zz += w2[i] * dz[i] * dz[i];
-    }
-    const double c[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
-    for (int i = 0; i < 10; ++i) wx[i] = wy[i] = wz[i] = 0.0;
-    const double det = c[0][0] * (c[1][1] * c[2][2] - c[1][2] * c[2][1]) -
-                       c[0][1] * (c[1][0] * c[2][2] - c[1][2] * c[2][0]) +
-                       c[0][2] * (c[1][0] * c[2][1] - c[1][1] * c[2][0]);
-    if (std::fabs(det) <= 1.0e-16) return;
-    const double r = 1.0 / det;
-    double m[3][3];
-    m[0][0] = (c[1][1] * c[2][2] - c[1][2] * c[2][1]) * r;
-    m[0][1] = (c[0][2] * c[2][1] - c[0][1] * c[2][2]) * r;
-    m[0][2] = (c[0][1] * c[1][2] - c[0][2] * c[1][1]) * r;
-    m[1][0] = (c[1][2] * c[2][0] - c[1][0] * c[2][2]) * r;
-    m[1][1] = (c[0][0] * c[2][2] - c[0][2] * c[2][0]) * r;
-    m[1][2] = (c[0][2] * c[1][0] - c[0][0] * c[1][2]) * r;
-    m[2][0] = (c[1][0] * c[2][1] - c[1][1] * c[2][0]) * r;
-    m[2][1] = (c[0][1] * c[2][0] - c[0][0] * c[2][1]) * r;
-    m[2][2] = (c[0][0] * c[1][1] - c[0][1] * c[1][0]) * r;
-    for (int i = 0; i < n; ++i) {
-        wx[i] = w2[i] * (m[0][0] * dx[i] + m[0][1] * dy[i] + m[0][2] * dz[i]);
-        wy[i] = w2[i] * (m[1][0] * dx[i] + m[1][1] * dy[i] + m[1][2] * dz[i]);
-        wz[i] = w2[i] * (m[2][0] * dx[i] + m[2][1] * dy[i] + m[2][2] * dz[i]);
-    }
-}
-
-// Least-squares weights of every face of an active block, [3][10][nft], clouds in the order
-// the viscous kernel visits them: the cells beside the face (minus side first; on a
-// boundary the interior cell and, unless the side is a wall, the first ghost cell, placed
-// at f + (f - c) as block.cu:656-780 places it), then each interior cell's faces in the two
-// other directions, lower direction first, low side first.
-void block_weights(const HostBlock& b, const std::vector<double>& fpos, std::vector<double>& lsq)
-{
-    const int nic = b.nic, njc = b.njc, nkc = b.nkc;
-    const size_t n = (size_t)b.n;
-    const size_t nfi = (size_t)(nic + 1) * njc * nkc, nfj = (size_t)nic * (njc + 1) * nkc;
-    const size_t nft = nfi + nfj + (size_t)nic * njc * (nkc + 1);
-    lsq.assign(30 * nft, 0.0);
-    auto cell = [&](int i, int j, int k) {
-        size_t c = (size_t)i + (size_t)nic * (j + (size_t)njc * k);
-        return P3{b.pos[c], b.pos[n + c], b.pos[2 * n + c]};
-    };
-    auto face = [&](size_t ff) { return P3{fpos[ff], fpos[nft + ff], fpos[2 * nft + ff]}; };
-    auto cell_face = [&](int d, int hi, int i, int j, int k) -> size_t {
-        if (d == 0) return (size_t)(i + hi) + (size_t)(nic + 1) * (j + (size_t)njc * k);
-        if (d == 1) return nfi + (size_t)i + (size_t)nic * ((j + hi) + (size_t)(njc + 1) * k);
-        return nfi + nfj + (size_t)i + (size_t)nic * (j + (size_t)njc * (k + hi));
-    };
-    const int nl[3] = {nic, njc, nkc};
-    for (int dir = 0; dir < 3; ++dir) {
-        const int na = dir == 0 ? njc : nic, nc = dir == 2 ? njc : nkc;
-        const size_t base = dir == 0 ? 0 : (dir == 1 ? nfi : nfi + nfj);
-        const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int c = 0; c < nc; ++c)
-            for (int a = 0; a < na; ++a)
-                for (int m = 0; m <= nl[dir]; ++m) {
-                    auto ijk = [&](int mm, int& i, int& j, int& k) {
-                        if (dir == 0) { i = mm; j = a; k = c; }
-                        else if (dir == 1) { i = a; j = mm; k = c; }
-                        else { i = a; j = c; k = mm; }
-                    };
-                    size_t f;
-                    {
-                        int i, j, k;
-                        ijk(m, i, j, k);
-                        if (dir == 0) f = (size_t)i + (size_t)(nic + 1) * (j + (size_t)njc * k);
-                        else if (dir == 1) f = (size_t)i + (size_t)nic * (j + (size_t)(njc + 1) * k);
-                        else f = (size_t)i + (size_t)nic * (j + (size_t)njc * k);
-                    }
-                    const size_t ff = base + f;
-                    const P3 fc = face(ff);
-                    P3 pts[10];
-                    int np = 0;
-                    auto interior = [&](int mm) {
-                        int i, j, k;
-                        ijk(mm, i, j, k);
-                        return cell(i, j, k);
-                    };
-                    auto ghost = [&](int mm) {  // ghost across this face from interior mm
-                        const P3 c0 = interior(mm);
-                        const double dx = fc.x - c0.x, dy = fc.y - c0.y, dz = fc.z - c0.z;
-                        return P3{fc.x + dx, fc.y + dy, fc.z + dz};
-                    };
-                    if (m == 0) {
-                        if (!is_wall(b.bc[2 * dir])) pts[np++] = ghost(0);
-                        pts[np++] = interior(0);
-                    } else if (m == nl[dir]) {
-                        pts[np++] = interior(m - 1);
-                        if (!is_wall(b.bc[2 * dir + 1])) pts[np++] = ghost(m - 1);
-                    } else {
-                        pts[np++] = interior(m - 1);
-                        pts[np++] = interior(m);
-                    }
-                    for (int mc = m - 1; mc <= m; ++mc) {
-                        if (mc < 0 || mc >= nl[dir]) continue;
-                        int i, j, k;
-                        ijk(mc, i, j, k);
-                        pts[np++] = face(cell_face(d1, 0, i, j, k));
-                        pts[np++] = face(cell_face(d1, 1, i, j, k));
-                        pts[np++] = face(cell_face(d2, 0, i, j, k));
-                        pts[np++] = face(cell_face(d2, 1, i, j, k));
-                    }
-                    double wx[10], wy[10], wz[10];
-                    face_weights(fc, pts, np, wx, wy, wz);
-                    for (int q = 0; q < 10; ++q) {
-                        lsq[(size_t)q * nft + ff] = wx[q];
-                        lsq[(size_t)(10 + q) * nft + ff] = wy[q];
-                        lsq[(size_t)(20 + q) * nft + ff] = wz[q];
-                    }
-                }
-    }
-}
 
 size_t face_count(const HostBlock& b)
 {
@@ -226,13 +86,23 @@ void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vecto
     HIPCHK(hipMalloc(&b.F, sizeof(double) * 6 * nft));
     bd.F = b.F;
     if (!s->viscous) return;
+    // Centres as (x, y, z, 0) records: one cloud point is two vector loads.
+    auto records = [](const std::vector<double>& soa, size_t m) {
+        std::vector<double> r(4 * m, 0.0);
+        for (size_t q = 0; q < m; ++q)
+            for (int d = 0; d < 3; ++d) r[4 * q + d] = soa[d * m + q];
+        return r;
+    };
+    const size_t n = (size_t)b.n;
+    const std::vector<double> fr = records(fpos, nft), cr = records(b.pos, n);
     HIPCHK(hipMalloc(&b.FS, sizeof(double) * 4 * nft));
-    std::vector<double> lsq;
-    block_weights(b, fpos, lsq);
-    HIPCHK(hipMalloc(&b.lsq, sizeof(double) * lsq.size()));
-    HIPCHK(hipMemcpy(b.lsq, lsq.data(), sizeof(double) * lsq.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&b.fpos, sizeof(double) * 4 * nft));
+    HIPCHK(hipMalloc(&b.cpos, sizeof(double) * 4 * n));
+    HIPCHK(hipMemcpy(b.fpos, fr.data(), sizeof(double) * 4 * nft, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.cpos, cr.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
     bd.FS = b.FS;
-    bd.lsq = b.lsq;
+    bd.fpos = b.fpos;
+    bd.cpos = b.cpos;
 }
 
 void split_block_gas(chkn_sim* s, HostBlock& b, BlockDev& bd)
@@ -326,8 +196,8 @@ void split_tables(chkn_sim* s)
 void split_free(chkn_sim* s)
 {
     for (auto& b : s->blocks) {
-        dfree(b.F); dfree(b.FS); dfree(b.lsq); dfree(b.gas);
-        b.F = b.FS = b.lsq = b.gas = nullptr;
+        dfree(b.F); dfree(b.FS); dfree(b.fpos); dfree(b.cpos); dfree(b.gas);
+        b.F = b.FS = b.fpos = b.cpos = b.gas = nullptr;
     }
     dfree(s->d_split_faces); dfree(s->d_split_cells); dfree(s->d_inflow_gas);
     s->d_split_faces = nullptr;

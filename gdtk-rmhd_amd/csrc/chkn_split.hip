@@ -211,10 +211,9 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
                 if (code == 2) T = B->twall[2 * dir + side];
             }
         }
-        B->FS[p.ff] = T;
-        B->FS[nft + p.ff] = vx;
-        B->FS[2 * nft + p.ff] = vy;
-        B->FS[3 * nft + p.ff] = vz;
+        double2* rec = reinterpret_cast<double2*>(B->FS) + 2 * (long long)p.ff;
+        rec[0] = make_double2(T, vx);
+        rec[1] = make_double2(vy, vz);
     }
 }
 
@@ -227,33 +226,90 @@ __device__ __forceinline__ int cell_face(const BlockDev* B, int d, int hi, int i
     return (int)(B->nfi + B->nfj) + i + nic * (j + njc * (k + hi));
 }
 
-struct Grad {
-    double T[3], x[3], y[3], z[3];  // gradients of T, vx, vy, vz
-};
-
-// One cloud point of weight index q: accumulate w_q * (value - face value) in the
-// reference's order (block.cu:109-131).
-__device__ __forceinline__ void cloud_add(Grad& G, const double* __restrict__ lsq, long long nft,
-                                          int ff, int q, double T0, double vx0, double vy0,
-                                          double vz0, double T, double vx, double vy, double vz)
+// 32-byte records: (x, y, z, unused) centres and (T, vx, vy, vz) face states.
+__device__ __forceinline__ void ld_rec4(const double* base, long long r, double& a, double& b,
+                                        double& c, double& d)
 {
-    const double wx = lsq[(long long)q * nft + ff];
-    const double wy = lsq[(long long)(10 + q) * nft + ff];
-    const double wz = lsq[(long long)(20 + q) * nft + ff];
-    double dq = T - T0;
-    G.T[0] += wx * dq; G.T[1] += wy * dq; G.T[2] += wz * dq;
-    dq = vx - vx0;
-    G.x[0] += wx * dq; G.x[1] += wy * dq; G.x[2] += wz * dq;
-    dq = vy - vy0;
-    G.y[0] += wx * dq; G.y[1] += wy * dq; G.y[2] += wz * dq;
-    dq = vz - vz0;
-    G.z[0] += wx * dq; G.z[1] += wy * dq; G.z[2] += wz * dq;
+    const double2* q = reinterpret_cast<const double2*>(base) + 2 * r;
+    const double2 u = q[0], v = q[1];
+    a = u.x; b = u.y; c = v.x; d = v.y;
+}
+__device__ __forceinline__ void ld_rec(const double* base, long long r, double& x, double& y,
+                                       double& z)
+{
+    const double2* q = reinterpret_cast<const double2*>(base) + 2 * r;
+    const double2 u = q[0];
+    x = u.x; y = u.y; z = q[1].x;
 }
 
-// Cloud (block.cu:317-358, :418-459, :519-560): the cells beside the face, minus side
-// first -- on a boundary the interior cell and, unless the side is a wall, the first ghost
-// cell -- then for each of those interior cells its faces in the two other directions,
-// lower direction first, low side before high side.
+// The cloud of face p (block.cu:317-358, :418-459, :519-560), in the order the reference
+// lists it: the cells beside the face, minus side first -- on a boundary the interior cell
+// and, unless the side is a wall, the first ghost cell -- then for each of those interior
+// cells its faces in the two other directions, lower direction first, low side first.
+// visit(kind, index): kind 0 = cell at line position index (a ghost when outside the
+// block), kind 1 = face index in the face list.
+template <class Visit>
+__device__ __forceinline__ void visit_cloud(const BlockDev* B, int dir, const FacePos& p,
+                                            Visit&& visit)
+{
+    const int nl = line_length(B, dir);
+    if (p.m == 0) {
+        if (!is_wall(B->bc[2 * dir])) visit(0, -1);
+        visit(0, 0);
+    } else if (p.m == nl) {
+        visit(0, nl - 1);
+        if (!is_wall(B->bc[2 * dir + 1])) visit(0, nl);
+    } else {
+        visit(0, p.m - 1);
+        visit(0, p.m);
+    }
+    const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+    for (int mc = p.m - 1; mc <= p.m; ++mc) {
+        if (mc < 0 || mc >= nl) continue;
+        int i, j, k;
+        if (dir == 0) { i = mc; j = p.a; k = p.c; }
+        else if (dir == 1) { i = p.a; j = mc; k = p.c; }
+        else { i = p.a; j = p.c; k = mc; }
+        visit(1, cell_face(B, d1, 0, i, j, k));
+        visit(1, cell_face(B, d1, 1, i, j, k));
+        visit(1, cell_face(B, d2, 0, i, j, k));
+        visit(1, cell_face(B, d2, 1, i, j, k));
+    }
+}
+
+// Offset of a cloud point from the face centre (fx, fy, fz) and its inverse squared
+// distance (block.cu:40-47).  Ghost cells sit at f + (f - c) for the interior cell c
+// beside the boundary face (block.cu:656-780).
+__device__ __forceinline__ void cloud_offset(const BlockDev* B, int dir, const FacePos& p,
+                                             double fx, double fy, double fz,
+                                             int kind, int idx, double& dx, double& dy,
+                                             double& dz, double& w2)
+{
+    double x, y, z;
+    if (kind == 1) {
+        ld_rec(B->fpos, idx, x, y, z);
+    } else {
+        const int nl = line_length(B, dir);
+        const bool ghost = idx < 0 || idx >= nl;
+        const int c = cell_index(B->nic, B->njc, dir, ghost ? (idx < 0 ? 0 : nl - 1) : idx, p.a, p.c);
+        ld_rec(B->cpos, c, x, y, z);
+        if (ghost) {
+            const double ex = fx - x, ey = fy - y, ez = fz - z;
+            x = fx + ex;
+            y = fy + ey;
+            z = fz + ez;
+        }
+    }
+    dx = x - fx;
+    dy = y - fy;
+    dz = z - fz;
+    w2 = 1.0 / (dx * dx + dy * dy + dz * dz);
+}
+
+// Viscous flux of every face (block.cu:88-162): least-squares gradients of T and velocity
+// over the face's cloud, with the weights of block.cu:28-86 (normal matrix inverted by
+// cofactors, rsla.cu:53-72) formed on the fly from the cell and face centres -- two passes
+// over the cloud, the second recomputing each offset -- rather than read from memory.
 __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A)
 {
     const FaceChunk ch = A.faces[blockIdx.x];
@@ -264,56 +320,78 @@ __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A)
     const FacePos p = face_pos(B, dir, f);
     const long long nft = B->nfi + B->nfj + B->nfk;
     const double* __restrict__ FS = B->FS;
-    const double* __restrict__ lsq = B->lsq;
-    const double T0 = FS[p.ff], vx0 = FS[nft + p.ff], vy0 = FS[2 * nft + p.ff],
-                 vz0 = FS[3 * nft + p.ff];
-    Grad G;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) G.T[r] = G.x[r] = G.y[r] = G.z[r] = 0.0;
-    const int nl = line_length(B, dir);
-    int q = 0;
-    auto add_cell = [&](int m) {
-        const Point pt = point<true>(A, B, dir, m, p.a, p.c);
-        cloud_add(G, lsq, nft, p.ff, q++, T0, vx0, vy0, vz0, pt.g.T, pt.w.vx, pt.w.vy, pt.w.vz);
-    };
-    if (p.m == 0) {
-        if (!is_wall(B->bc[2 * dir])) add_cell(-1);
-        add_cell(0);
-    } else if (p.m == nl) {
-        add_cell(nl - 1);
-        if (!is_wall(B->bc[2 * dir + 1])) add_cell(nl);
-    } else {
-        add_cell(p.m - 1);
-        add_cell(p.m);
+    double fx, fy, fz;
+    ld_rec(B->fpos, p.ff, fx, fy, fz);
+    // Normal matrix of the cloud.
+    double xx = 0.0, xy = 0.0, xz = 0.0, yy = 0.0, yz = 0.0, zz = 0.0;
+    visit_cloud(B, dir, p, [&](int kind, int idx) {
+        double dx, dy, dz, w2;
+        cloud_offset(B, dir, p, fx, fy, fz, kind, idx, dx, dy, dz, w2);
+        xx += w2 * dx * dx;
+        xy += w2 * dx * dy;
+        xz += w2 * dx * dz;
+        yy += w2 * dy * dy;
+        yz += w2 * dy * dz;
+        zz += w2 * dz * dz;
+    });
+    // Inverse by cofactors of the symmetric matrix [[xx,xy,xz],[xy,yy,yz],[xz,yz,zz]]; a
+    // singular matrix (|det| <= 1e-16) gives zero weights, as in the reference.
+    double m00 = 0.0, m01 = 0.0, m02 = 0.0, m10 = 0.0, m11 = 0.0, m12 = 0.0, m20 = 0.0,
+           m21 = 0.0, m22 = 0.0;
+    const double det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+    if (fabs(det) > 1.0e-16) {
+        const double r = 1.0 / det;
+        m00 = (yy * zz - yz * yz) * r;
+        m01 = (xz * yz - xy * zz) * r;
+        m02 = (xy * yz - xz * yy) * r;
+        m10 = (yz * xz - xy * zz) * r;
+        m11 = (xx * zz - xz * xz) * r;
+        m12 = (xz * xy - xx * yz) * r;
+        m20 = (xy * yz - yy * xz) * r;
+        m21 = (xy * xz - xx * yz) * r;
+        m22 = (xx * yy - xy * xy) * r;
     }
-    const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
-    for (int mc = p.m - 1; mc <= p.m; ++mc) {
-        if (mc < 0 || mc >= nl) continue;
-        int i, j, k;
-        if (dir == 0) { i = mc; j = p.a; k = p.c; }
-        else if (dir == 1) { i = p.a; j = mc; k = p.c; }
-        else { i = p.a; j = p.c; k = mc; }
-        const int fl[4] = {cell_face(B, d1, 0, i, j, k), cell_face(B, d1, 1, i, j, k),
-                           cell_face(B, d2, 0, i, j, k), cell_face(B, d2, 1, i, j, k)};
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int o = fl[r];
-            cloud_add(G, lsq, nft, p.ff, q++, T0, vx0, vy0, vz0, FS[o], FS[nft + o],
-                      FS[2 * nft + o], FS[3 * nft + o]);
+    double T0, vx0, vy0, vz0;
+    ld_rec4(FS, p.ff, T0, vx0, vy0, vz0);
+    double gTx = 0.0, gTy = 0.0, gTz = 0.0, gxx = 0.0, gxy = 0.0, gxz = 0.0;
+    double gyx = 0.0, gyy = 0.0, gyz = 0.0, gzx = 0.0, gzy = 0.0, gzz = 0.0;
+    visit_cloud(B, dir, p, [&](int kind, int idx) {
+        double dx, dy, dz, w2;
+        cloud_offset(B, dir, p, fx, fy, fz, kind, idx, dx, dy, dz, w2);
+        const double wx = w2 * (m00 * dx + m01 * dy + m02 * dz);
+        const double wy = w2 * (m10 * dx + m11 * dy + m12 * dz);
+        const double wz = w2 * (m20 * dx + m21 * dy + m22 * dz);
+        double T, vx, vy, vz;
+        if (kind == 1) {
+            ld_rec4(FS, idx, T, vx, vy, vz);
+        } else {
+            const Point pt = point<true>(A, B, dir, idx, p.a, p.c);
+            T = pt.g.T;
+            vx = pt.w.vx;
+            vy = pt.w.vy;
+            vz = pt.w.vz;
         }
-    }
+        double dq = T - T0;
+        gTx += wx * dq; gTy += wy * dq; gTz += wz * dq;
+        dq = vx - vx0;
+        gxx += wx * dq; gxy += wy * dq; gxz += wz * dq;
+        dq = vy - vy0;
+        gyx += wx * dq; gyy += wy * dq; gyz += wz * dq;
+        dq = vz - vz0;
+        gzx += wx * dq; gzy += wy * dq; gzz += wz * dq;
+    });
     // Transport coefficients at the face temperature (gas.cu:95-100), stresses and heat
     // flux (block.cu:134-161).
     const double mu = kMuRef * sqrt(T0 / kTRef) * (T0 / kTRef) * (kTRef + kSuth) / (T0 + kSuth);
     const double kcond = kCp * mu / kPrandtl;
     const double lmbda = -2.0 / 3.0 * mu;
-    const double tau_xx = 2.0 * mu * G.x[0] + lmbda * (G.x[0] + G.y[1] + G.z[2]);
-    const double tau_yy = 2.0 * mu * G.y[1] + lmbda * (G.x[0] + G.y[1] + G.z[2]);
-    const double tau_zz = 2.0 * mu * G.z[2] + lmbda * (G.x[0] + G.y[1] + G.z[2]);
-    const double tau_xy = mu * (G.x[1] + G.y[0]);
-    const double tau_xz = mu * (G.x[2] + G.z[0]);
-    const double tau_yz = mu * (G.y[2] + G.z[1]);
-    const double qx = kcond * G.T[0], qy = kcond * G.T[1], qz = kcond * G.T[2];
+    const double tau_xx = 2.0 * mu * gxx + lmbda * (gxx + gyy + gzz);
+    const double tau_yy = 2.0 * mu * gyy + lmbda * (gxx + gyy + gzz);
+    const double tau_zz = 2.0 * mu * gzz + lmbda * (gxx + gyy + gzz);
+    const double tau_xy = mu * (gxy + gyx);
+    const double tau_xz = mu * (gxz + gzx);
+    const double tau_yz = mu * (gyz + gzy);
+    const double qx = kcond * gTx, qy = kcond * gTy, qz = kcond * gTz;
     const double* gf = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
     const int nf = (int)(dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk));
     const double nx = gf[nf + f], ny = gf[2 * nf + f], nz = gf[3 * nf + f];

@@ -69,7 +69,7 @@ typedef struct {
     double t0;               /* start time (times.data entry for the start index) */
     int start_tindx;
     int reacting;            /* must be 0: not built */
-    int viscous;             /* 1 = add viscous fluxes (block.cu:88-162) */
+    int viscous;             /* 1 = add viscous fluxes (block.cu:88-162; simulate.cu:201,235-238) */
     const double* bc_twall;  /* 6 per block: TWall of wall_no_slip_fixed_T sides (NaN: TWall
                                 from a function, not built); may be NULL without such walls */
 } chkn_desc;
@@ -101,7 +101,7 @@ int chkn_create(const chkn_desc* desc, chkn_sim** out);
 
 /* Ensemble form of initialize_simulation (simulate.cu:55-155): ninst independent jobs in one
  * handle.  Each instance keeps its own block array, inflow states, time, time step, CFL and
- * plot schedules; all instances must share x_order and cfl_count.  Every stage of every
+ * plot schedules; all instances must share x_order, viscous and cfl_count.  Every stage of every
  * instance is computed by one launch over all blocks (the reference runs one job per
  * process and one launch per block, simulate.cu:605-625).  Blocks are numbered globally:
  * instance q owns blocks [first, first+count) of chkn_instance_blocks.  chkn_step advances
