@@ -275,3 +275,57 @@ def viscous_plate(nics=(10, 8), njc=4, nkcs=(8, 6), max_step=30, seed=5, x_order
                     for key, val in st.items()}
             prep.FluidBlock(i=i, j=0, k=k, grid=sub, initialState=init, bcs=bcs)
     return JobData.from_prep()
+
+
+def bl_profile_numbers(p_e=1.013e3, T_e=300.0, u_e=1390.0, T_peak=480.0, delta=0.004):
+    """A laminar boundary-layer profile in the layout of InflowBoundaryLayerBC.data: p_e,
+    then (y, T, velx) at 12 samples from the wall (y = 0, velx = 0, T = T_e) to the edge
+    (y = delta, free-stream values), samples clustered toward the wall."""
+    out = [p_e]
+    for i in range(12):
+        s = i / 11.0
+        y = delta * s ** 1.4
+        eta = y / delta
+        T = T_e + (T_peak - T_e) * math.sin(math.pi * eta) * (1.0 - eta) * 1.6
+        u = u_e * math.tanh(3.2 * eta) / math.tanh(3.2)
+        out += [y, T, u]
+    return out
+
+
+def bl_plate(nics=(10, 8), njc=3, nkc=10, max_step=30, seed=4, x_order=2, twall_fun=True):
+    """Supersonic flow over a plate entering through a laminar boundary-layer profile
+    (the layout of examples/chicken/boundary-layer/short.py on two blocks): i along x,
+    j across a thin periodic span (-z), k from the plate (y = 0) upward, clustered to the
+    wall.  iminus: inflow_function laminar_boundary_layer; kminus: no-slip wall at a TWall
+    varying along x (given per face, as the TWall files hold it) or 300 K; kplus: inflow;
+    iplus: outflow; j: periodic exchange.  Initial state: the free stream with smooth
+    variations.  Returns (JobData, profile numbers); the job reads the profile given."""
+    _common(max_step, title="laminar boundary-layer inflow (seed %d)" % seed, cfl=0.4, dt_init=1.0e-8)
+    prep.config.viscous = True
+    prep.config.x_order = x_order
+    L0, L, H, W = 0.5, 0.53, 0.012, 0.002
+    NI = sum(nics)
+    xs = L0 + (L - L0) * np.linspace(0.0, 1.0, NI + 1)
+    ys = H * np.linspace(0.0, 1.0, nkc + 1) ** 1.5
+    zs = W * (1.0 - np.linspace(0.0, 1.0, njc + 1))
+    X, Z, Y = np.meshgrid(xs, zs, ys, indexing="ij")
+    rng = np.random.default_rng(seed)
+    X = X + 0.02 * (L - L0) / NI * rng.uniform(-1, 1, X.shape) * (ys[None, None, :] > 0)
+    g = StructuredGrid(x=X, y=Y, z=Z)
+    fs = prep.FlowState(p=1.013e3, T=300.0, velx=1390.0)
+    cc = cell_centres(g)
+    T = 300.0 * (1.0 + 0.03 * np.sin(2 * np.pi * (cc.x - L0) / (L - L0)) * np.exp(-cc.y / H))
+    st = _state_arrays(1.013e3, T, 1390.0 * (1.0 - 0.3 * np.exp(-cc.y / (0.2 * H))))
+    st["vely"] = 10.0 * np.sin(2 * np.pi * cc.y / H)
+    twall = (lambda x, y, z: 300.0 + 150.0 * (x - L0) / (L - L0)) if twall_fun else 300.0
+    io = np.cumsum([0] + list(nics))
+    for i in range(2):
+        bcs = {"iminus": prep.InflowFunctionBC("laminar_boundary_layer") if i == 0 else prep.ExchangeBC(),
+               "iplus": prep.OutflowBC() if i == 1 else prep.ExchangeBC(),
+               "jminus": prep.ExchangeBC(), "jplus": prep.ExchangeBC(),
+               "kminus": prep.WallNoSlipFixedTBC(twall), "kplus": prep.InflowBC(fs)}
+        sub = g.subgrid(i0=io[i], j0=0, k0=0, niv=nics[i] + 1, njv=njc + 1, nkv=nkc + 1)
+        init = {key: val[io[i]:io[i + 1]] if np.ndim(val) == 3 else val for key, val in st.items()}
+        prep.FluidBlock(i=i, j=0, k=0, grid=sub, initialState=init, bcs=bcs)
+    prof = bl_profile_numbers()
+    return JobData.from_prep(bl_profile=prof), prof

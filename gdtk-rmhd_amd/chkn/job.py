@@ -15,6 +15,11 @@ BC_CODES = {"wall_with_slip": 0, "wall_no_slip_adiabatic": 1, "wall_no_slip": 1,
             "wall_no_slip_fixed_T": 2, "exchange": 3, "inflow": 4, "outflow": 5,
             "inflow_function": 6}
 FACE_NAMES = ("iminus", "iplus", "jminus", "jplus", "kminus", "kplus")
+# face.cu:51-68: inflow functions; unknown names mean none.
+BC_FUNCTIONS = {"none": 0, "supersonic_vortex": 1, "laminar_boundary_layer": 2,
+                "manufactured_solution": 3}
+BL_SAMPLES = 12  # config.cu:31
+BL_PROFILE_FILE = "InflowBoundaryLayerBC.data"
 
 
 def bc_code(tag):
@@ -22,9 +27,35 @@ def bc_code(tag):
     return BC_CODES.get(tag, 0)
 
 
+def read_bl_profile(path):
+    """Laminar boundary-layer profile (config.cu:37-60): p_e, the sample count (must be 12),
+    then y, T, velx per sample.  Returns the 37 numbers p_e, y0, T0, velx0, y1, ..."""
+    if not os.path.isfile(path):
+        raise RuntimeError("Boundary-layer file is not present in current directory.")
+    with open(path) as f:
+        tok = f.read().split()
+    if int(tok[1]) != BL_SAMPLES:
+        raise RuntimeError("Boundary-layer file has unexpected number of sample points.")
+    return [float(tok[0])] + [float(v) for v in tok[2:2 + 3 * BL_SAMPLES]]
+
+
+def write_bl_profile(path, numbers):
+    """Write a profile in the file layout read_bl_profile reads (17 significant digits)."""
+    with open(path, "w") as f:
+        f.write("%.17g\n%d\n" % (numbers[0], BL_SAMPLES))
+        for i in range(BL_SAMPLES):
+            f.write("%.17g %.17g %.17g\n" % tuple(numbers[1 + 3 * i:4 + 3 * i]))
+
+
+def boundary_faces(blk, side):
+    """Number of faces on a block side."""
+    d = side // 2
+    return (blk.njc * blk.nkc, blk.nic * blk.nkc, blk.nic * blk.njc)[d]
+
+
 class JobBlock:
     __slots__ = ("i", "j", "k", "nic", "njc", "nkc", "active", "bc", "bc_fs", "bc_twall", "vertices",
-                 "iovars", "label")
+                 "iovars", "label", "bc_fun", "twall_faces")
 
     @property
     def ncells(self):
@@ -54,6 +85,7 @@ class JobData:
         self.start_tindx = 0
         self.reacting = False
         self.viscous = False
+        self.bl_profile = None  # read_bl_profile() numbers when a side uses the BL function
 
     # ------------------------------------------------------------------ construction
     def _apply_config(self, cfg):
@@ -90,13 +122,29 @@ class JobData:
             # config.cu:463-467; NaN marks a TWall given as a function (not built).
             jb.bc_twall = [(float("nan") if b["bcs"][f].get("form") == "fun" else float(b["bcs"][f].get("value", 0.0)))
                            if jb.bc[q] == 2 else 0.0 for q, f in enumerate(FACE_NAMES)]
+            jb.bc_fun = [BC_FUNCTIONS.get(b["bcs"][f].get("fun_name", "none"), 0) if jb.bc[q] == 6 else 0
+                         for q, f in enumerate(FACE_NAMES)]
+            jb.twall_faces = {}  # side -> TWall per boundary face, for function-form walls
             jb.label = b.get("label", "")
             blocks.append(jb)
         self.blocks = blocks
 
+    def _read_side_data(self, twall_reader, bl_profile=None):
+        """Per-face TWall of function-form walls and, when a side uses the laminar
+        boundary-layer inflow, the profile given or else the one in the current directory
+        (config.cu:476)."""
+        for jb in self.blocks:
+            for q, f in enumerate(FACE_NAMES):
+                if jb.bc[q] == 2 and np.isnan(jb.bc_twall[q]):
+                    jb.twall_faces[q] = np.asarray(twall_reader(jb, f), dtype=float)[:boundary_faces(jb, q)]
+        if any(jb.bc[q] == 6 and jb.bc_fun[q] == 2 for jb in self.blocks for q in range(6)):
+            self.bl_profile = (list(bl_profile) if bl_profile is not None
+                               else read_bl_profile(os.path.join(".", BL_PROFILE_FILE)))
+
     @classmethod
-    def from_prep(cls):
-        """From the job described through chkn.prep (no files written)."""
+    def from_prep(cls, bl_profile=None):
+        """From the job described through chkn.prep (no files written).  bl_profile: the
+        boundary-layer profile numbers (else ./InflowBoundaryLayerBC.data when used)."""
         from . import prep
         cfg = json.loads(prep.config_json_text())
         jd = cls()
@@ -104,10 +152,14 @@ class JobData:
         for jb, fb in zip(jd.blocks, prep.fluidBlocksList):
             jb.vertices = fb.grid.xyz_flat()
             jb.iovars = fb.iovars()
+        # The values as the TWall files hold them ('%g').
+        blocks = {(fb.i, fb.j, fb.k): fb for fb in prep.fluidBlocksList}
+        jd._read_side_data(lambda jb, f: [float("%g" % v) for v in blocks[(jb.i, jb.j, jb.k)].twall_values[f]],
+                           bl_profile)
         return jd
 
     @classmethod
-    def from_job_dir(cls, job, tindx=0, binary=True):
+    def from_job_dir(cls, job, tindx=0, binary=True, bl_profile=None):
         with open(os.path.join(job, "config.json")) as f:
             cfg = json.load(f)
         jd = cls()
@@ -121,6 +173,13 @@ class JobData:
                                                     "flow-%04d-%04d-%04d%s" % (jb.i, jb.j, jb.k, ext)),
                                        jb.ncells, binary)
         jd.t0, jd.start_tindx = read_times(job, tindx)
+
+        def twall(jb, f):  # block.cu:566-611
+            import gzip
+            path = os.path.join(job, "TWall-%04d-%04d-%04d-%s.gz" % (jb.i, jb.j, jb.k, f))
+            with gzip.open(path, "rt") as fh:
+                return [float(v) for v in fh.read().split()]
+        jd._read_side_data(twall, bl_profile)
         return jd
 
     # ------------------------------------------------------------------ flattening
@@ -143,6 +202,10 @@ class JobData:
             "dt_plot_times": dbl(self.dt_plot_times),
             "dt_plot_values": dbl(self.dt_plot_values),
             "bc_twall": dbl([getattr(b, "bc_twall", [0.0] * 6) for b in self.blocks]).ravel(),
+            "bc_fun": ints([getattr(b, "bc_fun", [0] * 6) for b in self.blocks]).ravel(),
+            "bl_profile": dbl(self.bl_profile if self.bl_profile is not None else []),
+            "twall_faces": dbl(np.concatenate([np.zeros(0)] + [
+                b.twall_faces[q] for b in self.blocks for q in sorted(getattr(b, "twall_faces", {}))])),
         }
 
     @property

@@ -27,7 +27,8 @@ class ChknDesc(ctypes.Structure):
                 ("n_cfl", ctypes.c_int), ("cfl_times", _D), ("cfl_values", _D),
                 ("n_dt_plot", ctypes.c_int), ("dt_plot_times", _D), ("dt_plot_values", _D),
                 ("t0", ctypes.c_double), ("start_tindx", ctypes.c_int), ("reacting", ctypes.c_int),
-                ("viscous", ctypes.c_int), ("bc_twall", _D)]
+                ("viscous", ctypes.c_int), ("bc_twall", _D), ("bc_fun", _I), ("bl_profile", _D),
+                ("n_twall_faces", ctypes.c_int), ("twall_faces", _D)]
 
 
 class ChknReport(ctypes.Structure):
@@ -157,6 +158,10 @@ def _desc(job, keep):
     d.n_dt_plot = len(job.dt_plot_times)
     d.t0, d.start_tindx = job.t0, job.start_tindx
     d.reacting, d.viscous = int(job.reacting), int(job.viscous)
+    d.bc_fun = _ptr(f["bc_fun"], _I)
+    d.bl_profile = _ptr(f["bl_profile"], _D) if len(f["bl_profile"]) else None
+    d.n_twall_faces = len(f["twall_faces"])
+    d.twall_faces = _ptr(f["twall_faces"], _D) if len(f["twall_faces"]) else None
     return d
 
 

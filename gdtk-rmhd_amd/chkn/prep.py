@@ -16,6 +16,7 @@ per-cell arrays, so large blocks need no per-cell Python call.
 Use ``reset()`` to start a new job in the same process.
 """
 import collections
+import gzip
 import json
 import math
 import os
@@ -268,6 +269,50 @@ class WallNoSlipFixedTBC(BoundaryCondition):
     def to_json(self):
         return '{"tag": "%s", "form": "%s", "value": %g}' % (self.tag, self.form, self.value)
 
+    def generate_values(self, grid, face):
+        """TWall at the midpoint of every face of the block side, in the order the solver
+        reads them (chkn_prep.py:479-545, block.cu:566-611): i-sides k-major, j-sides
+        k-major, k-sides j-major.  Midpoints are 0.25*(p0+p1+p2+p3) of the four face
+        vertices, with the vertex order of the reference's j- and k-sides; its i-side slices
+        are malformed (they cannot broadcast), so the i-sides take the same pattern."""
+        v = grid.vertices
+        nic, njc, nkc = grid.niv - 1, grid.njv - 1, grid.nkv - 1
+        def mid(p0, p1, p2, p3):
+            return tuple(0.25 * (p0[q] + p1[q] + p2[q] + p3[q]) for q in range(3))
+        xyz = (v.x, v.y, v.z)
+        values = []
+        if face in ("iminus", "iplus"):
+            i = 0 if face == "iminus" else nic
+            m = mid([a[i, :-1, :-1] for a in xyz], [a[i, 1:, :-1] for a in xyz],
+                    [a[i, 1:, 1:] for a in xyz], [a[i, :-1, 1:] for a in xyz])
+            for k in range(nkc):
+                for j in range(njc):
+                    values.append(self.fun(m[0][j, k], m[1][j, k], m[2][j, k]))
+        elif face in ("jminus", "jplus"):
+            j = 0 if face == "jminus" else njc
+            m = mid([a[1:, j, :-1] for a in xyz], [a[:-1, j, :-1] for a in xyz],
+                    [a[:-1, j, 1:] for a in xyz], [a[1:, j, 1:] for a in xyz])
+            for k in range(nkc):
+                for i in range(nic):
+                    values.append(self.fun(m[0][i, k], m[1][i, k], m[2][i, k]))
+        elif face in ("kminus", "kplus"):
+            k = 0 if face == "kminus" else nkc
+            m = mid([a[:-1, :-1, k] for a in xyz], [a[1:, :-1, k] for a in xyz],
+                    [a[:-1, 1:, k] for a in xyz], [a[1:, 1:, k] for a in xyz])
+            for j in range(njc):
+                for i in range(nic):
+                    values.append(self.fun(m[0][i, j], m[1][i, j], m[2][i, j]))
+        else:
+            raise RuntimeError("Unknown face: " + str(face))
+        return [float(val) for val in values]
+
+
+def write_twall_file(fileName, values):
+    """TWall-*.gz: one value per line, '%g' as chkn_prep.py:469-472 writes them."""
+    with gzip.open(fileName, "wt") as f:
+        for val in values:
+            f.write("%g\n" % val)
+
 
 class InflowBC(BoundaryCondition):
     __slots__ = ["tag", "fs", "fsi"]
@@ -326,7 +371,7 @@ class FluidBlock:
     """One structured block: grid, initial flow and boundary conditions."""
 
     __slots__ = ("indx", "active", "i", "j", "k", "grid", "initialState", "nic", "njc", "nkc",
-                 "bcs", "label", "cellc", "cellv", "flow")
+                 "bcs", "label", "cellc", "cellv", "flow", "twall_values")
 
     def __init__(self, i=0, j=0, k=0, grid=None, initialState=None, bcs={}, active=True, label=""):
         if not isinstance(grid, StructuredGrid):
@@ -368,6 +413,10 @@ class FluidBlock:
             raise RuntimeError("Need to supply a FlowState object or a function that produces"
                                " a FlowState object as a function of (x,y,z) location.")
         self.bcs = _resolve_bcs(bcs)
+        # Per-face TWall of function-form walls, kept per block (chkn_prep.py:688-692 keeps
+        # them on the boundary-condition object, which blocks may share).
+        self.twall_values = {face: bc.generate_values(grid, face) for face, bc in self.bcs.items()
+                             if type(bc) is WallNoSlipFixedTBC and bc.form == "fun"}
         for bc in bcs.values():
             if type(bc) is InflowBC:
                 bc.fsi = len(flowStatesList)
@@ -503,6 +552,9 @@ def write_initial_files(binaryData, job_dir=None):
     for fb in fluidBlocksList:
         name = os.path.join(flow_dir, "flow-%04d-%04d-%04d" % (fb.i, fb.j, fb.k))
         fb.write_flow_data_to_file(name + (".bin" if binaryData else ".gz"), binaryData)
+        for face, values in fb.twall_values.items():  # chkn_prep.py:919-925
+            write_twall_file(os.path.join(job, "TWall-%04d-%04d-%04d-%s.gz" % (fb.i, fb.j, fb.k, face)),
+                             values)
     with open(os.path.join(job, "times.data"), "w") as fp:
         fp.write("# tindx t\n")
         fp.write("0 0.0\n")

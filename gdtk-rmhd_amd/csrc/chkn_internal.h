@@ -60,6 +60,11 @@ struct BlockDev {
     const double* cpos;
     const double* gas;     // null when the block's file state is not needed
     double twall[6];       // TWall of each wall_no_slip_fixed_T side
+    // Per side, or null: inflow_function sides hold the ghost states of depth 0 and 1
+    // ([2][9][nt]: rho, e, YB, vx, vy, vz, p, T, a) then the face state ([4][nt]: T, vx, vy,
+    // vz); fixed-T walls with TWall per face hold [nt] values.  nt = faces of the side,
+    // index a + na*c as in the ghost lines.
+    const double* side[6];
 };
 
 struct Tile {

@@ -53,6 +53,15 @@ int bc_code(const std::string& name)
     return 0;
 }
 
+// face.cu:61-68: unknown names fall back to none.
+int bc_function(const std::string& name)
+{
+    if (name == "supersonic_vortex") return 1;
+    if (name == "laminar_boundary_layer") return 2;
+    if (name == "manufactured_solution") return 3;
+    return 0;
+}
+
 const char* kFaceNames[6] = {"iminus", "iplus", "jminus", "jplus", "kminus", "kplus"};
 const char* kIoNames[13] = {"posx", "posy", "posz", "vol", "p", "T", "rho", "e", "YB", "a",
                             "velx", "vely", "velz"};
@@ -121,6 +130,11 @@ JobConfig read_config_json(const std::string& path)
             if (blk.bc[f] == 2) {  // config.cu:463-467
                 blk.bc_twall[f] = bc["value"].as_num();
                 if (bc["form"].as_str() == "fun") blk.bc_twall[f] = std::nan("");
+            }
+            if (blk.bc[f] == 6) {  // config.cu:469-478; the profile is read from the current directory
+                blk.bc_fun[f] = bc_function(bc["fun_name"].as_str());
+                if (blk.bc_fun[f] == 2 && c.bl_profile.empty())
+                    c.bl_profile = read_bl_profile("./InflowBoundaryLayerBC.data");
             }
         }
         if (blk.i < 0 || blk.i >= c.nib || blk.j < 0 || blk.j >= c.njb || blk.k < 0 || blk.k >= c.nkb)
@@ -257,6 +271,47 @@ void append_times(const std::string& job, int tindx, double t)
     char buf[64];
     std::snprintf(buf, sizeof buf, "%d %g\n", tindx, t);
     f << buf;
+}
+
+std::vector<double> read_bl_profile(const std::string& path)
+{
+    std::error_code ec;
+    if (!std::filesystem::is_regular_file(path, ec))
+        throw std::runtime_error("Boundary-layer file is not present in current directory.");
+    std::istringstream in(slurp(path));
+    double p_e = 0.0;
+    int nsample = 0;
+    in >> p_e >> nsample;
+    if (nsample != 12) throw std::runtime_error("Boundary-layer file has unexpected number of sample points.");
+    std::vector<double> out(1, p_e);
+    for (int q = 0; q < 3 * nsample; ++q) {
+        double v;
+        if (!(in >> v)) throw std::runtime_error("Boundary-layer file is too short: " + path);
+        out.push_back(v);
+    }
+    return out;
+}
+
+std::vector<double> read_twall(const std::string& job, const JobBlock& b, int side, int nfaces)
+{
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "/TWall-%04d-%04d-%04d-%s.gz", b.i, b.j, b.k, kFaceNames[side]);
+    const std::string path = job + buf;
+    std::string text;
+    try {
+        text = slurp_gz(path);
+    } catch (const std::runtime_error&) {
+        throw std::runtime_error("Did not open gzipped TWall file successfully: " + path);
+    }
+    std::vector<double> v((size_t)nfaces);
+    const char* p = text.c_str();
+    for (int q = 0; q < nfaces; ++q) {
+        char* end = nullptr;
+        v[q] = std::strtod(p, &end);
+        if (end == p) throw std::runtime_error("TWall file too short: " + path);
+        p = end;
+    }
+    return v;
 }
 
 std::string block_file(const std::string& job, const char* kind, int tindx, int i, int j, int k,

@@ -17,6 +17,7 @@ struct JobBlock {
     int bc[6] = {0, 0, 0, 0, 0, 0};
     int bc_fs[6] = {0, 0, 0, 0, 0, 0};
     double bc_twall[6] = {0, 0, 0, 0, 0, 0};  // fixed-T walls; NaN when TWall is a function
+    int bc_fun[6] = {0, 0, 0, 0, 0, 0};       // inflow_function sides: face.cu:51-59 index
     std::string label;
 };
 
@@ -31,6 +32,7 @@ struct JobConfig {
     std::vector<double> cfl_times, cfl_values, dt_plot_times, dt_plot_values;
     bool reacting = false, viscous = false;
     int source_terms = 0;
+    std::vector<double> bl_profile;  // p_e, then (y, T, velx) x 12, when a side uses it
 };
 
 // Throw std::runtime_error with the reference's message text on failure.
@@ -43,6 +45,11 @@ void write_flow(const std::string& path, bool binary, const double* iovars, int 
 // Start time and next plot index for time index tindx from times.data (t = 0 if absent).
 void read_times(const std::string& job, int tindx, double& t, int& next_plot_indx);
 void append_times(const std::string& job, int tindx, double t);
+// Laminar boundary-layer profile file (config.cu:37-60): p_e, then y, T, velx of 12 samples.
+std::vector<double> read_bl_profile(const std::string& path);
+// TWall of every face of a fixed-T side given as a function: <job>/TWall-i-j-k-<face>.gz
+// (block.cu:566-611), nfaces values in the file's order.
+std::vector<double> read_twall(const std::string& job, const JobBlock& b, int side, int nfaces);
 std::string block_file(const std::string& job, const char* kind, int tindx, int i, int j, int k,
                        bool binary);
 

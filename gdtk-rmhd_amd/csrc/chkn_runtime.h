@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/chkn.h"
+#include "chkn_bl.h"
 #include "chkn_internal.h"
 
 namespace chkn {
@@ -53,6 +54,9 @@ struct HostBlock {
     int tile_first = 0, tile_count = 0;
     // face-list stage (viscous jobs; first-order jobs with verbatim gas states)
     double twall[6] = {0, 0, 0, 0, 0, 0};
+    int bc_fun[6] = {0, 0, 0, 0, 0, 0};      // inflow_function sides: function index
+    std::vector<double> twall_faces[6];      // fixed-T sides with TWall per face
+    double* side[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // device side data
     bool gas_differs = false;  // some cell's file p, T, a differ from the equation of state
     double* F = nullptr;       // 6 x nft face fluxes
     double* FS = nullptr;      // nft x 4 face T, vx, vy, vz (viscous)
@@ -202,6 +206,9 @@ bool fresh_needs_file_gas(const chkn_sim* s);
 // Per block after its geometry: face fluxes, and for viscous jobs face states and the
 // cell and face centres (fpos, [3][nft]) the least-squares gradients are taken over.
 void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vector<double>& fpos);
+// Per-side data of an active block: inflow-function ghost and face states evaluated at their
+// positions (prof: the instance's boundary-layer profile, or null), TWall per face.
+void side_data_setup(HostBlock& b, BlockDev& bd, const std::vector<double>& fpos, const bl::Profile* prof);
 // The file gas states of block b from its IO variables (re-run when they are replaced).
 void split_block_gas(chkn_sim* s, HostBlock& b, BlockDev& bd);
 // Inflow gas states from the configured p, T, YB of every flow state.

@@ -321,7 +321,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             s->inflow_w.insert(s->inflow_w.end(), w, w + 6);
             inflow_pTYB.insert(inflow_pTYB.end(), f, f + 3);
         }
-        size_t ooff = 0;
+        size_t ooff = 0, twall_used = 0;
         for (int lb = 0; lb < d->nblocks; ++lb) {
             HostBlock& b = s->blocks[blk_base + lb];
             b.inst = qi;
@@ -332,8 +332,22 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
                 b.bc[f] = d->bc_codes[6 * lb + f];
                 b.bc_fs[f] = d->bc_fs[6 * lb + f] + fs_base;
                 b.twall[f] = d->bc_twall ? d->bc_twall[6 * lb + f] : 0.0;
-                if (s->viscous && b.active && b.bc[f] == CHKN_BC_WALL_NO_SLIP_FIXED_T && !std::isfinite(b.twall[f]))
-                    throw std::invalid_argument("TWall given as a function is not built");
+                b.bc_fun[f] = d->bc_fun ? d->bc_fun[6 * lb + f] : 0;
+                b.twall_faces[f].clear();
+                if (b.bc[f] == CHKN_BC_WALL_NO_SLIP_FIXED_T && std::isnan(b.twall[f])) {
+                    // TWall per face, in block and side order (block.cu:566-611).
+                    const size_t nf = f < 2 ? (size_t)b.njc * b.nkc : (f < 4 ? (size_t)b.nic * b.nkc : (size_t)b.nic * b.njc);
+                    if (!d->twall_faces || twall_used + nf > (size_t)d->n_twall_faces)
+                        throw std::invalid_argument("TWall given per face, but twall_faces holds too few values");
+                    b.twall_faces[f].assign(d->twall_faces + twall_used, d->twall_faces + twall_used + nf);
+                    twall_used += nf;
+                }
+                if (b.active && b.bc[f] == CHKN_BC_INFLOW_FUNCTION) {  // bcs.cu:321-352
+                    if (b.bc_fun[f] < 1 || b.bc_fun[f] > 3)
+                        throw std::invalid_argument("inflow_function side without a known fun_name");
+                    if (b.bc_fun[f] == 2 && !d->bl_profile)
+                        throw std::invalid_argument("Boundary-layer file is not present in current directory.");
+                }
             }
             // block.cu:282-284 and its j/k twins.
             if (b.nic < 2) throw std::invalid_argument("Too few cells in the i-index direction.");
@@ -347,8 +361,6 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             if (I.blk_ids[((size_t)b.bi * d->njb + b.bj) * d->nkb + b.bk] != lb)
                 throw std::invalid_argument("blk_id=" + std::to_string(lb) + " is inconsistent");
             for (int f = 0; f < 6; ++f) {
-                if (b.active && b.bc[f] == CHKN_BC_INFLOW_FUNCTION)
-                    throw std::invalid_argument("inflow_function boundaries are not built");
                 if (b.active && b.bc[f] == CHKN_BC_INFLOW &&
                     (d->bc_fs[6 * lb + f] < 0 || d->bc_fs[6 * lb + f] >= d->n_flow_states))
                     throw std::invalid_argument("inflow flow_state_index out of range");
@@ -399,11 +411,14 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     for (int qi = 0; qi < ninst; ++qi) {
         const chkn_desc* d = &descs[qi];
         size_t voff = 0;
+        bl::Profile profile;  // the instance's laminar boundary-layer profile (config.cu:37-60)
+        if (d->bl_profile) profile.set(d->bl_profile);
         for (int lb = 0; lb < d->nblocks; ++lb) {
             int ib = s->inst[qi].first_block + lb;
             HostBlock& b = s->blocks[ib];
             std::vector<double> gi, gj, gk, volinv, len, fpos;
-            const bool want_fpos = s->viscous && b.active;
+            bool want_fpos = s->viscous && b.active;
+            for (int f = 0; f < 6; ++f) want_fpos = want_fpos || (b.active && b.bc[f] == CHKN_BC_INFLOW_FUNCTION);
             block_geometry(b, d->vertices + voff, gi, gj, gk, volinv, len, want_fpos ? &fpos : nullptr);
             voff += (size_t)3 * (b.nic + 1) * (b.njc + 1) * (b.nkc + 1);
             size_t n = (size_t)b.n, sz6 = sizeof(double) * 6 * n;
@@ -443,6 +458,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             upload_state(s, b, b.io0.data());
             split_block_setup(s, b, bd, fpos);
             split_block_gas(s, b, bd);
+            side_data_setup(b, bd, fpos, d->bl_profile ? &profile : nullptr);
         }
     }
     s->bdev = bdev;
@@ -790,7 +806,7 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
     *out = nullptr;
     std::string job(job_dir);
     JobConfig c;
-    std::vector<double> verts, io;
+    std::vector<double> verts, io, twall_faces;
     double t0 = 0.0;
     int next_plot = 1;
     try {
@@ -804,11 +820,18 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
             io.insert(io.end(), f.begin(), f.end());
         }
         read_times(job, tindx, t0, next_plot);
+        for (auto& b : c.blocks)
+            for (int f = 0; f < 6; ++f)
+                if (b.bc[f] == CHKN_BC_WALL_NO_SLIP_FIXED_T && std::isnan(b.bc_twall[f])) {
+                    const int nf = f < 2 ? b.njc * b.nkc : (f < 4 ? b.nic * b.nkc : b.nic * b.njc);
+                    std::vector<double> tw = read_twall(job, b, f, nf);
+                    twall_faces.insert(twall_faces.end(), tw.begin(), tw.end());
+                }
     } catch (const std::runtime_error& e) {
         std::string m = e.what();
         return set_error(nullptr, m == "Incorrect gas model data" ? CHKN_ERR_CONFIG : CHKN_ERR_IO, m);
     }
-    std::vector<int> ijk, dims, active, bcc, bcf;
+    std::vector<int> ijk, dims, active, bcc, bcf, bcfun;
     std::vector<double> twall;
     for (auto& b : c.blocks) {
         ijk.insert(ijk.end(), {b.i, b.j, b.k});
@@ -817,6 +840,7 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
         bcc.insert(bcc.end(), b.bc, b.bc + 6);
         bcf.insert(bcf.end(), b.bc_fs, b.bc_fs + 6);
         twall.insert(twall.end(), b.bc_twall, b.bc_twall + 6);
+        bcfun.insert(bcfun.end(), b.bc_fun, b.bc_fun + 6);
     }
     chkn_desc d;
     std::memset(&d, 0, sizeof d);
@@ -834,6 +858,10 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
     d.t0 = t0; d.start_tindx = next_plot - 1;
     d.reacting = c.reacting; d.viscous = c.viscous;
     d.bc_twall = twall.data();
+    d.bc_fun = bcfun.data();
+    d.bl_profile = c.bl_profile.empty() ? nullptr : c.bl_profile.data();
+    d.n_twall_faces = (int)twall_faces.size();
+    d.twall_faces = twall_faces.empty() ? nullptr : twall_faces.data();
     int rc = chkn_create(&d, out);
     if (rc == CHKN_OK) (*out)->inst[0].job = job;
     return rc;

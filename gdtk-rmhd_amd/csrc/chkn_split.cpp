@@ -58,6 +58,7 @@ bool split_always(const chkn_sim* s)
         const HostBlock& b = s->blocks[ib];
         if (!b.active) continue;
         for (int f = 0; f < 6; ++f) {
+            if (b.bc[f] == CHKN_BC_INFLOW_FUNCTION) return true;  // ghosts set by update_from_pT
             if (b.bc[f] == CHKN_BC_INFLOW && s->inflow_differs[b.bc_fs[f]]) return true;
             if (b.bc[f] == CHKN_BC_EXCHANGE) {
                 const int o = s->bdev[ib].bc_arg[f];
@@ -103,6 +104,65 @@ void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vecto
     bd.FS = b.FS;
     bd.fpos = b.fpos;
     bd.cpos = b.cpos;
+}
+
+void side_data_setup(HostBlock& b, BlockDev& bd, const std::vector<double>& fpos, const bl::Profile* prof)
+{
+    if (!b.active) return;
+    const int nic = b.nic, njc = b.njc, nkc = b.nkc;
+    const size_t n = (size_t)b.n;
+    const size_t nfi = (size_t)(nic + 1) * njc * nkc, nfj = (size_t)nic * (njc + 1) * nkc;
+    const size_t nft = face_count(b);
+    for (int bf = 0; bf < 6; ++bf) {
+        const int dir = bf / 2, hi = bf & 1;
+        const int na = dir == 0 ? njc : nic, nc = dir == 2 ? njc : nkc, nt = na * nc;
+        std::vector<double> data;
+        if (b.bc[bf] == CHKN_BC_INFLOW_FUNCTION) {
+            data.assign((size_t)22 * nt, 0.0);
+            for (int c = 0; c < nc; ++c)
+                for (int a = 0; a < na; ++a) {
+                    const int t = a + na * c;
+                    size_t ff, cell;  // boundary face and the interior cell beside it
+                    if (dir == 0) {
+                        ff = (size_t)(hi ? nic : 0) + (size_t)(nic + 1) * (a + (size_t)njc * c);
+                        cell = (size_t)(hi ? nic - 1 : 0) + (size_t)nic * (a + (size_t)njc * c);
+                    } else if (dir == 1) {
+                        ff = nfi + (size_t)a + (size_t)nic * ((hi ? njc : 0) + (size_t)(njc + 1) * c);
+                        cell = (size_t)a + (size_t)nic * ((hi ? njc - 1 : 0) + (size_t)njc * c);
+                    } else {
+                        ff = nfi + nfj + (size_t)a + (size_t)nic * (c + (size_t)njc * (hi ? nkc : 0));
+                        cell = (size_t)a + (size_t)nic * (c + (size_t)njc * (hi ? nkc - 1 : 0));
+                    }
+                    // Ghost positions f + d and f + 3d, d = f - c (block.cu:656-780).
+                    double f[3], d[3];
+                    for (int q = 0; q < 3; ++q) {
+                        f[q] = fpos[q * nft + ff];
+                        d[q] = f[q] - b.pos[q * n + cell];
+                    }
+                    const double y0 = f[1] + d[1];
+                    const double y1 = f[1] + d[1] * 3.0;
+                    const bl::State st[2] = {bl::inflow_function(b.bc_fun[bf], prof, y0),
+                                             bl::inflow_function(b.bc_fun[bf], prof, y1)};
+                    for (int dep = 0; dep < 2; ++dep) {
+                        const double v[9] = {st[dep].rho, st[dep].e, st[dep].YB, st[dep].vx, st[dep].vy,
+                                             st[dep].vz, st[dep].p, st[dep].T, st[dep].a};
+                        for (int q = 0; q < 9; ++q) data[(size_t)(dep * 9 + q) * nt + t] = v[q];
+                    }
+                    const bl::State fs = bl::inflow_function(b.bc_fun[bf], prof, f[1]);
+                    data[(size_t)18 * nt + t] = fs.T;
+                    data[(size_t)19 * nt + t] = fs.vx;
+                    data[(size_t)20 * nt + t] = fs.vy;
+                    data[(size_t)21 * nt + t] = fs.vz;
+                }
+        } else if (b.bc[bf] == CHKN_BC_WALL_NO_SLIP_FIXED_T && !b.twall_faces[bf].empty()) {
+            data = b.twall_faces[bf];
+        } else {
+            continue;
+        }
+        HIPCHK(hipMalloc(&b.side[bf], sizeof(double) * data.size()));
+        HIPCHK(hipMemcpy(b.side[bf], data.data(), sizeof(double) * data.size(), hipMemcpyHostToDevice));
+        bd.side[bf] = b.side[bf];
+    }
 }
 
 void split_block_gas(chkn_sim* s, HostBlock& b, BlockDev& bd)
@@ -198,6 +258,10 @@ void split_free(chkn_sim* s)
     for (auto& b : s->blocks) {
         dfree(b.F); dfree(b.FS); dfree(b.fpos); dfree(b.cpos); dfree(b.gas);
         b.F = b.FS = b.fpos = b.cpos = b.gas = nullptr;
+        for (auto& p : b.side) {
+            dfree(p);
+            p = nullptr;
+        }
     }
     dfree(s->d_split_faces); dfree(s->d_split_cells); dfree(s->d_inflow_gas);
     s->d_split_faces = nullptr;

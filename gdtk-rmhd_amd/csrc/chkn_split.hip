@@ -103,6 +103,14 @@ __device__ __forceinline__ Point point(const SplitArgs& A, const BlockDev* B, in
         }
         return r;
     }
+    if (code == 6) {  // inflow function: the state at the ghost cell's position
+        const int na = dir == 0 ? B->njc : B->nic;
+        const int nt = na * (dir == 2 ? B->njc : B->nkc);
+        const double* s = B->side[bf] + (long long)depth * 9 * nt + (a + na * c);
+        r.w = Prim{s[0], s[nt], s[2 * nt], s[3 * nt], s[4 * nt], s[5 * nt]};
+        if (GAS) r.g = Gas{s[6 * nt], s[7 * nt], s[8 * nt]};
+        return r;
+    }
     const int idx = cell_index(B->nic, B->njc, dir, side ? nl - 1 : 0, a, c);
     r.w = ld_prim(B->W + A.win * B->wstride, (int)B->n, idx);
     if (GAS) r.g = cell_gas(A, B, r.w, idx);
@@ -203,12 +211,21 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
         const int nl = line_length(B, dir);
         const int side = (p.m == 0) ? 0 : ((p.m == nl) ? 1 : -1);
         if (side >= 0) {
-            const int code = B->bc[2 * dir + side];
+            const int bf = 2 * dir + side;
+            const int code = B->bc[bf];
+            const double* sd = B->side[bf];
+            const int t = p.a + (dir == 0 ? B->njc : B->nic) * p.c;
             if (code == 1 || code == 2) {  // no-slip walls; fixed-T walls also set T
                 vx = 0.0;
                 vy = 0.0;
                 vz = 0.0;
-                if (code == 2) T = B->twall[2 * dir + side];
+                if (code == 2) T = sd ? sd[t] : B->twall[bf];
+            } else if (code == 6) {  // inflow function: its state at the face centre
+                const int nt = (dir == 0 ? B->njc : B->nic) * (dir == 2 ? B->njc : B->nkc);
+                T = sd[18 * nt + t];
+                vx = sd[19 * nt + t];
+                vy = sd[20 * nt + t];
+                vz = sd[21 * nt + t];
             }
         }
         double2* rec = reinterpret_cast<double2*>(B->FS) + 2 * (long long)p.ff;

@@ -84,6 +84,14 @@ __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const double*
         p.rho = s[0]; p.e = s[1]; p.YB = s[2]; p.vx = s[3]; p.vy = s[4]; p.vz = s[5];
         return p;
     }
+    if (code == 6) {  // inflow function: states at the ghost positions (bcs.cu:355-432)
+        const int na = dir == 0 ? B->njc : B->nic;
+        const int nt = na * (dir == 2 ? B->njc : B->nkc);
+        const double* s = B->side[bf] + (long long)depth * 9 * nt + (a + na * c);
+        Prim p;
+        p.rho = s[0]; p.e = s[nt]; p.YB = s[2 * nt]; p.vx = s[3 * nt]; p.vy = s[4 * nt]; p.vz = s[5 * nt];
+        return p;
+    }
     Prim in = ld_prim(B->W + win * B->wstride, (int)B->n,
                       cell_index(B->nic, B->njc, dir, side ? n - 1 : 0, a, c));
     if (code == 5) return in;  // outflow copies the adjacent interior cell (bcs.cu:435-503)

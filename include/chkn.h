@@ -71,7 +71,16 @@ typedef struct {
     int reacting;            /* must be 0: not built */
     int viscous;             /* 1 = add viscous fluxes (block.cu:88-162; simulate.cu:201,235-238) */
     const double* bc_twall;  /* 6 per block: TWall of wall_no_slip_fixed_T sides (NaN: TWall
-                                from a function, not built); may be NULL without such walls */
+                                given per face in twall_faces); may be NULL without such walls */
+    const int* bc_fun;       /* 6 per block: function of inflow_function sides (face.cu:51-59:
+                                1 supersonic_vortex, 2 laminar_boundary_layer,
+                                3 manufactured_solution); may be NULL without such sides */
+    const double* bl_profile;  /* laminar boundary-layer profile (config.cu:31-60): p_e, then
+                                  y, T, velx at each of 12 samples (37 numbers); NULL if unused */
+    int n_twall_faces;
+    const double* twall_faces; /* TWall per boundary face of every fixed-T side with NaN in
+                                  bc_twall, blocks in order, sides in order, faces in the
+                                  TWall-file order (block.cu:566-611); NULL if none */
 } chkn_desc;
 
 /* Progress of the march, filled by chkn_step and chkn_march. */

@@ -59,7 +59,7 @@ typedef struct {
     int L1, L0, R0, R1;
     FS fs; /* face-average state, face.cu:373 */
     /* viscous: boundary code (-1 interior), cloud of cells and faces, LSQ weights */
-    int bcode, nc, nf, cc[2], fcl[8];
+    int bcode, bfun, nc, nf, cc[2], fcl[8];
     double TWall, wx[10], wy[10], wz[10];
 } Face;
 
@@ -67,8 +67,9 @@ typedef struct {
     int bi, bj, bk, nic, njc, nkc, active;
     int ncells, n0c[6], n1c[6], nghost[6], first_ghost[6], ntot;
     int nif, njf, nkf, nfaces;
-    int bc[6], bcfs[6];
+    int bc[6], bcfs[6], bcfun[6];
     double twall[6];
+    double* twf[6]; /* TWall per boundary face, or NULL */
     Cell* cells;
     Face* faces;
     V3* vtx;
@@ -76,7 +77,13 @@ typedef struct {
     double* dQ; /* 3 levels */
 } Blk;
 
+/* spline.cu:20-109, 12 knots. */
+#define ORC_BL_N 11
+typedef struct { double xs[ORC_BL_N + 1], ys[ORC_BL_N + 1], a[ORC_BL_N], b[ORC_BL_N], c[ORC_BL_N]; } Spline;
+typedef struct { double p_e; Spline T, velx; int set; } BLProfile; /* config.cu:33-93 */
+
 struct orc_sim {
+    BLProfile bl;
     int nblocks, nib, njb, nkb;
     int* blk_ids;
     Blk* b;
@@ -258,6 +265,68 @@ static void update_from_pT(FS* s) /* gas.cu:119-126 */
     s->a = sqrt(GAS_G * GAS_R * s->T);
 }
 
+/* spline.cu:26-73: natural cubic spline; the interior second derivatives solve a
+ * tridiagonal system by Gauss-Jordan elimination with partial pivoting (rsla.cu:201-227).
+ * Returns -1 when a pivot is below 1e-16. */
+static int spline_set(Spline* sp, const double* x, const double* y)
+{
+    enum { n = ORC_BL_N, m = ORC_BL_N - 1 };
+    double h[n], A[m][m + 1], sigma[n + 1];
+    for (int i = 0; i <= n; ++i) { sp->xs[i] = x[i]; sp->ys[i] = y[i]; }
+    for (int i = 0; i < n; ++i) h[i] = sp->xs[i + 1] - sp->xs[i];
+    for (int r = 0; r < m; ++r) for (int q = 0; q <= m; ++q) A[r][q] = 0.0;
+    for (int k = 0; k < m; ++k) {
+        int i = k + 1;
+        if (k > 0) A[k][k - 1] = h[i - 1];
+        A[k][k] = 2.0 * (h[i - 1] + h[i]);
+        if (k < m - 1) A[k][k + 1] = h[i];
+        A[k][m] = 6.0 * ((sp->ys[i + 1] - sp->ys[i]) / h[i] - (sp->ys[i] - sp->ys[i - 1]) / h[i - 1]);
+    }
+    for (int j = 0; j < m; ++j) {
+        int p = j;
+        for (int i = j + 1; i < m; ++i) if (fabs(A[i][j]) > fabs(A[p][j])) p = i;
+        if (fabs(A[p][j]) < 1.0e-16) return -1;
+        if (p != j) for (int q = 0; q <= m; ++q) { double t = A[p][q]; A[p][q] = A[j][q]; A[j][q] = t; }
+        double d = A[j][j];
+        for (int q = 0; q <= m; ++q) A[j][q] /= d;
+        for (int i = 0; i < m; ++i) {
+            if (i == j) continue;
+            double f = A[i][j];
+            for (int q = 0; q <= m; ++q) A[i][q] -= f * A[j][q];
+        }
+    }
+    sigma[0] = 0.0; sigma[n] = 0.0;
+    for (int i = 0; i < m; ++i) sigma[i + 1] = A[i][m];
+    for (int i = 0; i < n; ++i) {
+        sp->a[i] = (sigma[i + 1] - sigma[i]) / (6.0 * h[i]);
+        sp->b[i] = sigma[i] / 2.0;
+        sp->c[i] = (sp->ys[i + 1] - sp->ys[i]) / h[i] - (2.0 * sigma[i] + sigma[i + 1]) * h[i] / 6.0;
+    }
+    return 0;
+}
+
+static double spline_eval(const Spline* sp, double x) /* spline.cu:86-99 */
+{
+    int i = 0;
+    if (!(x < sp->xs[0])) {
+        i = ORC_BL_N - 1;
+        while (x < sp->xs[i] && i > 0) i -= 1;
+    }
+    double dx = x - sp->xs[i];
+    return ((sp->a[i] * dx + sp->b[i]) * dx + sp->c[i]) * dx + sp->ys[i];
+}
+
+/* config.cu:37-60: p_e, then y, T, velx per sample. */
+static int bl_set(BLProfile* bl, const double* v)
+{
+    double ys[ORC_BL_N + 1], Ts[ORC_BL_N + 1], us[ORC_BL_N + 1];
+    bl->p_e = v[0];
+    for (int i = 0; i <= ORC_BL_N; ++i) { ys[i] = v[1 + 3 * i]; Ts[i] = v[2 + 3 * i]; us[i] = v[3 + 3 * i]; }
+    if (spline_set(&bl->T, ys, Ts) || spline_set(&bl->velx, ys, us)) return -1;
+    bl->set = 1;
+    return 0;
+}
+
 static void update_from_rhoe(FS* s) /* gas.cu:128-135 */
 {
     s->T = (s->e + s->YB * GAS_Q) / GAS_CV;
@@ -303,6 +372,29 @@ static void set_as_average(FS* s, const FS* a, const FS* b) /* flow.cu:220-225 *
     s->vel.x = half * (a->vel.x + b->vel.x);
     s->vel.y = half * (a->vel.y + b->vel.y);
     s->vel.z = half * (a->vel.z + b->vel.z);
+}
+
+/* bcs.cu:321-352: state of inflow function fn at pos.  The boundary-layer profile clamps to
+ * its end samples (config.cu:73-93); supersonic_vortex and manufactured_solution are fixed
+ * placeholder states in the reference. */
+static FS inflow_function(const orc_sim* s, int fn, V3 pos)
+{
+    FS f;
+    memset(&f, 0, sizeof f);
+    if (fn == 2) {
+        const BLProfile* bl = &s->bl;
+        double y = pos.y, T, u;
+        if (y <= bl->T.xs[0]) { T = bl->T.ys[0]; u = bl->velx.ys[0]; }
+        else if (y >= bl->T.xs[ORC_BL_N]) { T = bl->T.ys[ORC_BL_N]; u = bl->velx.ys[ORC_BL_N]; }
+        else { T = spline_eval(&bl->T, y); u = spline_eval(&bl->velx, y); }
+        f.p = bl->p_e; f.YB = 0.0; f.T = T;
+        update_from_pT(&f);
+        f.vel.x = u; f.vel.y = 0.0; f.vel.z = 0.0;
+    } else {
+        f.p = 100.0e3; f.T = 300.0; f.YB = 0.0;
+        update_from_pT(&f);
+    }
+    return f;
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -499,9 +591,14 @@ static int is_wall(int code)
 /* Clouds of cells and faces for the viscous gradients and the face's boundary code:
  * block.cu:317-358 (i-faces), :418-459 (j-faces), :519-560 (k-faces).  A wall face uses
  * only its interior cell; another boundary face also its first ghost cell. */
+/* TWall of boundary face t of a fixed-T side: the side's value, overwritten by the file's
+ * per-face values when given (block.cu:566-611). */
+static double side_twall(const Blk* b, int side, int t) { return b->twf[side] ? b->twf[side][t] : b->twall[side]; }
+
 static void set_cloud(Blk* b, Face* f, int side_bc, int c_minus, int c_plus, const int* fl, int nfl)
 {
     f->bcode = side_bc;
+    f->bfun = 0;
     f->TWall = 0.0;  /* the caller sets it on boundary faces */
     f->nc = 0;
     if (c_minus >= 0) f->cc[f->nc++] = c_minus;
@@ -521,12 +618,12 @@ static void configure_clouds(Blk* b)
                     int fl[4] = {jfi(b, i, j, k), jfi(b, i, j + 1, k), kfi(b, i, j, k), kfi(b, i, j, k + 1)};
                     int g = is_wall(b->bc[IM]) ? -1 : ghost(b, IM, j, k, 0);
                     set_cloud(b, f, b->bc[IM], g, cidx(b, i, j, k), fl, 4);
-                    f->TWall = b->twall[IM];
+                    f->TWall = side_twall(b, IM, j + njc * k); f->bfun = b->bcfun[IM];
                 } else if (i == nic) {
                     int fl[4] = {jfi(b, i - 1, j, k), jfi(b, i - 1, j + 1, k), kfi(b, i - 1, j, k), kfi(b, i - 1, j, k + 1)};
                     int g = is_wall(b->bc[IP]) ? -1 : ghost(b, IP, j, k, 0);
                     set_cloud(b, f, b->bc[IP], cidx(b, i - 1, j, k), g, fl, 4);
-                    f->TWall = b->twall[IP];
+                    f->TWall = side_twall(b, IP, j + njc * k); f->bfun = b->bcfun[IP];
                 } else {
                     int fl[8] = {jfi(b, i - 1, j, k), jfi(b, i - 1, j + 1, k), kfi(b, i - 1, j, k), kfi(b, i - 1, j, k + 1),
                                  jfi(b, i, j, k), jfi(b, i, j + 1, k), kfi(b, i, j, k), kfi(b, i, j, k + 1)};
@@ -541,12 +638,12 @@ static void configure_clouds(Blk* b)
                     int fl[4] = {ifi(b, i, j, k), ifi(b, i + 1, j, k), kfi(b, i, j, k), kfi(b, i, j, k + 1)};
                     int g = is_wall(b->bc[JM]) ? -1 : ghost(b, JM, i, k, 0);
                     set_cloud(b, f, b->bc[JM], g, cidx(b, i, j, k), fl, 4);
-                    f->TWall = b->twall[JM];
+                    f->TWall = side_twall(b, JM, i + nic * k); f->bfun = b->bcfun[JM];
                 } else if (j == njc) {
                     int fl[4] = {ifi(b, i, j - 1, k), ifi(b, i + 1, j - 1, k), kfi(b, i, j - 1, k), kfi(b, i, j - 1, k + 1)};
                     int g = is_wall(b->bc[JP]) ? -1 : ghost(b, JP, i, k, 0);
                     set_cloud(b, f, b->bc[JP], cidx(b, i, j - 1, k), g, fl, 4);
-                    f->TWall = b->twall[JP];
+                    f->TWall = side_twall(b, JP, i + nic * k); f->bfun = b->bcfun[JP];
                 } else {
                     int fl[8] = {ifi(b, i, j - 1, k), ifi(b, i + 1, j - 1, k), kfi(b, i, j - 1, k), kfi(b, i, j - 1, k + 1),
                                  ifi(b, i, j, k), ifi(b, i + 1, j, k), kfi(b, i, j, k), kfi(b, i, j, k + 1)};
@@ -561,12 +658,12 @@ static void configure_clouds(Blk* b)
                     int fl[4] = {ifi(b, i, j, k), ifi(b, i + 1, j, k), jfi(b, i, j, k), jfi(b, i, j + 1, k)};
                     int g = is_wall(b->bc[KM]) ? -1 : ghost(b, KM, i, j, 0);
                     set_cloud(b, f, b->bc[KM], g, cidx(b, i, j, k), fl, 4);
-                    f->TWall = b->twall[KM];
+                    f->TWall = side_twall(b, KM, i + nic * j); f->bfun = b->bcfun[KM];
                 } else if (k == nkc) {
                     int fl[4] = {ifi(b, i, j, k - 1), ifi(b, i + 1, j, k - 1), jfi(b, i, j, k - 1), jfi(b, i, j + 1, k - 1)};
                     int g = is_wall(b->bc[KP]) ? -1 : ghost(b, KP, i, j, 0);
                     set_cloud(b, f, b->bc[KP], cidx(b, i, j, k - 1), g, fl, 4);
-                    f->TWall = b->twall[KP];
+                    f->TWall = side_twall(b, KP, i + nic * j); f->bfun = b->bcfun[KP];
                 } else {
                     int fl[8] = {ifi(b, i, j, k - 1), ifi(b, i + 1, j, k - 1), jfi(b, i, j, k - 1), jfi(b, i, j + 1, k - 1),
                                  ifi(b, i, j, k), ifi(b, i + 1, j, k), jfi(b, i, j, k), jfi(b, i, j + 1, k)};
@@ -771,6 +868,10 @@ static void apply_bc(orc_sim* s, int ib, int bf)
                 b->cells[g0].fs = s->inflow[b->bcfs[bf]];
                 b->cells[g1].fs = s->inflow[b->bcfs[bf]];
                 break;
+            case ORC_INFLOW_FUNCTION: /* bcs.cu:355-432: the function at each ghost's position */
+                b->cells[g0].fs = inflow_function(s, b->bcfun[bf], b->cells[g0].pos);
+                b->cells[g1].fs = inflow_function(s, b->bcfun[bf], b->cells[g1].pos);
+                break;
             case ORC_OUTFLOW: /* bcs.cu:435-503 */
                 b->cells[g0].fs = *in;
                 b->cells[g1].fs = *in;
@@ -913,8 +1014,12 @@ static void setup_lsq(Blk* b, Face* f)
 }
 
 /* bcs.cu:777-797 (the inflow-function case is not built). */
-static void viscous_bc(Face* f)
+static void viscous_bc(const orc_sim* s, Face* f)
 {
+    if (f->bcode == ORC_INFLOW_FUNCTION) { /* bcs.cu:790-792 */
+        f->fs = inflow_function(s, f->bfun, f->pos);
+        return;
+    }
     if (f->bcode == ORC_WALL_NOSLIP_ADIABATIC) {
         f->fs.vel.x = 0.0; f->fs.vel.y = 0.0; f->fs.vel.z = 0.0;
     } else if (f->bcode == ORC_WALL_NOSLIP_FIXED_T) {
@@ -1010,8 +1115,12 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
     s->cfl_v = (double*)malloc(sizeof(double) * (size_t)p->n_cfl);
     memcpy(s->cfl_t, p->cfl_times, sizeof(double) * (size_t)p->n_cfl);
     memcpy(s->cfl_v, p->cfl_values, sizeof(double) * (size_t)p->n_cfl);
+    if (p->bl_profile && bl_set(&s->bl, p->bl_profile)) {
+        set_err("Gauss-Jordan elimination failed while solving for sigma."); orc_destroy(s); return NULL;
+    }
     s->b = (Blk*)calloc((size_t)nblocks, sizeof(Blk));
     size_t voff = 0, ooff = 0;
+    int twall_used = 0;
     for (int ib = 0; ib < nblocks; ++ib) {
         Blk* b = &s->b[ib];
         b->bi = blk_ijk[3 * ib]; b->bj = blk_ijk[3 * ib + 1]; b->bk = blk_ijk[3 * ib + 2];
@@ -1020,6 +1129,19 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
         for (int f = 0; f < 6; ++f) {
             b->bc[f] = bc_codes[6 * ib + f]; b->bcfs[f] = bc_fs[6 * ib + f];
             b->twall[f] = p->bc_twall ? p->bc_twall[6 * ib + f] : 0.0;
+            b->bcfun[f] = p->bc_fun ? p->bc_fun[6 * ib + f] : 0;
+            if (b->bc[f] == ORC_WALL_NOSLIP_FIXED_T && isnan(b->twall[f])) {
+                int nt = f < 2 ? b->njc * b->nkc : (f < 4 ? b->nic * b->nkc : b->nic * b->njc);
+                if (!p->twall_faces || twall_used + nt > p->n_twall_faces) {
+                    set_err("TWall per face: twall_faces holds too few values"); orc_destroy(s); return NULL;
+                }
+                b->twf[f] = (double*)malloc(sizeof(double) * (size_t)nt);
+                memcpy(b->twf[f], p->twall_faces + twall_used, sizeof(double) * (size_t)nt);
+                twall_used += nt;
+            }
+            if (b->active && b->bc[f] == ORC_INFLOW_FUNCTION && b->bcfun[f] == 2 && !s->bl.set) {
+                set_err("Boundary-layer file is not present in current directory."); orc_destroy(s); return NULL;
+            }
         }
         if (configure(b)) { orc_destroy(s); return NULL; }
         size_t nv = (size_t)(b->nic + 1) * (b->njc + 1) * (b->nkc + 1);
@@ -1077,6 +1199,7 @@ void orc_destroy(orc_sim* s)
         for (int ib = 0; ib < s->nblocks; ++ib) {
             free(s->b[ib].cells); free(s->b[ib].faces); free(s->b[ib].vtx);
             free(s->b[ib].Q); free(s->b[ib].dQ);
+            for (int f = 0; f < 6; ++f) free(s->b[ib].twf[f]);
         }
     free(s->b); free(s->blk_ids); free(s->inflow); free(s->cfl_t); free(s->cfl_v);
     free(s);
@@ -1108,7 +1231,7 @@ int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
                 if (!b->active) continue;
                 block_fluxes(s, b);
                 if (s->viscous) { /* simulate.cu:235-238 */
-                    for (int fi = 0; fi < b->nfaces; ++fi) viscous_bc(&b->faces[fi]);
+                    for (int fi = 0; fi < b->nfaces; ++fi) viscous_bc(s, &b->faces[fi]);
                     for (int fi = 0; fi < b->nfaces; ++fi) add_viscous(b, &b->faces[fi]);
                 }
                 bad += block_update(b, stage, s->dt);

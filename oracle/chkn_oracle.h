@@ -33,7 +33,11 @@ typedef struct {
     const double* cfl_values;
     double t0;            /* starting time (times.data) */
     int viscous;          /* add viscous fluxes, block.cu:88-162 */
-    const double* bc_twall; /* 6 per block, TWall of fixed-T walls (may be NULL) */
+    const double* bc_twall; /* 6 per block, TWall of fixed-T walls (may be NULL); NaN: per face */
+    const int* bc_fun;    /* 6 per block, function of inflow_function sides (may be NULL) */
+    const double* bl_profile; /* p_e, then (y, T, velx) x 12 (config.cu:37-60), or NULL */
+    int n_twall_faces;    /* TWall per face of the NaN sides, block and side order, each side */
+    const double* twall_faces; /* in the TWall-file order (block.cu:566-611) */
 } orc_params;
 
 typedef struct orc_sim orc_sim;

@@ -26,7 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "gdtk-rmhd_amd"))
 REF = os.path.join(ROOT, "oracle", "_ref", "chkn_ref")
 
 from chkn import cases, prep  # noqa: E402
-from chkn.job import JobData, read_flow_file  # noqa: E402
+from chkn.job import BL_PROFILE_FILE, JobData, read_flow_file, write_bl_profile  # noqa: E402
 
 GAMMA, R = 1.4, 287.1
 CV = R / (GAMMA - 1.0)
@@ -124,6 +124,8 @@ STEP_JOBS = {
     "tiny_n2": (lambda: _two_cell_block(), 20),
     "viscous": (lambda: cases.viscous_plate(max_step=30), 30),
     "viscous_x1": (lambda: cases.viscous_plate(max_step=20, x_order=1, seed=8), 20),
+    "bl_plate": (lambda: cases.bl_plate(max_step=30), 30),
+    "bl_plate_x1": (lambda: cases.bl_plate(max_step=20, x_order=1, seed=6), 20),
 }
 
 
@@ -145,13 +147,16 @@ def _two_cell_block():
 
 
 def step_fixture(name, builder, steps):
-    job = builder()
+    built = builder()
+    prof = built[1] if isinstance(built, tuple) else None
     with tempfile.TemporaryDirectory() as d:
         prep.write_initial_files(True, d)
+        if prof is not None:  # the solver reads the profile from its working directory
+            write_bl_profile(os.path.join(d, BL_PROFILE_FILE), prof)
         dtlog = os.path.join(d, "dt.log")
-        subprocess.run([REF, "run", d, "0", str(steps), dtlog], check=True, capture_output=True)
+        subprocess.run([REF, "run", d, "0", str(steps), dtlog], check=True, capture_output=True, cwd=d)
         dts = np.loadtxt(dtlog, ndmin=2)[:, 1]
-        jd = JobData.from_job_dir(d)
+        jd = JobData.from_job_dir(d, bl_profile=prof)
         fields = [read_flow_file(os.path.join(d, "flow", "t0001", "flow-%04d-%04d-%04d.bin" % (b.i, b.j, b.k)),
                                  b.ncells) for b in jd.blocks]
     np.savez_compressed(os.path.join(HERE, "step_%s.npz" % name), dt=dts, steps=steps,

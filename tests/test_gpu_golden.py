@@ -2,7 +2,8 @@
 
 Each tests/golden/step_<name>.npz holds what the reference's CPU driver, compiled from its
 sources, computed for a small job: the dt of every step and the final IO variables.  The job
-directory is written again and loaded through chkn_load_job (the reference's file formats),
+directory is written again and loaded through chkn_load_job (the reference's file formats;
+the boundary-layer profile is read from the working directory, as the reference reads it),
 marched for the same steps, and compared bit for bit: the kernels keep the reference's
 operation order without contraction, so the dt sequence and every IO variable are equal.
 """
@@ -11,10 +12,8 @@ import os
 import numpy as np
 import pytest
 
-from chkn import prep
-from chkn.job import JobData
 from chkn.lib import Sim
-from test_oracle_golden import STEP_JOBS, job_digest
+from test_oracle_golden import STEP_JOBS, build_step_job, job_digest
 
 pytestmark = pytest.mark.gpu
 
@@ -22,11 +21,9 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 @pytest.mark.parametrize("name", sorted(STEP_JOBS))
-def test_hip_march_matches_reference_build(name, tmp_path):
-    builder, steps = STEP_JOBS[name]
-    builder()
-    prep.write_initial_files(True, str(tmp_path))
-    jd = JobData.from_job_dir(str(tmp_path))
+def test_hip_march_matches_reference_build(name, tmp_path, monkeypatch):
+    jd, steps, _ = build_step_job(name, str(tmp_path))
+    monkeypatch.chdir(tmp_path)
     gold = np.load(os.path.join(GOLD, "step_%s.npz" % name))
     assert str(gold["digest"]) == job_digest(jd)
     sim = Sim(job_dir=str(tmp_path))

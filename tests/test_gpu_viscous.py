@@ -85,10 +85,11 @@ def test_first_order_with_file_and_inflow_gas_states():
     assert worst <= FIELD_TOL
 
 
-def test_twall_function_is_rejected():
+def test_twall_per_face_needs_its_values():
+    # NaN marks a TWall given per face; without the values the job is rejected.
     job = cases.viscous_plate(max_step=5)
     job.blocks[1].bc_twall = [0.0, 0.0, 0.0, 0.0, math.nan, 0.0]
-    with pytest.raises(ChknError, match="TWall given as a function"):
+    with pytest.raises(ChknError, match="twall_faces holds too few values"):
         Sim(job)
 
 
@@ -97,3 +98,38 @@ def test_viscous_gradient_is_rejected():
     with pytest.raises(ChknError, match="adjoint of the viscous fluxes"):
         sim.grad_forward(2, 2)
     sim.close()
+
+
+@pytest.mark.parametrize("x_order,steps", [(2, 30), (1, 20)])
+def test_boundary_layer_inflow_function_matches_oracle(x_order, steps):
+    # Inflow through the laminar boundary-layer profile (spline in y), a no-slip wall with
+    # TWall per face, inflow above, periodic span.
+    job, _ = cases.bl_plate(max_step=steps, x_order=x_order, seed=2 + x_order)
+    compare(job, steps)
+
+
+def test_boundary_layer_launch_options():
+    job, _ = cases.bl_plate(max_step=20)
+    ref = _run(job, 20)
+    got = _run(job, 20, launch=dict(use_graph=True), splits=[3, 17])
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+
+
+def test_inviscid_inflow_function_matches_oracle():
+    # The inflow-function ghosts in the fused stage kernel (no viscous fluxes).
+    job, _ = cases.bl_plate(max_step=25)
+    job.viscous = False
+    compare(job, 25)
+
+
+def test_missing_boundary_layer_file_and_twall_file(tmp_path, monkeypatch):
+    cases.bl_plate(max_step=5)
+    prep.write_initial_files(True, str(tmp_path))
+    monkeypatch.chdir(tmp_path)
+    with pytest.raises(ChknError, match="Boundary-layer file is not present"):
+        Sim(job_dir=str(tmp_path))
+    (tmp_path / "InflowBoundaryLayerBC.data").write_text("1013\n12\n" + "0 300 0\n" * 12)
+    (tmp_path / "TWall-0000-0000-0000-kminus.gz").unlink()
+    with pytest.raises(ChknError, match="TWall file"):
+        Sim(job_dir=str(tmp_path))

@@ -12,7 +12,7 @@ import pytest
 
 import oracle_bind as ob
 from chkn import cases, prep
-from chkn.job import JobData
+from chkn.job import BL_PROFILE_FILE, JobData, write_bl_profile
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -63,7 +63,21 @@ STEP_JOBS = {
     "tiny_n2": (_two_cell, 20),
     "viscous": (lambda: cases.viscous_plate(max_step=30), 30),
     "viscous_x1": (lambda: cases.viscous_plate(max_step=20, x_order=1, seed=8), 20),
+    "bl_plate": (lambda: cases.bl_plate(max_step=30), 30),
+    "bl_plate_x1": (lambda: cases.bl_plate(max_step=20, x_order=1, seed=6), 20),
 }
+
+
+def build_step_job(name, job_dir):
+    """Build the named job, write its files (and boundary-layer profile) into job_dir and
+    read it back as the golden run did.  Returns (JobData, steps, profile or None)."""
+    builder, steps = STEP_JOBS[name]
+    built = builder()
+    prof = built[1] if isinstance(built, tuple) else None
+    prep.write_initial_files(True, job_dir)
+    if prof is not None:
+        write_bl_profile(os.path.join(job_dir, BL_PROFILE_FILE), prof)
+    return JobData.from_job_dir(job_dir, bl_profile=prof), steps, prof
 
 
 def job_digest(job):
@@ -84,11 +98,8 @@ def load_step_fixture(name):
 
 @pytest.mark.parametrize("name", sorted(STEP_JOBS))
 def test_march_matches_reference_bitwise(name, tmp_path):
-    builder, steps = STEP_JOBS[name]
-    job = builder()
     # The golden run read the job back from binary files; do the same.
-    prep.write_initial_files(True, str(tmp_path))
-    jd = JobData.from_job_dir(str(tmp_path))
+    jd, steps, _ = build_step_job(name, str(tmp_path))
     gold = load_step_fixture(name)
     assert str(gold["digest"]) == job_digest(jd), "job generator changed; regenerate golden"
     o = ob.Oracle(jd)
