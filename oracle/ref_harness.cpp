@@ -13,7 +13,9 @@
 // and :768-792 (finalise), calling the reference's Block/BC functions for all arithmetic.
 //
 // Modes:
-//   chkn_ref run  <job> <tindx> <nsteps|-1> <dtlog|->   march like chkn-run --binary
+//   chkn_ref run  <job> <tindx> <nsteps|-1> <dtlog|-> [text]
+//                               march like chkn-run --binary (with "text": like chkn-run
+//                               without --binary, gzipped text files)
 //   chkn_ref flux <in> <out>    flux probe: records of 4x9 states + 9 frame doubles
 //                               + (x_order, flux_calc) as doubles -> 6 fluxes each
 //   chkn_ref geom <in> <out>    geometry probe: 8 hex vertices -> pos, vol, lengths;
@@ -112,10 +114,10 @@ static int stage(int which)
     return total;
 }
 
-static int run(const char* job, int tindx, int nsteps, const char* dtlog)
+static int run(const char* job, int tindx, int nsteps, const char* dtlog, bool binary)
 {
     Config::job = job;
-    init_sim(tindx, true);
+    init_sim(tindx, binary);
     H::dt = Config::dt_init;
     H::step = 0;
     int nb = Config::nFluidBlocks;
@@ -156,14 +158,14 @@ static int run(const char* job, int tindx, int nsteps, const char* dtlog)
         H::steps_since_last_plot += 1;
         done += 1;
         if (H::t >= H::t_plot) {
-            write_flow(H::next_plot_indx, H::t, true);
+            write_flow(H::next_plot_indx, H::t, binary);
             H::steps_since_last_plot = 0;
             H::next_plot_indx += 1;
             H::t_plot = H::t + Config::dt_plot_schedule.get_value(H::t);
         }
     }
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-    if (H::steps_since_last_plot > 0) write_flow(H::next_plot_indx, H::t, true);
+    if (H::steps_since_last_plot > 0) write_flow(H::next_plot_indx, H::t, binary);
     if (fdt) std::fclose(fdt);
     long long cells = 0;
     for (auto& c : blk_configs) if (c.active) cells += (long long)c.nic * c.njc * c.nkc;
@@ -241,7 +243,9 @@ int main(int argc, char** argv)
     if (argc < 2) { std::cerr << "usage: chkn_ref run|flux|geom ..." << std::endl; return 2; }
     std::string mode = argv[1];
     try {
-        if (mode == "run" && argc >= 6) return run(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), argv[5]);
+        if (mode == "run" && argc >= 6)
+            return run(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), argv[5],
+                       !(argc >= 7 && std::string(argv[6]) == "text"));
         if (mode == "flux" && argc >= 4) return flux_probe(argv[2], argv[3]);
         if (mode == "geom" && argc >= 4) return geom_probe(argv[2], argv[3]);
     } catch (const std::exception& e) {

@@ -9,6 +9,9 @@ that script) in this container only.  Outputs, all data:
                      quad_properties (vector3.cu:196-245, :350-433)
   step_<name>.npz    small jobs marched by the reference driver: dt per step and the final
                      IO variables of every block, plus a digest of the job's inputs
+  march_text.npz     a multi-block job in gzipped-text files marched to the end with
+                     snapshots (times.data text, SHA-256 of every flow file written), then
+                     restarted from time index 2 (the same after the restart)
 
 Usage: python tests/golden/make_golden.py [step fixture name ...]   (default: all)
 """
@@ -163,12 +166,55 @@ def step_fixture(name, builder, steps):
                         digest=job_digest(jd), **{"io%d" % b: f for b, f in enumerate(fields)})
 
 
+def text_march_job():
+    """mixed_bc_array for 40 steps with a snapshot every 1.5e-5 s (five snapshots)."""
+    cases.mixed_bc_array(max_step=40)
+    prep.config.dt_plot_list = [(0.0, 1.5e-5, 1.5e-5)]
+    return JobData.from_prep()
+
+
+def written_flow_digests(job_dir, first_tindx):
+    """{relative path: sha256 of the decompressed text} of the flow files at time indices
+    >= first_tindx."""
+    import glob
+    import gzip
+    out = {}
+    for path in sorted(glob.glob(os.path.join(job_dir, "flow", "t*", "*.gz"))):
+        tindx = int(os.path.basename(os.path.dirname(path))[1:])
+        if tindx >= first_tindx:
+            with gzip.open(path, "rb") as f:
+                out[os.path.relpath(path, job_dir)] = hashlib.sha256(f.read()).hexdigest()
+    return out
+
+
+def text_march_fixture():
+    text_march_job()
+    with tempfile.TemporaryDirectory() as d:
+        prep.write_initial_files(False, d)
+        ref = os.path.abspath(REF)
+        subprocess.run([ref, "run", d, "0", "-1", "-", "text"], check=True, capture_output=True, cwd=d)
+        with open(os.path.join(d, "times.data")) as f:
+            times1 = f.read()
+        dig1 = written_flow_digests(d, 1)
+        subprocess.run([ref, "run", d, "2", "-1", "-", "text"], check=True, capture_output=True, cwd=d)
+        with open(os.path.join(d, "times.data")) as f:
+            times2 = f.read()
+        dig2 = written_flow_digests(d, 3)
+    np.savez_compressed(os.path.join(HERE, "march_text.npz"), times=times1, times_restart=times2,
+                        names=np.array(sorted(dig1)), digests=np.array([dig1[k] for k in sorted(dig1)]),
+                        names_restart=np.array(sorted(dig2)),
+                        digests_restart=np.array([dig2[k] for k in sorted(dig2)]))
+
+
 def main(names):
     if not os.path.exists(REF):
         sys.exit("build oracle/_ref first (oracle/build_ref.sh)")
-    if names:  # only the named step fixtures
+    if names:  # only the named fixtures
         for name in names:
-            step_fixture(name, *STEP_JOBS[name])
+            if name == "march_text":
+                text_march_fixture()
+            else:
+                step_fixture(name, *STEP_JOBS[name])
         return
     rng = np.random.default_rng(20240601)
     rec = flux_vectors(rng)
@@ -179,6 +225,7 @@ def main(names):
     np.savez_compressed(os.path.join(HERE, "geom_vectors.npz"), p=g, out=G)
     for name, (builder, steps) in STEP_JOBS.items():
         step_fixture(name, builder, steps)
+    text_march_fixture()
     print("golden vectors written to", HERE)
 
 
