@@ -329,3 +329,36 @@ def bl_plate(nics=(10, 8), njc=3, nkc=10, max_step=30, seed=4, x_order=2, twall_
         prep.FluidBlock(i=i, j=0, k=0, grid=sub, initialState=init, bcs=bcs)
     prof = bl_profile_numbers()
     return JobData.from_prep(bl_profile=prof), prof
+
+
+def sbp_channel(nics=(10, 8), njc=6, nkc=5, max_step=25, seed=3, flux_calc="sbp_asf"):
+    """Smooth subsonic flow through a periodic channel on two warped blocks, for the
+    central SBP-ASF flux (face.cu:268-342): inflow at iminus, outflow at iplus, periodic
+    exchange in j and k, a smooth density, pressure and velocity wave."""
+    _common(max_step, title="SBP-ASF channel (seed %d)" % seed, cfl=0.4, dt_init=2.0e-7)
+    prep.config.flux_calc = flux_calc
+    NI = sum(nics)
+    g = box_grid(NI, njc, nkc, L=(0.2, 0.05, 0.04))
+    rng = np.random.default_rng(seed)
+    v = g.vertices
+    for a, Lx in ((v.x, 0.2), (v.y, 0.05), (v.z, 0.04)):
+        a[1:-1, 1:-1, 1:-1] += Lx * 0.05 / NI * rng.uniform(-1, 1, size=a[1:-1, 1:-1, 1:-1].shape)
+    fs = freestream(mach=0.4, p=5.0e4, T=300.0)
+    cc = cell_centres(g)
+    X, Y, Z = cc.x / 0.2, cc.y / 0.05, cc.z / 0.04
+    wave = np.sin(2 * np.pi * (X + 0.3 * Y)) * np.cos(2 * np.pi * Z)
+    p = fs.gas.p * (1.0 + 0.02 * wave)
+    T = fs.gas.T * (1.0 + 0.01 * wave)
+    st = _state_arrays(p, T, fs.vel.x * (1.0 + 0.05 * wave))
+    st["vely"] = 3.0 * np.sin(2 * np.pi * (X + Z))
+    st["velz"] = 2.0 * np.cos(2 * np.pi * (X - Y))
+    io = np.cumsum([0] + list(nics))
+    for i in range(2):
+        bcs = {"iminus": prep.InflowBC(fs) if i == 0 else prep.ExchangeBC(),
+               "iplus": prep.OutflowBC() if i == 1 else prep.ExchangeBC(),
+               "jminus": prep.ExchangeBC(), "jplus": prep.ExchangeBC(),
+               "kminus": prep.ExchangeBC(), "kplus": prep.ExchangeBC()}
+        sub = g.subgrid(i0=io[i], j0=0, k0=0, niv=nics[i] + 1, njv=njc + 1, nkv=nkc + 1)
+        init = {key: val[io[i]:io[i + 1]] if np.ndim(val) == 3 else val for key, val in st.items()}
+        prep.FluidBlock(i=i, j=0, k=0, grid=sub, initialState=init, bcs=bcs)
+    return JobData.from_prep()

@@ -148,8 +148,8 @@ struct SplitArgs {
     int win, wout;
     int file_gas;
 };
-hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order, bool viscous,
-                              hipStream_t s);
+hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order, int flux_calc,
+                              bool viscous, hipStream_t s);
 hipError_t launch_split_viscous(const SplitArgs& a, int nface_chunks, hipStream_t s);
 hipError_t launch_split_update(const SplitArgs& a, int ncell_chunks, int stage, bool dt_reduce,
                                hipStream_t s);

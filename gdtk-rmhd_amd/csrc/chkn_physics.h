@@ -165,6 +165,51 @@ CHKN_HD void face_flux(const Prim& L1, const Prim& L0, const Prim& R0, const Pri
     ausmdv(rhoL, eL, pL, aL, YBL, vxL, vyL, vzL, rhoR, eR, pR, aR, YBR, vxR, vyR, vzR, f, F);
 }
 
+// Summation-by-parts alpha-split flux (face.cu:268-342, after Fisher et al., NASA/TM-2011-
+// 217307): fourth-order central divergence-form and product-rule fluxes of the factored
+// conservation terms over the four-cell stencil, blended with the alpha weights 1 (mass),
+// 1/2 (momentum, internal energy), 0 (kinetic energy, pressure).  Uses each state's own p
+// and e.  The alpha products are kept as written: 0*x is not 0 for all x.
+CHKN_HD void sbp_asf(const Prim* s, const double* p, const Frame& f, double F[6])
+{
+    constexpr double one_twelfth = 1.0 / 12.0;
+    double v[10][4], w[10][4];
+    for (int i = 0; i < 4; ++i) {
+        double velx, vely, velz;
+        to_local(f, s[i].vx, s[i].vy, s[i].vz, velx, vely, velz);
+        const double rho = s[i].rho;
+        v[0][i] = rho;                 w[0][i] = velx;
+        v[1][i] = velx * rho;          w[1][i] = velx;
+        v[2][i] = vely * rho;          w[2][i] = velx;
+        v[3][i] = velz * rho;          w[3][i] = velx;
+        v[4][i] = s[i].e * rho;        w[4][i] = velx;
+        v[5][i] = velx * velx * rho;   w[5][i] = velx;
+        v[6][i] = vely * vely * rho;   w[6][i] = velx;
+        v[7][i] = velz * velz * rho;   w[7][i] = velx;
+        v[8][i] = p[i];                w[8][i] = velx;
+        v[9][i] = p[i];                w[9][i] = 1.0;
+    }
+    double fc[10], fe[10];
+    for (int j = 0; j < 10; ++j) {
+        fc[j] = one_twelfth * (-v[j][0] * w[j][0] + 7.0 * v[j][1] * w[j][1] + 7.0 * v[j][2] * w[j][2] -
+                               v[j][3] * w[j][3]);
+        fe[j] = one_twelfth * (-v[j][0] * w[j][2] - v[j][2] * w[j][0] + 8.0 * v[j][1] * w[j][2] +
+                               8.0 * v[j][2] * w[j][1] - v[j][1] * w[j][3] - v[j][3] * w[j][1]);
+    }
+    const double a_mass = 1.0, a_mom = 0.5, a_ie = 0.5, a_ke = 0.0, a_p = 0.0;
+    const double mass = a_mass * fc[0] + (1.0 - a_mass) * fe[0];
+    F[0] = mass;
+    const double mx = a_mom * fc[1] + (1.0 - a_mom) * fe[1] + (a_p * fc[9] + (1.0 - a_p) * fe[9]);
+    const double my = a_mom * fc[2] + (1.0 - a_mom) * fe[2];
+    const double mz = a_mom * fc[3] + (1.0 - a_mom) * fe[3];
+    to_global(f, mx, my, mz, F[1], F[2], F[3]);
+    F[4] = a_ie * fc[4] + (1.0 - a_ie) * fe[4] +
+           0.5 * (a_ke * fc[5] + (1.0 - a_ke) * fe[5] + a_ke * fc[6] + (1.0 - a_ke) * fe[6] +
+                  a_ke * fc[7] + (1.0 - a_ke) * fe[7]) +
+           a_p * fc[8] + (1.0 - a_p) * fe[8];
+    F[5] = mass * ((mass >= 0.0) ? s[1].YB : s[2].YB);
+}
+
 // flow.cu:227-243
 CHKN_HD void encode(const Prim& s, double U[6])
 {

@@ -267,7 +267,7 @@ void check_desc(const chkn_desc* d)
 {
     if (d->nblocks <= 0) throw std::invalid_argument("no blocks");
     if (d->reacting) throw std::invalid_argument("the reacting-gas model is not built");
-    if (d->flux_calc != 0) throw std::invalid_argument("flux_calc sbp_asf is not built");
+    if (d->flux_calc != 0 && d->flux_calc != 1) throw std::invalid_argument("flux_calc must be 0 (ausmdv) or 1 (sbp_asf)");
     if (d->x_order != 1 && d->x_order != 2) throw std::invalid_argument("x_order must be 1 or 2");
     if (d->n_cfl <= 0) throw std::invalid_argument("need at least one cfl value");
     if (d->cfl_count <= 0) throw std::invalid_argument("cfl_count must be positive");
@@ -284,6 +284,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     s->viscous = descs[0].viscous != 0;
     for (int q = 1; q < ninst; ++q) {
         if (descs[q].x_order != s->x_order) throw std::invalid_argument("instances must share x_order");
+        if (descs[q].flux_calc != s->flux_calc) throw std::invalid_argument("instances must share flux_calc");
         if ((descs[q].viscous != 0) != s->viscous) throw std::invalid_argument("instances must share viscous");
         if (descs[q].cfl_count != descs[0].cfl_count) throw std::invalid_argument("instances must share cfl_count");
     }

@@ -38,7 +38,7 @@ bool eos_agrees(double rho, double e, double YB, double p, double T, double a)
     return T2 == T && eos_p(rho, T2) == p && eos_a(T2) == a;
 }
 
-bool need_gas(const chkn_sim* s) { return s->x_order == 1 || s->viscous; }
+bool need_gas(const chkn_sim* s) { return s->x_order == 1 || s->viscous || s->flux_calc == 1; }
 
 size_t face_count(const HostBlock& b)
 {
@@ -46,12 +46,12 @@ size_t face_count(const HostBlock& b)
            (size_t)b.nic * b.njc * (b.nkc + 1);
 }
 
-// The face-list form runs every stage: viscous jobs, and first-order jobs in which an inflow
+// The face-list form runs every stage: viscous jobs, SBP-ASF jobs, and first-order jobs in which an inflow
 // state used by an active block, or an inactive block that an active block exchanges with,
 // holds gas states the equation of state does not reproduce.
 bool split_always(const chkn_sim* s)
 {
-    if (s->viscous) return true;
+    if (s->viscous || s->flux_calc == 1) return true;
     if (s->x_order != 1) return false;
     const int nb = (int)s->blocks.size();
     for (int ib = 0; ib < nb; ++ib) {
@@ -292,7 +292,7 @@ void launch_stage_form(chkn_sim* s, const StageArgs& a, int stage, bool dtred, b
         sa.win = a.win;
         sa.wout = a.wout;
         sa.file_gas = file_gas ? 1 : 0;
-        HIPCHK(launch_split_faces(sa, s->nsplit_faces, s->x_order, s->viscous, st));
+        HIPCHK(launch_split_faces(sa, s->nsplit_faces, s->x_order, s->flux_calc, s->viscous, st));
         if (s->viscous) HIPCHK(launch_split_viscous(sa, s->nsplit_faces, st));
         HIPCHK(launch_split_update(sa, s->nsplit_cells, stage, dtred, st));
         return;

@@ -8,7 +8,8 @@
 //  * first-order jobs whose cells or inflow ghost cells hold gas states (p, T, a) that the
 //    equation of state does not reproduce from (rho, e): the reference's first-order flux
 //    reads p and a of the two cells beside the face verbatim (face.cu:356-371), as read from
-//    the flow file (cell.cu:112-131) or configured for the inflow (config.cu:398-406).
+//    the flow file (cell.cu:112-131) or configured for the inflow (config.cu:398-406);
+//  * the SBP-ASF flux (face.cu:268-342), which reads p of all four stencil cells verbatim.
 // Three launches per stage, every active block of every instance in each:
 //   faces:   one thread per face: convective flux and, for viscous jobs, the face state
 //            set_as_average(L, R) (flow.cu:220-225) with the viscous boundary conditions
@@ -161,7 +162,10 @@ __device__ __forceinline__ int line_length(const BlockDev* B, int dir)
 
 __device__ __forceinline__ bool is_wall(int code) { return code >= 0 && code <= 2; }
 
-template <int XORDER, bool VISC>
+// FLUX 0: AUSMDV on the MUSCL (XORDER 2) or first-order states, face state the average of
+// the two reconstructed states; FLUX 1: SBP-ASF, face state the average of the two cells
+// beside the face (face.cu:373-376).
+template <int XORDER, bool VISC, int FLUX>
 __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
 {
     const FaceChunk ch = A.faces[blockIdx.x];
@@ -174,7 +178,20 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
     const int nf = (int)(dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk));
     const Frame fr = ld_frame(g, nf, f);
     double rhoL, rhoR, eL, eR, YBL, YBR, vxL, vxR, vyL, vyR, vzL, vzR, pL, pR, aL, aR;
-    if (XORDER > 1) {
+    double F[6];
+    if (FLUX == 1) {
+        Prim s[4];
+        double ps[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const Point pt = point<true>(A, B, dir, p.m - 2 + q, p.a, p.c);
+            s[q] = pt.w;
+            ps[q] = pt.g.p;
+        }
+        sbp_asf(s, ps, fr, F);
+        rhoL = s[1].rho; eL = s[1].e; YBL = s[1].YB; vxL = s[1].vx; vyL = s[1].vy; vzL = s[1].vz;
+        rhoR = s[2].rho; eR = s[2].e; YBR = s[2].YB; vxR = s[2].vx; vyR = s[2].vy; vzR = s[2].vz;
+    } else if (XORDER > 1) {
         // face.cu:359-368: rho, e, YB, then the equation of state, then the velocity.
         const Prim s0 = point<false>(A, B, dir, p.m - 2, p.a, p.c).w;
         const Prim s1 = point<false>(A, B, dir, p.m - 1, p.a, p.c).w;
@@ -199,8 +216,8 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
         pL = L.g.p; aL = L.g.a;
         pR = R.g.p; aR = R.g.a;
     }
-    double F[6];
-    ausmdv(rhoL, eL, pL, aL, YBL, vxL, vyL, vzL, rhoR, eR, pR, aR, YBR, vxR, vyR, vzR, fr, F);
+    if (FLUX == 0)
+        ausmdv(rhoL, eL, pL, aL, YBL, vxL, vyL, vzL, rhoR, eR, pR, aR, YBR, vxR, vyR, vzR, fr, F);
     const long long nft = B->nfi + B->nfj + B->nfk;
 #pragma unroll
     for (int q = 0; q < 6; ++q) B->F[q * nft + p.ff] = F[q];
@@ -471,17 +488,20 @@ __global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A)
 
 }  // namespace
 
-hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order, bool viscous,
-                              hipStream_t s)
+hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order, int flux_calc,
+                              bool viscous, hipStream_t s)
 {
     if (nface_chunks <= 0) return hipSuccess;
     const dim3 grid(nface_chunks), blk(kChunk);
-    if (x_order > 1) {
-        if (viscous) hipLaunchKernelGGL((split_face_kernel<2, true>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((split_face_kernel<2, false>), grid, blk, 0, s, a);
+    if (flux_calc == 1) {
+        if (viscous) hipLaunchKernelGGL((split_face_kernel<2, true, 1>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((split_face_kernel<2, false, 1>), grid, blk, 0, s, a);
+    } else if (x_order > 1) {
+        if (viscous) hipLaunchKernelGGL((split_face_kernel<2, true, 0>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((split_face_kernel<2, false, 0>), grid, blk, 0, s, a);
     } else {
-        if (viscous) hipLaunchKernelGGL((split_face_kernel<1, true>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((split_face_kernel<1, false>), grid, blk, 0, s, a);
+        if (viscous) hipLaunchKernelGGL((split_face_kernel<1, true, 0>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((split_face_kernel<1, false, 0>), grid, blk, 0, s, a);
     }
     return hipGetLastError();
 }

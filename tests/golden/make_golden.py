@@ -4,7 +4,8 @@ Runs oracle/_ref/chkn_ref (built by oracle/build_ref.sh from the reference tree;
 that script) in this container only.  Outputs, all data:
 
   flux_vectors.npz   seeded L1/L0/R0/R1 flow states + face frames -> reference face flux
-                     (FVFace::calculate_convective_flux, face.cu:347-378), x_order 1 and 2
+                     (FVFace::calculate_convective_flux, face.cu:347-378), AUSMDV at x_order
+                     1 and 2, then SBP-ASF (flux_calc 1) from a second seeded generator
   geom_vectors.npz   seeded warped hexahedra -> reference hex_cell_properties and
                      quad_properties (vector3.cu:196-245, :350-433)
   step_<name>.npz    small jobs marched by the reference driver: dt per step and the final
@@ -86,6 +87,23 @@ def flux_vectors(rng, nper=160):
     return np.concatenate(recs)
 
 
+def sbp_flux_vectors(rng, nper=160):
+    """Stencils of four nearby states for the SBP-ASF flux (flux_calc 1)."""
+    recs = []
+    for regime in ("sub", "trans", "super"):
+        base = random_states(rng, nper, regime)
+        st = np.stack([base * (1.0 + 0.1 * rng.normal(size=base.shape)) for _ in range(4)], axis=1)
+        st[:, :, 3] = np.abs(st[:, :, 3])
+        st[:, :, 0] = np.abs(st[:, :, 0])
+        st[:, :, 1] = st[:, :, 3] * CV
+        st[:, :, 2] = st[:, :, 0] * R * st[:, :, 3]
+        st[:, :, 5] = np.sqrt(GAMMA * R * st[:, :, 3])
+        st[:, :, 4] = np.abs(rng.normal(size=(nper, 4))) * (regime == "super")  # YB upwinding
+        recs.append(np.column_stack([st.reshape(nper, 36), random_frames(rng, nper), np.full(nper, 2),
+                                     np.ones(nper)]))
+    return np.concatenate(recs)
+
+
 def geom_vectors(rng, n=240):
     base = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1],
                      [0, 1, 1]], dtype=float)
@@ -129,7 +147,15 @@ STEP_JOBS = {
     "viscous_x1": (lambda: cases.viscous_plate(max_step=20, x_order=1, seed=8), 20),
     "bl_plate": (lambda: cases.bl_plate(max_step=30), 30),
     "bl_plate_x1": (lambda: cases.bl_plate(max_step=20, x_order=1, seed=6), 20),
+    "sbp": (lambda: cases.sbp_channel(max_step=25), 25),
+    "sbp_viscous": (lambda: _viscous_sbp(), 20),
 }
+
+
+def _viscous_sbp():
+    cases.sbp_channel(max_step=20, seed=5)
+    prep.config.viscous = True
+    return JobData.from_prep()
 
 
 def _cfl_limited_ramp():
@@ -217,7 +243,7 @@ def main(names):
                 step_fixture(name, *STEP_JOBS[name])
         return
     rng = np.random.default_rng(20240601)
-    rec = flux_vectors(rng)
+    rec = np.concatenate([flux_vectors(rng), sbp_flux_vectors(np.random.default_rng(20240607))])
     F = run_ref("flux", rec).reshape(-1, 6)
     np.savez_compressed(os.path.join(HERE, "flux_vectors.npz"), rec=rec, F=F)
     g = geom_vectors(rng)

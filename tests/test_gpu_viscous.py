@@ -1,4 +1,5 @@
-"""Viscous fluxes and the face-list stage form on the GPU (through the C-ABI).
+"""Viscous fluxes, inflow functions, the SBP-ASF flux and the face-list stage form on the GPU
+(through the C-ABI).
 
 The viscous plate (inflow, outflow, periodic exchange, a no-slip adiabatic wall and a
 no-slip fixed-T wall, warped grid, varied initial state) against the oracle at x_order 2 and
@@ -133,3 +134,20 @@ def test_missing_boundary_layer_file_and_twall_file(tmp_path, monkeypatch):
     (tmp_path / "TWall-0000-0000-0000-kminus.gz").unlink()
     with pytest.raises(ChknError, match="TWall file"):
         Sim(job_dir=str(tmp_path))
+
+
+def test_sbp_asf_matches_oracle_and_launch_options():
+    # The central SBP-ASF flux (face.cu:268-342), inflow/outflow and periodic exchanges.
+    job = cases.sbp_channel(max_step=25)
+    compare(job, 25)
+    ref = _run(job, 25)
+    got = _run(job, 25, launch=dict(use_graph=True), splits=[2, 23])
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+
+
+def test_sbp_asf_gradient_is_rejected():
+    sim = Sim(cases.sbp_channel(max_step=5))
+    with pytest.raises(ChknError, match="adjoint of the SBP-ASF flux"):
+        sim.grad_forward(2, 2)
+    sim.close()

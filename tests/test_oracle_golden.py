@@ -65,7 +65,15 @@ STEP_JOBS = {
     "viscous_x1": (lambda: cases.viscous_plate(max_step=20, x_order=1, seed=8), 20),
     "bl_plate": (lambda: cases.bl_plate(max_step=30), 30),
     "bl_plate_x1": (lambda: cases.bl_plate(max_step=20, x_order=1, seed=6), 20),
+    "sbp": (lambda: cases.sbp_channel(max_step=25), 25),
+    "sbp_viscous": (lambda: _viscous_sbp(), 20),
 }
+
+
+def _viscous_sbp():
+    cases.sbp_channel(max_step=20, seed=5)
+    prep.config.viscous = True
+    return JobData.from_prep()
 
 
 def build_step_job(name, job_dir):
