@@ -362,3 +362,39 @@ def sbp_channel(nics=(10, 8), njc=6, nkc=5, max_step=25, seed=3, flux_calc="sbp_
         init = {key: val[io[i]:io[i + 1]] if np.ndim(val) == 3 else val for key, val in st.items()}
         prep.FluidBlock(i=i, j=0, k=0, grid=sub, initialState=init, bcs=bcs)
     return JobData.from_prep()
+
+
+def ab_reacting(nics=(16, 12), njc=6, nkc=5, max_step=30, seed=2, reacting=True):
+    """The reacting A->B gas (gas.cu:25-48, examples/chicken/oblique-detonation-wave): a
+    Mach 2.4 stream (p = 86.1 kPa, T = 300 K) into gas at 28.7 kPa holding a hot, partly
+    burnt slug above the ignition temperature, on two blocks: inflow at iminus, outflow at
+    iplus, slip walls in j, periodic in k."""
+    _common(max_step, title="A->B reacting gas (seed %d)" % seed, cfl=0.5, dt_init=5.0e-7)
+    gm = prep.GasModel(gamma=6.0 / 5.0, R=287.0, q=300000.0, alpha=1000.0, Ti=362.58)
+    prep.config.gas_model = gm
+    prep.config.reacting = bool(reacting)
+    NI = sum(nics)
+    g = box_grid(NI, njc, nkc, L=(0.32, 0.08, 0.08))
+    rng = np.random.default_rng(seed)
+    v = g.vertices
+    for a, Lx in ((v.x, 0.32), (v.y, 0.08), (v.z, 0.08)):
+        a[1:-1, 1:-1, 1:-1] += Lx * 0.04 / NI * rng.uniform(-1, 1, size=a[1:-1, 1:-1, 1:-1].shape)
+    inflow = prep.FlowState(p=86.1e3, T=300.0, velx=964.302)
+    cc = cell_centres(g)
+    bump = np.exp(-((cc.x - 0.16) / 0.05) ** 2) * (1.0 + 0.2 * np.sin(2 * np.pi * cc.z / 0.08))
+    p = np.full_like(bump, 28.7e3)
+    T = 300.0 + 320.0 * bump
+    YB = 0.3 * bump
+    st = {"p": p, "T": T, "YB": YB, "e": T * gm.Cv - YB * gm.q, "rho": p / (T * gm.R),
+          "a": np.sqrt(gm.gamma * gm.R * T), "velx": 40.0 * bump, "vely": np.zeros_like(T),
+          "velz": 5.0 * np.sin(2 * np.pi * cc.x / 0.32)}
+    io = np.cumsum([0] + list(nics))
+    for i in range(2):
+        bcs = {"iminus": prep.InflowBC(inflow) if i == 0 else prep.ExchangeBC(),
+               "iplus": prep.OutflowBC() if i == 1 else prep.ExchangeBC(),
+               "jminus": prep.WallWithSlipBC(), "jplus": prep.WallWithSlipBC(),
+               "kminus": prep.ExchangeBC(), "kplus": prep.ExchangeBC()}
+        sub = g.subgrid(i0=io[i], j0=0, k0=0, niv=nics[i] + 1, njv=njc + 1, nkv=nkc + 1)
+        init = {key: np.ascontiguousarray(val[io[i]:io[i + 1]]) for key, val in st.items()}
+        prep.FluidBlock(i=i, j=0, k=0, grid=sub, initialState=init, bcs=bcs)
+    return JobData.from_prep()

@@ -19,6 +19,17 @@ FACE_NAMES = ("iminus", "iplus", "jminus", "jplus", "kminus", "kplus")
 BC_FUNCTIONS = {"none": 0, "supersonic_vortex": 1, "laminar_boundary_layer": 2,
                 "manufactured_solution": 3}
 BL_SAMPLES = 12  # config.cu:31
+# gas.cu:20-48: the two gas models the solver is built for (gamma, R, q, alpha, Ti).
+IDEAL_AIR = (1.4, 287.1, 0.0, 0.0, 0.0)
+AB_REACTING_GAS = (6.0 / 5.0, 287.0, 300000.0, 1000.0, 362.58)
+
+
+def gas_flavour(gas_model):
+    """"ab" when the gas model (gamma, R, q, alpha, Ti, ...) is the reacting A->B gas, else
+    "air" (the library is built for one of them, as the reference's executables are)."""
+    if gas_model is not None and all(abs(a - b) < 1.0e-4 for a, b in zip(AB_REACTING_GAS, gas_model[:5])):
+        return "ab"
+    return "air"
 BL_PROFILE_FILE = "InflowBoundaryLayerBC.data"
 
 
@@ -86,6 +97,7 @@ class JobData:
         self.reacting = False
         self.viscous = False
         self.bl_profile = None  # read_bl_profile() numbers when a side uses the BL function
+        self.gas_model = list(IDEAL_AIR) + [IDEAL_AIR[1] / (IDEAL_AIR[0] - 1.0)]  # + Cv
 
     # ------------------------------------------------------------------ construction
     def _apply_config(self, cfg):
@@ -109,6 +121,8 @@ class JobData:
         self.dt_plot_times = list(cfg["t_change"])
         self.dt_plot_values = list(cfg["dt_plot"])
         self.reacting = bool(cfg["reacting"])
+        gm = cfg["gas_model"]
+        self.gas_model = [float(gm[k]) for k in ("gamma", "R", "q", "alpha", "Ti", "Cv")]
         self.viscous = bool(cfg["viscous"])
         blocks = []
         for b in cfg["fluid_blocks"]:
@@ -204,6 +218,7 @@ class JobData:
             "bc_twall": dbl([getattr(b, "bc_twall", [0.0] * 6) for b in self.blocks]).ravel(),
             "bc_fun": ints([getattr(b, "bc_fun", [0] * 6) for b in self.blocks]).ravel(),
             "bl_profile": dbl(self.bl_profile if self.bl_profile is not None else []),
+            "gas_model": dbl(self.gas_model),
             "twall_faces": dbl(np.concatenate([np.zeros(0)] + [
                 b.twall_faces[q] for b in self.blocks for q in sorted(getattr(b, "twall_faces", {}))])),
         }

@@ -11,6 +11,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # CHKN_LIB selects another build of the library (for A/B measurements of kernel variants).
 LIB_PATH = os.environ.get("CHKN_LIB") or os.path.join(_HERE, "libchkn.so")
+# The reacting A->B gas flavour (the reference's AB_REACTING_GAS build).
+LIB_AB_PATH = os.path.join(_HERE, "libchkn_ab.so")
 
 _I = ctypes.POINTER(ctypes.c_int)
 _D = ctypes.POINTER(ctypes.c_double)
@@ -28,7 +30,7 @@ class ChknDesc(ctypes.Structure):
                 ("n_dt_plot", ctypes.c_int), ("dt_plot_times", _D), ("dt_plot_values", _D),
                 ("t0", ctypes.c_double), ("start_tindx", ctypes.c_int), ("reacting", ctypes.c_int),
                 ("viscous", ctypes.c_int), ("bc_twall", _D), ("bc_fun", _I), ("bl_profile", _D),
-                ("n_twall_faces", ctypes.c_int), ("twall_faces", _D)]
+                ("n_twall_faces", ctypes.c_int), ("twall_faces", _D), ("gas_model", _D)]
 
 
 class ChknReport(ctypes.Structure):
@@ -80,23 +82,26 @@ EXPORTS = {
     "chkn_destroy": (None, [ctypes.c_void_p]),
 }
 
-_lib = None
 
 
-def load():
-    """Load libchkn.so and declare its C-ABI; raises if it has not been built."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError("libchkn.so is not built (expected at %s); run "
-                               "__graft_entry__.build() or make -C gdtk-rmhd_amd" % LIB_PATH)
-        lib = ctypes.CDLL(LIB_PATH)
+_libs = {}
+
+
+def load(flavour="air"):
+    """Load libchkn.so (flavour "air") or libchkn_ab.so ("ab", the reacting A->B gas) and
+    declare its C-ABI; raises if it has not been built."""
+    if flavour not in _libs:
+        path = LIB_AB_PATH if flavour == "ab" else LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built (expected at %s); run __graft_entry__.build() "
+                               "or make -C gdtk-rmhd_amd" % (os.path.basename(path), path))
+        lib = ctypes.CDLL(path)
         for name, (res, args) in EXPORTS.items():
             fn = getattr(lib, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = lib
-    return _lib
+        _libs[flavour] = lib
+    return _libs[flavour]
 
 
 class ChknError(RuntimeError):
@@ -162,6 +167,7 @@ def _desc(job, keep):
     d.bl_profile = _ptr(f["bl_profile"], _D) if len(f["bl_profile"]) else None
     d.n_twall_faces = len(f["twall_faces"])
     d.twall_faces = _ptr(f["twall_faces"], _D) if len(f["twall_faces"]) else None
+    d.gas_model = _ptr(f["gas_model"], _D)
     return d
 
 
@@ -170,7 +176,15 @@ class Sim:
     of independent jobs stepped together (``jobs``)."""
 
     def __init__(self, job=None, job_dir=None, tindx=0, binary=True, jobs=None):
-        lib = load()
+        from .job import gas_flavour
+        if job_dir is not None:
+            import json
+            with open(os.path.join(job_dir, "config.json")) as fp:
+                gm = json.load(fp).get("gas_model", {})
+            flavour = gas_flavour([gm.get(k, 0.0) for k in ("gamma", "R", "q", "alpha", "Ti")])
+        else:
+            flavour = gas_flavour((jobs or [job])[0].gas_model)
+        lib = load(flavour)
         h = ctypes.c_void_p()
         self._keep = []
         if job_dir is not None:

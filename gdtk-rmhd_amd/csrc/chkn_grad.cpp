@@ -280,6 +280,7 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
     if (C <= 0 || C > kMaxSeg) return set_error(s, CHKN_ERR_ARG, "checkpoint interval must be in 1..64");
     if (s->viscous) return set_error(s, CHKN_ERR_ARG, "the adjoint of the viscous fluxes is not built");
     if (s->flux_calc != 0) return set_error(s, CHKN_ERR_ARG, "the adjoint of the SBP-ASF flux is not built");
+    if (s->any_reacting) return set_error(s, CHKN_ERR_ARG, "the adjoint of the A->B reaction is not built");
     // The unroll is a function of the conserved state: gas states follow from it, also in
     // the first stage (the recomputation from checkpoints does the same).
     s->fresh = false;

@@ -81,6 +81,7 @@ struct StepCtl {
     const int* run;                   // [ninst] 1 if the instance takes this step
     unsigned long long* sigmin_bits;  // [ninst] min over cells of the cfl-free signal time
     int* bad;                         // [3] bad cells per stage
+    const double* chem;               // [ninst] exp(-alpha dt) of reacting instances, else -1
 };
 
 struct StageArgs {
@@ -153,6 +154,12 @@ hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order,
 hipError_t launch_split_viscous(const SplitArgs& a, int nface_chunks, hipStream_t s);
 hipError_t launch_split_update(const SplitArgs& a, int ncell_chunks, int stage, bool dt_reduce,
                                hipStream_t s);
+
+// The A->B reaction after a step on W slot `slot` and U of every active block of the
+// reacting instances (ctl.chem >= 0); with dt_reduce also the signal-time minimum of every
+// instance (then taken here rather than in the stage-3 launch).
+hipError_t launch_chemistry(const BlockDev* blocks, const CellChunk* cells, int ncell_chunks,
+                            const StepCtl& ctl, int slot, bool dt_reduce, hipStream_t s);
 
 // Launch one stage over tiles [tile_base, tile_base + ntiles) of the table.
 // stage in 1..3; x_order 1 or 2; tile shape (ti, tj) must be a built variant.

@@ -12,6 +12,7 @@
 #include <stdexcept>
 
 #include "chkn_json.h"
+#include "chkn_physics.h"
 
 namespace chkn {
 
@@ -54,6 +55,17 @@ int bc_code(const std::string& name)
 }
 
 // face.cu:61-68: unknown names fall back to none.
+}  // namespace
+
+bool gas_model_matches(const double* m)
+{
+    auto close = [](double a, double b) { return std::fabs(a - b) < 1.0e-4; };  // number.cu:48-50
+    return close(kGamma, m[0]) && close(kR, m[1]) && close(kQ, m[2]) && close(kAlpha, m[3]) &&
+           close(kTi, m[4]) && close(kCv, m[5]);
+}
+
+namespace {
+
 int bc_function(const std::string& name)
 {
     if (name == "supersonic_vortex") return 1;
@@ -80,14 +92,12 @@ JobConfig read_config_json(const std::string& path)
     Json js = JsonParser(slurp(path)).parse();
     JobConfig c;
     c.title = js["title"].as_str();
-    // Gas-model consistency check, config.cu:369-386 (ideal air is the built model).
+    // Gas-model consistency check against the built model, config.cu:369-386.
     const Json& gm = js["gas_model"];
-    const double g = 1.4, R = 287.1, Cv = R / (g - 1.0);
-    auto close = [](double a, double b) { return std::fabs(a - b) < 1.0e-4; };
-    if (!close(g, gm["gamma"].as_num()) || !close(R, gm["R"].as_num()) || !close(0.0, gm["q"].as_num()) ||
-        !close(0.0, gm["alpha"].as_num()) || !close(0.0, gm["Ti"].as_num()) ||
-        !close(Cv, gm["Cv"].as_num()))
-        throw std::runtime_error("Incorrect gas model data");
+    const double model[6] = {gm["gamma"].as_num(), gm["R"].as_num(), gm["q"].as_num(),
+                             gm["alpha"].as_num(), gm["Ti"].as_num(), gm["Cv"].as_num()};
+    if (!gas_model_matches(model)) throw std::runtime_error("Incorrect gas model data");
+    c.gas_model.assign(model, model + 6);
     const Json& ion = js["iovar_names"];
     for (int q = 0; q < 13 && q < (int)ion.size(); ++q)
         if (ion[q].as_str() != kIoNames[q])

@@ -33,7 +33,12 @@ struct JobConfig {
     bool reacting = false, viscous = false;
     int source_terms = 0;
     std::vector<double> bl_profile;  // p_e, then (y, T, velx) x 12, when a side uses it
+    std::vector<double> gas_model;   // gamma, R, q, alpha, Ti, Cv
 };
+
+// The job's gas model (gamma, R, q, alpha, Ti, Cv) is the built one, to 1e-4 absolute as
+// config.cu:369-386 checks it.
+bool gas_model_matches(const double* model);
 
 // Throw std::runtime_error with the reference's message text on failure.
 JobConfig read_config_json(const std::string& path);

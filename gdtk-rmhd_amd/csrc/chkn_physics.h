@@ -13,10 +13,21 @@
 
 namespace chkn {
 
-// Ideal air, gas.cu:86-93 (the IDEAL_AIR build).
+// Gas model, chosen at build time as the reference chooses it (gas.cu:20-48): ideal air, or
+// with CHKN_AB_REACTING_GAS the reacting A->B gas (libchkn_ab.so).
+#ifdef CHKN_AB_REACTING_GAS
+constexpr double kGamma = 6.0 / 5.0;
+constexpr double kR = 287.0;
+constexpr double kQ = 300000.0;
+constexpr double kAlpha = 1000.0;  // reaction rate, 1/s
+constexpr double kTi = 362.58;     // ignition temperature, K
+#else
 constexpr double kGamma = 1.4;
 constexpr double kR = 287.1;
 constexpr double kQ = 0.0;
+constexpr double kAlpha = 0.0;
+constexpr double kTi = 0.0;
+#endif
 constexpr double kCv = kR / (kGamma - 1.0);
 constexpr double kOneSixth = 1.0 / 6.0;   // number.cu:34
 constexpr double kFourSixth = 4.0 / 6.0;  // number.cu:35

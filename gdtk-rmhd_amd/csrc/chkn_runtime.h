@@ -83,6 +83,7 @@ struct Instance {
     double t = 0.0, dt = 0.0, t_plot = 0.0, sigmin = 0.0;
     int step = 0, next_plot_indx = 1, steps_since_last_plot = 0;
     bool have_sig = false;
+    bool reacting = false;  // the A->B reaction after every step (simulate.cu:296-303)
     bool running() const { return step < max_step && t < max_time; }
 };
 
@@ -139,6 +140,7 @@ struct chkn_sim {
     std::vector<double> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz (all instances)
     int x_order = 2, flux_calc = 0;
     bool viscous = false;
+    bool any_reacting = false;
     // Gas states (p, T, a) of the inflow states as configured (update_from_pT), and whether
     // the equation of state of their (rho, e) gives other values.
     std::vector<double> inflow_gas;
@@ -188,6 +190,9 @@ double* ctl_dt(char* base, int q, int N);
 unsigned long long* ctl_sig(char* base, int q, int N);
 int* ctl_run(char* base, int q, int N);
 int* ctl_bad(char* base, int q, int N);
+double* ctl_chem(char* base, int q, int N);
+// Fill the chemistry factors of step slot q from the instances' dt (-1: not reacting).
+void fill_chem(chkn_sim* s, int q);
 StageArgs stage_args(chkn_sim* s, int q);
 // Runs K steps whose dt and run flags are already in the host control area.
 void run_segment(chkn_sim* s, int K, bool dtred_last);

@@ -258,6 +258,20 @@ int* ctl_run(char* base, int q, int N)
 {
     return reinterpret_cast<int*>(base + 2 * sizeof(double) * kMaxSeg * N) + (size_t)q * N;
 }
+double* ctl_chem(char* base, int q, int N)
+{
+    return reinterpret_cast<double*>(base + (2 * sizeof(double) + sizeof(int)) * kMaxSeg * N +
+                                     sizeof(int) * 3 * kMaxSeg) + (size_t)q * N;
+}
+
+void fill_chem(chkn_sim* s, int q)
+{
+    const int N = (int)s->inst.size();
+    double* c = ctl_chem(s->h_ctl, q, N);
+    const double* dt = ctl_dt(s->h_ctl, q, N);
+    for (int qi = 0; qi < N; ++qi) c[qi] = s->inst[qi].reacting ? std::exp(-kAlpha * dt[qi]) : -1.0;
+}
+
 int* ctl_bad(char* base, int q, int N)
 {
     return reinterpret_cast<int*>(base + 2 * sizeof(double) * kMaxSeg * N + sizeof(int) * kMaxSeg * N) + 3 * q;
@@ -266,7 +280,7 @@ int* ctl_bad(char* base, int q, int N)
 void check_desc(const chkn_desc* d)
 {
     if (d->nblocks <= 0) throw std::invalid_argument("no blocks");
-    if (d->reacting) throw std::invalid_argument("the reacting-gas model is not built");
+    if (d->gas_model && !gas_model_matches(d->gas_model)) throw std::invalid_argument("Incorrect gas model data");
     if (d->flux_calc != 0 && d->flux_calc != 1) throw std::invalid_argument("flux_calc must be 0 (ausmdv) or 1 (sbp_asf)");
     if (d->x_order != 1 && d->x_order != 2) throw std::invalid_argument("x_order must be 1 or 2");
     if (d->n_cfl <= 0) throw std::invalid_argument("need at least one cfl value");
@@ -302,6 +316,8 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
         I.nib = d->nib; I.njb = d->njb; I.nkb = d->nkb;
         I.blk_ids.assign(d->blk_ids, d->blk_ids + (size_t)d->nib * d->njb * d->nkb);
         I.cfl_count = d->cfl_count; I.max_step = d->max_step; I.print_count = d->print_count;
+        I.reacting = d->reacting != 0;
+        s->any_reacting = s->any_reacting || I.reacting;
         I.max_time = d->max_time; I.dt_init = d->dt_init;
         I.cfl_t.assign(d->cfl_times, d->cfl_times + d->n_cfl);
         I.cfl_v.assign(d->cfl_values, d->cfl_values + d->n_cfl);
@@ -470,7 +486,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     split_inflow_gas(s, inflow_pTYB);
     split_tables(s);
     const int N = ninst;
-    s->ctl_bytes = (2 * sizeof(double) + sizeof(int)) * (size_t)kMaxSeg * N + sizeof(int) * 3 * kMaxSeg;
+    s->ctl_bytes = (3 * sizeof(double) + sizeof(int)) * (size_t)kMaxSeg * N + sizeof(int) * 3 * kMaxSeg;
     HIPCHK(hipMalloc(&s->d_ctl, s->ctl_bytes));
     HIPCHK(hipHostMalloc(&s->h_ctl, s->ctl_bytes, hipHostMallocDefault));
     std::memset(s->h_ctl, 0, s->ctl_bytes);
@@ -505,6 +521,7 @@ StageArgs stage_args(chkn_sim* s, int q)
     a.ctl.sigmin_bits = ctl_sig(s->d_ctl, q, N);
     a.ctl.run = ctl_run(s->d_ctl, q, N);
     a.ctl.bad = ctl_bad(s->d_ctl, q, N);
+    a.ctl.chem = ctl_chem(s->d_ctl, q, N);
     a.tile_base = 0;
     a.ntiles = 0;
     a.win = a.wout = 0;
@@ -521,7 +538,12 @@ void launch_segment(chkn_sim* s, int K, bool dtred_last, bool fresh)
             a.win = stage - 1;
             a.wout = stage % 3;
             bool dtred = (stage == 3) && dtred_last && (q == K - 1);
-            launch_stage_form(s, a, stage, dtred, fresh && q == 0 && stage == 1, s->stream);
+            // With reacting instances the reduction follows the reaction.
+            launch_stage_form(s, a, stage, dtred && !s->any_reacting, fresh && q == 0 && stage == 1,
+                              s->stream);
+            if (stage == 3 && s->any_reacting)
+                HIPCHK(launch_chemistry(s->d_blocks, s->d_split_cells, s->nsplit_cells, a.ctl, a.wout,
+                                        dtred, s->stream));
         }
     }
 }
@@ -529,7 +551,7 @@ void launch_segment(chkn_sim* s, int K, bool dtred_last, bool fresh)
 long long launches_per_segment(chkn_sim* s, int K, bool fresh)
 {
     const long long per_stage = launches_per_stage(s, false);
-    long long n = 3LL * K * per_stage;
+    long long n = 3LL * K * per_stage + (s->any_reacting ? K : 0);
     if (fresh) n += launches_per_stage(s, true) - per_stage;  // first stage in face-list form
     return n;
 }
@@ -670,6 +692,7 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
                 dtv[qi] = s->inst[qi].dt;
                 run[qi] = (q < nrun[qi]) ? 1 : 0;
             }
+            fill_chem(s, q);
         }
         bool dtred = ((s->gstep + K) % cfl_count) == 0;
         run_segment(s, K, dtred);
@@ -860,6 +883,7 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
     d.reacting = c.reacting; d.viscous = c.viscous;
     d.bc_twall = twall.data();
     d.bc_fun = bcfun.data();
+    d.gas_model = c.gas_model.data();
     d.bl_profile = c.bl_profile.empty() ? nullptr : c.bl_profile.data();
     d.n_twall_faces = (int)twall_faces.size();
     d.twall_faces = twall_faces.empty() ? nullptr : twall_faces.data();

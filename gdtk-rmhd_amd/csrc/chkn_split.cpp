@@ -226,7 +226,7 @@ void split_tables(chkn_sim* s)
     s->d_split_faces = nullptr;
     s->d_split_cells = nullptr;
     s->nsplit_faces = s->nsplit_cells = 0;
-    if (!need_gas(s)) return;
+    if (!need_gas(s) && !s->any_reacting) return;
     std::vector<FaceChunk> fc;
     std::vector<CellChunk> cc;
     for (size_t ib = 0; ib < s->blocks.size(); ++ib) {

@@ -486,7 +486,59 @@ __global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A)
     }
 }
 
+// The A->B reaction of every cell after a step (simulate.cu:296-303, block.cu:1174-1184,
+// gas.cu:102-111): YA = 1 - YB; where T > Ti, YB = 1 - YA exp(-alpha dt), with the factor
+// computed on the host for the instance's dt; then U = encode(state) (level 0, every cell of
+// a reacting instance).
+template <bool DTRED>
+__global__ __launch_bounds__(kChunk) void chemistry_kernel(const BlockDev* blocks,
+                                                           const CellChunk* cells, StepCtl ctl,
+                                                           int slot)
+{
+    const CellChunk ch = cells[blockIdx.x];
+    const BlockDev* B = blocks + ch.blk;
+    if (!ctl.run[B->inst]) return;  // uniform over the work-group
+    const double factor = ctl.chem[B->inst];
+    const int cidx = ch.c0 + (int)threadIdx.x;
+    unsigned long long sigmin = ~0ull;
+    if (cidx < ch.c1) {
+        const int n = (int)B->n;
+        double* __restrict__ W = B->W + slot * B->wstride;
+        Prim w = ld_prim(W, n, cidx);
+        if (factor >= 0.0) {
+            const double T = eos_T(w.e, w.YB);
+            const double YA = 1.0 - w.YB;
+            if (T > kTi) {
+                w.YB = 1.0 - YA * factor;
+                W[2 * n + cidx] = w.YB;
+            }
+            double U[6];
+            encode(w, U);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) B->U[q * n + cidx] = U[q];
+        }
+        if (DTRED) {
+            const int nic = B->nic, njc = B->njc;
+            sigmin = signal_bits(B, n, cidx % nic, (cidx / nic) % njc, cidx / (nic * njc), cidx, w);
+        }
+    }
+    if (DTRED) {
+        const unsigned long long wm = wave_min_u64(sigmin);
+        if ((threadIdx.x & 63) == 0 && wm != ~0ull) atomicMin(&ctl.sigmin_bits[B->inst], wm);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_chemistry(const BlockDev* blocks, const CellChunk* cells, int ncell_chunks,
+                            const StepCtl& ctl, int slot, bool dt_reduce, hipStream_t s)
+{
+    if (ncell_chunks <= 0) return hipSuccess;
+    const dim3 grid(ncell_chunks), blk(kChunk);
+    if (dt_reduce) hipLaunchKernelGGL((chemistry_kernel<true>), grid, blk, 0, s, blocks, cells, ctl, slot);
+    else hipLaunchKernelGGL((chemistry_kernel<false>), grid, blk, 0, s, blocks, cells, ctl, slot);
+    return hipGetLastError();
+}
 
 hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order, int flux_calc,
                               bool viscous, hipStream_t s)

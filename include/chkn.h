@@ -68,7 +68,7 @@ typedef struct {
     const double* dt_plot_values;
     double t0;               /* start time (times.data entry for the start index) */
     int start_tindx;
-    int reacting;            /* must be 0: not built */
+    int reacting;            /* 1: the A->B reaction after every step (simulate.cu:296-303) */
     int viscous;             /* 1 = add viscous fluxes (block.cu:88-162; simulate.cu:201,235-238) */
     const double* bc_twall;  /* 6 per block: TWall of wall_no_slip_fixed_T sides (NaN: TWall
                                 given per face in twall_faces); may be NULL without such walls */
@@ -81,6 +81,9 @@ typedef struct {
     const double* twall_faces; /* TWall per boundary face of every fixed-T side with NaN in
                                   bc_twall, blocks in order, sides in order, faces in the
                                   TWall-file order (block.cu:566-611); NULL if none */
+    const double* gas_model;   /* gamma, R, q, alpha, Ti, Cv of the job (config.json
+                                  "gas_model"); must be the library's built model
+                                  (config.cu:369-386); NULL: take the built model */
 } chkn_desc;
 
 /* Progress of the march, filled by chkn_step and chkn_march. */

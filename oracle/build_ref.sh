@@ -13,7 +13,10 @@ if [ ! -f "$REF/block.cu" ]; then
     exit 0
 fi
 mkdir -p "$HERE/_ref"
-/opt/rocm/bin/hipcc -x hip --offload-host-only -std=c++17 -O2 -fopenmp -DIDEAL_AIR \
-    -ffp-contract=off -Wno-deprecated-literal-operator -w \
-    -I"$REF" "$HERE/ref_harness.cpp" -o "$HERE/_ref/chkn_ref" -lz
-echo "built $HERE/_ref/chkn_ref"
+# Two gas-model flavours, as the reference's makefile builds them (GAS_MODEL).
+for flavour in IDEAL_AIR:chkn_ref AB_REACTING_GAS:chkn_ref_ab; do
+    /opt/rocm/bin/hipcc -x hip --offload-host-only -std=c++17 -O2 -fopenmp -D"${flavour%%:*}" \
+        -ffp-contract=off -Wno-deprecated-literal-operator -w \
+        -I"$REF" "$HERE/ref_harness.cpp" -o "$HERE/_ref/${flavour##*:}" -lz
+    echo "built $HERE/_ref/${flavour##*:}"
+done

@@ -32,9 +32,16 @@ static const double six = 6.0, seven = 7.0, eight = 8.0;
 #define FOUR_SIXTH (4.0 / 6.0)
 #define ONE_TWELFTH (1.0 / 12.0)
 
+/* gas.cu:20-48: the gas model is chosen when the oracle is built, as the reference's is. */
+#ifdef ORC_AB_REACTING_GAS
+static const double GAS_G = 6.0 / 5.0;
+static const double GAS_R = 287.0;
+static const double GAS_Q = 300000.0;
+#else
 static const double GAS_G = 1.4;
 static const double GAS_R = 287.1;
 static const double GAS_Q = 0.0;
+#endif
 #define GAS_CV (GAS_R / (GAS_G - 1.0))
 
 static char g_err[512];
@@ -89,7 +96,7 @@ struct orc_sim {
     Blk* b;
     int nfs;
     FS* inflow;
-    int x_order, flux_calc, cfl_count, max_step, viscous;
+    int x_order, flux_calc, cfl_count, max_step, viscous, reacting;
     double max_time, dt_init;
     int n_cfl;
     double* cfl_t;
@@ -325,6 +332,21 @@ static int bl_set(BLProfile* bl, const double* v)
     if (spline_set(&bl->T, ys, Ts) || spline_set(&bl->velx, ys, us)) return -1;
     bl->set = 1;
     return 0;
+}
+
+static void update_from_rhoe(FS* s);
+
+/* gas.cu:102-111: the A->B reaction in an isolated blob of gas over dt. */
+static void update_chemistry(FS* s, double dt)
+{
+#ifdef ORC_AB_REACTING_GAS
+    const double alpha = 1000.0, Ti = 362.58;
+#else
+    const double alpha = 0.0, Ti = 0.0;
+#endif
+    double YA = one - s->YB;
+    if (s->T > Ti) s->YB = one - YA * exp(-alpha * dt);
+    update_from_rhoe(s);
 }
 
 static void update_from_rhoe(FS* s) /* gas.cu:128-135 */
@@ -1109,6 +1131,7 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
     }
     s->x_order = p->x_order; s->flux_calc = p->flux_calc; s->cfl_count = p->cfl_count;
     s->viscous = p->viscous;
+    s->reacting = p->reacting;
     s->max_step = p->max_step; s->max_time = p->max_time; s->dt_init = p->dt_init;
     s->n_cfl = p->n_cfl;
     s->cfl_t = (double*)malloc(sizeof(double) * (size_t)p->n_cfl);
@@ -1246,6 +1269,15 @@ int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
             if (!b->active) continue;
             memcpy(b->Q, b->Q + (size_t)b->ncells * 6, sizeof(double) * (size_t)b->ncells * 6);
         }
+        if (s->reacting) /* :296-303, block.cu:1174-1184 */
+            for (int ib = 0; ib < s->nblocks; ++ib) {
+                Blk* b = &s->b[ib];
+                if (!b->active) continue;
+                for (int c = 0; c < b->ncells; ++c) {
+                    update_chemistry(&b->cells[c].fs, s->dt);
+                    encode(&b->cells[c].fs, &b->Q[(size_t)c * 6]);
+                }
+            }
         if (dt_log) dt_log[done] = s->dt;
         s->t += s->dt;
         s->step += 1;

@@ -16,6 +16,9 @@
 extern "C" {
 #endif
 
+/* Built twice: liborc.so for ideal air and liborc_ab.so (ORC_AB_REACTING_GAS) for the
+ * reacting A->B gas, as the reference selects its gas model at build time (gas.cu:20-48). */
+
 /* Boundary-condition codes, face.cu:22-36. */
 enum { ORC_WALL_SLIP = 0, ORC_WALL_NOSLIP_ADIABATIC = 1, ORC_WALL_NOSLIP_FIXED_T = 2,
        ORC_EXCHANGE = 3, ORC_INFLOW = 4, ORC_OUTFLOW = 5, ORC_INFLOW_FUNCTION = 6 };
@@ -38,6 +41,7 @@ typedef struct {
     const double* bl_profile; /* p_e, then (y, T, velx) x 12 (config.cu:37-60), or NULL */
     int n_twall_faces;    /* TWall per face of the NaN sides, block and side order, each side */
     const double* twall_faces; /* in the TWall-file order (block.cu:566-611) */
+    int reacting;         /* the A->B reaction after every step (simulate.cu:296-303) */
 } orc_params;
 
 typedef struct orc_sim orc_sim;

@@ -152,6 +152,11 @@ static int run(const char* job, int tindx, int nsteps, const char* dtlog, bool b
 #pragma omp parallel for
         for (int ib = 0; ib < nb; ++ib)
             if (blk_configs[ib].active) fluidBlocks[ib].copy_conserved_data(blk_configs[ib], 1, 0);
+        if (Config::reacting) { // simulate.cu:296-303
+#pragma omp parallel for
+            for (int ib = 0; ib < nb; ++ib)
+                if (blk_configs[ib].active) fluidBlocks[ib].update_chemistry(blk_configs[ib], H::dt);
+        }
         if (fdt) std::fprintf(fdt, "%d %.17g\n", H::step, H::dt);
         H::t += H::dt;
         H::step += 1;

@@ -28,9 +28,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "gdtk-rmhd_amd"))
 REF = os.path.join(ROOT, "oracle", "_ref", "chkn_ref")
+REF_AB = os.path.join(ROOT, "oracle", "_ref", "chkn_ref_ab")  # the reacting A->B gas build
 
 from chkn import cases, prep  # noqa: E402
-from chkn.job import BL_PROFILE_FILE, JobData, read_flow_file, write_bl_profile  # noqa: E402
+from chkn.job import BL_PROFILE_FILE, JobData, gas_flavour, read_flow_file, write_bl_profile  # noqa: E402
 
 GAMMA, R = 1.4, 287.1
 CV = R / (GAMMA - 1.0)
@@ -149,6 +150,8 @@ STEP_JOBS = {
     "bl_plate_x1": (lambda: cases.bl_plate(max_step=20, x_order=1, seed=6), 20),
     "sbp": (lambda: cases.sbp_channel(max_step=25), 25),
     "sbp_viscous": (lambda: _viscous_sbp(), 20),
+    "ab_reacting": (lambda: cases.ab_reacting(max_step=30), 30),
+    "ab_inert": (lambda: cases.ab_reacting(max_step=20, seed=5, reacting=False), 20),
 }
 
 
@@ -178,12 +181,14 @@ def _two_cell_block():
 def step_fixture(name, builder, steps):
     built = builder()
     prof = built[1] if isinstance(built, tuple) else None
+    job = built[0] if isinstance(built, tuple) else built
+    ref = REF_AB if gas_flavour(job.gas_model) == "ab" else REF
     with tempfile.TemporaryDirectory() as d:
         prep.write_initial_files(True, d)
         if prof is not None:  # the solver reads the profile from its working directory
             write_bl_profile(os.path.join(d, BL_PROFILE_FILE), prof)
         dtlog = os.path.join(d, "dt.log")
-        subprocess.run([REF, "run", d, "0", str(steps), dtlog], check=True, capture_output=True, cwd=d)
+        subprocess.run([ref, "run", d, "0", str(steps), dtlog], check=True, capture_output=True, cwd=d)
         dts = np.loadtxt(dtlog, ndmin=2)[:, 1]
         jd = JobData.from_job_dir(d, bl_profile=prof)
         fields = [read_flow_file(os.path.join(d, "flow", "t0001", "flow-%04d-%04d-%04d.bin" % (b.i, b.j, b.k)),

@@ -9,6 +9,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "liborc.so")
+LIB_AB = os.path.join(ROOT, "oracle", "liborc_ab.so")  # the reacting A->B gas build
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "chkn_ref")
 
 _I = ctypes.POINTER(ctypes.c_int)
@@ -21,18 +22,20 @@ class OrcParams(ctypes.Structure):
                 ("dt_init", ctypes.c_double), ("n_cfl", ctypes.c_int), ("cfl_times", _D),
                 ("cfl_values", _D), ("t0", ctypes.c_double), ("viscous", ctypes.c_int),
                 ("bc_twall", _D), ("bc_fun", _I), ("bl_profile", _D), ("n_twall_faces", ctypes.c_int),
-                ("twall_faces", _D)]
+                ("twall_faces", _D), ("reacting", ctypes.c_int)]
 
 
-_lib = None
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
-            raise RuntimeError("oracle/liborc.so is not built; run make -C oracle")
-        lib = ctypes.CDLL(LIB)
+_libs = {}
+
+
+def load(flavour="air"):
+    if flavour not in _libs:
+        path = LIB_AB if flavour == "ab" else LIB
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built; run make -C oracle" % path)
+        lib = ctypes.CDLL(path)
         lib.orc_create.restype = ctypes.c_void_p
         lib.orc_create.argtypes = [ctypes.c_int] * 4 + [_I] * 6 + [ctypes.c_int, _D, _D, _D,
                                                                   ctypes.POINTER(OrcParams)]
@@ -55,8 +58,8 @@ def load():
         lib.orc_decode.argtypes = [_D, _D]
         lib.orc_decode.restype = ctypes.c_int
         lib.orc_encode.argtypes = [_D, _D]
-        _lib = lib
-    return _lib
+        _libs[flavour] = lib
+    return _libs[flavour]
 
 
 def _p(a, t):
@@ -67,14 +70,16 @@ class Oracle:
     """CPU restatement of the reference march on a JobData."""
 
     def __init__(self, job):
-        lib = load()
+        from chkn.job import gas_flavour
+        lib = load(gas_flavour(job.gas_model))
         f = job.flat()
         self._keep = f
         prm = OrcParams(job.x_order, job.flux_calc, job.cfl_count, job.max_step, job.max_time,
                         job.dt_init, len(job.cfl_times), _p(f["cfl_times"], _D),
                         _p(f["cfl_values"], _D), job.t0, int(job.viscous), _p(f["bc_twall"], _D),
                         _p(f["bc_fun"], _I), _p(f["bl_profile"], _D) if len(f["bl_profile"]) else None,
-                        len(f["twall_faces"]), _p(f["twall_faces"], _D) if len(f["twall_faces"]) else None)
+                        len(f["twall_faces"]), _p(f["twall_faces"], _D) if len(f["twall_faces"]) else None,
+                        int(job.reacting))
         self._prm = prm
         h = lib.orc_create(len(job.blocks), job.nib, job.njb, job.nkb, _p(f["blk_ids"], _I),
                            _p(f["blk_ijk"], _I), _p(f["blk_dims"], _I), _p(f["active"], _I),

@@ -16,11 +16,13 @@ def declared_functions():
 
 
 def test_every_declared_symbol_is_exported():
+    # Both gas-model flavours: ideal air and the reacting A->B gas.
     names = declared_functions()
     assert len(names) >= 15
-    so = ctypes.CDLL(lib.LIB_PATH)
-    for n in names:
-        assert hasattr(so, n), n
+    for path in (lib.LIB_PATH, lib.LIB_AB_PATH):
+        so = ctypes.CDLL(path)
+        for n in names:
+            assert hasattr(so, n), (path, n)
     assert set(names) <= set(lib.EXPORTS), set(names) - set(lib.EXPORTS)
 
 
@@ -40,6 +42,13 @@ def test_gas_model_mismatch_is_a_config_error(tmp_path):
     prep.write_initial_files(True, str(tmp_path))
     so = lib.load()
     h = ctypes.c_void_p()
+    rc = so.chkn_load_job(str(tmp_path).encode(), 0, 1, ctypes.byref(h))
+    assert rc == -3
+    assert so.chkn_last_error(None) == b"Incorrect gas model data"
+    # ... and the ideal-air job in the reacting-gas build.
+    cases.sod_box(4, 4, 4, max_step=1)
+    prep.write_initial_files(True, str(tmp_path))
+    so = lib.load("ab")
     rc = so.chkn_load_job(str(tmp_path).encode(), 0, 1, ctypes.byref(h))
     assert rc == -3
     assert so.chkn_last_error(None) == b"Incorrect gas model data"

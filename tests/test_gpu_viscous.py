@@ -151,3 +151,28 @@ def test_sbp_asf_gradient_is_rejected():
     with pytest.raises(ChknError, match="adjoint of the SBP-ASF flux"):
         sim.grad_forward(2, 2)
     sim.close()
+
+
+def test_reacting_gas_matches_oracle():
+    # The A->B reacting gas build (libchkn_ab.so): gas constants and the reaction step.
+    compare(cases.ab_reacting(max_step=30), 30)
+
+
+def test_reacting_ensemble_and_launch_options():
+    # A reacting and an inert instance in one handle equal each run alone; hipGraph replay
+    # and split calls change nothing.
+    jobs = [cases.ab_reacting(max_step=24, seed=2), cases.ab_reacting(max_step=24, seed=5, reacting=False)]
+    ens = _run(jobs, 24, launch=dict(use_graph=True), splits=[5, 19])
+    off = 0
+    for job in jobs:
+        single = _run(job, 24)
+        for b in range(len(job.blocks)):
+            assert np.array_equal(single[b], ens[off + b])
+        off += len(job.blocks)
+
+
+def test_reacting_gradient_is_rejected():
+    sim = Sim(cases.ab_reacting(max_step=5))
+    with pytest.raises(ChknError, match="adjoint of the A->B reaction"):
+        sim.grad_forward(2, 2)
+    sim.close()

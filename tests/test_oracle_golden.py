@@ -67,6 +67,8 @@ STEP_JOBS = {
     "bl_plate_x1": (lambda: cases.bl_plate(max_step=20, x_order=1, seed=6), 20),
     "sbp": (lambda: cases.sbp_channel(max_step=25), 25),
     "sbp_viscous": (lambda: _viscous_sbp(), 20),
+    "ab_reacting": (lambda: cases.ab_reacting(max_step=30), 30),
+    "ab_inert": (lambda: cases.ab_reacting(max_step=20, seed=5, reacting=False), 20),
 }
 
 
