@@ -99,6 +99,12 @@ struct StageArgs {
 struct FaceChunk {
     int blk, dir, f0, f1;  // faces [f0, f1) of direction dir, at most 128
 };
+// Faces of direction dir at k-level k whose (i, j) lie in the kPatchI x kPatchJ patch at
+// (i0, j0).
+constexpr int kPatchI = 16, kPatchJ = 16;
+struct FacePatch {
+    int blk, dir, i0, j0, k;
+};
 struct CellChunk {
     int blk, c0, c1;       // cells [c0, c1), at most 256
 };
@@ -144,6 +150,7 @@ struct SplitArgs {
     const double* inflow;
     const double* inflow_gas;
     const FaceChunk* faces;  // at most 256 faces each
+    const FacePatch* patches;  // the viscous pass's work items
     const CellChunk* cells;  // at most 256 cells each
     StepCtl ctl;
     int win, wout;
@@ -151,7 +158,7 @@ struct SplitArgs {
 };
 hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order, int flux_calc,
                               bool viscous, hipStream_t s);
-hipError_t launch_split_viscous(const SplitArgs& a, int nface_chunks, hipStream_t s);
+hipError_t launch_split_viscous(const SplitArgs& a, int npatches, hipStream_t s);
 hipError_t launch_split_update(const SplitArgs& a, int ncell_chunks, int stage, bool dt_reduce,
                                hipStream_t s);
 

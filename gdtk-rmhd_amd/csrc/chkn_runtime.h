@@ -156,8 +156,9 @@ struct chkn_sim {
     double* d_inflow = nullptr;
     double* d_inflow_gas = nullptr;
     // face-list stage work tables
-    int nsplit_faces = 0, nsplit_cells = 0;
+    int nsplit_faces = 0, nsplit_cells = 0, nsplit_patches = 0;
     chkn::FaceChunk* d_split_faces = nullptr;
+    chkn::FacePatch* d_split_patches = nullptr;
     chkn::CellChunk* d_split_cells = nullptr;
     // Control area, kMaxSeg step slots: dt[S][N], sigmin[S][N], run[S][N], bad[S][3].
     char* d_ctl = nullptr;

@@ -223,12 +223,15 @@ void split_tables(chkn_sim* s)
 {
     dfree(s->d_split_faces);
     dfree(s->d_split_cells);
+    dfree(s->d_split_patches);
     s->d_split_faces = nullptr;
     s->d_split_cells = nullptr;
-    s->nsplit_faces = s->nsplit_cells = 0;
+    s->d_split_patches = nullptr;
+    s->nsplit_faces = s->nsplit_cells = s->nsplit_patches = 0;
     if (!need_gas(s) && !s->any_reacting) return;
     std::vector<FaceChunk> fc;
     std::vector<CellChunk> cc;
+    std::vector<FacePatch> fp;  // viscous pass: block, k, patch, direction
     for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
         const HostBlock& b = s->blocks[ib];
         if (!b.active) continue;
@@ -240,6 +243,21 @@ void split_tables(chkn_sim* s)
                 fc.push_back(FaceChunk{(int)ib, d, (int)f0, (int)std::min<long long>(f0 + kChunk, nf[d])});
         for (int c0 = 0; c0 < b.n; c0 += kChunk)
             cc.push_back(CellChunk{(int)ib, c0, std::min(c0 + kChunk, b.n)});
+        if (s->viscous)
+            for (int k = 0; k <= b.nkc; ++k)
+                for (int j0 = 0; j0 <= b.njc; j0 += kPatchJ)
+                    for (int i0 = 0; i0 <= b.nic; i0 += kPatchI)
+                        for (int d = 0; d < 3; ++d) {
+                            if (k == b.nkc && d != 2) continue;          // only k-faces at nkc
+                            if (d != 0 && i0 >= b.nic) continue;         // i = nic: i-faces only
+                            if (d != 1 && j0 >= b.njc) continue;         // j = njc: j-faces only
+                            fp.push_back(FacePatch{(int)ib, d, i0, j0, k});
+                        }
+    }
+    s->nsplit_patches = (int)fp.size();
+    if (!fp.empty()) {
+        HIPCHK(hipMalloc(&s->d_split_patches, sizeof(FacePatch) * fp.size()));
+        HIPCHK(hipMemcpy(s->d_split_patches, fp.data(), sizeof(FacePatch) * fp.size(), hipMemcpyHostToDevice));
     }
     s->nsplit_faces = (int)fc.size();
     s->nsplit_cells = (int)cc.size();
@@ -263,7 +281,8 @@ void split_free(chkn_sim* s)
             p = nullptr;
         }
     }
-    dfree(s->d_split_faces); dfree(s->d_split_cells); dfree(s->d_inflow_gas);
+    dfree(s->d_split_faces); dfree(s->d_split_cells); dfree(s->d_inflow_gas); dfree(s->d_split_patches);
+    s->d_split_patches = nullptr;
     s->d_split_faces = nullptr;
     s->d_split_cells = nullptr;
     s->d_inflow_gas = nullptr;
@@ -287,13 +306,14 @@ void launch_stage_form(chkn_sim* s, const StageArgs& a, int stage, bool dtred, b
         sa.inflow = a.inflow;
         sa.inflow_gas = s->d_inflow_gas;
         sa.faces = s->d_split_faces;
+        sa.patches = s->d_split_patches;
         sa.cells = s->d_split_cells;
         sa.ctl = a.ctl;
         sa.win = a.win;
         sa.wout = a.wout;
         sa.file_gas = file_gas ? 1 : 0;
         HIPCHK(launch_split_faces(sa, s->nsplit_faces, s->x_order, s->flux_calc, s->viscous, st));
-        if (s->viscous) HIPCHK(launch_split_viscous(sa, s->nsplit_faces, st));
+        if (s->viscous) HIPCHK(launch_split_viscous(sa, s->nsplit_patches, st));
         HIPCHK(launch_split_update(sa, s->nsplit_cells, stage, dtred, st));
         return;
     }

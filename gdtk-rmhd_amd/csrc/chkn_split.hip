@@ -344,13 +344,23 @@ __device__ __forceinline__ void cloud_offset(const BlockDev* B, int dir, const F
 // over the face's cloud, with the weights of block.cu:28-86 (normal matrix inverted by
 // cofactors, rsla.cu:53-72) formed on the fly from the cell and face centres -- two passes
 // over the cloud, the second recomputing each offset -- rather than read from memory.
-__global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A)
+// A work-group takes a kPatchI x kPatchJ patch of faces of one direction at one k-level; the patch
+// table runs block, k, patch, direction, and work-group b (on XCD b % 8) takes the b/8-th
+// entry of XCD b%8's contiguous range, so the clouds of neighbouring patches, which share
+// most of their faces and cells, are read through one L2.
+__global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int npatches)
 {
-    const FaceChunk ch = A.faces[blockIdx.x];
+    const int per_xcd = (npatches + 7) >> 3;
+    const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (item >= npatches) return;
+    const FacePatch ch = A.patches[item];
     const BlockDev* B = A.blocks + ch.blk;
-    const int f = ch.f0 + (int)threadIdx.x;
-    if (f >= ch.f1 || !A.ctl.run[B->inst]) return;
     const int dir = ch.dir;
+    const int i = ch.i0 + (int)threadIdx.x % kPatchI, j = ch.j0 + (int)threadIdx.x / kPatchI;
+    const int nic = B->nic, njc = B->njc;
+    if (i > nic - (dir != 0) || j > njc - (dir != 1) || !A.ctl.run[B->inst]) return;
+    const int f = dir == 0 ? i + (nic + 1) * (j + njc * ch.k)
+                           : (dir == 1 ? i + nic * (j + (njc + 1) * ch.k) : i + nic * (j + njc * ch.k));
     const FacePos p = face_pos(B, dir, f);
     const long long nft = B->nfi + B->nfj + B->nfk;
     const double* __restrict__ FS = B->FS;
@@ -558,10 +568,11 @@ hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order,
     return hipGetLastError();
 }
 
-hipError_t launch_split_viscous(const SplitArgs& a, int nface_chunks, hipStream_t s)
+hipError_t launch_split_viscous(const SplitArgs& a, int npatches, hipStream_t s)
 {
-    if (nface_chunks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(split_viscous_kernel, dim3(nface_chunks), dim3(kChunk), 0, s, a);
+    if (npatches <= 0) return hipSuccess;
+    const int grid = 8 * ((npatches + 7) / 8);
+    hipLaunchKernelGGL(split_viscous_kernel, dim3(grid), dim3(kChunk), 0, s, a, npatches);
     return hipGetLastError();
 }
 
