@@ -44,7 +44,7 @@ class _Unroll(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, inflow, sim, nsteps, checkpoint_every, dts):
         src = u0.detach().contiguous()
-        torch.cuda.synchronize()  # the library works on its own stream
+        sim.set_stream(torch.cuda.current_stream())  # ordered with torch's work on the device
         sim.write_conserved(src)
         w = inflow.detach().to("cpu", torch.float64).numpy()
         for s in range(w.shape[0]):
@@ -52,6 +52,7 @@ class _Unroll(torch.autograd.Function):
         ctx.dts = sim.grad_forward(nsteps, checkpoint_every, dts)
         uN = torch.empty_like(u0)
         sim.read_conserved(uN)
+        sim.set_stream(None)
         ctx.sim = sim
         ctx.inflow_device = inflow.device
         return uN
@@ -60,8 +61,9 @@ class _Unroll(torch.autograd.Function):
     def backward(ctx, g):
         lam = g.contiguous()
         gu0 = torch.empty_like(lam)
-        torch.cuda.synchronize()
+        ctx.sim.set_stream(torch.cuda.current_stream())
         gin = ctx.sim.grad_backward(lam, gu0)
+        ctx.sim.set_stream(None)
         return gu0, torch.as_tensor(gin, dtype=torch.float64, device=ctx.inflow_device), None, None, None, None
 
 

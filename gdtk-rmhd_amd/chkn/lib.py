@@ -54,6 +54,8 @@ EXPORTS = {
     "chkn_get_instance_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                                 ctypes.POINTER(ChknReport)]),
     "chkn_state_size": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "chkn_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "chkn_reset_stream": (ctypes.c_int, [ctypes.c_void_p]),
     "chkn_num_flow_states": (ctypes.c_int, [ctypes.c_void_p]),
     "chkn_read_conserved_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "chkn_write_conserved_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -328,6 +330,15 @@ class Sim:
                                                  ctypes.c_void_p(grad_u0.data_ptr()),
                                                  _ptr(gin, _D)))
         return gin[:self.num_flow_states]
+
+    def set_stream(self, stream):
+        """Issue the handle's work on `stream` (a torch.cuda.Stream, or a raw hipStream_t as
+        int; 0 is the null stream), or on its own stream again with None."""
+        if stream is None:
+            self._check(self._lib.chkn_reset_stream(self.h))
+            return
+        raw = int(getattr(stream, "cuda_stream", stream))
+        self._check(self._lib.chkn_set_stream(self.h, ctypes.c_void_p(raw or None)))
 
     def last_timing(self):
         ms = ctypes.c_double()

@@ -149,7 +149,8 @@ struct chkn_sim {
     bool fresh = true;   // no step taken since the state was set from IO variables
     bool failed = false;
     // device
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream all work goes to: own_stream or the caller's
+    hipStream_t own_stream = nullptr;  // created by the handle
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     BlockDev* d_blocks = nullptr;
     Tile* d_tiles = nullptr;

@@ -78,9 +78,9 @@ void free_device(chkn_sim* s)
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
     s->d_blocks = nullptr; s->d_tiles = nullptr; s->d_inflow = nullptr; s->d_ctl = nullptr;
-    s->h_ctl = nullptr; s->ev0 = s->ev1 = nullptr; s->stream = nullptr;
+    s->h_ctl = nullptr; s->ev0 = s->ev1 = nullptr; s->stream = s->own_stream = nullptr;
 }
 
 void upload_blocks(chkn_sim* s)
@@ -421,7 +421,8 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             if (partner[ib][f] >= 0 && s->blocks[partner[ib][f]].active)
                 rpart[partner[ib][f]][f ^ 1] = (int)ib;
     // Device.
-    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking));
+    s->stream = s->own_stream;
     HIPCHK(hipEventCreate(&s->ev0));
     HIPCHK(hipEventCreate(&s->ev1));
     std::vector<BlockDev> bdev(s->blocks.size());
@@ -569,7 +570,7 @@ void run_segment(chkn_sim* s, int K, bool dtred_last)
     HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, s->ctl_bytes, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipEventRecord(s->ev0, s->stream));
     const bool fresh = fresh_needs_file_gas(s);
-    if (s->opts.use_graph) {
+    if (s->opts.use_graph && s->stream != nullptr) {  // the null stream cannot be captured
         GraphKey key{K, dtred_last, fresh};
         auto it = s->graphs.find(key);
         if (it == s->graphs.end()) {
@@ -965,6 +966,26 @@ int chkn_set_iovars(chkn_sim* sim, int blk, const double* iovars)
         sim->fresh = true;
         sim->failed = false;
         for (auto& I : sim->inst) I.have_sig = false;
+        return CHKN_OK;
+    });
+}
+
+int chkn_set_stream(chkn_sim* sim, void* stream)
+{
+    if (!sim || !sim->own_stream) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] {
+        HIPCHK(hipStreamSynchronize(sim->stream));  // work queued so far completes first
+        sim->stream = static_cast<hipStream_t>(stream);
+        return CHKN_OK;
+    });
+}
+
+int chkn_reset_stream(chkn_sim* sim)
+{
+    if (!sim || !sim->own_stream) return set_error(sim, CHKN_ERR_ARG, "bad argument");
+    return guarded(sim, [&] {
+        HIPCHK(hipStreamSynchronize(sim->stream));
+        sim->stream = sim->own_stream;
         return CHKN_OK;
     });
 }

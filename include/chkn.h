@@ -160,6 +160,16 @@ int chkn_num_blocks(chkn_sim* sim);
 int chkn_block_dims(chkn_sim* sim, int blk, int* dims3);
 long long chkn_active_cells(chkn_sim* sim);
 
+/* Issue all further work of the handle on `stream`, a hipStream_t the caller owns (for
+ * example the current stream of a torch device; NULL is the null stream), so the handle's
+ * work is ordered with the caller's without device-wide synchronisation.  Work already
+ * queued completes first.  The caller keeps the stream alive while the handle uses it; on the
+ * null stream, segments run without hipGraph capture.  Each entry point still returns only
+ * after its work on the stream is done. */
+int chkn_set_stream(chkn_sim* sim, void* stream);
+/* Back to the handle's own stream (after the work on the caller's stream completes). */
+int chkn_reset_stream(chkn_sim* sim);
+
 /* Device time of the last chkn_step call's kernels, measured with HIP events on the
  * handle's stream (ms), and the number of stage launches it issued. */
 int chkn_last_timing(chkn_sim* sim, double* ms, long long* launches);

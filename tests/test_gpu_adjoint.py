@@ -233,3 +233,27 @@ def test_first_order_gradient_matches_finite_differences():
     short = Problem([job], nsteps=4, C=3, mpt=[[2.5, 5.0e3, 260.0]])
     _, gu0s, _ = short.gradient()
     _assert_rows(rows + _fd_rows_u0(short, gu0s))
+
+
+def test_work_on_the_callers_stream_gives_identical_results():
+    # chkn_set_stream: a torch side stream, and the null stream with hipGraph replay requested
+    # (it runs without capture there), give the same fields as the handle's own stream.
+    job = cases.ramp(24, 8, 10, max_step=20)
+    ref = Sim(job)
+    ref.step(20)
+    side = Sim(job)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        side.set_stream(torch.cuda.current_stream())
+        side.step(7)
+        side.step(13)
+        side.set_stream(None)
+    null = Sim(job)
+    null.set_launch(use_graph=True)
+    null.set_stream(0)
+    null.step(20)
+    null.set_stream(None)
+    for sim in (side, null):
+        assert np.array_equal(sim.conserved(0), ref.conserved(0))
+        sim.close()
+    ref.close()
