@@ -31,6 +31,9 @@ BYTES_PER_CELL_STEP_VISCOUS = BYTES_PER_CELL_STEP + 3 * 96
 
 WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300, "diff": 500,
                   "ensemble-diff": 300, "viscous": 300}
+# Steps of the reference CPU build timed for cpu_baseline: a sample of about 10-30 s on one
+# core (the viscous step runs about 6x slower there than the inviscid one).
+CPU_STEPS = {"viscous": 1}
 
 
 def build_workload(name, steps_total):
@@ -209,7 +212,7 @@ def main():
     ap.add_argument("--per-block", type=int, default=0)
     ap.add_argument("--compare-per-block", type=int, default=-1,
                     help="also time one launch per block per stage (default: batched only)")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="reference CPU steps timed (default 4; viscous 1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device copy (scale for HBM byte counters)")
@@ -304,7 +307,7 @@ def main():
     if not args.no_cpu:
         # chkn.prep holds the job built last: for an ensemble, its last member (the reference
         # runs one job per process).
-        line["cpu_baseline"] = cpu_baseline(jobs[-1], args.cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(jobs[-1], args.cpu_steps or CPU_STEPS.get(args.workload, 4))
     print(json.dumps(line))
 
 

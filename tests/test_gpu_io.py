@@ -71,3 +71,22 @@ def test_text_march_through_the_library(tmp_path):
     sim.finalize(binary=False)
     sim.close()
     _check(d, gold, "times", "names", "digests", 1, mg)
+
+
+def test_reacting_gas_cli_matches_reference_build(tmp_path):
+    # chkn-run-ab (the A->B gas build) marches the reacting fixture job to its max_step and
+    # writes the final snapshot: bit for bit the reference AB build's fields.
+    from chkn.job import read_flow_file
+    from test_oracle_golden import build_step_job
+    jd, steps, _ = build_step_job("ab_reacting", str(tmp_path))
+    assert jd.max_step == steps
+    gold = np.load(os.path.join(HERE, "golden", "step_ab_reacting.npz"))
+    r = subprocess.run([CHKN_RUN + "-ab", "--job=" + str(tmp_path), "--binary"], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0 and "Error" not in r.stderr
+    last = max(int(line.split()[0]) for line in open(tmp_path / "times.data") if line[0] != "#")
+    for b, blk in enumerate(jd.blocks):
+        path = tmp_path / "flow" / ("t%04d" % last) / ("flow-%04d-%04d-%04d.bin" % (blk.i, blk.j, blk.k))
+        io = read_flow_file(str(path), blk.ncells)
+        assert np.array_equal(io.view(np.uint64), np.ascontiguousarray(gold["io%d" % b]).view(np.uint64))
