@@ -13,6 +13,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("CHKN_LIB") or os.path.join(_HERE, "libchkn.so")
 # The reacting A->B gas flavour (the reference's AB_REACTING_GAS build).
 LIB_AB_PATH = os.path.join(_HERE, "libchkn_ab.so")
+# FP32 flavours (the reference's FLOAT_NUMBERS builds): every state, geometry and time value
+# is a float; the C-ABI still takes and returns doubles (exactly widened on output).
+LIB_F32_PATH = os.path.join(_HERE, "libchkn_f32.so")
+LIB_AB_F32_PATH = os.path.join(_HERE, "libchkn_ab_f32.so")
+LIB_PATHS = {"air": LIB_PATH, "ab": LIB_AB_PATH, "air_f32": LIB_F32_PATH, "ab_f32": LIB_AB_F32_PATH}
 
 _I = ctypes.POINTER(ctypes.c_int)
 _D = ctypes.POINTER(ctypes.c_double)
@@ -80,6 +85,7 @@ EXPORTS = {
     "chkn_block_dims": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I]),
     "chkn_active_cells": (ctypes.c_longlong, [ctypes.c_void_p]),
     "chkn_last_timing": (ctypes.c_int, [ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_longlong)]),
+    "chkn_real_size": (ctypes.c_int, []),
     "chkn_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "chkn_destroy": (None, [ctypes.c_void_p]),
 }
@@ -90,10 +96,12 @@ _libs = {}
 
 
 def load(flavour="air"):
-    """Load libchkn.so (flavour "air") or libchkn_ab.so ("ab", the reacting A->B gas) and
-    declare its C-ABI; raises if it has not been built."""
+    """Load libchkn.so (flavour "air"), libchkn_ab.so ("ab", the reacting A->B gas) or their
+    FP32 builds ("air_f32", "ab_f32") and declare the C-ABI; raises if it has not been built."""
+    if flavour not in LIB_PATHS:
+        raise ValueError("unknown library flavour %r" % (flavour,))
     if flavour not in _libs:
-        path = LIB_AB_PATH if flavour == "ab" else LIB_PATH
+        path = LIB_PATHS[flavour]
         if not os.path.exists(path):
             raise RuntimeError("%s is not built (expected at %s); run __graft_entry__.build() "
                                "or make -C gdtk-rmhd_amd" % (os.path.basename(path), path))
@@ -177,7 +185,9 @@ class Sim:
     """One simulation handle (chkn_sim): a single job (``job`` or ``job_dir``) or an ensemble
     of independent jobs stepped together (``jobs``)."""
 
-    def __init__(self, job=None, job_dir=None, tindx=0, binary=True, jobs=None):
+    def __init__(self, job=None, job_dir=None, tindx=0, binary=True, jobs=None, fp32=False):
+        """fp32: run the FP32 build (as the reference's chkn-run-fp32); its job files must be
+        gzipped text (binary=False), as the reference's FP32 build reads no binary files."""
         from .job import gas_flavour
         if job_dir is not None:
             import json
@@ -186,7 +196,7 @@ class Sim:
             flavour = gas_flavour([gm.get(k, 0.0) for k in ("gamma", "R", "q", "alpha", "Ti")])
         else:
             flavour = gas_flavour((jobs or [job])[0].gas_model)
-        lib = load(flavour)
+        lib = load(flavour + ("_f32" if fp32 else ""))
         h = ctypes.c_void_p()
         self._keep = []
         if job_dir is not None:

@@ -1,4 +1,5 @@
-// chkn_geometry.h -- cell and face geometry of a structured hexahedral block (FP64).
+// chkn_geometry.h -- cell and face geometry of a structured hexahedral block, in the
+// build's real type (the reference computes it in its number type).
 //
 // Face centroid, unit normal/tangents and area follow quad_properties
 // (vector3.cu:196-245); cell centroid, volume and i/j/k lengths follow
@@ -10,42 +11,46 @@
 
 #include <math.h>
 
+#include "chkn_real.h"
+
 namespace chkn {
 namespace geom {
 
 struct V3 {
-    double x, y, z;
+    real x, y, z;
 };
 
 inline V3 normalized(V3 v)  // vector3.cu:77-93
 {
-    double m = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
-    if (m > 0.0) { v.x /= m; v.y /= m; v.z /= m; }
-    else { v.x = v.y = v.z = 0.0; }
-    if (fabs(v.x) < 1.0e-30) v.x = 0.0;
-    if (fabs(v.y) < 1.0e-30) v.y = 0.0;
-    if (fabs(v.z) < 1.0e-30) v.z = 0.0;
+    real m = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    if (m > real(0)) { v.x /= m; v.y /= m; v.z /= m; }
+    else { v.x = v.y = v.z = real(0); }
+    const real small = real(1.0e-30);
+    if (fabs(v.x) < small) v.x = real(0);
+    if (fabs(v.y) < small) v.y = real(0);
+    if (fabs(v.z) < small) v.z = real(0);
     return v;
 }
 
 struct Quad {
     V3 c, n, t1, t2;
-    double area;
+    real area;
 };
 
 inline Quad quad(const V3& p0, const V3& p1, const V3& p2, const V3& p3)
 {
     Quad q;
-    q.c.x = 0.25 * (p0.x + p1.x + p2.x + p3.x);
-    q.c.y = 0.25 * (p0.y + p1.y + p2.y + p3.y);
-    q.c.z = 0.25 * (p0.z + p1.z + p2.z + p3.z);
+    const real quarter = real(0.25);
+    q.c.x = quarter * (p0.x + p1.x + p2.x + p3.x);
+    q.c.y = quarter * (p0.y + p1.y + p2.y + p3.y);
+    q.c.z = quarter * (p0.z + p1.z + p2.z + p3.z);
     // Edge-pair sums along the two quad directions.
-    double ax = p1.x - p0.x + p2.x - p3.x, ay = p1.y - p0.y + p2.y - p3.y, az = p1.z - p0.z + p2.z - p3.z;
-    double bx = p3.x - p0.x + p2.x - p1.x, by = p3.y - p0.y + p2.y - p1.y, bz = p3.z - p0.z + p2.z - p1.z;
-    double vx = ay * bz - az * by, vy = az * bx - ax * bz, vz = ax * by - ay * bx;
-    vx *= 0.25; vy *= 0.25; vz *= 0.25;
+    real ax = p1.x - p0.x + p2.x - p3.x, ay = p1.y - p0.y + p2.y - p3.y, az = p1.z - p0.z + p2.z - p3.z;
+    real bx = p3.x - p0.x + p2.x - p1.x, by = p3.y - p0.y + p2.y - p1.y, bz = p3.z - p0.z + p2.z - p1.z;
+    real vx = ay * bz - az * by, vy = az * bx - ax * bz, vz = ax * by - ay * bx;
+    vx *= quarter; vy *= quarter; vz *= quarter;
     q.area = sqrt(vx * vx + vy * vy + vz * vz);
-    if (q.area > 1.0e-20) {
+    if (q.area > real(1.0e-20)) {
         q.n = V3{vx / q.area, vy / q.area, vz / q.area};
         q.t1 = normalized(V3{ax, ay, az});
         // t2 = n x t1, written as cross(v3, v1, v2) of vector3.cu:150-157 with v1 = n, v2 = t1.
@@ -53,28 +58,29 @@ inline Quad quad(const V3& p0, const V3& p1, const V3& p2, const V3& p3)
              q.n.x * q.t1.y - q.t1.x * q.n.y};
         q.t2 = normalized(c);
     } else {
-        q.n = V3{1.0, 0.0, 0.0};
-        q.t1 = V3{0.0, 1.0, 0.0};
-        q.t2 = V3{0.0, 0.0, 1.0};
+        q.n = V3{real(1), real(0), real(0)};
+        q.t1 = V3{real(0), real(1), real(0)};
+        q.t2 = V3{real(0), real(0), real(1)};
     }
     return q;
 }
 
-inline double tet_vol(const V3& p0, const V3& p1, const V3& p2, const V3& p3)  // vector3.cu:259-270
+inline real tet_vol(const V3& p0, const V3& p1, const V3& p2, const V3& p3)  // vector3.cu:259-270
 {
-    double ax = p1.x - p0.x, ay = p1.y - p0.y, az = p1.z - p0.z;
-    double bx = p2.x - p0.x, by = p2.y - p0.y, bz = p2.z - p0.z;
-    double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-    double dx = p3.x - p0.x, dy = p3.y - p0.y, dz = p3.z - p0.z;
-    return (dx * cx + dy * cy + dz * cz) / 6.0;
+    real ax = p1.x - p0.x, ay = p1.y - p0.y, az = p1.z - p0.z;
+    real bx = p2.x - p0.x, by = p2.y - p0.y, bz = p2.z - p0.z;
+    real cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+    real dx = p3.x - p0.x, dy = p3.y - p0.y, dz = p3.z - p0.z;
+    return (dx * cx + dy * cy + dz * cz) / real(6);
 }
 
 // Volume of the dipyramid over base (b0, b1, b2, b3) with apex `apex` (vector3.cu:309-348).
-inline double pyramid_vol(const V3& b0, const V3& b1, const V3& b2, const V3& b3, const V3& apex)
+inline real pyramid_vol(const V3& b0, const V3& b1, const V3& b2, const V3& b3, const V3& apex)
 {
-    V3 m{0.25 * (b0.x + b1.x + b2.x + b3.x), 0.25 * (b0.y + b1.y + b2.y + b3.y),
-         0.25 * (b0.z + b1.z + b2.z + b3.z)};
-    double v = 0.0;
+    const real quarter = real(0.25);
+    V3 m{quarter * (b0.x + b1.x + b2.x + b3.x), quarter * (b0.y + b1.y + b2.y + b3.y),
+         quarter * (b0.z + b1.z + b2.z + b3.z)};
+    real v = real(0);
     v += tet_vol(b0, b1, m, apex);
     v += tet_vol(b1, b2, m, apex);
     v += tet_vol(b2, b3, m, apex);
@@ -84,18 +90,19 @@ inline double pyramid_vol(const V3& b0, const V3& b1, const V3& b2, const V3& b3
 
 struct Hex {
     V3 c;
-    double vol, iL, jL, kL;
+    real vol, iL, jL, kL;
 };
 
 inline V3 mid4(const V3& a, const V3& b, const V3& c, const V3& d)
 {
-    return V3{0.25 * (a.x + b.x + c.x + d.x), 0.25 * (a.y + b.y + c.y + d.y),
-              0.25 * (a.z + b.z + c.z + d.z)};
+    const real quarter = real(0.25);
+    return V3{quarter * (a.x + b.x + c.x + d.x), quarter * (a.y + b.y + c.y + d.y),
+              quarter * (a.z + b.z + c.z + d.z)};
 }
 
-inline double dist(const V3& a, const V3& b)
+inline real dist(const V3& a, const V3& b)
 {
-    double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    real dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
     return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
@@ -103,9 +110,10 @@ inline double dist(const V3& a, const V3& b)
 inline Hex hex(const V3 p[8])
 {
     Hex h;
-    h.c.x = 0.125 * (p[0].x + p[1].x + p[2].x + p[3].x + p[4].x + p[5].x + p[6].x + p[7].x);
-    h.c.y = 0.125 * (p[0].y + p[1].y + p[2].y + p[3].y + p[4].y + p[5].y + p[6].y + p[7].y);
-    h.c.z = 0.125 * (p[0].z + p[1].z + p[2].z + p[3].z + p[4].z + p[5].z + p[6].z + p[7].z);
+    const real eighth = real(0.125);
+    h.c.x = eighth * (p[0].x + p[1].x + p[2].x + p[3].x + p[4].x + p[5].x + p[6].x + p[7].x);
+    h.c.y = eighth * (p[0].y + p[1].y + p[2].y + p[3].y + p[4].y + p[5].y + p[6].y + p[7].y);
+    h.c.z = eighth * (p[0].z + p[1].z + p[2].z + p[3].z + p[4].z + p[5].z + p[6].z + p[7].z);
     V3 north = mid4(p[3], p[2], p[6], p[7]);
     V3 east = mid4(p[1], p[2], p[6], p[5]);
     V3 south = mid4(p[0], p[1], p[5], p[4]);
@@ -115,14 +123,14 @@ inline Hex hex(const V3 p[8])
     h.iL = dist(east, west);
     h.jL = dist(north, south);
     h.kL = dist(top, bottom);
-    double v = 0.0;
+    real v = real(0);
     v += pyramid_vol(p[6], p[7], p[3], p[2], h.c);
     v += pyramid_vol(p[5], p[6], p[2], p[1], h.c);
     v += pyramid_vol(p[4], p[5], p[1], p[0], h.c);
     v += pyramid_vol(p[7], p[4], p[0], p[3], h.c);
     v += pyramid_vol(p[7], p[6], p[5], p[4], h.c);
     v += pyramid_vol(p[0], p[1], p[2], p[3], h.c);
-    if ((v < 0.0 && fabs(v) < 1.0e-12) || (v >= 0.0 && v < 1.0e-20)) v = 0.0;
+    if ((v < real(0) && fabs(v) < real(1.0e-12)) || (v >= real(0) && v < real(1.0e-20))) v = real(0);
     h.vol = v;
     return h;
 }

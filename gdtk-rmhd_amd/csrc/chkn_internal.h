@@ -1,7 +1,8 @@
 // chkn_internal.h -- device data layout and kernel launch interface shared by the host
 // runtime (chkn_sim.cpp) and the gfx950 kernels (chkn_kernels.hip).
 //
-// HBM layout (per active block of n = nic*njc*nkc cells, structure of arrays, FP64):
+// HBM layout (per active block of n = nic*njc*nkc cells, structure of arrays, `real`:
+// FP64, or FP32 in the FP32 flavour):
 //   W[slots][6][n] primitive stage states (rho, e, YB, vx, vy, vz): a 3-slot ring while
 //                stepping, or the 3C+1 stage states of a C-step segment while recording a
 //                tape for the adjoint; slot r starts at W + r*wstride
@@ -26,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "chkn_real.h"
+
 namespace chkn {
 
 constexpr int kNQ = 6;      // conserved quantities, flow.cu:17-31
@@ -38,15 +41,15 @@ struct BlockDev {
     int bc_arg[6];   // inflow flow-state index, or neighbour block id for exchange
     long long n;     // cells
     long long nfi, nfj, nfk;
-    double* W;             // slot 0 of the primitive stage states
-    long long wstride;     // doubles between slots: 6n (active) or 0 (inactive)
-    double* U;
-    double* L;
-    const double* gi;
-    const double* gj;
-    const double* gk;
-    const double* volinv;
-    const double* len;
+    real* W;               // slot 0 of the primitive stage states
+    long long wstride;     // values between slots: 6n (active) or 0 (inactive)
+    real* U;
+    real* L;
+    const real* gi;
+    const real* gj;
+    const real* gk;
+    const real* volinv;
+    const real* len;
     // adjoint
     int rp[6];             // block whose opposite side exchanges into this side, or -1
     int icol;              // first inflow line of this block in the inflow accumulator
@@ -54,17 +57,17 @@ struct BlockDev {
     double* G;             // face stencil adjoints: [24][nfi], then [24][nfj], then [24][nfk]
     double* adj;           // A0 (adjoint of U at the step boundary), P, D: [3][6][n]
     // face-list stage
-    double* F;
-    double* FS;
-    const double* fpos;
-    const double* cpos;
-    const double* gas;     // null when the block's file state is not needed
-    double twall[6];       // TWall of each wall_no_slip_fixed_T side
+    real* F;
+    real* FS;
+    const real* fpos;
+    const real* cpos;
+    const real* gas;       // null when the block's file state is not needed
+    real twall[6];         // TWall of each wall_no_slip_fixed_T side
     // Per side, or null: inflow_function sides hold the ghost states of depth 0 and 1
     // ([2][9][nt]: rho, e, YB, vx, vy, vz, p, T, a) then the face state ([4][nt]: T, vx, vy,
     // vz); fixed-T walls with TWall per face hold [nt] values.  nt = faces of the side,
     // index a + na*c as in the ghost lines.
-    const double* side[6];
+    const real* side[6];
 };
 
 struct Tile {
@@ -77,11 +80,11 @@ struct Tile {
 // One slot per instance for dt, run flag and signal-time minimum; one bad-cell counter per
 // stage.
 struct StepCtl {
-    const double* dt;                 // [ninst] time step of each instance
+    const real* dt;                   // [ninst] time step of each instance
     const int* run;                   // [ninst] 1 if the instance takes this step
     unsigned long long* sigmin_bits;  // [ninst] min over cells of the cfl-free signal time
     int* bad;                         // [3] bad cells per stage
-    const double* chem;               // [ninst] exp(-alpha dt) of reacting instances, else -1
+    const real* chem;                 // [ninst] exp(-alpha dt) of reacting instances, else -1
 };
 
 struct StageArgs {
@@ -89,7 +92,7 @@ struct StageArgs {
     const Tile* tiles;
     int ntiles;
     int tile_base;         // offset into the tile table (per-block launches)
-    const double* inflow;  // 6 doubles per flow state: rho, e, YB, vx, vy, vz
+    const real* inflow;  // 6 values per flow state: rho, e, YB, vx, vy, vz
     StepCtl ctl;
     int win, wout;         // W slots read / written by this stage
 };
@@ -147,8 +150,8 @@ hipError_t launch_flux_vjp_points(const double* in, double* out, int n, int x_or
 // configured p, T, a (inflow_gas, 3 per flow state).
 struct SplitArgs {
     const BlockDev* blocks;
-    const double* inflow;
-    const double* inflow_gas;
+    const real* inflow;
+    const real* inflow_gas;
     const FaceChunk* faces;  // at most 256 faces each
     const FacePatch* patches;  // the viscous pass's work items
     const CellChunk* cells;  // at most 256 cells each

@@ -59,9 +59,19 @@ int bc_code(const std::string& name)
 
 bool gas_model_matches(const double* m)
 {
-    auto close = [](double a, double b) { return std::fabs(a - b) < 1.0e-4; };  // number.cu:48-50
+    // number.cu:48-50 on the build's constants and the file's values read as number
+    // (config.cu:369-377).
+    auto close = [](real a, double b) { return std::fabs((double)a - (double)(real)b) < 1.0e-4; };
     return close(kGamma, m[0]) && close(kR, m[1]) && close(kQ, m[2]) && close(kAlpha, m[3]) &&
            close(kTi, m[4]) && close(kCv, m[5]);
+}
+
+// One number of a text file as the reference reads it into its number type (`>> number`,
+// "%lf" / "%f"): strtod, or strtof in the FP32 flavour; widened to double (exactly).
+double parse_real(const char* p, char** end)
+{
+    if (sizeof(real) == sizeof(float)) return (double)std::strtof(p, end);
+    return std::strtod(p, end);
 }
 
 namespace {
@@ -181,6 +191,8 @@ std::vector<double> read_grid(const std::string& path, bool binary, int nic, int
     int niv, njv, nkv;
     std::vector<double> v;
     if (binary) {  // block.cu:793-827: [dims, 0, 0, niv, njv, nkv] then xyz per vertex
+        if (sizeof(real) != sizeof(double))  // block.cu:794-796
+            throw std::runtime_error("Do not ask to FP32 code to read grid file with binary data.");
         std::string raw = slurp(path);
         if (raw.size() < 6 * sizeof(double)) throw std::runtime_error("Did not open binary grid file successfully: " + path);
         const double* d = reinterpret_cast<const double*>(raw.data());
@@ -206,9 +218,16 @@ std::vector<double> read_grid(const std::string& path, bool binary, int nic, int
         v.resize(3 * nv);
         for (size_t q = 0; q < nv; ++q) {
             std::getline(in, line);
-            double x = 0, y = 0, z = 0;
-            std::sscanf(line.c_str(), "%lf %lf %lf", &x, &y, &z);
-            v[3 * q] = x; v[3 * q + 1] = y; v[3 * q + 2] = z;
+            const char* p = line.c_str();
+            char* end = nullptr;
+            double xyz[3] = {0.0, 0.0, 0.0};  // block.cu:866-871 (sscanf %lf, or %f for FP32)
+            for (int a = 0; a < 3; ++a) {
+                const double val = parse_real(p, &end);
+                if (end == p) break;
+                xyz[a] = val;
+                p = end;
+            }
+            v[3 * q] = xyz[0]; v[3 * q + 1] = xyz[1]; v[3 * q + 2] = xyz[2];
         }
     }
     return v;
@@ -218,6 +237,8 @@ std::vector<double> read_flow(const std::string& path, bool binary, int ncells)
 {
     std::vector<double> v((size_t)13 * ncells);
     if (binary) {  // block.cu:886-907
+        if (sizeof(real) != sizeof(double))  // block.cu:887-889
+            throw std::runtime_error("Do not ask to FP32 code to read flow file with binary data.");
         std::string raw = slurp(path);
         if (raw.size() < v.size() * sizeof(double)) throw std::runtime_error("Did not open binary flow file successfully: " + path);
         std::memcpy(v.data(), raw.data(), v.size() * sizeof(double));
@@ -226,7 +247,7 @@ std::vector<double> read_flow(const std::string& path, bool binary, int ncells)
         const char* p = text.c_str();
         for (size_t q = 0; q < v.size(); ++q) {
             char* end = nullptr;
-            v[q] = std::strtod(p, &end);
+            v[q] = parse_real(p, &end);
             if (end == p) throw std::runtime_error("Flow file too short: " + path);
             p = end;
         }
@@ -234,13 +255,13 @@ std::vector<double> read_flow(const std::string& path, bool binary, int ncells)
     return v;
 }
 
-void write_flow(const std::string& path, bool binary, const double* iovars, int ncells)
+void write_flow(const std::string& path, bool binary, const real* iovars, int ncells)
 {
     size_t n = (size_t)13 * ncells;
-    if (binary) {  // block.cu:938-955
+    if (binary) {  // block.cu:938-955: the number type's bytes (4 in the FP32 flavour)
         std::ofstream f(path, std::ios::binary);
         if (!f) throw std::runtime_error("Did not open binary flow file successfully for writing: " + path);
-        f.write(reinterpret_cast<const char*>(iovars), (std::streamsize)(n * sizeof(double)));
+        f.write(reinterpret_cast<const char*>(iovars), (std::streamsize)(n * sizeof(real)));
     } else {  // block.cu:956-974: ostream default formatting, i.e. %g with 6 digits
         gzFile g = gzopen(path.c_str(), "wb");
         if (!g) throw std::runtime_error("Did not open gzipped flow file successfully for writing: " + path);
@@ -248,7 +269,7 @@ void write_flow(const std::string& path, bool binary, const double* iovars, int 
         buf.reserve(1 << 20);
         char tmp[64];
         for (size_t q = 0; q < n; ++q) {
-            int len = std::snprintf(tmp, sizeof tmp, "%g\n", iovars[q]);
+            int len = std::snprintf(tmp, sizeof tmp, "%g\n", (double)iovars[q]);
             buf.append(tmp, (size_t)len);
             if (buf.size() > (1 << 20)) { gzwrite(g, buf.data(), (unsigned)buf.size()); buf.clear(); }
         }
@@ -259,7 +280,7 @@ void write_flow(const std::string& path, bool binary, const double* iovars, int 
 
 void read_times(const std::string& job, int tindx, double& t, int& next_plot_indx)
 {
-    t = 0.0;
+    t = 0.0;  // simulate.cu:132-149 reads the time as number
     next_plot_indx = 1;
     std::ifstream f(job + "/times.data");
     if (!f.good()) return;
@@ -268,18 +289,18 @@ void read_times(const std::string& job, int tindx, double& t, int& next_plot_ind
         if (line.empty() || line.find('#') != std::string::npos) continue;
         std::istringstream ss(line);
         int ti;
-        double tm;
+        real tm;
         if (ss >> ti >> tm) {
-            if (ti == tindx) { t = tm; next_plot_indx = ti + 1; }
+            if (ti == tindx) { t = (double)tm; next_plot_indx = ti + 1; }
         }
     }
 }
 
-void append_times(const std::string& job, int tindx, double t)
+void append_times(const std::string& job, int tindx, real t)
 {
     std::ofstream f(job + "/times.data", std::ios::app);
     char buf[64];
-    std::snprintf(buf, sizeof buf, "%d %g\n", tindx, t);
+    std::snprintf(buf, sizeof buf, "%d %g\n", tindx, (double)t);
     f << buf;
 }
 
@@ -289,15 +310,15 @@ std::vector<double> read_bl_profile(const std::string& path)
     if (!std::filesystem::is_regular_file(path, ec))
         throw std::runtime_error("Boundary-layer file is not present in current directory.");
     std::istringstream in(slurp(path));
-    double p_e = 0.0;
+    real p_e = 0;  // config.cu:37-60 reads number values
     int nsample = 0;
     in >> p_e >> nsample;
     if (nsample != 12) throw std::runtime_error("Boundary-layer file has unexpected number of sample points.");
-    std::vector<double> out(1, p_e);
+    std::vector<double> out(1, (double)p_e);
     for (int q = 0; q < 3 * nsample; ++q) {
-        double v;
+        real v;
         if (!(in >> v)) throw std::runtime_error("Boundary-layer file is too short: " + path);
-        out.push_back(v);
+        out.push_back((double)v);
     }
     return out;
 }
@@ -317,7 +338,7 @@ std::vector<double> read_twall(const std::string& job, const JobBlock& b, int si
     const char* p = text.c_str();
     for (int q = 0; q < nfaces; ++q) {
         char* end = nullptr;
-        v[q] = std::strtod(p, &end);
+        v[q] = parse_real(p, &end);
         if (end == p) throw std::runtime_error("TWall file too short: " + path);
         p = end;
     }

@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "chkn_real.h"
+
 namespace chkn {
 
 struct JobBlock {
@@ -40,16 +42,20 @@ struct JobConfig {
 // config.cu:369-386 checks it.
 bool gas_model_matches(const double* model);
 
-// Throw std::runtime_error with the reference's message text on failure.
+// Throw std::runtime_error with the reference's message text on failure.  Numbers are
+// returned as double holding the value the reference reads into its number type: text is
+// parsed as real (float in the FP32 flavour, whose binary readers refuse, block.cu:794-796,
+// :887-889), configuration values are rounded to real where they are used.
 JobConfig read_config_json(const std::string& path);
 // Vertices (nic+1)(njc+1)(nkc+1) xyz triples, i fastest.
 std::vector<double> read_grid(const std::string& path, bool binary, int nic, int njc, int nkc);
 // 13*ncells IO variables, variable-major.
 std::vector<double> read_flow(const std::string& path, bool binary, int ncells);
-void write_flow(const std::string& path, bool binary, const double* iovars, int ncells);
+// Binary files hold the real type's bytes; text is %g (6 significant digits).
+void write_flow(const std::string& path, bool binary, const real* iovars, int ncells);
 // Start time and next plot index for time index tindx from times.data (t = 0 if absent).
 void read_times(const std::string& job, int tindx, double& t, int& next_plot_indx);
-void append_times(const std::string& job, int tindx, double t);
+void append_times(const std::string& job, int tindx, real t);
 // Laminar boundary-layer profile file (config.cu:37-60): p_e, then y, T, velx of 12 samples.
 std::vector<double> read_bl_profile(const std::string& path);
 // TWall of every face of a fixed-T side given as a function: <job>/TWall-i-j-k-<face>.gz

@@ -23,7 +23,7 @@ namespace chkn {
 namespace {
 
 __device__ __forceinline__ Prim stencil_prim(const StageArgs& A, const BlockDev* B,
-                                             const double* __restrict__ W, int n_line, int base,
+                                             const real* __restrict__ W, int n_line, int base,
                                              int stride, int dir, int m, int a, int c)
 {
     if (m >= 0 && m < n_line) return ld_prim(W, (int)B->n, base + m * stride);
@@ -48,8 +48,8 @@ struct TileLds {
 // tile can reference it.
 template <int TI, int TJ>
 __device__ __forceinline__ void fill_plane_position(const StageArgs& A, const BlockDev* B,
-                                                    const double* __restrict__ Win, int i0,
-                                                    int j0, int k, int p, double* Wp)
+                                                    const real* __restrict__ Win, int i0,
+                                                    int j0, int k, int p, real* Wp)
 {
     using L = TileLds<TI, TJ>;
     int ii, jj;
@@ -84,7 +84,7 @@ __device__ __forceinline__ void fill_plane_position(const StageArgs& A, const Bl
     Wp[5 * L::NP + o] = w.vz;
 }
 
-__device__ __forceinline__ Prim lds_prim(const double* Wp, int np, int o)
+__device__ __forceinline__ Prim lds_prim(const real* Wp, int np, int o)
 {
     Prim p;
     p.rho = Wp[o];
@@ -99,8 +99,8 @@ __device__ __forceinline__ Prim lds_prim(const double* Wp, int np, int o)
 // Area-weighted flux of the k-face (i, j, kf) between cells kf-1 and kf.
 template <int XORDER>
 __device__ __forceinline__ void k_face_af(const StageArgs& A, const BlockDev* B,
-                                          const double* __restrict__ Win, int i, int j, int kf,
-                                          double AF[6])
+                                          const real* __restrict__ Win, int i, int j, int kf,
+                                          real AF[6])
 {
     const int nic = B->nic, njc = B->njc, nkc = B->nkc;
     const int base = i + nic * j, stride = nic * njc;
@@ -110,37 +110,37 @@ __device__ __forceinline__ void k_face_af(const StageArgs& A, const BlockDev* B,
     const Prim s3 = stencil_prim(A, B, Win, nkc, base, stride, 2, kf + 1, i, j);
     const int fi = i + nic * (j + njc * kf);
     const Frame fr = ld_frame(B->gk, (int)B->nfk, fi);
-    const double area = B->gk[fi];
-    double F[6];
+    const real area = B->gk[fi];
+    real F[6];
     face_flux<XORDER>(s0, s1, s2, s3, fr, F);
 #pragma unroll
     for (int q = 0; q < 6; ++q) AF[q] = area * F[q];
 }
 
 template <int STAGE, int TI, int TJ, int XORDER, bool DTRED>
-__device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile, double* lds)
+__device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile, real* lds)
 {
     using L = TileLds<TI, TJ>;
     constexpr int NT = TI * TJ;
     constexpr int NFI = L::NFI, NFJ = L::NFJ, NP = L::NP, PW = L::PW;
-    double* Wp = lds;               // [6][TJ+4][TI+4]
-    double* AFi = Wp + 6 * NP;      // [6][TJ][TI+1]
-    double* AFj = AFi + 6 * NFI;    // [6][TJ+1][TI]
+    real* Wp = lds;               // [6][TJ+4][TI+4]
+    real* AFi = Wp + 6 * NP;      // [6][TJ][TI+1]
+    real* AFj = AFi + 6 * NFI;    // [6][TJ+1][TI]
 
     const BlockDev* B = A.blocks + tile.blk;
     const int nic = B->nic, njc = B->njc;
     const int n = (int)B->n;
-    const double* __restrict__ Win = B->W + A.win * B->wstride;
+    const real* __restrict__ Win = B->W + A.win * B->wstride;
     const int t = threadIdx.x;
     const int ti = t % TI, tj = t / TI;
     const int i = tile.i0 + ti, j = tile.j0 + tj;
     const bool own = (i < nic) && (j < njc);
     const int kbeg = tile.k0, kend = tile.k0 + tile.nk;
-    const double dt = A.ctl.dt[B->inst];
+    const real dt = A.ctl.dt[B->inst];
     int bad = 0;
     unsigned long long sigmin = ~0ull;
 
-    double AFkm[6] = {0, 0, 0, 0, 0, 0};
+    real AFkm[6] = {0, 0, 0, 0, 0, 0};
     if (own) k_face_af<XORDER>(A, B, Win, i, j, kbeg, AFkm);
 
     for (int k = kbeg; k < kend; ++k) {
@@ -148,14 +148,14 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
         // plane up is evaluated from global memory meanwhile.
         for (int p = t; p < L::NLOAD; p += NT)
             fill_plane_position<TI, TJ>(A, B, Win, tile.i0, tile.j0, k, p, Wp);
-        double AFkp[6] = {0, 0, 0, 0, 0, 0};
+        real AFkp[6] = {0, 0, 0, 0, 0, 0};
         if (own) k_face_af<XORDER>(A, B, Win, i, j, k + 1, AFkp);
         __syncthreads();
         // i- and j-faces of the plane, one flattened list, stencils read from LDS.
         for (int f = t; f < NFI + NFJ; f += NT) {
             int base, stride, fa, fb, fi, nf;
-            const double* g;
-            double* dst;
+            const real* g;
+            real* dst;
             int lstride;
             bool valid;
             if (f < NFI) {
@@ -180,12 +180,12 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
             }
             if (valid) {
                 const Frame fr = ld_frame(g, nf, fi);
-                const double area = g[fi];
+                const real area = g[fi];
                 const Prim s0 = lds_prim(Wp, NP, base);
                 const Prim s1 = lds_prim(Wp, NP, base + stride);
                 const Prim s2 = lds_prim(Wp, NP, base + 2 * stride);
                 const Prim s3 = lds_prim(Wp, NP, base + 3 * stride);
-                double F[6];
+                real F[6];
                 face_flux<XORDER>(s0, s1, s2, s3, fr, F);
 #pragma unroll
                 for (int q = 0; q < 6; ++q) dst[q * lstride] = area * F[q];
@@ -194,11 +194,11 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
         __syncthreads();
         if (own) {
             const int cidx = i + nic * (j + njc * k);
-            const double vinv = B->volinv[cidx];
-            double d[6];
+            const real vinv = B->volinv[cidx];
+            real d[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                double s = AFi[q * NFI + tj * (TI + 1) + ti] - AFi[q * NFI + tj * (TI + 1) + ti + 1];
+                real s = AFi[q * NFI + tj * (TI + 1) + ti] - AFi[q * NFI + tj * (TI + 1) + ti + 1];
                 s = s + AFj[q * NFJ + tj * TI + ti];
                 s = s - AFj[q * NFJ + (tj + 1) * TI + ti];
                 s = s + AFkm[q];
@@ -232,7 +232,7 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
 template <int STAGE, int TI, int TJ, int XORDER, bool DTRED>
 __global__ __launch_bounds__(TI* TJ) void stage_kernel(StageArgs A, int persistent)
 {
-    __shared__ double lds[TileLds<TI, TJ>::DOUBLES];
+    __shared__ real lds[TileLds<TI, TJ>::DOUBLES];
     // XCD-aware tile mapping: work-group b runs on XCD b % 8, and each XCD takes a
     // contiguous range of the (spatially ordered) tile table, so halo cells shared by
     // neighbouring tiles are re-read from the same L2.
@@ -304,12 +304,12 @@ __global__ __launch_bounds__(256) void decode_all_kernel(const BlockDev* blocks,
     const int n = (int)B->n;
     const int cidx = ch.c0 + (int)threadIdx.x;
     if (cidx >= ch.c1) return;
-    double U[6];
+    real U[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) U[q] = B->U[(long long)q * n + cidx];
     Prim w{0, 0, 0, 0, 0, 0};
     decode(U, w);
-    double* W = B->W;
+    real* W = B->W;
     W[cidx] = w.rho;
     W[n + cidx] = w.e;
     W[2 * n + cidx] = w.YB;

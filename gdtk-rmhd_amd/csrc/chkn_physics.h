@@ -1,62 +1,68 @@
-// chkn_physics.h -- pointwise FP64 physics of the chicken step, for gfx950 device code and
-// the host geometry pre-pass.
+// chkn_physics.h -- pointwise physics of the chicken step, for gfx950 device code and the
+// host geometry pre-pass, in the build's `real` (double, or float for the FP32 flavour).
 //
 // Every expression keeps the operand order and association of the reference so that, with
 // floating-point contraction disabled (-ffp-contract=off), the GPU reproduces the
 // reference CPU build bit for bit.  References are to src/chicken/ in the reference tree.
+// Literals are written real(...) where the reference uses its `number` constants; where
+// the reference's expression mixes in a double literal, that part is evaluated in double
+// as the reference's FP32 build evaluates it.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include "chkn_real.h"
 
 #define CHKN_HD __host__ __device__ __forceinline__
 
 namespace chkn {
 
 // Gas model, chosen at build time as the reference chooses it (gas.cu:20-48): ideal air, or
-// with CHKN_AB_REACTING_GAS the reacting A->B gas (libchkn_ab.so).
+// with CHKN_AB_REACTING_GAS the reacting A->B gas (libchkn_ab.so).  The constants are the
+// double literals of gas.cu:34-40 rounded to real; Cv divides in double (g-1.0).
 #ifdef CHKN_AB_REACTING_GAS
-constexpr double kGamma = 6.0 / 5.0;
-constexpr double kR = 287.0;
-constexpr double kQ = 300000.0;
-constexpr double kAlpha = 1000.0;  // reaction rate, 1/s
-constexpr double kTi = 362.58;     // ignition temperature, K
+constexpr real kGamma = real(6.0 / 5.0);
+constexpr real kR = real(287.0);
+constexpr real kQ = real(300000.0);
+constexpr real kAlpha = real(1000.0);  // reaction rate, 1/s
+constexpr real kTi = real(362.58);     // ignition temperature, K
 #else
-constexpr double kGamma = 1.4;
-constexpr double kR = 287.1;
-constexpr double kQ = 0.0;
-constexpr double kAlpha = 0.0;
-constexpr double kTi = 0.0;
+constexpr real kGamma = real(1.4);
+constexpr real kR = real(287.1);
+constexpr real kQ = real(0.0);
+constexpr real kAlpha = real(0.0);
+constexpr real kTi = real(0.0);
 #endif
-constexpr double kCv = kR / (kGamma - 1.0);
-constexpr double kOneSixth = 1.0 / 6.0;   // number.cu:34
-constexpr double kFourSixth = 4.0 / 6.0;  // number.cu:35
+constexpr real kCv = real(double(kR) / (double(kGamma) - 1.0));
+constexpr real kOneSixth = real(1) / real(6);   // number.cu:34 (1.0f/6.0f in the FP32 build)
+constexpr real kFourSixth = real(4) / real(6);  // number.cu:35
 
 // Primitive state carried between stages: the six values the reconstruction reads
 // (face.cu:359-368).  p, T and a follow from rho and e through the equation of state.
 struct Prim {
-    double rho, e, YB, vx, vy, vz;
+    real rho, e, YB, vx, vy, vz;
 };
 
 struct Frame {
-    double nx, ny, nz, t1x, t1y, t1z, t2x, t2y, t2z;
+    real nx, ny, nz, t1x, t1y, t1z, t2x, t2y, t2z;
 };
 
 // gas.cu:128-135 (update_from_rhoe): T, then p = rho*R*T, a = sqrt(g*R*T).
-CHKN_HD double eos_T(double e, double YB) { return (e + YB * kQ) / kCv; }
-CHKN_HD double eos_p(double rho, double T) { return rho * kR * T; }
-CHKN_HD double eos_a(double T) { return sqrt(kGamma * kR * T); }
+CHKN_HD real eos_T(real e, real YB) { return (e + YB * kQ) / kCv; }
+CHKN_HD real eos_p(real rho, real T) { return rho * kR * T; }
+CHKN_HD real eos_a(real T) { return sqrt(kGamma * kR * T); }
 
 // vector3.cu:97-109 (global -> local) and :111-119 (local -> global).
-CHKN_HD void to_local(const Frame& f, double x, double y, double z, double& u, double& v,
-                      double& w)
+CHKN_HD void to_local(const Frame& f, real x, real y, real z, real& u, real& v,
+                      real& w)
 {
     u = x * f.nx + y * f.ny + z * f.nz;
     v = x * f.t1x + y * f.t1y + z * f.t1z;
     w = x * f.t2x + y * f.t2y + z * f.t2z;
 }
-CHKN_HD void to_global(const Frame& f, double u, double v, double w, double& x, double& y,
-                       double& z)
+CHKN_HD void to_global(const Frame& f, real u, real v, real w, real& x, real& y,
+                       real& z)
 {
     x = u * f.nx + v * f.t1x + w * f.t2x;
     y = u * f.ny + v * f.t1y + w * f.t2y;
@@ -67,7 +73,7 @@ CHKN_HD void to_global(const Frame& f, double u, double v, double w, double& x, 
 // frame (bcs.cu:26-37).
 CHKN_HD Prim reflect(const Prim& s, const Frame& f)
 {
-    double u, v, w;
+    real u, v, w;
     to_local(f, s.vx, s.vy, s.vz, u, v, w);
     u = -(u);
     Prim r = s;
@@ -76,76 +82,76 @@ CHKN_HD Prim reflect(const Prim& s, const Frame& f)
 }
 
 // face.cu:72-79
-CHKN_HD double van_albada(double a, double b)
+CHKN_HD real van_albada(real a, real b)
 {
-    const double eps = 1.0e-12;
+    const real eps = real(1.0e-12);
     return (a * b + fabs(a * b) + eps) / (a * a + b * b + eps);
 }
 
 // face.cu:81-96
-CHKN_HD void muscl(double qL1, double qL0, double qR0, double qR1, double& qL, double& qR)
+CHKN_HD void muscl(real qL1, real qL0, real qR0, real qR1, real& qL, real& qR)
 {
-    double dm = (qL0 - qL1);
-    double d = (qR0 - qL0);
-    double dp = (qR1 - qR0);
-    double sL = van_albada(dm, d);
-    double sR = van_albada(d, dp);
-    qL = qL0 + sL * 0.125 * (3.0 * d + dm);
-    qR = qR0 - sR * 0.125 * (dp + 3.0 * d);
+    real dm = (qL0 - qL1);
+    real d = (qR0 - qL0);
+    real dp = (qR1 - qR0);
+    real sL = van_albada(dm, d);
+    real sR = van_albada(d, dp);
+    qL = qL0 + sL * real(0.125) * (real(3.0) * d + dm);
+    qR = qR0 - sR * real(0.125) * (dp + real(3.0) * d);
 }
 
 // AUSMDV flux of Wada & Liou (1994) on reconstructed face states, face.cu:163-265.
 // Inputs are the gas states (rho, e, p, a, YB) and global-frame velocities on each side.
-CHKN_HD void ausmdv(double rhoL, double eL, double pL, double aL, double YBL, double vxL,
-                    double vyL, double vzL, double rhoR, double eR, double pR, double aR,
-                    double YBR, double vxR, double vyR, double vzR, const Frame& f,
-                    double F[6])
+CHKN_HD void ausmdv(real rhoL, real eL, real pL, real aL, real YBL, real vxL,
+                    real vyL, real vzL, real rhoR, real eR, real pR, real aR,
+                    real YBR, real vxR, real vyR, real vzR, const Frame& f,
+                    real F[6])
 {
-    double velxL, velyL, velzL, velxR, velyR, velzR;
+    real velxL, velyL, velzL, velxR, velyR, velzR;
     to_local(f, vxL, vyL, vzL, velxL, velyL, velzL);
     to_local(f, vxR, vyR, vzR, velxR, velyR, velzR);
-    double pLrL = pL / rhoL;
-    double keL = 0.5 * (velxL * velxL + velyL * velyL + velzL * velzL);
-    double HL = eL + pLrL + keL;
-    double pRrR = pR / rhoR;
-    double keR = 0.5 * (velxR * velxR + velyR * velyR + velzR * velzR);
-    double HR = eR + pR / rhoR + keR;
-    double alphaL = 2.0 * pLrL / (pLrL + pRrR);
-    double alphaR = 2.0 * pRrR / (pLrL + pRrR);
-    double am = fmax(aL, aR);
-    double ML = velxL / am;
-    double MR = velxR / am;
-    double pLplus, velxLplus;
-    double dvelxL = 0.5 * (velxL + fabs(velxL));
-    if (fabs(ML) <= 1.0) {
-        pLplus = pL * (ML + 1.0) * (ML + 1.0) * (2.0 - ML) * 0.25;
-        velxLplus = alphaL * ((velxL + am) * (velxL + am) / (4.0 * am) - dvelxL) + dvelxL;
+    real pLrL = pL / rhoL;
+    real keL = real(0.5) * (velxL * velxL + velyL * velyL + velzL * velzL);
+    real HL = eL + pLrL + keL;
+    real pRrR = pR / rhoR;
+    real keR = real(0.5) * (velxR * velxR + velyR * velyR + velzR * velzR);
+    real HR = eR + pR / rhoR + keR;
+    real alphaL = real(2.0) * pLrL / (pLrL + pRrR);
+    real alphaR = real(2.0) * pRrR / (pLrL + pRrR);
+    real am = fmax(aL, aR);
+    real ML = velxL / am;
+    real MR = velxR / am;
+    real pLplus, velxLplus;
+    real dvelxL = real(0.5) * (velxL + fabs(velxL));
+    if (fabs(ML) <= real(1.0)) {
+        pLplus = pL * (ML + real(1.0)) * (ML + real(1.0)) * (real(2.0) - ML) * real(0.25);
+        velxLplus = alphaL * ((velxL + am) * (velxL + am) / (real(4.0) * am) - dvelxL) + dvelxL;
     } else {
         pLplus = pL * dvelxL / velxL;
         velxLplus = dvelxL;
     }
-    double pRminus, velxRminus;
-    double dvelxR = 0.5 * (velxR - fabs(velxR));
-    if (fabs(MR) <= 1.0) {
-        pRminus = pR * (MR - 1.0) * (MR - 1.0) * (2.0 + MR) * 0.25;
-        velxRminus = alphaR * (-(velxR - am) * (velxR - am) / (4.0 * am) - dvelxR) + dvelxR;
+    real pRminus, velxRminus;
+    real dvelxR = real(0.5) * (velxR - fabs(velxR));
+    if (fabs(MR) <= real(1.0)) {
+        pRminus = pR * (MR - real(1.0)) * (MR - real(1.0)) * (real(2.0) + MR) * real(0.25);
+        velxRminus = alphaR * (-(velxR - am) * (velxR - am) / (real(4.0) * am) - dvelxR) + dvelxR;
     } else {
         pRminus = pR * dvelxR / velxR;
         velxRminus = dvelxR;
     }
-    double massL = velxLplus * rhoL;
-    double massR = velxRminus * rhoR;
-    double mass_half = massL + massR;
-    double p_half = pLplus + pRminus;
-    double dp = pL - pR;
-    dp = 10.0 * fabs(dp) / fmin(pL, pR);
-    double s = 0.5 * fmin(1.0, dp);
-    double rv2_V = massL * velxL + massR * velxR;
-    double rv2_D = 0.5 * (mass_half * (velxL + velxR) - fabs(mass_half) * (velxR - velxL));
-    double rv2 = (0.5 + s) * rv2_V + (0.5 - s) * rv2_D;
-    bool up = (mass_half >= 0.0);
-    double vely = up ? velyL : velyR;
-    double velz = up ? velzL : velzR;
+    real massL = velxLplus * rhoL;
+    real massR = velxRminus * rhoR;
+    real mass_half = massL + massR;
+    real p_half = pLplus + pRminus;
+    real dp = pL - pR;
+    dp = real(10.0) * fabs(dp) / fmin(pL, pR);
+    real s = real(0.5) * fmin(real(1.0), dp);
+    real rv2_V = massL * velxL + massR * velxR;
+    real rv2_D = real(0.5) * (mass_half * (velxL + velxR) - fabs(mass_half) * (velxR - velxL));
+    real rv2 = (real(0.5) + s) * rv2_V + (real(0.5) - s) * rv2_D;
+    bool up = (mass_half >= real(0.0));
+    real vely = up ? velyL : velyR;
+    real velz = up ? velzL : velzR;
     F[0] = mass_half;
     to_global(f, rv2 + p_half, mass_half * vely, mass_half * velz, F[1], F[2], F[3]);
     F[4] = mass_half * (up ? HL : HR);
@@ -156,9 +162,9 @@ CHKN_HD void ausmdv(double rhoL, double eL, double pL, double aL, double YBL, do
 // the equation of state, then reconstructs the velocity; XORDER 1 uses the adjacent cells.
 template <int XORDER>
 CHKN_HD void face_flux(const Prim& L1, const Prim& L0, const Prim& R0, const Prim& R1,
-                       const Frame& f, double F[6])
+                       const Frame& f, real F[6])
 {
-    double rhoL, rhoR, eL, eR, YBL, YBR, vxL, vxR, vyL, vyR, vzL, vzR;
+    real rhoL, rhoR, eL, eR, YBL, YBR, vxL, vxR, vyL, vyR, vzL, vzR;
     if (XORDER > 1) {
         muscl(L1.rho, L0.rho, R0.rho, R1.rho, rhoL, rhoR);
         muscl(L1.e, L0.e, R0.e, R1.e, eL, eR);
@@ -170,9 +176,9 @@ CHKN_HD void face_flux(const Prim& L1, const Prim& L0, const Prim& R0, const Pri
         rhoL = L0.rho; eL = L0.e; YBL = L0.YB; vxL = L0.vx; vyL = L0.vy; vzL = L0.vz;
         rhoR = R0.rho; eR = R0.e; YBR = R0.YB; vxR = R0.vx; vyR = R0.vy; vzR = R0.vz;
     }
-    double TL = eos_T(eL, YBL), TR = eos_T(eR, YBR);
-    double pL = eos_p(rhoL, TL), pR = eos_p(rhoR, TR);
-    double aL = eos_a(TL), aR = eos_a(TR);
+    real TL = eos_T(eL, YBL), TR = eos_T(eR, YBR);
+    real pL = eos_p(rhoL, TL), pR = eos_p(rhoR, TR);
+    real aL = eos_a(TL), aR = eos_a(TR);
     ausmdv(rhoL, eL, pL, aL, YBL, vxL, vyL, vzL, rhoR, eR, pR, aR, YBR, vxR, vyR, vzR, f, F);
 }
 
@@ -181,14 +187,14 @@ CHKN_HD void face_flux(const Prim& L1, const Prim& L0, const Prim& R0, const Pri
 // conservation terms over the four-cell stencil, blended with the alpha weights 1 (mass),
 // 1/2 (momentum, internal energy), 0 (kinetic energy, pressure).  Uses each state's own p
 // and e.  The alpha products are kept as written: 0*x is not 0 for all x.
-CHKN_HD void sbp_asf(const Prim* s, const double* p, const Frame& f, double F[6])
+CHKN_HD void sbp_asf(const Prim* s, const real* p, const Frame& f, real F[6])
 {
-    constexpr double one_twelfth = 1.0 / 12.0;
-    double v[10][4], w[10][4];
+    constexpr real one_twelfth = real(1.0) / real(12.0);
+    real v[10][4], w[10][4];
     for (int i = 0; i < 4; ++i) {
-        double velx, vely, velz;
+        real velx, vely, velz;
         to_local(f, s[i].vx, s[i].vy, s[i].vz, velx, vely, velz);
-        const double rho = s[i].rho;
+        const real rho = s[i].rho;
         v[0][i] = rho;                 w[0][i] = velx;
         v[1][i] = velx * rho;          w[1][i] = velx;
         v[2][i] = vely * rho;          w[2][i] = velx;
@@ -198,55 +204,55 @@ CHKN_HD void sbp_asf(const Prim* s, const double* p, const Frame& f, double F[6]
         v[6][i] = vely * vely * rho;   w[6][i] = velx;
         v[7][i] = velz * velz * rho;   w[7][i] = velx;
         v[8][i] = p[i];                w[8][i] = velx;
-        v[9][i] = p[i];                w[9][i] = 1.0;
+        v[9][i] = p[i];                w[9][i] = real(1.0);
     }
-    double fc[10], fe[10];
+    real fc[10], fe[10];
     for (int j = 0; j < 10; ++j) {
-        fc[j] = one_twelfth * (-v[j][0] * w[j][0] + 7.0 * v[j][1] * w[j][1] + 7.0 * v[j][2] * w[j][2] -
+        fc[j] = one_twelfth * (-v[j][0] * w[j][0] + real(7.0) * v[j][1] * w[j][1] + real(7.0) * v[j][2] * w[j][2] -
                                v[j][3] * w[j][3]);
-        fe[j] = one_twelfth * (-v[j][0] * w[j][2] - v[j][2] * w[j][0] + 8.0 * v[j][1] * w[j][2] +
-                               8.0 * v[j][2] * w[j][1] - v[j][1] * w[j][3] - v[j][3] * w[j][1]);
+        fe[j] = one_twelfth * (-v[j][0] * w[j][2] - v[j][2] * w[j][0] + real(8.0) * v[j][1] * w[j][2] +
+                               real(8.0) * v[j][2] * w[j][1] - v[j][1] * w[j][3] - v[j][3] * w[j][1]);
     }
-    const double a_mass = 1.0, a_mom = 0.5, a_ie = 0.5, a_ke = 0.0, a_p = 0.0;
-    const double mass = a_mass * fc[0] + (1.0 - a_mass) * fe[0];
+    const real a_mass = real(1.0), a_mom = real(0.5), a_ie = real(0.5), a_ke = real(0.0), a_p = real(0.0);
+    const real mass = a_mass * fc[0] + (real(1.0) - a_mass) * fe[0];
     F[0] = mass;
-    const double mx = a_mom * fc[1] + (1.0 - a_mom) * fe[1] + (a_p * fc[9] + (1.0 - a_p) * fe[9]);
-    const double my = a_mom * fc[2] + (1.0 - a_mom) * fe[2];
-    const double mz = a_mom * fc[3] + (1.0 - a_mom) * fe[3];
+    const real mx = a_mom * fc[1] + (real(1.0) - a_mom) * fe[1] + (a_p * fc[9] + (real(1.0) - a_p) * fe[9]);
+    const real my = a_mom * fc[2] + (real(1.0) - a_mom) * fe[2];
+    const real mz = a_mom * fc[3] + (real(1.0) - a_mom) * fe[3];
     to_global(f, mx, my, mz, F[1], F[2], F[3]);
-    F[4] = a_ie * fc[4] + (1.0 - a_ie) * fe[4] +
-           0.5 * (a_ke * fc[5] + (1.0 - a_ke) * fe[5] + a_ke * fc[6] + (1.0 - a_ke) * fe[6] +
-                  a_ke * fc[7] + (1.0 - a_ke) * fe[7]) +
-           a_p * fc[8] + (1.0 - a_p) * fe[8];
-    F[5] = mass * ((mass >= 0.0) ? s[1].YB : s[2].YB);
+    F[4] = a_ie * fc[4] + (real(1.0) - a_ie) * fe[4] +
+           real(0.5) * (a_ke * fc[5] + (real(1.0) - a_ke) * fe[5] + a_ke * fc[6] + (real(1.0) - a_ke) * fe[6] +
+                  a_ke * fc[7] + (real(1.0) - a_ke) * fe[7]) +
+           a_p * fc[8] + (real(1.0) - a_p) * fe[8];
+    F[5] = mass * ((mass >= real(0.0)) ? s[1].YB : s[2].YB);
 }
 
 // flow.cu:227-243
-CHKN_HD void encode(const Prim& s, double U[6])
+CHKN_HD void encode(const Prim& s, real U[6])
 {
-    double rho = s.rho;
+    real rho = s.rho;
     U[0] = rho;
     U[1] = rho * s.vx;
     U[2] = rho * s.vy;
     U[3] = rho * s.vz;
-    double ke = 0.5 * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+    real ke = real(0.5) * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
     U[4] = rho * (s.e + ke);
     U[5] = rho * s.YB;
 }
 
 // flow.cu:245-268 without the equation-of-state tail.  Returns 1 for a bad cell.
-CHKN_HD int decode(const double U[6], Prim& s)
+CHKN_HD int decode(const real U[6], Prim& s)
 {
     bool nan = isnan(U[0]) || isnan(U[1]) || isnan(U[2]) || isnan(U[3]) || isnan(U[4]) || isnan(U[5]);
     if (nan) return 1;
-    double rho = U[0];
-    if (rho <= 0.0) return 1;
-    double dinv = 1.0 / rho;
+    real rho = U[0];
+    if (rho <= real(0.0)) return 1;
+    real dinv = real(1.0) / rho;
     s.vx = U[1] * dinv;
     s.vy = U[2] * dinv;
     s.vz = U[3] * dinv;
-    double e = U[4] * dinv;
-    double ke = 0.5 * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+    real e = U[4] * dinv;
+    real ke = real(0.5) * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
     e -= ke;
     s.rho = rho;
     s.e = e;
@@ -257,12 +263,12 @@ CHKN_HD int decode(const double U[6], Prim& s)
 // Signal-speed time-step estimate without the cfl factor, cell.cu:155-164.  The cfl
 // multiply is applied after the minimum: rounding is monotone, so
 // min(fl(cfl*x_i)) == fl(cfl*min(x_i)) for cfl > 0.
-CHKN_HD double local_signal(const Prim& s, double a, double iL, double jL, double kL,
+CHKN_HD real local_signal(const Prim& s, real a, real iL, real jL, real kL,
                             const Frame& fi, const Frame& fj, const Frame& fk)
 {
-    double is = iL / (fabs(s.vx * fi.nx + s.vy * fi.ny + s.vz * fi.nz) + a);
-    double js = jL / (fabs(s.vx * fj.nx + s.vy * fj.ny + s.vz * fj.nz) + a);
-    double ks = kL / (fabs(s.vx * fk.nx + s.vy * fk.ny + s.vz * fk.nz) + a);
+    real is = iL / (fabs(s.vx * fi.nx + s.vy * fi.ny + s.vz * fi.nz) + a);
+    real js = jL / (fabs(s.vx * fj.nx + s.vy * fj.ny + s.vz * fj.nz) + a);
+    real ks = kL / (fabs(s.vx * fk.nx + s.vy * fk.ny + s.vz * fk.nz) + a);
     return fmin(fmin(is, js), ks);
 }
 

@@ -39,30 +39,30 @@ struct HostBlock {
     bool active;
     int bc[6], bc_fs[6];
     int n;
-    std::vector<double> io0;  // IO variables as given at start (flow part used while fresh)
-    std::vector<double> pos;  // 3*n computed centroids
-    std::vector<double> vol;  // n computed volumes
+    std::vector<real> io0;  // IO variables as given at start (flow part used while fresh)
+    std::vector<real> pos;  // 3*n computed centroids
+    std::vector<real> vol;  // n computed volumes
     // device
-    double* W = nullptr;  // 3 buffers (active) or 1 (inactive)
-    double* U = nullptr;
-    double* L = nullptr;
-    double* gi = nullptr;
-    double* gj = nullptr;
-    double* gk = nullptr;
-    double* volinv = nullptr;
-    double* len = nullptr;
+    real* W = nullptr;  // 3 buffers (active) or 1 (inactive)
+    real* U = nullptr;
+    real* L = nullptr;
+    real* gi = nullptr;
+    real* gj = nullptr;
+    real* gk = nullptr;
+    real* volinv = nullptr;
+    real* len = nullptr;
     int tile_first = 0, tile_count = 0;
     // face-list stage (viscous jobs; first-order jobs with verbatim gas states)
-    double twall[6] = {0, 0, 0, 0, 0, 0};
+    real twall[6] = {0, 0, 0, 0, 0, 0};
     int bc_fun[6] = {0, 0, 0, 0, 0, 0};      // inflow_function sides: function index
-    std::vector<double> twall_faces[6];      // fixed-T sides with TWall per face
-    double* side[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // device side data
+    std::vector<real> twall_faces[6];      // fixed-T sides with TWall per face
+    real* side[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // device side data
     bool gas_differs = false;  // some cell's file p, T, a differ from the equation of state
-    double* F = nullptr;       // 6 x nft face fluxes
-    double* FS = nullptr;      // nft x 4 face T, vx, vy, vz (viscous)
-    double* fpos = nullptr;    // nft x 4 face centres (viscous)
-    double* cpos = nullptr;    // n x 4 cell centres (viscous)
-    double* gas = nullptr;     // 3 x n file p, T, a (only when gas_differs)
+    real* F = nullptr;       // 6 x nft face fluxes
+    real* FS = nullptr;      // nft x 4 face T, vx, vy, vz (viscous)
+    real* fpos = nullptr;    // nft x 4 face centres (viscous)
+    real* cpos = nullptr;    // n x 4 cell centres (viscous)
+    real* gas = nullptr;     // 3 x n file p, T, a (only when gas_differs)
     // adjoint (allocated by the first gradient call)
     double* tape = nullptr;    // (3C+1) W slots of the segment being reversed
     double* tape2 = nullptr;   // second tape: the next segment is recomputed into it meanwhile
@@ -77,10 +77,10 @@ struct Instance {
     int nib = 0, njb = 0, nkb = 0;
     std::vector<int> blk_ids;  // global block ids
     int cfl_count = 10, max_step = 100, print_count = 20;
-    double max_time = 1e-3, dt_init = 1e-6;
-    std::vector<double> cfl_t, cfl_v, plot_t, plot_v;
+    real max_time = real(1e-3), dt_init = real(1e-6);
+    std::vector<real> cfl_t, cfl_v, plot_t, plot_v;
     std::string job;
-    double t = 0.0, dt = 0.0, t_plot = 0.0, sigmin = 0.0;
+    real t = 0.0, dt = 0.0, t_plot = 0.0, sigmin = 0.0;
     int step = 0, next_plot_indx = 1, steps_since_last_plot = 0;
     bool have_sig = false;
     bool reacting = false;  // the A->B reaction after every step (simulate.cu:296-303)
@@ -137,13 +137,13 @@ struct chkn_sim {
     // configuration
     std::vector<HostBlock> blocks;
     std::vector<Instance> inst;
-    std::vector<double> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz (all instances)
+    std::vector<chkn::real> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz (all instances)
     int x_order = 2, flux_calc = 0;
     bool viscous = false;
     bool any_reacting = false;
     // Gas states (p, T, a) of the inflow states as configured (update_from_pT), and whether
     // the equation of state of their (rho, e) gives other values.
-    std::vector<double> inflow_gas;
+    std::vector<chkn::real> inflow_gas;
     std::vector<char> inflow_differs;
     int gstep = 0;       // lockstep step counter shared by the instances
     bool fresh = true;   // no step taken since the state was set from IO variables
@@ -154,8 +154,8 @@ struct chkn_sim {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     BlockDev* d_blocks = nullptr;
     Tile* d_tiles = nullptr;
-    double* d_inflow = nullptr;
-    double* d_inflow_gas = nullptr;
+    chkn::real* d_inflow = nullptr;
+    chkn::real* d_inflow_gas = nullptr;
     // face-list stage work tables
     int nsplit_faces = 0, nsplit_cells = 0, nsplit_patches = 0;
     chkn::FaceChunk* d_split_faces = nullptr;
@@ -188,11 +188,11 @@ namespace rt {
 int set_error(chkn_sim* s, int code, const std::string& msg);
 void dfree(void* p);
 // Views of the control area for step slot q (N instances).
-double* ctl_dt(char* base, int q, int N);
+real* ctl_dt(char* base, int q, int N);
 unsigned long long* ctl_sig(char* base, int q, int N);
 int* ctl_run(char* base, int q, int N);
 int* ctl_bad(char* base, int q, int N);
-double* ctl_chem(char* base, int q, int N);
+real* ctl_chem(char* base, int q, int N);
 // Fill the chemistry factors of step slot q from the instances' dt (-1: not reacting).
 void fill_chem(chkn_sim* s, int q);
 StageArgs stage_args(chkn_sim* s, int q);
@@ -212,14 +212,14 @@ bool fresh_needs_file_gas(const chkn_sim* s);
 // Face-list stage set-up (chkn_split.cpp).
 // Per block after its geometry: face fluxes, and for viscous jobs face states and the
 // cell and face centres (fpos, [3][nft]) the least-squares gradients are taken over.
-void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vector<double>& fpos);
+void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vector<real>& fpos);
 // Per-side data of an active block: inflow-function ghost and face states evaluated at their
 // positions (prof: the instance's boundary-layer profile, or null), TWall per face.
-void side_data_setup(HostBlock& b, BlockDev& bd, const std::vector<double>& fpos, const bl::Profile* prof);
+void side_data_setup(HostBlock& b, BlockDev& bd, const std::vector<real>& fpos, const bl::Profile* prof);
 // The file gas states of block b from its IO variables (re-run when they are replaced).
 void split_block_gas(chkn_sim* s, HostBlock& b, BlockDev& bd);
 // Inflow gas states from the configured p, T, YB of every flow state.
-void split_inflow_gas(chkn_sim* s, const std::vector<double>& pTYB);
+void split_inflow_gas(chkn_sim* s, const std::vector<real>& pTYB);
 // Work tables over every active block, after all blocks are set up.
 void split_tables(chkn_sim* s);
 void split_free(chkn_sim* s);

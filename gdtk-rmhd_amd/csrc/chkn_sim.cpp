@@ -50,12 +50,19 @@ int set_error(chkn_sim* s, int code, const std::string& msg)
     return code;
 }
 
-double schedule_get(const std::vector<double>& ts, const std::vector<double>& vs, double t)
+real schedule_get(const std::vector<real>& ts, const std::vector<real>& vs, real t)
 {
     // config.cu:263-269
     int i = (int)ts.size() - 1;
     while (i > 0 && t < ts[i]) i--;
     return vs[i];
+}
+
+std::vector<real> to_real(const double* v, size_t n)
+{
+    std::vector<real> r(n);
+    for (size_t i = 0; i < n; ++i) r[i] = (real)v[i];
+    return r;
 }
 
 void dfree(void* p)
@@ -91,9 +98,9 @@ void upload_blocks(chkn_sim* s)
 
 // Geometry pre-pass of one block (block.cu:633-652) into device-layout host arrays.
 // fpos, when given, receives the face centres [3][nft] (i-, then j-, then k-faces).
-void block_geometry(HostBlock& b, const double* vtx, std::vector<double>& gi, std::vector<double>& gj,
-                    std::vector<double>& gk, std::vector<double>& volinv, std::vector<double>& len,
-                    std::vector<double>* fpos)
+void block_geometry(HostBlock& b, const real* vtx, std::vector<real>& gi, std::vector<real>& gj,
+                    std::vector<real>& gk, std::vector<real>& volinv, std::vector<real>& len,
+                    std::vector<real>* fpos)
 {
     const int nic = b.nic, njc = b.njc, nkc = b.nkc;
     auto V = [&](int i, int j, int k) {
@@ -116,10 +123,10 @@ void block_geometry(HostBlock& b, const double* vtx, std::vector<double>& gi, st
                 size_t c = (size_t)i + (size_t)nic * (j + (size_t)njc * k);
                 b.pos[c] = h.c.x; b.pos[n + c] = h.c.y; b.pos[2 * n + c] = h.c.z;
                 b.vol[c] = h.vol;
-                volinv[c] = 1.0 / h.vol;  // cell.cu:190
+                volinv[c] = real(1) / h.vol;  // cell.cu:190
                 len[c] = h.iL; len[n + c] = h.jL; len[2 * n + c] = h.kL;
             }
-    auto put = [](std::vector<double>& g, size_t nf, size_t f, const geom::Quad& q) {
+    auto put = [](std::vector<real>& g, size_t nf, size_t f, const geom::Quad& q) {
         g[f] = q.area;
         g[nf + f] = q.n.x; g[2 * nf + f] = q.n.y; g[3 * nf + f] = q.n.z;
         g[4 * nf + f] = q.t1.x; g[5 * nf + f] = q.t1.y; g[6 * nf + f] = q.t1.z;
@@ -167,7 +174,7 @@ void block_geometry(HostBlock& b, const double* vtx, std::vector<double>& gi, st
 }
 
 // Primitive stage state from IO variables (rho, e, YB, vel from the flow file).
-void prim_from_io(const double* io, int n, std::vector<double>& w)
+void prim_from_io(const real* io, int n, std::vector<real>& w)
 {
     w.assign((size_t)6 * n, 0.0);
     for (int c = 0; c < n; ++c) {
@@ -180,23 +187,23 @@ void prim_from_io(const double* io, int n, std::vector<double>& w)
     }
 }
 
-void conserved_from_prim(const std::vector<double>& w, int n, std::vector<double>& u)
+void conserved_from_prim(const std::vector<real>& w, int n, std::vector<real>& u)
 {
     u.assign((size_t)6 * n, 0.0);
     for (int c = 0; c < n; ++c) {
         Prim p{w[c], w[(size_t)n + c], w[2 * (size_t)n + c], w[3 * (size_t)n + c],
                w[4 * (size_t)n + c], w[5 * (size_t)n + c]};
-        double U[6];
+        real U[6];
         encode(p, U);  // simulate.cu:196-203 via flow.cu:227-243
         for (int q = 0; q < 6; ++q) u[(size_t)q * n + c] = U[q];
     }
 }
 
-void upload_state(chkn_sim* s, HostBlock& b, const double* io)
+void upload_state(chkn_sim* s, HostBlock& b, const real* io)
 {
-    std::vector<double> w, u;
+    std::vector<real> w, u;
     prim_from_io(io, b.n, w);
-    size_t bytes = sizeof(double) * 6 * (size_t)b.n;
+    size_t bytes = sizeof(real) * 6 * (size_t)b.n;
     int nbuf = b.active ? 3 : 1;
     for (int r = 0; r < nbuf; ++r) HIPCHK(hipMemcpy(b.W + (size_t)r * 6 * b.n, w.data(), bytes, hipMemcpyHostToDevice));
     if (b.active) {
@@ -248,33 +255,50 @@ void build_tiles(chkn_sim* s)
     s->graphs.clear();
 }
 
-// Control-area views for segment slot q.
-double* ctl_dt(char* base, int q, int N) { return reinterpret_cast<double*>(base) + (size_t)q * N; }
+// Control-area views for segment slot q: dt[S][N], sigmin[S][N], run[S][N], bad[S][3],
+// chem[S][N] with S = kMaxSeg step slots and N instances.
+namespace {
+size_t ctl_off_sig(int N) { return sizeof(real) * kMaxSeg * (size_t)N; }
+size_t ctl_off_run(int N) { return ctl_off_sig(N) + sizeof(unsigned long long) * kMaxSeg * (size_t)N; }
+size_t ctl_off_bad(int N) { return ctl_off_run(N) + sizeof(int) * kMaxSeg * (size_t)N; }
+size_t ctl_off_chem(int N)
+{
+    const size_t o = ctl_off_bad(N) + sizeof(int) * 3 * kMaxSeg;
+    return (o + 7) & ~size_t(7);
+}
+}  // namespace
+
+real* ctl_dt(char* base, int q, int N) { return reinterpret_cast<real*>(base) + (size_t)q * N; }
 unsigned long long* ctl_sig(char* base, int q, int N)
 {
-    return reinterpret_cast<unsigned long long*>(base + sizeof(double) * kMaxSeg * N) + (size_t)q * N;
+    return reinterpret_cast<unsigned long long*>(base + ctl_off_sig(N)) + (size_t)q * N;
 }
-int* ctl_run(char* base, int q, int N)
+int* ctl_run(char* base, int q, int N) { return reinterpret_cast<int*>(base + ctl_off_run(N)) + (size_t)q * N; }
+int* ctl_bad(char* base, int q, int N) { return reinterpret_cast<int*>(base + ctl_off_bad(N)) + 3 * q; }
+real* ctl_chem(char* base, int q, int N) { return reinterpret_cast<real*>(base + ctl_off_chem(N)) + (size_t)q * N; }
+size_t ctl_size(int N) { return ctl_off_chem(N) + sizeof(real) * kMaxSeg * (size_t)N; }
+
+// The minimum signal time from its bit pattern (non-negative reals order as their patterns).
+real sig_value(unsigned long long bits)
 {
-    return reinterpret_cast<int*>(base + 2 * sizeof(double) * kMaxSeg * N) + (size_t)q * N;
-}
-double* ctl_chem(char* base, int q, int N)
-{
-    return reinterpret_cast<double*>(base + (2 * sizeof(double) + sizeof(int)) * kMaxSeg * N +
-                                     sizeof(int) * 3 * kMaxSeg) + (size_t)q * N;
+    if (bits == ~0ull) return std::numeric_limits<real>::max();
+    real m;
+    if (sizeof(real) == sizeof(unsigned long long)) {
+        std::memcpy(&m, &bits, sizeof m);
+    } else {
+        const unsigned int lo = (unsigned int)bits;
+        std::memcpy(&m, &lo, sizeof m);
+    }
+    return m;
 }
 
+// exp(-alpha dt) of the A->B reaction (gas.cu:108), the same for every cell of an instance.
 void fill_chem(chkn_sim* s, int q)
 {
     const int N = (int)s->inst.size();
-    double* c = ctl_chem(s->h_ctl, q, N);
-    const double* dt = ctl_dt(s->h_ctl, q, N);
-    for (int qi = 0; qi < N; ++qi) c[qi] = s->inst[qi].reacting ? std::exp(-kAlpha * dt[qi]) : -1.0;
-}
-
-int* ctl_bad(char* base, int q, int N)
-{
-    return reinterpret_cast<int*>(base + 2 * sizeof(double) * kMaxSeg * N + sizeof(int) * kMaxSeg * N) + 3 * q;
+    real* c = ctl_chem(s->h_ctl, q, N);
+    const real* dt = ctl_dt(s->h_ctl, q, N);
+    for (int qi = 0; qi < N; ++qi) c[qi] = s->inst[qi].reacting ? std::exp(-kAlpha * dt[qi]) : real(-1);
 }
 
 void check_desc(const chkn_desc* d)
@@ -307,7 +331,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     for (int q = 0; q < ninst; ++q) nblocks_total += descs[q].nblocks;
     s->blocks.resize(nblocks_total);
     int fs_base = 0, blk_base = 0;
-    std::vector<double> inflow_pTYB;  // configured p, T, YB of every flow state
+    std::vector<real> inflow_pTYB;  // configured p, T, YB of every flow state
     for (int qi = 0; qi < ninst; ++qi) {
         const chkn_desc* d = &descs[qi];
         Instance& I = s->inst[qi];
@@ -318,25 +342,27 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
         I.cfl_count = d->cfl_count; I.max_step = d->max_step; I.print_count = d->print_count;
         I.reacting = d->reacting != 0;
         s->any_reacting = s->any_reacting || I.reacting;
-        I.max_time = d->max_time; I.dt_init = d->dt_init;
-        I.cfl_t.assign(d->cfl_times, d->cfl_times + d->n_cfl);
-        I.cfl_v.assign(d->cfl_values, d->cfl_values + d->n_cfl);
+        // Configuration values are held as real, as the reference holds them as number
+        // (config.cu:494-504).
+        I.max_time = (real)d->max_time; I.dt_init = (real)d->dt_init;
+        I.cfl_t = to_real(d->cfl_times, d->n_cfl);
+        I.cfl_v = to_real(d->cfl_values, d->n_cfl);
         if (d->n_dt_plot > 0) {
-            I.plot_t.assign(d->dt_plot_times, d->dt_plot_times + d->n_dt_plot);
-            I.plot_v.assign(d->dt_plot_values, d->dt_plot_values + d->n_dt_plot);
+            I.plot_t = to_real(d->dt_plot_times, d->n_dt_plot);
+            I.plot_v = to_real(d->dt_plot_values, d->n_dt_plot);
         } else {
-            I.plot_t.assign(1, 0.0);
-            I.plot_v.assign(1, d->max_time);
+            I.plot_t.assign(1, real(0));
+            I.plot_v.assign(1, I.max_time);
         }
         // Inflow states: update_from_pT of the configured p, T, YB (config.cu:398-406).
         for (int q = 0; q < d->n_flow_states; ++q) {
-            const double* f = d->flow_states + 6 * q;
-            double p = f[0], T = f[1], YB = f[2];
-            double e = T * kCv - YB * kQ;   // gas.cu:123
-            double rho = p / (T * kR);      // gas.cu:124
-            double w[6] = {rho, e, YB, f[3], f[4], f[5]};
+            const std::vector<real> f = to_real(d->flow_states + 6 * q, 6);
+            real p = f[0], T = f[1], YB = f[2];
+            real e = T * kCv - YB * kQ;   // gas.cu:123
+            real rho = p / (T * kR);      // gas.cu:124
+            real w[6] = {rho, e, YB, f[3], f[4], f[5]};
             s->inflow_w.insert(s->inflow_w.end(), w, w + 6);
-            inflow_pTYB.insert(inflow_pTYB.end(), f, f + 3);
+            inflow_pTYB.insert(inflow_pTYB.end(), f.begin(), f.begin() + 3);
         }
         size_t ooff = 0, twall_used = 0;
         for (int lb = 0; lb < d->nblocks; ++lb) {
@@ -348,7 +374,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             for (int f = 0; f < 6; ++f) {
                 b.bc[f] = d->bc_codes[6 * lb + f];
                 b.bc_fs[f] = d->bc_fs[6 * lb + f] + fs_base;
-                b.twall[f] = d->bc_twall ? d->bc_twall[6 * lb + f] : 0.0;
+                b.twall[f] = d->bc_twall ? (real)d->bc_twall[6 * lb + f] : real(0);
                 b.bc_fun[f] = d->bc_fun ? d->bc_fun[6 * lb + f] : 0;
                 b.twall_faces[f].clear();
                 if (b.bc[f] == CHKN_BC_WALL_NO_SLIP_FIXED_T && std::isnan(b.twall[f])) {
@@ -356,7 +382,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
                     const size_t nf = f < 2 ? (size_t)b.njc * b.nkc : (f < 4 ? (size_t)b.nic * b.nkc : (size_t)b.nic * b.njc);
                     if (!d->twall_faces || twall_used + nf > (size_t)d->n_twall_faces)
                         throw std::invalid_argument("TWall given per face, but twall_faces holds too few values");
-                    b.twall_faces[f].assign(d->twall_faces + twall_used, d->twall_faces + twall_used + nf);
+                    b.twall_faces[f] = to_real(d->twall_faces + twall_used, nf);
                     twall_used += nf;
                 }
                 if (b.active && b.bc[f] == CHKN_BC_INFLOW_FUNCTION) {  // bcs.cu:321-352
@@ -382,7 +408,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
                     (d->bc_fs[6 * lb + f] < 0 || d->bc_fs[6 * lb + f] >= d->n_flow_states))
                     throw std::invalid_argument("inflow flow_state_index out of range");
             }
-            b.io0.assign(d->iovars + ooff, d->iovars + ooff + (size_t)13 * b.n);
+            b.io0 = to_real(d->iovars + ooff, (size_t)13 * b.n);
             ooff += (size_t)13 * b.n;
         }
         for (auto& id : I.blk_ids) {
@@ -392,7 +418,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
         fs_base += d->n_flow_states;
         blk_base += d->nblocks;
     }
-    if (s->inflow_w.empty()) s->inflow_w.assign(6, 0.0);
+    if (s->inflow_w.empty()) s->inflow_w.assign(6, real(0));
     // Exchange partners (bcs.cu:145-252, periodic wrap at the array edges).
     std::vector<std::array<int, 6>> partner(s->blocks.size());
     for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
@@ -430,16 +456,18 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
         const chkn_desc* d = &descs[qi];
         size_t voff = 0;
         bl::Profile profile;  // the instance's laminar boundary-layer profile (config.cu:37-60)
-        if (d->bl_profile) profile.set(d->bl_profile);
+        if (d->bl_profile) profile.set(to_real(d->bl_profile, bl::kProfileValues).data());
         for (int lb = 0; lb < d->nblocks; ++lb) {
             int ib = s->inst[qi].first_block + lb;
             HostBlock& b = s->blocks[ib];
-            std::vector<double> gi, gj, gk, volinv, len, fpos;
+            std::vector<real> gi, gj, gk, volinv, len, fpos;
             bool want_fpos = s->viscous && b.active;
             for (int f = 0; f < 6; ++f) want_fpos = want_fpos || (b.active && b.bc[f] == CHKN_BC_INFLOW_FUNCTION);
-            block_geometry(b, d->vertices + voff, gi, gj, gk, volinv, len, want_fpos ? &fpos : nullptr);
-            voff += (size_t)3 * (b.nic + 1) * (b.njc + 1) * (b.nkc + 1);
-            size_t n = (size_t)b.n, sz6 = sizeof(double) * 6 * n;
+            const size_t nv = (size_t)3 * (b.nic + 1) * (b.njc + 1) * (b.nkc + 1);
+            block_geometry(b, to_real(d->vertices + voff, nv).data(), gi, gj, gk, volinv, len,
+                           want_fpos ? &fpos : nullptr);
+            voff += nv;
+            size_t n = (size_t)b.n, sz6 = sizeof(real) * 6 * n;
             HIPCHK(hipMalloc(&b.W, sz6 * (b.active ? 3 : 1)));
             BlockDev& bd = bdev[ib];
             std::memset(&bd, 0, sizeof bd);
@@ -455,16 +483,16 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
                 HIPCHK(hipMalloc(&b.U, sz6));
                 HIPCHK(hipMalloc(&b.L, sz6));
                 HIPCHK(hipMemset(b.L, 0, sz6));
-                HIPCHK(hipMalloc(&b.gi, sizeof(double) * gi.size()));
-                HIPCHK(hipMalloc(&b.gj, sizeof(double) * gj.size()));
-                HIPCHK(hipMalloc(&b.gk, sizeof(double) * gk.size()));
-                HIPCHK(hipMalloc(&b.volinv, sizeof(double) * n));
-                HIPCHK(hipMalloc(&b.len, sizeof(double) * 3 * n));
-                HIPCHK(hipMemcpy(b.gi, gi.data(), sizeof(double) * gi.size(), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(b.gj, gj.data(), sizeof(double) * gj.size(), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(b.gk, gk.data(), sizeof(double) * gk.size(), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(b.volinv, volinv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(b.len, len.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+                HIPCHK(hipMalloc(&b.gi, sizeof(real) * gi.size()));
+                HIPCHK(hipMalloc(&b.gj, sizeof(real) * gj.size()));
+                HIPCHK(hipMalloc(&b.gk, sizeof(real) * gk.size()));
+                HIPCHK(hipMalloc(&b.volinv, sizeof(real) * n));
+                HIPCHK(hipMalloc(&b.len, sizeof(real) * 3 * n));
+                HIPCHK(hipMemcpy(b.gi, gi.data(), sizeof(real) * gi.size(), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(b.gj, gj.data(), sizeof(real) * gj.size(), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(b.gk, gk.data(), sizeof(real) * gk.size(), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(b.volinv, volinv.data(), sizeof(real) * n, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(b.len, len.data(), sizeof(real) * 3 * n, hipMemcpyHostToDevice));
                 bd.U = b.U; bd.L = b.L; bd.gi = b.gi; bd.gj = b.gj; bd.gk = b.gk;
                 bd.volinv = b.volinv; bd.len = b.len;
             }
@@ -482,12 +510,12 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     s->bdev = bdev;
     HIPCHK(hipMalloc(&s->d_blocks, sizeof(BlockDev) * bdev.size()));
     upload_blocks(s);
-    HIPCHK(hipMalloc(&s->d_inflow, sizeof(double) * s->inflow_w.size()));
-    HIPCHK(hipMemcpy(s->d_inflow, s->inflow_w.data(), sizeof(double) * s->inflow_w.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&s->d_inflow, sizeof(real) * s->inflow_w.size()));
+    HIPCHK(hipMemcpy(s->d_inflow, s->inflow_w.data(), sizeof(real) * s->inflow_w.size(), hipMemcpyHostToDevice));
     split_inflow_gas(s, inflow_pTYB);
     split_tables(s);
     const int N = ninst;
-    s->ctl_bytes = (3 * sizeof(double) + sizeof(int)) * (size_t)kMaxSeg * N + sizeof(int) * 3 * kMaxSeg;
+    s->ctl_bytes = ctl_size(N);
     HIPCHK(hipMalloc(&s->d_ctl, s->ctl_bytes));
     HIPCHK(hipHostMalloc(&s->h_ctl, s->ctl_bytes, hipHostMallocDefault));
     std::memset(s->h_ctl, 0, s->ctl_bytes);
@@ -499,8 +527,8 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     // SimState at start (simulate.cu:132-150, :190-191).
     for (int qi = 0; qi < ninst; ++qi) {
         Instance& I = s->inst[qi];
-        I.t = descs[qi].t0;
-        I.dt = descs[qi].dt_init;
+        I.t = (real)descs[qi].t0;
+        I.dt = I.dt_init;
         I.step = 0;
         I.next_plot_indx = descs[qi].start_tindx + 1;
         I.t_plot = I.t + schedule_get(I.plot_t, I.plot_v, I.t);
@@ -593,7 +621,7 @@ void run_segment(chkn_sim* s, int K, bool dtred_last)
     HIPCHK(hipStreamSynchronize(s->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-    s->last_ms += ms;
+    s->last_ms += (double)ms;
 }
 
 // Minimum signal time of each instance's current state when no stage-3 launch reduced it
@@ -614,9 +642,7 @@ void signal_reduction(chkn_sim* s)
                           hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     for (int qi = 0; qi < N; ++qi) {
-        double m;
-        std::memcpy(&m, &sg[qi], sizeof m);
-        s->inst[qi].sigmin = (sg[qi] == ~0ull) ? DBL_MAX : m;
+        s->inst[qi].sigmin = sig_value(sg[qi]);
         s->inst[qi].have_sig = true;
     }
 }
@@ -627,7 +653,7 @@ void fill_report(chkn_sim* s, int qi, chkn_report* rep, int done)
 {
     if (!rep) return;
     const Instance& I = s->inst[qi];
-    rep->step = I.step; rep->t = I.t; rep->dt = I.dt;
+    rep->step = I.step; rep->t = (double)I.t; rep->dt = (double)I.dt;
     rep->bad_stage = 0; rep->bad_count = 0; rep->steps_done = done;
 }
 
@@ -652,9 +678,9 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
             if (need) signal_reduction(s);
             for (auto& I : s->inst) {
                 if (!I.running()) continue;
-                double cfl = schedule_get(I.cfl_t, I.cfl_v, I.t);
-                double smallest = cfl * I.sigmin;
-                I.dt = std::fmin(1.5 * I.dt, smallest);
+                real cfl = schedule_get(I.cfl_t, I.cfl_v, I.t);
+                real smallest = cfl * I.sigmin;
+                I.dt = std::fmin(real(1.5) * I.dt, smallest);
                 I.have_sig = false;
             }
         }
@@ -665,7 +691,7 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
         int Kuse = 0;
         for (int qi = 0; qi < N; ++qi) {
             Instance& I = s->inst[qi];
-            double tt = I.t;
+            real tt = I.t;
             int st = I.step, q = 0;
             while (q < K && st < I.max_step && tt < I.max_time) {
                 tt += I.dt;
@@ -679,7 +705,7 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
         if (plot_writes)  // a snapshot of any instance ends the segment for all
             for (int qi = 0; qi < N; ++qi) {
                 Instance& I = s->inst[qi];
-                double tt = I.t;
+                real tt = I.t;
                 for (int q = 0; q < nrun[qi]; ++q) {
                     tt += I.dt;
                     if (tt >= I.t_plot) { Kuse = std::min(Kuse, q + 1); break; }
@@ -687,7 +713,7 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
             }
         K = Kuse;
         for (int q = 0; q < K; ++q) {
-            double* dtv = ctl_dt(s->h_ctl, q, N);
+            real* dtv = ctl_dt(s->h_ctl, q, N);
             int* run = ctl_run(s->h_ctl, q, N);
             for (int qi = 0; qi < N; ++qi) {
                 dtv[qi] = s->inst[qi].dt;
@@ -715,15 +741,13 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
             const unsigned long long* sg = ctl_sig(s->h_ctl, K - 1, N);
             for (int qi = 0; qi < N; ++qi) {
                 if (nrun[qi] < K) continue;
-                double m;
-                std::memcpy(&m, &sg[qi], sizeof m);
-                s->inst[qi].sigmin = (sg[qi] == ~0ull) ? DBL_MAX : m;
+                s->inst[qi].sigmin = sig_value(sg[qi]);
                 s->inst[qi].have_sig = true;
             }
         }
         if (s->dt_log)
             for (int q = 0; q < K; ++q)
-                for (int qi = 0; qi < N; ++qi) s->dt_log->push_back(q < nrun[qi] ? s->inst[qi].dt : 0.0);
+                for (int qi = 0; qi < N; ++qi) s->dt_log->push_back(q < nrun[qi] ? (double)s->inst[qi].dt : 0.0);
         for (int qi = 0; qi < N; ++qi) {  // simulate.cu:306-308
             Instance& I = s->inst[qi];
             for (int q = 0; q < nrun[qi]; ++q) {
@@ -751,21 +775,22 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
     return CHKN_OK;
 }
 
-void block_iovars(chkn_sim* s, int ib, double* out)
+// The 13 IO variables of block ib in the build's real type.
+void block_iovars(chkn_sim* s, int ib, real* out)
 {
     HostBlock& b = s->blocks[ib];
     size_t n = (size_t)b.n;
-    std::memcpy(out, b.pos.data(), sizeof(double) * 3 * n);
-    std::memcpy(out + 3 * n, b.vol.data(), sizeof(double) * n);
+    std::memcpy(out, b.pos.data(), sizeof(real) * 3 * n);
+    std::memcpy(out + 3 * n, b.vol.data(), sizeof(real) * n);
     if (!b.active || s->fresh) {
-        std::memcpy(out + 4 * n, b.io0.data() + 4 * n, sizeof(double) * 9 * n);
+        std::memcpy(out + 4 * n, b.io0.data() + 4 * n, sizeof(real) * 9 * n);
         return;
     }
-    std::vector<double> w(6 * n);
-    HIPCHK(hipMemcpy(w.data(), b.W, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+    std::vector<real> w(6 * n);
+    HIPCHK(hipMemcpy(w.data(), b.W, sizeof(real) * 6 * n, hipMemcpyDeviceToHost));
     for (size_t c = 0; c < n; ++c) {
-        double rho = w[c], e = w[n + c], YB = w[2 * n + c];
-        double T = eos_T(e, YB), p = eos_p(rho, T), a = eos_a(T);  // gas.cu:128-135
+        real rho = w[c], e = w[n + c], YB = w[2 * n + c];
+        real T = eos_T(e, YB), p = eos_p(rho, T), a = eos_a(T);  // gas.cu:128-135
         out[4 * n + c] = p; out[5 * n + c] = T; out[6 * n + c] = rho; out[7 * n + c] = e;
         out[8 * n + c] = YB; out[9 * n + c] = a;
         out[10 * n + c] = w[3 * n + c]; out[11 * n + c] = w[4 * n + c]; out[12 * n + c] = w[5 * n + c];
@@ -782,7 +807,7 @@ int write_instance_flow(chkn_sim* s, int qi, int tindx, bool binary)
     std::filesystem::create_directories(I.job + buf);
     for (int lb = 0; lb < I.nblocks; ++lb) {
         HostBlock& b = s->blocks[I.first_block + lb];
-        std::vector<double> io((size_t)13 * b.n);
+        std::vector<real> io((size_t)13 * b.n);
         block_iovars(s, I.first_block + lb, io.data());
         write_flow(block_file(I.job, "flow", tindx, b.bi, b.bj, b.bk, binary), binary, io.data(), b.n);
     }
@@ -938,7 +963,12 @@ int chkn_write_flow(chkn_sim* sim, int tindx, int binary)
 int chkn_get_iovars(chkn_sim* sim, int blk, double* out)
 {
     if (!sim || !out || blk < 0 || blk >= (int)sim->blocks.size()) return set_error(sim, CHKN_ERR_ARG, "bad argument");
-    return guarded(sim, [&] { block_iovars(sim, blk, out); return CHKN_OK; });
+    return guarded(sim, [&] {
+        std::vector<real> io((size_t)13 * sim->blocks[blk].n);
+        block_iovars(sim, blk, io.data());
+        std::copy(io.begin(), io.end(), out);
+        return CHKN_OK;
+    });
 }
 
 int chkn_get_conserved(chkn_sim* sim, int blk, double* out)
@@ -947,7 +977,9 @@ int chkn_get_conserved(chkn_sim* sim, int blk, double* out)
     return guarded(sim, [&] {
         HostBlock& b = sim->blocks[blk];
         if (!b.active) { std::memset(out, 0, sizeof(double) * 6 * (size_t)b.n); return CHKN_OK; }
-        HIPCHK(hipMemcpy(out, b.U, sizeof(double) * 6 * (size_t)b.n, hipMemcpyDeviceToHost));
+        std::vector<real> u((size_t)6 * b.n);
+        HIPCHK(hipMemcpy(u.data(), b.U, sizeof(real) * u.size(), hipMemcpyDeviceToHost));
+        std::copy(u.begin(), u.end(), out);
         return CHKN_OK;
     });
 }
@@ -957,7 +989,8 @@ int chkn_set_iovars(chkn_sim* sim, int blk, const double* iovars)
     if (!sim || !iovars || blk < 0 || blk >= (int)sim->blocks.size()) return set_error(sim, CHKN_ERR_ARG, "bad argument");
     return guarded(sim, [&] {
         HostBlock& b = sim->blocks[blk];
-        std::memcpy(b.io0.data() + 4 * (size_t)b.n, iovars + 4 * (size_t)b.n, sizeof(double) * 9 * (size_t)b.n);
+        const std::vector<real> v = to_real(iovars + 4 * (size_t)b.n, 9 * (size_t)b.n);
+        std::copy(v.begin(), v.end(), b.io0.begin() + 4 * (size_t)b.n);
         upload_state(sim, b, b.io0.data());
         split_block_gas(sim, b, sim->bdev[blk]);
         upload_blocks(sim);
@@ -1057,6 +1090,8 @@ int chkn_last_timing(chkn_sim* sim, double* ms, long long* launches)
 }
 
 const char* chkn_last_error(chkn_sim* sim) { return sim ? sim->err.c_str() : g_create_error.c_str(); }
+
+int chkn_real_size(void) { return (int)sizeof(real); }
 
 void chkn_destroy(chkn_sim* sim)
 {

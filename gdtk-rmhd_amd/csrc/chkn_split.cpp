@@ -32,9 +32,9 @@ namespace {
 constexpr int kChunk = 256;  // faces or cells per work-group
 
 // The equation of state reproduces (p, T, a) from (rho, e, YB) bit for bit.
-bool eos_agrees(double rho, double e, double YB, double p, double T, double a)
+bool eos_agrees(real rho, real e, real YB, real p, real T, real a)
 {
-    const double T2 = eos_T(e, YB);
+    const real T2 = eos_T(e, YB);
     return T2 == T && eos_p(rho, T2) == p && eos_a(T2) == a;
 }
 
@@ -79,34 +79,34 @@ bool fresh_needs_file_gas(const chkn_sim* s)
     return false;
 }
 
-void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vector<double>& fpos)
+void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vector<real>& fpos)
 {
     for (int f = 0; f < 6; ++f) bd.twall[f] = b.twall[f];
     if (!b.active || !need_gas(s)) return;
     const size_t nft = face_count(b);
-    HIPCHK(hipMalloc(&b.F, sizeof(double) * 6 * nft));
+    HIPCHK(hipMalloc(&b.F, sizeof(real) * 6 * nft));
     bd.F = b.F;
     if (!s->viscous) return;
     // Centres as (x, y, z, 0) records: one cloud point is two vector loads.
-    auto records = [](const std::vector<double>& soa, size_t m) {
-        std::vector<double> r(4 * m, 0.0);
+    auto records = [](const std::vector<real>& soa, size_t m) {
+        std::vector<real> r(4 * m, 0.0);
         for (size_t q = 0; q < m; ++q)
             for (int d = 0; d < 3; ++d) r[4 * q + d] = soa[d * m + q];
         return r;
     };
     const size_t n = (size_t)b.n;
-    const std::vector<double> fr = records(fpos, nft), cr = records(b.pos, n);
-    HIPCHK(hipMalloc(&b.FS, sizeof(double) * 4 * nft));
-    HIPCHK(hipMalloc(&b.fpos, sizeof(double) * 4 * nft));
-    HIPCHK(hipMalloc(&b.cpos, sizeof(double) * 4 * n));
-    HIPCHK(hipMemcpy(b.fpos, fr.data(), sizeof(double) * 4 * nft, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b.cpos, cr.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+    const std::vector<real> fr = records(fpos, nft), cr = records(b.pos, n);
+    HIPCHK(hipMalloc(&b.FS, sizeof(real) * 4 * nft));
+    HIPCHK(hipMalloc(&b.fpos, sizeof(real) * 4 * nft));
+    HIPCHK(hipMalloc(&b.cpos, sizeof(real) * 4 * n));
+    HIPCHK(hipMemcpy(b.fpos, fr.data(), sizeof(real) * 4 * nft, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.cpos, cr.data(), sizeof(real) * 4 * n, hipMemcpyHostToDevice));
     bd.FS = b.FS;
     bd.fpos = b.fpos;
     bd.cpos = b.cpos;
 }
 
-void side_data_setup(HostBlock& b, BlockDev& bd, const std::vector<double>& fpos, const bl::Profile* prof)
+void side_data_setup(HostBlock& b, BlockDev& bd, const std::vector<real>& fpos, const bl::Profile* prof)
 {
     if (!b.active) return;
     const int nic = b.nic, njc = b.njc, nkc = b.nkc;
@@ -116,7 +116,7 @@ void side_data_setup(HostBlock& b, BlockDev& bd, const std::vector<double>& fpos
     for (int bf = 0; bf < 6; ++bf) {
         const int dir = bf / 2, hi = bf & 1;
         const int na = dir == 0 ? njc : nic, nc = dir == 2 ? njc : nkc, nt = na * nc;
-        std::vector<double> data;
+        std::vector<real> data;
         if (b.bc[bf] == CHKN_BC_INFLOW_FUNCTION) {
             data.assign((size_t)22 * nt, 0.0);
             for (int c = 0; c < nc; ++c)
@@ -134,17 +134,17 @@ void side_data_setup(HostBlock& b, BlockDev& bd, const std::vector<double>& fpos
                         cell = (size_t)a + (size_t)nic * (c + (size_t)njc * (hi ? nkc - 1 : 0));
                     }
                     // Ghost positions f + d and f + 3d, d = f - c (block.cu:656-780).
-                    double f[3], d[3];
+                    real f[3], d[3];
                     for (int q = 0; q < 3; ++q) {
                         f[q] = fpos[q * nft + ff];
                         d[q] = f[q] - b.pos[q * n + cell];
                     }
-                    const double y0 = f[1] + d[1];
-                    const double y1 = f[1] + d[1] * 3.0;
+                    const real y0 = f[1] + d[1];
+                    const real y1 = f[1] + d[1] * real(3.0);  // Vector3::mul(number)
                     const bl::State st[2] = {bl::inflow_function(b.bc_fun[bf], prof, y0),
                                              bl::inflow_function(b.bc_fun[bf], prof, y1)};
                     for (int dep = 0; dep < 2; ++dep) {
-                        const double v[9] = {st[dep].rho, st[dep].e, st[dep].YB, st[dep].vx, st[dep].vy,
+                        const real v[9] = {st[dep].rho, st[dep].e, st[dep].YB, st[dep].vx, st[dep].vy,
                                              st[dep].vz, st[dep].p, st[dep].T, st[dep].a};
                         for (int q = 0; q < 9; ++q) data[(size_t)(dep * 9 + q) * nt + t] = v[q];
                     }
@@ -159,8 +159,8 @@ void side_data_setup(HostBlock& b, BlockDev& bd, const std::vector<double>& fpos
         } else {
             continue;
         }
-        HIPCHK(hipMalloc(&b.side[bf], sizeof(double) * data.size()));
-        HIPCHK(hipMemcpy(b.side[bf], data.data(), sizeof(double) * data.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&b.side[bf], sizeof(real) * data.size()));
+        HIPCHK(hipMemcpy(b.side[bf], data.data(), sizeof(real) * data.size(), hipMemcpyHostToDevice));
         bd.side[bf] = b.side[bf];
     }
 }
@@ -169,32 +169,32 @@ void split_block_gas(chkn_sim* s, HostBlock& b, BlockDev& bd)
 {
     if (!need_gas(s)) return;
     const size_t n = (size_t)b.n;
-    const double* io = b.io0.data();
-    std::vector<double> g(3 * n);
+    const real* io = b.io0.data();
+    std::vector<real> g(3 * n);
     bool differs = false;
     for (size_t c = 0; c < n; ++c) {
-        const double p = io[4 * n + c], T = io[5 * n + c], rho = io[6 * n + c], e = io[7 * n + c];
-        const double YB = io[8 * n + c], a = io[9 * n + c];
+        const real p = io[4 * n + c], T = io[5 * n + c], rho = io[6 * n + c], e = io[7 * n + c];
+        const real YB = io[8 * n + c], a = io[9 * n + c];
         g[c] = p; g[n + c] = T; g[2 * n + c] = a;
         if (!eos_agrees(rho, e, YB, p, T, a)) differs = true;
     }
     b.gas_differs = differs;
     if (differs) {
-        if (!b.gas) HIPCHK(hipMalloc(&b.gas, sizeof(double) * 3 * n));
-        HIPCHK(hipMemcpy(b.gas, g.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        if (!b.gas) HIPCHK(hipMalloc(&b.gas, sizeof(real) * 3 * n));
+        HIPCHK(hipMemcpy(b.gas, g.data(), sizeof(real) * 3 * n, hipMemcpyHostToDevice));
     }
     bd.gas = differs ? b.gas : nullptr;
 }
 
-void split_inflow_gas(chkn_sim* s, const std::vector<double>& pTYB)
+void split_inflow_gas(chkn_sim* s, const std::vector<real>& pTYB)
 {
     const size_t ns = pTYB.size() / 3;
     s->inflow_gas.assign(3 * std::max<size_t>(ns, 1), 0.0);
     s->inflow_differs.assign(std::max<size_t>(ns, 1), 0);
     for (size_t q = 0; q < ns; ++q) {
-        const double p = pTYB[3 * q], T = pTYB[3 * q + 1], YB = pTYB[3 * q + 2];
-        const double* w = s->inflow_w.data() + 6 * q;  // rho, e from update_from_pT
-        const double a = eos_a(T);                      // gas.cu:72
+        const real p = pTYB[3 * q], T = pTYB[3 * q + 1], YB = pTYB[3 * q + 2];
+        const real* w = s->inflow_w.data() + 6 * q;  // rho, e from update_from_pT
+        const real a = eos_a(T);                      // gas.cu:72
         s->inflow_gas[3 * q] = p;
         s->inflow_gas[3 * q + 1] = T;
         s->inflow_gas[3 * q + 2] = a;
@@ -202,16 +202,16 @@ void split_inflow_gas(chkn_sim* s, const std::vector<double>& pTYB)
     }
     dfree(s->d_inflow_gas);
     s->d_inflow_gas = nullptr;
-    HIPCHK(hipMalloc(&s->d_inflow_gas, sizeof(double) * s->inflow_gas.size()));
-    HIPCHK(hipMemcpy(s->d_inflow_gas, s->inflow_gas.data(), sizeof(double) * s->inflow_gas.size(),
+    HIPCHK(hipMalloc(&s->d_inflow_gas, sizeof(real) * s->inflow_gas.size()));
+    HIPCHK(hipMemcpy(s->d_inflow_gas, s->inflow_gas.data(), sizeof(real) * s->inflow_gas.size(),
                      hipMemcpyHostToDevice));
 }
 
 void split_set_inflow(chkn_sim* s, int q)
 {
-    const double* w = s->inflow_w.data() + 6 * (size_t)q;
-    const double T = eos_T(w[1], w[2]);
-    const double g[3] = {eos_p(w[0], T), T, eos_a(T)};
+    const real* w = s->inflow_w.data() + 6 * (size_t)q;
+    const real T = eos_T(w[1], w[2]);
+    const real g[3] = {eos_p(w[0], T), T, eos_a(T)};
     std::copy(g, g + 3, s->inflow_gas.begin() + 3 * (size_t)q);
     s->inflow_differs[q] = 0;
     HIPCHK(hipMemcpy(s->d_inflow_gas + 3 * (size_t)q, g, sizeof g, hipMemcpyHostToDevice));

@@ -30,15 +30,24 @@ namespace {
 
 constexpr int kChunk = 256;
 
+// Two reals: the half of a 4-value record one load moves.
+#ifdef CHKN_FP32
+using real2 = float2;
+__device__ __forceinline__ real2 make_real2(real a, real b) { return make_float2(a, b); }
+#else
+using real2 = double2;
+__device__ __forceinline__ real2 make_real2(real a, real b) { return make_double2(a, b); }
+#endif
+
 // gas.cu:36-48: Cp and Sutherland's law for air.
-constexpr double kCp = kCv + kR;
-constexpr double kMuRef = 1.716e-5;
-constexpr double kTRef = 273.0;
-constexpr double kSuth = 111.0;
-constexpr double kPrandtl = 0.72;
+constexpr real kCp = kCv + kR;
+constexpr real kMuRef = real(1.716e-5);
+constexpr real kTRef = real(273.0);
+constexpr real kSuth = real(111.0);
+constexpr real kPrandtl = real(0.72);
 
 struct Gas {
-    double p, T, a;
+    real p, T, a;
 };
 
 __device__ __forceinline__ Gas eos_gas(const Prim& w)  // gas.cu:77-83
@@ -96,10 +105,10 @@ __device__ __forceinline__ Point point(const SplitArgs& A, const BlockDev* B, in
         return r;
     }
     if (code == 4) {  // inflow: the configured flow state
-        const double* s = A.inflow + 6 * B->bc_arg[bf];
+        const real* s = A.inflow + 6 * B->bc_arg[bf];
         r.w = Prim{s[0], s[1], s[2], s[3], s[4], s[5]};
         if (GAS) {
-            const double* g = A.inflow_gas + 3 * B->bc_arg[bf];
+            const real* g = A.inflow_gas + 3 * B->bc_arg[bf];
             r.g = Gas{g[0], g[1], g[2]};
         }
         return r;
@@ -107,7 +116,7 @@ __device__ __forceinline__ Point point(const SplitArgs& A, const BlockDev* B, in
     if (code == 6) {  // inflow function: the state at the ghost cell's position
         const int na = dir == 0 ? B->njc : B->nic;
         const int nt = na * (dir == 2 ? B->njc : B->nkc);
-        const double* s = B->side[bf] + (long long)depth * 9 * nt + (a + na * c);
+        const real* s = B->side[bf] + (long long)depth * 9 * nt + (a + na * c);
         r.w = Prim{s[0], s[nt], s[2 * nt], s[3 * nt], s[4 * nt], s[5 * nt]};
         if (GAS) r.g = Gas{s[6 * nt], s[7 * nt], s[8 * nt]};
         return r;
@@ -116,7 +125,7 @@ __device__ __forceinline__ Point point(const SplitArgs& A, const BlockDev* B, in
     r.w = ld_prim(B->W + A.win * B->wstride, (int)B->n, idx);
     if (GAS) r.g = cell_gas(A, B, r.w, idx);
     if (code == 5) return r;  // outflow copies the interior cell
-    const double* g;          // walls reflect it; p, T, a are copied unchanged
+    const real* g;          // walls reflect it; p, T, a are copied unchanged
     int nf, fi;
     boundary_face(B, dir, side, a, c, g, nf, fi);
     r.w = reflect(r.w, ld_frame(g, nf, fi));
@@ -174,14 +183,14 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
     if (f >= ch.f1 || !A.ctl.run[B->inst]) return;
     const int dir = ch.dir;
     const FacePos p = face_pos(B, dir, f);
-    const double* g = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
+    const real* g = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
     const int nf = (int)(dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk));
     const Frame fr = ld_frame(g, nf, f);
-    double rhoL, rhoR, eL, eR, YBL, YBR, vxL, vxR, vyL, vyR, vzL, vzR, pL, pR, aL, aR;
-    double F[6];
+    real rhoL, rhoR, eL, eR, YBL, YBR, vxL, vxR, vyL, vyR, vzL, vzR, pL, pR, aL, aR;
+    real F[6];
     if (FLUX == 1) {
         Prim s[4];
-        double ps[4];
+        real ps[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const Point pt = point<true>(A, B, dir, p.m - 2 + q, p.a, p.c);
@@ -203,7 +212,7 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
         muscl(s0.vx, s1.vx, s2.vx, s3.vx, vxL, vxR);
         muscl(s0.vy, s1.vy, s2.vy, s3.vy, vyL, vyR);
         muscl(s0.vz, s1.vz, s2.vz, s3.vz, vzL, vzR);
-        const double TL = eos_T(eL, YBL), TR = eos_T(eR, YBR);
+        const real TL = eos_T(eL, YBL), TR = eos_T(eR, YBR);
         pL = eos_p(rhoL, TL);
         pR = eos_p(rhoR, TR);
         aL = eos_a(TL);
@@ -223,19 +232,20 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
     for (int q = 0; q < 6; ++q) B->F[q * nft + p.ff] = F[q];
     if (VISC) {
         // Face state: average of the two reconstructed states, T from the mean e and YB.
-        double T = eos_T(0.5 * (eL + eR), 0.5 * (YBL + YBR));
-        double vx = 0.5 * (vxL + vxR), vy = 0.5 * (vyL + vyR), vz = 0.5 * (vzL + vzR);
+        const real half = real(0.5);
+        real T = eos_T(half * (eL + eR), half * (YBL + YBR));
+        real vx = half * (vxL + vxR), vy = half * (vyL + vyR), vz = half * (vzL + vzR);
         const int nl = line_length(B, dir);
         const int side = (p.m == 0) ? 0 : ((p.m == nl) ? 1 : -1);
         if (side >= 0) {
             const int bf = 2 * dir + side;
             const int code = B->bc[bf];
-            const double* sd = B->side[bf];
+            const real* sd = B->side[bf];
             const int t = p.a + (dir == 0 ? B->njc : B->nic) * p.c;
             if (code == 1 || code == 2) {  // no-slip walls; fixed-T walls also set T
-                vx = 0.0;
-                vy = 0.0;
-                vz = 0.0;
+                vx = real(0);
+                vy = real(0);
+                vz = real(0);
                 if (code == 2) T = sd ? sd[t] : B->twall[bf];
             } else if (code == 6) {  // inflow function: its state at the face centre
                 const int nt = (dir == 0 ? B->njc : B->nic) * (dir == 2 ? B->njc : B->nkc);
@@ -245,9 +255,9 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
                 vz = sd[21 * nt + t];
             }
         }
-        double2* rec = reinterpret_cast<double2*>(B->FS) + 2 * (long long)p.ff;
-        rec[0] = make_double2(T, vx);
-        rec[1] = make_double2(vy, vz);
+        real2* rec = reinterpret_cast<real2*>(B->FS) + 2 * (long long)p.ff;
+        rec[0] = make_real2(T, vx);
+        rec[1] = make_real2(vy, vz);
     }
 }
 
@@ -260,19 +270,19 @@ __device__ __forceinline__ int cell_face(const BlockDev* B, int d, int hi, int i
     return (int)(B->nfi + B->nfj) + i + nic * (j + njc * (k + hi));
 }
 
-// 32-byte records: (x, y, z, unused) centres and (T, vx, vy, vz) face states.
-__device__ __forceinline__ void ld_rec4(const double* base, long long r, double& a, double& b,
-                                        double& c, double& d)
+// 4-value records: (x, y, z, unused) centres and (T, vx, vy, vz) face states.
+__device__ __forceinline__ void ld_rec4(const real* base, long long r, real& a, real& b,
+                                        real& c, real& d)
 {
-    const double2* q = reinterpret_cast<const double2*>(base) + 2 * r;
-    const double2 u = q[0], v = q[1];
+    const real2* q = reinterpret_cast<const real2*>(base) + 2 * r;
+    const real2 u = q[0], v = q[1];
     a = u.x; b = u.y; c = v.x; d = v.y;
 }
-__device__ __forceinline__ void ld_rec(const double* base, long long r, double& x, double& y,
-                                       double& z)
+__device__ __forceinline__ void ld_rec(const real* base, long long r, real& x, real& y,
+                                       real& z)
 {
-    const double2* q = reinterpret_cast<const double2*>(base) + 2 * r;
-    const double2 u = q[0];
+    const real2* q = reinterpret_cast<const real2*>(base) + 2 * r;
+    const real2 u = q[0];
     x = u.x; y = u.y; z = q[1].x;
 }
 
@@ -315,11 +325,11 @@ __device__ __forceinline__ void visit_cloud(const BlockDev* B, int dir, const Fa
 // distance (block.cu:40-47).  Ghost cells sit at f + (f - c) for the interior cell c
 // beside the boundary face (block.cu:656-780).
 __device__ __forceinline__ void cloud_offset(const BlockDev* B, int dir, const FacePos& p,
-                                             double fx, double fy, double fz,
-                                             int kind, int idx, double& dx, double& dy,
-                                             double& dz, double& w2)
+                                             real fx, real fy, real fz,
+                                             int kind, int idx, real& dx, real& dy,
+                                             real& dz, real& w2)
 {
-    double x, y, z;
+    real x, y, z;
     if (kind == 1) {
         ld_rec(B->fpos, idx, x, y, z);
     } else {
@@ -328,7 +338,7 @@ __device__ __forceinline__ void cloud_offset(const BlockDev* B, int dir, const F
         const int c = cell_index(B->nic, B->njc, dir, ghost ? (idx < 0 ? 0 : nl - 1) : idx, p.a, p.c);
         ld_rec(B->cpos, c, x, y, z);
         if (ghost) {
-            const double ex = fx - x, ey = fy - y, ez = fz - z;
+            const real ex = fx - x, ey = fy - y, ez = fz - z;
             x = fx + ex;
             y = fy + ey;
             z = fz + ez;
@@ -337,7 +347,7 @@ __device__ __forceinline__ void cloud_offset(const BlockDev* B, int dir, const F
     dx = x - fx;
     dy = y - fy;
     dz = z - fz;
-    w2 = 1.0 / (dx * dx + dy * dy + dz * dz);
+    w2 = (real)(1.0 / (double)(dx * dx + dy * dy + dz * dz));  // block.cu:54 divides in double
 }
 
 // Viscous flux of every face (block.cu:88-162): least-squares gradients of T and velocity
@@ -363,13 +373,13 @@ __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int 
                            : (dir == 1 ? i + nic * (j + (njc + 1) * ch.k) : i + nic * (j + njc * ch.k));
     const FacePos p = face_pos(B, dir, f);
     const long long nft = B->nfi + B->nfj + B->nfk;
-    const double* __restrict__ FS = B->FS;
-    double fx, fy, fz;
+    const real* __restrict__ FS = B->FS;
+    real fx, fy, fz;
     ld_rec(B->fpos, p.ff, fx, fy, fz);
     // Normal matrix of the cloud.
-    double xx = 0.0, xy = 0.0, xz = 0.0, yy = 0.0, yz = 0.0, zz = 0.0;
+    real xx = 0.0, xy = 0.0, xz = 0.0, yy = 0.0, yz = 0.0, zz = 0.0;
     visit_cloud(B, dir, p, [&](int kind, int idx) {
-        double dx, dy, dz, w2;
+        real dx, dy, dz, w2;
         cloud_offset(B, dir, p, fx, fy, fz, kind, idx, dx, dy, dz, w2);
         xx += w2 * dx * dx;
         xy += w2 * dx * dy;
@@ -380,11 +390,11 @@ __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int 
     });
     // Inverse by cofactors of the symmetric matrix [[xx,xy,xz],[xy,yy,yz],[xz,yz,zz]]; a
     // singular matrix (|det| <= 1e-16) gives zero weights, as in the reference.
-    double m00 = 0.0, m01 = 0.0, m02 = 0.0, m10 = 0.0, m11 = 0.0, m12 = 0.0, m20 = 0.0,
+    real m00 = 0.0, m01 = 0.0, m02 = 0.0, m10 = 0.0, m11 = 0.0, m12 = 0.0, m20 = 0.0,
            m21 = 0.0, m22 = 0.0;
-    const double det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
-    if (fabs(det) > 1.0e-16) {
-        const double r = 1.0 / det;
+    const real det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+    if (!(fabs(det) <= real(1.0e-16))) {  // rsla.cu:54-59
+        const real r = real(1) / det;
         m00 = (yy * zz - yz * yz) * r;
         m01 = (xz * yz - xy * zz) * r;
         m02 = (xy * yz - xz * yy) * r;
@@ -395,17 +405,17 @@ __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int 
         m21 = (xy * xz - xx * yz) * r;
         m22 = (xx * yy - xy * xy) * r;
     }
-    double T0, vx0, vy0, vz0;
+    real T0, vx0, vy0, vz0;
     ld_rec4(FS, p.ff, T0, vx0, vy0, vz0);
-    double gTx = 0.0, gTy = 0.0, gTz = 0.0, gxx = 0.0, gxy = 0.0, gxz = 0.0;
-    double gyx = 0.0, gyy = 0.0, gyz = 0.0, gzx = 0.0, gzy = 0.0, gzz = 0.0;
+    real gTx = 0.0, gTy = 0.0, gTz = 0.0, gxx = 0.0, gxy = 0.0, gxz = 0.0;
+    real gyx = 0.0, gyy = 0.0, gyz = 0.0, gzx = 0.0, gzy = 0.0, gzz = 0.0;
     visit_cloud(B, dir, p, [&](int kind, int idx) {
-        double dx, dy, dz, w2;
+        real dx, dy, dz, w2;
         cloud_offset(B, dir, p, fx, fy, fz, kind, idx, dx, dy, dz, w2);
-        const double wx = w2 * (m00 * dx + m01 * dy + m02 * dz);
-        const double wy = w2 * (m10 * dx + m11 * dy + m12 * dz);
-        const double wz = w2 * (m20 * dx + m21 * dy + m22 * dz);
-        double T, vx, vy, vz;
+        const real wx = w2 * (m00 * dx + m01 * dy + m02 * dz);
+        const real wy = w2 * (m10 * dx + m11 * dy + m12 * dz);
+        const real wz = w2 * (m20 * dx + m21 * dy + m22 * dz);
+        real T, vx, vy, vz;
         if (kind == 1) {
             ld_rec4(FS, idx, T, vx, vy, vz);
         } else {
@@ -415,7 +425,7 @@ __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int 
             vy = pt.w.vy;
             vz = pt.w.vz;
         }
-        double dq = T - T0;
+        real dq = T - T0;
         gTx += wx * dq; gTy += wy * dq; gTz += wz * dq;
         dq = vx - vx0;
         gxx += wx * dq; gxy += wy * dq; gxz += wz * dq;
@@ -426,20 +436,21 @@ __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int 
     });
     // Transport coefficients at the face temperature (gas.cu:95-100), stresses and heat
     // flux (block.cu:134-161).
-    const double mu = kMuRef * sqrt(T0 / kTRef) * (T0 / kTRef) * (kTRef + kSuth) / (T0 + kSuth);
-    const double kcond = kCp * mu / kPrandtl;
-    const double lmbda = -2.0 / 3.0 * mu;
-    const double tau_xx = 2.0 * mu * gxx + lmbda * (gxx + gyy + gzz);
-    const double tau_yy = 2.0 * mu * gyy + lmbda * (gxx + gyy + gzz);
-    const double tau_zz = 2.0 * mu * gzz + lmbda * (gxx + gyy + gzz);
-    const double tau_xy = mu * (gxy + gyx);
-    const double tau_xz = mu * (gxz + gzx);
-    const double tau_yz = mu * (gyz + gzy);
-    const double qx = kcond * gTx, qy = kcond * gTy, qz = kcond * gTz;
-    const double* gf = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
+    const real mu = kMuRef * sqrt(T0 / kTRef) * (T0 / kTRef) * (kTRef + kSuth) / (T0 + kSuth);
+    const real kcond = kCp * mu / kPrandtl;
+    const real two = real(2);
+    const real lmbda = -two / real(3) * mu;
+    const real tau_xx = two * mu * gxx + lmbda * (gxx + gyy + gzz);
+    const real tau_yy = two * mu * gyy + lmbda * (gxx + gyy + gzz);
+    const real tau_zz = two * mu * gzz + lmbda * (gxx + gyy + gzz);
+    const real tau_xy = mu * (gxy + gyx);
+    const real tau_xz = mu * (gxz + gzx);
+    const real tau_yz = mu * (gyz + gzy);
+    const real qx = kcond * gTx, qy = kcond * gTy, qz = kcond * gTz;
+    const real* gf = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
     const int nf = (int)(dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk));
-    const double nx = gf[nf + f], ny = gf[2 * nf + f], nz = gf[3 * nf + f];
-    double* __restrict__ F = B->F;
+    const real nx = gf[nf + f], ny = gf[2 * nf + f], nz = gf[3 * nf + f];
+    real* __restrict__ F = B->F;
     F[nft + p.ff] -= tau_xx * nx + tau_xy * ny + tau_xz * nz;
     F[2 * nft + p.ff] -= tau_xy * nx + tau_yy * ny + tau_yz * nz;
     F[3 * nft + p.ff] -= tau_xz * nx + tau_yz * ny + tau_zz * nz;
@@ -465,15 +476,15 @@ __global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A)
         const int fim = i + (nic + 1) * (j + njc * k);
         const int fjm = i + nic * (j + (njc + 1) * k);
         const int fkm = cidx;
-        const double aim = B->gi[fim], aip = B->gi[fim + 1];
-        const double ajm = B->gj[fjm], ajp = B->gj[fjm + nic];
-        const double akm = B->gk[fkm], akp = B->gk[fkm + nic * njc];
-        const double vinv = B->volinv[cidx];
-        double d[6];
+        const real aim = B->gi[fim], aip = B->gi[fim + 1];
+        const real ajm = B->gj[fjm], ajp = B->gj[fjm + nic];
+        const real akm = B->gk[fkm], akp = B->gk[fkm + nic * njc];
+        const real vinv = B->volinv[cidx];
+        real d[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            const double* Fq = B->F + q * nft;
-            double s = aim * Fq[fim] - aip * Fq[fim + 1];
+            const real* Fq = B->F + q * nft;
+            real s = aim * Fq[fim] - aip * Fq[fim + 1];
             s = s + ajm * Fq[nfi + fjm];
             s = s - ajp * Fq[nfi + fjm + nic];
             s = s + akm * Fq[nfi + nfj + fkm];
@@ -508,21 +519,21 @@ __global__ __launch_bounds__(kChunk) void chemistry_kernel(const BlockDev* block
     const CellChunk ch = cells[blockIdx.x];
     const BlockDev* B = blocks + ch.blk;
     if (!ctl.run[B->inst]) return;  // uniform over the work-group
-    const double factor = ctl.chem[B->inst];
+    const real factor = ctl.chem[B->inst];
     const int cidx = ch.c0 + (int)threadIdx.x;
     unsigned long long sigmin = ~0ull;
     if (cidx < ch.c1) {
         const int n = (int)B->n;
-        double* __restrict__ W = B->W + slot * B->wstride;
+        real* __restrict__ W = B->W + slot * B->wstride;
         Prim w = ld_prim(W, n, cidx);
-        if (factor >= 0.0) {
-            const double T = eos_T(w.e, w.YB);
-            const double YA = 1.0 - w.YB;
+        if (factor >= real(0)) {
+            const real T = eos_T(w.e, w.YB);
+            const real YA = real(1) - w.YB;
             if (T > kTi) {
-                w.YB = 1.0 - YA * factor;
+                w.YB = real(1) - YA * factor;
                 W[2 * n + cidx] = w.YB;
             }
-            double U[6];
+            real U[6];
             encode(w, U);
 #pragma unroll
             for (int q = 0; q < 6; ++q) B->U[q * n + cidx] = U[q];

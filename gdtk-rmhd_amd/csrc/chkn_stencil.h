@@ -8,7 +8,7 @@
 
 namespace chkn {
 
-__device__ __forceinline__ Prim ld_prim(const double* __restrict__ W, int n, int c)
+__device__ __forceinline__ Prim ld_prim(const real* __restrict__ W, int n, int c)
 {
     Prim p;
     p.rho = W[c];
@@ -20,7 +20,7 @@ __device__ __forceinline__ Prim ld_prim(const double* __restrict__ W, int n, int
     return p;
 }
 
-__device__ __forceinline__ Frame ld_frame(const double* __restrict__ g, int nf, int f)
+__device__ __forceinline__ Frame ld_frame(const real* __restrict__ g, int nf, int f)
 {
     Frame r;
     r.nx = g[1 * nf + f];
@@ -46,7 +46,7 @@ __device__ __forceinline__ int cell_index(int nic, int njc, int dir, int m, int 
 
 // Boundary face (side 0 = minus, 1 = plus) of the line through (a, c) in direction dir.
 __device__ __forceinline__ void boundary_face(const BlockDev* B, int dir, int side, int a, int c,
-                                              const double*& g, int& nf, int& fi)
+                                              const real*& g, int& nf, int& fi)
 {
     if (dir == 0) {
         g = B->gi; nf = (int)B->nfi;
@@ -64,7 +64,7 @@ __device__ __forceinline__ void boundary_face(const BlockDev* B, int dir, int si
 // an out-of-line call would make every caller preserve its live registers across the
 // call ABI, which costs about 30 VGPRs and one wave per SIMD of occupancy.
 // Depth d = -m-1 on the minus side and m-n on the plus side (block.cu:272-316).
-__device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const double* inflow, int win,
+__device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const real* inflow, int win,
                                            const BlockDev* B, int dir, int m, int a, int c)
 {
     int n = dir == 0 ? B->nic : (dir == 1 ? B->njc : B->nkc);
@@ -79,7 +79,7 @@ __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const double*
         return ld_prim(O->W + win * O->wstride, (int)O->n, cell_index(O->nic, O->njc, dir, mo, a, c));
     }
     if (code == 4) {  // inflow: the configured flow state (bcs.cu:257-319)
-        const double* s = inflow + 6 * B->bc_arg[bf];
+        const real* s = inflow + 6 * B->bc_arg[bf];
         Prim p;
         p.rho = s[0]; p.e = s[1]; p.YB = s[2]; p.vx = s[3]; p.vy = s[4]; p.vz = s[5];
         return p;
@@ -87,7 +87,7 @@ __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const double*
     if (code == 6) {  // inflow function: states at the ghost positions (bcs.cu:355-432)
         const int na = dir == 0 ? B->njc : B->nic;
         const int nt = na * (dir == 2 ? B->njc : B->nkc);
-        const double* s = B->side[bf] + (long long)depth * 9 * nt + (a + na * c);
+        const real* s = B->side[bf] + (long long)depth * 9 * nt + (a + na * c);
         Prim p;
         p.rho = s[0]; p.e = s[nt]; p.YB = s[2 * nt]; p.vx = s[3 * nt]; p.vy = s[4 * nt]; p.vz = s[5 * nt];
         return p;
@@ -96,9 +96,20 @@ __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const double*
                       cell_index(B->nic, B->njc, dir, side ? n - 1 : 0, a, c));
     if (code == 5) return in;  // outflow copies the adjacent interior cell (bcs.cu:435-503)
     // Walls: both ghost layers reflect the adjacent interior cell (bcs.cu:16-132).
-    const double* g; int nf, fi;
+    const real* g; int nf, fi;
     boundary_face(B, dir, side, a, c, g, nf, fi);
     return reflect(in, ld_frame(g, nf, fi));
+}
+
+// Bit pattern of a non-negative real as an unsigned 64-bit integer: non-negative values
+// order as their patterns, so integer minima give the minimum value.
+__device__ __forceinline__ unsigned long long order_bits(double x)
+{
+    return (unsigned long long)__double_as_longlong(x);
+}
+__device__ __forceinline__ unsigned long long order_bits(float x)
+{
+    return (unsigned long long)__float_as_uint(x);
 }
 
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
@@ -114,12 +125,12 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 // d in L, stage 2 replaces L by L/6 + d/6, stage 3 writes the new U.  Decodes the stage
 // result (flow.cu:245-268) into W slot wout and returns the decode's bad-cell flag.
 template <int STAGE>
-__device__ __forceinline__ int stage_update(const BlockDev* B, int n, int cidx, double dt,
-                                            const double d[6], int wout, Prim& w)
+__device__ __forceinline__ int stage_update(const BlockDev* B, int n, int cidx, real dt,
+                                            const real d[6], int wout, Prim& w)
 {
-    double* __restrict__ U = B->U;
-    double* __restrict__ Lr = B->L;
-    double Un[6];
+    real* __restrict__ U = B->U;
+    real* __restrict__ Lr = B->L;
+    real Un[6];
     if (STAGE == 1) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -129,8 +140,8 @@ __device__ __forceinline__ int stage_update(const BlockDev* B, int n, int cidx, 
     } else if (STAGE == 2) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            const double l0 = Lr[q * n + cidx];
-            Un[q] = U[q * n + cidx] + 0.25 * dt * (l0 + d[q]);
+            const real l0 = Lr[q * n + cidx];
+            Un[q] = U[q * n + cidx] + real(0.25) * dt * (l0 + d[q]);
             Lr[q * n + cidx] = kOneSixth * l0 + kOneSixth * d[q];
         }
     } else {
@@ -141,7 +152,7 @@ __device__ __forceinline__ int stage_update(const BlockDev* B, int n, int cidx, 
         }
     }
     const int flag = decode(Un, w);
-    double* __restrict__ Wout = B->W + wout * B->wstride;
+    real* __restrict__ Wout = B->W + wout * B->wstride;
     Wout[cidx] = w.rho;
     Wout[n + cidx] = w.e;
     Wout[2 * n + cidx] = w.YB;
@@ -152,19 +163,19 @@ __device__ __forceinline__ int stage_update(const BlockDev* B, int n, int cidx, 
 }
 
 // Bits of the cfl-free signal time of cell (i, j, k) = cidx in state w (cell.cu:155-164);
-// positive doubles order as their bit patterns, so integer minima give the minimum time.
+// non-negative reals order as their bit patterns, so integer minima give the minimum time.
 __device__ __forceinline__ unsigned long long signal_bits(const BlockDev* B, int n, int i, int j,
                                                          int k, int cidx, const Prim& w)
 {
     const int nic = B->nic, njc = B->njc;
-    const double T = eos_T(w.e, w.YB);
-    const double sa = eos_a(T);
+    const real T = eos_T(w.e, w.YB);
+    const real sa = eos_a(T);
     const Frame fi_ = ld_frame(B->gi, (int)B->nfi, i + (nic + 1) * (j + njc * k));
     const Frame fj_ = ld_frame(B->gj, (int)B->nfj, i + nic * (j + (njc + 1) * k));
     const Frame fk_ = ld_frame(B->gk, (int)B->nfk, cidx);
-    const double sig = local_signal(w, sa, B->len[cidx], B->len[n + cidx], B->len[2 * n + cidx],
+    const real sig = local_signal(w, sa, B->len[cidx], B->len[n + cidx], B->len[2 * n + cidx],
                                     fi_, fj_, fk_);
-    return (unsigned long long)__double_as_longlong(sig);
+    return order_bits(sig);
 }
 
 }  // namespace chkn

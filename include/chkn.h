@@ -174,6 +174,12 @@ int chkn_reset_stream(chkn_sim* sim);
  * handle's stream (ms), and the number of stage launches it issued. */
 int chkn_last_timing(chkn_sim* sim, double* ms, long long* launches);
 
+/* Bytes of the build's number type: 8, or 4 in the FP32 flavours (libchkn_f32.so,
+ * libchkn_ab_f32.so; the reference's FLOAT_NUMBERS build, number.cu:10-27).  The C-ABI takes
+ * and returns doubles in every flavour; an FP32 build rounds inputs to float and widens
+ * outputs exactly. */
+int chkn_real_size(void);
+
 /* ---- Gradients of an unrolled march (no counterpart in the reference, which has no
  * adjoint; these serve differentiable pipelines built on the step).  Flat state arrays
  * concatenate every block's 6*ncells conserved quantities (quantity-major per block, block

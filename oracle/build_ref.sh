@@ -13,9 +13,12 @@ if [ ! -f "$REF/block.cu" ]; then
     exit 0
 fi
 mkdir -p "$HERE/_ref"
-# Two gas-model flavours, as the reference's makefile builds them (GAS_MODEL).
-for flavour in IDEAL_AIR:chkn_ref AB_REACTING_GAS:chkn_ref_ab; do
-    /opt/rocm/bin/hipcc -x hip --offload-host-only -std=c++17 -O2 -fopenmp -D"${flavour%%:*}" \
+# The gas-model flavours (GAS_MODEL), each with double and float numbers (FLOAT_NUMBERS),
+# as the reference's makefile builds them.
+for flavour in "IDEAL_AIR:chkn_ref" "AB_REACTING_GAS:chkn_ref_ab" \
+               "IDEAL_AIR -DFLOAT_NUMBERS:chkn_ref_f32" "AB_REACTING_GAS -DFLOAT_NUMBERS:chkn_ref_ab_f32"; do
+    # shellcheck disable=SC2086
+    /opt/rocm/bin/hipcc -x hip --offload-host-only -std=c++17 -O2 -fopenmp -D${flavour%%:*} \
         -ffp-contract=off -Wno-deprecated-literal-operator -w \
         -I"$REF" "$HERE/ref_harness.cpp" -o "$HERE/_ref/${flavour##*:}" -lz
     echo "built $HERE/_ref/${flavour##*:}"

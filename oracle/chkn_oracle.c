@@ -2,8 +2,10 @@
  *
  * TEST INFRASTRUCTURE ONLY (see chkn_oracle.h).  The product never links this file.
  *
- * Plain C99, FP64, built with -ffp-contract=off so that every a*b+c is two roundings, as in
- * the reference's gcc -O2 x86-64 build.  Every expression keeps the reference's operand
+ * Plain C99 in the reference's number type (FP64, or FP32 for liborc_f32.so), built with
+ * -ffp-contract=off so that every a*b+c is two roundings, as in the reference's gcc -O2
+ * x86-64 build.  Where the reference's FP32 build evaluates an expression in double (a
+ * double literal or argument), this code does the same with an explicit cast.  Every expression keeps the reference's operand
  * order and association; comments give the reference file:line each piece restates
  * (paths relative to the reference's src/chicken/).
  *
@@ -18,6 +20,20 @@
 
 #include <float.h>
 #include <math.h>
+#include <tgmath.h> /* sqrt, fabs, fmin, fmax, exp of the real type */
+#ifdef I
+#undef I
+#endif
+
+/* The number type (number.cu:10-45): double, or float in the FP32 build (ORC_FP32, the
+ * reference's FLOAT_NUMBERS).  Literals are the reference's number constants. */
+#ifdef ORC_FP32
+typedef float real;
+#define REAL_MAX FLT_MAX
+#else
+typedef double real;
+#define REAL_MAX DBL_MAX
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -25,24 +41,25 @@
 /* ------------------------------------------------------------------------------------ */
 /* Constants: number.cu:29-45, gas.cu:21-49 (ideal air, the IDEAL_AIR build).            */
 
-static const double zero = 0.0, half = 0.5, one_quarter = 0.25;
-static const double one_eighth = 0.125, one = 1.0, two = 2.0, three = 3.0, four = 4.0;
-static const double six = 6.0, seven = 7.0, eight = 8.0;
-#define ONE_SIXTH (1.0 / 6.0)
-#define FOUR_SIXTH (4.0 / 6.0)
-#define ONE_TWELFTH (1.0 / 12.0)
+static const real zero = 0.0, half = 0.5, one_quarter = 0.25;
+static const real one_eighth = 0.125, one = 1.0, two = 2.0, three = 3.0, four = 4.0;
+static const real six = 6.0, seven = 7.0, eight = 8.0;
+#define ONE_SIXTH ((real)1 / (real)6)
+#define FOUR_SIXTH ((real)4 / (real)6)
+#define ONE_TWELFTH ((real)1 / (real)12)
 
 /* gas.cu:20-48: the gas model is chosen when the oracle is built, as the reference's is. */
 #ifdef ORC_AB_REACTING_GAS
-static const double GAS_G = 6.0 / 5.0;
-static const double GAS_R = 287.0;
-static const double GAS_Q = 300000.0;
+static const real GAS_G = (real)(6.0 / 5.0);
+static const real GAS_R = (real)287.0;
+static const real GAS_Q = (real)300000.0;
 #else
-static const double GAS_G = 1.4;
-static const double GAS_R = 287.1;
-static const double GAS_Q = 0.0;
+static const real GAS_G = (real)1.4;
+static const real GAS_R = (real)287.1;
+static const real GAS_Q = (real)0.0;
 #endif
-#define GAS_CV (GAS_R / (GAS_G - 1.0))
+/* gas.cu:40: Cv = R / (g - 1.0), the subtraction and division in double. */
+#define GAS_CV ((real)((double)GAS_R / ((double)GAS_G - 1.0)))
 
 static char g_err[512];
 const char* orc_last_error(void) { return g_err; }
@@ -51,23 +68,23 @@ static void set_err(const char* m) { snprintf(g_err, sizeof g_err, "%s", m); }
 /* ------------------------------------------------------------------------------------ */
 /* Types.                                                                                 */
 
-typedef struct { double x, y, z; } V3;
-typedef struct { double rho, e, p, T, YB, a; V3 vel; } FS; /* FlowState, flow.cu:38-40 */
+typedef struct { real x, y, z; } V3;
+typedef struct { real rho, e, p, T, YB, a; V3 vel; } FS; /* FlowState, flow.cu:38-40 */
 
 typedef struct {
-    V3 pos; double vol, iL, jL, kL;
+    V3 pos; real vol, iL, jL, kL;
     FS fs;
     int face[6];
 } Cell;
 
 typedef struct {
-    V3 pos, n, t1, t2; double area;
-    double F[6];
+    V3 pos, n, t1, t2; real area;
+    real F[6];
     int L1, L0, R0, R1;
     FS fs; /* face-average state, face.cu:373 */
     /* viscous: boundary code (-1 interior), cloud of cells and faces, LSQ weights */
     int bcode, bfun, nc, nf, cc[2], fcl[8];
-    double TWall, wx[10], wy[10], wz[10];
+    real TWall, wx[10], wy[10], wz[10];
 } Face;
 
 typedef struct {
@@ -75,19 +92,19 @@ typedef struct {
     int ncells, n0c[6], n1c[6], nghost[6], first_ghost[6], ntot;
     int nif, njf, nkf, nfaces;
     int bc[6], bcfs[6], bcfun[6];
-    double twall[6];
-    double* twf[6]; /* TWall per boundary face, or NULL */
+    real twall[6];
+    real* twf[6]; /* TWall per boundary face, or NULL */
     Cell* cells;
     Face* faces;
     V3* vtx;
-    double* Q;  /* 2 levels x ncells x 6, cell-major (Q[level*ncells+i][j]) */
-    double* dQ; /* 3 levels */
+    real* Q;  /* 2 levels x ncells x 6, cell-major (Q[level*ncells+i][j]) */
+    real* dQ; /* 3 levels */
 } Blk;
 
 /* spline.cu:20-109, 12 knots. */
 #define ORC_BL_N 11
-typedef struct { double xs[ORC_BL_N + 1], ys[ORC_BL_N + 1], a[ORC_BL_N], b[ORC_BL_N], c[ORC_BL_N]; } Spline;
-typedef struct { double p_e; Spline T, velx; int set; } BLProfile; /* config.cu:33-93 */
+typedef struct { real xs[ORC_BL_N + 1], ys[ORC_BL_N + 1], a[ORC_BL_N], b[ORC_BL_N], c[ORC_BL_N]; } Spline;
+typedef struct { real p_e; Spline T, velx; int set; } BLProfile; /* config.cu:33-93 */
 
 struct orc_sim {
     BLProfile bl;
@@ -97,18 +114,18 @@ struct orc_sim {
     int nfs;
     FS* inflow;
     int x_order, flux_calc, cfl_count, max_step, viscous, reacting;
-    double max_time, dt_init;
+    real max_time, dt_init;
     int n_cfl;
-    double* cfl_t;
-    double* cfl_v;
-    double t, dt;
+    real* cfl_t;
+    real* cfl_v;
+    real t, dt;
     int step;
 };
 
 /* ------------------------------------------------------------------------------------ */
 /* Vector3 (vector3.cu).                                                                  */
 
-static double dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; } /* :72-75 */
+static real dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; } /* :72-75 */
 
 /* vector3.cu:97-109 */
 static V3 to_local(V3 v, V3 n, V3 t1, V3 t2)
@@ -129,10 +146,10 @@ static V3 to_global(V3 v, V3 n, V3 t1, V3 t2)
 /* vector3.cu:77-93 */
 static V3 normalize(V3 v)
 {
-    double m = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    real m = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
     if (m > zero) { v.x /= m; v.y /= m; v.z /= m; }
     else { v.x = v.y = v.z = zero; }
-    const double small = 1.0e-30;
+    const real small = (real)1.0e-30;
     if (fabs(v.x) < small) v.x = zero;
     if (fabs(v.y) < small) v.y = zero;
     if (fabs(v.z) < small) v.z = zero;
@@ -150,8 +167,8 @@ static V3 cross_v(V3 v1, V3 v2)
 }
 
 /* vector3.cu:179-187, cross_product(a, b, c) */
-static void cross_c(double ax, double ay, double az, double bx, double by, double bz,
-                    double* cx, double* cy, double* cz)
+static void cross_c(real ax, real ay, real az, real bx, real by, real bz,
+                    real* cx, real* cy, real* cz)
 {
     *cx = ay * bz - az * by;
     *cy = az * bx - ax * bz;
@@ -160,22 +177,22 @@ static void cross_c(double ax, double ay, double az, double bx, double by, doubl
 
 /* vector3.cu:196-245 */
 static void quad_properties(V3 p0, V3 p1, V3 p2, V3 p3, V3* c, V3* n, V3* t1, V3* t2,
-                            double* area)
+                            real* area)
 {
     c->x = one_quarter * (p0.x + p1.x + p2.x + p3.x);
     c->y = one_quarter * (p0.y + p1.y + p2.y + p3.y);
     c->z = one_quarter * (p0.z + p1.z + p2.z + p3.z);
-    double p01x = p1.x - p0.x + p2.x - p3.x;
-    double p01y = p1.y - p0.y + p2.y - p3.y;
-    double p01z = p1.z - p0.z + p2.z - p3.z;
-    double p03x = p3.x - p0.x + p2.x - p1.x;
-    double p03y = p3.y - p0.y + p2.y - p1.y;
-    double p03z = p3.z - p0.z + p2.z - p1.z;
-    double vx, vy, vz;
+    real p01x = p1.x - p0.x + p2.x - p3.x;
+    real p01y = p1.y - p0.y + p2.y - p3.y;
+    real p01z = p1.z - p0.z + p2.z - p3.z;
+    real p03x = p3.x - p0.x + p2.x - p1.x;
+    real p03y = p3.y - p0.y + p2.y - p1.y;
+    real p03z = p3.z - p0.z + p2.z - p1.z;
+    real vx, vy, vz;
     cross_c(p01x, p01y, p01z, p03x, p03y, p03z, &vx, &vy, &vz);
     vx *= one_quarter; vy *= one_quarter; vz *= one_quarter;
     *area = sqrt(vx * vx + vy * vy + vz * vz);
-    if (*area > 1.0e-20) {
+    if (*area > (real)1.0e-20) {
         n->x = vx / *area; n->y = vy / *area; n->z = vz / *area;
         V3 t = {p01x, p01y, p01z};
         *t1 = normalize(t);
@@ -188,32 +205,32 @@ static void quad_properties(V3 p0, V3 p1, V3 p2, V3 p3, V3* c, V3* n, V3* t1, V3
 }
 
 /* vector3.cu:259-270 */
-static double tet_volume(V3 p0, V3 p1, V3 p2, V3 p3)
+static real tet_volume(V3 p0, V3 p1, V3 p2, V3 p3)
 {
-    double dx01 = p1.x - p0.x, dy01 = p1.y - p0.y, dz01 = p1.z - p0.z;
-    double dx02 = p2.x - p0.x, dy02 = p2.y - p0.y, dz02 = p2.z - p0.z;
-    double cx, cy, cz;
+    real dx01 = p1.x - p0.x, dy01 = p1.y - p0.y, dz01 = p1.z - p0.z;
+    real dx02 = p2.x - p0.x, dy02 = p2.y - p0.y, dz02 = p2.z - p0.z;
+    real cx, cy, cz;
     cross_c(dx01, dy01, dz01, dx02, dy02, dz02, &cx, &cy, &cz);
-    double dx03 = p3.x - p0.x, dy03 = p3.y - p0.y, dz03 = p3.z - p0.z;
+    real dx03 = p3.x - p0.x, dy03 = p3.y - p0.y, dz03 = p3.z - p0.z;
     return (dx03 * cx + dy03 * cy + dz03 * cz) / six;
 }
 
 /* vector3.cu:309-348 with true_centroid = true, as called from hex_cell_properties */
-static void pyramid_props(V3 p0, V3 p1, V3 p2, V3 p3, V3 p4, V3* cen, double* vol)
+static void pyramid_props(V3 p0, V3 p1, V3 p2, V3 p3, V3 p4, V3* cen, real* vol)
 {
     V3 pmB;
     pmB.x = one_quarter * (p0.x + p1.x + p2.x + p3.x);
     pmB.y = one_quarter * (p0.y + p1.y + p2.y + p3.y);
     pmB.z = one_quarter * (p0.z + p1.z + p2.z + p3.z);
     V3 q[4][4] = {{p0, p1, pmB, p4}, {p1, p2, pmB, p4}, {p2, p3, pmB, p4}, {p3, p0, pmB, p4}};
-    double v = zero; V3 m = {zero, zero, zero};
+    real v = zero; V3 m = {zero, zero, zero};
     for (int t = 0; t < 4; ++t) {
         /* tetrahedron_properties, vector3.cu:272-281 */
         V3 tc;
         tc.x = one_quarter * (q[t][0].x + q[t][1].x + q[t][2].x + q[t][3].x);
         tc.y = one_quarter * (q[t][0].y + q[t][1].y + q[t][2].y + q[t][3].y);
         tc.z = one_quarter * (q[t][0].z + q[t][1].z + q[t][2].z + q[t][3].z);
-        double tv = tet_volume(q[t][0], q[t][1], q[t][2], q[t][3]);
+        real tv = tet_volume(q[t][0], q[t][1], q[t][2], q[t][3]);
         v += tv;
         tc.x *= tv; tc.y *= tv; tc.z *= tv;
         m.x += tc.x; m.y += tc.y; m.z += tc.z;
@@ -223,8 +240,8 @@ static void pyramid_props(V3 p0, V3 p1, V3 p2, V3 p3, V3 p4, V3* cen, double* vo
 }
 
 /* vector3.cu:350-433 with true_centroid = false (block.cu:642-646) */
-static void hex_props(const V3 p[8], V3* cen, double* vol, double* iL, double* jL,
-                      double* kL)
+static void hex_props(const V3 p[8], V3* cen, real* vol, real* iL, real* jL,
+                      real* kL)
 {
     cen->x = one_eighth * (p[0].x + p[1].x + p[2].x + p[3].x + p[4].x + p[5].x + p[6].x + p[7].x);
     cen->y = one_eighth * (p[0].y + p[1].y + p[2].y + p[3].y + p[4].y + p[5].y + p[6].y + p[7].y);
@@ -241,7 +258,7 @@ static void hex_props(const V3 p[8], V3* cen, double* vol, double* iL, double* j
     MID4(4, 5, 6, 7, pmT);
     MID4(0, 1, 2, 3, pmB);
 #undef MID4
-    double dx, dy, dz;
+    real dx, dy, dz;
     dx = pmE.x - pmW.x; dy = pmE.y - pmW.y; dz = pmE.z - pmW.z;
     *iL = sqrt(dx * dx + dy * dy + dz * dz);
     dx = pmN.x - pmS.x; dy = pmN.y - pmS.y; dz = pmN.z - pmS.z;
@@ -250,15 +267,15 @@ static void hex_props(const V3 p[8], V3* cen, double* vol, double* iL, double* j
     *kL = sqrt(dx * dx + dy * dy + dz * dz);
     static const int pyr[6][4] = {{6, 7, 3, 2}, {5, 6, 2, 1}, {4, 5, 1, 0},
                                   {7, 4, 0, 3}, {7, 6, 5, 4}, {0, 1, 2, 3}};
-    double v = zero; V3 m = {zero, zero, zero};
+    real v = zero; V3 m = {zero, zero, zero};
     for (int q = 0; q < 6; ++q) {
-        V3 sc; double sv;
+        V3 sc; real sv;
         pyramid_props(p[pyr[q][0]], p[pyr[q][1]], p[pyr[q][2]], p[pyr[q][3]], *cen, &sc, &sv);
         v += sv;
         sc.x *= sv; sc.y *= sv; sc.z *= sv;
         m.x += sc.x; m.y += sc.y; m.z += sc.z;
     }
-    if ((v < zero && fabs(v) < 1.0e-12) || (v >= zero && v < 1.0e-20)) v = zero;
+    if ((v < zero && fabs(v) < (real)1.0e-12) || (v >= zero && v < (real)1.0e-20)) v = zero;
     *vol = v;
 }
 
@@ -275,58 +292,58 @@ static void update_from_pT(FS* s) /* gas.cu:119-126 */
 /* spline.cu:26-73: natural cubic spline; the interior second derivatives solve a
  * tridiagonal system by Gauss-Jordan elimination with partial pivoting (rsla.cu:201-227).
  * Returns -1 when a pivot is below 1e-16. */
-static int spline_set(Spline* sp, const double* x, const double* y)
+static int spline_set(Spline* sp, const real* x, const real* y)
 {
     enum { n = ORC_BL_N, m = ORC_BL_N - 1 };
-    double h[n], A[m][m + 1], sigma[n + 1];
+    real h[n], A[m][m + 1], sigma[n + 1];
     for (int i = 0; i <= n; ++i) { sp->xs[i] = x[i]; sp->ys[i] = y[i]; }
     for (int i = 0; i < n; ++i) h[i] = sp->xs[i + 1] - sp->xs[i];
     for (int r = 0; r < m; ++r) for (int q = 0; q <= m; ++q) A[r][q] = 0.0;
     for (int k = 0; k < m; ++k) {
         int i = k + 1;
         if (k > 0) A[k][k - 1] = h[i - 1];
-        A[k][k] = 2.0 * (h[i - 1] + h[i]);
+        A[k][k] = (real)(2.0 * (double)(h[i - 1] + h[i])); /* double literal */
         if (k < m - 1) A[k][k + 1] = h[i];
-        A[k][m] = 6.0 * ((sp->ys[i + 1] - sp->ys[i]) / h[i] - (sp->ys[i] - sp->ys[i - 1]) / h[i - 1]);
+        A[k][m] = (real)(6.0 * (double)((sp->ys[i + 1] - sp->ys[i]) / h[i] - (sp->ys[i] - sp->ys[i - 1]) / h[i - 1]));
     }
     for (int j = 0; j < m; ++j) {
         int p = j;
         for (int i = j + 1; i < m; ++i) if (fabs(A[i][j]) > fabs(A[p][j])) p = i;
-        if (fabs(A[p][j]) < 1.0e-16) return -1;
-        if (p != j) for (int q = 0; q <= m; ++q) { double t = A[p][q]; A[p][q] = A[j][q]; A[j][q] = t; }
-        double d = A[j][j];
+        if ((double)fabs(A[p][j]) < 1.0e-16) return -1; /* rsla.cu:202,211: a double threshold */
+        if (p != j) for (int q = 0; q <= m; ++q) { real t = A[p][q]; A[p][q] = A[j][q]; A[j][q] = t; }
+        real d = A[j][j];
         for (int q = 0; q <= m; ++q) A[j][q] /= d;
         for (int i = 0; i < m; ++i) {
             if (i == j) continue;
-            double f = A[i][j];
+            real f = A[i][j];
             for (int q = 0; q <= m; ++q) A[i][q] -= f * A[j][q];
         }
     }
     sigma[0] = 0.0; sigma[n] = 0.0;
     for (int i = 0; i < m; ++i) sigma[i + 1] = A[i][m];
     for (int i = 0; i < n; ++i) {
-        sp->a[i] = (sigma[i + 1] - sigma[i]) / (6.0 * h[i]);
-        sp->b[i] = sigma[i] / 2.0;
-        sp->c[i] = (sp->ys[i + 1] - sp->ys[i]) / h[i] - (2.0 * sigma[i] + sigma[i + 1]) * h[i] / 6.0;
+        sp->a[i] = (sigma[i + 1] - sigma[i]) / (six * h[i]);
+        sp->b[i] = sigma[i] / two;
+        sp->c[i] = (sp->ys[i + 1] - sp->ys[i]) / h[i] - (two * sigma[i] + sigma[i + 1]) * h[i] / six;
     }
     return 0;
 }
 
-static double spline_eval(const Spline* sp, double x) /* spline.cu:86-99 */
+static real spline_eval(const Spline* sp, real x) /* spline.cu:86-99 */
 {
     int i = 0;
     if (!(x < sp->xs[0])) {
         i = ORC_BL_N - 1;
         while (x < sp->xs[i] && i > 0) i -= 1;
     }
-    double dx = x - sp->xs[i];
-    return ((sp->a[i] * dx + sp->b[i]) * dx + sp->c[i]) * dx + sp->ys[i];
+    double dx = (double)(x - sp->xs[i]); /* spline.cu:94-95: the piece in double */
+    return (real)((((double)sp->a[i] * dx + (double)sp->b[i]) * dx + (double)sp->c[i]) * dx + (double)sp->ys[i]);
 }
 
 /* config.cu:37-60: p_e, then y, T, velx per sample. */
-static int bl_set(BLProfile* bl, const double* v)
+static int bl_set(BLProfile* bl, const real* v)
 {
-    double ys[ORC_BL_N + 1], Ts[ORC_BL_N + 1], us[ORC_BL_N + 1];
+    real ys[ORC_BL_N + 1], Ts[ORC_BL_N + 1], us[ORC_BL_N + 1];
     bl->p_e = v[0];
     for (int i = 0; i <= ORC_BL_N; ++i) { ys[i] = v[1 + 3 * i]; Ts[i] = v[2 + 3 * i]; us[i] = v[3 + 3 * i]; }
     if (spline_set(&bl->T, ys, Ts) || spline_set(&bl->velx, ys, us)) return -1;
@@ -337,14 +354,14 @@ static int bl_set(BLProfile* bl, const double* v)
 static void update_from_rhoe(FS* s);
 
 /* gas.cu:102-111: the A->B reaction in an isolated blob of gas over dt. */
-static void update_chemistry(FS* s, double dt)
+static void update_chemistry(FS* s, real dt)
 {
 #ifdef ORC_AB_REACTING_GAS
-    const double alpha = 1000.0, Ti = 362.58;
+    const real alpha = (real)1000.0, Ti = (real)362.58;
 #else
-    const double alpha = 0.0, Ti = 0.0;
+    const real alpha = (real)0.0, Ti = (real)0.0;
 #endif
-    double YA = one - s->YB;
+    real YA = one - s->YB;
     if (s->T > Ti) s->YB = one - YA * exp(-alpha * dt);
     update_from_rhoe(s);
 }
@@ -356,27 +373,27 @@ static void update_from_rhoe(FS* s) /* gas.cu:128-135 */
     s->a = sqrt(GAS_G * GAS_R * s->T);
 }
 
-static void encode(const FS* s, double* U) /* flow.cu:227-243 */
+static void encode(const FS* s, real* U) /* flow.cu:227-243 */
 {
-    double rho = s->rho;
+    real rho = s->rho;
     U[0] = rho;
     U[1] = rho * s->vel.x;
     U[2] = rho * s->vel.y;
     U[3] = rho * s->vel.z;
-    double ke = half * (s->vel.x * s->vel.x + s->vel.y * s->vel.y + s->vel.z * s->vel.z);
+    real ke = half * (s->vel.x * s->vel.x + s->vel.y * s->vel.y + s->vel.z * s->vel.z);
     U[4] = rho * (s->e + ke);
     U[5] = rho * s->YB;
 }
 
-static int decode(FS* s, const double* U) /* flow.cu:245-268 */
+static int decode(FS* s, const real* U) /* flow.cu:245-268 */
 {
     for (int i = 0; i < 6; ++i) if (isnan(U[i])) return 1;
-    double rho = U[0];
+    real rho = U[0];
     if (rho <= zero) return 1;
-    double dinv = one / rho;
+    real dinv = one / rho;
     s->vel.x = U[1] * dinv; s->vel.y = U[2] * dinv; s->vel.z = U[3] * dinv;
-    double e = U[4] * dinv;
-    double ke = half * (s->vel.x * s->vel.x + s->vel.y * s->vel.y + s->vel.z * s->vel.z);
+    real e = U[4] * dinv;
+    real ke = half * (s->vel.x * s->vel.x + s->vel.y * s->vel.y + s->vel.z * s->vel.z);
     e -= ke;
     s->rho = rho;
     s->e = e;
@@ -405,7 +422,7 @@ static FS inflow_function(const orc_sim* s, int fn, V3 pos)
     memset(&f, 0, sizeof f);
     if (fn == 2) {
         const BLProfile* bl = &s->bl;
-        double y = pos.y, T, u;
+        real y = pos.y, T, u;
         if (y <= bl->T.xs[0]) { T = bl->T.ys[0]; u = bl->velx.ys[0]; }
         else if (y >= bl->T.xs[ORC_BL_N]) { T = bl->T.ys[ORC_BL_N]; u = bl->velx.ys[ORC_BL_N]; }
         else { T = spline_eval(&bl->T, y); u = spline_eval(&bl->velx, y); }
@@ -422,48 +439,48 @@ static FS inflow_function(const orc_sim* s, int fn, V3 pos)
 /* ------------------------------------------------------------------------------------ */
 /* Face flux (face.cu).                                                                   */
 
-static double van_albada(double a, double b) /* face.cu:72-79 */
+static real van_albada(real a, real b) /* face.cu:72-79 */
 {
-    const double eps = 1.0e-12;
+    const real eps = (real)1.0e-12;
     return (a * b + fabs(a * b) + eps) / (a * a + b * b + eps);
 }
 
 /* face.cu:81-96 */
-static void interp_l2r2(double qL1, double qL0, double qR0, double qR1, double* qL, double* qR)
+static void interp_l2r2(real qL1, real qL0, real qR0, real qR1, real* qL, real* qR)
 {
-    double delLminus = (qL0 - qL1);
-    double del = (qR0 - qL0);
-    double delRplus = (qR1 - qR0);
-    double sL = van_albada(delLminus, del);
-    double sR = van_albada(del, delRplus);
+    real delLminus = (qL0 - qL1);
+    real del = (qR0 - qL0);
+    real delRplus = (qR1 - qR0);
+    real sL = van_albada(delLminus, del);
+    real sR = van_albada(del, delRplus);
     *qL = qL0 + sL * one_eighth * (three * del + delLminus);
     *qR = qR0 - sR * one_eighth * (delRplus + three * del);
 }
 
 /* face.cu:163-265 */
-static void ausmdv(const FS* fsL, const FS* fsR, V3 n, V3 t1, V3 t2, double* F)
+static void ausmdv(const FS* fsL, const FS* fsR, V3 n, V3 t1, V3 t2, real* F)
 {
     V3 velL = to_local(fsL->vel, n, t1, t2);
     V3 velR = to_local(fsR->vel, n, t1, t2);
-    double rhoL = fsL->rho, pL = fsL->p, pLrL = pL / rhoL;
-    double velxL = velL.x, velyL = velL.y, velzL = velL.z;
-    double uL = fsL->e, aL = fsL->a;
-    double keL = half * (velxL * velxL + velyL * velyL + velzL * velzL);
-    double HL = uL + pLrL + keL;
-    double YBL = fsL->YB;
-    double rhoR = fsR->rho, pR = fsR->p, pRrR = pR / rhoR;
-    double velxR = velR.x, velyR = velR.y, velzR = velR.z;
-    double uR = fsR->e, aR = fsR->a;
-    double keR = half * (velxR * velxR + velyR * velyR + velzR * velzR);
-    double HR = uR + pR / rhoR + keR;
-    double YBR = fsR->YB;
-    double alphaL = two * pLrL / (pLrL + pRrR);
-    double alphaR = two * pRrR / (pLrL + pRrR);
-    double am = fmax(aL, aR);
-    double ML = velxL / am;
-    double MR = velxR / am;
-    double pLplus, velxLplus;
-    double dvelxL = half * (velxL + fabs(velxL));
+    real rhoL = fsL->rho, pL = fsL->p, pLrL = pL / rhoL;
+    real velxL = velL.x, velyL = velL.y, velzL = velL.z;
+    real uL = fsL->e, aL = fsL->a;
+    real keL = half * (velxL * velxL + velyL * velyL + velzL * velzL);
+    real HL = uL + pLrL + keL;
+    real YBL = fsL->YB;
+    real rhoR = fsR->rho, pR = fsR->p, pRrR = pR / rhoR;
+    real velxR = velR.x, velyR = velR.y, velzR = velR.z;
+    real uR = fsR->e, aR = fsR->a;
+    real keR = half * (velxR * velxR + velyR * velyR + velzR * velzR);
+    real HR = uR + pR / rhoR + keR;
+    real YBR = fsR->YB;
+    real alphaL = two * pLrL / (pLrL + pRrR);
+    real alphaR = two * pRrR / (pLrL + pRrR);
+    real am = fmax(aL, aR);
+    real ML = velxL / am;
+    real MR = velxR / am;
+    real pLplus, velxLplus;
+    real dvelxL = half * (velxL + fabs(velxL));
     if (fabs(ML) <= one) {
         pLplus = pL * (ML + one) * (ML + one) * (two - ML) * one_quarter;
         velxLplus = alphaL * ((velxL + am) * (velxL + am) / (four * am) - dvelxL) + dvelxL;
@@ -471,8 +488,8 @@ static void ausmdv(const FS* fsL, const FS* fsR, V3 n, V3 t1, V3 t2, double* F)
         pLplus = pL * dvelxL / velxL;
         velxLplus = dvelxL;
     }
-    double pRminus, velxRminus;
-    double dvelxR = half * (velxR - fabs(velxR));
+    real pRminus, velxRminus;
+    real dvelxR = half * (velxR - fabs(velxR));
     if (fabs(MR) <= one) {
         pRminus = pR * (MR - one) * (MR - one) * (two + MR) * one_quarter;
         velxRminus = alphaR * (-(velxR - am) * (velxR - am) / (four * am) - dvelxR) + dvelxR;
@@ -480,40 +497,40 @@ static void ausmdv(const FS* fsL, const FS* fsR, V3 n, V3 t1, V3 t2, double* F)
         pRminus = pR * dvelxR / velxR;
         velxRminus = dvelxR;
     }
-    double massL = velxLplus * rhoL;
-    double massR = velxRminus * rhoR;
-    double mass_half = massL + massR;
-    double p_half = pLplus + pRminus;
-    double dp = pL - pR;
-    const double K_SWITCH = 10.0;
+    real massL = velxLplus * rhoL;
+    real massR = velxRminus * rhoR;
+    real mass_half = massL + massR;
+    real p_half = pLplus + pRminus;
+    real dp = pL - pR;
+    const real K_SWITCH = 10.0;
     dp = K_SWITCH * fabs(dp) / fmin(pL, pR);
-    double s = half * fmin(one, dp);
-    double rvel2_AUSMV = massL * velxL + massR * velxR;
-    double rvel2_AUSMD = half * (mass_half * (velxL + velxR) - fabs(mass_half) * (velxR - velxL));
-    double rvel2_half = (half + s) * rvel2_AUSMV + (half - s) * rvel2_AUSMD;
+    real s = half * fmin(one, dp);
+    real rvel2_AUSMV = massL * velxL + massR * velxR;
+    real rvel2_AUSMD = half * (mass_half * (velxL + velxR) - fabs(mass_half) * (velxR - velxL));
+    real rvel2_half = (half + s) * rvel2_AUSMV + (half - s) * rvel2_AUSMD;
     F[0] = mass_half;
-    double vely = (mass_half >= zero) ? velyL : velyR;
-    double velz = (mass_half >= zero) ? velzL : velzR;
+    real vely = (mass_half >= zero) ? velyL : velyR;
+    real velz = (mass_half >= zero) ? velzL : velzR;
     V3 mom = {rvel2_half + p_half, mass_half * vely, mass_half * velz};
     mom = to_global(mom, n, t1, t2);
     F[1] = mom.x; F[2] = mom.y; F[3] = mom.z;
-    double H = (mass_half >= zero) ? HL : HR;
+    real H = (mass_half >= zero) ? HL : HR;
     F[4] = mass_half * H;
-    double YB = (mass_half >= zero) ? YBL : YBR;
+    real YB = (mass_half >= zero) ? YBL : YBR;
     F[5] = mass_half * YB;
 }
 
 /* face.cu:268-342 */
 static void sbp_asf(const FS* s0, const FS* s1, const FS* s2, const FS* s3, V3 n, V3 t1, V3 t2,
-                    double* F)
+                    real* F)
 {
     const FS* g[4] = {s0, s1, s2, s3};
     V3 vel[4];
     for (int i = 0; i < 4; ++i) vel[i] = to_local(g[i]->vel, n, t1, t2);
-    double v[10][4], w[10][4];
+    real v[10][4], w[10][4];
     for (int i = 0; i < 4; ++i) {
-        double rho = g[i]->rho, p = g[i]->p, e = g[i]->e;
-        double velx = vel[i].x, vely = vel[i].y, velz = vel[i].z;
+        real rho = g[i]->rho, p = g[i]->p, e = g[i]->e;
+        real velx = vel[i].x, vely = vel[i].y, velz = vel[i].z;
         v[0][i] = rho;             w[0][i] = velx;
         v[1][i] = velx * rho;      w[1][i] = velx;
         v[2][i] = vely * rho;      w[2][i] = velx;
@@ -525,7 +542,7 @@ static void sbp_asf(const FS* s0, const FS* s1, const FS* s2, const FS* s3, V3 n
         v[8][i] = p;               w[8][i] = velx;
         v[9][i] = p;               w[9][i] = 1;
     }
-    double f_c[10], f_e[10];
+    real f_c[10], f_e[10];
     for (int j = 0; j < 10; ++j) {
         f_c[j] = ONE_TWELFTH * (-v[j][0] * w[j][0] + seven * v[j][1] * w[j][1]
                                 + seven * v[j][2] * w[j][2] - v[j][3] * w[j][3]);
@@ -533,13 +550,13 @@ static void sbp_asf(const FS* s0, const FS* s1, const FS* s2, const FS* s3, V3 n
                                 + eight * v[j][2] * w[j][1] - v[j][1] * w[j][3]
                                 - v[j][3] * w[j][1]);
     }
-    const double alpha_mass = 1.0, alpha_mom = 0.5, alpha_ie = 0.5, alpha_ke = 0.0, alpha_p = 0.0;
-    double mass_flux = alpha_mass * f_c[0] + (one - alpha_mass) * f_e[0];
+    const real alpha_mass = 1.0, alpha_mom = 0.5, alpha_ie = 0.5, alpha_ke = 0.0, alpha_p = 0.0;
+    real mass_flux = alpha_mass * f_c[0] + (one - alpha_mass) * f_e[0];
     F[0] = mass_flux;
-    double mom_x = alpha_mom * f_c[1] + (one - alpha_mom) * f_e[1]
+    real mom_x = alpha_mom * f_c[1] + (one - alpha_mom) * f_e[1]
                    + (alpha_p * f_c[9] + (one - alpha_p) * f_e[9]);
-    double mom_y = alpha_mom * f_c[2] + (one - alpha_mom) * f_e[2];
-    double mom_z = alpha_mom * f_c[3] + (one - alpha_mom) * f_e[3];
+    real mom_y = alpha_mom * f_c[2] + (one - alpha_mom) * f_e[2];
+    real mom_z = alpha_mom * f_c[3] + (one - alpha_mom) * f_e[3];
     V3 mom = {mom_x, mom_y, mom_z};
     mom = to_global(mom, n, t1, t2);
     F[1] = mom.x; F[2] = mom.y; F[3] = mom.z;
@@ -615,7 +632,7 @@ static int is_wall(int code)
  * only its interior cell; another boundary face also its first ghost cell. */
 /* TWall of boundary face t of a fixed-T side: the side's value, overwritten by the file's
  * per-face values when given (block.cu:566-611). */
-static double side_twall(const Blk* b, int side, int t) { return b->twf[side] ? b->twf[side][t] : b->twall[side]; }
+static real side_twall(const Blk* b, int side, int t) { return b->twf[side] ? b->twf[side][t] : b->twall[side]; }
 
 static void set_cloud(Blk* b, Face* f, int side_bc, int c_minus, int c_plus, const int* fl, int nfl)
 {
@@ -719,8 +736,8 @@ static int configure(Blk* b) /* config.cu:152-185, block.cu:194-564 */
     b->faces = (Face*)calloc((size_t)b->nfaces, sizeof(Face));
     b->vtx = (V3*)calloc((size_t)(nic + 1) * (njc + 1) * (nkc + 1), sizeof(V3));
     if (b->active) {
-        b->Q = (double*)calloc((size_t)b->ncells * 2 * 6, sizeof(double));
-        b->dQ = (double*)calloc((size_t)b->ncells * 3 * 6, sizeof(double));
+        b->Q = (real*)calloc((size_t)b->ncells * 2 * 6, sizeof(real));
+        b->dQ = (real*)calloc((size_t)b->ncells * 3 * 6, sizeof(real));
     }
     if (!b->cells || !b->faces || !b->vtx || (b->active && (!b->Q || !b->dQ))) {
         set_err("allocation failed"); return -1;
@@ -819,7 +836,7 @@ static void compute_geometry(Blk* b)
                     V3 d = {f->pos.x - c0->pos.x, f->pos.y - c0->pos.y, f->pos.z - c0->pos.z};
                     g0->pos.x = f->pos.x + d.x; g0->pos.y = f->pos.y + d.y; g0->pos.z = f->pos.z + d.z;
                     g1->iL = c0->iL; g1->jL = c0->jL; g1->kL = c0->kL;
-                    d.x *= 3.0; d.y *= 3.0; d.z *= 3.0;
+                    d.x *= three; d.y *= three; d.z *= three; /* Vector3::mul(number) */
                     g1->pos.x = f->pos.x + d.x; g1->pos.y = f->pos.y + d.y; g1->pos.z = f->pos.z + d.z;
                 }
         }
@@ -918,24 +935,24 @@ static void apply_all_bcs(orc_sim* s)
 /* Cell-level operations (cell.cu, block.cu:978-1172).                                    */
 
 /* cell.cu:155-164 */
-static double local_dt(const Blk* b, const Cell* c, double cfl)
+static real local_dt(const Blk* b, const Cell* c, real cfl)
 {
     V3 in = b->faces[c->face[IM]].n, jn = b->faces[c->face[JM]].n, kn = b->faces[c->face[KM]].n;
-    double is = c->iL / (fabs(dot3(c->fs.vel, in)) + c->fs.a);
-    double js = c->jL / (fabs(dot3(c->fs.vel, jn)) + c->fs.a);
-    double ks = c->kL / (fabs(dot3(c->fs.vel, kn)) + c->fs.a);
+    real is = c->iL / (fabs(dot3(c->fs.vel, in)) + c->fs.a);
+    real js = c->jL / (fabs(dot3(c->fs.vel, jn)) + c->fs.a);
+    real ks = c->kL / (fabs(dot3(c->fs.vel, kn)) + c->fs.a);
     return cfl * fmin(fmin(is, js), ks);
 }
 
 /* cell.cu:186-217 */
-static void eval_dudt(const Blk* b, const Cell* c, double* d)
+static void eval_dudt(const Blk* b, const Cell* c, real* d)
 {
-    double vol_inv = one / c->vol;
+    real vol_inv = one / c->vol;
     const Face* fim = &b->faces[c->face[IM]]; const Face* fip = &b->faces[c->face[IP]];
     const Face* fjm = &b->faces[c->face[JM]]; const Face* fjp = &b->faces[c->face[JP]];
     const Face* fkm = &b->faces[c->face[KM]]; const Face* fkp = &b->faces[c->face[KP]];
     for (int i = 0; i < 6; ++i) {
-        double si = fim->area * fim->F[i] - fip->area * fip->F[i]
+        real si = fim->area * fim->F[i] - fip->area * fip->F[i]
                     + fjm->area * fjm->F[i] - fjp->area * fjp->F[i]
                     + fkm->area * fkm->F[i] - fkp->area * fkp->F[i];
         d[i] = vol_inv * si;
@@ -953,16 +970,16 @@ static void block_fluxes(orc_sim* s, Blk* b)
 }
 
 /* block.cu:1090-1160 */
-static int block_update(Blk* b, int stage, double dt)
+static int block_update(Blk* b, int stage, real dt)
 {
     int bad = 0, n = b->ncells;
     for (int i = 0; i < n; ++i) {
         Cell* c = &b->cells[i];
-        double* d0 = &b->dQ[(size_t)i * 6];
-        double* d1 = &b->dQ[((size_t)n + i) * 6];
-        double* d2 = &b->dQ[((size_t)2 * n + i) * 6];
-        double* U0 = &b->Q[(size_t)i * 6];
-        double* U1 = &b->Q[((size_t)n + i) * 6];
+        real* d0 = &b->dQ[(size_t)i * 6];
+        real* d1 = &b->dQ[((size_t)n + i) * 6];
+        real* d2 = &b->dQ[((size_t)2 * n + i) * 6];
+        real* U0 = &b->Q[(size_t)i * 6];
+        real* U1 = &b->Q[((size_t)n + i) * 6];
         if (stage == 1) {
             eval_dudt(b, c, d0);
             for (int j = 0; j < 6; ++j) U1[j] = U0[j] + dt * d0[j];
@@ -980,13 +997,13 @@ static int block_update(Blk* b, int stage, double dt)
 }
 
 /* rsla.cu:53-72: 3x3 inverse by cofactors; returns -1 when |det| <= very_small_value. */
-static int m_inverse3(double c[3][3], double ci[3][3], double very_small_value)
+static int m_inverse3(real c[3][3], real ci[3][3], real very_small_value)
 {
-    double det = c[0][0] * (c[1][1] * c[2][2] - c[1][2] * c[2][1])
+    real det = c[0][0] * (c[1][1] * c[2][2] - c[1][2] * c[2][1])
                  - c[0][1] * (c[1][0] * c[2][2] - c[1][2] * c[2][0])
                  + c[0][2] * (c[1][0] * c[2][1] - c[1][1] * c[2][0]);
     if (fabs(det) <= very_small_value) return -1;
-    double one_over_det = one / det;
+    real one_over_det = one / det;
     ci[0][0] = (c[1][1] * c[2][2] - c[1][2] * c[2][1]) * one_over_det;
     ci[0][1] = (c[0][2] * c[2][1] - c[0][1] * c[2][2]) * one_over_det;
     ci[0][2] = (c[0][1] * c[1][2] - c[0][2] * c[1][1]) * one_over_det;
@@ -1008,15 +1025,15 @@ static void setup_lsq(Blk* b, Face* f)
     int n = 0;
     for (int q = 0; q < f->nc; ++q) pos[n++] = b->cells[f->cc[q]].pos;
     for (int q = 0; q < f->nf; ++q) pos[n++] = b->faces[f->fcl[q]].pos;
-    double w2[10], dx[10], dy[10], dz[10];
-    double x0 = f->pos.x, y0 = f->pos.y, z0 = f->pos.z;
+    real w2[10], dx[10], dy[10], dz[10];
+    real x0 = f->pos.x, y0 = f->pos.y, z0 = f->pos.z;
     for (int i = 0; i < n; ++i) {
         dx[i] = pos[i].x - x0;
         dy[i] = pos[i].y - y0;
         dz[i] = pos[i].z - z0;
-        w2[i] = 1.0 / (dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i]);
+        w2[i] = (real)(1.0 / (double)(dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i])); /* block.cu:54, in double */
     }
-    double xx = zero, xy = zero, xz = zero, yy = zero, yz = zero, zz = zero;
+    real xx = zero, xy = zero, xz = zero, yy = zero, yz = zero, zz = zero;
     for (int i = 0; i < n; ++i) {
         xx += w2[i] * dx[i] * dx[i];
         xy += w2[i] * dx[i] * dy[i];
@@ -1025,9 +1042,9 @@ static void setup_lsq(Blk* b, Face* f)
         yz += w2[i] * dy[i] * dz[i];
         zz += w2[i] * dz[i] * dz[i];
     }
-    double m[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}}, mi[3][3];
+    real m[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}}, mi[3][3];
     for (int i = 0; i < 10; ++i) f->wx[i] = f->wy[i] = f->wz[i] = zero;
-    if (m_inverse3(m, mi, 1.0e-16)) return;
+    if (m_inverse3(m, mi, (real)1.0e-16)) return;
     for (int i = 0; i < n; ++i) {
         f->wx[i] = w2[i] * (mi[0][0] * dx[i] + mi[0][1] * dy[i] + mi[0][2] * dz[i]);
         f->wy[i] = w2[i] * (mi[1][0] * dx[i] + mi[1][1] * dy[i] + mi[1][2] * dz[i]);
@@ -1051,10 +1068,10 @@ static void viscous_bc(const orc_sim* s, Face* f)
 }
 
 /* gas.cu:95-100, Sutherland's law for air. */
-static void trans_coeffs(double T, double* mu, double* k)
+static void trans_coeffs(real T, real* mu, real* k)
 {
-    const double mu_ref = 1.716e-5, T_ref = 273.0, S = 111.0, Prandtl = 0.72;
-    const double Cp = GAS_CV + GAS_R;
+    const real mu_ref = (real)1.716e-5, T_ref = (real)273.0, S = (real)111.0, Prandtl = (real)0.72;
+    const real Cp = GAS_CV + GAS_R;
     *mu = mu_ref * sqrt(T / T_ref) * (T / T_ref) * (T_ref + S) / (T + S);
     *k = Cp * *mu / Prandtl;
 }
@@ -1068,9 +1085,9 @@ static void add_viscous(Blk* b, Face* f)
     for (int q = 0; q < f->nf; ++q) cl[n++] = b->faces[f->fcl[q]].fs;
     FS fs0 = f->fs;
     V3 gT = {zero, zero, zero}, gx = {zero, zero, zero}, gy = {zero, zero, zero}, gz = {zero, zero, zero};
-    double dq = zero;
+    real dq = zero;
     for (int i = 0; i < n; ++i) {
-        double wx = f->wx[i], wy = f->wy[i], wz = f->wz[i];
+        real wx = f->wx[i], wy = f->wy[i], wz = f->wz[i];
         dq = cl[i].T - fs0.T;
         gT.x += wx * dq; gT.y += wy * dq; gT.z += wz * dq;
         dq = cl[i].vel.x - fs0.vel.x;
@@ -1080,16 +1097,16 @@ static void add_viscous(Blk* b, Face* f)
         dq = cl[i].vel.z - fs0.vel.z;
         gz.x += wx * dq; gz.y += wy * dq; gz.z += wz * dq;
     }
-    double mu, k;
+    real mu, k;
     trans_coeffs(fs0.T, &mu, &k);
-    double lmbda = -two / three * mu;
-    double tau_xx = two * mu * gx.x + lmbda * (gx.x + gy.y + gz.z);
-    double tau_yy = two * mu * gy.y + lmbda * (gx.x + gy.y + gz.z);
-    double tau_zz = two * mu * gz.z + lmbda * (gx.x + gy.y + gz.z);
-    double tau_xy = mu * (gx.y + gy.x);
-    double tau_xz = mu * (gx.z + gz.x);
-    double tau_yz = mu * (gy.z + gz.y);
-    double qx = k * gT.x, qy = k * gT.y, qz = k * gT.z;
+    real lmbda = -two / three * mu;
+    real tau_xx = two * mu * gx.x + lmbda * (gx.x + gy.y + gz.z);
+    real tau_yy = two * mu * gy.y + lmbda * (gx.x + gy.y + gz.z);
+    real tau_zz = two * mu * gz.z + lmbda * (gx.x + gy.y + gz.z);
+    real tau_xy = mu * (gx.y + gy.x);
+    real tau_xz = mu * (gx.z + gz.x);
+    real tau_yz = mu * (gy.z + gz.y);
+    real qx = k * gT.x, qy = k * gT.y, qz = k * gT.z;
     V3 nn = f->n;
     f->F[1] -= tau_xx * nn.x + tau_xy * nn.y + tau_xz * nn.z;
     f->F[2] -= tau_xy * nn.x + tau_yy * nn.y + tau_yz * nn.z;
@@ -1100,7 +1117,7 @@ static void add_viscous(Blk* b, Face* f)
 }
 
 /* config.cu:263-269 */
-static double schedule_value(const orc_sim* s, double t)
+static real schedule_value(const orc_sim* s, real t)
 {
     int i = s->n_cfl - 1;
     while (i > 0 && t < s->cfl_t[i]) i--;
@@ -1125,21 +1142,27 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
     s->inflow = (FS*)calloc((size_t)(n_flow_states > 0 ? n_flow_states : 1), sizeof(FS));
     for (int i = 0; i < n_flow_states; ++i) { /* config.cu:398-406 */
         FS* f = &s->inflow[i];
-        f->p = flow_states[6 * i + 0]; f->T = flow_states[6 * i + 1]; f->YB = flow_states[6 * i + 2];
+        f->p = (real)flow_states[6 * i + 0]; f->T = (real)flow_states[6 * i + 1];
+        f->YB = (real)flow_states[6 * i + 2];
         update_from_pT(f);
-        f->vel.x = flow_states[6 * i + 3]; f->vel.y = flow_states[6 * i + 4]; f->vel.z = flow_states[6 * i + 5];
+        f->vel.x = (real)flow_states[6 * i + 3]; f->vel.y = (real)flow_states[6 * i + 4];
+        f->vel.z = (real)flow_states[6 * i + 5];
     }
     s->x_order = p->x_order; s->flux_calc = p->flux_calc; s->cfl_count = p->cfl_count;
     s->viscous = p->viscous;
     s->reacting = p->reacting;
-    s->max_step = p->max_step; s->max_time = p->max_time; s->dt_init = p->dt_init;
+    /* Configuration values are held in the number type (config.cu:494-504). */
+    s->max_step = p->max_step; s->max_time = (real)p->max_time; s->dt_init = (real)p->dt_init;
     s->n_cfl = p->n_cfl;
-    s->cfl_t = (double*)malloc(sizeof(double) * (size_t)p->n_cfl);
-    s->cfl_v = (double*)malloc(sizeof(double) * (size_t)p->n_cfl);
-    memcpy(s->cfl_t, p->cfl_times, sizeof(double) * (size_t)p->n_cfl);
-    memcpy(s->cfl_v, p->cfl_values, sizeof(double) * (size_t)p->n_cfl);
-    if (p->bl_profile && bl_set(&s->bl, p->bl_profile)) {
-        set_err("Gauss-Jordan elimination failed while solving for sigma."); orc_destroy(s); return NULL;
+    s->cfl_t = (real*)malloc(sizeof(real) * (size_t)p->n_cfl);
+    s->cfl_v = (real*)malloc(sizeof(real) * (size_t)p->n_cfl);
+    for (int i = 0; i < p->n_cfl; ++i) { s->cfl_t[i] = (real)p->cfl_times[i]; s->cfl_v[i] = (real)p->cfl_values[i]; }
+    if (p->bl_profile) {
+        real prof[1 + 3 * (ORC_BL_N + 1)];
+        for (int i = 0; i < 1 + 3 * (ORC_BL_N + 1); ++i) prof[i] = (real)p->bl_profile[i];
+        if (bl_set(&s->bl, prof)) {
+            set_err("Gauss-Jordan elimination failed while solving for sigma."); orc_destroy(s); return NULL;
+        }
     }
     s->b = (Blk*)calloc((size_t)nblocks, sizeof(Blk));
     size_t voff = 0, ooff = 0;
@@ -1151,15 +1174,15 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
         b->active = active[ib];
         for (int f = 0; f < 6; ++f) {
             b->bc[f] = bc_codes[6 * ib + f]; b->bcfs[f] = bc_fs[6 * ib + f];
-            b->twall[f] = p->bc_twall ? p->bc_twall[6 * ib + f] : 0.0;
+            b->twall[f] = p->bc_twall ? (real)p->bc_twall[6 * ib + f] : zero;
             b->bcfun[f] = p->bc_fun ? p->bc_fun[6 * ib + f] : 0;
             if (b->bc[f] == ORC_WALL_NOSLIP_FIXED_T && isnan(b->twall[f])) {
                 int nt = f < 2 ? b->njc * b->nkc : (f < 4 ? b->nic * b->nkc : b->nic * b->njc);
                 if (!p->twall_faces || twall_used + nt > p->n_twall_faces) {
                     set_err("TWall per face: twall_faces holds too few values"); orc_destroy(s); return NULL;
                 }
-                b->twf[f] = (double*)malloc(sizeof(double) * (size_t)nt);
-                memcpy(b->twf[f], p->twall_faces + twall_used, sizeof(double) * (size_t)nt);
+                b->twf[f] = (real*)malloc(sizeof(real) * (size_t)nt);
+                for (int q = 0; q < nt; ++q) b->twf[f][q] = (real)p->twall_faces[twall_used + q];
                 twall_used += nt;
             }
             if (b->active && b->bc[f] == ORC_INFLOW_FUNCTION && b->bcfun[f] == 2 && !s->bl.set) {
@@ -1169,8 +1192,8 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
         if (configure(b)) { orc_destroy(s); return NULL; }
         size_t nv = (size_t)(b->nic + 1) * (b->njc + 1) * (b->nkc + 1);
         for (size_t v = 0; v < nv; ++v) {
-            b->vtx[v].x = vertices[voff + 3 * v]; b->vtx[v].y = vertices[voff + 3 * v + 1];
-            b->vtx[v].z = vertices[voff + 3 * v + 2];
+            b->vtx[v].x = (real)vertices[voff + 3 * v]; b->vtx[v].y = (real)vertices[voff + 3 * v + 1];
+            b->vtx[v].z = (real)vertices[voff + 3 * v + 2];
         }
         voff += 3 * nv;
         /* readFlow, block.cu:882-907: 13 IO variables, cell order i fastest. */
@@ -1178,11 +1201,12 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
         for (int c = 0; c < n; ++c) {
             Cell* cl = &b->cells[c];
             const double* io = iovars + ooff;
-            cl->pos.x = io[0 * n + c]; cl->pos.y = io[1 * n + c]; cl->pos.z = io[2 * n + c];
-            cl->vol = io[3 * n + c];
-            cl->fs.p = io[4 * n + c]; cl->fs.T = io[5 * n + c]; cl->fs.rho = io[6 * n + c];
-            cl->fs.e = io[7 * n + c]; cl->fs.YB = io[8 * n + c]; cl->fs.a = io[9 * n + c];
-            cl->fs.vel.x = io[10 * n + c]; cl->fs.vel.y = io[11 * n + c]; cl->fs.vel.z = io[12 * n + c];
+            cl->pos.x = (real)io[0 * n + c]; cl->pos.y = (real)io[1 * n + c]; cl->pos.z = (real)io[2 * n + c];
+            cl->vol = (real)io[3 * n + c];
+            cl->fs.p = (real)io[4 * n + c]; cl->fs.T = (real)io[5 * n + c]; cl->fs.rho = (real)io[6 * n + c];
+            cl->fs.e = (real)io[7 * n + c]; cl->fs.YB = (real)io[8 * n + c]; cl->fs.a = (real)io[9 * n + c];
+            cl->fs.vel.x = (real)io[10 * n + c]; cl->fs.vel.y = (real)io[11 * n + c];
+            cl->fs.vel.z = (real)io[12 * n + c];
         }
         ooff += 13 * (size_t)n;
         compute_geometry(b);
@@ -1205,7 +1229,7 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
                                         : (o->nic == b->nic && o->njc == b->njc);
                 if (!ok_dims) { set_err("exchange neighbour has mismatched face dimensions"); orc_destroy(s); return NULL; }
             }
-    s->t = p->t0; s->dt = p->dt_init; s->step = 0;
+    s->t = (real)p->t0; s->dt = s->dt_init; s->step = 0;
     /* simulate.cu:196-203: encode level 0 from the file's flow states. */
     for (int ib = 0; ib < nblocks; ++ib) {
         Blk* b = &s->b[ib];
@@ -1234,16 +1258,16 @@ int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
     int done = 0;
     while (done < nsteps && s->step < s->max_step && s->t < s->max_time) {
         if (s->step > 0 && (s->step % s->cfl_count) == 0) { /* :208-221 */
-            double smallest = DBL_MAX;
-            double cfl = schedule_value(s, s->t);
+            real smallest = REAL_MAX;
+            real cfl = schedule_value(s, s->t);
             for (int ib = 0; ib < s->nblocks; ++ib) {
                 Blk* b = &s->b[ib];
                 if (!b->active) continue;
-                double bmin = DBL_MAX; /* block.cu:978-990 */
+                real bmin = REAL_MAX; /* block.cu:978-990 */
                 for (int c = 0; c < b->ncells; ++c) bmin = fmin(bmin, local_dt(b, &b->cells[c], cfl));
                 smallest = fmin(smallest, bmin);
             }
-            s->dt = fmin(1.5 * s->dt, smallest);
+            s->dt = fmin((real)1.5 * s->dt, smallest);
         }
         for (int stage = 1; stage <= 3; ++stage) { /* :223-287 */
             apply_all_bcs(s);
@@ -1267,7 +1291,7 @@ int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
         for (int ib = 0; ib < s->nblocks; ++ib) { /* :288-294, copy level 1 -> 0 */
             Blk* b = &s->b[ib];
             if (!b->active) continue;
-            memcpy(b->Q, b->Q + (size_t)b->ncells * 6, sizeof(double) * (size_t)b->ncells * 6);
+            memcpy(b->Q, b->Q + (size_t)b->ncells * 6, sizeof(real) * (size_t)b->ncells * 6);
         }
         if (s->reacting) /* :296-303, block.cu:1174-1184 */
             for (int ib = 0; ib < s->nblocks; ++ib) {
@@ -1278,7 +1302,7 @@ int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
                     encode(&b->cells[c].fs, &b->Q[(size_t)c * 6]);
                 }
             }
-        if (dt_log) dt_log[done] = s->dt;
+        if (dt_log) dt_log[done] = (double)s->dt;
         s->t += s->dt;
         s->step += 1;
         done += 1;
@@ -1288,8 +1312,8 @@ int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
 }
 
 int orc_step_count(const orc_sim* s) { return s->step; }
-double orc_time(const orc_sim* s) { return s->t; }
-double orc_dt(const orc_sim* s) { return s->dt; }
+double orc_time(const orc_sim* s) { return (double)s->t; }
+double orc_dt(const orc_sim* s) { return (double)s->dt; }
 
 /* cell.cu:133-153 */
 void orc_get_iovars(const orc_sim* s, int blk, double* out)
@@ -1327,8 +1351,9 @@ void orc_get_dudt(const orc_sim* s, int blk, int level, double* out)
 static FS fs_from9(const double* w)
 {
     FS f;
-    f.rho = w[0]; f.e = w[1]; f.p = w[2]; f.T = w[3]; f.YB = w[4]; f.a = w[5];
-    f.vel.x = w[6]; f.vel.y = w[7]; f.vel.z = w[8];
+    f.rho = (real)w[0]; f.e = (real)w[1]; f.p = (real)w[2]; f.T = (real)w[3]; f.YB = (real)w[4];
+    f.a = (real)w[5];
+    f.vel.x = (real)w[6]; f.vel.y = (real)w[7]; f.vel.z = (real)w[8];
     return f;
 }
 
@@ -1336,9 +1361,9 @@ void orc_face_flux(const double* states, const double* frame, int x_order, int f
 {
     Face f;
     memset(&f, 0, sizeof f);
-    f.n.x = frame[0]; f.n.y = frame[1]; f.n.z = frame[2];
-    f.t1.x = frame[3]; f.t1.y = frame[4]; f.t1.z = frame[5];
-    f.t2.x = frame[6]; f.t2.y = frame[7]; f.t2.z = frame[8];
+    f.n.x = (real)frame[0]; f.n.y = (real)frame[1]; f.n.z = (real)frame[2];
+    f.t1.x = (real)frame[3]; f.t1.y = (real)frame[4]; f.t1.z = (real)frame[5];
+    f.t2.x = (real)frame[6]; f.t2.y = (real)frame[7]; f.t2.z = (real)frame[8];
     FS s0 = fs_from9(states), s1 = fs_from9(states + 9), s2 = fs_from9(states + 18),
        s3 = fs_from9(states + 27);
     face_flux(&f, &s0, &s1, &s2, &s3, flux_calc, x_order);
@@ -1348,8 +1373,8 @@ void orc_face_flux(const double* states, const double* frame, int x_order, int f
 void orc_hex_props(const double* p, double* out)
 {
     V3 v[8];
-    for (int i = 0; i < 8; ++i) { v[i].x = p[3 * i]; v[i].y = p[3 * i + 1]; v[i].z = p[3 * i + 2]; }
-    V3 c; double vol, iL, jL, kL;
+    for (int i = 0; i < 8; ++i) { v[i].x = (real)p[3 * i]; v[i].y = (real)p[3 * i + 1]; v[i].z = (real)p[3 * i + 2]; }
+    V3 c; real vol, iL, jL, kL;
     hex_props(v, &c, &vol, &iL, &jL, &kL);
     out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = vol; out[4] = iL; out[5] = jL; out[6] = kL;
 }
@@ -1357,8 +1382,8 @@ void orc_hex_props(const double* p, double* out)
 void orc_quad_props(const double* p, double* out)
 {
     V3 v[4];
-    for (int i = 0; i < 4; ++i) { v[i].x = p[3 * i]; v[i].y = p[3 * i + 1]; v[i].z = p[3 * i + 2]; }
-    V3 c, n, t1, t2; double area;
+    for (int i = 0; i < 4; ++i) { v[i].x = (real)p[3 * i]; v[i].y = (real)p[3 * i + 1]; v[i].z = (real)p[3 * i + 2]; }
+    V3 c, n, t1, t2; real area;
     quad_properties(v[0], v[1], v[2], v[3], &c, &n, &t1, &t2, &area);
     out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = n.x; out[4] = n.y; out[5] = n.z;
     out[6] = t1.x; out[7] = t1.y; out[8] = t1.z; out[9] = t2.x; out[10] = t2.y; out[11] = t2.z;
@@ -1369,7 +1394,9 @@ int orc_decode(const double* U, double* W)
 {
     FS f;
     memset(&f, 0, sizeof f);
-    int bad = decode(&f, U);
+    real u[6];
+    for (int i = 0; i < 6; ++i) u[i] = (real)U[i];
+    int bad = decode(&f, u);
     W[0] = f.rho; W[1] = f.e; W[2] = f.p; W[3] = f.T; W[4] = f.YB; W[5] = f.a;
     W[6] = f.vel.x; W[7] = f.vel.y; W[8] = f.vel.z;
     return bad;
@@ -1378,5 +1405,7 @@ int orc_decode(const double* U, double* W)
 void orc_encode(const double* W, double* U)
 {
     FS f = fs_from9(W);
-    encode(&f, U);
+    real u[6];
+    encode(&f, u);
+    for (int i = 0; i < 6; ++i) U[i] = (double)u[i];
 }

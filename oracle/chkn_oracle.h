@@ -5,8 +5,10 @@
  * checker of the HIP path.  The product (libchkn) never links or calls it.
  *
  * The restatement follows the reference's CPU flavour (march_in_time_using_cpu_only,
- * src/chicken/simulate.cu:182-342) operation for operation, in FP64, so that a build
- * without FMA contraction reproduces the reference's arithmetic bit for bit.
+ * src/chicken/simulate.cu:182-342) operation for operation, in the reference's number type
+ * (FP64, or FP32 as its FLOAT_NUMBERS build), so that a build without FMA contraction
+ * reproduces the reference's arithmetic bit for bit.  The interface below is double
+ * throughout; the FP32 builds round inputs to float and widen outputs exactly.
  * Each function in chkn_oracle.c cites the reference file:line it restates.
  */
 #ifndef CHKN_ORACLE_H
@@ -16,8 +18,10 @@
 extern "C" {
 #endif
 
-/* Built twice: liborc.so for ideal air and liborc_ab.so (ORC_AB_REACTING_GAS) for the
- * reacting A->B gas, as the reference selects its gas model at build time (gas.cu:20-48). */
+/* Built four times: liborc.so for ideal air and liborc_ab.so (ORC_AB_REACTING_GAS) for the
+ * reacting A->B gas, as the reference selects its gas model at build time (gas.cu:20-48),
+ * and their FP32 twins liborc_f32.so and liborc_ab_f32.so (ORC_FP32, number.cu:10-27),
+ * which have no complex-step probe. */
 
 /* Boundary-condition codes, face.cu:22-36. */
 enum { ORC_WALL_SLIP = 0, ORC_WALL_NOSLIP_ADIABATIC = 1, ORC_WALL_NOSLIP_FIXED_T = 2,

@@ -12,10 +12,16 @@
 // sequence of simulate.cu:55-155 (initialise), :158-179 (write), :182-342 (CPU march)
 // and :768-792 (finalise), calling the reference's Block/BC functions for all arithmetic.
 //
+// Built per gas model and number type as the reference's makefile builds them (GAS_MODEL,
+// FLOAT_NUMBERS); the time and time step are the reference's number type, as SimState holds
+// them (simulate.cu:32-39).
+//
 // Modes:
-//   chkn_ref run  <job> <tindx> <nsteps|-1> <dtlog|-> [text]
+//   chkn_ref run  <job> <tindx> <nsteps|-1> <dtlog|-> [text|textin]
 //                               march like chkn-run --binary (with "text": like chkn-run
-//                               without --binary, gzipped text files)
+//                               without --binary, gzipped text files; with "textin": read
+//                               gzipped text, write binary flow files -- the FP32 build
+//                               reads text only, block.cu:794-796)
 //   chkn_ref flux <in> <out>    flux probe: records of 4x9 states + 9 frame doubles
 //                               + (x_order, flux_calc) as doubles -> 6 fluxes each
 //   chkn_ref geom <in> <out>    geometry probe: 8 hex vertices -> pos, vol, lengths;
@@ -37,11 +43,11 @@ std::vector<BConfig> blk_configs;
 #include "bcs.cu"
 
 namespace H {
-double dt = 0.0, t = 0.0, t_plot = 0.0;
+number dt = 0.0, t = 0.0, t_plot = 0.0;
 int step = 0, next_plot_indx = 1, steps_since_last_plot = 0;
 }
 
-static void init_sim(int tindx, bool binary)
+static void init_sim(int tindx, bool binary_in)
 {
     blk_configs = read_config_file(Config::job + "/config.json");
     char buf[256];
@@ -54,9 +60,9 @@ static void init_sim(int tindx, bool binary)
         Block blk;
         blk.configure(cfg);
         snprintf(buf, sizeof buf, "/grid/grid-%04d-%04d-%04d", i, j, k);
-        blk.readGrid(cfg, Config::job + buf + (binary ? ".bin" : ".gz"), binary);
+        blk.readGrid(cfg, Config::job + buf + (binary_in ? ".bin" : ".gz"), binary_in);
         snprintf(buf, sizeof buf, "/flow/t%04d/flow-%04d-%04d-%04d", tindx, i, j, k);
-        blk.readFlow(cfg, Config::job + buf + (binary ? ".bin" : ".gz"), binary);
+        blk.readFlow(cfg, Config::job + buf + (binary_in ? ".bin" : ".gz"), binary_in);
         blk.computeGeometry(cfg);
         fluidBlocks.push_back(blk);
     }
@@ -67,7 +73,7 @@ static void init_sim(int tindx, bool binary)
     while (std::getline(tf, line)) {
         if (line.empty() || line.find('#') != std::string::npos) continue;
         std::stringstream ss(line);
-        int ti; double tm;
+        int ti; number tm;
         ss >> ti >> tm;
         if (ti == tindx) { H::t = tm; H::next_plot_indx = ti + 1; }
     }
@@ -75,7 +81,7 @@ static void init_sim(int tindx, bool binary)
     H::steps_since_last_plot = 0;
 }
 
-static void write_flow(int tindx, double tme, bool binary)
+static void write_flow(int tindx, number tme, bool binary)
 {
     char buf[512];
     snprintf(buf, sizeof buf, "%s/flow/t%04d", Config::job.c_str(), tindx);
@@ -114,10 +120,11 @@ static int stage(int which)
     return total;
 }
 
-static int run(const char* job, int tindx, int nsteps, const char* dtlog, bool binary)
+static int run(const char* job, int tindx, int nsteps, const char* dtlog, bool binary_in,
+               bool binary)
 {
     Config::job = job;
-    init_sim(tindx, binary);
+    init_sim(tindx, binary_in);
     H::dt = Config::dt_init;
     H::step = 0;
     int nb = Config::nFluidBlocks;
@@ -132,14 +139,14 @@ static int run(const char* job, int tindx, int nsteps, const char* dtlog, bool b
     int done = 0;
     while (H::step < Config::max_step && H::t < Config::max_time && (nsteps < 0 || done < nsteps)) {
         if (H::step > 0 && (H::step % Config::cfl_count) == 0) {
-            double smallest = numeric_limits<double>::max();
-            std::vector<double> adt(nb, smallest);
-            double cfl = Config::cfl_schedule.get_value(H::t);
+            number smallest = numeric_limits<number>::max();
+            std::vector<number> adt(nb, smallest);
+            number cfl = Config::cfl_schedule.get_value(H::t);
 #pragma omp parallel for
             for (int ib = 0; ib < nb; ++ib)
                 if (blk_configs[ib].active) adt[ib] = fluidBlocks[ib].estimate_allowed_dt(blk_configs[ib], cfl);
-            for (double a : adt) smallest = fmin(smallest, a);
-            H::dt = fmin(1.5 * H::dt, smallest);
+            for (number a : adt) smallest = fmin(smallest, a);
+            H::dt = fmin((number)1.5 * H::dt, smallest);
         }
         for (int s = 1; s <= 3; ++s) {
             int bad = stage(s);
@@ -157,7 +164,7 @@ static int run(const char* job, int tindx, int nsteps, const char* dtlog, bool b
             for (int ib = 0; ib < nb; ++ib)
                 if (blk_configs[ib].active) fluidBlocks[ib].update_chemistry(blk_configs[ib], H::dt);
         }
-        if (fdt) std::fprintf(fdt, "%d %.17g\n", H::step, H::dt);
+        if (fdt) std::fprintf(fdt, "%d %.17g\n", H::step, (double)H::dt);
         H::t += H::dt;
         H::step += 1;
         H::steps_since_last_plot += 1;
@@ -174,8 +181,8 @@ static int run(const char* job, int tindx, int nsteps, const char* dtlog, bool b
     if (fdt) std::fclose(fdt);
     long long cells = 0;
     for (auto& c : blk_configs) if (c.active) cells += (long long)c.nic * c.njc * c.nkc;
-    std::printf("steps=%d t=%.17g dt=%.17g cells=%lld march_seconds=%.6f\n", H::step, H::t, H::dt,
-                cells, secs);
+    std::printf("steps=%d t=%.17g dt=%.17g cells=%lld march_seconds=%.6f\n", H::step, (double)H::t,
+                (double)H::dt, cells, secs);
     return 0;
 }
 
@@ -248,9 +255,11 @@ int main(int argc, char** argv)
     if (argc < 2) { std::cerr << "usage: chkn_ref run|flux|geom ..." << std::endl; return 2; }
     std::string mode = argv[1];
     try {
-        if (mode == "run" && argc >= 6)
+        if (mode == "run" && argc >= 6) {
+            const std::string io = argc >= 7 ? argv[6] : "";
             return run(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), argv[5],
-                       !(argc >= 7 && std::string(argv[6]) == "text"));
+                       io != "text" && io != "textin", io != "text");
+        }
         if (mode == "flux" && argc >= 4) return flux_probe(argv[2], argv[3]);
         if (mode == "geom" && argc >= 4) return geom_probe(argv[2], argv[3]);
     } catch (const std::exception& e) {

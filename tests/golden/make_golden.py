@@ -13,6 +13,9 @@ that script) in this container only.  Outputs, all data:
   march_text.npz     a multi-block job in gzipped-text files marched to the end with
                      snapshots (times.data text, SHA-256 of every flow file written), then
                      restarted from time index 2 (the same after the restart)
+  step_f32_<name>.npz  the same small jobs marched by the reference's FP32 build
+                     (FLOAT_NUMBERS, oracle/_ref/chkn_ref*_f32), which reads gzipped-text
+                     job files only: dt per step and the final IO variables as float32
 
 Usage: python tests/golden/make_golden.py [step fixture name ...]   (default: all)
 """
@@ -29,6 +32,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "gdtk-rmhd_amd"))
 REF = os.path.join(ROOT, "oracle", "_ref", "chkn_ref")
 REF_AB = os.path.join(ROOT, "oracle", "_ref", "chkn_ref_ab")  # the reacting A->B gas build
+REF_F32 = os.path.join(ROOT, "oracle", "_ref", "chkn_ref_f32")  # FLOAT_NUMBERS builds
+REF_AB_F32 = os.path.join(ROOT, "oracle", "_ref", "chkn_ref_ab_f32")
 
 from chkn import cases, prep  # noqa: E402
 from chkn.job import BL_PROFILE_FILE, JobData, gas_flavour, read_flow_file, write_bl_profile  # noqa: E402
@@ -197,6 +202,50 @@ def step_fixture(name, builder, steps):
                         digest=job_digest(jd), **{"io%d" % b: f for b, f in enumerate(fields)})
 
 
+# Step jobs also marched by the FP32 build.
+F32_STEP_JOBS = ["sod", "mixed", "ramp_cfl", "xorder1", "viscous", "bl_plate", "sbp", "ab_reacting"]
+
+
+def f32_gas_model(job_dir):
+    """Replace config.json's gas_model Cv by the FP32 build's own value.
+
+    The FP32 build computes Cv = R/(g - 1.0) from g and R rounded to float (gas.cu:34-40);
+    for the A->B gas that is 1434.9996..., which its 1e-4 gas-model check (config.cu:369-377)
+    does not accept as the 1435 the prep writes.  Jobs for that build carry its own value."""
+    import json
+    path = os.path.join(job_dir, "config.json")
+    with open(path) as f:
+        cfg = json.load(f)
+    gm = cfg["gas_model"]
+    g, r = float(np.float32(gm["gamma"])), float(np.float32(gm["R"]))
+    gm["Cv"] = float(np.float32(r / (g - 1.0)))
+    with open(path, "w") as f:
+        json.dump(cfg, f, indent=2)
+
+
+def step_fixture_f32(name, builder, steps):
+    built = builder()
+    prof = built[1] if isinstance(built, tuple) else None
+    job = built[0] if isinstance(built, tuple) else built
+    ab = gas_flavour(job.gas_model) == "ab"
+    ref = REF_AB_F32 if ab else REF_F32
+    with tempfile.TemporaryDirectory() as d:
+        prep.write_initial_files(False, d)  # block.cu:794-796: the FP32 build reads text only
+        if ab:
+            f32_gas_model(d)
+        if prof is not None:
+            write_bl_profile(os.path.join(d, BL_PROFILE_FILE), prof)
+        dtlog = os.path.join(d, "dt.log")
+        subprocess.run([ref, "run", d, "0", str(steps), dtlog, "textin"], check=True, capture_output=True,
+                       cwd=d)
+        dts = np.loadtxt(dtlog, ndmin=2)[:, 1].astype(np.float32)
+        jd = JobData.from_job_dir(d, binary=False, bl_profile=prof)
+        fields = [np.fromfile(os.path.join(d, "flow", "t0001", "flow-%04d-%04d-%04d.bin" % (b.i, b.j, b.k)),
+                              dtype=np.float32, count=13 * b.ncells).reshape(13, b.ncells) for b in jd.blocks]
+    np.savez_compressed(os.path.join(HERE, "step_f32_%s.npz" % name), dt=dts, steps=steps,
+                        digest=job_digest(jd), **{"io%d" % b: f for b, f in enumerate(fields)})
+
+
 def text_march_job():
     """mixed_bc_array for 40 steps with a snapshot every 1.5e-5 s (five snapshots)."""
     cases.mixed_bc_array(max_step=40)
@@ -244,6 +293,8 @@ def main(names):
         for name in names:
             if name == "march_text":
                 text_march_fixture()
+            elif name.startswith("f32_"):
+                step_fixture_f32(name[4:], *STEP_JOBS[name[4:]])
             else:
                 step_fixture(name, *STEP_JOBS[name])
         return
@@ -256,6 +307,8 @@ def main(names):
     np.savez_compressed(os.path.join(HERE, "geom_vectors.npz"), p=g, out=G)
     for name, (builder, steps) in STEP_JOBS.items():
         step_fixture(name, builder, steps)
+    for name in F32_STEP_JOBS:
+        step_fixture_f32(name, *STEP_JOBS[name])
     text_march_fixture()
     print("golden vectors written to", HERE)
 

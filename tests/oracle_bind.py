@@ -10,6 +10,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "liborc.so")
 LIB_AB = os.path.join(ROOT, "oracle", "liborc_ab.so")  # the reacting A->B gas build
+# FP32 builds (the reference's FLOAT_NUMBERS): float arithmetic behind the same double API.
+LIBS = {"air": LIB, "ab": LIB_AB, "air_f32": os.path.join(ROOT, "oracle", "liborc_f32.so"),
+        "ab_f32": os.path.join(ROOT, "oracle", "liborc_ab_f32.so")}
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "chkn_ref")
 
 _I = ctypes.POINTER(ctypes.c_int)
@@ -32,7 +35,7 @@ _libs = {}
 
 def load(flavour="air"):
     if flavour not in _libs:
-        path = LIB_AB if flavour == "ab" else LIB
+        path = LIBS[flavour]
         if not os.path.exists(path):
             raise RuntimeError("%s is not built; run make -C oracle" % path)
         lib = ctypes.CDLL(path)
@@ -52,7 +55,8 @@ def load(flavour="air"):
             getattr(lib, name).argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
         lib.orc_get_dudt.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         lib.orc_face_flux.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, _D]
-        lib.orc_face_flux_cs.argtypes = [_D, _D, ctypes.c_int, _D, ctypes.c_double, _D, _D]
+        if hasattr(lib, "orc_face_flux_cs"):  # FP64 builds only
+            lib.orc_face_flux_cs.argtypes = [_D, _D, ctypes.c_int, _D, ctypes.c_double, _D, _D]
         lib.orc_hex_props.argtypes = [_D, _D]
         lib.orc_quad_props.argtypes = [_D, _D]
         lib.orc_decode.argtypes = [_D, _D]
@@ -67,11 +71,11 @@ def _p(a, t):
 
 
 class Oracle:
-    """CPU restatement of the reference march on a JobData."""
+    """CPU restatement of the reference march on a JobData (fp32: the FP32 build)."""
 
-    def __init__(self, job):
+    def __init__(self, job, fp32=False):
         from chkn.job import gas_flavour
-        lib = load(gas_flavour(job.gas_model))
+        lib = load(gas_flavour(job.gas_model) + ("_f32" if fp32 else ""))
         f = job.flat()
         self._keep = f
         prm = OrcParams(job.x_order, job.flux_calc, job.cfl_count, job.max_step, job.max_time,

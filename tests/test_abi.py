@@ -16,14 +16,46 @@ def declared_functions():
 
 
 def test_every_declared_symbol_is_exported():
-    # Both gas-model flavours: ideal air and the reacting A->B gas.
+    # Every flavour: ideal air and the reacting A->B gas, each in FP64 and FP32.
     names = declared_functions()
     assert len(names) >= 15
-    for path in (lib.LIB_PATH, lib.LIB_AB_PATH):
+    for flavour, path in lib.LIB_PATHS.items():
         so = ctypes.CDLL(path)
         for n in names:
             assert hasattr(so, n), (path, n)
+        assert lib.load(flavour).chkn_real_size() == (4 if flavour.endswith("_f32") else 8)
     assert set(names) <= set(lib.EXPORTS), set(names) - set(lib.EXPORTS)
+
+
+def test_fp32_build_reads_text_jobs_only(tmp_path):
+    # block.cu:794-796: the FP32 build refuses binary grid files, with the reference's text.
+    from chkn import cases, prep
+    cases.sod_box(4, 4, 4, max_step=1)
+    prep.write_initial_files(True, str(tmp_path))
+    so = lib.load("air_f32")
+    h = ctypes.c_void_p()
+    rc = so.chkn_load_job(str(tmp_path).encode(), 0, 1, ctypes.byref(h))
+    assert rc == -2
+    assert so.chkn_last_error(None) == b"Do not ask to FP32 code to read grid file with binary data."
+
+
+def test_fp32_reacting_build_rejects_the_prep_gas_model(tmp_path):
+    # The FP32 A->B build's Cv = R/(g - 1.0) from float g and R is 1434.9996..., outside the
+    # 1e-4 check against the 1435 the prep writes (config.cu:369-386): the reference's FP32
+    # build refuses the job the same way (oracle/_ref/chkn_ref_ab_f32, where it is built).
+    import subprocess
+    from chkn import cases, prep
+    cases.ab_reacting(max_step=1)
+    prep.write_initial_files(False, str(tmp_path))
+    so = lib.load("ab_f32")
+    h = ctypes.c_void_p()
+    rc = so.chkn_load_job(str(tmp_path).encode(), 0, 0, ctypes.byref(h))
+    assert rc == -3
+    assert so.chkn_last_error(None) == b"Incorrect gas model data"
+    ref = os.path.join(ROOT, "oracle", "_ref", "chkn_ref_ab_f32")
+    if os.path.exists(ref):
+        r = subprocess.run([ref, "run", str(tmp_path), "0", "1", "-", "textin"], capture_output=True, text=True)
+        assert r.returncode == 3 and "Incorrect gas model data" in r.stderr
 
 
 def test_load_job_errors_without_gpu(tmp_path):
