@@ -96,21 +96,16 @@ __device__ __forceinline__ Prim lds_prim(const real* Wp, int np, int o)
     return p;
 }
 
-// Area-weighted flux of the k-face (i, j, kf) between cells kf-1 and kf.
+// Area-weighted flux of the k-face (i, j, kf) between cells kf-1 and kf from its stencil
+// states (cells kf-2 .. kf+1 of the column).
 template <int XORDER>
-__device__ __forceinline__ void k_face_af(const StageArgs& A, const BlockDev* B,
-                                          const real* __restrict__ Win, int i, int j, int kf,
-                                          real AF[6])
+__device__ __forceinline__ void k_face_flux(const BlockDev* B, int i, int j, int kf, const Prim& s0,
+                                            const Prim& s1, const Prim& s2, const Prim& s3,
+                                            real AF[6])
 {
-    const int nic = B->nic, njc = B->njc, nkc = B->nkc;
-    const int base = i + nic * j, stride = nic * njc;
-    const Prim s0 = stencil_prim(A, B, Win, nkc, base, stride, 2, kf - 2, i, j);
-    const Prim s1 = stencil_prim(A, B, Win, nkc, base, stride, 2, kf - 1, i, j);
-    const Prim s2 = stencil_prim(A, B, Win, nkc, base, stride, 2, kf, i, j);
-    const Prim s3 = stencil_prim(A, B, Win, nkc, base, stride, 2, kf + 1, i, j);
-    const int fi = i + nic * (j + njc * kf);
+    const int fi = i + B->nic * (j + B->njc * kf);
     const Frame fr = ld_frame(B->gk, (int)B->nfk, fi);
-    const real area = B->gk[fi];
+    const real area = gld(B->gk, boff(fi));
     real F[6];
     face_flux<XORDER>(s0, s1, s2, s3, fr, F);
 #pragma unroll
@@ -140,16 +135,22 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
     int bad = 0;
     unsigned long long sigmin = ~0ull;
 
+    // The column's k-faces: the stencil of face k+1 is cells k-1 .. k+2, of which cell k is
+    // read from the LDS plane.
+    const int nkc = B->nkc, kbase = i + nic * j, kstride = nic * njc;
     real AFkm[6] = {0, 0, 0, 0, 0, 0};
-    if (own) k_face_af<XORDER>(A, B, Win, i, j, kbeg, AFkm);
+    if (own) {
+        const Prim s0 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, kbeg - 2, i, j);
+        const Prim s1 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, kbeg - 1, i, j);
+        const Prim s2 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, kbeg, i, j);
+        const Prim s3 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, kbeg + 1, i, j);
+        k_face_flux<XORDER>(B, i, j, kbeg, s0, s1, s2, s3, AFkm);
+    }
 
     for (int k = kbeg; k < kend; ++k) {
-        // Stage the plane (with halo and ghost cells) in LDS; the thread's own k-face one
-        // plane up is evaluated from global memory meanwhile.
+        // Stage the plane (with halo and ghost cells) in LDS.
         for (int p = t; p < L::NLOAD; p += NT)
             fill_plane_position<TI, TJ>(A, B, Win, tile.i0, tile.j0, k, p, Wp);
-        real AFkp[6] = {0, 0, 0, 0, 0, 0};
-        if (own) k_face_af<XORDER>(A, B, Win, i, j, k + 1, AFkp);
         __syncthreads();
         // i- and j-faces of the plane, one flattened list, stencils read from LDS.
         for (int f = t; f < NFI + NFJ; f += NT) {
@@ -180,7 +181,7 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
             }
             if (valid) {
                 const Frame fr = ld_frame(g, nf, fi);
-                const real area = g[fi];
+                const real area = gld(g, boff(fi));
                 const Prim s0 = lds_prim(Wp, NP, base);
                 const Prim s1 = lds_prim(Wp, NP, base + stride);
                 const Prim s2 = lds_prim(Wp, NP, base + 2 * stride);
@@ -193,8 +194,18 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
         }
         __syncthreads();
         if (own) {
+            // The column's k-face one plane up, evaluated after the plane's i- and j-faces so
+            // that its flux is not held in registers through the face loop.
+            real AFkp[6];
+            {
+                const Prim sk = lds_prim(Wp, NP, (tj + 2) * PW + ti + 2);
+                const Prim s2 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, k + 1, i, j);
+                const Prim s3 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, k + 2, i, j);
+                const Prim s0 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, k - 1, i, j);
+                k_face_flux<XORDER>(B, i, j, k + 1, s0, sk, s2, s3, AFkp);
+            }
             const int cidx = i + nic * (j + njc * k);
-            const real vinv = B->volinv[cidx];
+            const real vinv = gld(B->volinv, boff(cidx));
             real d[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) {

@@ -8,30 +8,71 @@
 
 namespace chkn {
 
-__device__ __forceinline__ Prim ld_prim(const real* __restrict__ W, int n, int c)
+// Global-memory view of a device pointer held in a block table or argument struct.  The
+// compiler cannot tell such pointers from LDS ones and would access them with flat_*
+// instructions, which also count against the LDS wait counter: every wait for an LDS read
+// would then also wait for the outstanding HBM loads.  Through this view they are global_*.
+#define CHKN_GMEM __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ CHKN_GMEM T* gmem(T* p)
 {
+    return (CHKN_GMEM T*)p;
+}
+
+// Element at byte offset `off` of a global array.  With a uniform base and a 32-bit offset
+// the compiler issues one global load with the base in scalar registers and the offset in a
+// single VGPR shared by all arrays indexed alike, instead of a 64-bit address per load.
+// Every per-block array (one slot of W, U, L, one direction's face geometry) is below 4 GiB.
+__device__ __forceinline__ unsigned boff(int c)
+{
+    return (unsigned)c * (unsigned)sizeof(real);
+}
+__device__ __forceinline__ real gld(const real* base, unsigned off)
+{
+    return *(const CHKN_GMEM real*)((const CHKN_GMEM char*)gmem(base) + off);
+}
+__device__ __forceinline__ void gst(real* base, unsigned off, real v)
+{
+    *(CHKN_GMEM real*)((CHKN_GMEM char*)gmem(base) + off) = v;
+}
+// Non-temporal forms, for data a stage streams through once (U and L of the RK stage and
+// the stores of the next stage's state): they are kept from evicting the W stencil lines
+// that neighbouring planes and tiles read again shortly.
+__device__ __forceinline__ real gld_nt(const real* base, unsigned off)
+{
+    return __builtin_nontemporal_load((const CHKN_GMEM real*)((const CHKN_GMEM char*)gmem(base) + off));
+}
+__device__ __forceinline__ void gst_nt(real* base, unsigned off, real v)
+{
+    __builtin_nontemporal_store(v, (CHKN_GMEM real*)((CHKN_GMEM char*)gmem(base) + off));
+}
+
+__device__ __forceinline__ Prim ld_prim(const real* W, int n, int c)
+{
+    const unsigned o = boff(c);
     Prim p;
-    p.rho = W[c];
-    p.e = W[n + c];
-    p.YB = W[2 * n + c];
-    p.vx = W[3 * n + c];
-    p.vy = W[4 * n + c];
-    p.vz = W[5 * n + c];
+    p.rho = gld(W, o);
+    p.e = gld(W + n, o);
+    p.YB = gld(W + 2 * n, o);
+    p.vx = gld(W + 3 * n, o);
+    p.vy = gld(W + 4 * n, o);
+    p.vz = gld(W + 5 * n, o);
     return p;
 }
 
-__device__ __forceinline__ Frame ld_frame(const real* __restrict__ g, int nf, int f)
+__device__ __forceinline__ Frame ld_frame(const real* g, int nf, int f)
 {
+    const unsigned o = boff(f);
     Frame r;
-    r.nx = g[1 * nf + f];
-    r.ny = g[2 * nf + f];
-    r.nz = g[3 * nf + f];
-    r.t1x = g[4 * nf + f];
-    r.t1y = g[5 * nf + f];
-    r.t1z = g[6 * nf + f];
-    r.t2x = g[7 * nf + f];
-    r.t2y = g[8 * nf + f];
-    r.t2z = g[9 * nf + f];
+    r.nx = gld(g + 1 * nf, o);
+    r.ny = gld(g + 2 * nf, o);
+    r.nz = gld(g + 3 * nf, o);
+    r.t1x = gld(g + 4 * nf, o);
+    r.t1y = gld(g + 5 * nf, o);
+    r.t1z = gld(g + 6 * nf, o);
+    r.t2x = gld(g + 7 * nf, o);
+    r.t2y = gld(g + 8 * nf, o);
+    r.t2z = gld(g + 9 * nf, o);
     return r;
 }
 
@@ -79,7 +120,7 @@ __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const real* i
         return ld_prim(O->W + win * O->wstride, (int)O->n, cell_index(O->nic, O->njc, dir, mo, a, c));
     }
     if (code == 4) {  // inflow: the configured flow state (bcs.cu:257-319)
-        const real* s = inflow + 6 * B->bc_arg[bf];
+        const CHKN_GMEM real* s = gmem(inflow) + 6 * B->bc_arg[bf];
         Prim p;
         p.rho = s[0]; p.e = s[1]; p.YB = s[2]; p.vx = s[3]; p.vy = s[4]; p.vz = s[5];
         return p;
@@ -87,7 +128,7 @@ __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const real* i
     if (code == 6) {  // inflow function: states at the ghost positions (bcs.cu:355-432)
         const int na = dir == 0 ? B->njc : B->nic;
         const int nt = na * (dir == 2 ? B->njc : B->nkc);
-        const real* s = B->side[bf] + (long long)depth * 9 * nt + (a + na * c);
+        const CHKN_GMEM real* s = gmem(B->side[bf]) + (long long)depth * 9 * nt + (a + na * c);
         Prim p;
         p.rho = s[0]; p.e = s[nt]; p.YB = s[2 * nt]; p.vx = s[3 * nt]; p.vy = s[4 * nt]; p.vz = s[5 * nt];
         return p;
@@ -128,37 +169,38 @@ template <int STAGE>
 __device__ __forceinline__ int stage_update(const BlockDev* B, int n, int cidx, real dt,
                                             const real d[6], int wout, Prim& w)
 {
-    real* __restrict__ U = B->U;
-    real* __restrict__ Lr = B->L;
+    real* U = B->U;
+    real* Lr = B->L;
+    const unsigned o = boff(cidx);
     real Un[6];
     if (STAGE == 1) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            Un[q] = U[q * n + cidx] + dt * d[q];
-            Lr[q * n + cidx] = d[q];
+            Un[q] = gld_nt(U + q * n, o) + dt * d[q];
+            gst_nt(Lr + q * n, o, d[q]);
         }
     } else if (STAGE == 2) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            const real l0 = Lr[q * n + cidx];
-            Un[q] = U[q * n + cidx] + real(0.25) * dt * (l0 + d[q]);
-            Lr[q * n + cidx] = kOneSixth * l0 + kOneSixth * d[q];
+            const real l0 = gld_nt(Lr + q * n, o);
+            Un[q] = gld_nt(U + q * n, o) + real(0.25) * dt * (l0 + d[q]);
+            gst_nt(Lr + q * n, o, kOneSixth * l0 + kOneSixth * d[q]);
         }
     } else {
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            Un[q] = U[q * n + cidx] + dt * (Lr[q * n + cidx] + kFourSixth * d[q]);
-            U[q * n + cidx] = Un[q];
+            Un[q] = gld_nt(U + q * n, o) + dt * (gld_nt(Lr + q * n, o) + kFourSixth * d[q]);
+            gst_nt(U + q * n, o, Un[q]);
         }
     }
     const int flag = decode(Un, w);
-    real* __restrict__ Wout = B->W + wout * B->wstride;
-    Wout[cidx] = w.rho;
-    Wout[n + cidx] = w.e;
-    Wout[2 * n + cidx] = w.YB;
-    Wout[3 * n + cidx] = w.vx;
-    Wout[4 * n + cidx] = w.vy;
-    Wout[5 * n + cidx] = w.vz;
+    real* Wout = B->W + wout * B->wstride;
+    gst_nt(Wout, o, w.rho);
+    gst_nt(Wout + n, o, w.e);
+    gst_nt(Wout + 2 * n, o, w.YB);
+    gst_nt(Wout + 3 * n, o, w.vx);
+    gst_nt(Wout + 4 * n, o, w.vy);
+    gst_nt(Wout + 5 * n, o, w.vz);
     return flag;
 }
 
@@ -173,8 +215,9 @@ __device__ __forceinline__ unsigned long long signal_bits(const BlockDev* B, int
     const Frame fi_ = ld_frame(B->gi, (int)B->nfi, i + (nic + 1) * (j + njc * k));
     const Frame fj_ = ld_frame(B->gj, (int)B->nfj, i + nic * (j + (njc + 1) * k));
     const Frame fk_ = ld_frame(B->gk, (int)B->nfk, cidx);
-    const real sig = local_signal(w, sa, B->len[cidx], B->len[n + cidx], B->len[2 * n + cidx],
-                                    fi_, fj_, fk_);
+    const unsigned o = boff(cidx);
+    const real sig = local_signal(w, sa, gld(B->len, o), gld(B->len + n, o), gld(B->len + 2 * n, o),
+                                  fi_, fj_, fk_);
     return order_bits(sig);
 }
 
