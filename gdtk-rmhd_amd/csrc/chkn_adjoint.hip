@@ -64,8 +64,9 @@ __device__ __forceinline__ void add_slot(const BlockDev* B, int d, int fi, int p
 {
     const double* g = gface(B, d);
     const long long nf = nfaces(B, d);
+    const unsigned o = boff(fi);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) acc[q] += g[(long long)(6 * p + q) * nf + fi];
+    for (int q = 0; q < 6; ++q) acc[q] += gld(g + (6 * p + q) * nf, o);
 }
 
 constexpr int kFaceThreads = 128;
@@ -278,24 +279,25 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
     const Prim w = ld_prim(B->W + A.slot * B->wstride, n, cidx);
     double Ub[6];
     decode_vjp(w, Wb, Ub);
-    double* A0 = B->adj;
-    double* Pp = B->adj + 6 * (long long)n;
-    double* D = B->adj + 12 * (long long)n;
     const double dt = A.dt[B->inst];
+    const unsigned o = boff(cidx);
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-        const long long o = (long long)q * n + cidx;
-        A0[o] += Ub[q];
+        double* A0 = B->adj + (long long)q * n;
+        double* Pp = B->adj + (long long)(6 + q) * n;
+        double* D = B->adj + (long long)(12 + q) * n;
+        const double a0 = gld(A0, o) + Ub[q];
+        gst(A0, o, a0);
         if (STAGE == 3) {
-            const double v = Pp[o] + 0.25 * dt * Ub[q];
-            D[o] = v;
-            Pp[o] = v;
+            const double v = gld(Pp, o) + 0.25 * dt * Ub[q];
+            gst(D, o, v);
+            gst(Pp, o, v);
         } else if (STAGE == 2) {
-            D[o] = Pp[o] + dt * Ub[q];
+            gst(D, o, gld(Pp, o) + dt * Ub[q]);
         } else if (A.dt_next) {
             const double dtn = A.dt_next[B->inst];
-            Pp[o] = kOneSixth * dtn * A0[o];
-            D[o] = kFourSixth * dtn * A0[o];
+            gst(Pp, o, kOneSixth * dtn * a0);
+            gst(D, o, kFourSixth * dtn * a0);
         }
     }
 }

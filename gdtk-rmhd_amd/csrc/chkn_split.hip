@@ -68,7 +68,8 @@ __device__ __forceinline__ Gas cell_gas(const SplitArgs& A, const BlockDev* B, c
 {
     if (B->gas && (A.file_gas || !B->active)) {
         const int n = (int)B->n;
-        return Gas{B->gas[c], B->gas[n + c], B->gas[2 * n + c]};
+        const unsigned o = boff(c);
+        return Gas{gld(B->gas, o), gld(B->gas + n, o), gld(B->gas + 2 * n, o)};
     }
     return eos_gas(w);
 }
@@ -116,7 +117,7 @@ __device__ __forceinline__ Point point(const SplitArgs& A, const BlockDev* B, in
     if (code == 6) {  // inflow function: the state at the ghost cell's position
         const int na = dir == 0 ? B->njc : B->nic;
         const int nt = na * (dir == 2 ? B->njc : B->nkc);
-        const real* s = B->side[bf] + (long long)depth * 9 * nt + (a + na * c);
+        const CHKN_GMEM real* s = gmem(B->side[bf]) + (long long)depth * 9 * nt + (a + na * c);
         r.w = Prim{s[0], s[nt], s[2 * nt], s[3 * nt], s[4 * nt], s[5 * nt]};
         if (GAS) r.g = Gas{s[6 * nt], s[7 * nt], s[8 * nt]};
         return r;
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
         ausmdv(rhoL, eL, pL, aL, YBL, vxL, vyL, vzL, rhoR, eR, pR, aR, YBR, vxR, vyR, vzR, fr, F);
     const long long nft = B->nfi + B->nfj + B->nfk;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) B->F[q * nft + p.ff] = F[q];
+    for (int q = 0; q < 6; ++q) gmem(B->F)[q * nft + p.ff] = F[q];
     if (VISC) {
         // Face state: average of the two reconstructed states, T from the mean e and YB.
         const real half = real(0.5);
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
         if (side >= 0) {
             const int bf = 2 * dir + side;
             const int code = B->bc[bf];
-            const real* sd = B->side[bf];
+            const CHKN_GMEM real* sd = gmem(B->side[bf]);
             const int t = p.a + (dir == 0 ? B->njc : B->nic) * p.c;
             if (code == 1 || code == 2) {  // no-slip walls; fixed-T walls also set T
                 vx = real(0);
@@ -255,9 +256,11 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
                 vz = sd[21 * nt + t];
             }
         }
-        real2* rec = reinterpret_cast<real2*>(B->FS) + 2 * (long long)p.ff;
-        rec[0] = make_real2(T, vx);
-        rec[1] = make_real2(vy, vz);
+        const unsigned o = boff(4 * p.ff);
+        gst(B->FS, o, T);
+        gst(B->FS + 1, o, vx);
+        gst(B->FS + 2, o, vy);
+        gst(B->FS + 3, o, vz);
     }
 }
 
@@ -274,16 +277,14 @@ __device__ __forceinline__ int cell_face(const BlockDev* B, int d, int hi, int i
 __device__ __forceinline__ void ld_rec4(const real* base, long long r, real& a, real& b,
                                         real& c, real& d)
 {
-    const real2* q = reinterpret_cast<const real2*>(base) + 2 * r;
-    const real2 u = q[0], v = q[1];
-    a = u.x; b = u.y; c = v.x; d = v.y;
+    const unsigned o = boff((int)(4 * r));
+    a = gld(base, o); b = gld(base + 1, o); c = gld(base + 2, o); d = gld(base + 3, o);
 }
 __device__ __forceinline__ void ld_rec(const real* base, long long r, real& x, real& y,
                                        real& z)
 {
-    const real2* q = reinterpret_cast<const real2*>(base) + 2 * r;
-    const real2 u = q[0];
-    x = u.x; y = u.y; z = q[1].x;
+    const unsigned o = boff((int)(4 * r));
+    x = gld(base, o); y = gld(base + 1, o); z = gld(base + 2, o);
 }
 
 // The cloud of face p (block.cu:317-358, :418-459, :519-560), in the order the reference
@@ -449,8 +450,9 @@ __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int 
     const real qx = kcond * gTx, qy = kcond * gTy, qz = kcond * gTz;
     const real* gf = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
     const int nf = (int)(dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk));
-    const real nx = gf[nf + f], ny = gf[2 * nf + f], nz = gf[3 * nf + f];
-    real* __restrict__ F = B->F;
+    const unsigned of = boff(f);
+    const real nx = gld(gf + nf, of), ny = gld(gf + 2 * nf, of), nz = gld(gf + 3 * nf, of);
+    CHKN_GMEM real* __restrict__ F = gmem(B->F);
     F[nft + p.ff] -= tau_xx * nx + tau_xy * ny + tau_xz * nz;
     F[2 * nft + p.ff] -= tau_xy * nx + tau_yy * ny + tau_yz * nz;
     F[3 * nft + p.ff] -= tau_xz * nx + tau_yz * ny + tau_zz * nz;
@@ -476,14 +478,17 @@ __global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A)
         const int fim = i + (nic + 1) * (j + njc * k);
         const int fjm = i + nic * (j + (njc + 1) * k);
         const int fkm = cidx;
-        const real aim = B->gi[fim], aip = B->gi[fim + 1];
-        const real ajm = B->gj[fjm], ajp = B->gj[fjm + nic];
-        const real akm = B->gk[fkm], akp = B->gk[fkm + nic * njc];
+        const CHKN_GMEM real* gi = gmem(B->gi);
+        const CHKN_GMEM real* gj = gmem(B->gj);
+        const CHKN_GMEM real* gk = gmem(B->gk);
+        const real aim = gi[fim], aip = gi[fim + 1];
+        const real ajm = gj[fjm], ajp = gj[fjm + nic];
+        const real akm = gk[fkm], akp = gk[fkm + nic * njc];
         const real vinv = B->volinv[cidx];
         real d[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            const real* Fq = B->F + q * nft;
+            const CHKN_GMEM real* Fq = gmem(B->F) + q * nft;
             real s = aim * Fq[fim] - aip * Fq[fim + 1];
             s = s + ajm * Fq[nfi + fjm];
             s = s - ajp * Fq[nfi + fjm + nic];
@@ -524,19 +529,19 @@ __global__ __launch_bounds__(kChunk) void chemistry_kernel(const BlockDev* block
     unsigned long long sigmin = ~0ull;
     if (cidx < ch.c1) {
         const int n = (int)B->n;
-        real* __restrict__ W = B->W + slot * B->wstride;
+        real* W = B->W + slot * B->wstride;
         Prim w = ld_prim(W, n, cidx);
         if (factor >= real(0)) {
             const real T = eos_T(w.e, w.YB);
             const real YA = real(1) - w.YB;
             if (T > kTi) {
                 w.YB = real(1) - YA * factor;
-                W[2 * n + cidx] = w.YB;
+                gst(W + 2 * n, boff(cidx), w.YB);
             }
             real U[6];
             encode(w, U);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) B->U[q * n + cidx] = U[q];
+            for (int q = 0; q < 6; ++q) gst(B->U + q * n, boff(cidx), U[q]);
         }
         if (DTRED) {
             const int nic = B->nic, njc = B->njc;
