@@ -59,14 +59,16 @@ __device__ __forceinline__ double* gface(const BlockDev* B, int d)
     return B->G + 24 * off;
 }
 
-// Adds stencil slot p (0 L1, 1 L0, 2 R0, 3 R1) of face fi of direction d to acc.
+// Adds stencil slot p (0 L1, 1 L0, 2 R0, 3 R1) of face fi of direction d to acc.  G is
+// written once by the face pass and read once here: both sides use non-temporal accesses so
+// that it does not evict the tape states and residual adjoints the passes re-read.
 __device__ __forceinline__ void add_slot(const BlockDev* B, int d, int fi, int p, double acc[6])
 {
     const double* g = gface(B, d);
     const long long nf = nfaces(B, d);
     const unsigned o = boff(fi);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) acc[q] += gld(g + (6 * p + q) * nf, o);
+    for (int q = 0; q < 6; ++q) acc[q] += gld_nt(g + (6 * p + q) * nf, o);
 }
 
 constexpr int kFaceThreads = 128;
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
     double* G = gface(B, d);
     if (!any) {
 #pragma unroll
-        for (int k = 0; k < 24; ++k) G[(long long)k * nf + fi] = 0.0;
+        for (int k = 0; k < 24; ++k) gst_nt(G + (long long)k * nf, boff(fi), 0.0);
         return;
     }
     const double* W = B->W + A.slot * B->wstride;
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
             gq[2] = side[1][q];
         }
 #pragma unroll
-        for (int p = 0; p < 4; ++p) G[(long long)(6 * p + q) * nf + fi] = gq[p];
+        for (int p = 0; p < 4; ++p) gst_nt(G + (long long)(6 * p + q) * nf, boff(fi), gq[p]);
     }
 }
 
