@@ -397,7 +397,17 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             if (b.njc < 2) throw std::invalid_argument("Too few cells in the j-index direction.");
             if (b.nkc < 2) throw std::invalid_argument("Too few cells in the k-index direction.");
             long long n = (long long)b.nic * b.njc * b.nkc;
-            if (n > (1LL << 30)) throw std::invalid_argument("block too large for 32-bit cell indexing");
+            // The kernels address each per-block array (a W slot, U, L, one direction's face
+            // geometry) with 32-bit byte offsets, and the face-list records (4 values per face
+            // of the block) likewise.
+            const long long nf_dir = std::max({(long long)(b.nic + 1) * b.njc * b.nkc,
+                                               (long long)b.nic * (b.njc + 1) * b.nkc,
+                                               (long long)b.nic * b.njc * (b.nkc + 1)});
+            const long long nft = (long long)(b.nic + 1) * b.njc * b.nkc + (long long)b.nic * (b.njc + 1) * b.nkc +
+                                  (long long)b.nic * b.njc * (b.nkc + 1);
+            const long long lim = (1LL << 32) / (long long)sizeof(real);
+            if (nf_dir >= lim || (s->viscous && 4 * nft >= std::min(lim, 1LL << 31)))
+                throw std::invalid_argument("block too large for 32-bit array offsets");
             b.n = (int)n;
             if (b.bi < 0 || b.bi >= d->nib || b.bj < 0 || b.bj >= d->njb || b.bk < 0 || b.bk >= d->nkb)
                 throw std::invalid_argument("block position outside the block array");

@@ -93,3 +93,19 @@ def test_tuner_candidates_are_valid_launch_options():
     for o in c:
         assert set(o) == {"tile_i", "tile_j", "tile_k", "tile_order", "persistent"}
         assert o["tile_i"] * o["tile_j"] <= 1024
+
+
+def test_blocks_beyond_32_bit_offsets_are_rejected():
+    # The kernels address each per-block array with 32-bit byte offsets: a block with 2^29 or
+    # more faces in one direction (FP64) is refused at set-up, before any device call.
+    from chkn import cases
+    job = cases.sod_box(4, 4, 4, max_step=1)
+    keep = []
+    d = lib._desc(job, keep)
+    dims = keep[0]["blk_dims"]
+    dims[:3] = [1024, 1024, 512]
+    so = lib.load()
+    h = ctypes.c_void_p()
+    rc = so.chkn_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == -1
+    assert so.chkn_last_error(None) == b"block too large for 32-bit array offsets"
