@@ -18,7 +18,7 @@ import os
 from .lib import BadCellError, Sim
 
 TILES = [(32, 8), (64, 4), (16, 16), (64, 8), (16, 4), (8, 8)]
-TILE_K = [0, 4, 8, 16]
+TILE_K = [0, 1, 2, 4, 8, 16]
 ORDERS = [0, 1]
 PERSISTENT = [0, 1]
 

@@ -223,12 +223,13 @@ void build_tiles(chkn_sim* s)
             int nti = (b.nic + TI - 1) / TI, ntj = (b.njc + TJ - 1) / TJ;
             int kc = s->opts.tile_k > 0 ? s->opts.tile_k : s->kc;
             if (s->opts.tile_k <= 0) {
-                // Default k-chunk: aim for about 2048 work-groups over the whole launch.
+                // Default k-chunk: aim for about 2048 work-groups over the whole launch (one
+                // plane per work-group for jobs too small to fill the device otherwise).
                 long long cols = 0;
                 for (auto& o : s->blocks)
                     if (o.active) cols += (long long)((o.nic + TI - 1) / TI) * ((o.njc + TJ - 1) / TJ) * o.nkc;
                 long long want = std::max(1LL, cols / 2048);
-                kc = (int)std::min<long long>(std::max<long long>(want, 2), 32);
+                kc = (int)std::min<long long>(want, 32);
             }
             if (s->opts.tile_order == 1) {
                 for (int tj = 0; tj < ntj; ++tj)
