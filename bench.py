@@ -187,7 +187,9 @@ def cpu_baseline(job, steps):
 
 def copy_calibration(torch):
     """Three copies of a 2 GiB FP64 buffer: each dispatch reads and writes exactly 2 GiB, so
-    the HBM byte counters of these dispatches give the counters' scale factor."""
+    the HBM byte counters of these dispatches give the counters' scale factor.  Then the
+    device's streaming rates for reference: a timed 2 GiB copy (read + write) and a timed
+    2 GiB sum (read only)."""
     n = 1 << 28
     a = torch.ones(n, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
@@ -195,7 +197,26 @@ def copy_calibration(torch):
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_ms = e0.elapsed_time(e1) / 10
+    for _ in range(3):
+        s = a.sum()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        s = a.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    sum_ms = e0.elapsed_time(e1) / 10
+    del s
+    print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8,
+                      "copy_TBps_read_plus_write": 2 * 8 * n / copy_ms / 1e9,
+                      "sum_TBps_read": 8 * n / sum_ms / 1e9}))
 
 
 def main():
