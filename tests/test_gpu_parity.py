@@ -178,3 +178,21 @@ def test_scale_ramp_at_full_size_matches_oracle():
     assert np.array_equal(sim.iovars(0), orc.iovars(0))
     assert np.array_equal(sim.conserved(0), orc.conserved(0))
     sim.close()
+
+
+def test_batched_array_at_full_size_matches_oracle():
+    # The batched workload at its real size: a 5x4x2 array of 40 uneven blocks (6.44M cells,
+    # slab sizes seeded in [16, 96]) with its seeded exchange / wall connection graph, one
+    # launch per stage; 11 steps (through the first CFL update) bit for bit against the oracle.
+    job = cases.ramp_array(seed=7, max_step=11)
+    sim = Sim(job)
+    sim.set_launch(tile_i=32, tile_j=8, tile_k=4, use_graph=True)
+    rep = sim.step(11)
+    orc = ob.Oracle(job)
+    rc, _, dts = orc.march(11)
+    assert rc == 0 and rep.step == orc.step == 11 and rep.t == orc.t and rep.dt == dts[10]
+    for b in range(len(job.blocks)):
+        assert np.array_equal(sim.iovars(b), orc.iovars(b)), b
+        if job.blocks[b].active:
+            assert np.array_equal(sim.conserved(b), orc.conserved(b)), b
+    sim.close()
