@@ -205,3 +205,17 @@ def test_viscous_closed_box_conserves_mass_at_size():
 def test_viscous_ramp_matches_oracle_mid_size():
     # The viscous ramp of the viscous workload at 48x24x24, against the oracle.
     compare(cases.ramp(48, 24, 24, max_step=6, viscous=True), 6)
+
+
+def test_viscous_ramp_at_full_size_matches_oracle():
+    # The viscous workload at its real size (one 256x128x128 block, Mach 5 ramp, no-slip ramp
+    # wall at 300 K, Sutherland air), 2 steps bit for bit against the oracle.
+    job = cases.ramp(256, 128, 128, max_step=2, viscous=True)
+    sim = Sim(job)
+    rep = sim.step(2)
+    orc = ob.Oracle(job)
+    rc, _, _ = orc.march(2)
+    assert rc == 0 and rep.step == orc.step == 2 and rep.t == orc.t
+    assert np.array_equal(sim.iovars(0), orc.iovars(0))
+    assert np.array_equal(sim.conserved(0), orc.conserved(0))
+    sim.close()
