@@ -2,13 +2,13 @@
 //
 // One launch per RK stage covers every tile of every active block (or one block's tiles
 // when the reference's per-block launch structure is requested).  A work-group owns a
-// TI x TJ column of cells and marches up a chunk of k-planes.  Per plane it evaluates a
-// single flattened list of faces -- the (TI+1)*TJ i-faces and TI*(TJ+1) j-faces of the
-// plane and the TI*TJ k-faces one plane up -- so every face flux of the tile is computed
-// exactly once by one call site, with lanes of a wave working on whichever face type
-// falls in their slot.  Area-weighted fluxes go to LDS; the owning thread then forms the
-// cell's flux divergence in the reference's summation order (cell.cu:206-213), applies
-// the TVD-RK3 stage (block.cu:1090-1160), decodes (flow.cu:245-268) and writes the next
+// TI x TJ column of cells and marches up a chunk of k-planes.  Per plane it stages the
+// plane's states (with a 2-cell halo) in LDS and evaluates the plane's (TI+1)*TJ i-faces
+// and TI*(TJ+1) j-faces as one flattened list, lanes of a wave taking whichever face type
+// falls in their slot; the area-weighted fluxes go to LDS.  Each thread then evaluates its
+// column's k-face one plane up (the face below was the previous plane's), forms its cell's
+// flux divergence in the reference's summation order (cell.cu:206-213), applies the
+// TVD-RK3 stage (block.cu:1090-1160), decodes (flow.cu:245-268) and writes the next
 // stage's primitive state.  Ghost cells are never stored: a stencil index outside the
 // block is resolved on the fly by its boundary condition (bcs.cu:16-503), with exchange
 // reading the neighbour block's current state directly.
