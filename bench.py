@@ -6,7 +6,7 @@ the CFL time-step reduction every cfl_count steps.  Default workload: the 256x12
 Mach 5 compression-ramp block (FP64, 1% seeded density perturbation), inputs resident in
 HBM before timing.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload scale|plain|batched|ensemble]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload scale|plain|batched|ensemble|...]
 
 The launch shape is autotuned first (chkn.tune) unless --tile is given.
 """
@@ -30,7 +30,7 @@ BYTES_PER_CELL_STEP_VISCOUS = BYTES_PER_CELL_STEP + 3 * 96
 
 
 WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300, "diff": 500,
-                  "ensemble-diff": 300, "viscous": 300}
+                  "ensemble-diff": 300, "viscous": 300, "shard": 500}
 # Steps of the reference CPU build timed for cpu_baseline: a sample of about 10-30 s on one
 # core (the viscous step runs about 6x slower there than the inviscid one).
 CPU_STEPS = {"viscous": 1}
@@ -48,6 +48,12 @@ def build_workload(name, steps_total):
         desc = {"workload": "viscous: one 256x128x128 block, Mach 5 flow over a 15-degree ramp "
                             "with viscous fluxes (least-squares face gradients, Sutherland air), "
                             "no-slip ramp wall at 300 K, 1% seeded density perturbation, FP64"}
+    elif name == "shard":
+        jobs = [cases.ramp_slab(64, 32, max_step=steps_total)]
+        desc = {"workload": "shard: the 32x128x128 i-slab (cells 64-95, the ramp corner) of the "
+                            "256x128x128 Mach 5 ramp block, one GPU's share of an 8-way split "
+                            "along i, run standalone (inflow and outflow in place of the "
+                            "neighbouring slabs), FP64"}
     elif name == "plain":
         jobs = [cases.sod_box(64, 32, 32, max_step=steps_total)]
         desc = {"workload": "plain: one 64x32x32 block, 3D Sod shock tube, slip walls, FP64"}

@@ -129,6 +129,22 @@ def ramp(nic=256, njc=128, nkc=128, max_step=500, seed=1, mach=5.0, p=2.0e3, T=2
     return JobData.from_prep()
 
 
+def ramp_slab(i0=64, ni=32, nic=256, njc=128, nkc=128, max_step=500, seed=1, mach=5.0, p=2.0e3,
+              T=220.0, title="Mach 5 ramp, one i-slab"):
+    """The i-slab [i0, i0+ni) of the ramp block of ramp(nic, njc, nkc): what one GPU would own
+    if the block were split along i over nic/ni GPUs (the default slab holds the ramp corner).
+    Standalone, its i-sides take the freestream (inflow) and an outflow where the neighbouring
+    slabs' exchange boundaries would be."""
+    _common(max_step, title=title + " %d-%d of %d (seed %d)" % (i0, i0 + ni, nic, seed))
+    grid = ramp_vertices(nic, njc, nkc).subgrid(i0, 0, 0, ni + 1, njc + 1, nkc + 1)
+    fs = freestream(mach, p, T)
+    st = perturbed_state(grid, fs, seed=seed)
+    bcs = {"iminus": prep.InflowBC(fs), "iplus": prep.OutflowBC(), "kminus": prep.WallWithSlipBC(),
+           "kplus": prep.OutflowBC(), "jminus": prep.WallWithSlipBC(), "jplus": prep.WallWithSlipBC()}
+    prep.FluidBlock(i=0, j=0, k=0, grid=grid, initialState=st, bcs=bcs)
+    return JobData.from_prep()
+
+
 def uneven_sizes(seed, nib=5, njb=4, nkb=2, lo=16, hi=96):
     rng = np.random.default_rng(seed)
     return (rng.integers(lo, hi + 1, size=nib), rng.integers(lo, hi + 1, size=njb),

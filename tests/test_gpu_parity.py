@@ -196,3 +196,17 @@ def test_batched_array_at_full_size_matches_oracle():
         if job.blocks[b].active:
             assert np.array_equal(sim.conserved(b), orc.conserved(b)), b
     sim.close()
+
+
+def test_ramp_slab_matches_oracle():
+    # One GPU's share of the 256x128x128 ramp split 8 ways along i (the 32x128x128 slab with
+    # the ramp corner), run standalone: 3 steps bit for bit against the oracle.
+    job = cases.ramp_slab(64, 32, max_step=3)
+    sim = Sim(job)
+    rep = sim.step(3)
+    orc = ob.Oracle(job)
+    rc, _, _ = orc.march(3)
+    assert rc == 0 and rep.step == orc.step == 3 and rep.t == orc.t
+    assert np.array_equal(sim.iovars(0), orc.iovars(0))
+    assert np.array_equal(sim.conserved(0), orc.conserved(0))
+    sim.close()
