@@ -266,7 +266,8 @@ def main():
         t0 = time.perf_counter()
         opts, table = tune.autotune(jobs=jobs, per_block_launch=bool(args.per_block))
         tuned = {"candidates": len(table), "seconds": round(time.perf_counter() - t0, 1),
-                 "best_ms_per_step": table[0][0], "worst_ms_per_step": table[-1][0]}
+                 "best_ms_per_step": table[0][0], "worst_ms_per_step": table[-1][0],
+                 "chosen_sweep_ms_per_step": next(ms for ms, o in table if o == opts)}
     else:
         opts = {}
     opts = dict(opts, use_graph=bool(args.graph), per_block_launch=bool(args.per_block))

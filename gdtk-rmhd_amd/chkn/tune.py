@@ -59,9 +59,11 @@ def _scratch(jobs):
 
 
 def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=None,
-             per_block_launch=False, verbose=False):
+             per_block_launch=False, verbose=False, refine=4, refine_steps=50):
     """Time every candidate for `steps` steps (after `warmup` untimed steps) on a scratch
-    handle and return (best_options, [(ms_per_step, options), ...] sorted fastest first).
+    handle, then time the `refine` fastest twice more for `refine_steps` steps each, and
+    return (best_options, [(ms_per_step, options), ...] of the sweep, fastest first).  The
+    short sweep cannot separate shapes a few percent apart; the longer runs pick among them.
 
     Timing is the device time of the step kernels (HIP events around the launches).  The
     scratch handle's state advances during the sweep; a candidate that meets a bad cell
@@ -80,29 +82,43 @@ def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=Non
     def fresh():
         return Sim(jobs=scratch) if len(scratch) > 1 else Sim(scratch[0])
 
-    sim = fresh()
+    sim = [fresh()]
+
+    def time_steps(opts, n):
+        for attempt in range(2):
+            try:
+                sim[0].set_launch(**opts)
+                if warmup:
+                    sim[0].step(warmup)
+                sim[0].step(n)
+                break
+            except BadCellError:
+                sim[0].close()
+                sim[0] = fresh()
+                if attempt == 1:
+                    raise
+        ms, _ = sim[0].last_timing()
+        return ms / n
+
     table = []
     for c in cands:
         opts = dict(c, use_graph=True, per_block_launch=per_block_launch)
-        for attempt in range(2):
-            try:
-                sim.set_launch(**opts)
-                if warmup:
-                    sim.step(warmup)
-                sim.step(steps)
-                break
-            except BadCellError:
-                sim.close()
-                sim = fresh()
-                if attempt == 1:
-                    raise
-        ms, _ = sim.last_timing()
-        table.append((ms / steps, opts))
+        table.append((time_steps(opts, steps), opts))
         if verbose:
-            print("%8.3f ms/step  %s" % (ms / steps, opts), flush=True)
-    sim.close()
+            print("%8.3f ms/step  %s" % (table[-1][0], opts), flush=True)
     table.sort(key=lambda r: r[0])
     best = table[0][1]
+    top = [o for _, o in table[:max(1, refine)]]
+    if len(top) > 1:
+        best_ms = [float("inf")] * len(top)
+        for _ in range(2):
+            for q, opts in enumerate(top):
+                best_ms[q] = min(best_ms[q], time_steps(opts, refine_steps))
+        best = top[min(range(len(top)), key=lambda q: best_ms[q])]
+        if verbose:
+            for q, opts in enumerate(top):
+                print("refined %8.3f ms/step  %s" % (best_ms[q], opts), flush=True)
+    sim[0].close()
     if cache_path:
         cache[key] = {"best": best, "table": table}
         d = os.path.dirname(os.path.abspath(cache_path))

@@ -176,6 +176,11 @@ hipError_t launch_chemistry(const BlockDev* blocks, const CellChunk* cells, int 
 hipError_t launch_stage(int stage, int x_order, int ti, int tj, bool dt_reduce, bool persistent,
                         int grid_limit, const StageArgs& a, hipStream_t s);
 bool tile_shape_supported(int ti, int tj);
+// Work-groups of a tile shape's stage kernel resident per CU (0 if unknown).
+int stage_slots_per_cu(int ti, int tj, int x_order);
+// The stage kernel maps work-group b to XCD b mod kXcds and gives each XCD a contiguous
+// range of the tile table.
+constexpr int kXcds = 8;
 
 // Minimum over the block's cells of the cfl-free signal time of its W[0] state, folded into
 // *sigmin_bits (used when no stage-3 launch has reduced the current state).

@@ -348,6 +348,27 @@ hipError_t launch_signal_min(const BlockDev* block, int ncells, unsigned long lo
     return hipGetLastError();
 }
 
+template <int TI, int TJ>
+int slots_shape(int x_order)
+{
+    int n = 0;
+    const hipError_t e =
+        x_order > 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<2, TI, TJ, 2, false>, TI * TJ, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<2, TI, TJ, 1, false>, TI * TJ, 0);
+    return e == hipSuccess ? n : 0;
+}
+
+int stage_slots_per_cu(int ti, int tj, int x_order)
+{
+    if (ti == 32 && tj == 8) return slots_shape<32, 8>(x_order);
+    if (ti == 64 && tj == 4) return slots_shape<64, 4>(x_order);
+    if (ti == 16 && tj == 16) return slots_shape<16, 16>(x_order);
+    if (ti == 64 && tj == 8) return slots_shape<64, 8>(x_order);
+    if (ti == 16 && tj == 4) return slots_shape<16, 4>(x_order);
+    if (ti == 8 && tj == 8) return slots_shape<8, 8>(x_order);
+    return 0;
+}
+
 bool tile_shape_supported(int ti, int tj)
 {
     return (ti == 32 && tj == 8) || (ti == 64 && tj == 4) || (ti == 16 && tj == 16) ||

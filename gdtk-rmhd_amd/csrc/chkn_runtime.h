@@ -170,6 +170,7 @@ struct chkn_sim {
     chkn_launch_opts opts{32, 8, 0, 0, 0, 0, 0};
     int kc = 8;
     int grid_limit = 0;
+    int ncu = 0;           // compute units of the device
     std::map<GraphKey, hipGraphExec_t> graphs;
     // timing of the last chkn_step call
     double last_ms = 0.0;

@@ -212,10 +212,32 @@ void upload_state(chkn_sim* s, HostBlock& b, const real* io)
     }
 }
 
+// k-chunk lengths of a single block's plane-major tile table that give every XCD the same
+// planes and a whole number of rounds of its resident work-groups: the planes in kXcds equal
+// slabs (one per XCD's range of the table), each slab in p layers of near-equal length,
+// longer first, for the smallest p (chunks of at most 8 planes) with cols * p a multiple of
+// the XCD's slots.  Empty when the planes do not divide into kXcds slabs or no p fits.  A
+// launch of, say, 5.33 rounds leaves the XCDs two-thirds idle through its last round.
+static std::vector<int> xcd_layer_lengths(int nkc, long long cols, long long slots_per_xcd)
+{
+    std::vector<int> len;
+    if (nkc % kXcds != 0 || slots_per_xcd <= 0) return len;
+    const int pg = nkc / kXcds;
+    for (int p = std::max(1, (pg + 7) / 8); p <= std::min(8, pg); ++p) {
+        if ((cols * p) % slots_per_xcd != 0) continue;
+        for (int g = 0; g < kXcds; ++g)
+            for (int c = 0; c < p; ++c) len.push_back(pg / p + (c < pg % p ? 1 : 0));
+        return len;
+    }
+    return len;
+}
+
 void build_tiles(chkn_sim* s)
 {
     std::vector<Tile> tiles;
     const int TI = s->opts.tile_i, TJ = s->opts.tile_j;
+    int nactive = 0;
+    for (auto& o : s->blocks) nactive += o.active ? 1 : 0;
     for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
         HostBlock& b = s->blocks[ib];
         b.tile_first = (int)tiles.size();
@@ -231,7 +253,20 @@ void build_tiles(chkn_sim* s)
                 long long want = std::max(1LL, cols / 2048);
                 kc = (int)std::min<long long>(want, 32);
             }
-            if (s->opts.tile_order == 1) {
+            std::vector<int> layers;
+            if (s->opts.tile_k <= 0 && s->opts.tile_order == 0 && nactive == 1) {
+                const long long slots = (long long)(s->ncu / kXcds) * stage_slots_per_cu(TI, TJ, s->x_order);
+                layers = xcd_layer_lengths(b.nkc, (long long)nti * ntj, slots);
+            }
+            if (!layers.empty()) {
+                int k0 = 0;
+                for (int nk : layers) {
+                    for (int tj = 0; tj < ntj; ++tj)
+                        for (int ti = 0; ti < nti; ++ti) tiles.push_back(Tile{(int)ib, ti * TI, tj * TJ, k0, nk});
+                    k0 += nk;
+                }
+                if (k0 != b.nkc) throw std::logic_error("k-chunk layers do not cover the block");
+            } else if (s->opts.tile_order == 1) {
                 for (int tj = 0; tj < ntj; ++tj)
                     for (int ti = 0; ti < nti; ++ti)
                         for (int k0 = 0; k0 < b.nkc; k0 += kc)
@@ -530,11 +565,12 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     HIPCHK(hipMalloc(&s->d_ctl, s->ctl_bytes));
     HIPCHK(hipHostMalloc(&s->h_ctl, s->ctl_bytes, hipHostMallocDefault));
     std::memset(s->h_ctl, 0, s->ctl_bytes);
-    build_tiles(s);
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    s->ncu = ncu;
     s->grid_limit = ncu * 2;
+    build_tiles(s);
     // SimState at start (simulate.cu:132-150, :190-191).
     for (int qi = 0; qi < ninst; ++qi) {
         Instance& I = s->inst[qi];
