@@ -292,12 +292,15 @@ def main():
     alg_bytes = cells * (per_step * steps + BYTES_PER_CELL_DT * reductions)
     value = cells * steps / wall
     achieved = alg_bytes / (kernel_ms * 1e-3)
+    # Measured HBM bytes per stage launch for this launch shape (TIxTJxTK, TK 0 = the default
+    # k-chunking), from the profile summary of the workload; null for a shape not profiled.
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "stage_kernel_traffic_%s.txt" % args.workload)
-    if os.path.exists(tfile):
+    if os.path.exists(tfile) and opts:
+        shape = "%dx%dx%d" % (opts.get("tile_i", 0), opts.get("tile_j", 0), opts.get("tile_k", 0))
         with open(tfile) as f:
             for ln in f:
-                if ln.startswith("hbm_bytes_per_launch:"):
+                if ln.startswith("hbm_bytes_per_launch %s:" % shape):
                     traffic = float(ln.split(":")[1])
     config = dict(desc, launch=opts, steps_completed=rep.step)
     if tuned:

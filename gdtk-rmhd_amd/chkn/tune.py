@@ -61,7 +61,8 @@ def _scratch(jobs):
 def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=None,
              per_block_launch=False, verbose=False, refine=4, refine_steps=50):
     """Time every candidate for `steps` steps (after `warmup` untimed steps) on a scratch
-    handle, then time the `refine` fastest twice more for `refine_steps` steps each, and
+    handle, then time the `refine` fastest twice more for `refine_steps` steps each (each run
+    on a fresh handle from the initial state), and
     return (best_options, [(ms_per_step, options), ...] of the sweep, fastest first).  The
     short sweep cannot separate shapes a few percent apart; the longer runs pick among them.
 
@@ -108,17 +109,21 @@ def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=Non
             print("%8.3f ms/step  %s" % (table[-1][0], opts), flush=True)
     table.sort(key=lambda r: r[0])
     best = table[0][1]
+    sim[0].close()
     top = [o for _, o in table[:max(1, refine)]]
     if len(top) > 1:
+        # Each from a fresh handle: the sweep's handle has stepped thousands of times, and the
+        # ranking of shapes a few percent apart can differ on that developed flow.
         best_ms = [float("inf")] * len(top)
         for _ in range(2):
             for q, opts in enumerate(top):
+                sim[0] = fresh()
                 best_ms[q] = min(best_ms[q], time_steps(opts, refine_steps))
+                sim[0].close()
         best = top[min(range(len(top)), key=lambda q: best_ms[q])]
         if verbose:
             for q, opts in enumerate(top):
                 print("refined %8.3f ms/step  %s" % (best_ms[q], opts), flush=True)
-    sim[0].close()
     if cache_path:
         cache[key] = {"best": best, "table": table}
         d = os.path.dirname(os.path.abspath(cache_path))
