@@ -59,7 +59,7 @@ def _scratch(jobs):
 
 
 def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=None,
-             per_block_launch=False, verbose=False, refine=4, refine_steps=50):
+             per_block_launch=False, verbose=False, refine=8, refine_steps=50):
     """Time every candidate for `steps` steps (after `warmup` untimed steps) on a scratch
     handle, then time the `refine` fastest twice more for `refine_steps` steps each (each run
     on a fresh handle from the initial state), and

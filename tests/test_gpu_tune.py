@@ -19,7 +19,7 @@ def test_autotune_picks_a_timed_candidate_and_caches(tmp_path):
     best, table = tune.autotune(job, steps=4, warmup=1, cands=cands, cache_path=cache)
     assert len(table) == len(cands)
     assert all(ms > 0 for ms, _ in table)
-    assert best in [o for _, o in table[:4]]  # re-timed among the sweep's four fastest
+    assert best in [o for _, o in table[:8]]  # re-timed among the sweep's eight fastest
     assert [ms for ms, _ in table] == sorted(ms for ms, _ in table)
     with open(cache) as f:
         assert tune.shape_key([job]) in json.load(f)
