@@ -234,6 +234,8 @@ def main():
     ap.add_argument("--tile", default="", help="TIxTJxTK launch tile, e.g. 32x8x8 (skips tuning)")
     ap.add_argument("--order", type=int, default=0, help="tile order with --tile")
     ap.add_argument("--persistent", type=int, default=0, help="persistent work-groups with --tile")
+    ap.add_argument("--vertex-frames", type=int, default=0,
+                    help="with --tile: the stage kernel forms face frames from the vertices")
     ap.add_argument("--tune", type=int, default=1, help="autotune the launch shape first")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--per-block", type=int, default=0)
@@ -261,7 +263,7 @@ def main():
     if args.tile:
         ti, tj, tk = (int(v) for v in args.tile.split("x"))
         opts = dict(tile_i=ti, tile_j=tj, tile_k=tk, tile_order=args.order,
-                    persistent=bool(args.persistent))
+                    persistent=bool(args.persistent), vertex_frames=bool(args.vertex_frames))
     elif args.tune and not jobs[0].viscous:  # the face-list stage has no tile shape
         t0 = time.perf_counter()
         opts, table = tune.autotune(jobs=jobs, per_block_launch=bool(args.per_block))

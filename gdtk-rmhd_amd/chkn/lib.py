@@ -47,7 +47,8 @@ class ChknReport(ctypes.Structure):
 class ChknLaunchOpts(ctypes.Structure):
     _fields_ = [("tile_i", ctypes.c_int), ("tile_j", ctypes.c_int), ("tile_k", ctypes.c_int),
                 ("per_block_launch", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("persistent", ctypes.c_int), ("tile_order", ctypes.c_int)]
+                ("persistent", ctypes.c_int), ("tile_order", ctypes.c_int),
+                ("vertex_frames", ctypes.c_int)]
 
 
 EXPORTS = {
@@ -261,9 +262,9 @@ class Sim:
         self._check(self._lib.chkn_set_iovars(self.h, blk, _ptr(io, _D)))
 
     def set_launch(self, tile_i=0, tile_j=0, tile_k=0, per_block_launch=False, use_graph=False,
-                   persistent=False, tile_order=0):
+                   persistent=False, tile_order=0, vertex_frames=False):
         o = ChknLaunchOpts(tile_i, tile_j, tile_k, int(per_block_launch), int(use_graph),
-                           int(persistent), int(tile_order))
+                           int(persistent), int(tile_order), int(vertex_frames))
         self._check(self._lib.chkn_set_launch_opts(self.h, ctypes.byref(o)))
 
     def report(self):

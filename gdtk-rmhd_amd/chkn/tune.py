@@ -21,9 +21,12 @@ TILES = [(32, 8), (64, 4), (16, 16), (64, 8), (16, 4), (8, 8)]
 TILE_K = [0, 1, 2, 4, 8, 16]
 ORDERS = [0, 1]
 PERSISTENT = [0, 1]
+# Shapes with the vertex-frame form of the stage kernel (face frames formed from the vertices,
+# x_order 2): tried with the default k-chunking and plane-major order.
+VERTEX_FRAME_TILES = [(32, 8), (16, 16)]
 
 
-def candidates(tiles=TILES, tile_k=TILE_K, orders=ORDERS, persistent=PERSISTENT):
+def candidates(tiles=TILES, tile_k=TILE_K, orders=ORDERS, persistent=PERSISTENT, x_order=2):
     """Every launch option combination to try (graph replay is always on)."""
     out = []
     for ti, tj in tiles:
@@ -32,6 +35,11 @@ def candidates(tiles=TILES, tile_k=TILE_K, orders=ORDERS, persistent=PERSISTENT)
                 for p in persistent:
                     out.append(dict(tile_i=ti, tile_j=tj, tile_k=tk, tile_order=o,
                                     persistent=bool(p)))
+    if x_order > 1:
+        for ti, tj in tiles:
+            if (ti, tj) in VERTEX_FRAME_TILES:
+                out.append(dict(tile_i=ti, tile_j=tj, tile_k=0, tile_order=0, persistent=False,
+                                vertex_frames=True))
     return out
 
 
@@ -70,7 +78,7 @@ def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=Non
     scratch handle's state advances during the sweep; a candidate that meets a bad cell
     rebuilds the handle from the initial state and is timed again."""
     jobs = list(jobs) if jobs is not None else [job]
-    cands = cands if cands is not None else candidates()
+    cands = cands if cands is not None else candidates(x_order=jobs[0].x_order)
     key = shape_key(jobs) + ("-pb" if per_block_launch else "")
     cache = {}
     if cache_path and os.path.exists(cache_path):

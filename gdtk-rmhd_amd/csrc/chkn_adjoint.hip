@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
         m = r / njc;
     }
     const int nline = line_len(B, d);
-    const int n = (int)B->n;
+    const long long n = B->n;
     const double* D = B->adj + 12 * (long long)n;
     const double* g = geom(B, d);
     const double area = g[fi];
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
 {
     const CellChunk ch = A.cells[blockIdx.x];
     const BlockDev* B = A.blocks + ch.blk;
-    const int n = (int)B->n;
+    const long long n = B->n;
     const int cidx = ch.c0 + (int)threadIdx.x;
     if (cidx >= ch.c1) return;
     const int nic = B->nic, njc = B->njc;
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void adj_seed_kernel(AdjArgs A)
 {
     const CellChunk ch = A.cells[blockIdx.x];
     const BlockDev* B = A.blocks + ch.blk;
-    const int n = (int)B->n;
+    const long long n = B->n;
     const int cidx = ch.c0 + (int)threadIdx.x;
     if (cidx >= ch.c1) return;
     const double dt = A.dt[B->inst];

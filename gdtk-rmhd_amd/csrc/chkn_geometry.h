@@ -6,12 +6,19 @@
 // hex_cell_properties with true_centroid = false (vector3.cu:350-433, block.cu:640-647),
 // whose volume is the sum of six tetragonal dipyramids (J. Grandy 1997), each the sum of
 // four tetrahedra about the base mid-point.  Operand order matches the reference so that
-// the values are bit-identical to its computeGeometry (block.cu:633-783).
+// the values are bit-identical to its computeGeometry (block.cu:633-783).  The face frame
+// part is host and device code: the stage kernels can form a face's frame from its four
+// vertices with the same operations, so the result has the same bits as the host pre-pass.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <math.h>
 
 #include "chkn_real.h"
+
+#ifndef CHKN_HD
+#define CHKN_HD __host__ __device__ __forceinline__
+#endif
 
 namespace chkn {
 namespace geom {
@@ -20,7 +27,7 @@ struct V3 {
     real x, y, z;
 };
 
-inline V3 normalized(V3 v)  // vector3.cu:77-93
+CHKN_HD V3 normalized(V3 v)  // vector3.cu:77-93
 {
     real m = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
     if (m > real(0)) { v.x /= m; v.y /= m; v.z /= m; }
@@ -37,13 +44,10 @@ struct Quad {
     real area;
 };
 
-inline Quad quad(const V3& p0, const V3& p1, const V3& p2, const V3& p3)
+// Area, unit normal and tangents of the quadrilateral p0 p1 p2 p3 (vector3.cu:205-244).
+CHKN_HD void quad_frame(const V3& p0, const V3& p1, const V3& p2, const V3& p3, Quad& q)
 {
-    Quad q;
     const real quarter = real(0.25);
-    q.c.x = quarter * (p0.x + p1.x + p2.x + p3.x);
-    q.c.y = quarter * (p0.y + p1.y + p2.y + p3.y);
-    q.c.z = quarter * (p0.z + p1.z + p2.z + p3.z);
     // Edge-pair sums along the two quad directions.
     real ax = p1.x - p0.x + p2.x - p3.x, ay = p1.y - p0.y + p2.y - p3.y, az = p1.z - p0.z + p2.z - p3.z;
     real bx = p3.x - p0.x + p2.x - p1.x, by = p3.y - p0.y + p2.y - p1.y, bz = p3.z - p0.z + p2.z - p1.z;
@@ -62,6 +66,16 @@ inline Quad quad(const V3& p0, const V3& p1, const V3& p2, const V3& p3)
         q.t1 = V3{real(0), real(1), real(0)};
         q.t2 = V3{real(0), real(0), real(1)};
     }
+}
+
+inline Quad quad(const V3& p0, const V3& p1, const V3& p2, const V3& p3)
+{
+    Quad q;
+    const real quarter = real(0.25);
+    q.c.x = quarter * (p0.x + p1.x + p2.x + p3.x);
+    q.c.y = quarter * (p0.y + p1.y + p2.y + p3.y);
+    q.c.z = quarter * (p0.z + p1.z + p2.z + p3.z);
+    quad_frame(p0, p1, p2, p3, q);
     return q;
 }
 

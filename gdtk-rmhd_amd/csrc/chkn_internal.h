@@ -13,6 +13,7 @@
 //     j-faces: nic x (njc+1) x nkc, index i + nic*(j + (njc+1)*k)
 //     k-faces: nic x njc x (nkc+1), index i + nic*(j + njc*k)
 //   volinv[n] = 1/volume, lengths[3][n] (iLength, jLength, kLength)
+//   vtx[3][nv]   vertex x, y, z, nv = (nic+1)(njc+1)(nkc+1), index i + (nic+1)*(j + (njc+1)*k)
 // Face-list stage (viscous jobs; first-order jobs whose file or inflow gas states are not
 // reproduced by the equation of state), faces numbered i-faces, then j-faces (+nfi), then
 // k-faces (+nfi+nfj), nft faces in all:
@@ -50,6 +51,10 @@ struct BlockDev {
     const real* gk;
     const real* volinv;
     const real* len;
+    // Vertices [3][nv] (x, y, z), nv = (nic+1)(njc+1)(nkc+1), index i + (nic+1)*(j + (njc+1)*k):
+    // the stage kernel forms face frames from them (active blocks).
+    const real* vtx;
+    long long nv;
     // adjoint
     int rp[6];             // block whose opposite side exchanges into this side, or -1
     int icol;              // first inflow line of this block in the inflow accumulator
@@ -172,10 +177,12 @@ hipError_t launch_chemistry(const BlockDev* blocks, const CellChunk* cells, int 
                             const StepCtl& ctl, int slot, bool dt_reduce, hipStream_t s);
 
 // Launch one stage over tiles [tile_base, tile_base + ntiles) of the table.
-// stage in 1..3; x_order 1 or 2; tile shape (ti, tj) must be a built variant.
-hipError_t launch_stage(int stage, int x_order, int ti, int tj, bool dt_reduce, bool persistent,
-                        int grid_limit, const StageArgs& a, hipStream_t s);
+// stage in 1..3; x_order 1 or 2; tile shape (ti, tj) must be built; vertex_frames 1 forms
+// the face frames from the block's vertices (built for 32x8 and 16x16 at x_order 2, FP64).
+hipError_t launch_stage(int stage, int x_order, int ti, int tj, int vertex_frames, bool dt_reduce,
+                        bool persistent, int grid_limit, const StageArgs& a, hipStream_t s);
 bool tile_shape_supported(int ti, int tj);
+bool vertex_frames_supported(int ti, int tj, int x_order);
 // Work-groups of a tile shape's stage kernel resident per CU (0 if unknown).
 int stage_slots_per_cu(int ti, int tj, int x_order);
 // The stage kernel maps work-group b to XCD b mod kXcds and gives each XCD a contiguous

@@ -2,14 +2,15 @@
 //
 // One launch per RK stage covers every tile of every active block (or one block's tiles
 // when the reference's per-block launch structure is requested).  A work-group owns a
-// TI x TJ column of cells and marches up a chunk of k-planes.  Per plane it stages the
-// plane's states (with a 2-cell halo) in LDS and evaluates the plane's (TI+1)*TJ i-faces
-// and TI*(TJ+1) j-faces as one flattened list, lanes of a wave taking whichever face type
-// falls in their slot; the area-weighted fluxes go to LDS.  Each thread then evaluates its
-// column's k-face one plane up (the face below was the previous plane's), forms its cell's
-// flux divergence in the reference's summation order (cell.cu:206-213), applies the
-// TVD-RK3 stage (block.cu:1090-1160), decodes (flow.cu:245-268) and writes the next
-// stage's primitive state.  Ghost cells are never stored: a stencil index outside the
+// TI x TJ column of cells and marches up a chunk of k-planes.  With the plane's states (and
+// a 2-cell halo) staged in LDS, it evaluates the plane's (TI+1)*TJ i-faces and TI*(TJ+1)
+// j-faces as one flattened list, lanes of a wave taking whichever face type falls in their
+// slot; the area-weighted fluxes go to LDS.  Each thread then evaluates its column's k-face
+// one plane up (the face below was the previous plane's), forms its cell's flux divergence
+// in the reference's summation order (cell.cu:206-213), applies the TVD-RK3 stage
+// (block.cu:1090-1160), decodes (flow.cu:245-268) and writes the next stage's primitive
+// state, while the work-group loads the next plane's halo into LDS (two barriers per
+// plane).  Ghost cells are never stored: a stencil index outside the
 // block is resolved on the fly by its boundary condition (bcs.cu:16-503), with exchange
 // reading the neighbour block's current state directly.
 //
@@ -22,12 +23,13 @@ namespace chkn {
 
 namespace {
 
+template <bool VF>
 __device__ __forceinline__ Prim stencil_prim(const StageArgs& A, const BlockDev* B,
                                              const real* __restrict__ W, int n_line, int base,
                                              int stride, int dir, int m, int a, int c)
 {
-    if (m >= 0 && m < n_line) return ld_prim(W, (int)B->n, base + m * stride);
-    return ghost_prim(A.blocks, A.inflow, A.win, B, dir, m, a, c);
+    if (m >= 0 && m < n_line) return ld_prim(W, B->n, base + m * stride);
+    return ghost_prim<VF>(A.blocks, A.inflow, A.win, B, dir, m, a, c);
 }
 
 // LDS layout of a tile: the k-plane of primitive states with a 2-cell halo in i and j
@@ -45,11 +47,12 @@ struct TileLds {
 
 // Load one plane position (ii, jj relative to the tile origin) for plane k into LDS:
 // an interior cell, a ghost cell across an i- or j-boundary, or nothing if no face of the
-// tile can reference it.
-template <int TI, int TJ>
+// tile can reference it.  skip_own: the tile's own interior cells are already in place (the
+// threads that own them stored them from their k-stencil loads of the previous plane).
+template <int TI, int TJ, bool VF>
 __device__ __forceinline__ void fill_plane_position(const StageArgs& A, const BlockDev* B,
                                                     const real* __restrict__ Win, int i0,
-                                                    int j0, int k, int p, real* Wp)
+                                                    int j0, int k, int p, real* Wp, bool skip_own)
 {
     using L = TileLds<TI, TJ>;
     int ii, jj;
@@ -67,11 +70,12 @@ __device__ __forceinline__ void fill_plane_position(const StageArgs& A, const Bl
     const bool in_i = (gi >= 0) && (gi < nic), in_j = (gj >= 0) && (gj < njc);
     Prim w;
     if (in_i && in_j) {
-        w = ld_prim(Win, (int)B->n, gi + nic * (gj + njc * k));
+        if (skip_own && ii >= 0 && ii < TI && jj >= 0 && jj < TJ) return;
+        w = ld_prim(Win, B->n, gi + nic * (gj + njc * k));
     } else if (in_j && gi >= -2 && gi <= nic + 1) {
-        w = ghost_prim(A.blocks, A.inflow, A.win, B, 0, gi, gj, k);
+        w = ghost_prim<VF>(A.blocks, A.inflow, A.win, B, 0, gi, gj, k);
     } else if (in_i && gj >= -2 && gj <= njc + 1) {
-        w = ghost_prim(A.blocks, A.inflow, A.win, B, 1, gj, gi, k);
+        w = ghost_prim<VF>(A.blocks, A.inflow, A.win, B, 1, gj, gi, k);
     } else {
         return;
     }
@@ -96,23 +100,47 @@ __device__ __forceinline__ Prim lds_prim(const real* Wp, int np, int o)
     return p;
 }
 
+__device__ __forceinline__ void lds_put(real* Wp, int np, int o, const Prim& p)
+{
+    Wp[o] = p.rho;
+    Wp[np + o] = p.e;
+    Wp[2 * np + o] = p.YB;
+    Wp[3 * np + o] = p.vx;
+    Wp[4 * np + o] = p.vy;
+    Wp[5 * np + o] = p.vz;
+}
+
 // Area-weighted flux of the k-face (i, j, kf) between cells kf-1 and kf from its stencil
 // states (cells kf-2 .. kf+1 of the column).
-template <int XORDER>
+template <int XORDER, bool VF>
 __device__ __forceinline__ void k_face_flux(const BlockDev* B, int i, int j, int kf, const Prim& s0,
                                             const Prim& s1, const Prim& s2, const Prim& s3,
                                             real AF[6])
 {
-    const int fi = i + B->nic * (j + B->njc * kf);
-    const Frame fr = ld_frame(B->gk, (int)B->nfk, fi);
-    const real area = gld(B->gk, boff(fi));
+    Frame fr;
+    real area;
+    if (VF) {
+        face_frame_v<2>(B, i, j, kf, fr, area);
+    } else {
+        const int fi = i + B->nic * (j + B->njc * kf);
+        fr = ld_frame(B->gk, B->nfk, fi);
+        area = gld(B->gk, boff(fi));
+    }
     real F[6];
     face_flux<XORDER>(s0, s1, s2, s3, fr, F);
 #pragma unroll
     for (int q = 0; q < 6; ++q) AF[q] = area * F[q];
 }
 
-template <int STAGE, int TI, int TJ, int XORDER, bool DTRED>
+// One tile: TI x TJ columns of cells marching planes k0 .. k0+nk-1.  Per plane, after the
+// plane's states are in LDS: (1) the plane's i- and j-faces as one flattened list, fluxes to
+// LDS; barrier; (2) each thread's cell: its column's k-face one plane up, the flux
+// divergence, the RK stage and decode, while all threads fill the next plane's positions
+// (nothing in this part reads them: a thread's own cell is read back by that thread only,
+// and it stores its k-stencil cell k+1 there itself); barrier.  VF: face frames formed from
+// the block's vertices (same bits as the stored geometry; 3 words per vertex read instead of
+// 10 per face, for more arithmetic).
+template <int STAGE, int TI, int TJ, int XORDER, bool DTRED, bool VF>
 __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile, real* lds)
 {
     using L = TileLds<TI, TJ>;
@@ -124,7 +152,7 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
 
     const BlockDev* B = A.blocks + tile.blk;
     const int nic = B->nic, njc = B->njc;
-    const int n = (int)B->n;
+    const long long n = B->n;
     const real* __restrict__ Win = B->W + A.win * B->wstride;
     const int t = threadIdx.x;
     const int ti = t % TI, tj = t / TI;
@@ -132,6 +160,8 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
     const bool own = (i < nic) && (j < njc);
     const int kbeg = tile.k0, kend = tile.k0 + tile.nk;
     const real dt = A.ctl.dt[B->inst];
+    const int vsj = nic + 1, vsk = (nic + 1) * (njc + 1);
+    const int o = (tj + 2) * PW + ti + 2;  // the thread's own cell in the LDS plane
     int bad = 0;
     unsigned long long sigmin = ~0ull;
 
@@ -140,48 +170,59 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
     const int nkc = B->nkc, kbase = i + nic * j, kstride = nic * njc;
     real AFkm[6] = {0, 0, 0, 0, 0, 0};
     if (own) {
-        const Prim s0 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, kbeg - 2, i, j);
-        const Prim s1 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, kbeg - 1, i, j);
-        const Prim s2 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, kbeg, i, j);
-        const Prim s3 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, kbeg + 1, i, j);
-        k_face_flux<XORDER>(B, i, j, kbeg, s0, s1, s2, s3, AFkm);
+        const Prim s0 = stencil_prim<VF>(A, B, Win, nkc, kbase, kstride, 2, kbeg - 2, i, j);
+        const Prim s1 = stencil_prim<VF>(A, B, Win, nkc, kbase, kstride, 2, kbeg - 1, i, j);
+        const Prim s2 = stencil_prim<VF>(A, B, Win, nkc, kbase, kstride, 2, kbeg, i, j);
+        const Prim s3 = stencil_prim<VF>(A, B, Win, nkc, kbase, kstride, 2, kbeg + 1, i, j);
+        k_face_flux<XORDER, VF>(B, i, j, kbeg, s0, s1, s2, s3, AFkm);
     }
+    // The first plane, with halo and ghost cells.
+    for (int p = t; p < L::NLOAD; p += NT)
+        fill_plane_position<TI, TJ, VF>(A, B, Win, tile.i0, tile.j0, kbeg, p, Wp, false);
+    __syncthreads();
 
     for (int k = kbeg; k < kend; ++k) {
-        // Stage the plane (with halo and ghost cells) in LDS.
-        for (int p = t; p < L::NLOAD; p += NT)
-            fill_plane_position<TI, TJ>(A, B, Win, tile.i0, tile.j0, k, p, Wp);
-        __syncthreads();
         // i- and j-faces of the plane, one flattened list, stencils read from LDS.
         for (int f = t; f < NFI + NFJ; f += NT) {
-            int base, stride, fa, fb, fi, nf;
-            const real* g;
+            int base, stride, fi, gi, gj;
             real* dst;
             int lstride;
-            bool valid;
+            bool valid, idir;
             if (f < NFI) {
-                fa = f % (TI + 1);
-                fb = f / (TI + 1);
-                const int gi = tile.i0 + fa, gj = tile.j0 + fb;
+                const int fa = f % (TI + 1), fb = f / (TI + 1);
+                gi = tile.i0 + fa; gj = tile.j0 + fb;
                 valid = (gi <= nic) && (gj < njc);
                 base = (fb + 2) * PW + fa;  // cells fa-2 .. fa+1 of row fb
                 stride = 1;
-                g = B->gi; nf = (int)B->nfi; fi = gi + (nic + 1) * (gj + njc * k);
+                idir = true;
+                fi = gi + (nic + 1) * (gj + njc * k);
                 dst = AFi + fb * (TI + 1) + fa; lstride = NFI;
             } else {
                 const int h = f - NFI;
-                fa = h % TI;
-                fb = h / TI;
-                const int gi = tile.i0 + fa, gj = tile.j0 + fb;
+                const int fa = h % TI, fb = h / TI;
+                gi = tile.i0 + fa; gj = tile.j0 + fb;
                 valid = (gi < nic) && (gj <= njc);
                 base = fb * PW + fa + 2;    // rows fb-2 .. fb+1 of column fa
                 stride = PW;
-                g = B->gj; nf = (int)B->nfj; fi = gi + nic * (gj + (njc + 1) * k);
+                idir = false;
+                fi = gi + nic * (gj + (njc + 1) * k);
                 dst = AFj + fb * TI + fa; lstride = NFJ;
             }
             if (valid) {
-                const Frame fr = ld_frame(g, nf, fi);
-                const real area = gld(g, boff(fi));
+                Frame fr;
+                real area;
+                if (VF) {
+                    // i-face vertices (i,j,k) (i,j+1,k) (i,j+1,k+1) (i,j,k+1); j-face
+                    // (i+1,j,k) (i,j,k) (i,j,k+1) (i+1,j,k+1).
+                    const int v = gi + vsj * gj + vsk * k;
+                    if (idir) vertex_frame(B, v, vsj, vsj + vsk, vsk, fr, area);
+                    else vertex_frame(B, v + 1, -1, vsk - 1, vsk, fr, area);
+                } else {
+                    const real* g = idir ? B->gi : B->gj;
+                    const long long nf = idir ? B->nfi : B->nfj;
+                    fr = ld_frame(g, nf, fi);
+                    area = gld(g, boff(fi));
+                }
                 const Prim s0 = lds_prim(Wp, NP, base);
                 const Prim s1 = lds_prim(Wp, NP, base + stride);
                 const Prim s2 = lds_prim(Wp, NP, base + 2 * stride);
@@ -193,16 +234,23 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
             }
         }
         __syncthreads();
+        Prim sk{0, 0, 0, 0, 0, 0};
+        if (own) sk = lds_prim(Wp, NP, o);
+        // The next plane's positions other than the tile's own cells.
+        if (k + 1 < kend) {
+            for (int p = t; p < L::NLOAD; p += NT)
+                fill_plane_position<TI, TJ, VF>(A, B, Win, tile.i0, tile.j0, k + 1, p, Wp, true);
+        }
         if (own) {
             // The column's k-face one plane up, evaluated after the plane's i- and j-faces so
             // that its flux is not held in registers through the face loop.
             real AFkp[6];
             {
-                const Prim sk = lds_prim(Wp, NP, (tj + 2) * PW + ti + 2);
-                const Prim s2 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, k + 1, i, j);
-                const Prim s3 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, k + 2, i, j);
-                const Prim s0 = stencil_prim(A, B, Win, nkc, kbase, kstride, 2, k - 1, i, j);
-                k_face_flux<XORDER>(B, i, j, k + 1, s0, sk, s2, s3, AFkp);
+                const Prim s2 = stencil_prim<VF>(A, B, Win, nkc, kbase, kstride, 2, k + 1, i, j);
+                const Prim s3 = stencil_prim<VF>(A, B, Win, nkc, kbase, kstride, 2, k + 2, i, j);
+                const Prim s0 = stencil_prim<VF>(A, B, Win, nkc, kbase, kstride, 2, k - 1, i, j);
+                k_face_flux<XORDER, VF>(B, i, j, k + 1, s0, sk, s2, s3, AFkp);
+                if (k + 1 < kend) lds_put(Wp, NP, o, s2);  // the next plane's own cell
             }
             const int cidx = i + nic * (j + njc * k);
             const real vinv = gld(B->volinv, boff(cidx));
@@ -220,7 +268,7 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
             const int flag = stage_update<STAGE>(B, n, cidx, dt, d, A.wout, w);
             bad += flag;
             if (DTRED && !flag) {
-                const unsigned long long bits = signal_bits(B, n, i, j, k, cidx, w);
+                const unsigned long long bits = signal_bits<VF>(B, n, i, j, k, cidx, w);
                 sigmin = bits < sigmin ? bits : sigmin;
             }
 #pragma unroll
@@ -240,7 +288,7 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
     }
 }
 
-template <int STAGE, int TI, int TJ, int XORDER, bool DTRED>
+template <int STAGE, int TI, int TJ, int XORDER, bool DTRED, bool VF>
 __global__ __launch_bounds__(TI* TJ) void stage_kernel(StageArgs A, int persistent)
 {
     __shared__ real lds[TileLds<TI, TJ>::DOUBLES];
@@ -258,12 +306,13 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(StageArgs A, int persiste
     for (int tt = t0 + local; tt < t1; tt += per_xcd) {
         const Tile tile = A.tiles[A.tile_base + tt];
         // Instances that have reached their end time skip the step (uniform per tile).
-        if (A.ctl.run[A.blocks[tile.blk].inst]) process_tile<STAGE, TI, TJ, XORDER, DTRED>(A, tile, lds);
+        if (A.ctl.run[A.blocks[tile.blk].inst])
+            process_tile<STAGE, TI, TJ, XORDER, DTRED, VF>(A, tile, lds);
         if (!persistent) break;
     }
 }
 
-template <int TI, int TJ, int XORDER>
+template <int TI, int TJ, int XORDER, bool VF>
 hipError_t launch_shape(int stage, bool dtred, bool persistent, int grid_limit,
                         const StageArgs& a, hipStream_t s)
 {
@@ -274,34 +323,57 @@ hipError_t launch_shape(int stage, bool dtred, bool persistent, int grid_limit,
     dim3 blk(TI * TJ);
     int p = persistent ? 1 : 0;
     if (stage == 1)
-        hipLaunchKernelGGL((stage_kernel<1, TI, TJ, XORDER, false>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<1, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
     else if (stage == 2)
-        hipLaunchKernelGGL((stage_kernel<2, TI, TJ, XORDER, false>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<2, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
     else if (dtred)
-        hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, true>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, true, VF>), dim3(grid), blk, 0, s, a, p);
     else
-        hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, false>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
     return hipGetLastError();
 }
 
+// Shapes with the vertex-frame form built (FP64, second-order reconstruction).
+template <int TI, int TJ, int XORDER>
+constexpr bool vertex_frames_built()
+{
+#ifdef CHKN_FP32
+    return false;
+#else
+    return XORDER == 2 && ((TI == 32 && TJ == 8) || (TI == 16 && TJ == 16));
+#endif
+}
+
+template <int TI, int TJ, int XORDER>
+hipError_t launch_frames(int vertex_frames, int stage, bool dtred, bool persistent, int grid_limit,
+                          const StageArgs& a, hipStream_t s)
+{
+    if constexpr (vertex_frames_built<TI, TJ, XORDER>()) {
+        if (vertex_frames) return launch_shape<TI, TJ, XORDER, true>(stage, dtred, persistent, grid_limit, a, s);
+    } else {
+        if (vertex_frames) return hipErrorInvalidValue;
+    }
+    return launch_shape<TI, TJ, XORDER, false>(stage, dtred, persistent, grid_limit, a, s);
+}
+
 template <int TI, int TJ>
-hipError_t launch_order(int stage, int x_order, bool dtred, bool persistent, int grid_limit,
+hipError_t launch_order(int vertex_frames, int stage, int x_order, bool dtred, bool persistent, int grid_limit,
                         const StageArgs& a, hipStream_t s)
 {
-    if (x_order > 1) return launch_shape<TI, TJ, 2>(stage, dtred, persistent, grid_limit, a, s);
-    return launch_shape<TI, TJ, 1>(stage, dtred, persistent, grid_limit, a, s);
+    if (x_order > 1) return launch_frames<TI, TJ, 2>(vertex_frames, stage, dtred, persistent, grid_limit, a, s);
+    return launch_frames<TI, TJ, 1>(vertex_frames, stage, dtred, persistent, grid_limit, a, s);
 }
 
 __global__ __launch_bounds__(256) void signal_min_kernel(const BlockDev* B,
                                                         unsigned long long* sigmin_bits)
 {
-    const int n = (int)B->n;
+    const long long n = B->n;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long v = ~0ull;
     if (c < n) {
         const int nic = B->nic, njc = B->njc;
         const int i = c % nic, j = (c / nic) % njc, k = c / (nic * njc);
-        v = signal_bits(B, n, i, j, k, c, ld_prim(B->W, n, c));
+        v = signal_bits<false>(B, n, i, j, k, c, ld_prim(B->W, n, c));
     }
     v = wave_min_u64(v);
     if ((threadIdx.x & 63) == 0 && v != ~0ull) atomicMin(sigmin_bits, v);
@@ -312,7 +384,7 @@ __global__ __launch_bounds__(256) void decode_all_kernel(const BlockDev* blocks,
 {
     const CellChunk ch = cells[blockIdx.x];
     const BlockDev* B = blocks + ch.blk;
-    const int n = (int)B->n;
+    const long long n = B->n;
     const int cidx = ch.c0 + (int)threadIdx.x;
     if (cidx >= ch.c1) return;
     real U[6];
@@ -353,8 +425,8 @@ int slots_shape(int x_order)
 {
     int n = 0;
     const hipError_t e =
-        x_order > 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<2, TI, TJ, 2, false>, TI * TJ, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<2, TI, TJ, 1, false>, TI * TJ, 0);
+        x_order > 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<2, TI, TJ, 2, false, false>, TI * TJ, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<2, TI, TJ, 1, false, false>, TI * TJ, 0);
     return e == hipSuccess ? n : 0;
 }
 
@@ -375,16 +447,23 @@ bool tile_shape_supported(int ti, int tj)
            (ti == 64 && tj == 8) || (ti == 16 && tj == 4) || (ti == 8 && tj == 8);
 }
 
-hipError_t launch_stage(int stage, int x_order, int ti, int tj, bool dt_reduce, bool persistent,
+bool vertex_frames_supported(int ti, int tj, int x_order)
+{
+    if (ti == 32 && tj == 8) return x_order > 1 ? vertex_frames_built<32, 8, 2>() : vertex_frames_built<32, 8, 1>();
+    if (ti == 16 && tj == 16) return x_order > 1 ? vertex_frames_built<16, 16, 2>() : vertex_frames_built<16, 16, 1>();
+    return false;
+}
+
+hipError_t launch_stage(int stage, int x_order, int ti, int tj, int vertex_frames, bool dt_reduce, bool persistent,
                         int grid_limit, const StageArgs& a, hipStream_t s)
 {
     if (a.ntiles <= 0) return hipSuccess;
-    if (ti == 32 && tj == 8) return launch_order<32, 8>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
-    if (ti == 64 && tj == 4) return launch_order<64, 4>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
-    if (ti == 16 && tj == 16) return launch_order<16, 16>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
-    if (ti == 64 && tj == 8) return launch_order<64, 8>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
-    if (ti == 16 && tj == 4) return launch_order<16, 4>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
-    if (ti == 8 && tj == 8) return launch_order<8, 8>(stage, x_order, dt_reduce, persistent, grid_limit, a, s);
+    if (ti == 32 && tj == 8) return launch_order<32, 8>(vertex_frames, stage, x_order, dt_reduce, persistent, grid_limit, a, s);
+    if (ti == 64 && tj == 4) return launch_order<64, 4>(vertex_frames, stage, x_order, dt_reduce, persistent, grid_limit, a, s);
+    if (ti == 16 && tj == 16) return launch_order<16, 16>(vertex_frames, stage, x_order, dt_reduce, persistent, grid_limit, a, s);
+    if (ti == 64 && tj == 8) return launch_order<64, 8>(vertex_frames, stage, x_order, dt_reduce, persistent, grid_limit, a, s);
+    if (ti == 16 && tj == 4) return launch_order<16, 4>(vertex_frames, stage, x_order, dt_reduce, persistent, grid_limit, a, s);
+    if (ti == 8 && tj == 8) return launch_order<8, 8>(vertex_frames, stage, x_order, dt_reduce, persistent, grid_limit, a, s);
     return hipErrorInvalidValue;
 }
 

@@ -51,6 +51,7 @@ struct HostBlock {
     real* gk = nullptr;
     real* volinv = nullptr;
     real* len = nullptr;
+    real* vtx = nullptr;  // [3][nv] vertices (active blocks)
     int tile_first = 0, tile_count = 0;
     // face-list stage (viscous jobs; first-order jobs with verbatim gas states)
     real twall[6] = {0, 0, 0, 0, 0, 0};
@@ -167,7 +168,7 @@ struct chkn_sim {
     size_t ctl_bytes = 0;
     int ntiles = 0;
     // launch options
-    chkn_launch_opts opts{32, 8, 0, 0, 0, 0, 0};
+    chkn_launch_opts opts{32, 8, 0, 0, 0, 0, 0, 0};
     int kc = 8;
     int grid_limit = 0;
     int ncu = 0;           // compute units of the device

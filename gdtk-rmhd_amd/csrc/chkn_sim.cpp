@@ -78,8 +78,8 @@ void free_device(chkn_sim* s)
     s->graphs.clear();
     for (auto& b : s->blocks) {
         dfree(b.W); dfree(b.U); dfree(b.L); dfree(b.gi); dfree(b.gj); dfree(b.gk);
-        dfree(b.volinv); dfree(b.len);
-        b.W = b.U = b.L = b.gi = b.gj = b.gk = b.volinv = b.len = nullptr;
+        dfree(b.volinv); dfree(b.len); dfree(b.vtx);
+        b.W = b.U = b.L = b.gi = b.gj = b.gk = b.volinv = b.len = b.vtx = nullptr;
     }
     dfree(s->d_blocks); dfree(s->d_tiles); dfree(s->d_inflow); dfree(s->d_ctl);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
@@ -510,8 +510,8 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             bool want_fpos = s->viscous && b.active;
             for (int f = 0; f < 6; ++f) want_fpos = want_fpos || (b.active && b.bc[f] == CHKN_BC_INFLOW_FUNCTION);
             const size_t nv = (size_t)3 * (b.nic + 1) * (b.njc + 1) * (b.nkc + 1);
-            block_geometry(b, to_real(d->vertices + voff, nv).data(), gi, gj, gk, volinv, len,
-                           want_fpos ? &fpos : nullptr);
+            const std::vector<real> vxyz = to_real(d->vertices + voff, nv);
+            block_geometry(b, vxyz.data(), gi, gj, gk, volinv, len, want_fpos ? &fpos : nullptr);
             voff += nv;
             size_t n = (size_t)b.n, sz6 = sizeof(real) * 6 * n;
             HIPCHK(hipMalloc(&b.W, sz6 * (b.active ? 3 : 1)));
@@ -541,6 +541,15 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
                 HIPCHK(hipMemcpy(b.len, len.data(), sizeof(real) * 3 * n, hipMemcpyHostToDevice));
                 bd.U = b.U; bd.L = b.L; bd.gi = b.gi; bd.gj = b.gj; bd.gk = b.gk;
                 bd.volinv = b.volinv; bd.len = b.len;
+                // Vertices as [3][nv] (x, then y, then z) for the frames formed on the device.
+                const size_t nvert = nv / 3;
+                std::vector<real> soa(nv);
+                for (size_t v = 0; v < nvert; ++v)
+                    for (int c = 0; c < 3; ++c) soa[c * nvert + v] = vxyz[3 * v + c];
+                HIPCHK(hipMalloc(&b.vtx, sizeof(real) * nv));
+                HIPCHK(hipMemcpy(b.vtx, soa.data(), sizeof(real) * nv, hipMemcpyHostToDevice));
+                bd.vtx = b.vtx;
+                bd.nv = (long long)nvert;
             }
             for (int f = 0; f < 6; ++f) {
                 bd.rp[f] = rpart[ib][f];
@@ -1078,6 +1087,10 @@ int chkn_set_launch_opts(chkn_sim* sim, const chkn_launch_opts* o)
     if (n.tile_j == 0) n.tile_j = 8;
     if (!tile_shape_supported(n.tile_i, n.tile_j))
         return set_error(sim, CHKN_ERR_ARG, "unsupported tile shape");
+    if (n.vertex_frames != 0 && n.vertex_frames != 1)
+        return set_error(sim, CHKN_ERR_ARG, "vertex_frames must be 0 or 1");
+    if (n.vertex_frames && !vertex_frames_supported(n.tile_i, n.tile_j, sim->x_order))
+        return set_error(sim, CHKN_ERR_ARG, "vertex-frame stage kernel not built for this tile shape");
     if (n.tile_k < 0) return set_error(sim, CHKN_ERR_ARG, "tile_k must be >= 0");
     if (n.tile_order != 0 && n.tile_order != 1) return set_error(sim, CHKN_ERR_ARG, "tile_order must be 0 or 1");
     return guarded(sim, [&] {

@@ -67,7 +67,7 @@ __device__ __forceinline__ Gas cell_gas(const SplitArgs& A, const BlockDev* B, c
                                         int c)
 {
     if (B->gas && (A.file_gas || !B->active)) {
-        const int n = (int)B->n;
+        const long long n = B->n;
         const unsigned o = boff(c);
         return Gas{gld(B->gas, o), gld(B->gas + n, o), gld(B->gas + 2 * n, o)};
     }
@@ -89,7 +89,7 @@ __device__ __forceinline__ Point point(const SplitArgs& A, const BlockDev* B, in
     Point r;
     if (m >= 0 && m < nl) {
         const int idx = cell_index(B->nic, B->njc, dir, m, a, c);
-        r.w = ld_prim(B->W + A.win * B->wstride, (int)B->n, idx);
+        r.w = ld_prim(B->W + A.win * B->wstride, B->n, idx);
         if (GAS) r.g = cell_gas(A, B, r.w, idx);
         return r;
     }
@@ -101,7 +101,7 @@ __device__ __forceinline__ Point point(const SplitArgs& A, const BlockDev* B, in
         const BlockDev* O = A.blocks + B->bc_arg[bf];
         const int no = dir == 0 ? O->nic : (dir == 1 ? O->njc : O->nkc);
         const int idx = cell_index(O->nic, O->njc, dir, side ? depth : no - 1 - depth, a, c);
-        r.w = ld_prim(O->W + A.win * O->wstride, (int)O->n, idx);
+        r.w = ld_prim(O->W + A.win * O->wstride, O->n, idx);
         if (GAS) r.g = cell_gas(A, O, r.w, idx);
         return r;
     }
@@ -123,7 +123,7 @@ __device__ __forceinline__ Point point(const SplitArgs& A, const BlockDev* B, in
         return r;
     }
     const int idx = cell_index(B->nic, B->njc, dir, side ? nl - 1 : 0, a, c);
-    r.w = ld_prim(B->W + A.win * B->wstride, (int)B->n, idx);
+    r.w = ld_prim(B->W + A.win * B->wstride, B->n, idx);
     if (GAS) r.g = cell_gas(A, B, r.w, idx);
     if (code == 5) return r;  // outflow copies the interior cell
     const real* g;          // walls reflect it; p, T, a are copied unchanged
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A)
     unsigned long long sigmin = ~0ull;
     if (cidx < ch.c1) {
         const int nic = B->nic, njc = B->njc;
-        const int n = (int)B->n;
+        const long long n = B->n;
         const int i = cidx % nic, j = (cidx / nic) % njc, k = cidx / (nic * njc);
         const long long nfi = B->nfi, nfj = B->nfj, nft = nfi + B->nfj + B->nfk;
         const int fim = i + (nic + 1) * (j + njc * k);
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(kChunk) void chemistry_kernel(const BlockDev* block
     const int cidx = ch.c0 + (int)threadIdx.x;
     unsigned long long sigmin = ~0ull;
     if (cidx < ch.c1) {
-        const int n = (int)B->n;
+        const long long n = B->n;
         real* W = B->W + slot * B->wstride;
         Prim w = ld_prim(W, n, cidx);
         if (factor >= real(0)) {

@@ -3,6 +3,7 @@
 // ghost-cell states of the boundary conditions.
 #pragma once
 
+#include "chkn_geometry.h"
 #include "chkn_internal.h"
 #include "chkn_physics.h"
 
@@ -22,7 +23,9 @@ __device__ __forceinline__ CHKN_GMEM T* gmem(T* p)
 // Element at byte offset `off` of a global array.  With a uniform base and a 32-bit offset
 // the compiler issues one global load with the base in scalar registers and the offset in a
 // single VGPR shared by all arrays indexed alike, instead of a 64-bit address per load.
-// Every per-block array (one slot of W, U, L, one direction's face geometry) is below 4 GiB.
+// Every per-block array component (one of the 6 of a W slot, U or L; one of the 10 of a
+// direction's face geometry; one of the 3 of the vertices) is below 4 GiB; component bases
+// are formed in 64-bit arithmetic.
 __device__ __forceinline__ unsigned boff(int c)
 {
     return (unsigned)c * (unsigned)sizeof(real);
@@ -47,7 +50,7 @@ __device__ __forceinline__ void gst_nt(real* base, unsigned off, real v)
     __builtin_nontemporal_store(v, (CHKN_GMEM real*)((CHKN_GMEM char*)gmem(base) + off));
 }
 
-__device__ __forceinline__ Prim ld_prim(const real* W, int n, int c)
+__device__ __forceinline__ Prim ld_prim(const real* W, long long n, int c)
 {
     const unsigned o = boff(c);
     Prim p;
@@ -60,7 +63,7 @@ __device__ __forceinline__ Prim ld_prim(const real* W, int n, int c)
     return p;
 }
 
-__device__ __forceinline__ Frame ld_frame(const real* g, int nf, int f)
+__device__ __forceinline__ Frame ld_frame(const real* g, long long nf, int f)
 {
     const unsigned o = boff(f);
     Frame r;
@@ -74,6 +77,41 @@ __device__ __forceinline__ Frame ld_frame(const real* g, int nf, int f)
     r.t2y = gld(g + 8 * nf, o);
     r.t2z = gld(g + 9 * nf, o);
     return r;
+}
+
+// Vertex v = i + (nic+1)*(j + (njc+1)*k) of a block.
+__device__ __forceinline__ geom::V3 ld_vertex(const BlockDev* B, int v)
+{
+    const unsigned o = boff(v);
+    const long long nv = B->nv;
+    return geom::V3{gld(B->vtx, o), gld(B->vtx + nv, o), gld(B->vtx + 2 * nv, o)};
+}
+
+// Area and frame of the face whose vertices are v0 + d[0..3] (in the reference's vertex
+// order), formed as the host geometry pass forms them (quad_properties, vector3.cu:196-245),
+// so the bits are those of the stored face geometry.
+__device__ __forceinline__ void vertex_frame(const BlockDev* B, int v0, int d1, int d2, int d3,
+                                             Frame& f, real& area)
+{
+    geom::Quad q;
+    geom::quad_frame(ld_vertex(B, v0), ld_vertex(B, v0 + d1), ld_vertex(B, v0 + d2),
+                     ld_vertex(B, v0 + d3), q);
+    area = q.area;
+    f.nx = q.n.x; f.ny = q.n.y; f.nz = q.n.z;
+    f.t1x = q.t1.x; f.t1y = q.t1.y; f.t1z = q.t1.z;
+    f.t2x = q.t2.x; f.t2y = q.t2.y; f.t2z = q.t2.z;
+}
+
+// Frame of the dir-face (i, j, k): i-face vertex order block.cu:267-270, j-face :368-371,
+// k-face :469-472 (face index conventions as the stored geometry).
+template <int DIR>
+__device__ __forceinline__ void face_frame_v(const BlockDev* B, int i, int j, int k, Frame& f, real& area)
+{
+    const int sj = B->nic + 1, sk = (B->nic + 1) * (B->njc + 1);
+    const int v = i + sj * j + sk * k;
+    if (DIR == 0) vertex_frame(B, v, sj, sj + sk, sk, f, area);
+    else if (DIR == 1) vertex_frame(B, v + 1, -1, sk - 1, sk, f, area);
+    else vertex_frame(B, v, 1, 1 + sj, sj, f, area);
 }
 
 // Linear cell index of line position m in direction dir with transverse indices (a, c):
@@ -105,6 +143,9 @@ __device__ __forceinline__ void boundary_face(const BlockDev* B, int dir, int si
 // an out-of-line call would make every caller preserve its live registers across the
 // call ABI, which costs about 30 VGPRs and one wave per SIMD of occupancy.
 // Depth d = -m-1 on the minus side and m-n on the plus side (block.cu:272-316).
+// VFRAME: wall frames formed from the block's vertices rather than read from the stored
+// face geometry (same bits).
+template <bool VFRAME = false>
 __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const real* inflow, int win,
                                            const BlockDev* B, int dir, int m, int a, int c)
 {
@@ -117,7 +158,7 @@ __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const real* i
         const BlockDev* O = blocks + B->bc_arg[bf];
         int no = dir == 0 ? O->nic : (dir == 1 ? O->njc : O->nkc);
         int mo = side ? depth : no - 1 - depth;
-        return ld_prim(O->W + win * O->wstride, (int)O->n, cell_index(O->nic, O->njc, dir, mo, a, c));
+        return ld_prim(O->W + win * O->wstride, O->n, cell_index(O->nic, O->njc, dir, mo, a, c));
     }
     if (code == 4) {  // inflow: the configured flow state (bcs.cu:257-319)
         const CHKN_GMEM real* s = gmem(inflow) + 6 * B->bc_arg[bf];
@@ -133,10 +174,18 @@ __device__ __forceinline__ Prim ghost_prim(const BlockDev* blocks, const real* i
         p.rho = s[0]; p.e = s[nt]; p.YB = s[2 * nt]; p.vx = s[3 * nt]; p.vy = s[4 * nt]; p.vz = s[5 * nt];
         return p;
     }
-    Prim in = ld_prim(B->W + win * B->wstride, (int)B->n,
+    Prim in = ld_prim(B->W + win * B->wstride, B->n,
                       cell_index(B->nic, B->njc, dir, side ? n - 1 : 0, a, c));
     if (code == 5) return in;  // outflow copies the adjacent interior cell (bcs.cu:435-503)
     // Walls: both ghost layers reflect the adjacent interior cell (bcs.cu:16-132).
+    if (VFRAME) {
+        Frame f;
+        real area;
+        if (dir == 0) face_frame_v<0>(B, side ? B->nic : 0, a, c, f, area);
+        else if (dir == 1) face_frame_v<1>(B, a, side ? B->njc : 0, c, f, area);
+        else face_frame_v<2>(B, a, c, side ? B->nkc : 0, f, area);
+        return reflect(in, f);
+    }
     const real* g; int nf, fi;
     boundary_face(B, dir, side, a, c, g, nf, fi);
     return reflect(in, ld_frame(g, nf, fi));
@@ -166,7 +215,7 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 // d in L, stage 2 replaces L by L/6 + d/6, stage 3 writes the new U.  Decodes the stage
 // result (flow.cu:245-268) into W slot wout and returns the decode's bad-cell flag.
 template <int STAGE>
-__device__ __forceinline__ int stage_update(const BlockDev* B, int n, int cidx, real dt,
+__device__ __forceinline__ int stage_update(const BlockDev* B, long long n, int cidx, real dt,
                                             const real d[6], int wout, Prim& w)
 {
     real* U = B->U;
@@ -206,15 +255,24 @@ __device__ __forceinline__ int stage_update(const BlockDev* B, int n, int cidx, 
 
 // Bits of the cfl-free signal time of cell (i, j, k) = cidx in state w (cell.cu:155-164);
 // non-negative reals order as their bit patterns, so integer minima give the minimum time.
-__device__ __forceinline__ unsigned long long signal_bits(const BlockDev* B, int n, int i, int j,
+template <bool VFRAME = false>
+__device__ __forceinline__ unsigned long long signal_bits(const BlockDev* B, long long n, int i, int j,
                                                          int k, int cidx, const Prim& w)
 {
     const int nic = B->nic, njc = B->njc;
     const real T = eos_T(w.e, w.YB);
     const real sa = eos_a(T);
-    const Frame fi_ = ld_frame(B->gi, (int)B->nfi, i + (nic + 1) * (j + njc * k));
-    const Frame fj_ = ld_frame(B->gj, (int)B->nfj, i + nic * (j + (njc + 1) * k));
-    const Frame fk_ = ld_frame(B->gk, (int)B->nfk, cidx);
+    Frame fi_, fj_, fk_;
+    if (VFRAME) {
+        real area;
+        face_frame_v<0>(B, i, j, k, fi_, area);
+        face_frame_v<1>(B, i, j, k, fj_, area);
+        face_frame_v<2>(B, i, j, k, fk_, area);
+    } else {
+        fi_ = ld_frame(B->gi, B->nfi, i + (nic + 1) * (j + njc * k));
+        fj_ = ld_frame(B->gj, B->nfj, i + nic * (j + (njc + 1) * k));
+        fk_ = ld_frame(B->gk, B->nfk, cidx);
+    }
     const unsigned o = boff(cidx);
     const real sig = local_signal(w, sa, gld(B->len, o), gld(B->len + n, o), gld(B->len + 2 * n, o),
                                   fi_, fj_, fk_);

@@ -106,6 +106,10 @@ typedef struct {
     int persistent;       /* 1 = persistent work-groups loop over the tile table */
     int tile_order;       /* 0 = k-chunk layers outermost (plane-major tile table);
                              1 = the k-chunks of one (i, j) column consecutive */
+    int vertex_frames;    /* 1 = the stage kernel forms face frames from the vertices (3 words
+                             per vertex read, more arithmetic) instead of reading the stored
+                             10 words per face; built for 32x8 and 16x16 tiles, x_order 2,
+                             FP64 */
 } chkn_launch_opts;
 
 /* Replaces initialize_simulation (simulate.cu:55-155), array form. */

@@ -89,10 +89,14 @@ def test_gas_model_mismatch_is_a_config_error(tmp_path):
 def test_tuner_candidates_are_valid_launch_options():
     from chkn import tune
     c = tune.candidates()
-    assert len(c) == len(tune.TILES) * len(tune.TILE_K) * len(tune.ORDERS) * len(tune.PERSISTENT)
-    for o in c:
+    base = len(tune.TILES) * len(tune.TILE_K) * len(tune.ORDERS) * len(tune.PERSISTENT)
+    assert len(c) == base + len(tune.VERTEX_FRAME_TILES)
+    for o in c[:base]:
         assert set(o) == {"tile_i", "tile_j", "tile_k", "tile_order", "persistent"}
         assert o["tile_i"] * o["tile_j"] <= 1024
+    for o in c[base:]:
+        assert o["vertex_frames"] and (o["tile_i"], o["tile_j"]) in tune.VERTEX_FRAME_TILES
+    assert len(tune.candidates(x_order=1)) == base  # the vertex-frame form is second-order only
 
 
 def test_blocks_beyond_32_bit_offsets_are_rejected():

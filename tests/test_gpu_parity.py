@@ -79,6 +79,8 @@ def test_fields_match_oracle(name):
     dict(tile_i=64, tile_j=4, tile_k=3),
     dict(tile_i=16, tile_j=16, tile_k=1),
     dict(tile_i=64, tile_j=8, use_graph=True, per_block_launch=True),
+    dict(tile_i=32, tile_j=8, vertex_frames=True),
+    dict(tile_i=16, tile_j=16, tile_k=2, use_graph=True, vertex_frames=True),
 ])
 def test_launch_options_do_not_change_results(launch):
     compare(cases.ramp_array(seed=5, lo=4, hi=12, max_step=25), 25, launch=launch)
@@ -171,7 +173,8 @@ def test_scale_ramp_at_full_size_matches_oracle():
     rc, _, dts = orc.march(11)
     assert rc == 0 and orc.step == 11
     assert dts[10] != dts[9]  # the step after the reduction runs on the new dt
-    for launch in (dict(tile_i=32, tile_j=8, tile_k=4, use_graph=True), dict(tile_i=32, tile_j=8, use_graph=True)):
+    for launch in (dict(tile_i=32, tile_j=8, tile_k=4, use_graph=True), dict(tile_i=32, tile_j=8, use_graph=True),
+                   dict(tile_i=32, tile_j=8, use_graph=True, vertex_frames=True)):
         sim = Sim(job)
         sim.set_launch(**launch)
         rep = sim.step(11)
@@ -180,6 +183,30 @@ def test_scale_ramp_at_full_size_matches_oracle():
         assert np.array_equal(sim.iovars(0), orc.iovars(0)), launch
         assert np.array_equal(sim.conserved(0), orc.conserved(0)), launch
         sim.close()
+
+
+def test_plain_config_at_full_size_matches_oracle():
+    # The plain workload at its own size: one 64x32x32 block, 3D Sod tube in a slip-wall box,
+    # 200 steps.  dt grows 1.5x every 10 steps from 1e-6 until the CFL limit binds (near step
+    # 80); fields, conserved state and the whole dt sequence bit for bit against the oracle.
+    job = cases.sod_box(64, 32, 32, max_step=200)
+    orc = ob.Oracle(job)
+    rc, _, dts = orc.march(200)
+    assert rc == 0 and orc.step == 200
+    ramp = [1.0e-6 * 1.5 ** (s // 10) for s in range(200)]
+    limited = [s for s in range(200) if dts[s] < ramp[s] * (1 - 1e-12)]
+    assert limited and 60 <= limited[0] <= 100, limited[:3]  # the CFL limit is reached
+    sim = Sim(job)
+    sim.set_launch(use_graph=True)
+    dt_seen = []
+    for _ in range(20):
+        rep = sim.step(10)
+        dt_seen.append(rep.dt)
+    assert rep.step == 200 and rep.t == orc.t
+    assert dt_seen == [dts[10 * q + 9] for q in range(20)]
+    assert np.array_equal(sim.iovars(0), orc.iovars(0))
+    assert np.array_equal(sim.conserved(0), orc.conserved(0))
+    sim.close()
 
 
 def test_batched_array_at_full_size_matches_oracle():
