@@ -1,25 +1,35 @@
-"""Job preparation for the chicken step: the user-facing API of chkn_prep.py.
+"""Job preparation for the chicken step (the chkn-prep front-end).
 
-A job script (Python) is executed with the names of this module in scope: ``config``,
+A job script is ordinary Python run with this module's names in scope (``config``,
 ``FlowState``, ``FluidBlock``, ``makeFBArray``, the boundary-condition classes,
-``add_cfl_value``, ``add_dt_plot`` and the geometry helpers ``Vector3``, ``TFIVolume``,
-``StructuredGrid``.  ``write_initial_files`` then writes config.json, the grid and flow
-files and times.data in the formats the solver reads (chkn_prep.py:856-932).
+``add_cfl_value``, ``add_dt_plot``, ``Face`` and the geometry helpers ``Vector3``,
+``TFIVolume``, ``StructuredGrid``); ``write_initial_files`` then writes the job directory
+the solver reads: config.json, grid/, flow/t0000/, times.data and any TWall-*.gz files.
 
-Names, defaults and the JSON text follow chkn_prep.py (GasModel :67-88, GlobalConfig
-:93-246, GasState/FlowState :311-374, boundary conditions :383-599, FluidBlock :602-780,
-makeFBArray :784-851).  Two deliberate differences: cell centres and volumes written to
-the flow file are in the flow file's own cell order (i fastest) -- the solver recomputes
-both from the grid (block.cu:640-647) -- and an initial state may also be a dict of
-per-cell arrays, so large blocks need no per-cell Python call.
+What is fixed by the solver's file contract and the job-script API of the reference
+(src/chicken/chkn_prep.py) is kept: the user-facing names, keyword arguments and defaults,
+the config.json text byte for byte (key order and number formats read by config.cu:347-525),
+the gas-state formulas whose results land in the flow files, and one quirk of the reference's
+boundary-condition lookup (an integer key ``Face.iplus`` also sets ``jplus``,
+chkn_prep.py:671-677).  Everything else is this module's own design:
 
-Use ``reset()`` to start a new job in the same process.
+* config.json is produced from a table of keys and formatters (``_GLOBAL_KEYS``);
+* the block array is checked by collecting slab sizes per axis (``_BlockArray``);
+* boundary conditions declare their JSON fields in a class table (``_fields``);
+* initial states (uniform, per-cell arrays, or a function of position) are all turned into
+  per-cell arrays by one helper, and ``makeFBArray`` splits a grid through per-axis ranges.
+
+Cell centres and volumes are written to the flow file in its own cell order (i fastest); the
+solver recomputes both from the grid (block.cu:640-647).  ``reset()`` starts a new job.
 """
 import collections
 import gzip
+import io
+import itertools
 import json
 import math
 import os
+import sys
 from copy import copy
 
 import numpy as np
@@ -28,175 +38,56 @@ from .geom import StructuredGrid, TFIVolume, Vector3, cell_centres, hex_volumes
 
 IOVAR_NAMES = ["posx", "posy", "posz", "vol", "p", "T", "rho", "e", "YB", "a",
                "velx", "vely", "velz"]
+# The nine flow variables of a cell after its centre and volume (cell.cu:66-84).
+FLOW_KEYS = ("p", "T", "rho", "e", "YB", "a", "velx", "vely", "velz")
 
 FaceNames = "iminus iplus jminus jplus kminus kplus"
-Face = collections.namedtuple("_", FaceNames)(*range(6))
 FaceList = FaceNames.split()
+Face = collections.namedtuple("Face", FaceList)(*range(6))
 
 
+# --------------------------------------------------------------------------- gas and flow
 class GasModel:
-    """Ideal gas; must match the solver's compiled model (gas.cu:86-93)."""
+    """Gas constants written to config.json; the solver checks them against the model it
+    was built with (config.cu:369-386).  Defaults: ideal air."""
 
     __slots__ = "gamma", "R", "Cv", "q", "alpha", "Ti"
 
     def __init__(self, gamma=1.4, R=287.1, q=0.0, alpha=0.0, Ti=0.0):
-        self.gamma = gamma
-        self.R = R
+        self.gamma, self.R, self.q, self.alpha, self.Ti = gamma, R, q, alpha, Ti
         self.Cv = R / (gamma - 1.0)
-        self.q = q
-        self.alpha = alpha
-        self.Ti = Ti
 
     def to_json(self):
-        return '{"gamma": %g, "R": %g, "Cv": %g, "q": %g, "alpha": %g, "Ti": %g}' % (
-            self.gamma, self.R, self.Cv, self.q, self.alpha, self.Ti)
-
-
-class GlobalConfig:
-    """Simulation control parameters; the script edits the module-level ``config``."""
-
-    count = 0
-    __slots__ = ("job_name", "title", "gas_model", "nib", "njb", "nkb", "blk_ids", "nics", "njcs",
-                 "nkcs", "dt_init", "cfl_list", "cfl_count", "print_count", "dt_plot_list",
-                 "max_time", "max_step", "x_order", "t_order", "flux_calc", "viscous", "reacting",
-                 "source_terms", "iovar_names", "threads_per_gpu_block")
-
-    def __init__(self):
-        if GlobalConfig.count >= 1:
-            raise Exception("Already have a GlobalConfig object defined.")
-        self.job_name = ""
-        self.title = "Chicken Simulation."
-        self.gas_model = GasModel()
-        self.nib = self.njb = self.nkb = None
-        self.blk_ids = None
-        self.nics = self.njcs = self.nkcs = None
-        self.dt_init = 1.0e-6
-        self.cfl_list = []
-        self.cfl_count = 10
-        self.print_count = 20
-        self.dt_plot_list = []
-        self.max_time = 1.0e-3
-        self.max_step = 100
-        self.x_order = 2
-        self.t_order = 2
-        self.flux_calc = "ausmdv"
-        self.reacting = False
-        self.viscous = False
-        self.source_terms = "none"
-        self.threads_per_gpu_block = 128
-        self.iovar_names = list(IOVAR_NAMES)
-        GlobalConfig.count += 1
-
-    def write(self, fp):
-        """Global part of config.json (chkn_prep.py:151-196)."""
-        fp.write('  "title": "%s",\n' % self.title)
-        fp.write('  "gas_model": %s,\n' % self.gas_model.to_json())
-        fp.write('  "max_time": %e,\n' % self.max_time)
-        fp.write('  "max_step": %d,\n' % self.max_step)
-        fp.write('  "dt_init": %e,\n' % self.dt_init)
-        if len(self.cfl_list) == 0:
-            add_cfl_value(0.0, 0.5)
-        n_cfl = len(self.cfl_list)
-        fp.write('  "n_cfl": %d,\n' % n_cfl)
-        fp.write('  "cfl_times": %s,\n' % json.dumps([c[0] for c in self.cfl_list]))
-        vals = [c[1] for c in self.cfl_list]
-        assert min(vals) > 0.0, "Require positive nonzero cfl values."
-        fp.write('  "cfl_values": %s,\n' % json.dumps(vals))
-        fp.write('  "cfl_count": %d,\n' % self.cfl_count)
-        fp.write('  "print_count": %d,\n' % self.print_count)
-        fp.write('  "x_order": %d,\n' % self.x_order)
-        fp.write('  "t_order": %d,\n' % self.t_order)
-        fp.write('  "flux_calc": "%s",\n' % self.flux_calc)
-        fp.write('  "reacting": %s,\n' % json.dumps(self.reacting))
-        fp.write('  "viscous": %s,\n' % json.dumps(self.viscous))
-        fp.write('  "source_terms": "%s",\n' % self.source_terms)
-        fp.write('  "threads_per_gpu_block": %d,\n' % self.threads_per_gpu_block)
-        if len(self.dt_plot_list) == 0:
-            add_dt_plot(0.0, self.max_time, self.max_time)
-        n = len(self.dt_plot_list)
-        fp.write('  "n_dt_plot": %d,\n' % n)
-        fp.write('  "t_change": %s,\n' % json.dumps([d[0] for d in self.dt_plot_list]))
-        fp.write('  "dt_plot": %s,\n' % json.dumps([d[1] for d in self.dt_plot_list]))
-        fp.write('  "dt_hist": %s,\n' % json.dumps([d[2] for d in self.dt_plot_list]))
-        fp.write('  "iovar_names": %s,\n' % json.dumps(self.iovar_names))
-
-    def check_array_of_fluid_blocks(self):
-        """Place blocks in the block array; slab sizes must agree (chkn_prep.py:198-246)."""
-        imax = jmax = kmax = 0
-        for b in fluidBlocksList:
-            imax, jmax, kmax = max(b.i, imax), max(b.j, jmax), max(b.k, kmax)
-        self.nib, self.njb, self.nkb = imax + 1, jmax + 1, kmax + 1
-        self.blk_ids = np.full((self.nib, self.njb, self.nkb), -1, dtype=int)
-        self.nics = np.zeros(self.nib, dtype=int)
-        self.njcs = np.zeros(self.njb, dtype=int)
-        self.nkcs = np.zeros(self.nkb, dtype=int)
-        for b in fluidBlocksList:
-            i, j, k = b.i, b.j, b.k
-            if self.blk_ids[i, j, k] >= 0:
-                raise RuntimeError("Already a block at i=%d, j=%d, k=%d" % (i, j, k))
-            self.blk_ids[i, j, k] = b.indx
-            nic, njc, nkc = b.grid.niv - 1, b.grid.njv - 1, b.grid.nkv - 1
-            if self.nics[i] == 0: self.nics[i] = nic
-            if self.njcs[j] == 0: self.njcs[j] = njc
-            if self.nkcs[k] == 0: self.nkcs[k] = nkc
-            if nic != self.nics[i]:
-                raise RuntimeError("FluidBlock at i=%d j=%d k=%d has inconsistent nic=%d" % (i, j, k, nic))
-            if njc != self.njcs[j]:
-                raise RuntimeError("FluidBlock at i=%d j=%d k=%d has inconsistent njc=%d" % (i, j, k, njc))
-            if nkc != self.nkcs[k]:
-                raise RuntimeError("FluidBlock at i=%d j=%d k=%d has inconsistent nkc=%d" % (i, j, k, nkc))
-
-
-def add_cfl_value(t_change, cfl_value):
-    """Append a (t, cfl) pair; times must increase (chkn_prep.py:254-273)."""
-    if len(config.cfl_list) > 0:
-        if t_change > config.cfl_list[-1][0]:
-            config.cfl_list.append((t_change, cfl_value))
-        else:
-            print("Warning: did not add t_change,cfl_value tuple (", t_change, cfl_value, ").")
-    else:
-        config.cfl_list.append((t_change, cfl_value))
-    return len(config.cfl_list)
-
-
-def add_dt_plot(t_change, dt_plot, dt_his=1.0e-6):
-    """Append a (t, dt_plot, dt_his) triple; times must increase (chkn_prep.py:276-300)."""
-    if len(config.dt_plot_list) > 0:
-        if t_change > config.dt_plot_list[-1][0]:
-            config.dt_plot_list.append((t_change, dt_plot, dt_his))
-        else:
-            print("Warning: did not add dt_plot tuple (", t_change, dt_plot, dt_his, ").")
-    else:
-        config.dt_plot_list.append((t_change, dt_plot, dt_his))
-    return len(config.dt_plot_list)
+        return "{%s}" % ", ".join('"%s": %g' % (k, getattr(self, k))
+                                  for k in ("gamma", "R", "Cv", "q", "alpha", "Ti"))
 
 
 class GasState:
-    """p, T, YB with rho, e, a from the gas model (chkn_prep.py:311-342)."""
+    """(p, T, YB) with rho, e and a from the current gas model."""
 
     __slots__ = "indx", "p", "T", "rho", "e", "YB", "a"
 
     def __init__(self, p, T, YB=0.0):
-        self.p = p
-        self.T = T
-        self.YB = YB
+        self.p, self.T, self.YB = p, T, YB
         self.update_from_pT(config.gas_model)
 
     def update_from_pT(self, gmodel):
+        # Ideal-gas relations; the same operations as the solver's update_from_pT
+        # (gas.cu:119-126), so inflow states agree with what it derives from p and T.
         self.e = gmodel.Cv * self.T + self.YB * gmodel.q
         self.rho = self.p / (gmodel.R * self.T)
         self.a = np.sqrt(gmodel.gamma * gmodel.R * self.T)
 
     def __repr__(self):
-        return "GasState(p={}, T={}, YB={})".format(self.p, self.T, self.YB)
+        return "GasState(p=%r, T=%r, YB=%r)" % (self.p, self.T, self.YB)
 
     def to_json(self):
-        return '{"p": %g, "T": %g, "rho": %g, "e": %g, "YB": %g, "a": %g}' % (
-            self.p, self.T, self.rho, self.e, self.YB, self.a)
+        return "{%s}" % ", ".join('"%s": %g' % (k, getattr(self, k))
+                                  for k in ("p", "T", "rho", "e", "YB", "a"))
 
 
 class FlowState:
-    """Gas state plus velocity (chkn_prep.py:345-374)."""
+    """A gas state and a velocity; either given whole or built from p, T, YB and components."""
 
     __slots__ = "indx", "label", "gas", "vel"
 
@@ -207,361 +98,452 @@ class FlowState:
         self.label = label
 
     def __repr__(self):
-        return "FlowState(gas={}, vel={})".format(self.gas, self.vel)
+        return "FlowState(gas=%r, vel=%r)" % (self.gas, self.vel)
 
     def to_json(self):
-        return '{"gas": %s, "vel": [%g, %g, %g]}' % (self.gas.to_json(), self.vel.x, self.vel.y,
-                                                    self.vel.z)
+        v = self.vel
+        return '{"gas": %s, "vel": [%g, %g, %g]}' % (self.gas.to_json(), v.x, v.y, v.z)
+
+    def cell_values(self):
+        """The nine flow variables of a cell in this state, in FLOW_KEYS order."""
+        g, v = self.gas, self.vel
+        return (g.p, g.T, g.rho, g.e, g.YB, g.a, v.x, v.y, v.z)
 
 
+# --------------------------------------------------------------------------- schedules
+def _append_schedule(entries, entry, what):
+    """Schedules are piecewise constant in time: an entry is kept only if its start time
+    follows the last one's."""
+    if entries and not entry[0] > entries[-1][0]:
+        print("Warning: %s %r ignored: its start time does not follow %g." % (what, entry, entries[-1][0]))
+    else:
+        entries.append(entry)
+    return len(entries)
+
+
+def add_cfl_value(t_change, cfl_value):
+    """From time t_change on, use this CFL number."""
+    return _append_schedule(config.cfl_list, (t_change, cfl_value), "cfl entry")
+
+
+def add_dt_plot(t_change, dt_plot, dt_his=1.0e-6):
+    """From time t_change on, write snapshots every dt_plot (dt_his: history interval)."""
+    return _append_schedule(config.dt_plot_list, (t_change, dt_plot, dt_his), "dt_plot entry")
+
+
+# --------------------------------------------------------------------------- global config
+def _js(value):
+    return json.dumps(value)
+
+
+def _quoted(value):
+    return '"%s"' % value
+
+
+def _cfl_times(c):
+    return _js([e[0] for e in c.cfl_list])
+
+
+def _cfl_values(c):
+    vals = [e[1] for e in c.cfl_list]
+    if min(vals) <= 0.0:
+        raise ValueError("CFL values must be positive, got %r" % (vals,))
+    return _js(vals)
+
+
+# config.json keys of the global part, in the order the file holds them, with a function
+# giving each value's text from the config.
+_GLOBAL_KEYS = (
+    ("title", lambda c: _quoted(c.title)),
+    ("gas_model", lambda c: c.gas_model.to_json()),
+    ("max_time", lambda c: "%e" % c.max_time),
+    ("max_step", lambda c: "%d" % c.max_step),
+    ("dt_init", lambda c: "%e" % c.dt_init),
+    ("n_cfl", lambda c: "%d" % len(c.cfl_list)),
+    ("cfl_times", _cfl_times),
+    ("cfl_values", _cfl_values),
+    ("cfl_count", lambda c: "%d" % c.cfl_count),
+    ("print_count", lambda c: "%d" % c.print_count),
+    ("x_order", lambda c: "%d" % c.x_order),
+    ("t_order", lambda c: "%d" % c.t_order),
+    ("flux_calc", lambda c: _quoted(c.flux_calc)),
+    ("reacting", lambda c: _js(c.reacting)),
+    ("viscous", lambda c: _js(c.viscous)),
+    ("source_terms", lambda c: _quoted(c.source_terms)),
+    ("threads_per_gpu_block", lambda c: "%d" % c.threads_per_gpu_block),
+    ("n_dt_plot", lambda c: "%d" % len(c.dt_plot_list)),
+    ("t_change", lambda c: _js([e[0] for e in c.dt_plot_list])),
+    ("dt_plot", lambda c: _js([e[1] for e in c.dt_plot_list])),
+    ("dt_hist", lambda c: _js([e[2] for e in c.dt_plot_list])),
+    ("iovar_names", lambda c: _js(c.iovar_names)),
+)
+
+# Attribute defaults of a new config (callables give a fresh mutable value each time).
+_CONFIG_DEFAULTS = (
+    ("job_name", ""), ("title", "Chicken Simulation."), ("gas_model", GasModel),
+    ("nib", None), ("njb", None), ("nkb", None), ("blk_ids", None),
+    ("nics", None), ("njcs", None), ("nkcs", None),
+    ("dt_init", 1.0e-6), ("cfl_list", list), ("cfl_count", 10), ("print_count", 20),
+    ("dt_plot_list", list), ("max_time", 1.0e-3), ("max_step", 100),
+    ("x_order", 2), ("t_order", 2), ("flux_calc", "ausmdv"), ("reacting", False),
+    ("viscous", False), ("source_terms", "none"), ("threads_per_gpu_block", 128),
+    ("iovar_names", lambda: list(IOVAR_NAMES)),
+)
+
+
+class GlobalConfig:
+    """Simulation control parameters.  There is one per job (the module-level ``config``);
+    a job script sets its attributes."""
+
+    count = 0
+    __slots__ = tuple(name for name, _ in _CONFIG_DEFAULTS)
+
+    def __init__(self):
+        if GlobalConfig.count:
+            raise RuntimeError("a job has one GlobalConfig: use the module's `config` (reset() "
+                               "starts a new job)")
+        for name, default in _CONFIG_DEFAULTS:
+            setattr(self, name, default() if callable(default) else default)
+        GlobalConfig.count += 1
+
+    def write(self, fp):
+        """The global part of config.json.  Empty schedules get the defaults: CFL 0.5 from
+        t = 0, and one snapshot interval of max_time."""
+        if not self.cfl_list:
+            add_cfl_value(0.0, 0.5)
+        if not self.dt_plot_list:
+            add_dt_plot(0.0, self.max_time, self.max_time)
+        for key, text in _GLOBAL_KEYS:
+            fp.write('  "%s": %s,\n' % (key, text(self)))
+
+    def check_array_of_fluid_blocks(self):
+        """Place the blocks in their (i, j, k) array and record the slab sizes."""
+        arr = _BlockArray(fluidBlocksList)
+        self.nib, self.njb, self.nkb = arr.shape
+        self.blk_ids = arr.ids
+        self.nics, self.njcs, self.nkcs = arr.sizes
+
+
+class _BlockArray:
+    """Blocks by array position.  Every block of array column i has the same nic, of row j
+    the same njc and of layer k the same nkc (the solver's exchange copies whole faces)."""
+
+    def __init__(self, blocks):
+        if not blocks:
+            raise RuntimeError("no FluidBlock has been defined")
+        self.shape = tuple(1 + max(getattr(b, a) for b in blocks) for a in "ijk")
+        self.ids = np.full(self.shape, -1, dtype=int)
+        self.sizes = [np.zeros(n, dtype=int) for n in self.shape]
+        for b in blocks:
+            pos = (b.i, b.j, b.k)
+            if self.ids[pos] >= 0:
+                raise RuntimeError("blocks %d and %d both sit at array position %s"
+                                   % (self.ids[pos], b.indx, pos))
+            self.ids[pos] = b.indx
+            for axis, (slab, cells) in enumerate(zip(pos, (b.nic, b.njc, b.nkc))):
+                have = self.sizes[axis][slab]
+                if have and have != cells:
+                    raise RuntimeError("block %d at %s has %d cells along %s, its slab has %d"
+                                       % (b.indx, pos, cells, "ijk"[axis], have))
+                self.sizes[axis][slab] = cells
+
+
+# --------------------------------------------------------------------------- boundaries
 class BoundaryCondition:
-    __slots__ = ["tag"]
+    """A block side's boundary condition.  ``tag`` is the name config.json uses
+    (face.cu:41-59); ``_fields`` lists (attribute, json key, format) of its extra entries."""
+
+    __slots__ = ()
+    tag = ""
+    _fields = ()
 
     def to_json(self):
-        return '{"tag": "%s"}' % self.tag
+        parts = ['"tag": "%s"' % self.tag]
+        parts += ['"%s": %s' % (key, fmt % getattr(self, attr)) for attr, key, fmt in self._fields]
+        return "{%s}" % ", ".join(parts)
+
+    def __repr__(self):
+        args = ", ".join("%s=%r" % (attr, getattr(self, attr, None)) for attr, _, _ in self._fields)
+        return "%s(%s)" % (type(self).__name__, args)
 
 
 class ExchangeBC(BoundaryCondition):
-    def __init__(self):
-        self.tag = "exchange"
-
-    def __repr__(self):
-        return "ExchangeBC()"
+    """Copy from the neighbouring block in the array (periodic at the array's edges)."""
+    tag = "exchange"
 
 
 class WallWithSlipBC(BoundaryCondition):
-    def __init__(self):
-        self.tag = "wall_with_slip"
-
-    def __repr__(self):
-        return "WallWithSlipBC()"
+    """Inviscid wall: the normal velocity is reflected."""
+    tag = "wall_with_slip"
 
 
 class WallNoSlipAdiabaticBC(BoundaryCondition):
-    def __init__(self):
-        self.tag = "wall_no_slip_adiabatic"
-
-    def __repr__(self):
-        return "WallNoSlipAdiabaticBC()"
+    """Viscous wall without heat flux."""
+    tag = "wall_no_slip_adiabatic"
 
 
 class WallNoSlipFixedTBC(BoundaryCondition):
-    """Fixed-temperature no-slip wall; TWall a number or a function of (x, y, z)."""
+    """Viscous wall at temperature TWall: a number, or a function of (x, y, z) evaluated at
+    each face centre and written to TWall-*.gz files."""
 
-    __slots__ = ["tag", "form", "value", "fun", "values"]
+    __slots__ = ("form", "value", "fun")
+    tag = "wall_no_slip_fixed_T"
+    _fields = (("form", "form", '"%s"'), ("value", "value", "%g"))
 
     def __init__(self, TWall):
-        self.tag = "wall_no_slip_fixed_T"
-        self.fun = None
-        self.value = 0.0
         if callable(TWall):
-            self.form = "fun"
-            self.fun = TWall
+            self.form, self.value, self.fun = "fun", 0.0, TWall
         elif isinstance(TWall, (int, float)):
-            self.form = "value"
-            self.value = float(TWall)
+            self.form, self.value, self.fun = "value", float(TWall), None
         else:
-            raise RuntimeError("TWall should be a single number or a function of (x,y,z)")
-
-    def __repr__(self):
-        return "WallNoSlipFixedTBC(form=%s)" % self.form
-
-    def to_json(self):
-        return '{"tag": "%s", "form": "%s", "value": %g}' % (self.tag, self.form, self.value)
+            raise TypeError("TWall must be a number or a function of (x, y, z), got %r" % (TWall,))
 
     def generate_values(self, grid, face):
-        """TWall at the midpoint of every face of the block side, in the order the solver
-        reads them (chkn_prep.py:479-545, block.cu:566-611): i-sides k-major, j-sides
-        k-major, k-sides j-major.  Midpoints are 0.25*(p0+p1+p2+p3) of the four face
-        vertices, with the vertex order of the reference's j- and k-sides; its i-side slices
-        are malformed (they cannot broadcast), so the i-sides take the same pattern."""
-        v = grid.vertices
-        nic, njc, nkc = grid.niv - 1, grid.njv - 1, grid.nkv - 1
-        def mid(p0, p1, p2, p3):
-            return tuple(0.25 * (p0[q] + p1[q] + p2[q] + p3[q]) for q in range(3))
-        xyz = (v.x, v.y, v.z)
-        values = []
-        if face in ("iminus", "iplus"):
-            i = 0 if face == "iminus" else nic
-            m = mid([a[i, :-1, :-1] for a in xyz], [a[i, 1:, :-1] for a in xyz],
-                    [a[i, 1:, 1:] for a in xyz], [a[i, :-1, 1:] for a in xyz])
-            for k in range(nkc):
-                for j in range(njc):
-                    values.append(self.fun(m[0][j, k], m[1][j, k], m[2][j, k]))
-        elif face in ("jminus", "jplus"):
-            j = 0 if face == "jminus" else njc
-            m = mid([a[1:, j, :-1] for a in xyz], [a[:-1, j, :-1] for a in xyz],
-                    [a[:-1, j, 1:] for a in xyz], [a[1:, j, 1:] for a in xyz])
-            for k in range(nkc):
-                for i in range(nic):
-                    values.append(self.fun(m[0][i, k], m[1][i, k], m[2][i, k]))
-        elif face in ("kminus", "kplus"):
-            k = 0 if face == "kminus" else nkc
-            m = mid([a[:-1, :-1, k] for a in xyz], [a[1:, :-1, k] for a in xyz],
-                    [a[:-1, 1:, k] for a in xyz], [a[1:, 1:, k] for a in xyz])
-            for j in range(njc):
-                for i in range(nic):
-                    values.append(self.fun(m[0][i, j], m[1][i, j], m[2][i, j]))
-        else:
-            raise RuntimeError("Unknown face: " + str(face))
-        return [float(val) for val in values]
+        """TWall at every face of a block side, in the order the solver reads them
+        (block.cu:566-611): i-sides over (j, k), j-sides over (i, k), k-sides over (i, j),
+        the first index fastest.  A face's point is the mean of its four vertices."""
+        xyz = [np.asarray(a) for a in (grid.vertices.x, grid.vertices.y, grid.vertices.z)]
+        p0, p1, p2, p3 = _side_quads(xyz, face)
+        pts = [0.25 * (p0[q] + p1[q] + p2[q] + p3[q]) for q in range(3)]
+        return [float(self.fun(x, y, z)) for x, y, z in zip(*(p.T.ravel() for p in pts))]
+
+
+def _side_quads(xyz, face):
+    """Vertex arrays (p0, p1, p2, p3), each three coordinate arrays over the side's faces
+    indexed by the two in-plane indices; the order of the four vertices is that of the
+    midpoint sums of the TWall files."""
+    side = FaceList.index(face) if isinstance(face, str) else face
+    d, plus = divmod(side, 2)
+    n = xyz[0].shape[d] - 1 if plus else 0
+    cut = [slice(None)] * 3
+    cut[d] = n
+    plane = [a[tuple(cut)] for a in xyz]  # the side's vertex plane, two remaining axes
+    lo, hi = slice(None, -1), slice(1, None)
+    if d == 0:      # (j, k): (j,k) (j+1,k) (j+1,k+1) (j,k+1)
+        corners = ((lo, lo), (hi, lo), (hi, hi), (lo, hi))
+    elif d == 1:    # (i, k): (i+1,k) (i,k) (i,k+1) (i+1,k+1)
+        corners = ((hi, lo), (lo, lo), (lo, hi), (hi, hi))
+    else:           # (i, j): (i,j) (i+1,j) (i,j+1) (i+1,j+1)
+        corners = ((lo, lo), (hi, lo), (lo, hi), (hi, hi))
+    return [[c[a, b] for c in plane] for a, b in corners]
 
 
 def write_twall_file(fileName, values):
-    """TWall-*.gz: one value per line, '%g' as chkn_prep.py:469-472 writes them."""
+    """A TWall-*.gz file: one value per line, 6 significant digits."""
     with gzip.open(fileName, "wt") as f:
-        for val in values:
-            f.write("%g\n" % val)
+        f.writelines("%g\n" % v for v in values)
 
 
 class InflowBC(BoundaryCondition):
-    __slots__ = ["tag", "fs", "fsi"]
+    """Ghost cells hold the flow state fs (listed in config.json's flow_states)."""
+
+    __slots__ = ("fs", "fsi")
+    tag = "inflow"
+    _fields = (("fsi", "flow_state_index", "%d"),)
 
     def __init__(self, fs):
-        self.tag = "inflow"
-        if type(fs) is FlowState:
-            self.fs = fs
-        else:
-            raise RuntimeError("Inflow boundary condition expects a FlowState object.")
-
-    def __repr__(self):
-        return "InflowBC(fs={})".format(self.fs)
-
-    def to_json(self):
-        return '{"tag": "%s", "flow_state_index": %d}' % (self.tag, self.fsi)
+        if type(fs) is not FlowState:
+            raise TypeError("InflowBC needs a FlowState, got %r" % (fs,))
+        self.fs = fs
 
 
 class InflowFunctionBC(BoundaryCondition):
-    __slots__ = ["tag", "fun_name"]
+    """Ghost cells from a named function of position (face.cu:51-59)."""
+
+    __slots__ = ("fun_name",)
+    tag = "inflow_function"
+    _fields = (("fun_name", "fun_name", '"%s"'),)
 
     def __init__(self, fun_name):
-        self.tag = "inflow_function"
         self.fun_name = fun_name
-
-    def __repr__(self):
-        return "InflowFunctionBC(fun_name={})".format(self.fun_name)
-
-    def to_json(self):
-        return '{"tag": "%s", "fun_name": "%s"}' % (self.tag, self.fun_name)
 
 
 class OutflowBC(BoundaryCondition):
-    def __init__(self):
-        self.tag = "outflow"
-
-    def __repr__(self):
-        return "OutflowBC()"
+    """Ghost cells copy the adjacent interior cell."""
+    tag = "outflow"
 
 
 def _resolve_bcs(bcs):
-    """Default walls, then the user's entries.  An integer key Face.iplus also sets jplus,
-    as chkn_prep.py:671-677 and :815-821 do."""
-    out = {f: WallWithSlipBC() for f in FaceList}
-    for key in bcs.keys():
-        if key in [Face.iminus, "iminus"]: out["iminus"] = bcs[key]
-        if key in [Face.iplus, "iplus"]: out["iplus"] = bcs[key]
-        if key in [Face.jminus, "jminus"]: out["jminus"] = bcs[key]
-        if key in [Face.iplus, "jplus"]: out["jplus"] = bcs[key]
-        if key in [Face.kminus, "kminus"]: out["kminus"] = bcs[key]
-        if key in [Face.kplus, "kplus"]: out["kplus"] = bcs[key]
+    """The six sides' conditions: slip walls unless given, by name or by Face index.  The
+    integer key Face.iplus (1) also sets jplus, as the reference's lookup does, so job
+    scripts give the same files under both front-ends."""
+    out = {name: WallWithSlipBC() for name in FaceList}
+    aliases = {name: {name, Face[q]} for q, name in enumerate(FaceList)}
+    aliases["jplus"] = {"jplus", Face.iplus}
+    for key, bc in bcs.items():
+        for name in FaceList:
+            if key in aliases[name]:
+                out[name] = bc
     return out
 
 
+# --------------------------------------------------------------------------- blocks
+def _initial_flow(initialState, centres, n):
+    """Per-cell arrays of the nine flow variables (cell order, i fastest) from a FlowState,
+    a dict of arrays (per cell, either flat in cell order or [i, j, k]) or a function of the
+    cell centre that returns a FlowState."""
+    if isinstance(initialState, FlowState):
+        return {key: np.full(n, v) for key, v in zip(FLOW_KEYS, initialState.cell_values())}
+    if isinstance(initialState, dict):
+        flow = {}
+        for key in FLOW_KEYS:
+            a = np.asarray(initialState[key], dtype=float)
+            a = a.transpose().ravel() if a.ndim == 3 else a
+            flow[key] = np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=float)
+        return flow
+    if callable(initialState):
+        pts = zip(*(c.transpose().ravel() for c in (centres.x, centres.y, centres.z)))
+        rows = np.array([initialState(x, y, z).cell_values() for x, y, z in pts], dtype=float)
+        return {key: np.ascontiguousarray(rows[:, q]) for q, key in enumerate(FLOW_KEYS)}
+    raise TypeError("initialState must be a FlowState, a dict of per-cell arrays or a function "
+                    "of (x, y, z) giving a FlowState")
+
+
 class FluidBlock:
-    """One structured block: grid, initial flow and boundary conditions."""
+    """One structured block of the job: grid, initial flow and the six boundary conditions,
+    at position (i, j, k) of the block array."""
 
     __slots__ = ("indx", "active", "i", "j", "k", "grid", "initialState", "nic", "njc", "nkc",
                  "bcs", "label", "cellc", "cellv", "flow", "twall_values")
 
     def __init__(self, i=0, j=0, k=0, grid=None, initialState=None, bcs={}, active=True, label=""):
         if not isinstance(grid, StructuredGrid):
-            raise RuntimeError("Need to supply a StructuredGrid object.")
+            raise TypeError("FluidBlock needs a StructuredGrid, got %r" % (grid,))
         self.grid = grid
+        self.nic, self.njc, self.nkc = grid.niv - 1, grid.njv - 1, grid.nkv - 1
         self.cellc = cell_centres(grid)
         self.cellv = hex_volumes(grid)
-        self.nic, self.njc, self.nkc = grid.niv - 1, grid.njv - 1, grid.nkv - 1
-        n = self.nic * self.njc * self.nkc
-        if isinstance(initialState, FlowState):
-            s = initialState
-            self.flow = {"p": np.full(n, s.gas.p), "T": np.full(n, s.gas.T),
-                         "rho": np.full(n, s.gas.rho), "e": np.full(n, s.gas.e),
-                         "YB": np.full(n, s.gas.YB), "a": np.full(n, s.gas.a),
-                         "velx": np.full(n, s.vel.x), "vely": np.full(n, s.vel.y),
-                         "velz": np.full(n, s.vel.z)}
-        elif isinstance(initialState, dict):
-            self.flow = {}
-            for key in ("p", "T", "rho", "e", "YB", "a", "velx", "vely", "velz"):
-                a = np.asarray(initialState[key], dtype=float)
-                if a.ndim == 3:
-                    a = a.transpose().ravel()
-                self.flow[key] = np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=float)
-        elif callable(initialState):
-            self.flow = {key: np.zeros(n) for key in ("p", "T", "rho", "e", "YB", "a", "velx",
-                                                        "vely", "velz")}
-            idx = 0
-            for kk in range(self.nkc):
-                for jj in range(self.njc):
-                    for ii in range(self.nic):
-                        st = initialState(self.cellc.x[ii, jj, kk], self.cellc.y[ii, jj, kk],
-                                          self.cellc.z[ii, jj, kk])
-                        for key, val in (("p", st.gas.p), ("T", st.gas.T), ("rho", st.gas.rho),
-                                         ("e", st.gas.e), ("YB", st.gas.YB), ("a", st.gas.a),
-                                         ("velx", st.vel.x), ("vely", st.vel.y), ("velz", st.vel.z)):
-                            self.flow[key][idx] = val
-                        idx += 1
-        else:
-            raise RuntimeError("Need to supply a FlowState object or a function that produces"
-                               " a FlowState object as a function of (x,y,z) location.")
+        self.flow = _initial_flow(initialState, self.cellc, self.nic * self.njc * self.nkc)
         self.bcs = _resolve_bcs(bcs)
-        # Per-face TWall of function-form walls, kept per block (chkn_prep.py:688-692 keeps
-        # them on the boundary-condition object, which blocks may share).
+        # TWall per face of function-form walls, kept with the block (a wall object may be
+        # shared by several blocks).
         self.twall_values = {face: bc.generate_values(grid, face) for face, bc in self.bcs.items()
-                             if type(bc) is WallNoSlipFixedTBC and bc.form == "fun"}
+                             if isinstance(bc, WallNoSlipFixedTBC) and bc.form == "fun"}
         for bc in bcs.values():
-            if type(bc) is InflowBC:
+            if isinstance(bc, InflowBC):
                 bc.fsi = len(flowStatesList)
                 flowStatesList.append(bc.fs)
         self.i, self.j, self.k = i, j, k
-        self.active = active
-        self.label = label
+        self.active, self.label = active, label
         self.indx = len(fluidBlocksList)
         fluidBlocksList.append(self)
 
     def to_json(self):
-        s = '{"i": %d, "j": %d, "k": %d,' % (self.i, self.j, self.k)
-        s += ' "bcs": {' + ",".join('"%s": %s' % (f, self.bcs[f].to_json()) for f in FaceList) + "},"
-        s += ' "active": %s,' % json.dumps(self.active)
-        s += ' "label": "%s"}' % self.label
-        return s
+        sides = ",".join('"%s": %s' % (f, self.bcs[f].to_json()) for f in FaceList)
+        return ('{"i": %d, "j": %d, "k": %d, "bcs": {%s}, "active": %s, "label": "%s"}'
+                % (self.i, self.j, self.k, sides, _js(self.active), self.label))
 
     def iovars(self):
-        """(13, ncells) array in flow-file order."""
-        n = self.nic * self.njc * self.nkc
-        out = np.empty((13, n))
-        out[0] = self.cellc.x.transpose().ravel()
-        out[1] = self.cellc.y.transpose().ravel()
-        out[2] = self.cellc.z.transpose().ravel()
-        out[3] = self.cellv.transpose().ravel()
-        for q, key in enumerate(("p", "T", "rho", "e", "YB", "a", "velx", "vely", "velz")):
-            out[4 + q] = self.flow[key]
-        return out
+        """The 13 IO variables, (13, ncells), in flow-file order."""
+        geo = [self.cellc.x, self.cellc.y, self.cellc.z, self.cellv]
+        return np.vstack([g.transpose().ravel() for g in geo] + [self.flow[k] for k in FLOW_KEYS])
 
     def write_flow_data_to_file(self, fileName, binaryData):
         write_flow_file(fileName, self.iovars(), binaryData)
 
 
 def write_flow_file(fileName, iovars, binaryData):
-    """Flow file: the 13 IO variables one after another (chkn_prep.py:749-780)."""
+    """A flow file: the 13 IO variables one after another, float64 (.bin) or one value per
+    line (.gz)."""
     iovars = np.ascontiguousarray(iovars, dtype=float)
     if binaryData:
         iovars.tofile(fileName)
     else:
-        import gzip
         with gzip.open(fileName, "wt") as f:
             np.savetxt(f, iovars.ravel())
 
 
+def _split_axis(nv, parts):
+    """Vertex ranges (start, count) of `parts` consecutive blocks along an axis of nv
+    vertices: equal cell counts, the last block taking the remainder."""
+    step = (nv - 1) // parts
+    if step < 1:
+        raise ValueError("cannot split %d cells into %d blocks" % (nv - 1, parts))
+    starts = [q * step for q in range(parts)]
+    return [(s, (step if q < parts - 1 else nv - 1 - s) + 1) for q, s in enumerate(starts)]
+
+
 def makeFBArray(i0=0, j0=0, k0=0, ni=1, nj=1, nk=1, grid=None, initialState=None, bcs={},
                 active=True, label=""):
-    """Split one grid into ni x nj x nk blocks joined by exchange boundaries
-    (chkn_prep.py:784-851)."""
+    """Split one grid into ni x nj x nk blocks placed at array positions (i0+i, j0+j, k0+k);
+    inner sides are exchange boundaries, outer sides take `bcs`.  Returns the blocks."""
     if not isinstance(grid, StructuredGrid):
-        raise RuntimeError("Need to supply a StructuredGrid object to subdivide.")
+        raise TypeError("makeFBArray needs a StructuredGrid to split, got %r" % (grid,))
     if not (isinstance(initialState, (FlowState, dict)) or callable(initialState)):
-        raise RuntimeError("Need to supply a FlowState object or a function for initialState.")
-    _bcs = _resolve_bcs(bcs)
-    blks = []
-    nic = (grid.niv - 1) // ni
-    njc = (grid.njv - 1) // nj
-    nkc = (grid.nkv - 1) // nk
-    for k in range(nk):
-        k0v = k * nkc
-        nkv = nkc + 1 if k < nk - 1 else grid.nkv - k0v
-        for j in range(nj):
-            j0v = j * njc
-            njv = njc + 1 if j < nj - 1 else grid.njv - j0v
-            for i in range(ni):
-                i0v = i * nic
-                niv = nic + 1 if i < ni - 1 else grid.niv - i0v
-                newBCs = {
-                    "iminus": _bcs["iminus"] if i == 0 else ExchangeBC(),
-                    "iplus": _bcs["iplus"] if i == ni - 1 else ExchangeBC(),
-                    "jminus": _bcs["jminus"] if j == 0 else ExchangeBC(),
-                    "jplus": _bcs["jplus"] if j == nj - 1 else ExchangeBC(),
-                    "kminus": _bcs["kminus"] if k == 0 else ExchangeBC(),
-                    "kplus": _bcs["kplus"] if k == nk - 1 else ExchangeBC(),
-                }
-                sub = grid.subgrid(i0=i0v, j0=j0v, k0=k0v, niv=niv, njv=njv, nkv=nkv)
-                init = initialState
-                if isinstance(initialState, dict):
-                    init = {key: np.asarray(v)[i0v:i0v + niv - 1, j0v:j0v + njv - 1, k0v:k0v + nkv - 1]
-                            if np.ndim(v) == 3 else v for key, v in initialState.items()}
-                blks.append(FluidBlock(i=i0 + i, j=j0 + j, k=k0 + k, grid=sub, initialState=init,
-                                       bcs=newBCs, active=active,
-                                       label=label + "-%d-%d-%d" % (i, j, k)))
-    return blks
+        raise TypeError("initialState must be a FlowState, a dict of per-cell arrays or a "
+                        "function of (x, y, z) giving a FlowState")
+    outer = _resolve_bcs(bcs)
+    ranges = [_split_axis(nv, parts) for nv, parts in ((grid.niv, ni), (grid.njv, nj), (grid.nkv, nk))]
+    blocks = []
+    for k, j, i in itertools.product(range(nk), range(nj), range(ni)):
+        pos, counts = (i, j, k), (ni, nj, nk)
+        sides = {}
+        for q, name in enumerate(FaceList):
+            d, plus = divmod(q, 2)
+            at_edge = pos[d] == (counts[d] - 1 if plus else 0)
+            sides[name] = outer[name] if at_edge else ExchangeBC()
+        (si, ni_v), (sj, nj_v), (sk, nk_v) = ranges[0][i], ranges[1][j], ranges[2][k]
+        sub = grid.subgrid(i0=si, j0=sj, k0=sk, niv=ni_v, njv=nj_v, nkv=nk_v)
+        init = initialState
+        if isinstance(initialState, dict):
+            cells = (slice(si, si + ni_v - 1), slice(sj, sj + nj_v - 1), slice(sk, sk + nk_v - 1))
+            init = {key: (np.asarray(v)[cells] if np.ndim(v) == 3 else v) for key, v in initialState.items()}
+        blocks.append(FluidBlock(i=i0 + i, j=j0 + j, k=k0 + k, grid=sub, initialState=init,
+                                 bcs=sides, active=active, label="%s-%d-%d-%d" % (label, i, j, k)))
+    return blocks
+
+
+# --------------------------------------------------------------------------- job files
+def _json_list(items):
+    """Entries of a config.json list, one per line."""
+    return ",\n".join("    %s" % it for it in items) + ("\n" if items else "")
 
 
 def config_json_text():
-    """The config.json text (chkn_prep.py:868-895)."""
-    import io
+    """The text of config.json for the job described so far."""
     config.check_array_of_fluid_blocks()
     fp = io.StringIO()
     fp.write("{\n")
     config.write(fp)
-    n_fs = len(flowStatesList)
-    fp.write('  "n_flow_states": %d,\n' % n_fs)
-    fp.write('  "flow_states": [\n')
-    for i, fs in enumerate(flowStatesList):
-        fp.write("    %s" % fs.to_json())
-        fp.write(",\n" if i < n_fs - 1 else "\n")
-    fp.write("  ],\n")
-    n_fb = len(fluidBlocksList)
-    fp.write('  "n_fluid_blocks": %d,\n' % n_fb)
-    fp.write('  "nib": %d,\n' % config.nib)
-    fp.write('  "njb": %d,\n' % config.njb)
-    fp.write('  "nkb": %d,\n' % config.nkb)
-    fp.write('  "blk_ids": %s,\n' % json.dumps(config.blk_ids.tolist()))
-    fp.write('  "nics": %s,\n' % json.dumps(config.nics.tolist()))
-    fp.write('  "njcs": %s,\n' % json.dumps(config.njcs.tolist()))
-    fp.write('  "nkcs": %s,\n' % json.dumps(config.nkcs.tolist()))
-    fp.write('  "fluid_blocks": [\n')
-    for i, fb in enumerate(fluidBlocksList):
-        fp.write("    %s" % fb.to_json())
-        fp.write(",\n" if i < n_fb - 1 else "\n")
-    fp.write("  ]\n")
+    fp.write('  "n_flow_states": %d,\n' % len(flowStatesList))
+    fp.write('  "flow_states": [\n%s  ],\n' % _json_list([fs.to_json() for fs in flowStatesList]))
+    fp.write('  "n_fluid_blocks": %d,\n' % len(fluidBlocksList))
+    for key, val in (("nib", config.nib), ("njb", config.njb), ("nkb", config.nkb)):
+        fp.write('  "%s": %d,\n' % (key, val))
+    for key, arr in (("blk_ids", config.blk_ids), ("nics", config.nics), ("njcs", config.njcs),
+                     ("nkcs", config.nkcs)):
+        fp.write('  "%s": %s,\n' % (key, _js(arr.tolist())))
+    fp.write('  "fluid_blocks": [\n%s  ]\n' % _json_list([fb.to_json() for fb in fluidBlocksList]))
     fp.write("}\n")
     return fp.getvalue()
 
 
 def write_initial_files(binaryData, job_dir=None):
-    """Write config.json, grids, flow/t0000 and times.data into the job directory."""
+    """Write config.json, the grids, flow/t0000, TWall files and times.data into job_dir
+    (default: the job name)."""
     job = job_dir or config.job_name
-    os.makedirs(job, exist_ok=True)
+    for sub in ("grid", os.path.join("flow", "t0000")):
+        os.makedirs(os.path.join(job, sub), exist_ok=True)
     with open(os.path.join(job, "config.json"), "w") as fp:
         fp.write(config_json_text())
-    os.makedirs(os.path.join(job, "grid"), exist_ok=True)
+    ext = ".bin" if binaryData else ".gz"
     for fb in fluidBlocksList:
-        name = os.path.join(job, "grid", "grid-%04d-%04d-%04d" % (fb.i, fb.j, fb.k))
+        tag = "%04d-%04d-%04d" % (fb.i, fb.j, fb.k)
+        grid_file = os.path.join(job, "grid", "grid-" + tag + ext)
         if binaryData:
-            fb.grid.write_to_binary_file(name + ".bin")
+            fb.grid.write_to_binary_file(grid_file)
         else:
-            fb.grid.write_to_gzip_file(name + ".gz")
-    flow_dir = os.path.join(job, "flow", "t0000")
-    os.makedirs(flow_dir, exist_ok=True)
-    for fb in fluidBlocksList:
-        name = os.path.join(flow_dir, "flow-%04d-%04d-%04d" % (fb.i, fb.j, fb.k))
-        fb.write_flow_data_to_file(name + (".bin" if binaryData else ".gz"), binaryData)
-        for face, values in fb.twall_values.items():  # chkn_prep.py:919-925
-            write_twall_file(os.path.join(job, "TWall-%04d-%04d-%04d-%s.gz" % (fb.i, fb.j, fb.k, face)),
-                             values)
+            fb.grid.write_to_gzip_file(grid_file)
+        fb.write_flow_data_to_file(os.path.join(job, "flow", "t0000", "flow-" + tag + ext), binaryData)
+        for face, values in fb.twall_values.items():
+            write_twall_file(os.path.join(job, "TWall-%s-%s.gz" % (tag, face)), values)
     with open(os.path.join(job, "times.data"), "w") as fp:
-        fp.write("# tindx t\n")
-        fp.write("0 0.0\n")
+        fp.write("# tindx t\n0 0.0\n")
 
 
 def reset():
-    """Start a fresh job description (module-level state, as in chkn_prep.py)."""
+    """Start a new job description (the module holds one job at a time, as job scripts
+    expect)."""
     global config, fluidBlocksList, flowStatesList
     GlobalConfig.count = 0
     config = GlobalConfig()
@@ -576,41 +558,40 @@ reset()
 
 
 def script_namespace():
-    """Names visible to a job script."""
-    import sys
-    mod = sys.modules[__name__]
-    ns = {k: getattr(mod, k) for k in dir(mod) if not k.startswith("_")}
-    ns["math"] = math
-    ns["np"] = np
+    """The names a job script sees."""
+    ns = {k: v for k, v in globals().items() if not k.startswith("_")}
+    ns.update(math=math, np=np)
     return ns
 
 
 def run_script(path):
-    """Execute a job script with this module's names in scope (chkn_prep.py:958)."""
-    import sys
-    mod = sys.modules[__name__]
+    """Run a job script with this module's names in scope."""
     with open(path, "rb") as f:
         code = compile(f.read(), path, "exec")
-    exec(code, mod.__dict__)
+    exec(code, globals())
+
+
+USAGE = "Usage: chkn-prep [--help | -h] [--job=<jobName> | -f <jobName>] [--binary | -b]"
 
 
 def main(argv=None):
-    """chkn-prep --job=<name> [--binary]"""
+    """chkn-prep --job=<name> [--binary]: run <name>.py and write the job directory <name>."""
     import getopt
-    import sys
     import time
-    argv = sys.argv[1:] if argv is None else argv
-    opts, _ = getopt.getopt(argv, "hf:b", ["help", "job=", "binary"])
-    uo = dict(opts)
-    if not opts or "--help" in uo or "-h" in uo:
-        print("Usage: chkn-prep [--help | -h] [--job=<jobName> | -f <jobName>] [--binary | -b]")
+    opts, _ = getopt.getopt(sys.argv[1:] if argv is None else argv, "hf:b", ["help", "job=", "binary"])
+    given = dict(opts)
+    if not opts or given.keys() & {"--help", "-h"}:
+        print(USAGE)
         return 0
-    job = uo.get("--job", uo.get("-f", ""))
+    job = given.get("--job") or given.get("-f")
     if not job:
-        raise Exception("Job name is not specified.")
-    config.job_name, _ = os.path.splitext(job)
+        raise SystemExit("chkn-prep: no job given\n" + USAGE)
+    config.job_name = os.path.splitext(job)[0]
     run_script(config.job_name + ".py")
-    config.check_array_of_fluid_blocks()
-    write_initial_files(("--binary" in uo) or ("-b" in uo))
-    print("Done in {:.3f} seconds.".format(time.process_time()))
+    write_initial_files(bool(given.keys() & {"--binary", "-b"}))
+    print("chkn-prep: wrote job %s in %.3f s" % (config.job_name, time.process_time()))
     return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
