@@ -25,12 +25,15 @@ sys.path.insert(0, os.path.join(ROOT, "gdtk-rmhd_amd"))
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BYTES_PER_CELL_STEP = 864  # algorithmic FP64 bytes per cell-update (3 stages)
 BYTES_PER_CELL_DT = 144    # extra bytes per cell per CFL reduction
+# Two-stage predictor-corrector: stage 1 reads U0 and writes U1 and L0 (18 words), stage 2
+# reads U1, U0 and L0 and writes U (24 words), plus 10 geometry words per stage.
+BYTES_PER_CELL_STEP_PC = 496
 # Viscous step: also the cell centre and the three face centres a cell's clouds need, per stage.
 BYTES_PER_CELL_STEP_VISCOUS = BYTES_PER_CELL_STEP + 3 * 96
 
 
 WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300, "diff": 500,
-                  "ensemble-diff": 300, "viscous": 300, "shard": 500}
+                  "ensemble-diff": 300, "viscous": 300, "shard": 500, "pc": 500}
 # Steps of the reference CPU build timed for cpu_baseline: a sample of about 10-30 s on one
 # core (the viscous step runs about 6x slower there than the inviscid one).
 CPU_STEPS = {"viscous": 1}
@@ -43,6 +46,12 @@ def build_workload(name, steps_total):
         jobs = [cases.ramp(256, 128, 128, max_step=steps_total)]
         desc = {"workload": "plain-at-scale: one 256x128x128 block, Mach 5 flow over a 15-degree "
                             "ramp, p=2e3 Pa, T=220 K, 1% seeded density perturbation, FP64"}
+    elif name == "pc":
+        jobs = [cases.ramp(256, 128, 128, max_step=steps_total)]
+        jobs[0].time_scheme = 1
+        desc = {"workload": "predictor-corrector at scale: the 256x128x128 Mach 5 ramp block of "
+                            "plain-at-scale stepped by the two-stage predictor-corrector "
+                            "(time_scheme predictor_corrector), FP64"}
     elif name == "viscous":
         jobs = [cases.ramp(256, 128, 128, max_step=steps_total, viscous=True)]
         desc = {"workload": "viscous: one 256x128x128 block, Mach 5 flow over a 15-degree ramp "
@@ -77,9 +86,10 @@ def build_workload(name, steps_total):
                             "(Mach, p, T), checkpoint every 50 steps, FP64"}
     else:
         raise SystemExit("unknown workload " + name)
+    stepper = ("predictor-corrector (2 stages)" if jobs[0].time_scheme == 1 else "TVD-RK3 (3 stages)")
     desc.update({"cells": sum(j.active_cells for j in jobs),
                  "blocks": sum(len(j.blocks) for j in jobs), "instances": len(jobs),
-                 "scheme": "MUSCL/van Albada + AUSMDV, TVD-RK3 (3 stages)"
+                 "scheme": "MUSCL/van Albada + AUSMDV, " + stepper
                            + (", viscous fluxes" if jobs[0].viscous else ""), "cfl": 0.5,
                  "cfl_count": jobs[0].cfl_count})
     return jobs, desc
@@ -165,6 +175,9 @@ def cpu_baseline(job, steps):
     nblocks = sum(1 for b in job.blocks if b.active)
     if os.path.exists(ref):
         from chkn import prep
+        held = sum(b.nic * b.njc * b.nkc for b in prep.fluidBlocksList if b.active)
+        if held != cells:
+            raise RuntimeError("chkn.prep holds another job (%d active cells, not %d)" % (held, cells))
         d = tempfile.mkdtemp(prefix="chkn_cpu_")
         try:
             prep.write_initial_files(True, d)
@@ -243,6 +256,8 @@ def main():
                     help="also time one launch per block per stage (default: batched only)")
     ap.add_argument("--cpu-steps", type=int, default=0, help="reference CPU steps timed (default 4; viscous 1)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sub-records", type=int, default=1,
+                    help="scale workload: also report the batched and diff forms (default 1)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device copy (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -287,23 +302,7 @@ def main():
         print(json.dumps(line))
         return
     wall, kernel_ms, launches, rep, cells = run_timed(jobs, opts, steps, args.warmup)
-    first = args.warmup
-    reductions = sum(1 for s in range(first + 1, first + steps + 1) if s % jobs[0].cfl_count == 0)
-    viscous = bool(jobs[0].viscous)
-    per_step = BYTES_PER_CELL_STEP_VISCOUS if viscous else BYTES_PER_CELL_STEP
-    alg_bytes = cells * (per_step * steps + BYTES_PER_CELL_DT * reductions)
-    value = cells * steps / wall
-    achieved = alg_bytes / (kernel_ms * 1e-3)
-    # Measured HBM bytes per stage launch for this launch shape (TIxTJxTK, TK 0 = the default
-    # k-chunking), from the profile summary of the workload; null for a shape not profiled.
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "stage_kernel_traffic_%s.txt" % args.workload)
-    if os.path.exists(tfile) and opts:
-        shape = "%dx%dx%d" % (opts.get("tile_i", 0), opts.get("tile_j", 0), opts.get("tile_k", 0))
-        with open(tfile) as f:
-            for ln in f:
-                if ln.startswith("hbm_bytes_per_launch %s:" % shape):
-                    traffic = float(ln.split(":")[1])
+    rec = roofline_record(jobs, opts, steps, args.warmup, args.workload, kernel_ms, launches, cells)
     config = dict(desc, launch=opts, steps_completed=rep.step)
     if tuned:
         config["autotune"] = tuned
@@ -317,7 +316,7 @@ def main():
             "fused_speedup": pw / wall}
     line = {
         "metric": "cell-updates/s",
-        "value": value,
+        "value": cells * steps / wall,
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": steps,
@@ -329,19 +328,99 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": config,
-        "hbm_pct_of_peak": 100.0 * achieved / HBM_PEAK,
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": ("split_face_kernel + split_viscous_kernel + split_update_kernel "
-                                "(9 launches per step)") if viscous else "stage_kernel (3 launches per step)",
-                     "avg_launch_us": kernel_ms * 1e3 / max(launches, 1),
-                     "alg_bytes_per_launch": alg_bytes / max(launches, 1)},
+        "hbm_pct_of_peak": 100.0 * rec["frac"],
+        "roofline": rec,
     }
     if not args.no_cpu:
-        # chkn.prep holds the job built last: for an ensemble, its last member (the reference
-        # runs one job per process).
+        # chkn.prep holds the job built last (before the sub-records build theirs): for an
+        # ensemble, its last member (the reference runs one job per process).
         line["cpu_baseline"] = cpu_baseline(jobs[-1], args.cpu_steps or CPU_STEPS.get(args.workload, 4))
+    if args.workload == "scale" and args.sub_records:
+        # The rest of the metric: the batched form against the reference's per-block launch
+        # structure, and forward + adjoint against the plain forward.
+        line["batched"] = batched_record(args.graph)
+        line["diff"] = diff_record(args.graph)
     print(json.dumps(line))
+
+
+def roofline_record(jobs, opts, steps, warmup, workload, kernel_ms, launches, cells):
+    """The roofline object of a timed run: algorithmic bytes of the timed steps over the stage
+    kernels' device time (HIP events on the library's stream), and the measured HBM bytes per
+    launch of this launch shape from the workload's profile summary (null if not profiled)."""
+    reductions = sum(1 for s in range(warmup + 1, warmup + steps + 1) if s % jobs[0].cfl_count == 0)
+    viscous = bool(jobs[0].viscous)
+    pc = jobs[0].time_scheme == 1
+    per_step = (BYTES_PER_CELL_STEP_VISCOUS if viscous else
+                BYTES_PER_CELL_STEP_PC if pc else BYTES_PER_CELL_STEP)
+    alg_bytes = cells * (per_step * steps + BYTES_PER_CELL_DT * reductions)
+    achieved = alg_bytes / (kernel_ms * 1e-3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "stage_kernel_traffic_%s.txt" % workload)
+    if os.path.exists(tfile) and opts:
+        shape = "%dx%dx%d%s" % (opts.get("tile_i", 0), opts.get("tile_j", 0), opts.get("tile_k", 0),
+                                "-vf" if opts.get("vertex_frames") else "")
+        with open(tfile) as f:
+            for ln in f:
+                if ln.startswith("hbm_bytes_per_launch %s:" % shape):
+                    traffic = float(ln.split(":")[1])
+    nst = 2 if pc else 3
+    kernel = ("split_face_kernel + split_viscous_kernel + split_update_kernel (%d launches per step)"
+              % (3 * nst) if viscous else "stage_kernel (%d launches per step)" % nst)
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": kernel,
+            "avg_launch_us": kernel_ms * 1e3 / max(launches, 1),
+            "alg_bytes_per_launch": alg_bytes / max(launches, 1)}
+
+
+# Launch shapes of the sub-records, from the tuner's choices on these workloads (fused: the
+# stage kernel's default k-chunking; per block: one plane per work-group, without which a
+# small block's launch cannot fill the device).
+SUB_FUSED = [dict(tile_i=32, tile_j=8, tile_k=0), dict(tile_i=16, tile_j=16, tile_k=0),
+             dict(tile_i=32, tile_j=8, tile_k=4)]
+SUB_PER_BLOCK = [dict(tile_i=32, tile_j=8, tile_k=1), dict(tile_i=16, tile_j=16, tile_k=1),
+                 dict(tile_i=64, tile_j=4, tile_k=1)]
+
+
+def _best_of(jobs, candidates, steps, warmup, graph, per_block):
+    """Time each candidate launch shape on a fresh handle; the fastest run's numbers."""
+    best = None
+    for c in candidates:
+        opts = dict(c, use_graph=bool(graph), per_block_launch=per_block)
+        wall, kernel_ms, launches, rep, cells = run_timed(jobs, opts, steps, warmup)
+        if best is None or wall < best[0]:
+            best = (wall, kernel_ms, launches, rep, cells, opts)
+    return best
+
+
+def batched_record(graph, steps=60, warmup=10):
+    """The 40-block uneven array (5x4x2, slab sizes seeded in [16, 96]) in one launch per
+    stage for all blocks, against one launch per block per stage (the reference's launch
+    structure, simulate.cu:605-625), each with its best launch shape of a short list."""
+    jobs, desc = build_workload("batched", warmup + steps)
+    wall, kms, launches, rep, cells, opts = _best_of(jobs, SUB_FUSED, steps, warmup, graph, False)
+    rec = roofline_record(jobs, opts, steps, warmup, "batched", kms, launches, cells)
+    pw, _, pl, _, _, popts = _best_of(jobs, SUB_PER_BLOCK, steps, warmup, graph, True)
+    return {"workload": desc["workload"], "cells": cells, "steps": steps, "warmup": warmup,
+            "value": cells * steps / wall, "unit": "cell-updates/s", "ms_per_step": wall * 1e3 / steps,
+            "launch": opts, "roofline": rec,
+            "per_block_value": cells * steps / pw, "per_block_launches": pl, "per_block_launch": popts,
+            "fused_over_per_block": pw / wall}
+
+
+def diff_record(graph, steps=100):
+    """Forward (taped, checkpoint every 50 steps) + adjoint of the wall-pressure loss of the
+    128x64x64 ramp w.r.t. U0 and the inflow (Mach, p, T), against the plain forward of the
+    same steps."""
+    jobs, desc = build_workload("diff", steps)
+    opts = dict(tile_i=32, tile_j=8, tile_k=0, use_graph=bool(graph), per_block_launch=False)
+    r = run_diff(jobs, opts, steps, 5)
+    wall = r["seconds_forward_taped"] + r["seconds_adjoint"]
+    return {"workload": desc["workload"], "cells": r["cells"], "steps": steps, "checkpoint_every": 50,
+            "value": r["cells"] * steps / wall, "unit": "cell-updates/s (forward + adjoint)",
+            "plain_forward_value": r["cells"] * steps / r["seconds_plain_forward"],
+            "fwd_adj_over_fwd": r["fwd_adj_over_fwd"], "launch": opts,
+            "seconds": {k: r[k] for k in ("seconds_forward_taped", "seconds_adjoint",
+                                          "seconds_plain_forward")}}
 
 
 if __name__ == "__main__":

@@ -50,6 +50,7 @@ class _Unroll(torch.autograd.Function):
         for s in range(w.shape[0]):
             sim.set_inflow_state(s, w[s])
         ctx.dts = sim.grad_forward(nsteps, checkpoint_every, dts)
+        ctx.generation = sim.grad_generation
         uN = torch.empty_like(u0)
         sim.read_conserved(uN)
         sim.set_stream(None)
@@ -59,6 +60,11 @@ class _Unroll(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g):
+        # The handle holds the checkpoints and dts of its latest unroll only: reversing an
+        # earlier one with them would give a wrong gradient without any error.
+        if ctx.sim.grad_generation != ctx.generation:
+            raise RuntimeError("this unroll's tape was replaced by a later unroll on the same Sim: "
+                               "call backward before the next unroll, or unroll on separate Sims")
         lam = g.contiguous()
         gu0 = torch.empty_like(lam)
         ctx.sim.set_stream(torch.cuda.current_stream())

@@ -6,6 +6,8 @@
 * ``ramp_array``     -- the ramp split into an array of tensor-product uneven blocks with a
                         seeded exchange/wall connection graph.
 * ``mixed_bc_array`` -- a small block array exercising every built boundary condition.
+* ``forward_step``   -- Mach 3 flow over a forward-facing step (the reference's integration
+                        example), 25 active blocks plus the inactive blocks inside the step.
 
 Each function describes the job through chkn.prep and returns ``JobData``.
 """
@@ -414,3 +416,50 @@ def ab_reacting(nics=(16, 12), njc=6, nkc=5, max_step=30, seed=2, reacting=True)
         init = {key: np.ascontiguousarray(val[io[i]:io[i + 1]]) for key, val in st.items()}
         prep.FluidBlock(i=i, j=0, k=0, grid=sub, initialState=init, bcs=bcs)
     return JobData.from_prep()
+
+
+def forward_step(max_time=1.0e-3, dx=0.01, max_step=6000, dt_init=1.0e-6, cfl=0.5):
+    """Mach 3 air (p = 101.325 kPa, T = 300 K) entering a 3 x 1 x 0.1 m channel from x = 0,
+    with a step of height 0.2 m filling x >= 0.6 (the forward-facing-step example of the
+    reference, examples/chicken/forward-facing-step).  Cells are dx cubes in x and y and two
+    layers in z.  The array is 5 x 5 blocks of 0.6 x 0.2 m: column 0 (upstream of the step) and
+    rows 1-4 above the step are active, the four blocks inside the step (row 0, columns 1-4)
+    are inactive at rest.  Inflow on the upstream face, outflow at x = 3, slip walls
+    elsewhere (the step's face and top, the channel floor, roof and side planes)."""
+    _common(max_step, cfl=cfl, dt_init=dt_init, max_time=max_time,
+            title="Forward-facing step with supersonic flow.")
+    at_rest = prep.FlowState(p=101.325e3, T=300.0)
+    inflow = prep.FlowState(p=101.325e3, T=300.0, velx=3.0 * at_rest.gas.a)
+    ni, nj = int(round(0.6 / dx)), int(round(0.2 / dx))
+    grid = box_grid(5 * ni, 5 * nj, 2, L=(3.0, 1.0, 0.1))
+    inflow_bc = prep.InflowBC(inflow)
+    for i in range(5):
+        for j in range(5):
+            active = i == 0 or j > 0
+            bcs = {}
+            if active:
+                side = {"iminus": (i == 0, inflow_bc), "iplus": (i == 4, prep.OutflowBC()),
+                        "jminus": (j == 0 or (j == 1 and i > 0), prep.WallWithSlipBC()),
+                        "jplus": (j == 4, prep.WallWithSlipBC())}
+                for name, (outer, bc) in side.items():
+                    bcs[name] = bc if outer else prep.ExchangeBC()
+                if i == 0 and j == 0:
+                    bcs["iplus"] = prep.WallWithSlipBC()  # the step's upstream face
+            sub = grid.subgrid(i0=i * ni, j0=j * nj, k0=0, niv=ni + 1, njv=nj + 1, nkv=3)
+            prep.FluidBlock(i=i, j=j, k=0, grid=sub, initialState=inflow if active else at_rest,
+                            bcs=bcs, active=active, label="ffs-%d-%d" % (i, j))
+    return JobData.from_prep()
+
+
+def shock_position(x, p, p_shock=1047025.0, rise=0.01):
+    """Position of a normal shock on a line of cells (x, p ordered downstream-to-upstream or
+    upstream-to-downstream): the first cell whose pressure exceeds the line's first value by
+    `rise`, then linear interpolation between it and the cell before to the pressure p_shock
+    (examples/chicken/test_chicken.py:53-66 states the measure).  Returns (x_shock, cell
+    spacing there), or (None, None) when no cell rises."""
+    x, p = np.asarray(x, float), np.asarray(p, float)
+    up = np.nonzero(p > p[0] * (1.0 + rise))[0]
+    if len(up) == 0 or up[0] == 0:
+        return None, None
+    s = up[0]
+    return (x[s] - x[s - 1]) / (p[s] - p[s - 1]) * (p_shock - p[s - 1]) + x[s - 1], x[s] - x[s - 1]

@@ -15,6 +15,8 @@ BC_CODES = {"wall_with_slip": 0, "wall_no_slip_adiabatic": 1, "wall_no_slip": 1,
             "wall_no_slip_fixed_T": 2, "exchange": 3, "inflow": 4, "outflow": 5,
             "inflow_function": 6}
 FACE_NAMES = ("iminus", "iplus", "jminus", "jplus", "kminus", "kplus")
+# config.json "time_scheme" values (a key of this build; the reference always steps TVD-RK3).
+TIME_SCHEMES = ("rk3", "predictor_corrector")
 # face.cu:51-68: inflow functions; unknown names mean none.
 BC_FUNCTIONS = {"none": 0, "supersonic_vortex": 1, "laminar_boundary_layer": 2,
                 "manufactured_solution": 3}
@@ -82,6 +84,7 @@ class JobData:
         self.flow_states = np.zeros((0, 6))
         self.x_order = 2
         self.t_order = 2
+        self.time_scheme = 0  # TIME_SCHEMES index: 0 TVD-RK3 (the reference), 1 predictor-corrector
         self.flux_calc = 0
         self.cfl_count = 10
         self.max_step = 100
@@ -110,6 +113,7 @@ class JobData:
         self.flow_states = np.array(fs, dtype=float).reshape(-1, 6)
         self.x_order = cfg["x_order"]
         self.t_order = cfg["t_order"]
+        self.time_scheme = TIME_SCHEMES.index(cfg.get("time_scheme", "rk3"))
         self.flux_calc = 1 if cfg["flux_calc"] in ("sbp_asf", "sbp-asf") else 0
         self.cfl_count = cfg["cfl_count"]
         self.max_step = cfg["max_step"]

@@ -35,7 +35,8 @@ class ChknDesc(ctypes.Structure):
                 ("n_dt_plot", ctypes.c_int), ("dt_plot_times", _D), ("dt_plot_values", _D),
                 ("t0", ctypes.c_double), ("start_tindx", ctypes.c_int), ("reacting", ctypes.c_int),
                 ("viscous", ctypes.c_int), ("bc_twall", _D), ("bc_fun", _I), ("bl_profile", _D),
-                ("n_twall_faces", ctypes.c_int), ("twall_faces", _D), ("gas_model", _D)]
+                ("n_twall_faces", ctypes.c_int), ("twall_faces", _D), ("gas_model", _D),
+                ("time_scheme", ctypes.c_int)]
 
 
 class ChknReport(ctypes.Structure):
@@ -87,6 +88,10 @@ EXPORTS = {
     "chkn_active_cells": (ctypes.c_longlong, [ctypes.c_void_p]),
     "chkn_last_timing": (ctypes.c_int, [ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_longlong)]),
     "chkn_real_size": (ctypes.c_int, []),
+    "chkn_set_verbosity": (ctypes.c_int, [ctypes.c_int]),
+    "chkn_grad_generation": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "chkn_check_block_dims": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "chkn_device_bytes": (ctypes.c_longlong, [ctypes.c_void_p]),
     "chkn_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "chkn_destroy": (None, [ctypes.c_void_p]),
 }
@@ -179,6 +184,7 @@ def _desc(job, keep):
     d.n_twall_faces = len(f["twall_faces"])
     d.twall_faces = _ptr(f["twall_faces"], _D) if len(f["twall_faces"]) else None
     d.gas_model = _ptr(f["gas_model"], _D)
+    d.time_scheme = int(job.time_scheme)
     return d
 
 
@@ -330,6 +336,11 @@ class Sim:
         self._check(self._lib.chkn_grad_forward(self.h, int(nsteps), int(checkpoint_every), dptr,
                                                 _ptr(out, _D)))
         return out
+
+    @property
+    def grad_generation(self):
+        """Number of grad_forward calls so far; grad_backward reverses the latest one."""
+        return int(self._lib.chkn_grad_generation(self.h))
 
     def grad_backward(self, lam, grad_u0):
         """Reverse the last grad_forward: lam, grad_u0 are flat float64 device tensors.

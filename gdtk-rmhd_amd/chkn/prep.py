@@ -166,6 +166,7 @@ _GLOBAL_KEYS = (
     ("print_count", lambda c: "%d" % c.print_count),
     ("x_order", lambda c: "%d" % c.x_order),
     ("t_order", lambda c: "%d" % c.t_order),
+    ("time_scheme", lambda c: _quoted(c.time_scheme)),
     ("flux_calc", lambda c: _quoted(c.flux_calc)),
     ("reacting", lambda c: _js(c.reacting)),
     ("viscous", lambda c: _js(c.viscous)),
@@ -178,6 +179,12 @@ _GLOBAL_KEYS = (
     ("iovar_names", lambda c: _js(c.iovar_names)),
 )
 
+# Keys the reference's config.json does not have, written only when they differ from their
+# default: "time_scheme" selects the two-stage predictor-corrector ("predictor_corrector")
+# instead of the reference's TVD-RK3 ("rk3"); chicken itself reads t_order and ignores it.
+_OPTIONAL_KEYS = {"time_scheme": "rk3"}
+TIME_SCHEMES = ("rk3", "predictor_corrector")
+
 # Attribute defaults of a new config (callables give a fresh mutable value each time).
 _CONFIG_DEFAULTS = (
     ("job_name", ""), ("title", "Chicken Simulation."), ("gas_model", GasModel),
@@ -185,7 +192,7 @@ _CONFIG_DEFAULTS = (
     ("nics", None), ("njcs", None), ("nkcs", None),
     ("dt_init", 1.0e-6), ("cfl_list", list), ("cfl_count", 10), ("print_count", 20),
     ("dt_plot_list", list), ("max_time", 1.0e-3), ("max_step", 100),
-    ("x_order", 2), ("t_order", 2), ("flux_calc", "ausmdv"), ("reacting", False),
+    ("x_order", 2), ("t_order", 2), ("time_scheme", "rk3"), ("flux_calc", "ausmdv"), ("reacting", False),
     ("viscous", False), ("source_terms", "none"), ("threads_per_gpu_block", 128),
     ("iovar_names", lambda: list(IOVAR_NAMES)),
 )
@@ -213,7 +220,11 @@ class GlobalConfig:
             add_cfl_value(0.0, 0.5)
         if not self.dt_plot_list:
             add_dt_plot(0.0, self.max_time, self.max_time)
+        if self.time_scheme not in TIME_SCHEMES:
+            raise ValueError("time_scheme must be one of %s, got %r" % (TIME_SCHEMES, self.time_scheme))
         for key, text in _GLOBAL_KEYS:
+            if key in _OPTIONAL_KEYS and getattr(self, key) == _OPTIONAL_KEYS[key]:
+                continue
             fp.write('  "%s": %s,\n' % (key, text(self)))
 
     def check_array_of_fluid_blocks(self):

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
             qR[q] = st[12 + q][t];
         }
     }
-    const Frame fr = ld_frame(g, (int)nf, fi);
+    const Frame fr = ld_frame(g, nf, fi);
     const double TL = eos_T(qL[1], qL[2]), TR = eos_T(qR[1], qR[2]);
     const double pL = eos_p(qL[0], TL), pR = eos_p(qR[0], TR);
     const double aL = eos_a(TL), aR = eos_a(TR);

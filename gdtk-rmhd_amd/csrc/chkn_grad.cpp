@@ -281,6 +281,8 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
     if (s->viscous) return set_error(s, CHKN_ERR_ARG, "the adjoint of the viscous fluxes is not built");
     if (s->flux_calc != 0) return set_error(s, CHKN_ERR_ARG, "the adjoint of the SBP-ASF flux is not built");
     if (s->any_reacting) return set_error(s, CHKN_ERR_ARG, "the adjoint of the A->B reaction is not built");
+    if (s->time_scheme != CHKN_TIME_RK3)
+        return set_error(s, CHKN_ERR_ARG, "the adjoint is built for the TVD-RK3 step only");
     // The unroll is a function of the conserved state: gas states follow from it, also in
     // the first stage (the recomputation from checkpoints does the same).
     s->fresh = false;
@@ -343,6 +345,7 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
     s->last_launches = launches;
     if (dt_out) std::copy(g.dts.begin(), g.dts.end(), dt_out);
     g.valid = true;
+    g.generation += 1;
     return CHKN_OK;
 }
 
@@ -508,6 +511,12 @@ int chkn_grad_backward(chkn_sim* sim, const double* lambda_dev, double* grad_u0_
 }
 
 long long chkn_state_size(chkn_sim* sim) { return sim ? (long long)flat_size(sim) : -1; }
+
+long long chkn_grad_generation(chkn_sim* sim)
+{
+    if (!sim) return -1;
+    return sim->grad ? sim->grad->generation : 0;
+}
 
 int chkn_num_flow_states(chkn_sim* sim) { return sim ? (int)sim->inflow_w.size() / 6 : -1; }
 

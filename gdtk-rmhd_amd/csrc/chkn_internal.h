@@ -177,7 +177,8 @@ hipError_t launch_chemistry(const BlockDev* blocks, const CellChunk* cells, int 
                             const StepCtl& ctl, int slot, bool dt_reduce, hipStream_t s);
 
 // Launch one stage over tiles [tile_base, tile_base + ntiles) of the table.
-// stage in 1..3; x_order 1 or 2; tile shape (ti, tj) must be built; vertex_frames 1 forms
+// stage 1..3 (TVD-RK3) or 4 (corrector of the predictor-corrector, after stage 1); x_order
+// 1 or 2; tile shape (ti, tj) must be built; vertex_frames 1 forms
 // the face frames from the block's vertices (built for 32x8 and 16x16 at x_order 2, FP64).
 hipError_t launch_stage(int stage, int x_order, int ti, int tj, int vertex_frames, bool dt_reduce,
                         bool persistent, int grid_limit, const StageArgs& a, hipStream_t s);

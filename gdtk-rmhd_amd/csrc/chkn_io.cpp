@@ -183,6 +183,12 @@ JobConfig read_config_json(const std::string& path)
     c.dt_plot_values = num_list(js["dt_plot"]);
     c.reacting = js["reacting"].as_bool();
     c.viscous = js["viscous"].as_bool();
+    if (js.has("time_scheme")) {
+        const std::string ts = js["time_scheme"].as_str();
+        if (ts == "rk3") c.time_scheme = 0;
+        else if (ts == "predictor_corrector") c.time_scheme = 1;
+        else throw std::runtime_error("Unknown time_scheme: " + ts);
+    }
     return c;
 }
 

@@ -34,6 +34,7 @@ struct JobConfig {
     std::vector<double> cfl_times, cfl_values, dt_plot_times, dt_plot_values;
     bool reacting = false, viscous = false;
     int source_terms = 0;
+    int time_scheme = 0;  // "time_scheme": "rk3" (0) or "predictor_corrector" (1); not a reference key
     std::vector<double> bl_profile;  // p_e, then (y, T, velx) x 12, when a side uses it
     std::vector<double> gas_model;   // gamma, R, q, alpha, Ti, Cv
 };

@@ -280,7 +280,7 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
     if (__ballot(bad != 0)) {
         int cnt = bad;
         for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&A.ctl.bad[STAGE - 1], cnt);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&A.ctl.bad[stage_position(STAGE) - 1], cnt);
     }
     if (DTRED) {
         const unsigned long long wm = wave_min_u64(sigmin);
@@ -326,10 +326,14 @@ hipError_t launch_shape(int stage, bool dtred, bool persistent, int grid_limit,
         hipLaunchKernelGGL((stage_kernel<1, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
     else if (stage == 2)
         hipLaunchKernelGGL((stage_kernel<2, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
-    else if (dtred)
+    else if (stage == 3 && dtred)
         hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, true, VF>), dim3(grid), blk, 0, s, a, p);
-    else
+    else if (stage == 3)
         hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
+    else if (dtred)
+        hipLaunchKernelGGL((stage_kernel<4, TI, TJ, XORDER, true, VF>), dim3(grid), blk, 0, s, a, p);
+    else
+        hipLaunchKernelGGL((stage_kernel<4, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
     return hipGetLastError();
 }
 

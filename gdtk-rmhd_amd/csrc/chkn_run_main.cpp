@@ -50,12 +50,15 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "You need to provide a job name.\n");
         return 1;
     }
+    if (verbosity > 0) std::printf("tindx_start: %d\nbinary_data: %d\n", tindx, binary ? 1 : 0);
+    chkn_set_verbosity(verbosity);
     chkn_sim* sim = nullptr;
     int rc = chkn_load_job(job.c_str(), tindx, binary ? 1 : 0, &sim);
     if (rc != CHKN_OK) {
         std::fprintf(stderr, "%s\n", chkn_last_error(nullptr));
     } else {
-        std::printf("Cells in simulation: %lld\n", chkn_active_cells(sim));
+        std::printf("Cells in simulation: %lld\n", chkn_active_cells(sim));  // simulate.cu:89-90
+        std::printf("Bytes allocated on GPU: %.3fMB\n", (double)chkn_device_bytes(sim) / 1.0e6);
         chkn_report rep;
         rc = chkn_march(sim, binary ? 1 : 0, &rep);
         if (rc != CHKN_OK) std::fprintf(stderr, "%s\n", chkn_last_error(sim));

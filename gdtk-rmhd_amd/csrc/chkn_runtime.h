@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -21,6 +22,8 @@ constexpr int kMaxSeg = 64;  // steps per segment (device control slots)
 
 // Error text of the last failed create/load on this thread.
 extern thread_local std::string g_create_error;
+// Console verbosity of job loading and marching (chkn_set_verbosity).
+extern int g_verbosity;
 
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -98,6 +101,7 @@ struct GradState {
     std::vector<double> t0;       // time of each instance at the start
     int step0 = 0, gstep0 = 0;    // counters at the start
     bool valid = false;
+    long long generation = 0;     // number of successful chkn_grad_forward calls on the handle
     // face/cell work tables and inflow patches (device)
     int nface_chunks = 0, ncell_chunks = 0, ninflow_cols = 0;
     void* d_face_chunks = nullptr;
@@ -140,6 +144,7 @@ struct chkn_sim {
     std::vector<Instance> inst;
     std::vector<chkn::real> inflow_w;  // 6 per state: rho, e, YB, vx, vy, vz (all instances)
     int x_order = 2, flux_calc = 0;
+    int time_scheme = 0;  // CHKN_TIME_RK3 or CHKN_TIME_PC (shared by the instances)
     bool viscous = false;
     bool any_reacting = false;
     // Gas states (p, T, a) of the inflow states as configured (update_from_pT), and whether
@@ -175,6 +180,8 @@ struct chkn_sim {
     std::map<GraphKey, hipGraphExec_t> graphs;
     // timing of the last chkn_step call
     double last_ms = 0.0;
+    std::chrono::steady_clock::time_point march_start;  // wall clock of the progress lines
+    long long device_bytes = 0;  // device memory allocated at set-up
     long long last_launches = 0;
     std::string err;
     // host copy of the device block table (the adjoint re-points W at its tape)
@@ -189,6 +196,7 @@ namespace rt {
 
 int set_error(chkn_sim* s, int code, const std::string& msg);
 void dfree(void* p);
+void dalloc(chkn_sim* s, void** p, size_t bytes);
 // Views of the control area for step slot q (N instances).
 real* ctl_dt(char* base, int q, int N);
 unsigned long long* ctl_sig(char* base, int q, int N);
@@ -198,6 +206,8 @@ real* ctl_chem(char* base, int q, int N);
 // Fill the chemistry factors of step slot q from the instances' dt (-1: not reacting).
 void fill_chem(chkn_sim* s, int q);
 StageArgs stage_args(chkn_sim* s, int q);
+// Stage launches per step: 3 (TVD-RK3) or 2 (predictor-corrector).
+int stages_per_step(const chkn_sim* s);
 // Runs K steps whose dt and run flags are already in the host control area.
 void run_segment(chkn_sim* s, int K, bool dtred_last);
 int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report* rep);

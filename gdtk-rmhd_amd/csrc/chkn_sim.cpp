@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <array>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -42,6 +43,7 @@ namespace chkn {
 namespace rt {
 
 thread_local std::string g_create_error;
+int g_verbosity = 0;  // Config::verbosity of the reference (config.cu:312-344)
 
 int set_error(chkn_sim* s, int code, const std::string& msg)
 {
@@ -68,6 +70,13 @@ std::vector<real> to_real(const double* v, size_t n)
 void dfree(void* p)
 {
     if (p) (void)hipFree(p);
+}
+
+// hipMalloc counted in the handle's set-up total (reported by chkn_device_bytes).
+void dalloc(chkn_sim* s, void** p, size_t bytes)
+{
+    HIPCHK(hipMalloc(p, bytes));
+    s->device_bytes += (long long)bytes;
 }
 
 void free_device(chkn_sim* s)
@@ -337,6 +346,22 @@ void fill_chem(chkn_sim* s, int q)
     for (int qi = 0; qi < N; ++qi) c[qi] = s->inst[qi].reacting ? std::exp(-kAlpha * dt[qi]) : real(-1);
 }
 
+// Size limits of one block.  Kernels address one component of a per-block array (a W slot,
+// U, L, a direction's face geometry, the vertices) with an unsigned 32-bit byte offset from
+// its base, and form cell, face and vertex indices in 32-bit ints; component bases use 64-bit
+// arithmetic.  So every per-direction face count and the vertex count must stay below
+// 2^32 / sizeof(real) (2^29 in FP64), and for viscous jobs the 4-value face records of all
+// three directions as well.  Null when the block fits.
+const char* block_dims_error(long long nic, long long njc, long long nkc, bool viscous)
+{
+    const long long lim = (1LL << 32) / (long long)sizeof(real);
+    const long long nfi = (nic + 1) * njc * nkc, nfj = nic * (njc + 1) * nkc, nfk = nic * njc * (nkc + 1);
+    const long long nv = (nic + 1) * (njc + 1) * (nkc + 1);
+    if (std::max({nfi, nfj, nfk}) >= lim || nv >= lim || (viscous && 4 * (nfi + nfj + nfk) >= lim))
+        return "block too large for 32-bit array offsets";
+    return nullptr;
+}
+
 void check_desc(const chkn_desc* d)
 {
     if (d->nblocks <= 0) throw std::invalid_argument("no blocks");
@@ -345,6 +370,8 @@ void check_desc(const chkn_desc* d)
     if (d->x_order != 1 && d->x_order != 2) throw std::invalid_argument("x_order must be 1 or 2");
     if (d->n_cfl <= 0) throw std::invalid_argument("need at least one cfl value");
     if (d->cfl_count <= 0) throw std::invalid_argument("cfl_count must be positive");
+    if (d->time_scheme != CHKN_TIME_RK3 && d->time_scheme != CHKN_TIME_PC)
+        throw std::invalid_argument("time_scheme must be 0 (rk3) or 1 (predictor_corrector)");
 }
 
 // Set up instances from job descriptions: validation, inflow states, geometry pre-pass,
@@ -356,7 +383,9 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     s->x_order = descs[0].x_order;
     s->flux_calc = descs[0].flux_calc;
     s->viscous = descs[0].viscous != 0;
+    s->time_scheme = descs[0].time_scheme;
     for (int q = 1; q < ninst; ++q) {
+        if (descs[q].time_scheme != s->time_scheme) throw std::invalid_argument("instances must share time_scheme");
         if (descs[q].x_order != s->x_order) throw std::invalid_argument("instances must share x_order");
         if (descs[q].flux_calc != s->flux_calc) throw std::invalid_argument("instances must share flux_calc");
         if ((descs[q].viscous != 0) != s->viscous) throw std::invalid_argument("instances must share viscous");
@@ -432,18 +461,8 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             if (b.nic < 2) throw std::invalid_argument("Too few cells in the i-index direction.");
             if (b.njc < 2) throw std::invalid_argument("Too few cells in the j-index direction.");
             if (b.nkc < 2) throw std::invalid_argument("Too few cells in the k-index direction.");
-            long long n = (long long)b.nic * b.njc * b.nkc;
-            // The kernels address each per-block array (a W slot, U, L, one direction's face
-            // geometry) with 32-bit byte offsets, and the face-list records (4 values per face
-            // of the block) likewise.
-            const long long nf_dir = std::max({(long long)(b.nic + 1) * b.njc * b.nkc,
-                                               (long long)b.nic * (b.njc + 1) * b.nkc,
-                                               (long long)b.nic * b.njc * (b.nkc + 1)});
-            const long long nft = (long long)(b.nic + 1) * b.njc * b.nkc + (long long)b.nic * (b.njc + 1) * b.nkc +
-                                  (long long)b.nic * b.njc * (b.nkc + 1);
-            const long long lim = (1LL << 32) / (long long)sizeof(real);
-            if (nf_dir >= lim || (s->viscous && 4 * nft >= std::min(lim, 1LL << 31)))
-                throw std::invalid_argument("block too large for 32-bit array offsets");
+            if (const char* e = block_dims_error(b.nic, b.njc, b.nkc, s->viscous)) throw std::invalid_argument(e);
+            const long long n = (long long)b.nic * b.njc * b.nkc;
             b.n = (int)n;
             if (b.bi < 0 || b.bi >= d->nib || b.bj < 0 || b.bj >= d->njb || b.bk < 0 || b.bk >= d->nkb)
                 throw std::invalid_argument("block position outside the block array");
@@ -514,7 +533,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             block_geometry(b, vxyz.data(), gi, gj, gk, volinv, len, want_fpos ? &fpos : nullptr);
             voff += nv;
             size_t n = (size_t)b.n, sz6 = sizeof(real) * 6 * n;
-            HIPCHK(hipMalloc(&b.W, sz6 * (b.active ? 3 : 1)));
+            dalloc(s, (void**)&b.W, sz6 * (b.active ? 3 : 1));
             BlockDev& bd = bdev[ib];
             std::memset(&bd, 0, sizeof bd);
             bd.nic = b.nic; bd.njc = b.njc; bd.nkc = b.nkc; bd.active = b.active ? 1 : 0;
@@ -526,14 +545,14 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
             bd.W = b.W;
             bd.wstride = b.active ? (long long)(6 * n) : 0;
             if (b.active) {
-                HIPCHK(hipMalloc(&b.U, sz6));
-                HIPCHK(hipMalloc(&b.L, sz6));
+                dalloc(s, (void**)&b.U, sz6);
+                dalloc(s, (void**)&b.L, sz6);
                 HIPCHK(hipMemset(b.L, 0, sz6));
-                HIPCHK(hipMalloc(&b.gi, sizeof(real) * gi.size()));
-                HIPCHK(hipMalloc(&b.gj, sizeof(real) * gj.size()));
-                HIPCHK(hipMalloc(&b.gk, sizeof(real) * gk.size()));
-                HIPCHK(hipMalloc(&b.volinv, sizeof(real) * n));
-                HIPCHK(hipMalloc(&b.len, sizeof(real) * 3 * n));
+                dalloc(s, (void**)&b.gi, sizeof(real) * gi.size());
+                dalloc(s, (void**)&b.gj, sizeof(real) * gj.size());
+                dalloc(s, (void**)&b.gk, sizeof(real) * gk.size());
+                dalloc(s, (void**)&b.volinv, sizeof(real) * n);
+                dalloc(s, (void**)&b.len, sizeof(real) * 3 * n);
                 HIPCHK(hipMemcpy(b.gi, gi.data(), sizeof(real) * gi.size(), hipMemcpyHostToDevice));
                 HIPCHK(hipMemcpy(b.gj, gj.data(), sizeof(real) * gj.size(), hipMemcpyHostToDevice));
                 HIPCHK(hipMemcpy(b.gk, gk.data(), sizeof(real) * gk.size(), hipMemcpyHostToDevice));
@@ -546,7 +565,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
                 std::vector<real> soa(nv);
                 for (size_t v = 0; v < nvert; ++v)
                     for (int c = 0; c < 3; ++c) soa[c * nvert + v] = vxyz[3 * v + c];
-                HIPCHK(hipMalloc(&b.vtx, sizeof(real) * nv));
+                dalloc(s, (void**)&b.vtx, sizeof(real) * nv);
                 HIPCHK(hipMemcpy(b.vtx, soa.data(), sizeof(real) * nv, hipMemcpyHostToDevice));
                 bd.vtx = b.vtx;
                 bd.nv = (long long)nvert;
@@ -563,15 +582,15 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
         }
     }
     s->bdev = bdev;
-    HIPCHK(hipMalloc(&s->d_blocks, sizeof(BlockDev) * bdev.size()));
+    dalloc(s, (void**)&s->d_blocks, sizeof(BlockDev) * bdev.size());
     upload_blocks(s);
-    HIPCHK(hipMalloc(&s->d_inflow, sizeof(real) * s->inflow_w.size()));
+    dalloc(s, (void**)&s->d_inflow, sizeof(real) * s->inflow_w.size());
     HIPCHK(hipMemcpy(s->d_inflow, s->inflow_w.data(), sizeof(real) * s->inflow_w.size(), hipMemcpyHostToDevice));
     split_inflow_gas(s, inflow_pTYB);
     split_tables(s);
     const int N = ninst;
     s->ctl_bytes = ctl_size(N);
-    HIPCHK(hipMalloc(&s->d_ctl, s->ctl_bytes));
+    dalloc(s, (void**)&s->d_ctl, s->ctl_bytes);
     HIPCHK(hipHostMalloc(&s->h_ctl, s->ctl_bytes, hipHostMallocDefault));
     std::memset(s->h_ctl, 0, s->ctl_bytes);
     int dev = 0, ncu = 0;
@@ -594,6 +613,8 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     s->gstep = 0;
     s->fresh = true;
 }
+
+int stages_per_step(const chkn_sim* s) { return s->time_scheme == CHKN_TIME_PC ? 2 : 3; }
 
 StageArgs stage_args(chkn_sim* s, int q)
 {
@@ -619,14 +640,19 @@ void launch_segment(chkn_sim* s, int K, bool dtred_last, bool fresh)
 {
     for (int q = 0; q < K; ++q) {
         StageArgs a = stage_args(s, q);
-        for (int stage = 1; stage <= 3; ++stage) {
-            a.win = stage - 1;
-            a.wout = stage % 3;
-            bool dtred = (stage == 3) && dtred_last && (q == K - 1);
+        // Stages 1, 2, 3 (TVD-RK3) or 1, 4 (predictor-corrector) in a ring of W slots:
+        // each reads the last stage's output, the final stage writes slot 0.
+        const int nst = stages_per_step(s);
+        for (int pos = 1; pos <= nst; ++pos) {
+            const int stage = (s->time_scheme == CHKN_TIME_PC && pos == 2) ? 4 : pos;
+            const bool last = pos == nst;
+            a.win = pos - 1;
+            a.wout = pos % nst;
+            bool dtred = last && dtred_last && (q == K - 1);
             // With reacting instances the reduction follows the reaction.
-            launch_stage_form(s, a, stage, dtred && !s->any_reacting, fresh && q == 0 && stage == 1,
+            launch_stage_form(s, a, stage, dtred && !s->any_reacting, fresh && q == 0 && pos == 1,
                               s->stream);
-            if (stage == 3 && s->any_reacting)
+            if (last && s->any_reacting)
                 HIPCHK(launch_chemistry(s->d_blocks, s->d_split_cells, s->nsplit_cells, a.ctl, a.wout,
                                         dtred, s->stream));
         }
@@ -636,7 +662,7 @@ void launch_segment(chkn_sim* s, int K, bool dtred_last, bool fresh)
 long long launches_per_segment(chkn_sim* s, int K, bool fresh)
 {
     const long long per_stage = launches_per_stage(s, false);
-    long long n = 3LL * K * per_stage + (s->any_reacting ? K : 0);
+    long long n = (long long)stages_per_step(s) * K * per_stage + (s->any_reacting ? K : 0);
     if (fresh) n += launches_per_stage(s, true) - per_stage;  // first stage in face-list form
     return n;
 }
@@ -713,8 +739,26 @@ void fill_report(chkn_sim* s, int qi, chkn_report* rep, int done)
     rep->bad_stage = 0; rep->bad_count = 0; rep->steps_done = done;
 }
 
-// Advance up to nsteps lockstep steps; when plot_writes, stop segments at snapshot times and
-// write.  Restates the loop of simulate.cu:205-335 for every instance.
+// The march's progress line (simulate.cu:311-326): step, time, dt, CFL, wall clock since the
+// march started (whole milliseconds, as the reference counts them) and its estimates of the
+// wall clock to max_time and to max_step.
+void print_progress(const chkn_sim* s, const Instance& I, int qi)
+{
+    const long long ms = std::chrono::duration_cast<std::chrono::milliseconds>(
+                             std::chrono::steady_clock::now() - s->march_start).count();
+    const double wc = (double)ms / 1000.0;
+    const double per_step = wc / I.step;
+    const double to_ft = (double)(I.max_time - I.t) / (double)I.dt * per_step;
+    const double to_ms = (I.max_step - I.step) * per_step;
+    if (s->inst.size() > 1) std::printf("[instance %d] ", qi);
+    std::printf("Step=%d t=%.3e dt=%.3e cfl=%.3f WC=%.3fs WCtFT=%.3fs WCtMS=%.3fs\n", I.step, (double)I.t,
+                (double)I.dt, (double)schedule_get(I.cfl_t, I.cfl_v, I.t), wc, to_ft, to_ms);
+    std::fflush(stdout);
+}
+
+// Advance up to nsteps lockstep steps; when plot_writes (the march), stop segments at snapshot
+// times and at progress-line steps, write snapshots and print the progress lines.  Restates
+// the loop of simulate.cu:205-335 for every instance.
 int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report* rep)
 {
     if (s->failed) return set_error(s, CHKN_ERR_STATE, "simulation is in a failed state");
@@ -758,13 +802,14 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
             nrun[qi] = q;
             Kuse = std::max(Kuse, q);
         }
-        if (plot_writes)  // a snapshot of any instance ends the segment for all
+        if (plot_writes)  // a snapshot or a progress line of any instance ends the segment for all
             for (int qi = 0; qi < N; ++qi) {
                 Instance& I = s->inst[qi];
                 real tt = I.t;
                 for (int q = 0; q < nrun[qi]; ++q) {
                     tt += I.dt;
-                    if (tt >= I.t_plot) { Kuse = std::min(Kuse, q + 1); break; }
+                    const bool print = I.print_count > 0 && (I.step + q + 1) % I.print_count == 0;
+                    if (tt >= I.t_plot || print) { Kuse = std::min(Kuse, q + 1); break; }
                 }
             }
         K = Kuse;
@@ -783,8 +828,13 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
             const int* bad = ctl_bad(s->h_ctl, q, N);
             for (int st = 0; st < 3; ++st)
                 if (bad[st] > 0) {
+                    // Steps before the failing one completed (the rest of the segment's
+                    // launches ran on the failed state; the handle is left failed).
                     s->failed = true;
-                    for (int qi = 0; qi < N; ++qi) s->inst[qi].step += std::min(q, nrun[qi]);
+                    for (int qi = 0; qi < N; ++qi) {
+                        Instance& I = s->inst[qi];
+                        for (int r = 0; r < std::min(q, nrun[qi]); ++r) { I.t += I.dt; I.step += 1; }
+                    }
                     s->gstep += q;
                     if (rep) { rep->bad_stage = st + 1; rep->bad_count = bad[st]; }
                     char msg[128];
@@ -804,8 +854,9 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
         if (s->dt_log)
             for (int q = 0; q < K; ++q)
                 for (int qi = 0; qi < N; ++qi) s->dt_log->push_back(q < nrun[qi] ? (double)s->inst[qi].dt : 0.0);
-        for (int qi = 0; qi < N; ++qi) {  // simulate.cu:306-308
+        for (int qi = 0; qi < N; ++qi) {  // simulate.cu:306-308; the device ran min(nrun, K)
             Instance& I = s->inst[qi];
+            nrun[qi] = std::min(nrun[qi], K);
             for (int q = 0; q < nrun[qi]; ++q) {
                 I.t += I.dt;
                 I.step += 1;
@@ -816,8 +867,9 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
         s->fresh = false;
         done += K;
         if (plot_writes)
-            for (int qi = 0; qi < N; ++qi) {  // simulate.cu:329-334
+            for (int qi = 0; qi < N; ++qi) {  // simulate.cu:311-334
                 Instance& I = s->inst[qi];
+                if (nrun[qi] > 0 && I.print_count > 0 && I.step % I.print_count == 0) print_progress(s, I, qi);
                 if (nrun[qi] > 0 && I.t >= I.t_plot) {
                     int rc = write_instance_flow(s, qi, I.next_plot_indx, binary);
                     if (rc != CHKN_OK) return rc;
@@ -906,6 +958,42 @@ int chkn_create_batch(const chkn_desc* descs, int ninst, chkn_sim** out)
     return create_from(descs, ninst, out);
 }
 
+// Console echo of the job configuration at verbosity > 0 (after config.cu:367-523).
+static void echo_config(const JobConfig& c)
+{
+    auto list = [](const char* name, const std::vector<long long>& v) {
+        std::printf("  %s: [", name);
+        for (long long x : v) std::printf("%lld,", x);
+        std::printf("]\n");
+    };
+    std::printf("  title: %s\n", c.title.c_str());
+    std::printf("  nib: %d\n  njb: %d\n  nkb: %d\n", c.nib, c.njb, c.nkb);
+    std::vector<long long> nics(c.nib), njcs(c.njb), nkcs(c.nkb);
+    for (auto& b : c.blocks) { nics[b.i] = b.nic; njcs[b.j] = b.njc; nkcs[b.k] = b.nkc; }
+    list("nics", nics); list("njcs", njcs); list("nkcs", nkcs);
+    std::printf("  x_order=%d\n  t_order=%d\n  time_scheme=%s\n  flux_calc=%s\n", c.x_order, c.t_order,
+                c.time_scheme ? "predictor_corrector" : "rk3", c.flux_calc ? "sbp_asf" : "ausmdv");
+    std::printf("  dt_init=%g\n  max_time=%g\n  max_step=%d\n  cfl_count=%d\n  print_count=%d\n", c.dt_init,
+                c.max_time, c.max_step, c.cfl_count, c.print_count);
+    std::printf("  viscous=%d\n  reacting=%d\n", c.viscous ? 1 : 0, c.reacting ? 1 : 0);
+    std::printf("Finished reading JSON file.\n");
+}
+
+int chkn_check_block_dims(int nic, int njc, int nkc, int viscous)
+{
+    if (nic < 2 || njc < 2 || nkc < 2) return set_error(nullptr, CHKN_ERR_ARG, "Too few cells in a block direction.");
+    if (const char* e = block_dims_error(nic, njc, nkc, viscous != 0)) return set_error(nullptr, CHKN_ERR_ARG, e);
+    return CHKN_OK;
+}
+
+int chkn_set_verbosity(int level)
+{
+    g_verbosity = level;
+    return CHKN_OK;
+}
+
+long long chkn_device_bytes(chkn_sim* sim) { return sim ? sim->device_bytes : -1; }
+
 int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
 {
     if (!job_dir || !out) return set_error(nullptr, CHKN_ERR_ARG, "null argument");
@@ -918,7 +1006,9 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
     try {
         if (!std::filesystem::is_directory(job))
             return set_error(nullptr, CHKN_ERR_IO, "Job directory is not present in current directory.");
+        if (g_verbosity > 0) std::printf("Reading JSON config file.\n");
         c = read_config_json(job + "/config.json");
+        if (g_verbosity > 0) echo_config(c);
         for (auto& b : c.blocks) {
             std::vector<double> v = read_grid(block_file(job, "grid", 0, b.i, b.j, b.k, binary != 0), binary != 0, b.nic, b.njc, b.nkc);
             verts.insert(verts.end(), v.begin(), v.end());
@@ -963,6 +1053,7 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out)
     d.dt_plot_values = c.dt_plot_values.data();
     d.t0 = t0; d.start_tindx = next_plot - 1;
     d.reacting = c.reacting; d.viscous = c.viscous;
+    d.time_scheme = c.time_scheme;
     d.bc_twall = twall.data();
     d.bc_fun = bcfun.data();
     d.gas_model = c.gas_model.data();
@@ -985,7 +1076,11 @@ int chkn_march(chkn_sim* sim, int binary, chkn_report* rep)
     if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
     for (auto& I : sim->inst)
         if (I.job.empty()) return set_error(sim, CHKN_ERR_STATE, "chkn_march needs a job directory");
-    return guarded(sim, [&] { return do_steps(sim, 0x7fffffff, true, binary != 0, rep); });
+    sim->march_start = std::chrono::steady_clock::now();  // simulate.cu:189
+    if (g_verbosity > 0) std::printf("march_in_time start\n");
+    const int rc = guarded(sim, [&] { return do_steps(sim, 0x7fffffff, true, binary != 0, rep); });
+    if (g_verbosity > 0) std::printf("march_in_time end\n");
+    return rc;
 }
 
 int chkn_finalize(chkn_sim* sim, int binary)
@@ -1045,6 +1140,17 @@ int chkn_set_iovars(chkn_sim* sim, int blk, const double* iovars)
     if (!sim || !iovars || blk < 0 || blk >= (int)sim->blocks.size()) return set_error(sim, CHKN_ERR_ARG, "bad argument");
     return guarded(sim, [&] {
         HostBlock& b = sim->blocks[blk];
+        // The handle is marked fresh below: every block's first stage then reads its held
+        // IO variables' gas states.  Blocks that have stepped since they were set take their
+        // current state as held state (gas states from the equation of state of W, which the
+        // stage would compute anyway).
+        if (!sim->fresh)
+            for (size_t ib = 0; ib < sim->blocks.size(); ++ib) {
+                HostBlock& o = sim->blocks[ib];
+                if ((int)ib == blk || !o.active) continue;
+                block_iovars(sim, (int)ib, o.io0.data());
+                split_block_gas(sim, o, sim->bdev[ib]);
+            }
         const std::vector<real> v = to_real(iovars + 4 * (size_t)b.n, 9 * (size_t)b.n);
         std::copy(v.begin(), v.end(), b.io0.begin() + 4 * (size_t)b.n);
         upload_state(sim, b, b.io0.data());

@@ -84,7 +84,7 @@ void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vecto
     for (int f = 0; f < 6; ++f) bd.twall[f] = b.twall[f];
     if (!b.active || !need_gas(s)) return;
     const size_t nft = face_count(b);
-    HIPCHK(hipMalloc(&b.F, sizeof(real) * 6 * nft));
+    dalloc(s, (void**)&b.F, sizeof(real) * 6 * nft);
     bd.F = b.F;
     if (!s->viscous) return;
     // Centres as (x, y, z, 0) records: one cloud point is two vector loads.
@@ -96,9 +96,9 @@ void split_block_setup(chkn_sim* s, HostBlock& b, BlockDev& bd, const std::vecto
     };
     const size_t n = (size_t)b.n;
     const std::vector<real> fr = records(fpos, nft), cr = records(b.pos, n);
-    HIPCHK(hipMalloc(&b.FS, sizeof(real) * 4 * nft));
-    HIPCHK(hipMalloc(&b.fpos, sizeof(real) * 4 * nft));
-    HIPCHK(hipMalloc(&b.cpos, sizeof(real) * 4 * n));
+    dalloc(s, (void**)&b.FS, sizeof(real) * 4 * nft);
+    dalloc(s, (void**)&b.fpos, sizeof(real) * 4 * nft);
+    dalloc(s, (void**)&b.cpos, sizeof(real) * 4 * n);
     HIPCHK(hipMemcpy(b.fpos, fr.data(), sizeof(real) * 4 * nft, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b.cpos, cr.data(), sizeof(real) * 4 * n, hipMemcpyHostToDevice));
     bd.FS = b.FS;
@@ -202,7 +202,7 @@ void split_inflow_gas(chkn_sim* s, const std::vector<real>& pTYB)
     }
     dfree(s->d_inflow_gas);
     s->d_inflow_gas = nullptr;
-    HIPCHK(hipMalloc(&s->d_inflow_gas, sizeof(real) * s->inflow_gas.size()));
+    dalloc(s, (void**)&s->d_inflow_gas, sizeof(real) * s->inflow_gas.size());
     HIPCHK(hipMemcpy(s->d_inflow_gas, s->inflow_gas.data(), sizeof(real) * s->inflow_gas.size(),
                      hipMemcpyHostToDevice));
 }

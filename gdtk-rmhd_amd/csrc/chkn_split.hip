@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
     const int dir = ch.dir;
     const FacePos p = face_pos(B, dir, f);
     const real* g = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
-    const int nf = (int)(dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk));
+    const long long nf = dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk);
     const Frame fr = ld_frame(g, nf, f);
     real rhoL, rhoR, eL, eR, YBL, YBR, vxL, vxR, vyL, vyR, vzL, vzR, pL, pR, aL, aR;
     real F[6];
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int 
     const real tau_yz = mu * (gyz + gzy);
     const real qx = kcond * gTx, qy = kcond * gTy, qz = kcond * gTz;
     const real* gf = dir == 0 ? B->gi : (dir == 1 ? B->gj : B->gk);
-    const int nf = (int)(dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk));
+    const long long nf = dir == 0 ? B->nfi : (dir == 1 ? B->nfj : B->nfk);
     const unsigned of = boff(f);
     const real nx = gld(gf + nf, of), ny = gld(gf + 2 * nf, of), nz = gld(gf + 3 * nf, of);
     CHKN_GMEM real* __restrict__ F = gmem(B->F);
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A)
     if (__ballot(bad != 0)) {
         int cnt = bad;
         for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&A.ctl.bad[STAGE - 1], cnt);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&A.ctl.bad[stage_position(STAGE) - 1], cnt);
     }
     if (DTRED) {
         const unsigned long long wm = wave_min_u64(sigmin);
@@ -599,8 +599,10 @@ hipError_t launch_split_update(const SplitArgs& a, int ncell_chunks, int stage, 
     const dim3 grid(ncell_chunks), blk(kChunk);
     if (stage == 1) hipLaunchKernelGGL((split_update_kernel<1, false>), grid, blk, 0, s, a);
     else if (stage == 2) hipLaunchKernelGGL((split_update_kernel<2, false>), grid, blk, 0, s, a);
-    else if (dt_reduce) hipLaunchKernelGGL((split_update_kernel<3, true>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((split_update_kernel<3, false>), grid, blk, 0, s, a);
+    else if (stage == 3 && dt_reduce) hipLaunchKernelGGL((split_update_kernel<3, true>), grid, blk, 0, s, a);
+    else if (stage == 3) hipLaunchKernelGGL((split_update_kernel<3, false>), grid, blk, 0, s, a);
+    else if (dt_reduce) hipLaunchKernelGGL((split_update_kernel<4, true>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((split_update_kernel<4, false>), grid, blk, 0, s, a);
     return hipGetLastError();
 }
 

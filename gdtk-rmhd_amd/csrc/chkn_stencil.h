@@ -211,9 +211,15 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
     return v;
 }
 
-// TVD-RK3 stage of cell cidx from its flux divergence d (block.cu:1090-1160): stage 1 keeps
-// d in L, stage 2 replaces L by L/6 + d/6, stage 3 writes the new U.  Decodes the stage
-// result (flow.cu:245-268) into W slot wout and returns the decode's bad-cell flag.
+// Stages of the update are numbered 1, 2, 3 for TVD-RK3 and 1, 4 for the two-stage
+// predictor-corrector; the bad-cell counter of a stage is its position in the step.
+__host__ __device__ constexpr int stage_position(int stage) { return stage == 4 ? 2 : stage; }
+
+// RK stage of cell cidx from its flux divergence d.  TVD-RK3 (block.cu:1090-1160): stage 1
+// keeps d in L, stage 2 replaces L by L/6 + d/6, stage 3 writes the new U.  Predictor-
+// corrector (src/puffin/cell.d:294-303): stage 1 as RK3's, stage 4 writes
+// U + 0.5 dt (L + d).  Decodes the stage result (flow.cu:245-268) into W slot wout and
+// returns the decode's bad-cell flag.
 template <int STAGE>
 __device__ __forceinline__ int stage_update(const BlockDev* B, long long n, int cidx, real dt,
                                             const real d[6], int wout, Prim& w)
@@ -235,10 +241,16 @@ __device__ __forceinline__ int stage_update(const BlockDev* B, long long n, int 
             Un[q] = gld_nt(U + q * n, o) + real(0.25) * dt * (l0 + d[q]);
             gst_nt(Lr + q * n, o, kOneSixth * l0 + kOneSixth * d[q]);
         }
-    } else {
+    } else if (STAGE == 3) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             Un[q] = gld_nt(U + q * n, o) + dt * (gld_nt(Lr + q * n, o) + kFourSixth * d[q]);
+            gst_nt(U + q * n, o, Un[q]);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            Un[q] = gld_nt(U + q * n, o) + real(0.5) * dt * (gld_nt(Lr + q * n, o) + d[q]);
             gst_nt(U + q * n, o, Un[q]);
         }
     }

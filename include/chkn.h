@@ -56,7 +56,7 @@ typedef struct {
     const double* vertices;  /* per block (nic+1)(njc+1)(nkc+1) xyz triples, i fastest */
     const double* iovars;    /* per block 13*ncells, flow-file order */
     int x_order;             /* 1 or 2 */
-    int flux_calc;           /* 0 = ausmdv (only ausmdv is built) */
+    int flux_calc;           /* 0 = ausmdv, 1 = sbp_asf (face.cu:268-342, face-list stage) */
     int t_order;             /* parsed and ignored, as config.cu:491 */
     int cfl_count, max_step, print_count;
     double max_time, dt_init;
@@ -84,7 +84,16 @@ typedef struct {
     const double* gas_model;   /* gamma, R, q, alpha, Ti, Cv of the job (config.json
                                   "gas_model"); must be the library's built model
                                   (config.cu:369-386); NULL: take the built model */
+    int time_scheme;           /* CHKN_TIME_RK3 (the reference's step, simulate.cu:223-287)
+                                  or CHKN_TIME_PC: two-stage predictor-corrector, U1 = U0 +
+                                  dt L0 then U = U0 + 0.5 dt (L0 + L1) (the sibling solver
+                                  Lorikeet, src/puffin/cell.d:294-303; chicken parses
+                                  t_order and ignores it, config.cu:491).  config.json key
+                                  "time_scheme": "rk3" (default) or "predictor_corrector" */
 } chkn_desc;
+
+#define CHKN_TIME_RK3 0
+#define CHKN_TIME_PC 1
 
 /* Progress of the march, filled by chkn_step and chkn_march. */
 typedef struct {
@@ -138,7 +147,9 @@ int chkn_load_job(const char* job_dir, int tindx, int binary, chkn_sim** out);
 int chkn_step(chkn_sim* sim, int nsteps, chkn_report* rep);
 
 /* Replaces march_in_time_using_cpu_only/_gpu (simulate.cu:182-342): march to max_step or
- * max_time, writing snapshots on the dt_plot schedule.  Needs a job directory. */
+ * max_time, writing snapshots on the dt_plot schedule and printing the progress line
+ * "Step=... t=... dt=... cfl=... WC=...s WCtFT=...s WCtMS=...s" every print_count steps
+ * (simulate.cu:311-326).  Needs a job directory. */
 int chkn_march(chkn_sim* sim, int binary, chkn_report* rep);
 
 /* Replaces finalize_simulation (simulate.cu:768-792): final snapshot if steps are
@@ -210,6 +221,10 @@ int chkn_boundary_face_areas(chkn_sim* sim, int blk, int side, double* out);
  * given, dt_in[step*ninst + inst]; those used are written to dt_out when it is not NULL. */
 int chkn_grad_forward(chkn_sim* sim, int nsteps, int checkpoint_every, const double* dt_in,
                       double* dt_out);
+/* Number of successful chkn_grad_forward calls on the handle: chkn_grad_backward reverses the
+ * latest one only, so a caller that holds several unrolls can tell whether its tape is still
+ * the one on the handle. */
+long long chkn_grad_generation(chkn_sim* sim);
 /* Reverse the last chkn_grad_forward with its time steps held fixed: lambda_dev = dLoss/dU
  * of the final state (flat, device); grad_u0_dev receives dLoss/dU of the state the forward
  * started from (flat, device); grad_inflow (host, 6 per inflow state, may be NULL)
@@ -221,6 +236,23 @@ int chkn_grad_backward(chkn_sim* sim, const double* lambda_dev, double* grad_u0_
  * in: per point 4 x 6 stencil primitives (L1, L0, R0, R1; rho, e, YB, vx, vy, vz), the 9
  * frame components (n, t1, t2) and 6 flux adjoints; out: 24 stencil adjoints per point. */
 int chkn_flux_vjp_points(int npts, int x_order, const double* in, double* out);
+
+/* Whether one block of nic x njc x nkc cells fits the kernels' 32-bit index and offset
+ * arithmetic (no counterpart in the reference, whose CPU arrays are std::vector): every
+ * per-direction face count and the vertex count below 2^32 / sizeof(number) (2^29 in FP64),
+ * and for viscous jobs 4 x (all faces) as well.  CHKN_OK, or CHKN_ERR_ARG with the reason in
+ * chkn_last_error(NULL).  chkn_create applies the same check to every block. */
+int chkn_check_block_dims(int nic, int njc, int nkc, int viscous);
+
+/* Console verbosity of chkn_load_job and chkn_march (Config::verbosity, config.cu:312-344;
+ * set from chkn-run --verbosity): 0 prints only the march's progress lines every print_count
+ * steps (simulate.cu:311-326); > 0 also echoes the configuration and the march's start and
+ * end. */
+int chkn_set_verbosity(int level);
+
+/* Device memory allocated by the handle's set-up, in bytes (the counterpart of the
+ * reference's "Bytes allocated on CPU", simulate.cu:89-90). */
+long long chkn_device_bytes(chkn_sim* sim);
 
 /* Error text of the last failing call on this handle (or of the last failed create/load
  * when sim is NULL). */

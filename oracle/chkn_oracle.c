@@ -113,7 +113,7 @@ struct orc_sim {
     Blk* b;
     int nfs;
     FS* inflow;
-    int x_order, flux_calc, cfl_count, max_step, viscous, reacting;
+    int x_order, flux_calc, cfl_count, max_step, viscous, reacting, time_scheme;
     real max_time, dt_init;
     int n_cfl;
     real* cfl_t;
@@ -969,7 +969,8 @@ static void block_fluxes(orc_sim* s, Blk* b)
     }
 }
 
-/* block.cu:1090-1160 */
+/* block.cu:1090-1160; stage 4 is the corrector of the two-stage predictor-corrector
+ * (src/puffin/cell.d:294-303: U1 = U0 + 0.5*dt*(dUdt0 + dUdt1)); its predictor is stage 1. */
 static int block_update(Blk* b, int stage, real dt)
 {
     int bad = 0, n = b->ncells;
@@ -986,6 +987,9 @@ static int block_update(Blk* b, int stage, real dt)
         } else if (stage == 2) {
             eval_dudt(b, c, d1);
             for (int j = 0; j < 6; ++j) U1[j] = U0[j] + one_quarter * dt * (d0[j] + d1[j]);
+        } else if (stage == 4) {
+            eval_dudt(b, c, d1);
+            for (int j = 0; j < 6; ++j) U1[j] = U0[j] + half * dt * (d0[j] + d1[j]);
         } else {
             eval_dudt(b, c, d2);
             for (int j = 0; j < 6; ++j)
@@ -1151,6 +1155,7 @@ orc_sim* orc_create(int nblocks, int nib, int njb, int nkb, const int* blk_ids,
     s->x_order = p->x_order; s->flux_calc = p->flux_calc; s->cfl_count = p->cfl_count;
     s->viscous = p->viscous;
     s->reacting = p->reacting;
+    s->time_scheme = p->time_scheme;
     /* Configuration values are held in the number type (config.cu:494-504). */
     s->max_step = p->max_step; s->max_time = (real)p->max_time; s->dt_init = (real)p->dt_init;
     s->n_cfl = p->n_cfl;
@@ -1269,7 +1274,9 @@ int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
             }
             s->dt = fmin((real)1.5 * s->dt, smallest);
         }
-        for (int stage = 1; stage <= 3; ++stage) { /* :223-287 */
+        const int nstages = s->time_scheme == 1 ? 2 : 3;
+        for (int q = 1; q <= nstages; ++q) { /* :223-287 */
+            const int stage = (s->time_scheme == 1 && q == 2) ? 4 : q;
             apply_all_bcs(s);
             int bad = 0;
 #pragma omp parallel for schedule(dynamic) reduction(+ : bad)
@@ -1285,7 +1292,7 @@ int orc_march(orc_sim* s, int nsteps, double* dt_log, int* bad_count)
             }
             if (bad > 0) {
                 if (bad_count) *bad_count = bad;
-                return 100 * stage + (bad > 99 ? 99 : bad);
+                return 100 * q + (bad > 99 ? 99 : bad);
             }
         }
         for (int ib = 0; ib < s->nblocks; ++ib) { /* :288-294, copy level 1 -> 0 */

@@ -46,6 +46,9 @@ typedef struct {
     int n_twall_faces;    /* TWall per face of the NaN sides, block and side order, each side */
     const double* twall_faces; /* in the TWall-file order (block.cu:566-611) */
     int reacting;         /* the A->B reaction after every step (simulate.cu:296-303) */
+    int time_scheme;      /* 0 = TVD-RK3 (simulate.cu:223-287); 1 = two-stage predictor-
+                             corrector (Lorikeet, src/puffin/cell.d:294-303): not part of
+                             chicken, restated from the sibling code */
 } orc_params;
 
 typedef struct orc_sim orc_sim;

@@ -1,6 +1,10 @@
 import os
 import sys
 
+# The oracle parallelises over blocks with OpenMP; idle threads should sleep rather than spin,
+# or concurrent test workers (pytest -n) slow each other down many times over.
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(ROOT, "gdtk-rmhd_amd"), os.path.join(ROOT, "tests")):
     if p not in sys.path:

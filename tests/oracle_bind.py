@@ -25,7 +25,7 @@ class OrcParams(ctypes.Structure):
                 ("dt_init", ctypes.c_double), ("n_cfl", ctypes.c_int), ("cfl_times", _D),
                 ("cfl_values", _D), ("t0", ctypes.c_double), ("viscous", ctypes.c_int),
                 ("bc_twall", _D), ("bc_fun", _I), ("bl_profile", _D), ("n_twall_faces", ctypes.c_int),
-                ("twall_faces", _D), ("reacting", ctypes.c_int)]
+                ("twall_faces", _D), ("reacting", ctypes.c_int), ("time_scheme", ctypes.c_int)]
 
 
 
@@ -83,7 +83,7 @@ class Oracle:
                         _p(f["cfl_values"], _D), job.t0, int(job.viscous), _p(f["bc_twall"], _D),
                         _p(f["bc_fun"], _I), _p(f["bl_profile"], _D) if len(f["bl_profile"]) else None,
                         len(f["twall_faces"]), _p(f["twall_faces"], _D) if len(f["twall_faces"]) else None,
-                        int(job.reacting))
+                        int(job.reacting), int(job.time_scheme))
         self._prm = prm
         h = lib.orc_create(len(job.blocks), job.nib, job.njb, job.nkb, _p(f["blk_ids"], _I),
                            _p(f["blk_ijk"], _I), _p(f["blk_dims"], _I), _p(f["active"], _I),

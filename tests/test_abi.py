@@ -99,6 +99,24 @@ def test_tuner_candidates_are_valid_launch_options():
     assert len(tune.candidates(x_order=1)) == base  # the vertex-frame form is second-order only
 
 
+def test_block_size_limit_at_its_boundary():
+    # FP64: the vertex count (nic+1)(njc+1)(nkc+1), which bounds every per-direction face
+    # count, must stay below 2^29 (32-bit byte offsets of 8-byte numbers); viscous jobs also
+    # need 4 x (all faces) below 2^29.
+    so = lib.load()
+    assert so.chkn_check_block_dims(1023, 1023, 510, 0) == 0   # 2^20 * 511 vertices
+    assert so.chkn_check_block_dims(1023, 1023, 511, 0) == -1  # exactly 2^29 vertices
+    assert so.chkn_last_error(None) == b"block too large for 32-bit array offsets"
+    assert so.chkn_check_block_dims(2047, 511, 511, 0) == -1   # 2^11 * 2^9 * 2^9
+    assert so.chkn_check_block_dims(2046, 511, 511, 0) == 0
+    nft = lambda a, b, c: (a + 1) * b * c + a * (b + 1) * c + a * b * (c + 1)
+    assert 4 * nft(256, 256, 600) < 1 << 29 <= 4 * nft(256, 256, 700)
+    assert so.chkn_check_block_dims(256, 256, 600, 1) == 0
+    assert so.chkn_check_block_dims(256, 256, 700, 0) == 0
+    assert so.chkn_check_block_dims(256, 256, 700, 1) == -1
+    assert so.chkn_check_block_dims(1, 4, 4, 0) == -1          # fewer than 2 cells
+
+
 def test_blocks_beyond_32_bit_offsets_are_rejected():
     # The kernels address each per-block array with 32-bit byte offsets: a block with 2^29 or
     # more faces in one direction (FP64) is refused at set-up, before any device call.

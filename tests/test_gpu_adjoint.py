@@ -305,3 +305,26 @@ def test_ensemble_gradient_at_full_size_matches_finite_differences():
                 err.append(abs(fd - g) / abs(g))
                 print("job %d dL/d%s step %.0e fd %.10e adjoint %.10e rel %.1e" % (q, "MpT"[k], rel, fd, g, err[-1]))
             assert err[0] <= 1e-3 and min(err) <= 2e-4, (q, k, err)
+
+
+def test_backward_of_a_replaced_unroll_is_refused():
+    # The handle keeps the checkpoints and dts of its latest unroll only.  Chaining two unrolls
+    # on one Sim and differentiating through both would reverse the first with the second's
+    # tape: the backward must say so instead of returning a wrong gradient.
+    job = _small_ramp(max_step=1000)
+    sim = Sim(job)
+    u0 = adjoint.initial_state(sim).requires_grad_(True)
+    inflow = torch.as_tensor(np.asarray([sim.get_inflow_state(0)]), dtype=torch.float64,
+                             device="cuda").requires_grad_(True)
+    g0 = sim.grad_generation
+    u1 = adjoint.unroll(sim, u0, inflow, 3, 2)
+    assert sim.grad_generation == g0 + 1
+    u2 = adjoint.unroll(sim, u1, inflow, 3, 2)
+    loss = adjoint.WallPressure(sim, [job])(u2)
+    with pytest.raises(RuntimeError, match="replaced by a later unroll"):
+        loss.backward()
+    # One unroll per Sim is fine.
+    u1 = adjoint.unroll(sim, u0, inflow, 3, 2)
+    adjoint.WallPressure(sim, [job])(u1).backward()
+    assert torch.isfinite(u0.grad).all()
+    sim.close()

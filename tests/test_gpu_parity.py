@@ -118,6 +118,35 @@ def test_bad_cell_reports_stage_and_count():
     sim.close()
 
 
+def test_set_iovars_on_one_block_keeps_the_others_current():
+    # Replacing one block's state after steps: the other blocks still report their stepped
+    # state (not the creation-time one).  Re-setting them too re-encodes every block from its
+    # primitives, as a restart does (simulate.cu:196-203), and the next steps then match an
+    # oracle restarted from the same states -- at x_order 1, whose first stage reads the held
+    # gas states.
+    job = _first_order()
+    sim = Sim(job)
+    sim.step(6)
+    before = [sim.iovars(b) for b in range(len(job.blocks))]
+    new0 = before[0].copy()
+    new0[4] *= 1.001  # p of block 0
+    new0[6] *= 1.001  # rho: same T
+    sim.set_iovars(0, new0)
+    for b in range(1, len(job.blocks)):
+        assert np.array_equal(sim.iovars(b), before[b]), b
+    for b in range(1, len(job.blocks)):
+        sim.set_iovars(b, before[b])
+    sim.step(4)  # steps 6-9: the same dt as a restart's first steps (no CFL update yet)
+    for b, jb in enumerate(job.blocks):
+        jb.iovars = new0 if b == 0 else before[b]
+    orc = ob.Oracle(job)
+    rc, _, _ = orc.march(4)
+    assert rc == 0
+    for b in range(len(job.blocks)):
+        assert np.array_equal(sim.iovars(b), orc.iovars(b)), b
+    sim.close()
+
+
 def test_restart_continues_after_chkn_step_calls():
     # Several chkn_step calls (splitting CFL segments) equal one long call.
     job = _cfl_ramp()
