@@ -72,6 +72,7 @@ EXPORTS = {
     "chkn_grad_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
     "chkn_grad_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _D]),
     "chkn_flux_vjp_points": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D]),
+    "chkn_flux_vjp_points_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D]),
     "chkn_load_job": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_void_p)]),
     "chkn_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ChknReport)]),
@@ -145,15 +146,18 @@ def _check_dev(t, n):
         raise ValueError("expected a contiguous float64 device tensor of %d elements" % n)
 
 
-def flux_vjp_points(prims, frames, fbar, x_order=2):
+def flux_vjp_points(prims, frames, fbar, x_order=2, noalloc=False):
     """Face-flux vector-Jacobian products on the device (test hook of the adjoint kernels).
-    prims (n, 4, 6), frames (n, 9), fbar (n, 6) -> (n, 4, 6)."""
+    prims (n, 4, 6), frames (n, 9), fbar (n, 6) -> (n, 4, 6).  noalloc: the entry point that
+    stages through the library's fixed device array (at most 4096 points) instead of
+    allocating."""
     lib = load()
     n = len(prims)
     inp = np.ascontiguousarray(np.concatenate([np.reshape(prims, (n, 24)), np.reshape(frames, (n, 9)),
                                                np.reshape(fbar, (n, 6))], axis=1), dtype=np.float64)
     out = np.zeros((n, 24))
-    rc = lib.chkn_flux_vjp_points(n, x_order, _ptr(inp, _D), _ptr(out, _D))
+    fn = lib.chkn_flux_vjp_points_noalloc if noalloc else lib.chkn_flux_vjp_points
+    rc = fn(n, x_order, _ptr(inp, _D), _ptr(out, _D))
     if rc != 0:
         raise ChknError(rc, lib.chkn_last_error(None).decode())
     return out.reshape(n, 4, 6)

@@ -21,6 +21,9 @@
 //   stage 3 (W2): A0 += U2bar; D = P + dt/4 U2bar; P = D
 //   stage 2 (W1): A0 += U1bar; D = P + dt U1bar
 //   stage 1 (W0): A0 += U0bar        -> A0 = adjoint of U^n; seed of the next reversed step
+#include <mutex>
+#include <vector>
+
 #include "chkn_adjoint.h"
 #include "chkn_internal.h"
 #include "chkn_physics.h"
@@ -29,6 +32,12 @@
 namespace chkn {
 
 namespace {
+
+// Staging of the allocation-free point evaluation (flux_vjp_points_noalloc): the inputs
+// (39 values per point) then the outputs (24 per point) of up to kNoAllocPoints points, in
+// a file-scope device array (2 MB) that each GPU the library runs on holds until the process
+// ends.  Every call on a GPU shares that GPU's copy: calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[(39 + 24) * kNoAllocPoints];
 
 __device__ __forceinline__ int line_len(const BlockDev* B, int d)
 {
@@ -372,6 +381,45 @@ hipError_t launch_flux_vjp_points(const double* in, double* out, int n, int x_or
     hipLaunchKernelGGL(flux_vjp_points_kernel, dim3((n + 63) / 64), dim3(64), 0, s, in, out, n,
                        x_order);
     return hipGetLastError();
+}
+
+// Device address of the current device's copy of `scratch`, looked up once per device with
+// that device current.  In host code the array's name is only a stand-in for the symbol:
+// kernels and copies take the address returned here.
+static hipError_t scratch_address(double** out)
+{
+    static std::mutex mu;
+    static std::vector<double*> addr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = static_cast<double*>(p);
+    }
+    *out = addr[dev];
+    return hipSuccess;
+}
+
+hipError_t flux_vjp_points_noalloc(const double* in, double* out, int n, int x_order)
+{
+    if (n <= 0 || n > kNoAllocPoints) return hipErrorInvalidValue;
+    double* base = nullptr;
+    hipError_t e = scratch_address(&base);
+    if (e != hipSuccess) return e;
+    double* din = base;
+    double* dout = base + 39 * (size_t)kNoAllocPoints;
+    e = hipMemcpy(din, in, sizeof(double) * 39 * (size_t)n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    e = launch_flux_vjp_points(din, dout, n, x_order, nullptr);
+    if (e != hipSuccess) return e;
+    // The null stream orders the copy after the kernel; every output read back was written by
+    // this call (the kernel writes all 24 values of each of its n points).
+    return hipMemcpy(out, dout, sizeof(double) * 24 * (size_t)n, hipMemcpyDeviceToHost);
 }
 
 }  // namespace chkn

@@ -632,4 +632,15 @@ int chkn_flux_vjp_points(int npts, int x_order, const double* in, double* out)
     return CHKN_OK;
 }
 
+int chkn_flux_vjp_points_noalloc(int npts, int x_order, const double* in, double* out)
+{
+    if (npts <= 0 || !in || !out) return set_error(nullptr, CHKN_ERR_ARG, "bad argument");
+    if (npts > kNoAllocPoints)
+        return set_error(nullptr, CHKN_ERR_ARG,
+                         "npts exceeds the allocation-free limit of " + std::to_string(kNoAllocPoints) + " points");
+    const hipError_t e = flux_vjp_points_noalloc(in, out, npts, x_order);
+    if (e != hipSuccess) return set_error(nullptr, CHKN_ERR_HIP, std::string("HIP error ") + hipGetErrorString(e));
+    return CHKN_OK;
+}
+
 }  // extern "C"

@@ -47,5 +47,6 @@ int chkn_set_inflow_state(chkn_sim* sim, int, const double*) { return off(sim); 
 int chkn_get_inflow_state(chkn_sim* sim, int, double*) { return off(sim); }
 int chkn_boundary_face_areas(chkn_sim* sim, int, int, double*) { return off(sim); }
 int chkn_flux_vjp_points(int, int, const double*, double*) { return off(nullptr); }
+int chkn_flux_vjp_points_noalloc(int, int, const double*, double*) { return off(nullptr); }
 
 }  // extern "C"

@@ -146,6 +146,10 @@ hipError_t launch_decode_all(const BlockDev* blocks, const CellChunk* cells, int
 // Point evaluation of face_flux_vjp (test hook): in = per point 24 prims, 9 frame, 6 Fbar;
 // out = 24 stencil adjoints.
 hipError_t launch_flux_vjp_points(const double* in, double* out, int n, int x_order, hipStream_t s);
+// The same with host in/out staged through a file-scope device array instead of hipMalloc'd
+// buffers: no allocation, at most kNoAllocPoints points (hipErrorInvalidValue beyond).
+constexpr int kNoAllocPoints = 4096;
+hipError_t flux_vjp_points_noalloc(const double* in, double* out, int n, int x_order);
 
 // Face-list form of one stage: every face of every active block in one pass (convective
 // flux, and for viscous jobs the face state), then the viscous pass (gradients and viscous

@@ -236,6 +236,13 @@ int chkn_grad_backward(chkn_sim* sim, const double* lambda_dev, double* grad_u0_
  * in: per point 4 x 6 stencil primitives (L1, L0, R0, R1; rho, e, YB, vx, vy, vz), the 9
  * frame components (n, t1, t2) and 6 flux adjoints; out: 24 stencil adjoints per point. */
 int chkn_flux_vjp_points(int npts, int x_order, const double* in, double* out);
+/* The same evaluation without any device allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  in and out are host arrays as above; they are staged
+ * through a file-scope device array of 4096 points (2 MB, held by each GPU the library runs on
+ * until the process ends), so npts above 4096 is refused (CHKN_ERR_ARG).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int chkn_flux_vjp_points_noalloc(int npts, int x_order, const double* in, double* out);
 
 /* Whether one block of nic x njc x nkc cells fits the kernels' 32-bit index and offset
  * arithmetic (no counterpart in the reference, whose CPU arrays are std::vector): every
