@@ -2,17 +2,20 @@
 //
 // One reversed RK stage s with stage state W_s (read from the tape) and residual adjoint D
 // (the adjoint of dU/dt of that stage, per cell):
-//   1. face pass: every face of every active block forms Fbar = A (D_R/V_R - D_L/V_L) from
-//      its two adjacent cells (ghost sides contribute nothing) and the vector-Jacobian
-//      product of its flux (face_flux_vjp) with respect to its four stencil states, stored
-//      per face in G (24 values: L1, L0, R0, R1 x rho, e, YB, vx, vy, vz);
-//   2. gather: every cell sums, in a fixed order, the stencil adjoints of the faces whose
-//      stencils contain it -- its own block's faces, the wall and outflow ghosts copied from
-//      it, and the exchange ghosts other blocks read from it -- then maps the primitive
-//      adjoint to the conserved one (decode_vjp) and applies the transpose of the TVD-RK3
-//      stage combination (block.cu:1090-1160).  A cell next to an inflow boundary also adds
-//      the adjoints that land on its line of inflow ghosts into that line's accumulator (the
-//      gradient of the inflow state).
+//   1. face pass, per patch of 16 x 16 faces of one direction (16 consecutive faces along 16
+//      lines): every face forms Fbar = A (D_R/V_R - D_L/V_L) from its two adjacent cells
+//      (ghost sides contribute nothing) and the vector-Jacobian product of its flux
+//      (face_flux_vjp) with respect to its four stencil states; the 24 values go to LDS, and
+//      each of the patch's 13 owned cells per line sums the four that reach it (faces m-1,
+//      m, m+1, m+2 as R1, R0, L0, L1) into Gc[dir].  The thread of a line's first or last
+//      cell also sums the values that land on the block's ghost cells of that line, per
+//      ghost depth, into GB;
+//   2. gather: every cell adds its three direction sums, the ghost-line adjoints that come
+//      back to it (wall and outflow ghosts are copies of it; exchange ghosts of other blocks
+//      read it), maps the primitive adjoint to the conserved one (decode_vjp) and applies the
+//      transpose of the TVD-RK3 stage combination (block.cu:1090-1160).  A cell next to an
+//      inflow boundary also adds the adjoint of its line of inflow ghosts into that line's
+//      accumulator (the gradient of the inflow state).
 // No atomics: every sum has one writer and a fixed order, so the gradient is deterministic.
 // Gradients need no bit-exact arithmetic, so this file is built with floating-point
 // contraction and reciprocal arithmetic allowed (the stage kernels are not).
@@ -62,80 +65,54 @@ __device__ __forceinline__ const double* geom(const BlockDev* B, int d)
     return d == 0 ? B->gi : (d == 1 ? B->gj : B->gk);
 }
 
-__device__ __forceinline__ double* gface(const BlockDev* B, int d)
+// Ghost-line buffer of side `side` of a block: [2 depths][6][nt] per side, sides in order.
+__device__ __forceinline__ int side_lines(const BlockDev* B, int side)
 {
-    long long off = d == 0 ? 0 : (d == 1 ? B->nfi : B->nfi + B->nfj);
-    return B->G + 24 * off;
+    const int d = side >> 1;
+    return d == 0 ? B->njc * B->nkc : (d == 1 ? B->nic * B->nkc : B->nic * B->njc);
+}
+__device__ __forceinline__ double* ghost_buffer(const BlockDev* B, int side)
+{
+    long long off = 0;
+    for (int s = 0; s < side; ++s) off += 12LL * side_lines(B, s);
+    return B->GB + off;
 }
 
-// Adds stencil slot p (0 L1, 1 L0, 2 R0, 3 R1) of face fi of direction d to acc.  G is
-// written once by the face pass and read once here: both sides use non-temporal accesses so
-// that it does not evict the tape states and residual adjoints the passes re-read.
-__device__ __forceinline__ void add_slot(const BlockDev* B, int d, int fi, int p, double acc[6])
-{
-    const double* g = gface(B, d);
-    const long long nf = nfaces(B, d);
-    const unsigned o = boff(fi);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) acc[q] += gld_nt(g + (6 * p + q) * nf, o);
-}
-
-constexpr int kFaceThreads = 128;
-
-// One thread per face.  The four stencil states are staged in LDS (one column per thread,
-// read back only by that thread) so that they need not stay in registers across the
-// flux adjoint: reconstruction reads them, AUSMDV's adjoint runs on the reconstructed
-// states alone, and the limiter adjoints read them again and write G directly.
+// The stencil adjoints of the face at line position m (between cells m-1 and m) of the line
+// (a, c) in direction d, left in column t of st: st[6p+q][t] = adjoint of stencil slot p
+// (L1, L0, R0, R1), quantity q.  Zero for faces outside the block.
 template <int XORDER>
-__global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
+__device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, int d, int m, int a,
+                                         int c, double (&st)[24][kAdjPatchM * kAdjPatchT], int t)
 {
-    __shared__ double st[24][kFaceThreads];
-    const int t = (int)threadIdx.x;
-    const FaceChunk ch = A.faces[blockIdx.x];
-    const BlockDev* B = A.blocks + ch.blk;
-    const int d = ch.dir;
-    const long long nf = nfaces(B, d);
-    const int fi = ch.f0 + t;
-    if (fi >= ch.f1) return;
-    const int nic = B->nic, njc = B->njc;
-    int m, a, c;
-    if (d == 0) {
-        m = fi % (nic + 1);
-        const int r = fi / (nic + 1);
-        a = r % njc;
-        c = r / njc;
-    } else if (d == 1) {
-        a = fi % nic;
-        const int r = fi / nic;
-        m = r % (njc + 1);
-        c = r / (njc + 1);
-    } else {
-        a = fi % nic;
-        const int r = fi / nic;
-        c = r % njc;
-        m = r / njc;
-    }
     const int nline = line_len(B, d);
-    const long long n = B->n;
-    const double* D = B->adj + 12 * (long long)n;
-    const double* g = geom(B, d);
-    const double area = g[fi];
-    const int cl = (m - 1 >= 0) ? cell_index(nic, njc, d, m - 1, a, c) : -1;
-    const int cr = (m < nline) ? cell_index(nic, njc, d, m, a, c) : -1;
-    const double vl = cl >= 0 ? B->volinv[cl] : 0.0, vr = cr >= 0 ? B->volinv[cr] : 0.0;
-    double Fb[6];
+    const int na = d == 0 ? B->njc : B->nic;
+    const int nc = d == 2 ? B->njc : B->nkc;
+    const bool valid = m >= 0 && m <= nline && a < na && c < nc;
+    double Fb[6] = {0, 0, 0, 0, 0, 0};
     bool any = false;
+    const long long n = B->n;
+    const int nic = B->nic, njc = B->njc;
+    const long long nf = nfaces(B, d);
+    const double* g = geom(B, d);
+    const int fi = valid ? face_index(B, d, m, a, c) : 0;
+    if (valid) {
+        const double* D = B->adj + 12 * n;
+        const double area = g[fi];
+        const int cl = (m - 1 >= 0) ? cell_index(nic, njc, d, m - 1, a, c) : -1;
+        const int cr = (m < nline) ? cell_index(nic, njc, d, m, a, c) : -1;
+        const double vl = cl >= 0 ? B->volinv[cl] : 0.0, vr = cr >= 0 ? B->volinv[cr] : 0.0;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const double dl = cl >= 0 ? D[(long long)q * n + cl] : 0.0;
-        const double dr = cr >= 0 ? D[(long long)q * n + cr] : 0.0;
-        Fb[q] = area * (vr * dr - vl * dl);
-        any = any || (Fb[q] != 0.0);
+        for (int q = 0; q < 6; ++q) {
+            const double dl = cl >= 0 ? D[(long long)q * n + cl] : 0.0;
+            const double dr = cr >= 0 ? D[(long long)q * n + cr] : 0.0;
+            Fb[q] = area * (vr * dr - vl * dl);
+            any = any || (Fb[q] != 0.0);
+        }
     }
-    double* G = gface(B, d);
     if (!any) {
 #pragma unroll
-        for (int k = 0; k < 24; ++k) gst_nt(G + (long long)k * nf, boff(fi), 0.0);
+        for (int k = 0; k < 24; ++k) st[k][t] = 0.0;
         return;
     }
     const double* W = B->W + A.slot * B->wstride;
@@ -151,6 +128,9 @@ __global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
         st[6 * p + 4][t] = sp.vy;
         st[6 * p + 5][t] = sp.vz;
     }
+    // Compiler fence: the stencil values are read back from LDS where they are needed rather
+    // than forwarded from the stores above and held in registers through the flux adjoint.
+    __asm__ __volatile__("" ::: "memory");
     // Reconstructed face states (face.cu:347-371).
     double qL[6], qR[6];
 #pragma unroll
@@ -184,6 +164,8 @@ __global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
         side[1][2] = bR[4] + bT * kQ / kCv;
         side[1][3] = bR[5]; side[1][4] = bR[6]; side[1][5] = bR[7];
     }
+    __asm__ __volatile__("" ::: "memory");
+    // Each quantity's four stencil values are read before its four adjoints replace them.
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         double gq[4] = {0.0, 0.0, 0.0, 0.0};
@@ -194,27 +176,63 @@ __global__ __launch_bounds__(kFaceThreads) void adj_face_kernel(AdjArgs A)
             gq[2] = side[1][q];
         }
 #pragma unroll
-        for (int p = 0; p < 4; ++p) gst_nt(G + (long long)(6 * p + q) * nf, boff(fi), gq[p]);
+        for (int p = 0; p < 4; ++p) st[6 * p + q][t] = gq[p];
     }
 }
 
-// Adjoint reaching the ghost line of boundary side `side` at transverse (a, c): the sum over
-// both ghost layers, in the same order the gather uses for walls and outflow.
-__device__ __forceinline__ void ghost_line_adjoint(const BlockDev* B, int side, int a, int c,
-                                                   double acc[6])
+// Face pass over one patch (see AdjPatch).  Lanes run along i in every direction so that the
+// stencil, D and geometry loads of a wave are contiguous: in direction 0 a lane is a face
+// position along the line, in directions 1 and 2 a line.
+// Three work-groups per CU fit the LDS (48 KB each), three waves per SIMD.
+template <int XORDER>
+__global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjArgs A)
 {
-    const int d = side >> 1;
+    __shared__ double st[24][kAdjPatchM * kAdjPatchT];
+    const int t = (int)threadIdx.x;
+    const AdjPatch P = A.patches[blockIdx.x];
+    const BlockDev* B = A.blocks + P.blk;
+    const int d = P.dir;
+    const int tm = d == 0 ? t % kAdjPatchM : t / kAdjPatchT;  // face position in the patch
+    const int ta = d == 0 ? t / kAdjPatchM : t % kAdjPatchT;  // line in the patch
+    const int a = P.a0 + ta, c = P.c;
+    face_vjp<XORDER>(A, B, d, P.m0 - 1 + tm, a, c, st, t);
+    __syncthreads();
+    // Owned cell m = m0 + tm of line (a, c): faces m-1 .. m+2 sit at positions tm .. tm+3.
     const int nline = line_len(B, d);
-    if ((side & 1) == 0) {
-        const int f0 = face_index(B, d, 0, a, c), f1 = face_index(B, d, 1, a, c);
-        add_slot(B, d, f0, 1, acc);  // depth 0 as L0 of face 0
-        add_slot(B, d, f1, 0, acc);  // depth 0 as L1 of face 1
-        add_slot(B, d, f0, 0, acc);  // depth 1 as L1 of face 0
-    } else {
-        const int fa = face_index(B, d, nline - 1, a, c), fb = face_index(B, d, nline, a, c);
-        add_slot(B, d, fa, 3, acc);  // depth 0 as R1 of face n-1
-        add_slot(B, d, fb, 2, acc);  // depth 0 as R0 of face n
-        add_slot(B, d, fb, 3, acc);  // depth 1 as R1 of face n
+    const int na = d == 0 ? B->njc : B->nic;
+    const int m = P.m0 + tm;
+    if (tm >= kAdjPatchOwn || m >= nline || a >= na) return;
+    auto lane = [&](int pos) { return d == 0 ? pos + kAdjPatchM * ta : ta + kAdjPatchT * pos; };
+    const int l3 = lane(tm), l2 = lane(tm + 1), l1 = lane(tm + 2), l0 = lane(tm + 3);
+    const long long n = B->n;
+    const int cidx = cell_index(B->nic, B->njc, d, m, a, c);
+    double* Gd = B->Gc + 6 * (long long)d * n;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        gst_nt(Gd + (long long)q * n, boff(cidx), ((st[18 + q][l3] + st[12 + q][l2]) + st[6 + q][l1]) + st[q][l0]);
+    // Ghost lines: what lands on the ghosts at depth 0 and 1 of the line's minus side (faces 0
+    // and 1 as L0, L1 and face 0 as L1) and plus side (faces n-1 and n as R1, R0 and face n as
+    // R1).
+    const int line = a + na * c;
+    if (m == 0) {
+        const int f0 = lane(tm + 1), f1 = lane(tm + 2);
+        double* gb = ghost_buffer(B, 2 * d);
+        const int nt = side_lines(B, 2 * d);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            gb[(long long)q * nt + line] = st[6 + q][f0] + st[q][f1];
+            gb[(long long)(6 + q) * nt + line] = st[q][f0];
+        }
+    }
+    if (m == nline - 1) {
+        const int fa = lane(tm + 1), fb = lane(tm + 2);
+        double* gb = ghost_buffer(B, 2 * d + 1);
+        const int nt = side_lines(B, 2 * d + 1);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            gb[(long long)q * nt + line] = st[18 + q][fa] + st[12 + q][fb];
+            gb[(long long)(6 + q) * nt + line] = st[18 + q][fb];
+        }
     }
 }
 
@@ -228,31 +246,34 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
     if (cidx >= ch.c1) return;
     const int nic = B->nic, njc = B->njc;
     const int ijk[3] = {cidx % nic, (cidx / nic) % njc, cidx / (nic * njc)};
-    double Wb[6] = {0, 0, 0, 0, 0, 0};
+    const unsigned o = boff(cidx);
+    double Wb[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        Wb[q] = (gld_nt(B->Gc + (long long)q * n, o) + gld_nt(B->Gc + (long long)(6 + q) * n, o)) +
+                gld_nt(B->Gc + (long long)(12 + q) * n, o);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const int m = ijk[d];
         const int a = d == 0 ? ijk[1] : ijk[0];
         const int c = d == 2 ? ijk[1] : ijk[2];
         const int nline = line_len(B, d);
-        // This block's faces whose stencils hold the cell: m-1 (as R1), m (R0), m+1 (L0),
-        // m+2 (L1).
-        if (m - 1 >= 0) add_slot(B, d, face_index(B, d, m - 1, a, c), 3, Wb);
-        add_slot(B, d, face_index(B, d, m, a, c), 2, Wb);
-        add_slot(B, d, face_index(B, d, m + 1, a, c), 1, Wb);
-        if (m + 2 <= nline) add_slot(B, d, face_index(B, d, m + 2, a, c), 0, Wb);
-        // Wall and outflow ghosts of this block copy the cell next to the boundary.
+        const int line = a + (d == 0 ? njc : nic) * c;
+        // Wall, outflow and inflow ghosts of this block hold copies of (or fixed values next
+        // to) the cell next to the boundary: both depths come back to it.
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             if (m != (side ? nline - 1 : 0)) continue;
             const int code = B->bc[2 * d + side];
             if (code == 3 || code == 6) continue;
-            double gb[6] = {0, 0, 0, 0, 0, 0};
-            ghost_line_adjoint(B, 2 * d + side, a, c, gb);
+            const double* gbuf = ghost_buffer(B, 2 * d + side);
+            const int nt = side_lines(B, 2 * d + side);
+            double gb[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) gb[q] = gbuf[(long long)q * nt + line] + gbuf[(long long)(6 + q) * nt + line];
             if (code == 4) {
                 // Inflow ghosts hold the inflow state: this cell owns its line's accumulator.
-                const int na = d == 0 ? njc : nic;
-                const long long col = B->icol + B->icol_side[2 * d + side] + a + na * c;
+                const long long col = B->icol + B->icol_side[2 * d + side] + line;
 #pragma unroll
                 for (int q = 0; q < 6; ++q) A.inflow_acc[(long long)q * A.nicols + col] += gb[q];
             } else if (code == 5) {
@@ -264,34 +285,31 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
                 reflect_vjp(gb, ld_frame(g, nf, fi), Wb);
             }
         }
-        // Exchange ghosts of other blocks (or this one, periodically) that read the cell.
-        const int P = B->rp[2 * d];      // its plus-side ghosts at depth m read cells m = 0, 1
+        // Exchange ghosts of other blocks (or this one, periodically) that read the cell: the
+        // plus-side ghosts of P at depth m read cells m = 0, 1; the minus-side ghosts of Q at
+        // depth nline-1-m read cells m = nline-1, nline-2.
+        const int P = B->rp[2 * d];
         if (P >= 0 && m <= 1) {
             const BlockDev* O = A.blocks + P;
-            const int no = line_len(O, d);
-            if (m == 0) {
-                add_slot(O, d, face_index(O, d, no - 1, a, c), 3, Wb);
-                add_slot(O, d, face_index(O, d, no, a, c), 2, Wb);
-            } else {
-                add_slot(O, d, face_index(O, d, no, a, c), 3, Wb);
-            }
+            const double* gbuf = ghost_buffer(O, 2 * d + 1) + (long long)(6 * m) * side_lines(O, 2 * d + 1);
+            const int nt = side_lines(O, 2 * d + 1);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) Wb[q] += gbuf[(long long)q * nt + line];
         }
-        const int Q = B->rp[2 * d + 1];  // its minus-side ghosts at depth nline-1-m
+        const int Q = B->rp[2 * d + 1];
         if (Q >= 0 && m >= nline - 2) {
             const BlockDev* O = A.blocks + Q;
-            if (m == nline - 1) {
-                add_slot(O, d, face_index(O, d, 0, a, c), 1, Wb);
-                add_slot(O, d, face_index(O, d, 1, a, c), 0, Wb);
-            } else {
-                add_slot(O, d, face_index(O, d, 0, a, c), 0, Wb);
-            }
+            const int depth = nline - 1 - m;
+            const double* gbuf = ghost_buffer(O, 2 * d) + (long long)(6 * depth) * side_lines(O, 2 * d);
+            const int nt = side_lines(O, 2 * d);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) Wb[q] += gbuf[(long long)q * nt + line];
         }
     }
     const Prim w = ld_prim(B->W + A.slot * B->wstride, n, cidx);
     double Ub[6];
     decode_vjp(w, Wb, Ub);
     const double dt = A.dt[B->inst];
-    const unsigned o = boff(cidx);
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         double* A0 = B->adj + (long long)q * n;
@@ -358,11 +376,12 @@ hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_adj_faces(const AdjArgs& a, int nface_chunks, int x_order, hipStream_t s)
+hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, hipStream_t s)
 {
-    if (nface_chunks <= 0) return hipSuccess;
-    if (x_order > 1) hipLaunchKernelGGL(adj_face_kernel<2>, dim3(nface_chunks), dim3(kFaceThreads), 0, s, a);
-    else hipLaunchKernelGGL(adj_face_kernel<1>, dim3(nface_chunks), dim3(kFaceThreads), 0, s, a);
+    if (npatches <= 0) return hipSuccess;
+    const dim3 blk(kAdjPatchM * kAdjPatchT);
+    if (x_order > 1) hipLaunchKernelGGL(adj_patch_kernel<2>, dim3(npatches), blk, 0, s, a);
+    else hipLaunchKernelGGL(adj_patch_kernel<1>, dim3(npatches), blk, 0, s, a);
     return hipGetLastError();
 }
 

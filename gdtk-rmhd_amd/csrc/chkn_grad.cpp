@@ -31,18 +31,22 @@ void build_adj_tables(chkn_sim* s)
 {
     GradState& g = *s->grad;
     if (g.d_face_chunks) return;
-    std::vector<FaceChunk> fc;
+    std::vector<AdjPatch> fc;
     std::vector<CellChunk> cc;
     std::vector<InflowCol> ic;
     for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
         const HostBlock& b = s->blocks[ib];
         if (!b.active) continue;
-        const long long nf[3] = {(long long)(b.nic + 1) * b.njc * b.nkc,
-                                 (long long)b.nic * (b.njc + 1) * b.nkc,
-                                 (long long)b.nic * b.njc * (b.nkc + 1)};
-        for (int d = 0; d < 3; ++d)
-            for (long long f0 = 0; f0 < nf[d]; f0 += 128)
-                fc.push_back(FaceChunk{(int)ib, d, (int)f0, (int)std::min(f0 + 128, nf[d])});
+        // Face patches per direction: lines of kAdjPatchT first transverse indices, each
+        // covering kAdjPatchOwn cells along the line.
+        for (int d = 0; d < 3; ++d) {
+            const int nline = d == 0 ? b.nic : (d == 1 ? b.njc : b.nkc);
+            const int na = d == 0 ? b.njc : b.nic;
+            const int nc = d == 2 ? b.njc : b.nkc;
+            for (int c = 0; c < nc; ++c)
+                for (int a0 = 0; a0 < na; a0 += kAdjPatchT)
+                    for (int m0 = 0; m0 < nline; m0 += kAdjPatchOwn) fc.push_back(AdjPatch{(int)ib, d, m0, a0, c});
+        }
         for (int c0 = 0; c0 < b.n; c0 += 256) cc.push_back(CellChunk{(int)ib, c0, std::min(c0 + 256, b.n)});
         BlockDev& bd = s->bdev[ib];
         bd.icol = (int)ic.size();
@@ -61,12 +65,12 @@ void build_adj_tables(chkn_sim* s)
     g.nface_chunks = (int)fc.size();
     g.ncell_chunks = (int)cc.size();
     g.ninflow_cols = (int)ic.size();
-    HIPCHK(hipMalloc(&g.d_face_chunks, sizeof(FaceChunk) * std::max<size_t>(fc.size(), 1)));
+    HIPCHK(hipMalloc(&g.d_face_chunks, sizeof(AdjPatch) * std::max<size_t>(fc.size(), 1)));
     HIPCHK(hipMalloc(&g.d_cell_chunks, sizeof(CellChunk) * std::max<size_t>(cc.size(), 1)));
     HIPCHK(hipMalloc(&g.d_inflow_cols, sizeof(InflowCol) * std::max<size_t>(ic.size(), 1)));
     HIPCHK(hipMalloc(&g.d_inflow_acc, sizeof(double) * 6 * std::max<size_t>(ic.size(), 1)));
     if (!fc.empty())
-        HIPCHK(hipMemcpy(g.d_face_chunks, fc.data(), sizeof(FaceChunk) * fc.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(g.d_face_chunks, fc.data(), sizeof(AdjPatch) * fc.size(), hipMemcpyHostToDevice));
     if (!cc.empty())
         HIPCHK(hipMemcpy(g.d_cell_chunks, cc.data(), sizeof(CellChunk) * cc.size(), hipMemcpyHostToDevice));
     if (!ic.empty())
@@ -130,12 +134,14 @@ void alloc_adjoint(chkn_sim* s, int C)
             HIPCHK(hipMalloc(&b.tape, sizeof(double) * 6 * (size_t)b.n * (3 * (size_t)C + 1)));
         }
         if (!b.adj) {
-            const size_t nf = (size_t)(b.nic + 1) * b.njc * b.nkc + (size_t)b.nic * (b.njc + 1) * b.nkc +
-                              (size_t)b.nic * b.njc * (b.nkc + 1);
+            // Ghost lines of the six sides, two depths of six values each.
+            const size_t lines = 2 * ((size_t)b.njc * b.nkc + (size_t)b.nic * b.nkc + (size_t)b.nic * b.njc);
             HIPCHK(hipMalloc(&b.adj, sizeof(double) * 18 * (size_t)b.n));
-            HIPCHK(hipMalloc(&b.G, sizeof(double) * 24 * nf));
+            HIPCHK(hipMalloc(&b.Gc, sizeof(double) * 18 * (size_t)b.n));
+            HIPCHK(hipMalloc(&b.GB, sizeof(double) * 12 * lines));
             s->bdev[ib].adj = b.adj;
-            s->bdev[ib].G = b.G;
+            s->bdev[ib].Gc = b.Gc;
+            s->bdev[ib].GB = b.GB;
         }
     }
     if (g.tape_C < C) {
@@ -387,7 +393,7 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     AdjArgs A;
     std::memset(&A, 0, sizeof A);
     A.inflow = s->d_inflow;
-    A.faces = (const FaceChunk*)g.d_face_chunks;
+    A.patches = (const AdjPatch*)g.d_face_chunks;
     A.cells = (const CellChunk*)g.d_cell_chunks;
     A.icols = (const InflowCol*)g.d_inflow_cols;
     A.inflow_acc = g.d_inflow_acc;
@@ -477,8 +483,8 @@ namespace rt {
 void free_grad(chkn_sim* s)
 {
     for (auto& b : s->blocks) {
-        dfree(b.tape); dfree(b.tape2); dfree(b.ckpt); dfree(b.adj); dfree(b.G);
-        b.tape = b.tape2 = b.ckpt = b.adj = b.G = nullptr;
+        dfree(b.tape); dfree(b.tape2); dfree(b.ckpt); dfree(b.adj); dfree(b.Gc); dfree(b.GB);
+        b.tape = b.tape2 = b.ckpt = b.adj = b.Gc = b.GB = nullptr;
     }
     if (s->grad) {
         dfree(s->grad->d_face_chunks); dfree(s->grad->d_cell_chunks);

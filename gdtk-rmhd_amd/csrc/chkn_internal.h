@@ -59,7 +59,9 @@ struct BlockDev {
     int rp[6];             // block whose opposite side exchanges into this side, or -1
     int icol;              // first inflow line of this block in the inflow accumulator
     int icol_side[6];      // offset of each inflow side's lines from icol, or -1
-    double* G;             // face stencil adjoints: [24][nfi], then [24][nfj], then [24][nfk]
+    double* Gc;            // per direction d, the sum over the block's own faces of the adjoint
+                           // reaching each cell through their stencils: [3][6][n]
+    double* GB;            // adjoint reaching each side's ghost lines: per side [2 depths][6][nt]
     double* adj;           // A0 (adjoint of U at the step boundary), P, D: [3][6][n]
     // face-list stage
     real* F;
@@ -102,8 +104,16 @@ struct StageArgs {
     int win, wout;         // W slots read / written by this stage
 };
 
-// Work items of the adjoint kernels: up to 128 faces of one direction of one block, up to 256
-// cells of one block, one boundary column (line of ghost cells) of an inflow boundary.
+// Work items of the adjoint kernels: a patch of faces of one direction, up to 256 cells of one
+// block, one boundary column (line of ghost cells) of an inflow boundary.
+// Patch: faces at line positions m0-1 .. m0+kAdjPatchM-2 (m = face between cells m-1 and m)
+// of the kAdjPatchT lines with first transverse index a0 .. a0+kAdjPatchT-1 and second
+// transverse index c, in direction dir; it owns the cells m0 .. m0+kAdjPatchM-4 of those lines
+// (every face whose stencil holds an owned cell is in the patch).
+constexpr int kAdjPatchM = 16, kAdjPatchT = 16, kAdjPatchOwn = kAdjPatchM - 3;
+struct AdjPatch {
+    int blk, dir, m0, a0, c;
+};
 struct FaceChunk {
     int blk, dir, f0, f1;  // faces [f0, f1) of direction dir, at most 128
 };
@@ -123,7 +133,7 @@ struct InflowCol {
 struct AdjArgs {
     const BlockDev* blocks;   // W must point at the tape: slot = stage state being reversed
     const double* inflow;
-    const FaceChunk* faces;
+    const AdjPatch* patches;
     const CellChunk* cells;
     const InflowCol* icols;
     double* inflow_acc;       // [6][nicols]
@@ -133,12 +143,13 @@ struct AdjArgs {
     const double* dt_next;    // [ninst] time step of the step reversed next, or null (stage 1)
 };
 
-// Adjoint of one stage's residual: face pass (stencil adjoints of every face into G), then
-// the per-cell gather with the stage's RK combination (stage 3, 2, 1 in reverse order) and
+// Adjoint of one stage's residual: face pass (per patch, every face's stencil adjoints summed
+// per owned cell into Gc, ghost-line adjoints into GB), then the per-cell gather with the
+// stage's RK combination (stage 3, 2, 1 in reverse order) and
 // the inflow lines; the stage-1 gather also seeds the next reversed step when dt_next is
 // given.  seed: P = dt/6 A0, D = dt 4/6 A0 (start of the first reversed step).
 hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s);
-hipError_t launch_adj_faces(const AdjArgs& a, int nface_chunks, int x_order, hipStream_t s);
+hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, hipStream_t s);
 hipError_t launch_adj_gather(const AdjArgs& a, int ncell_chunks, int stage, hipStream_t s);
 // U and W = decode(U) of every active block from U (after the conserved state was replaced).
 hipError_t launch_decode_all(const BlockDev* blocks, const CellChunk* cells, int ncell_chunks,

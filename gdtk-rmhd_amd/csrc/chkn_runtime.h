@@ -72,7 +72,8 @@ struct HostBlock {
     double* tape2 = nullptr;   // second tape: the next segment is recomputed into it meanwhile
     double* ckpt = nullptr;    // checkpoints: U then W, 12n doubles each
     double* adj = nullptr;     // A0, P, D: 18n doubles
-    double* G = nullptr;       // face stencil adjoints: 24 x (nfi + nfj + nfk)
+    double* Gc = nullptr;      // per-cell direction sums of face stencil adjoints: 18 x n
+    double* GB = nullptr;      // ghost-line adjoints: 12 x (lines of the six sides)
 };
 
 // Time control of one instance (SimState of simulate.cu:32-39 plus its schedules).
@@ -103,7 +104,7 @@ struct GradState {
     bool valid = false;
     long long generation = 0;     // number of successful chkn_grad_forward calls on the handle
     // face/cell work tables and inflow patches (device)
-    int nface_chunks = 0, ncell_chunks = 0, ninflow_cols = 0;
+    int nface_chunks = 0, ncell_chunks = 0, ninflow_cols = 0;  // nface_chunks: face patches
     void* d_face_chunks = nullptr;
     void* d_cell_chunks = nullptr;
     void* d_inflow_cols = nullptr;
