@@ -78,12 +78,15 @@ def build_workload(name, steps_total):
         jobs = [cases.ramp(128, 64, 64, max_step=1 << 30)]
         desc = {"workload": "differentiation: one 128x64x64 block, Mach 5 ramp; forward + adjoint "
                             "of the wall-pressure loss w.r.t. U0 and inflow (Mach, p, T), "
-                            "checkpoint every 50 steps with recomputation, dt sequence frozen, FP64"}
+                            "checkpoint every 50 steps (segments taped by the forward pass while "
+                            "they fit in half the free device memory, recomputed otherwise), dt "
+                            "sequence frozen, FP64"}
     elif name == "ensemble-diff":
         jobs = cases.ramp_ensemble(seed=11, n=64, max_step=1 << 30)
         desc = {"workload": "batched differentiation: 64 independent Mach 3-8 ramp jobs (seed 11), "
                             "forward + adjoint of each wall-pressure loss w.r.t. its inflow "
-                            "(Mach, p, T), checkpoint every 50 steps, FP64"}
+                            "(Mach, p, T), checkpoint every 50 steps (segments taped while they fit "
+                            "in half the free device memory, recomputed otherwise), FP64"}
     else:
         raise SystemExit("unknown workload " + name)
     stepper = ("predictor-corrector (2 stages)" if jobs[0].time_scheme == 1 else "TVD-RK3 (3 stages)")
@@ -116,8 +119,10 @@ def run_timed(jobs, opts, steps, warmup):
 
 
 def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
-    """Forward (taped) + adjoint of the wall-pressure loss against the plain forward of the
-    same steps.  Returns a dict of timings."""
+    """Forward (checkpointed) + adjoint of the wall-pressure loss against the plain forward
+    of the same steps, with the default tape budget (the forward pass tapes the checkpoint
+    segments that fit in half the free device memory, the backward pass recomputes the rest)
+    and with every segment recomputed.  Returns a dict of timings."""
     import torch
     from chkn import adjoint
     from chkn.lib import Sim
@@ -146,8 +151,14 @@ def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
         t2 = time.perf_counter()
         return t1 - t0, t2 - t1, float(L.detach()), [m.grad for m in mp], u.grad
 
-    gradient(min(warmup, steps) or 1)  # allocates tapes and checkpoints
+    gradient(steps)  # allocates tapes and checkpoints
     t_fwd, t_adj, L, gm, gu = gradient(steps)
+    resident = sim.grad_resident_segments
+    sim.set_tape_budget(0)
+    gradient(steps)
+    rc_fwd, rc_adj, rc_L, rc_gm, rc_gu = gradient(steps)
+    same = rc_L == L and all(bool(torch.equal(a, b)) for a, b in zip(gm, rc_gm)) and bool(torch.equal(gu, rc_gu))
+    sim.set_tape_budget(None)
     # Plain forward of the same steps from the same state (stepping ring, graph replay).
     sim.write_conserved(u0)
     sim.step(1)
@@ -159,8 +170,14 @@ def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
     t_plain = time.perf_counter() - t0
     cells = sim.active_cells
     sim.close()
+    nseg = -(-steps // checkpoint_every)
     return {"seconds_forward_taped": t_fwd, "seconds_adjoint": t_adj, "seconds_plain_forward": t_plain,
-            "fwd_adj_over_fwd": (t_fwd + t_adj) / t_plain, "loss": L,
+            "fwd_adj_over_fwd": (t_fwd + t_adj) / t_plain,
+            "resident_segments": "%d of %d" % (resident, nseg),
+            "recompute_all": {"seconds_forward": rc_fwd, "seconds_adjoint": rc_adj,
+                              "fwd_adj_over_fwd": (rc_fwd + rc_adj) / t_plain,
+                              "gradient_identical": same},
+            "loss": L,
             "dL_dMpT_first": [float(v) for v in gm[0]], "grad_u0_norm": float(gu.norm()),
             "cells": cells}
 
@@ -408,7 +425,7 @@ def batched_record(graph, steps=60, warmup=10):
 
 
 def diff_record(graph, steps=100):
-    """Forward (taped, checkpoint every 50 steps) + adjoint of the wall-pressure loss of the
+    """Forward (checkpoint every 50 steps) + adjoint of the wall-pressure loss of the
     128x64x64 ramp w.r.t. U0 and the inflow (Mach, p, T), against the plain forward of the
     same steps."""
     jobs, desc = build_workload("diff", steps)
@@ -418,7 +435,8 @@ def diff_record(graph, steps=100):
     return {"workload": desc["workload"], "cells": r["cells"], "steps": steps, "checkpoint_every": 50,
             "value": r["cells"] * steps / wall, "unit": "cell-updates/s (forward + adjoint)",
             "plain_forward_value": r["cells"] * steps / r["seconds_plain_forward"],
-            "fwd_adj_over_fwd": r["fwd_adj_over_fwd"], "launch": opts,
+            "fwd_adj_over_fwd": r["fwd_adj_over_fwd"], "resident_segments": r["resident_segments"],
+            "recompute_all_fwd_adj_over_fwd": r["recompute_all"]["fwd_adj_over_fwd"], "launch": opts,
             "seconds": {k: r[k] for k in ("seconds_forward_taped", "seconds_adjoint",
                                           "seconds_plain_forward")}}
 

@@ -91,6 +91,8 @@ EXPORTS = {
     "chkn_real_size": (ctypes.c_int, []),
     "chkn_set_verbosity": (ctypes.c_int, [ctypes.c_int]),
     "chkn_grad_generation": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "chkn_grad_set_tape_budget": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
+    "chkn_grad_resident_segments": (ctypes.c_int, [ctypes.c_void_p]),
     "chkn_check_block_dims": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "chkn_device_bytes": (ctypes.c_longlong, [ctypes.c_void_p]),
     "chkn_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
@@ -340,6 +342,17 @@ class Sim:
         self._check(self._lib.chkn_grad_forward(self.h, int(nsteps), int(checkpoint_every), dptr,
                                                 _ptr(out, _D)))
         return out
+
+    def set_tape_budget(self, nbytes):
+        """Device bytes grad_forward may fill with resident tapes of its last checkpoint
+        segments (reversed without recomputation): None or a negative value for half of the
+        free device memory (the default), 0 to recompute every segment."""
+        self._check(self._lib.chkn_grad_set_tape_budget(self.h, -1 if nbytes is None else int(nbytes)))
+
+    @property
+    def grad_resident_segments(self):
+        """Checkpoint segments of the last grad_forward held on resident tapes."""
+        return int(self._lib.chkn_grad_resident_segments(self.h))
 
     @property
     def grad_generation(self):

@@ -3,13 +3,15 @@
 // chkn_grad_forward runs n steps from the handle's current state, keeping a checkpoint (U
 // and W) every C steps and the time step of every step.  chkn_grad_backward takes the
 // adjoint of the final conserved state and walks the checkpoint segments backwards: each
-// segment is recomputed from its checkpoint with the recorded time steps, writing the
-// 3C+1 stage states of the segment into a tape (the block table's W is re-pointed at the
-// tape, so the stage kernels are the forward ones), then the steps of the segment are
-// reversed with the adjoint kernels.  The time-step schedule is held fixed (the gradient of
-// the CFL min-reduction is not taken), so the gradient is that of the map with the
-// recorded dt sequence.  The recomputation repeats the forward arithmetic exactly, so the
-// gradient does not depend on C.
+// segment's 3C+1 stage states are read from a tape (a block table whose W points at the
+// tape, so the stage kernels are the forward ones) and the steps of the segment are
+// reversed with the adjoint kernels.  The last segments, as many as the tape budget holds
+// in device memory, are taped by the forward pass itself (resident tapes); the others are
+// recomputed from their checkpoints with the recorded time steps, the next one on a second
+// stream while the current one is reversed.  The time-step schedule is held fixed (the
+// gradient of the CFL min-reduction is not taken), so the gradient is that of the map with
+// the recorded dt sequence.  Recomputation and taping repeat the forward arithmetic
+// exactly, so the gradient depends neither on C nor on the budget.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -121,13 +123,117 @@ void alloc_checkpoints(chkn_sim* s, int nckpt)
     g.nckpt_alloc = nckpt;
 }
 
-void alloc_adjoint(chkn_sim* s, int C)
+// W slots (doubles) of one segment's tape of block b, and bytes of one segment's tapes over
+// the active blocks.
+size_t tape_doubles(const HostBlock& b, int C) { return 6 * (size_t)b.n * (3 * (size_t)C + 1); }
+size_t segment_tape_bytes(const chkn_sim* s, int C)
+{
+    size_t n = 0;
+    for (auto& b : s->blocks)
+        if (b.active) n += sizeof(double) * tape_doubles(b, C);
+    return n;
+}
+
+void free_resident(chkn_sim* s)
+{
+    GradState& g = *s->grad;
+    for (auto& b : s->blocks) {
+        dfree(b.rtape);
+        b.rtape = nullptr;
+    }
+    for (BlockDev* t : g.d_rtab) dfree(t);
+    g.d_rtab.clear();
+    g.nres_alloc = g.rtape_C = g.nres = 0;
+}
+
+// Resident tapes for the last segments of an nck-segment unroll, as many as the budget holds
+// (held between unrolls while the segment length and the budget allow).
+void plan_resident(chkn_sim* s, int C, int nck)
+{
+    GradState& g = *s->grad;
+    if (g.rtape_C != C) free_resident(s);
+    const size_t seg = segment_tape_bytes(s, C);
+    size_t budget = 0;
+    if (g.tape_budget < 0) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        budget = (fr + (size_t)g.nres_alloc * seg) / 2;
+    } else {
+        budget = (size_t)g.tape_budget;
+    }
+    int want = seg > 0 ? (int)std::min<size_t>((size_t)nck, budget / seg) : 0;
+    if (want > g.nres_alloc || (g.tape_budget >= 0 && want < g.nres_alloc)) {
+        free_resident(s);
+        for (auto& b : s->blocks) {
+            if (!b.active || want == 0) continue;
+            if (hipMalloc(&b.rtape, sizeof(double) * tape_doubles(b, C) * want) != hipSuccess) {
+                (void)hipGetLastError();
+                free_resident(s);
+                want = 0;
+            }
+        }
+        if (want > 0) {
+            g.d_rtab.assign(want, nullptr);
+            for (BlockDev*& t : g.d_rtab) HIPCHK(hipMalloc(&t, sizeof(BlockDev) * s->bdev.size()));
+            g.nres_alloc = want;
+            g.rtape_C = C;
+        }
+    }
+    g.nres = std::min(want, g.nres_alloc);
+}
+
+// W of resident tape r of block b.
+double* resident_tape(const HostBlock& b, int C, int r) { return b.rtape + (size_t)r * tape_doubles(b, C); }
+
+// Block tables with W at each resident tape, with the current adjoint pointers.
+void upload_resident_tables(chkn_sim* s)
+{
+    GradState& g = *s->grad;
+    for (int r = 0; r < g.nres; ++r) {
+        std::vector<BlockDev> tab = s->bdev;
+        for (size_t ib = 0; ib < s->blocks.size(); ++ib)
+            if (s->blocks[ib].active) tab[ib].W = resident_tape(s->blocks[ib], g.rtape_C, r);
+        HIPCHK(hipMemcpy(g.d_rtab[r], tab.data(), sizeof(BlockDev) * tab.size(), hipMemcpyHostToDevice));
+    }
+}
+
+// While alive, the handle's steps write their stage states into resident tape r: W of the
+// current state is copied into its slot 0 first, and finish(K) copies the final W of a
+// K-step segment back into the ring.
+struct Taping {
+    chkn_sim* s;
+    int r, C;
+    Taping(chkn_sim* sim, int res, int c) : s(sim), r(res), C(c)
+    {
+        for (auto& b : s->blocks)
+            if (b.active)
+                HIPCHK(hipMemcpyAsync(resident_tape(b, C, r), b.W, sizeof(double) * 6 * (size_t)b.n,
+                                      hipMemcpyDeviceToDevice, s->stream));
+        s->tape_tab = s->grad->d_rtab[r];
+        s->tape_base = 0;
+    }
+    void finish(int K)
+    {
+        const size_t last = (size_t)stages_per_step(s) * K;
+        for (auto& b : s->blocks)
+            if (b.active)
+                HIPCHK(hipMemcpyAsync(b.W, resident_tape(b, C, r) + 6 * (size_t)b.n * last,
+                                      sizeof(double) * 6 * (size_t)b.n, hipMemcpyDeviceToDevice, s->stream));
+    }
+    ~Taping()
+    {
+        s->tape_tab = nullptr;
+        s->tape_base = 0;
+    }
+};
+
+void alloc_adjoint(chkn_sim* s, int C, bool ring)
 {
     GradState& g = *s->grad;
     for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
         HostBlock& b = s->blocks[ib];
         if (!b.active) continue;
-        if (g.tape_C < C) {
+        if (ring && g.tape_C < C) {
             dfree(b.tape);
             dfree(b.tape2);
             b.tape = b.tape2 = nullptr;
@@ -144,7 +250,7 @@ void alloc_adjoint(chkn_sim* s, int C)
             s->bdev[ib].GB = b.GB;
         }
     }
-    if (g.tape_C < C) {
+    if (ring && g.tape_C < C) {
         // The second tape is optional: without it the segments are recomputed and reversed
         // one after the other.
         g.two_tapes = true;
@@ -163,7 +269,7 @@ void alloc_adjoint(chkn_sim* s, int C)
                 b.tape2 = nullptr;
             }
     }
-    g.tape_C = std::max(g.tape_C, C);
+    if (ring) g.tape_C = std::max(g.tape_C, C);
     if (!g.rec) {
         HIPCHK(hipStreamCreateWithFlags(&g.rec, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&g.ev_rev[0], hipEventDisableTiming));
@@ -173,10 +279,12 @@ void alloc_adjoint(chkn_sim* s, int C)
     }
 }
 
-// Block tables with W at tape (t = 0) and tape2 (t = 1), with the current adjoint pointers.
+// Block tables with W at tape (t = 0) and tape2 (t = 1) and at the resident tapes, with the
+// current adjoint pointers.
 void upload_tape_tables(chkn_sim* s)
 {
     GradState& g = *s->grad;
+    upload_resident_tables(s);
     for (int t = 0; t < 2; ++t) {
         std::vector<BlockDev> tab = s->bdev;
         for (size_t ib = 0; ib < s->blocks.size(); ++ib)
@@ -298,6 +406,8 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
     const int nck = (nsteps + C - 1) / C;
     alloc_checkpoints(s, nck + 1);  // + 1 slot to park the final state during the backward pass
     g.valid = false;
+    plan_resident(s, C, nck);
+    upload_resident_tables(s);
     g.C = C;
     g.nsteps = nsteps;
     g.dts.assign((size_t)nsteps * N, 0.0);
@@ -308,6 +418,8 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
     for (int j = 0; j < nck; ++j) {
         const int K = std::min(C, nsteps - j * C);
         save_checkpoint(s, j);
+        std::unique_ptr<Taping> tp;
+        if (j >= nck - g.nres) tp.reset(new Taping(s, j - (nck - g.nres), C));
         if (dt_in) {
             const double* dts = dt_in + (size_t)j * C * N;
             const int bad = fixed_dt_steps(s, K, dts);
@@ -345,6 +457,7 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
                 if (v == 0.0) return set_error(s, CHKN_ERR_STATE, "an instance stopped during the unroll");
             std::copy(log.begin(), log.end(), g.dts.begin() + (size_t)j * C * N);
         }
+        if (tp) tp->finish(K);
     }
     HIPCHK(hipStreamSynchronize(s->stream));
     s->last_ms = ms_total;
@@ -369,7 +482,9 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     const int N = (int)s->inst.size();
     const int C = g.C, nsteps = g.nsteps;
     const int nck = (nsteps + C - 1) / C;
-    alloc_adjoint(s, C);
+    const int nres = g.nres;
+    const int first_res = nck - nres;  // segments first_res .. nck-1 are on resident tapes
+    alloc_adjoint(s, C, first_res > 0);
     build_adj_tables(s);
     dfree(g.d_dts);
     g.d_dts = nullptr;
@@ -400,22 +515,28 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     A.nicols = g.ninflow_cols;
     long long launches = 0;
     int rc = CHKN_OK;
-    // With two tapes, segment j-1 is recomputed on the second stream while segment j is
-    // reversed; the recompute into a tape waits for the reversal that last read it.
-    const bool two = g.two_tapes;
+    // Segments below first_res are recomputed.  With two tapes, segment j-1 is recomputed on
+    // the second stream while segment j is reversed (the first one while the resident
+    // segments are); the recompute into a tape waits for the reversal that last read it.
+    const bool two = g.two_tapes && first_res > 0;
     hipStream_t rs = two ? g.rec : s->stream;
     auto seg_len = [&](int j) { return std::min(C, nsteps - j * C); };
-    queue_recompute(s, nck - 1, seg_len(nck - 1), (nck - 1) & (two ? 1 : 0), rs);
+    if (first_res > 0) queue_recompute(s, first_res - 1, seg_len(first_res - 1), two ? ((first_res - 1) & 1) : 0, rs);
     for (int j = nck - 1; j >= 0; --j) {
         const int K = seg_len(j);
         const int t = two ? (j & 1) : 0;
-        HIPCHK(hipStreamSynchronize(rs));
-        launches += 3LL * K;
-        if (recompute_bad(s, K) > 0) {
-            rc = set_error(s, CHKN_ERR_STATE, "bad cells while recomputing a checkpoint segment");
-            break;
+        const bool resident = j >= first_res;
+        if (resident) {
+            A.blocks = g.d_rtab[j - first_res];
+        } else {
+            HIPCHK(hipStreamSynchronize(rs));
+            launches += 3LL * K;
+            if (recompute_bad(s, K) > 0) {
+                rc = set_error(s, CHKN_ERR_STATE, "bad cells while recomputing a checkpoint segment");
+                break;
+            }
+            A.blocks = g.d_tab[t];
         }
-        A.blocks = g.d_tab[t];
         for (int q = K - 1; q >= 0; --q) {
             const int step = j * C + q;
             A.dt = g.d_dts + (size_t)step * N;
@@ -431,11 +552,11 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
                 launches += 2;
             }
         }
-        if (j > 0) {
+        if (!resident && j > 0) {
             if (two) {
                 HIPCHK(hipEventRecord(g.ev_rev[t], s->stream));
-                // Tape t^1 was last read by the reversal of segment j+1.
-                if (j + 1 <= nck - 1) HIPCHK(hipStreamWaitEvent(rs, g.ev_rev[t ^ 1], 0));
+                // Tape t^1 was last read by the reversal of segment j+1 (if that was recomputed).
+                if (j + 1 < first_res) HIPCHK(hipStreamWaitEvent(rs, g.ev_rev[t ^ 1], 0));
             }
             queue_recompute(s, j - 1, seg_len(j - 1), two ? (t ^ 1) : 0, rs);
         }
@@ -487,6 +608,7 @@ void free_grad(chkn_sim* s)
         b.tape = b.tape2 = b.ckpt = b.adj = b.Gc = b.GB = nullptr;
     }
     if (s->grad) {
+        free_resident(s);
         dfree(s->grad->d_face_chunks); dfree(s->grad->d_cell_chunks);
         dfree(s->grad->d_inflow_cols); dfree(s->grad->d_inflow_acc); dfree(s->grad->d_dts);
         dfree(s->grad->d_tab[0]); dfree(s->grad->d_tab[1]);
@@ -517,6 +639,21 @@ int chkn_grad_backward(chkn_sim* sim, const double* lambda_dev, double* grad_u0_
 }
 
 long long chkn_state_size(chkn_sim* sim) { return sim ? (long long)flat_size(sim) : -1; }
+
+int chkn_grad_set_tape_budget(chkn_sim* sim, long long bytes)
+{
+    if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
+    return guarded(sim, [&] {
+        grad_state(sim).tape_budget = bytes < 0 ? -1 : bytes;
+        return CHKN_OK;
+    });
+}
+
+int chkn_grad_resident_segments(chkn_sim* sim)
+{
+    if (!sim) return -1;
+    return sim->grad ? sim->grad->nres : 0;
+}
 
 long long chkn_grad_generation(chkn_sim* sim)
 {

@@ -40,6 +40,12 @@ long long chkn_grad_generation(chkn_sim* sim)
     off(sim);
     return -1;
 }
+int chkn_grad_set_tape_budget(chkn_sim* sim, long long) { return off(sim); }
+int chkn_grad_resident_segments(chkn_sim* sim)
+{
+    off(sim);
+    return -1;
+}
 int chkn_num_flow_states(chkn_sim* sim) { return sim ? (int)sim->inflow_w.size() / 6 : -1; }
 int chkn_read_conserved_dev(chkn_sim* sim, double*) { return off(sim); }
 int chkn_write_conserved_dev(chkn_sim* sim, const double*) { return off(sim); }

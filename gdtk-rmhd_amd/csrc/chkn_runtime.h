@@ -70,6 +70,7 @@ struct HostBlock {
     // adjoint (allocated by the first gradient call)
     double* tape = nullptr;    // (3C+1) W slots of the segment being reversed
     double* tape2 = nullptr;   // second tape: the next segment is recomputed into it meanwhile
+    double* rtape = nullptr;   // resident tapes: nres_alloc segments of (3C+1) W slots each
     double* ckpt = nullptr;    // checkpoints: U then W, 12n doubles each
     double* adj = nullptr;     // A0, P, D: 18n doubles
     double* Gc = nullptr;      // per-cell direction sums of face stencil adjoints: 18 x n
@@ -116,6 +117,12 @@ struct GradState {
     hipStream_t rec = nullptr;
     hipEvent_t ev_rev[2] = {nullptr, nullptr};
     BlockDev* d_tab[2] = {nullptr, nullptr};  // block tables with W at tape / tape2
+    // Resident tapes: the last nres segments of the unroll write their stage states during
+    // the forward pass and are reversed without recomputation.  tape_budget: bytes they may
+    // take (-1: half of the free device memory at chkn_grad_forward, 0: none).
+    long long tape_budget = -1;
+    int nres = 0, nres_alloc = 0, rtape_C = 0;
+    std::vector<BlockDev*> d_rtab;  // block tables with W at each resident tape
     std::vector<int> inflow_col_state;  // flow-state index of each inflow column
 };
 
@@ -189,6 +196,10 @@ struct chkn_sim {
     std::vector<chkn::BlockDev> bdev;
     std::unique_ptr<chkn::rt::GradState> grad;
     std::vector<double>* dt_log = nullptr;  // when set, do_steps appends [step][inst] dts
+    // When set, the stages read and write their states in a tape (block table tape_tab, W
+    // slots from tape_base on) instead of the ring; do_steps advances tape_base.
+    const chkn::BlockDev* tape_tab = nullptr;
+    int tape_base = 0;
 };
 
 

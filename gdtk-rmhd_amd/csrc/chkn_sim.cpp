@@ -641,19 +641,25 @@ void launch_segment(chkn_sim* s, int K, bool dtred_last, bool fresh)
     for (int q = 0; q < K; ++q) {
         StageArgs a = stage_args(s, q);
         // Stages 1, 2, 3 (TVD-RK3) or 1, 4 (predictor-corrector) in a ring of W slots:
-        // each reads the last stage's output, the final stage writes slot 0.
+        // each reads the last stage's output, the final stage writes slot 0.  On a tape the
+        // step's states follow the previous step's.
         const int nst = stages_per_step(s);
+        if (s->tape_tab) a.blocks = s->tape_tab;
         for (int pos = 1; pos <= nst; ++pos) {
             const int stage = (s->time_scheme == CHKN_TIME_PC && pos == 2) ? 4 : pos;
             const bool last = pos == nst;
             a.win = pos - 1;
             a.wout = pos % nst;
+            if (s->tape_tab) {
+                a.win = s->tape_base + nst * q + pos - 1;
+                a.wout = s->tape_base + nst * q + pos;
+            }
             bool dtred = last && dtred_last && (q == K - 1);
             // With reacting instances the reduction follows the reaction.
             launch_stage_form(s, a, stage, dtred && !s->any_reacting, fresh && q == 0 && pos == 1,
                               s->stream);
             if (last && s->any_reacting)
-                HIPCHK(launch_chemistry(s->d_blocks, s->d_split_cells, s->nsplit_cells, a.ctl, a.wout,
+                HIPCHK(launch_chemistry(a.blocks, s->d_split_cells, s->nsplit_cells, a.ctl, a.wout,
                                         dtred, s->stream));
         }
     }
@@ -680,7 +686,8 @@ void run_segment(chkn_sim* s, int K, bool dtred_last)
     HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, s->ctl_bytes, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipEventRecord(s->ev0, s->stream));
     const bool fresh = fresh_needs_file_gas(s);
-    if (s->opts.use_graph && s->stream != nullptr) {  // the null stream cannot be captured
+    // The null stream cannot be captured; tape segments are launched directly.
+    if (s->opts.use_graph && s->stream != nullptr && !s->tape_tab) {
         GraphKey key{K, dtred_last, fresh};
         auto it = s->graphs.find(key);
         if (it == s->graphs.end()) {
@@ -866,6 +873,7 @@ int do_steps(chkn_sim* s, int nsteps, bool plot_writes, bool binary, chkn_report
         s->gstep += K;
         s->fresh = false;
         done += K;
+        if (s->tape_tab) s->tape_base += stages_per_step(s) * K;
         if (plot_writes)
             for (int qi = 0; qi < N; ++qi) {  // simulate.cu:311-334
                 Instance& I = s->inst[qi];

@@ -221,6 +221,14 @@ int chkn_boundary_face_areas(chkn_sim* sim, int blk, int side, double* out);
  * given, dt_in[step*ninst + inst]; those used are written to dt_out when it is not NULL. */
 int chkn_grad_forward(chkn_sim* sim, int nsteps, int checkpoint_every, const double* dt_in,
                       double* dt_out);
+/* Device memory the forward pass may fill with the stage states of its last checkpoint
+ * segments (resident tapes), which the backward pass then reverses without recomputing them:
+ * bytes < 0 (the default) allows half of the free device memory at chkn_grad_forward, 0
+ * recomputes every segment.  The gradient does not depend on it.  No counterpart in the
+ * reference, which has no adjoint. */
+int chkn_grad_set_tape_budget(chkn_sim* sim, long long bytes);
+/* Checkpoint segments of the last chkn_grad_forward held on resident tapes. */
+int chkn_grad_resident_segments(chkn_sim* sim);
 /* Number of successful chkn_grad_forward calls on the handle: chkn_grad_backward reverses the
  * latest one only, so a caller that holds several unrolls can tell whether its tape is still
  * the one on the handle. */

@@ -6,8 +6,9 @@
   differences (relative step 1e-7, time-step sequence frozen) to 1e-5 relative, and the
   derivative along a seeded random smooth direction of U0 to 1e-3 at step 1e-7 and 1e-5 at
   step 1e-9 (see _fd_rows_u0 for why the two);
-* the gradient is deterministic and does not depend on the checkpoint interval (bit for
-  bit), and an ensemble member's gradient equals that of the job run alone (bit for bit).
+* the gradient is deterministic and depends neither on the checkpoint interval nor on how
+  many segments the forward pass keeps taped (bit for bit), and an ensemble member's gradient
+  equals that of the job run alone (bit for bit).
 """
 import numpy as np
 import pytest
@@ -208,6 +209,24 @@ def test_gradient_is_deterministic_and_independent_of_checkpoint_interval():
     _, g3, m3 = prob.gradient(C=25)
     assert torch.equal(g1, g2) and torch.equal(m1[0], m2[0])
     assert torch.equal(g1, g3) and torch.equal(m1[0], m3[0])
+
+
+def test_resident_tapes_give_the_recomputed_gradient():
+    # The forward pass tapes the last checkpoint segments that fit the tape budget and the
+    # backward pass recomputes the others.  All seven segments resident, none, and three give
+    # the same gradient bit for bit: the taped states are the forward's own and the recomputed
+    # ones repeat its arithmetic.
+    job = _small_ramp()
+    prob = Problem([job], nsteps=25, C=4, mpt=[[5.0, 2.0e3, 220.0]])
+    seg = 6 * (12 * 4 * 6) * (3 * 4 + 1) * 8  # bytes of one segment's tape
+    out = []
+    for budget, nres in ((None, 7), (0, 0), (3 * seg + seg // 2, 3), (None, 7)):
+        prob.sim.set_tape_budget(budget)
+        L, g, m = prob.gradient()
+        assert prob.sim.grad_resident_segments == nres, budget
+        out.append((L, g, m[0]))
+    for L, g, m in out[1:]:
+        assert L == out[0][0] and torch.equal(g, out[0][1]) and torch.equal(m, out[0][2])
 
 
 def test_ensemble_member_gradient_equals_single_job():
