@@ -393,8 +393,7 @@ def roofline_record(jobs, opts, steps, warmup, workload, kernel_ms, launches, ce
 # stage kernel's default k-chunking; per block: one plane per work-group, without which a
 # small block's launch cannot fill the device).
 SUB_FUSED = [dict(tile_i=32, tile_j=8, tile_k=0), dict(tile_i=16, tile_j=16, tile_k=0),
-             dict(tile_i=32, tile_j=8, tile_k=4), dict(tile_i=32, tile_j=8, tile_k=0, tile_order=2),
-             dict(tile_i=16, tile_j=16, tile_k=0, tile_order=2)]
+             dict(tile_i=32, tile_j=8, tile_k=4)]
 SUB_PER_BLOCK = [dict(tile_i=32, tile_j=8, tile_k=1), dict(tile_i=16, tile_j=16, tile_k=1),
                  dict(tile_i=64, tile_j=4, tile_k=1)]
 

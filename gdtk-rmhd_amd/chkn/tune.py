@@ -2,7 +2,7 @@
 
 The stage kernel's speed depends on the tile shape (TI x TJ cells per work-group), the
 number of k-planes a work-group marches (tile_k), the order of the tile table and whether
-work-groups are persistent (or hold equal shares of the work: the balanced table).  None of these changes the results (the parity tests check
+work-groups are persistent.  None of these changes the results (the parity tests check
 that), so the tuner times each candidate on a scratch handle built from the same job(s),
 on the real block sizes, and returns the fastest.  Results are cached per problem shape.
 
@@ -35,9 +35,6 @@ def candidates(tiles=TILES, tile_k=TILE_K, orders=ORDERS, persistent=PERSISTENT,
                 for p in persistent:
                     out.append(dict(tile_i=ti, tile_j=tj, tile_k=tk, tile_order=o,
                                     persistent=bool(p)))
-    # The balanced persistent table (tile_order 2): tile_k and persistence are implied.
-    for ti, tj in tiles:
-        out.append(dict(tile_i=ti, tile_j=tj, tile_k=0, tile_order=2, persistent=True))
     if x_order > 1:
         for ti, tj in tiles:
             if (ti, tj) in VERTEX_FRAME_TILES:

@@ -305,9 +305,8 @@ __global__ __launch_bounds__(TI* TJ) void stage_kernel(StageArgs A, int persiste
     if (t1 > nt) t1 = nt;
     for (int tt = t0 + local; tt < t1; tt += per_xcd) {
         const Tile tile = A.tiles[A.tile_base + tt];
-        // Instances that have reached their end time skip the step (uniform per tile); the
-        // balanced table pads short work lists with empty tiles.
-        if (tile.nk > 0 && A.ctl.run[A.blocks[tile.blk].inst])
+        // Instances that have reached their end time skip the step (uniform per tile).
+        if (A.ctl.run[A.blocks[tile.blk].inst])
             process_tile<STAGE, TI, TJ, XORDER, DTRED, VF>(A, tile, lds);
         if (!persistent) break;
     }

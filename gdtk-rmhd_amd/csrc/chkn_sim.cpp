@@ -241,67 +241,9 @@ static std::vector<int> xcd_layer_lengths(int nkc, long long cols, long long slo
     return len;
 }
 
-// Balanced persistent tile table: one work-group per concurrent slot of the device (slots
-// per XCD x 8), each given an equal share (to one plane) of the plane-tiles of every active
-// block.  The (i, j) tile columns are taken in order -- block by block, j then i -- and cut
-// into k-chunks wherever a work-group's share ends, so a work-group holds one or a few chunks
-// and the work-groups of one XCD hold neighbouring columns.  Work-group (x, l) of XCD x takes
-// entries x*range + l + r*spx, r = 0, 1, ... (the persistent loop of the stage kernel);
-// work-groups with fewer chunks than the longest list are padded with empty entries.
-// Returns false (table untouched) when the job has fewer plane-tiles than slots.
-static bool balanced_tiles(chkn_sim* s, std::vector<Tile>& tiles)
+void build_tiles(chkn_sim* s)
 {
-    const int TI = s->opts.tile_i, TJ = s->opts.tile_j;
-    const int spx = (s->ncu / kXcds) * stage_slots_per_cu(TI, TJ, s->x_order);
-    if (spx <= 0) return false;
-    const long long S = (long long)kXcds * spx;
-    struct Col { int blk, i0, j0, nk; };
-    std::vector<Col> cols;
-    long long P = 0;
-    for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
-        const HostBlock& b = s->blocks[ib];
-        if (!b.active) continue;
-        for (int j0 = 0; j0 < b.njc; j0 += TJ)
-            for (int i0 = 0; i0 < b.nic; i0 += TI) {
-                cols.push_back(Col{(int)ib, i0, j0, b.nkc});
-                P += b.nkc;
-            }
-    }
-    if (P < S) return false;
-    std::vector<std::vector<Tile>> lists((size_t)S);
-    size_t c = 0;
-    int k0 = 0;
-    size_t R = 0;
-    for (long long q = 0; q < S; ++q) {
-        long long need = P / S + (q < P % S ? 1 : 0);
-        while (need > 0 && c < cols.size()) {
-            const int take = (int)std::min<long long>(need, cols[c].nk - k0);
-            lists[q].push_back(Tile{cols[c].blk, cols[c].i0, cols[c].j0, k0, take});
-            need -= take;
-            k0 += take;
-            if (k0 == cols[c].nk) {
-                ++c;
-                k0 = 0;
-            }
-        }
-        R = std::max(R, lists[q].size());
-    }
-    if (c != cols.size()) throw std::logic_error("balanced tile table does not cover the blocks");
-    const size_t range = R * (size_t)spx;
-    tiles.assign((size_t)kXcds * range, Tile{cols[0].blk, 0, 0, 0, 0});
-    for (long long q = 0; q < S; ++q) {
-        const size_t x = (size_t)(q / spx), l = (size_t)(q % spx);
-        for (size_t r = 0; r < lists[q].size(); ++r) tiles[x * range + l + r * spx] = lists[q][r];
-    }
-    s->grid_limit = (int)S;
-    return true;
-}
-
-// Tile table of the launch options: per block, (i, j) tile columns cut into k-chunks, in
-// plane-major order (tile_order 0, with XCD-balanced layers for a single block) or column-major
-// order (tile_order 1).
-static void column_tiles(chkn_sim* s, std::vector<Tile>& tiles)
-{
+    std::vector<Tile> tiles;
     const int TI = s->opts.tile_i, TJ = s->opts.tile_j;
     int nactive = 0;
     for (auto& o : s->blocks) nactive += o.active ? 1 : 0;
@@ -346,17 +288,6 @@ static void column_tiles(chkn_sim* s, std::vector<Tile>& tiles)
             }
         }
         b.tile_count = (int)tiles.size() - b.tile_first;
-    }
-}
-
-void build_tiles(chkn_sim* s)
-{
-    std::vector<Tile> tiles;
-    s->grid_limit = s->ncu * 2;
-    if (s->opts.tile_order == 2 && !s->opts.per_block_launch && balanced_tiles(s, tiles)) {
-        for (auto& b : s->blocks) b.tile_first = b.tile_count = 0;
-    } else {
-        column_tiles(s, tiles);
     }
     dfree(s->d_tiles);
     s->d_tiles = nullptr;
@@ -666,6 +597,7 @@ void setup(chkn_sim* s, const chkn_desc* descs, int ninst)
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     s->ncu = ncu;
+    s->grid_limit = ncu * 2;
     build_tiles(s);
     // SimState at start (simulate.cu:132-150, :190-191).
     for (int qi = 0; qi < ninst; ++qi) {
@@ -1274,7 +1206,7 @@ int chkn_set_launch_opts(chkn_sim* sim, const chkn_launch_opts* o)
     if (n.vertex_frames && !vertex_frames_supported(n.tile_i, n.tile_j, sim->x_order))
         return set_error(sim, CHKN_ERR_ARG, "vertex-frame stage kernel not built for this tile shape");
     if (n.tile_k < 0) return set_error(sim, CHKN_ERR_ARG, "tile_k must be >= 0");
-    if (n.tile_order < 0 || n.tile_order > 2) return set_error(sim, CHKN_ERR_ARG, "tile_order must be 0, 1 or 2");
+    if (n.tile_order != 0 && n.tile_order != 1) return set_error(sim, CHKN_ERR_ARG, "tile_order must be 0 or 1");
     return guarded(sim, [&] {
         sim->opts = n;
         build_tiles(sim);

@@ -318,21 +318,19 @@ void launch_stage_form(chkn_sim* s, const StageArgs& a, int stage, bool dtred, b
         return;
     }
     StageArgs b = a;
-    // The balanced tile table is laid out for persistent work-groups.
-    const bool persistent = s->opts.persistent != 0 || s->opts.tile_order == 2;
     if (s->opts.per_block_launch) {
         for (auto& hb : s->blocks) {
             if (!hb.active || hb.tile_count == 0) continue;
             b.tile_base = hb.tile_first;
             b.ntiles = hb.tile_count;
             HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, s->opts.vertex_frames,
-                                dtred, persistent, s->grid_limit, b, st));
+                                dtred, s->opts.persistent != 0, s->grid_limit, b, st));
         }
     } else {
         b.tile_base = 0;
         b.ntiles = s->ntiles;
         HIPCHK(launch_stage(stage, s->x_order, s->opts.tile_i, s->opts.tile_j, s->opts.vertex_frames,
-                            dtred, persistent, s->grid_limit, b, st));
+                            dtred, s->opts.persistent != 0, s->grid_limit, b, st));
     }
 }
 

@@ -114,10 +114,7 @@ typedef struct {
     int use_graph;        /* 1 = replay steps from a captured hipGraph */
     int persistent;       /* 1 = persistent work-groups loop over the tile table */
     int tile_order;       /* 0 = k-chunk layers outermost (plane-major tile table);
-                             1 = the k-chunks of one (i, j) column consecutive;
-                             2 = balanced: one persistent work-group per concurrent slot of
-                             the device, each with an equal share of the plane-tiles (columns
-                             cut into k-chunks where shares end; tile_k is not used) */
+                             1 = the k-chunks of one (i, j) column consecutive */
     int vertex_frames;    /* 1 = the stage kernel forms face frames from the vertices (3 words
                              per vertex read, more arithmetic) instead of reading the stored
                              10 words per face; built for 32x8 and 16x16 tiles, x_order 2,

@@ -90,17 +90,13 @@ def test_tuner_candidates_are_valid_launch_options():
     from chkn import tune
     c = tune.candidates()
     base = len(tune.TILES) * len(tune.TILE_K) * len(tune.ORDERS) * len(tune.PERSISTENT)
-    balanced = len(tune.TILES)
-    assert len(c) == base + balanced + len(tune.VERTEX_FRAME_TILES)
-    for o in c[:base + balanced]:
+    assert len(c) == base + len(tune.VERTEX_FRAME_TILES)
+    for o in c[:base]:
         assert set(o) == {"tile_i", "tile_j", "tile_k", "tile_order", "persistent"}
         assert o["tile_i"] * o["tile_j"] <= 1024
-    for o in c[base:base + balanced]:
-        assert o["tile_order"] == 2 and o["persistent"] and o["tile_k"] == 0
-    for o in c[base + balanced:]:
+    for o in c[base:]:
         assert o["vertex_frames"] and (o["tile_i"], o["tile_j"]) in tune.VERTEX_FRAME_TILES
-    # the vertex-frame form is second-order only
-    assert len(tune.candidates(x_order=1)) == base + balanced
+    assert len(tune.candidates(x_order=1)) == base  # the vertex-frame form is second-order only
 
 
 def test_block_size_limit_at_its_boundary():

@@ -81,7 +81,6 @@ def test_fields_match_oracle(name):
     dict(tile_i=64, tile_j=8, use_graph=True, per_block_launch=True),
     dict(tile_i=32, tile_j=8, vertex_frames=True),
     dict(tile_i=16, tile_j=16, tile_k=2, use_graph=True, vertex_frames=True),
-    dict(tile_i=8, tile_j=8, tile_order=2),  # fewer plane-tiles than slots: column table
 ])
 def test_launch_options_do_not_change_results(launch):
     compare(cases.ramp_array(seed=5, lo=4, hi=12, max_step=25), 25, launch=launch)
@@ -197,16 +196,14 @@ def test_scale_ramp_at_full_size_matches_oracle():
     # ramp, 1% seeded density perturbation), 11 steps: the CFL reduction at step 10 and the
     # step after it, on the bench's tuned launch shape.  Bit for bit against the oracle,
     # fields and time-step sequence.
-    # The fixed 4-plane k-chunks, the default layout (per-XCD slabs of 6, 5, 5 planes), vertex
-    # frames and the balanced persistent table.
+    # Both the fixed 4-plane k-chunks and the default layout (per-XCD slabs of 6, 5, 5 planes).
     job = cases.ramp(256, 128, 128, max_step=11)
     orc = ob.Oracle(job)
     rc, _, dts = orc.march(11)
     assert rc == 0 and orc.step == 11
     assert dts[10] != dts[9]  # the step after the reduction runs on the new dt
     for launch in (dict(tile_i=32, tile_j=8, tile_k=4, use_graph=True), dict(tile_i=32, tile_j=8, use_graph=True),
-                   dict(tile_i=32, tile_j=8, use_graph=True, vertex_frames=True),
-                   dict(tile_i=16, tile_j=16, tile_order=2, use_graph=True)):
+                   dict(tile_i=32, tile_j=8, use_graph=True, vertex_frames=True)):
         sim = Sim(job)
         sim.set_launch(**launch)
         rep = sim.step(11)
@@ -245,23 +242,18 @@ def test_batched_array_at_full_size_matches_oracle():
     # The batched workload at its real size: a 5x4x2 array of 40 uneven blocks (6.44M cells,
     # slab sizes seeded in [16, 96]) with its seeded exchange / wall connection graph, one
     # launch per stage; 11 steps (through the first CFL update) bit for bit against the oracle.
-    # Fixed 4-plane k-chunks, and the balanced persistent table (768 work-groups, each with
-    # an equal share of the plane-tiles, columns cut across work-groups).
     job = cases.ramp_array(seed=7, max_step=11)
+    sim = Sim(job)
+    sim.set_launch(tile_i=32, tile_j=8, tile_k=4, use_graph=True)
+    rep = sim.step(11)
     orc = ob.Oracle(job)
     rc, _, dts = orc.march(11)
-    assert rc == 0 and orc.step == 11
-    for launch in (dict(tile_i=32, tile_j=8, tile_k=4, use_graph=True),
-                   dict(tile_i=32, tile_j=8, tile_order=2, use_graph=True)):
-        sim = Sim(job)
-        sim.set_launch(**launch)
-        rep = sim.step(11)
-        assert rep.step == 11 and rep.t == orc.t and rep.dt == dts[10], launch
-        for b in range(len(job.blocks)):
-            assert np.array_equal(sim.iovars(b), orc.iovars(b)), (launch, b)
-            if job.blocks[b].active:
-                assert np.array_equal(sim.conserved(b), orc.conserved(b)), (launch, b)
-        sim.close()
+    assert rc == 0 and rep.step == orc.step == 11 and rep.t == orc.t and rep.dt == dts[10]
+    for b in range(len(job.blocks)):
+        assert np.array_equal(sim.iovars(b), orc.iovars(b)), b
+        if job.blocks[b].active:
+            assert np.array_equal(sim.conserved(b), orc.conserved(b)), b
+    sim.close()
 
 
 def test_ramp_slab_matches_oracle():
