@@ -155,9 +155,14 @@ void plan_resident(chkn_sim* s, int C, int nck)
     const size_t seg = segment_tape_bytes(s, C);
     size_t budget = 0;
     if (g.tape_budget < 0) {
-        size_t fr = 0, tot = 0;
+        // Half of the memory free or held by the unroll's tapes (resident and recompute ring),
+        // so that the budget does not shrink from one unroll to the next.
+        size_t fr = 0, tot = 0, ring = 0;
         HIPCHK(hipMemGetInfo(&fr, &tot));
-        budget = (fr + (size_t)g.nres_alloc * seg) / 2;
+        if (g.tape_C > 0)
+            for (auto& b : s->blocks)
+                if (b.active) ring += sizeof(double) * tape_doubles(b, g.tape_C) * ((b.tape ? 1 : 0) + (b.tape2 ? 1 : 0));
+        budget = (fr + (size_t)g.nres_alloc * seg + ring) / 2;
     } else {
         budget = (size_t)g.tape_budget;
     }
@@ -230,6 +235,15 @@ struct Taping {
 void alloc_adjoint(chkn_sim* s, int C, bool ring)
 {
     GradState& g = *s->grad;
+    if (!ring && g.tape_C > 0) {  // every segment is on a resident tape: no recompute ring
+        for (auto& b : s->blocks) {
+            dfree(b.tape);
+            dfree(b.tape2);
+            b.tape = b.tape2 = nullptr;
+        }
+        g.tape_C = 0;
+        g.two_tapes = false;
+    }
     for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
         HostBlock& b = s->blocks[ib];
         if (!b.active) continue;
