@@ -95,6 +95,7 @@ EXPORTS = {
     "chkn_grad_resident_segments": (ctypes.c_int, [ctypes.c_void_p]),
     "chkn_check_block_dims": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "chkn_device_bytes": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "chkn_source_digest": (ctypes.c_char_p, []),
     "chkn_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "chkn_destroy": (None, [ctypes.c_void_p]),
 }
@@ -121,6 +122,29 @@ def load(flavour="air"):
             fn.argtypes = args
         _libs[flavour] = lib
     return _libs[flavour]
+
+
+def tree_digest():
+    """The digest chkn_source_digest() of a library built from the sources in this tree:
+    SHA-256 over gdtk-rmhd_amd/csrc/*.h, *.hip and *.cpp in name order, then include/chkn.h
+    (the Makefile's SRC_DIGEST), first 16 hex digits."""
+    import glob
+    import hashlib
+    pkg = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    files = []
+    for pat in ("*.h", "*.hip", "*.cpp"):
+        files += glob.glob(os.path.join(pkg, "csrc", pat))
+    files = sorted(files, key=lambda f: "csrc/" + os.path.basename(f))
+    h = hashlib.sha256()
+    for f in files + [os.path.join(os.path.dirname(pkg), "include", "chkn.h")]:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def source_digest(flavour="air"):
+    """chkn_source_digest() of a built library flavour."""
+    return load(flavour).chkn_source_digest().decode()
 
 
 class ChknError(RuntimeError):

@@ -265,6 +265,12 @@ int chkn_check_block_dims(int nic, int njc, int nkc, int viscous);
  * end. */
 int chkn_set_verbosity(int level);
 
+/* Digest of the sources the library was built from (no counterpart in the reference): the
+ * first 16 hex digits of SHA-256 over the .h, .hip and .cpp files of gdtk-rmhd_amd/csrc in
+ * name order followed by this header, so that a caller can tell a library built from other
+ * sources. */
+const char* chkn_source_digest(void);
+
 /* Device memory allocated by the handle's set-up, in bytes (the counterpart of the
  * reference's "Bytes allocated on CPU", simulate.cu:89-90). */
 long long chkn_device_bytes(chkn_sim* sim);

@@ -131,3 +131,11 @@ def test_blocks_beyond_32_bit_offsets_are_rejected():
     rc = so.chkn_create(ctypes.byref(d), ctypes.byref(h))
     assert rc == -1
     assert so.chkn_last_error(None) == b"block too large for 32-bit array offsets"
+
+
+def test_libraries_are_built_from_the_sources_in_the_tree():
+    # Each flavour carries the digest of the sources it was compiled from; a library left over
+    # from other sources (a stale build beside newer sources) fails here.
+    want = lib.tree_digest()
+    for flavour in lib.LIB_PATHS:
+        assert lib.source_digest(flavour) == want, flavour
