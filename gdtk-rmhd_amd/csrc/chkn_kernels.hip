@@ -289,9 +289,15 @@ __device__ __forceinline__ void process_tile(const StageArgs& A, const Tile tile
 }
 
 template <int STAGE, int TI, int TJ, int XORDER, bool DTRED, bool VF>
-__global__ __launch_bounds__(TI* TJ) void stage_kernel(StageArgs A, int persistent)
+__global__ __launch_bounds__(TI* TJ) void stage_kernel(StageArgs A0, const BlockDev* __restrict__ blocks,
+                                                      const Tile* __restrict__ tiles, int persistent)
 {
     __shared__ real lds[TileLds<TI, TJ>::DOUBLES];
+    // The block and tile tables are read-only during the launch: passed as restrict-qualified
+    // arguments, their loads can be scalar loads that no store of the kernel can clobber.
+    StageArgs A = A0;
+    A.blocks = blocks;
+    A.tiles = tiles;
     // XCD-aware tile mapping: work-group b runs on XCD b % 8, and each XCD takes a
     // contiguous range of the (spatially ordered) tile table, so halo cells shared by
     // neighbouring tiles are re-read from the same L2.
@@ -323,17 +329,17 @@ hipError_t launch_shape(int stage, bool dtred, bool persistent, int grid_limit,
     dim3 blk(TI * TJ);
     int p = persistent ? 1 : 0;
     if (stage == 1)
-        hipLaunchKernelGGL((stage_kernel<1, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<1, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, a.blocks, a.tiles, p);
     else if (stage == 2)
-        hipLaunchKernelGGL((stage_kernel<2, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<2, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, a.blocks, a.tiles, p);
     else if (stage == 3 && dtred)
-        hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, true, VF>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, true, VF>), dim3(grid), blk, 0, s, a, a.blocks, a.tiles, p);
     else if (stage == 3)
-        hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<3, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, a.blocks, a.tiles, p);
     else if (dtred)
-        hipLaunchKernelGGL((stage_kernel<4, TI, TJ, XORDER, true, VF>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<4, TI, TJ, XORDER, true, VF>), dim3(grid), blk, 0, s, a, a.blocks, a.tiles, p);
     else
-        hipLaunchKernelGGL((stage_kernel<4, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, p);
+        hipLaunchKernelGGL((stage_kernel<4, TI, TJ, XORDER, false, VF>), dim3(grid), blk, 0, s, a, a.blocks, a.tiles, p);
     return hipGetLastError();
 }
 
