@@ -18,7 +18,8 @@
 //      accumulator (the gradient of the inflow state).
 // No atomics: every sum has one writer and a fixed order, so the gradient is deterministic.
 // Gradients need no bit-exact arithmetic, so this file is built with floating-point
-// contraction and reciprocal arithmetic allowed (the stage kernels are not).
+// contraction, reciprocal arithmetic and approximate (Newton-refined reciprocal) division
+// allowed (the stage kernels are not).
 //
 // Reversed step with A0 = adjoint of U^{n+1} on entry (seed: P = dt/6 A0, D = 4dt/6 A0):
 //   stage 3 (W2): A0 += U2bar; D = P + dt/4 U2bar; P = D
@@ -185,8 +186,13 @@ __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, in
 // position along the line, in directions 1 and 2 a line.
 // Three work-groups per CU fit the LDS (48 KB each), three waves per SIMD.
 template <int XORDER>
-__global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjArgs A)
+__global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjArgs Ain, const BlockDev* __restrict__ blocks,
+                                                                          const AdjPatch* __restrict__ patches)
 {
+    // Read-only tables as restrict-qualified arguments: scalar loads no store can clobber.
+    AdjArgs A = Ain;
+    A.blocks = blocks;
+    A.patches = patches;
     __shared__ double st[24][kAdjPatchM * kAdjPatchT];
     const int t = (int)threadIdx.x;
     const AdjPatch P = A.patches[blockIdx.x];
@@ -237,8 +243,12 @@ __global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjA
 }
 
 template <int STAGE>
-__global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
+__global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs Ain, const BlockDev* __restrict__ blocks,
+                                                         const CellChunk* __restrict__ cells)
 {
+    AdjArgs A = Ain;
+    A.blocks = blocks;
+    A.cells = cells;
     const CellChunk ch = A.cells[blockIdx.x];
     const BlockDev* B = A.blocks + ch.blk;
     const long long n = B->n;
@@ -331,8 +341,12 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs A)
     }
 }
 
-__global__ __launch_bounds__(256) void adj_seed_kernel(AdjArgs A)
+__global__ __launch_bounds__(256) void adj_seed_kernel(AdjArgs Ain, const BlockDev* __restrict__ blocks,
+                                                       const CellChunk* __restrict__ cells)
 {
+    AdjArgs A = Ain;
+    A.blocks = blocks;
+    A.cells = cells;
     const CellChunk ch = A.cells[blockIdx.x];
     const BlockDev* B = A.blocks + ch.blk;
     const long long n = B->n;
@@ -372,7 +386,7 @@ __global__ __launch_bounds__(64) void flux_vjp_points_kernel(const double* in, d
 hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s)
 {
     if (ncell_chunks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_seed_kernel, dim3(ncell_chunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(adj_seed_kernel, dim3(ncell_chunks), dim3(256), 0, s, a, a.blocks, a.cells);
     return hipGetLastError();
 }
 
@@ -380,17 +394,17 @@ hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, hipStre
 {
     if (npatches <= 0) return hipSuccess;
     const dim3 blk(kAdjPatchM * kAdjPatchT);
-    if (x_order > 1) hipLaunchKernelGGL(adj_patch_kernel<2>, dim3(npatches), blk, 0, s, a);
-    else hipLaunchKernelGGL(adj_patch_kernel<1>, dim3(npatches), blk, 0, s, a);
+    if (x_order > 1) hipLaunchKernelGGL(adj_patch_kernel<2>, dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
+    else hipLaunchKernelGGL(adj_patch_kernel<1>, dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
     return hipGetLastError();
 }
 
 hipError_t launch_adj_gather(const AdjArgs& a, int ncell_chunks, int stage, hipStream_t s)
 {
     if (ncell_chunks <= 0) return hipSuccess;
-    if (stage == 3) hipLaunchKernelGGL(adj_gather_kernel<3>, dim3(ncell_chunks), dim3(256), 0, s, a);
-    else if (stage == 2) hipLaunchKernelGGL(adj_gather_kernel<2>, dim3(ncell_chunks), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(adj_gather_kernel<1>, dim3(ncell_chunks), dim3(256), 0, s, a);
+    if (stage == 3) hipLaunchKernelGGL(adj_gather_kernel<3>, dim3(ncell_chunks), dim3(256), 0, s, a, a.blocks, a.cells);
+    else if (stage == 2) hipLaunchKernelGGL(adj_gather_kernel<2>, dim3(ncell_chunks), dim3(256), 0, s, a, a.blocks, a.cells);
+    else hipLaunchKernelGGL(adj_gather_kernel<1>, dim3(ncell_chunks), dim3(256), 0, s, a, a.blocks, a.cells);
     return hipGetLastError();
 }
 

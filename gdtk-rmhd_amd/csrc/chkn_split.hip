@@ -176,8 +176,13 @@ __device__ __forceinline__ bool is_wall(int code) { return code >= 0 && code <= 
 // the two reconstructed states; FLUX 1: SBP-ASF, face state the average of the two cells
 // beside the face (face.cu:373-376).
 template <int XORDER, bool VISC, int FLUX>
-__global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A)
+__global__ __launch_bounds__(kChunk) void split_face_kernel(SplitArgs A0, const BlockDev* __restrict__ blocks,
+                                                            const FaceChunk* __restrict__ faces)
 {
+    // Read-only tables as restrict-qualified arguments: scalar loads no store can clobber.
+    SplitArgs A = A0;
+    A.blocks = blocks;
+    A.faces = faces;
     const FaceChunk ch = A.faces[blockIdx.x];
     const BlockDev* B = A.blocks + ch.blk;
     const int f = ch.f0 + (int)threadIdx.x;
@@ -359,8 +364,12 @@ __device__ __forceinline__ void cloud_offset(const BlockDev* B, int dir, const F
 // table runs block, k, patch, direction, and work-group b (on XCD b % 8) takes the b/8-th
 // entry of XCD b%8's contiguous range, so the clouds of neighbouring patches, which share
 // most of their faces and cells, are read through one L2.
-__global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int npatches)
+__global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A0, const BlockDev* __restrict__ blocks,
+                                                               const FacePatch* __restrict__ patches, int npatches)
 {
+    SplitArgs A = A0;
+    A.blocks = blocks;
+    A.patches = patches;
     const int per_xcd = (npatches + 7) >> 3;
     const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
     if (item >= npatches) return;
@@ -462,8 +471,12 @@ __global__ __launch_bounds__(kChunk) void split_viscous_kernel(SplitArgs A, int 
 }
 
 template <int STAGE, bool DTRED>
-__global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A)
+__global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A0, const BlockDev* __restrict__ blocks,
+                                                              const CellChunk* __restrict__ cells)
 {
+    SplitArgs A = A0;
+    A.blocks = blocks;
+    A.cells = cells;
     const CellChunk ch = A.cells[blockIdx.x];
     const BlockDev* B = A.blocks + ch.blk;
     if (!A.ctl.run[B->inst]) return;  // uniform over the work-group
@@ -517,8 +530,8 @@ __global__ __launch_bounds__(kChunk) void split_update_kernel(SplitArgs A)
 // computed on the host for the instance's dt; then U = encode(state) (level 0, every cell of
 // a reacting instance).
 template <bool DTRED>
-__global__ __launch_bounds__(kChunk) void chemistry_kernel(const BlockDev* blocks,
-                                                           const CellChunk* cells, StepCtl ctl,
+__global__ __launch_bounds__(kChunk) void chemistry_kernel(const BlockDev* __restrict__ blocks,
+                                                           const CellChunk* __restrict__ cells, StepCtl ctl,
                                                            int slot)
 {
     const CellChunk ch = cells[blockIdx.x];
@@ -572,14 +585,14 @@ hipError_t launch_split_faces(const SplitArgs& a, int nface_chunks, int x_order,
     if (nface_chunks <= 0) return hipSuccess;
     const dim3 grid(nface_chunks), blk(kChunk);
     if (flux_calc == 1) {
-        if (viscous) hipLaunchKernelGGL((split_face_kernel<2, true, 1>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((split_face_kernel<2, false, 1>), grid, blk, 0, s, a);
+        if (viscous) hipLaunchKernelGGL((split_face_kernel<2, true, 1>), grid, blk, 0, s, a, a.blocks, a.faces);
+        else hipLaunchKernelGGL((split_face_kernel<2, false, 1>), grid, blk, 0, s, a, a.blocks, a.faces);
     } else if (x_order > 1) {
-        if (viscous) hipLaunchKernelGGL((split_face_kernel<2, true, 0>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((split_face_kernel<2, false, 0>), grid, blk, 0, s, a);
+        if (viscous) hipLaunchKernelGGL((split_face_kernel<2, true, 0>), grid, blk, 0, s, a, a.blocks, a.faces);
+        else hipLaunchKernelGGL((split_face_kernel<2, false, 0>), grid, blk, 0, s, a, a.blocks, a.faces);
     } else {
-        if (viscous) hipLaunchKernelGGL((split_face_kernel<1, true, 0>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((split_face_kernel<1, false, 0>), grid, blk, 0, s, a);
+        if (viscous) hipLaunchKernelGGL((split_face_kernel<1, true, 0>), grid, blk, 0, s, a, a.blocks, a.faces);
+        else hipLaunchKernelGGL((split_face_kernel<1, false, 0>), grid, blk, 0, s, a, a.blocks, a.faces);
     }
     return hipGetLastError();
 }
@@ -588,7 +601,7 @@ hipError_t launch_split_viscous(const SplitArgs& a, int npatches, hipStream_t s)
 {
     if (npatches <= 0) return hipSuccess;
     const int grid = 8 * ((npatches + 7) / 8);
-    hipLaunchKernelGGL(split_viscous_kernel, dim3(grid), dim3(kChunk), 0, s, a, npatches);
+    hipLaunchKernelGGL(split_viscous_kernel, dim3(grid), dim3(kChunk), 0, s, a, a.blocks, a.patches, npatches);
     return hipGetLastError();
 }
 
@@ -597,12 +610,12 @@ hipError_t launch_split_update(const SplitArgs& a, int ncell_chunks, int stage, 
 {
     if (ncell_chunks <= 0) return hipSuccess;
     const dim3 grid(ncell_chunks), blk(kChunk);
-    if (stage == 1) hipLaunchKernelGGL((split_update_kernel<1, false>), grid, blk, 0, s, a);
-    else if (stage == 2) hipLaunchKernelGGL((split_update_kernel<2, false>), grid, blk, 0, s, a);
-    else if (stage == 3 && dt_reduce) hipLaunchKernelGGL((split_update_kernel<3, true>), grid, blk, 0, s, a);
-    else if (stage == 3) hipLaunchKernelGGL((split_update_kernel<3, false>), grid, blk, 0, s, a);
-    else if (dt_reduce) hipLaunchKernelGGL((split_update_kernel<4, true>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((split_update_kernel<4, false>), grid, blk, 0, s, a);
+    if (stage == 1) hipLaunchKernelGGL((split_update_kernel<1, false>), grid, blk, 0, s, a, a.blocks, a.cells);
+    else if (stage == 2) hipLaunchKernelGGL((split_update_kernel<2, false>), grid, blk, 0, s, a, a.blocks, a.cells);
+    else if (stage == 3 && dt_reduce) hipLaunchKernelGGL((split_update_kernel<3, true>), grid, blk, 0, s, a, a.blocks, a.cells);
+    else if (stage == 3) hipLaunchKernelGGL((split_update_kernel<3, false>), grid, blk, 0, s, a, a.blocks, a.cells);
+    else if (dt_reduce) hipLaunchKernelGGL((split_update_kernel<4, true>), grid, blk, 0, s, a, a.blocks, a.cells);
+    else hipLaunchKernelGGL((split_update_kernel<4, false>), grid, blk, 0, s, a, a.blocks, a.cells);
     return hipGetLastError();
 }
 
