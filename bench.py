@@ -32,7 +32,7 @@ BYTES_PER_CELL_STEP_PC = 496
 BYTES_PER_CELL_STEP_VISCOUS = BYTES_PER_CELL_STEP + 3 * 96
 
 
-WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300, "diff": 500,
+WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300, "diff": 500, "diff-pc": 500,
                   "ensemble-diff": 300, "viscous": 300, "shard": 500, "pc": 500}
 # Steps of the reference CPU build timed for cpu_baseline: a sample of about 10-30 s on one
 # core (the viscous step runs about 6x slower there than the inviscid one).
@@ -81,6 +81,14 @@ def build_workload(name, steps_total):
                             "checkpoint every 50 steps (segments taped by the forward pass while "
                             "they fit in half the free device memory, recomputed otherwise), dt "
                             "sequence frozen, FP64"}
+    elif name == "diff-pc":
+        jobs = [cases.ramp(128, 64, 64, max_step=1 << 30)]
+        jobs[0].time_scheme = 1
+        desc = {"workload": "differentiation, predictor-corrector: one 128x64x64 block, Mach 5 ramp "
+                            "stepped by the two-stage predictor-corrector; forward + adjoint of the "
+                            "wall-pressure loss w.r.t. U0 and inflow (Mach, p, T), checkpoint every "
+                            "50 steps (segments taped while they fit in half the free device memory, "
+                            "recomputed otherwise), dt sequence frozen, FP64"}
     elif name == "ensemble-diff":
         jobs = cases.ramp_ensemble(seed=11, n=64, max_step=1 << 30)
         desc = {"workload": "batched differentiation: 64 independent Mach 3-8 ramp jobs (seed 11), "
@@ -305,7 +313,7 @@ def main():
     else:
         opts = {}
     opts = dict(opts, use_graph=bool(args.graph), per_block_launch=bool(args.per_block))
-    if args.workload in ("diff", "ensemble-diff"):
+    if args.workload in ("diff", "diff-pc", "ensemble-diff"):
         r = run_diff(jobs, opts, steps, args.warmup)
         wall = r["seconds_forward_taped"] + r["seconds_adjoint"]
         line = {"metric": "cell-updates/s (forward + adjoint)",

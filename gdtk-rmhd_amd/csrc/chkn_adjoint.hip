@@ -21,10 +21,13 @@
 // contraction, reciprocal arithmetic and approximate (Newton-refined reciprocal) division
 // allowed (the stage kernels are not).
 //
-// Reversed step with A0 = adjoint of U^{n+1} on entry (seed: P = dt/6 A0, D = 4dt/6 A0):
+// Reversed TVD-RK3 step with A0 = adjoint of U^{n+1} on entry (seed: P = dt/6 A0, D = 4dt/6 A0):
 //   stage 3 (W2): A0 += U2bar; D = P + dt/4 U2bar; P = D
 //   stage 2 (W1): A0 += U1bar; D = P + dt U1bar
 //   stage 1 (W0): A0 += U0bar        -> A0 = adjoint of U^n; seed of the next reversed step
+// Reversed predictor-corrector step (U1 = U0 + dt L0, U = U0 + dt/2 (L0 + L1); seed: P = D =
+// dt/2 A0): the corrector (W1) as stage 2 above, A0 += U1bar, D = P + dt U1bar; the
+// predictor (W0) as stage 1.
 #include <mutex>
 #include <vector>
 
@@ -335,8 +338,8 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs Ain, const Bloc
             gst(D, o, gld(Pp, o) + dt * Ub[q]);
         } else if (A.dt_next) {
             const double dtn = A.dt_next[B->inst];
-            gst(Pp, o, kOneSixth * dtn * a0);
-            gst(D, o, kFourSixth * dtn * a0);
+            gst(Pp, o, A.seed_p * dtn * a0);
+            gst(D, o, A.seed_d * dtn * a0);
         }
     }
 }
@@ -359,8 +362,8 @@ __global__ __launch_bounds__(256) void adj_seed_kernel(AdjArgs Ain, const BlockD
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const long long o = (long long)q * n + cidx;
-        Pp[o] = kOneSixth * dt * A0[o];
-        D[o] = kFourSixth * dt * A0[o];
+        Pp[o] = A.seed_p * dt * A0[o];
+        D[o] = A.seed_d * dt * A0[o];
     }
 }
 

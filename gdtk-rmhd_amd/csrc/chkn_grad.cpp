@@ -125,12 +125,17 @@ void alloc_checkpoints(chkn_sim* s, int nckpt)
 
 // W slots (doubles) of one segment's tape of block b, and bytes of one segment's tapes over
 // the active blocks.
-size_t tape_doubles(const HostBlock& b, int C) { return 6 * (size_t)b.n * (3 * (size_t)C + 1); }
+// A tape holds every stage state of its C steps and the state after them: stages_per_step * C
+// + 1 W slots.
+size_t tape_doubles(const chkn_sim* s, const HostBlock& b, int C)
+{
+    return 6 * (size_t)b.n * ((size_t)stages_per_step(s) * C + 1);
+}
 size_t segment_tape_bytes(const chkn_sim* s, int C)
 {
     size_t n = 0;
     for (auto& b : s->blocks)
-        if (b.active) n += sizeof(double) * tape_doubles(b, C);
+        if (b.active) n += sizeof(double) * tape_doubles(s, b, C);
     return n;
 }
 
@@ -161,7 +166,7 @@ void plan_resident(chkn_sim* s, int C, int nck)
         HIPCHK(hipMemGetInfo(&fr, &tot));
         if (g.tape_C > 0)
             for (auto& b : s->blocks)
-                if (b.active) ring += sizeof(double) * tape_doubles(b, g.tape_C) * ((b.tape ? 1 : 0) + (b.tape2 ? 1 : 0));
+                if (b.active) ring += sizeof(double) * tape_doubles(s, b, g.tape_C) * ((b.tape ? 1 : 0) + (b.tape2 ? 1 : 0));
         budget = (fr + (size_t)g.nres_alloc * seg + ring) / 2;
     } else {
         budget = (size_t)g.tape_budget;
@@ -171,7 +176,7 @@ void plan_resident(chkn_sim* s, int C, int nck)
         free_resident(s);
         for (auto& b : s->blocks) {
             if (!b.active || want == 0) continue;
-            if (hipMalloc(&b.rtape, sizeof(double) * tape_doubles(b, C) * want) != hipSuccess) {
+            if (hipMalloc(&b.rtape, sizeof(double) * tape_doubles(s, b, C) * want) != hipSuccess) {
                 (void)hipGetLastError();
                 free_resident(s);
                 want = 0;
@@ -188,7 +193,10 @@ void plan_resident(chkn_sim* s, int C, int nck)
 }
 
 // W of resident tape r of block b.
-double* resident_tape(const HostBlock& b, int C, int r) { return b.rtape + (size_t)r * tape_doubles(b, C); }
+double* resident_tape(const chkn_sim* s, const HostBlock& b, int C, int r)
+{
+    return b.rtape + (size_t)r * tape_doubles(s, b, C);
+}
 
 // Block tables with W at each resident tape, with the current adjoint pointers.
 void upload_resident_tables(chkn_sim* s)
@@ -197,7 +205,7 @@ void upload_resident_tables(chkn_sim* s)
     for (int r = 0; r < g.nres; ++r) {
         std::vector<BlockDev> tab = s->bdev;
         for (size_t ib = 0; ib < s->blocks.size(); ++ib)
-            if (s->blocks[ib].active) tab[ib].W = resident_tape(s->blocks[ib], g.rtape_C, r);
+            if (s->blocks[ib].active) tab[ib].W = resident_tape(s, s->blocks[ib], g.rtape_C, r);
         HIPCHK(hipMemcpy(g.d_rtab[r], tab.data(), sizeof(BlockDev) * tab.size(), hipMemcpyHostToDevice));
     }
 }
@@ -212,7 +220,7 @@ struct Taping {
     {
         for (auto& b : s->blocks)
             if (b.active)
-                HIPCHK(hipMemcpyAsync(resident_tape(b, C, r), b.W, sizeof(double) * 6 * (size_t)b.n,
+                HIPCHK(hipMemcpyAsync(resident_tape(s, b, C, r), b.W, sizeof(double) * 6 * (size_t)b.n,
                                       hipMemcpyDeviceToDevice, s->stream));
         s->tape_tab = s->grad->d_rtab[r];
         s->tape_base = 0;
@@ -222,7 +230,7 @@ struct Taping {
         const size_t last = (size_t)stages_per_step(s) * K;
         for (auto& b : s->blocks)
             if (b.active)
-                HIPCHK(hipMemcpyAsync(b.W, resident_tape(b, C, r) + 6 * (size_t)b.n * last,
+                HIPCHK(hipMemcpyAsync(b.W, resident_tape(s, b, C, r) + 6 * (size_t)b.n * last,
                                       sizeof(double) * 6 * (size_t)b.n, hipMemcpyDeviceToDevice, s->stream));
     }
     ~Taping()
@@ -251,7 +259,7 @@ void alloc_adjoint(chkn_sim* s, int C, bool ring)
             dfree(b.tape);
             dfree(b.tape2);
             b.tape = b.tape2 = nullptr;
-            HIPCHK(hipMalloc(&b.tape, sizeof(double) * 6 * (size_t)b.n * (3 * (size_t)C + 1)));
+            HIPCHK(hipMalloc(&b.tape, sizeof(double) * tape_doubles(s, b, C)));
         }
         if (!b.adj) {
             // Ghost lines of the six sides, two depths of six values each.
@@ -270,7 +278,7 @@ void alloc_adjoint(chkn_sim* s, int C, bool ring)
         g.two_tapes = true;
         for (auto& b : s->blocks) {
             if (!b.active) continue;
-            if (hipMalloc(&b.tape2, sizeof(double) * 6 * (size_t)b.n * (3 * (size_t)C + 1)) != hipSuccess) {
+            if (hipMalloc(&b.tape2, sizeof(double) * tape_doubles(s, b, C)) != hipSuccess) {
                 (void)hipGetLastError();
                 b.tape2 = nullptr;
                 g.two_tapes = false;
@@ -308,7 +316,7 @@ void upload_tape_tables(chkn_sim* s)
 }
 
 // Queues the recomputation of checkpoint segment j (K steps) into tape t on stream st:
-// checkpoint load, control area, 3K stage launches, control area read-back.  The caller
+// checkpoint load, control area, the K steps' stage launches, control area read-back.  The caller
 // synchronises st before reading the bad-cell counters (recompute_bad).
 void queue_recompute(chkn_sim* s, int j, int K, int t, hipStream_t st)
 {
@@ -334,10 +342,11 @@ void queue_recompute(chkn_sim* s, int j, int K, int t, hipStream_t st)
         a.blocks = g.d_tab[t];
         a.tile_base = 0;
         a.ntiles = s->ntiles;
-        for (int stage = 1; stage <= 3; ++stage) {
-            a.win = 3 * q + stage - 1;
-            a.wout = 3 * q + stage;
-            launch_stage_form(s, a, stage, false, false, st);
+        const int nst = stages_per_step(s);
+        for (int pos = 1; pos <= nst; ++pos) {
+            a.win = nst * q + pos - 1;
+            a.wout = nst * q + pos;
+            launch_stage_form(s, a, (s->time_scheme == CHKN_TIME_PC && pos == 2) ? 4 : pos, false, false, st);
         }
     }
     HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, s->ctl_bytes, hipMemcpyDeviceToHost, st));
@@ -409,8 +418,6 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
     if (s->viscous) return set_error(s, CHKN_ERR_ARG, "the adjoint of the viscous fluxes is not built");
     if (s->flux_calc != 0) return set_error(s, CHKN_ERR_ARG, "the adjoint of the SBP-ASF flux is not built");
     if (s->any_reacting) return set_error(s, CHKN_ERR_ARG, "the adjoint of the A->B reaction is not built");
-    if (s->time_scheme != CHKN_TIME_RK3)
-        return set_error(s, CHKN_ERR_ARG, "the adjoint is built for the TVD-RK3 step only");
     // The unroll is a function of the conserved state: gas states follow from it, also in
     // the first stage (the recomputation from checkpoints does the same).
     s->fresh = false;
@@ -527,6 +534,9 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     A.icols = (const InflowCol*)g.d_inflow_cols;
     A.inflow_acc = g.d_inflow_acc;
     A.nicols = g.ninflow_cols;
+    // Seeds of a reversed step: the weights of the stages' residuals in U^{n+1}.
+    A.seed_p = s->time_scheme == CHKN_TIME_PC ? 0.5 : kOneSixth;
+    A.seed_d = s->time_scheme == CHKN_TIME_PC ? 0.5 : kFourSixth;
     long long launches = 0;
     int rc = CHKN_OK;
     // Segments below first_res are recomputed.  With two tapes, segment j-1 is recomputed on
@@ -544,7 +554,7 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
             A.blocks = g.d_rtab[j - first_res];
         } else {
             HIPCHK(hipStreamSynchronize(rs));
-            launches += 3LL * K;
+            launches += (long long)stages_per_step(s) * K;
             if (recompute_bad(s, K) > 0) {
                 rc = set_error(s, CHKN_ERR_STATE, "bad cells while recomputing a checkpoint segment");
                 break;
@@ -559,10 +569,13 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
                 HIPCHK(launch_adj_seed(A, g.ncell_chunks, s->stream));
                 launches += 1;
             }
-            for (int stage = 3; stage >= 1; --stage) {
-                A.slot = 3 * q + stage - 1;
+            // Stage positions in reverse; the predictor-corrector's corrector (position 2)
+            // gathers as the TVD-RK3 stage 2 (D = P + dt Ubar), its predictor as stage 1.
+            const int nst = stages_per_step(s);
+            for (int pos = nst; pos >= 1; --pos) {
+                A.slot = nst * q + pos - 1;
                 HIPCHK(launch_adj_faces(A, g.nface_chunks, s->x_order, s->stream));
-                HIPCHK(launch_adj_gather(A, g.ncell_chunks, stage, s->stream));
+                HIPCHK(launch_adj_gather(A, g.ncell_chunks, pos, s->stream));
                 launches += 2;
             }
         }

@@ -141,13 +141,16 @@ struct AdjArgs {
     int slot;                 // W slot of the stage state W_s
     const double* dt;         // [ninst] time step of the step being reversed
     const double* dt_next;    // [ninst] time step of the step reversed next, or null (stage 1)
+    double seed_p, seed_d;    // a reversed step starts from P = seed_p dt A0, D = seed_d dt A0:
+                              // 1/6 and 4/6 (TVD-RK3), 1/2 and 1/2 (predictor-corrector)
 };
 
 // Adjoint of one stage's residual: face pass (per patch, every face's stencil adjoints summed
 // per owned cell into Gc, ghost-line adjoints into GB), then the per-cell gather with the
-// stage's RK combination (stage 3, 2, 1 in reverse order) and
-// the inflow lines; the stage-1 gather also seeds the next reversed step when dt_next is
-// given.  seed: P = dt/6 A0, D = dt 4/6 A0 (start of the first reversed step).
+// stage's combination, in reverse order -- TVD-RK3 stages 3, 2, 1; predictor-corrector the
+// corrector (gathered as stage 2: D = P + dt Ubar) then the predictor (stage 1) -- and the
+// inflow lines; the stage-1 gather also seeds the next reversed step when dt_next is given.
+// seed: P = seed_p dt A0, D = seed_d dt A0 (start of the first reversed step).
 hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s);
 hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, hipStream_t s);
 hipError_t launch_adj_gather(const AdjArgs& a, int ncell_chunks, int stage, hipStream_t s);

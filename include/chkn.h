@@ -218,7 +218,9 @@ int chkn_get_inflow_state(chkn_sim* sim, int state, double* w6);
 int chkn_boundary_face_areas(chkn_sim* sim, int blk, int side, double* out);
 /* Advance nsteps steps keeping a checkpoint every checkpoint_every (1..64) steps (max_step
  * and max_time do not stop it).  The time steps follow the CFL schedule (dt_in NULL) or are
- * given, dt_in[step*ninst + inst]; those used are written to dt_out when it is not NULL. */
+ * given, dt_in[step*ninst + inst]; those used are written to dt_out when it is not NULL.
+ * Either time scheme (TVD-RK3 or predictor-corrector); inviscid AUSMDV jobs without the
+ * reaction. */
 int chkn_grad_forward(chkn_sim* sim, int nsteps, int checkpoint_every, const double* dt_in,
                       double* dt_out);
 /* Device memory the forward pass may fill with the stage states of its last checkpoint

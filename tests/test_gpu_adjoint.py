@@ -254,6 +254,53 @@ def test_first_order_gradient_matches_finite_differences():
     _assert_rows(rows + _fd_rows_u0(short, gu0s))
 
 
+def _pc(job):
+    job.time_scheme = 1  # the two-stage predictor-corrector
+    return job
+
+
+def test_predictor_corrector_gradient_matches_finite_differences():
+    # The adjoint of U1 = U0 + dt L0, U = U0 + dt/2 (L0 + L1): the ramp and the mixed-BC array
+    # (exchange, walls, outflow, an inactive block), against central differences of the
+    # predictor-corrector forward (which equals its oracle bit for bit).
+    job = _pc(_small_ramp())
+    prob = Problem([job], nsteps=40, C=7, mpt=[[5.0, 2.0e3, 220.0]])
+    L, _, gm = prob.gradient()
+    assert abs(prob.value() - L) <= 1e-15 * abs(L)
+    rows = _fd_rows_inflow(prob, gm)
+    short = Problem([job], nsteps=4, C=3, mpt=[[5.0, 2.0e3, 220.0]])
+    _, gu0s, _ = short.gradient()
+    rows += _fd_rows_u0(short, gu0s)
+    cases.mixed_bc_array(max_step=40)
+    mixed = _pc(_vary_state(JobData.from_prep(), 3, 808.0))
+    pm = Problem([mixed], nsteps=12, C=5, mpt=[[2.5, 5.0e3, 260.0]])
+    _, _, gmm = pm.gradient()
+    _assert_rows(rows + _fd_rows_inflow(pm, gmm))
+
+
+def test_predictor_corrector_gradient_is_independent_of_taping():
+    # Two stage states per step on a tape (2C + 1 slots): every segment resident, none, and
+    # three, and two checkpoint intervals give the same gradient bit for bit.
+    job = _pc(_small_ramp())
+    prob = Problem([job], nsteps=25, C=4, mpt=[[5.0, 2.0e3, 220.0]])
+    seg = 6 * (12 * 4 * 6) * (2 * 4 + 1) * 8  # bytes of one segment's tape
+    out = []
+    for budget, nres in ((None, 7), (0, 0), (3 * seg + seg // 2, 3)):
+        prob.sim.set_tape_budget(budget)
+        L, g, m = prob.gradient()
+        assert prob.sim.grad_resident_segments == nres, budget
+        out.append((L, g, m[0]))
+    prob.sim.set_tape_budget(0)
+    L, g, m = prob.gradient(C=9)
+    out.append((L, g, m[0]))
+    for L, g, m in out[1:]:
+        assert L == out[0][0] and torch.equal(g, out[0][1]) and torch.equal(m, out[0][2])
+    # Not the RK3 gradient: the scheme is the predictor-corrector's.
+    rk = Problem([_small_ramp()], nsteps=25, C=4, mpt=[[5.0, 2.0e3, 220.0]])
+    _, grk, _ = rk.gradient()
+    assert not torch.equal(grk, out[0][1])
+
+
 def test_work_on_the_callers_stream_gives_identical_results():
     # chkn_set_stream: a torch side stream, and the null stream with hipGraph replay requested
     # (it runs without capture there), give the same fields as the handle's own stream.

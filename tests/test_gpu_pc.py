@@ -14,7 +14,7 @@ import pytest
 import oracle_bind as ob
 from chkn import cases, prep
 from chkn.job import JobData
-from chkn.lib import BadCellError, ChknError, Sim
+from chkn.lib import BadCellError, Sim
 
 pytestmark = pytest.mark.gpu
 
@@ -115,13 +115,6 @@ def test_predictor_corrector_bad_cell_report():
         sim.step(5)
     rc, bad, _ = ob.Oracle(job).march(5)
     assert rc // 100 == ei.value.stage and bad == ei.value.count
-    sim.close()
-
-
-def test_predictor_corrector_gradient_is_rejected():
-    sim = Sim(pc(cases.ramp(8, 4, 4, max_step=100)))
-    with pytest.raises(ChknError, match="TVD-RK3"):
-        sim.grad_forward(2, 2)
     sim.close()
 
 
