@@ -349,10 +349,11 @@ def bl_plate(nics=(10, 8), njc=3, nkc=10, max_step=30, seed=4, x_order=2, twall_
     return JobData.from_prep(bl_profile=prof), prof
 
 
-def sbp_channel(nics=(10, 8), njc=6, nkc=5, max_step=25, seed=3, flux_calc="sbp_asf"):
+def sbp_channel(nics=(10, 8), njc=6, nkc=5, max_step=25, seed=3, flux_calc="sbp_asf", k_walls=False):
     """Smooth subsonic flow through a periodic channel on two warped blocks, for the
     central SBP-ASF flux (face.cu:268-342): inflow at iminus, outflow at iplus, periodic
-    exchange in j and k, a smooth density, pressure and velocity wave."""
+    exchange in j and k (slip walls in k with k_walls), a smooth density, pressure and
+    velocity wave."""
     _common(max_step, title="SBP-ASF channel (seed %d)" % seed, cfl=0.4, dt_init=2.0e-7)
     prep.config.flux_calc = flux_calc
     NI = sum(nics)
@@ -375,7 +376,8 @@ def sbp_channel(nics=(10, 8), njc=6, nkc=5, max_step=25, seed=3, flux_calc="sbp_
         bcs = {"iminus": prep.InflowBC(fs) if i == 0 else prep.ExchangeBC(),
                "iplus": prep.OutflowBC() if i == 1 else prep.ExchangeBC(),
                "jminus": prep.ExchangeBC(), "jplus": prep.ExchangeBC(),
-               "kminus": prep.ExchangeBC(), "kplus": prep.ExchangeBC()}
+               "kminus": prep.WallWithSlipBC() if k_walls else prep.ExchangeBC(),
+               "kplus": prep.WallWithSlipBC() if k_walls else prep.ExchangeBC()}
         sub = g.subgrid(i0=io[i], j0=0, k0=0, niv=nics[i] + 1, njv=njc + 1, nkv=nkc + 1)
         init = {key: val[io[i]:io[i + 1]] if np.ndim(val) == 3 else val for key, val in st.items()}
         prep.FluidBlock(i=i, j=0, k=0, grid=sub, initialState=init, bcs=bcs)

@@ -198,6 +198,101 @@ CHKN_HD void ausmdv_vjp(double rhoL, double eL, double pL, double aL, double YBL
     bR[5] += gx; bR[6] += gy; bR[7] += gz;
 }
 
+// Adjoint of the SBP-ASF flux (sbp_asf, face.cu:268-342) with each stencil state's p from the
+// equation of state of its (rho, e, YB): accumulates the adjoints of the four stencil states
+// (rho, e, YB, vx, vy, vz) into b[0..3][6].  The flux is a sum of products of the per-cell
+// factors v_j = (rho, u rho, v rho, w rho, e rho, u^2 rho, v^2 rho, w^2 rho, p, p) and
+// w_j = (u, ..., u, 1) in the face frame, so its adjoint runs back through the central (fc)
+// and product-rule (fe) combinations to the factors, then to each state.
+CHKN_HD void sbp_asf_vjp(const Prim s[4], const Frame& f, const double Fb[6], double b[4][6])
+{
+    constexpr double c = 1.0 / 12.0;
+    double u[4], v[4], w[4], T[4], p[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        to_local(f, s[i].vx, s[i].vy, s[i].vz, u[i], v[i], w[i]);
+        T[i] = eos_T(s[i].e, s[i].YB);
+        p[i] = eos_p(s[i].rho, T[i]);
+    }
+    // Factors of product j at cell i (forward values).
+    auto fv = [&](int j, int i) -> double {
+        const double rho = s[i].rho;
+        switch (j) {
+            case 0: return rho;
+            case 1: return u[i] * rho;
+            case 2: return v[i] * rho;
+            case 3: return w[i] * rho;
+            case 4: return s[i].e * rho;
+            case 5: return u[i] * u[i] * rho;
+            case 6: return v[i] * v[i] * rho;
+            case 7: return w[i] * w[i] * rho;
+            default: return p[i];
+        }
+    };
+    auto fw = [&](int j, int i) -> double { return j == 9 ? 1.0 : u[i]; };
+    const double mass = c * (-fv(0, 0) * fw(0, 0) + 7.0 * fv(0, 1) * fw(0, 1) + 7.0 * fv(0, 2) * fw(0, 2) -
+                             fv(0, 3) * fw(0, 3));
+    const bool up = mass >= 0.0;
+    // Adjoints of the blended sums: mass = fc0; (mx, my, mz) rotated back by to_global (its
+    // transpose is to_local); energy and the pressure terms with the alphas of sbp_asf.
+    double bmx, bmy, bmz;
+    to_local(f, Fb[1], Fb[2], Fb[3], bmx, bmy, bmz);
+    double bfc[10], bfe[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) bfc[j] = bfe[j] = 0.0;
+    bfc[0] = Fb[0] + Fb[5] * (up ? s[1].YB : s[2].YB);
+    bfc[1] = 0.5 * bmx; bfe[1] = 0.5 * bmx; bfe[9] = bmx;
+    bfc[2] = 0.5 * bmy; bfe[2] = 0.5 * bmy;
+    bfc[3] = 0.5 * bmz; bfe[3] = 0.5 * bmz;
+    bfc[4] = 0.5 * Fb[4]; bfe[4] = 0.5 * Fb[4];
+    bfe[5] = 0.5 * Fb[4]; bfe[6] = 0.5 * Fb[4]; bfe[7] = 0.5 * Fb[4];
+    bfe[8] = Fb[4];
+    // Adjoints of the factors: fc_j = c(-v0 w0 + 7 v1 w1 + 7 v2 w2 - v3 w3),
+    // fe_j = c(-v0 w2 - v2 w0 + 8 v1 w2 + 8 v2 w1 - v1 w3 - v3 w1).
+    double bv[10][4], bwu[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+        const double gc = c * bfc[j], ge = c * bfe[j];
+        const double v0 = fv(j, 0), v1 = fv(j, 1), v2 = fv(j, 2), v3 = fv(j, 3);
+        const double w0 = fw(j, 0), w1 = fw(j, 1), w2 = fw(j, 2), w3 = fw(j, 3);
+        bv[j][0] = -gc * w0 - ge * w2;
+        bv[j][1] = 7.0 * gc * w1 + ge * (8.0 * w2 - w3);
+        bv[j][2] = 7.0 * gc * w2 + ge * (8.0 * w1 - w0);
+        bv[j][3] = -gc * w3 - ge * w1;
+        if (j < 9) {  // w_j = u of each cell (w_9 = 1 is constant)
+            bwu[0] += -gc * v0 - ge * v2;
+            bwu[1] += 7.0 * gc * v1 + ge * (8.0 * v2 - v3);
+            bwu[2] += 7.0 * gc * v2 + ge * (8.0 * v1 - v0);
+            bwu[3] += -gc * v3 - ge * v1;
+        }
+    }
+    // Factors -> (rho, e, p, u, v, w) of each cell -> its primitive state.
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double rho = s[i].rho;
+        double brho = bv[0][i] + bv[1][i] * u[i] + bv[2][i] * v[i] + bv[3][i] * w[i] + bv[4][i] * s[i].e +
+                      bv[5][i] * u[i] * u[i] + bv[6][i] * v[i] * v[i] + bv[7][i] * w[i] * w[i];
+        const double bu = bv[1][i] * rho + 2.0 * bv[5][i] * u[i] * rho + bwu[i];
+        const double bvv = bv[2][i] * rho + 2.0 * bv[6][i] * v[i] * rho;
+        const double bww = bv[3][i] * rho + 2.0 * bv[7][i] * w[i] * rho;
+        double be = bv[4][i] * rho;
+        const double bp = bv[8][i] + bv[9][i];
+        // p = rho R T, T = (e + YB Q) / Cv
+        brho += bp * kR * T[i];
+        const double bT = bp * rho * kR;
+        be += bT / kCv;
+        double bx, by, bz;
+        to_global(f, bu, bvv, bww, bx, by, bz);
+        b[i][0] += brho;
+        b[i][1] += be;
+        b[i][2] += bT * kQ / kCv;
+        b[i][3] += bx;
+        b[i][4] += by;
+        b[i][5] += bz;
+    }
+    b[up ? 1 : 2][2] += Fb[5] * mass;
+}
+
 // Adjoint of face_flux<XORDER>: accumulates the adjoints of the four stencil states
 // (rho, e, YB, vx, vy, vz each) into b[0..3][6] (L1, L0, R0, R1).
 template <int XORDER>

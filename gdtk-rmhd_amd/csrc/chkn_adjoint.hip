@@ -85,7 +85,8 @@ __device__ __forceinline__ double* ghost_buffer(const BlockDev* B, int side)
 // The stencil adjoints of the face at line position m (between cells m-1 and m) of the line
 // (a, c) in direction d, left in column t of st: st[6p+q][t] = adjoint of stencil slot p
 // (L1, L0, R0, R1), quantity q.  Zero for faces outside the block.
-template <int XORDER>
+// FLUX 0: AUSMDV on MUSCL (XORDER 2) or adjacent states; FLUX 1: the SBP-ASF flux.
+template <int XORDER, int FLUX>
 __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, int d, int m, int a,
                                          int c, double (&st)[24][kAdjPatchM * kAdjPatchT], int t)
 {
@@ -135,6 +136,24 @@ __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, in
     // Compiler fence: the stencil values are read back from LDS where they are needed rather
     // than forwarded from the stores above and held in registers through the flux adjoint.
     __asm__ __volatile__("" ::: "memory");
+    if (FLUX == 1) {
+        Prim sp[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+            sp[p] = Prim{st[6 * p][t], st[6 * p + 1][t], st[6 * p + 2][t], st[6 * p + 3][t], st[6 * p + 4][t],
+                         st[6 * p + 5][t]};
+        double bs[4][6];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) bs[p][q] = 0.0;
+        sbp_asf_vjp(sp, ld_frame(g, nf, fi), Fb, bs);
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) st[6 * p + q][t] = bs[p][q];
+        return;
+    }
     // Reconstructed face states (face.cu:347-371).
     double qL[6], qR[6];
 #pragma unroll
@@ -188,7 +207,7 @@ __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, in
 // stencil, D and geometry loads of a wave are contiguous: in direction 0 a lane is a face
 // position along the line, in directions 1 and 2 a line.
 // Three work-groups per CU fit the LDS (48 KB each), three waves per SIMD.
-template <int XORDER>
+template <int XORDER, int FLUX>
 __global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjArgs Ain, const BlockDev* __restrict__ blocks,
                                                                           const AdjPatch* __restrict__ patches)
 {
@@ -204,7 +223,7 @@ __global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjA
     const int tm = d == 0 ? t % kAdjPatchM : t / kAdjPatchT;  // face position in the patch
     const int ta = d == 0 ? t / kAdjPatchM : t % kAdjPatchT;  // line in the patch
     const int a = P.a0 + ta, c = P.c;
-    face_vjp<XORDER>(A, B, d, P.m0 - 1 + tm, a, c, st, t);
+    face_vjp<XORDER, FLUX>(A, B, d, P.m0 - 1 + tm, a, c, st, t);
     __syncthreads();
     // Owned cell m = m0 + tm of line (a, c): faces m-1 .. m+2 sit at positions tm .. tm+3.
     const int nline = line_len(B, d);
@@ -378,7 +397,8 @@ __global__ __launch_bounds__(64) void flux_vjp_points_kernel(const double* in, d
         s[c] = Prim{p[6 * c], p[6 * c + 1], p[6 * c + 2], p[6 * c + 3], p[6 * c + 4], p[6 * c + 5]};
     const Frame f{p[24], p[25], p[26], p[27], p[28], p[29], p[30], p[31], p[32]};
     double b[4][6] = {};
-    if (x_order > 1) face_flux_vjp<2>(s[0], s[1], s[2], s[3], f, p + 33, b);
+    if (x_order == 0) sbp_asf_vjp(s, f, p + 33, b);
+    else if (x_order > 1) face_flux_vjp<2>(s[0], s[1], s[2], s[3], f, p + 33, b);
     else face_flux_vjp<1>(s[0], s[1], s[2], s[3], f, p + 33, b);
     for (int c = 0; c < 4; ++c)
         for (int q = 0; q < 6; ++q) out[24 * (long long)t + 6 * c + q] = b[c][q];
@@ -393,12 +413,13 @@ hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, hipStream_t s)
+hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, int flux_calc, hipStream_t s)
 {
     if (npatches <= 0) return hipSuccess;
     const dim3 blk(kAdjPatchM * kAdjPatchT);
-    if (x_order > 1) hipLaunchKernelGGL(adj_patch_kernel<2>, dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
-    else hipLaunchKernelGGL(adj_patch_kernel<1>, dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
+    if (flux_calc == 1) hipLaunchKernelGGL((adj_patch_kernel<2, 1>), dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
+    else if (x_order > 1) hipLaunchKernelGGL((adj_patch_kernel<2, 0>), dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
+    else hipLaunchKernelGGL((adj_patch_kernel<1, 0>), dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
     return hipGetLastError();
 }
 

@@ -416,7 +416,6 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
     if (nsteps <= 0) return set_error(s, CHKN_ERR_ARG, "nsteps must be positive");
     if (C <= 0 || C > kMaxSeg) return set_error(s, CHKN_ERR_ARG, "checkpoint interval must be in 1..64");
     if (s->viscous) return set_error(s, CHKN_ERR_ARG, "the adjoint of the viscous fluxes is not built");
-    if (s->flux_calc != 0) return set_error(s, CHKN_ERR_ARG, "the adjoint of the SBP-ASF flux is not built");
     if (s->any_reacting) return set_error(s, CHKN_ERR_ARG, "the adjoint of the A->B reaction is not built");
     // The unroll is a function of the conserved state: gas states follow from it, also in
     // the first stage (the recomputation from checkpoints does the same).
@@ -574,7 +573,7 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
             const int nst = stages_per_step(s);
             for (int pos = nst; pos >= 1; --pos) {
                 A.slot = nst * q + pos - 1;
-                HIPCHK(launch_adj_faces(A, g.nface_chunks, s->x_order, s->stream));
+                HIPCHK(launch_adj_faces(A, g.nface_chunks, s->x_order, s->flux_calc, s->stream));
                 HIPCHK(launch_adj_gather(A, g.ncell_chunks, pos, s->stream));
                 launches += 2;
             }
@@ -783,7 +782,7 @@ int chkn_boundary_face_areas(chkn_sim* sim, int blk, int side, double* out)
 
 int chkn_flux_vjp_points(int npts, int x_order, const double* in, double* out)
 {
-    if (npts <= 0 || !in || !out) return set_error(nullptr, CHKN_ERR_ARG, "bad argument");
+    if (npts <= 0 || !in || !out || x_order < 0 || x_order > 2) return set_error(nullptr, CHKN_ERR_ARG, "bad argument");
     double *din = nullptr, *dout = nullptr;
     try {
         HIPCHK(hipMalloc(&din, sizeof(double) * 39 * (size_t)npts));
@@ -804,7 +803,7 @@ int chkn_flux_vjp_points(int npts, int x_order, const double* in, double* out)
 
 int chkn_flux_vjp_points_noalloc(int npts, int x_order, const double* in, double* out)
 {
-    if (npts <= 0 || !in || !out) return set_error(nullptr, CHKN_ERR_ARG, "bad argument");
+    if (npts <= 0 || !in || !out || x_order < 0 || x_order > 2) return set_error(nullptr, CHKN_ERR_ARG, "bad argument");
     if (npts > kNoAllocPoints)
         return set_error(nullptr, CHKN_ERR_ARG,
                          "npts exceeds the allocation-free limit of " + std::to_string(kNoAllocPoints) + " points");

@@ -152,7 +152,7 @@ struct AdjArgs {
 // inflow lines; the stage-1 gather also seeds the next reversed step when dt_next is given.
 // seed: P = seed_p dt A0, D = seed_d dt A0 (start of the first reversed step).
 hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s);
-hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, hipStream_t s);
+hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, int flux_calc, hipStream_t s);
 hipError_t launch_adj_gather(const AdjArgs& a, int ncell_chunks, int stage, hipStream_t s);
 // U and W = decode(U) of every active block from U (after the conserved state was replaced).
 hipError_t launch_decode_all(const BlockDev* blocks, const CellChunk* cells, int ncell_chunks,

@@ -219,8 +219,8 @@ int chkn_boundary_face_areas(chkn_sim* sim, int blk, int side, double* out);
 /* Advance nsteps steps keeping a checkpoint every checkpoint_every (1..64) steps (max_step
  * and max_time do not stop it).  The time steps follow the CFL schedule (dt_in NULL) or are
  * given, dt_in[step*ninst + inst]; those used are written to dt_out when it is not NULL.
- * Either time scheme (TVD-RK3 or predictor-corrector); inviscid AUSMDV jobs without the
- * reaction. */
+ * Either time scheme (TVD-RK3 or predictor-corrector) and either flux (AUSMDV or SBP-ASF);
+ * inviscid jobs without the reaction. */
 int chkn_grad_forward(chkn_sim* sim, int nsteps, int checkpoint_every, const double* dt_in,
                       double* dt_out);
 /* Device memory the forward pass may fill with the stage states of its last checkpoint
@@ -244,7 +244,9 @@ int chkn_grad_backward(chkn_sim* sim, const double* lambda_dev, double* grad_u0_
                        double* grad_inflow);
 /* Test hook: vector-Jacobian products of the face flux at npts points on the device.
  * in: per point 4 x 6 stencil primitives (L1, L0, R0, R1; rho, e, YB, vx, vy, vz), the 9
- * frame components (n, t1, t2) and 6 flux adjoints; out: 24 stencil adjoints per point. */
+ * frame components (n, t1, t2) and 6 flux adjoints; out: 24 stencil adjoints per point.
+ * x_order 1 or 2: AUSMDV (face.cu:347-371) at that reconstruction order; 0: the SBP-ASF flux
+ * (face.cu:268-342, p from the equation of state of each stencil state). */
 int chkn_flux_vjp_points(int npts, int x_order, const double* in, double* out);
 /* The same evaluation without any device allocation (no counterpart in the reference): for
  * callers that cannot allocate device memory -- code under HIP graph capture, tight serving

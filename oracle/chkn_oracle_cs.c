@@ -3,8 +3,8 @@
  * TEST INFRASTRUCTURE ONLY: the product never links this file.
  *
  * The face flux of face.cu:347-371 (MUSCL/van Albada reconstruction, face.cu:72-96; the
- * ideal-gas equation of state, gas.cu:128-135; AUSMDV, face.cu:163-265) restated over
- * complex numbers.  For a real input x and direction v, Im F(x + i h v) / h is the
+ * ideal-gas equation of state, gas.cu:128-135; AUSMDV, face.cu:163-265; and the SBP-ASF flux,
+ * face.cu:268-342) restated over complex numbers.  For a real input x and direction v, Im F(x + i h v) / h is the
  * directional derivative J v to rounding error (no subtractive cancellation), which is
  * what the adjoint kernels' vector-Jacobian products are checked against:
  * <Fbar, J v> = <J^T Fbar, v>.
@@ -116,8 +116,50 @@ static void ausmdv_cs(const CS* L, const CS* R, const double* f, cx* F) /* face.
     F[5] = m * (up ? L->YB : R->YB);
 }
 
+/* face.cu:268-342: the SBP-ASF flux over the four stencil states, each with its own p (here
+ * from the equation of state); the alpha products are kept as written. */
+static void sbp_asf_cs(const CS* g, const double* f, cx* F)
+{
+    cx v[10][4], w[10][4];
+    for (int i = 0; i < 4; ++i) {
+        cx velx, vely, velz;
+        local_cs(f, g[i].vx, g[i].vy, g[i].vz, &velx, &vely, &velz);
+        const cx rho = g[i].rho;
+        v[0][i] = rho;                 w[0][i] = velx;
+        v[1][i] = velx * rho;          w[1][i] = velx;
+        v[2][i] = vely * rho;          w[2][i] = velx;
+        v[3][i] = velz * rho;          w[3][i] = velx;
+        v[4][i] = g[i].e * rho;        w[4][i] = velx;
+        v[5][i] = velx * velx * rho;   w[5][i] = velx;
+        v[6][i] = vely * vely * rho;   w[6][i] = velx;
+        v[7][i] = velz * velz * rho;   w[7][i] = velx;
+        v[8][i] = g[i].p;              w[8][i] = velx;
+        v[9][i] = g[i].p;              w[9][i] = 1.0;
+    }
+    cx fc[10], fe[10];
+    for (int j = 0; j < 10; ++j) {
+        fc[j] = (1.0 / 12.0) * (-v[j][0] * w[j][0] + 7.0 * v[j][1] * w[j][1] + 7.0 * v[j][2] * w[j][2] -
+                                v[j][3] * w[j][3]);
+        fe[j] = (1.0 / 12.0) * (-v[j][0] * w[j][2] - v[j][2] * w[j][0] + 8.0 * v[j][1] * w[j][2] +
+                                8.0 * v[j][2] * w[j][1] - v[j][1] * w[j][3] - v[j][3] * w[j][1]);
+    }
+    const double a_mass = 1.0, a_mom = 0.5, a_ie = 0.5, a_ke = 0.0, a_p = 0.0;
+    const cx mass = a_mass * fc[0] + (1.0 - a_mass) * fe[0];
+    F[0] = mass;
+    const cx mx = a_mom * fc[1] + (1.0 - a_mom) * fe[1] + (a_p * fc[9] + (1.0 - a_p) * fe[9]);
+    const cx my = a_mom * fc[2] + (1.0 - a_mom) * fe[2];
+    const cx mz = a_mom * fc[3] + (1.0 - a_mom) * fe[3];
+    global_cs(f, mx, my, mz, &F[1], &F[2], &F[3]);
+    F[4] = a_ie * fc[4] + (1.0 - a_ie) * fe[4] +
+           0.5 * (a_ke * fc[5] + (1.0 - a_ke) * fe[5] + a_ke * fc[6] + (1.0 - a_ke) * fe[6] +
+                  a_ke * fc[7] + (1.0 - a_ke) * fe[7]) +
+           a_p * fc[8] + (1.0 - a_p) * fe[8];
+    F[5] = mass * (creal(mass) >= 0.0 ? g[1].YB : g[2].YB);
+}
+
 /* prims: 4 states x (rho, e, YB, vx, vy, vz); frame: n, t1, t2; dir: 24 values.
- * Fre = Re F(x + i h dir), jv = Im F(x + i h dir) / h. */
+ * Fre = Re F(x + i h dir), jv = Im F(x + i h dir) / h.  x_order 1 or 2: AUSMDV on the
+ * reconstructed (x_order 2) or adjacent states; x_order 0: the SBP-ASF flux. */
 void orc_face_flux_cs(const double* prims, const double* frame, int x_order, const double* dir,
                       double h, double* Fre, double* jv)
 {
@@ -131,6 +173,16 @@ void orc_face_flux_cs(const double* prims, const double* frame, int x_order, con
         st[c].vx = p[3] + I * h * v[3];
         st[c].vy = p[4] + I * h * v[4];
         st[c].vz = p[5] + I * h * v[5];
+    }
+    if (x_order == 0) {
+        cx F[6];
+        for (int c = 0; c < 4; ++c) rhoe_cs(&st[c]);
+        sbp_asf_cs(st, frame, F);
+        for (int q = 0; q < 6; ++q) {
+            Fre[q] = creal(F[q]);
+            jv[q] = cimag(F[q]) / h;
+        }
+        return;
     }
     CS L = st[1], R = st[2]; /* face.cu:347-371 */
     if (x_order > 1) {

@@ -1,7 +1,8 @@
 """Adjoint (reverse-mode) gradients of the unrolled step, on the GPU.
 
-* flux level: the device vector-Jacobian product of the face flux against the complex-step
-  derivative of the oracle's flux: <J^T Fbar, v> = <Fbar, J v> to 1e-12 relative;
+* flux level: the device vector-Jacobian product of the face flux (AUSMDV at x_order 1 and 2,
+  and SBP-ASF) against the complex-step derivative of the oracle's flux:
+  <J^T Fbar, v> = <Fbar, J v> to 1e-12 relative;
 * unroll level: d(wall-pressure loss)/d(inflow Mach, p, T) against central finite
   differences (relative step 1e-7, time-step sequence frozen) to 1e-5 relative, and the
   derivative along a seeded random smooth direction of U0 to 1e-3 at step 1e-7 and 1e-5 at
@@ -44,7 +45,7 @@ def _points(rng, n, speed):
     return prims, frames
 
 
-@pytest.mark.parametrize("x_order", [1, 2])
+@pytest.mark.parametrize("x_order", [0, 1, 2])  # 0: the SBP-ASF flux
 def test_flux_vjp_matches_complex_step(x_order):
     rng = np.random.default_rng(10 + x_order)
     worst = 0.0
@@ -250,6 +251,19 @@ def test_first_order_gradient_matches_finite_differences():
     _, _, gm = prob.gradient()
     rows = _fd_rows_inflow(prob, gm)
     short = Problem([job], nsteps=4, C=3, mpt=[[2.5, 5.0e3, 260.0]])
+    _, gu0s, _ = short.gradient()
+    _assert_rows(rows + _fd_rows_u0(short, gu0s))
+
+
+def test_sbp_asf_gradient_matches_finite_differences():
+    # The SBP-ASF flux's adjoint in the unroll: the smooth channel on two warped blocks
+    # (inflow, outflow, periodic exchange in j, slip walls in k carrying the loss).
+    job = cases.sbp_channel(max_step=40, k_walls=True)
+    prob = Problem([job], nsteps=20, C=6, mpt=[[0.4, 5.0e4, 300.0]])
+    L, _, gm = prob.gradient()
+    assert L > 0 and abs(prob.value() - L) <= 1e-15 * abs(L)
+    rows = _fd_rows_inflow(prob, gm)
+    short = Problem([job], nsteps=4, C=3, mpt=[[0.4, 5.0e4, 300.0]])
     _, gu0s, _ = short.gradient()
     _assert_rows(rows + _fd_rows_u0(short, gu0s))
 

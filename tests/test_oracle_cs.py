@@ -57,3 +57,25 @@ def test_complex_step_matches_central_differences():
         ref = np.max(np.abs(jv)) + 1e-300
         worst = max(worst, float(np.max(np.abs(fd - jv)) / ref))
     assert worst < 1e-5, worst
+
+
+def test_sbp_asf_complex_step():
+    # The SBP-ASF flux (x_order 0 in the complex-step entry point): its real part is the real
+    # oracle's flux_calc 1 flux, and its directional derivatives agree with central
+    # differences of it.
+    rng = np.random.default_rng(2)
+    worst = 0.0
+    for trial in range(40):
+        prims, frame = _case(rng, [60.0, 250.0, 900.0][trial % 3])
+        scale = np.abs(prims).ravel() + 1.0
+        d = rng.normal(size=24) * scale
+        Fre, jv = ob.face_flux_cs(prims, frame, d, x_order=0)
+        F = ob.face_flux(_states9(prims), frame, flux_calc=1)
+        assert np.all((Fre == F) | (np.abs(Fre - F) < 1e-50)), (trial, Fre - F)
+        h = 1.0e-7
+        Fp = ob.face_flux(_states9(prims + h * d.reshape(4, 6)), frame, flux_calc=1)
+        Fm = ob.face_flux(_states9(prims - h * d.reshape(4, 6)), frame, flux_calc=1)
+        fd = (Fp - Fm) / (2 * h)
+        ref = np.max(np.abs(jv)) + 1e-300
+        worst = max(worst, float(np.max(np.abs(fd - jv)) / ref))
+    assert worst < 1e-6, worst
