@@ -363,7 +363,7 @@ def main():
     if args.workload == "scale" and args.sub_records:
         # The rest of the metric: the batched form against the reference's per-block launch
         # structure, and forward + adjoint against the plain forward.
-        line["batched"] = batched_record(args.graph)
+        line["batched"] = batched_record(args.graph, cpu=not args.no_cpu)
         line["diff"] = diff_record(args.graph)
     print(json.dumps(line))
 
@@ -417,19 +417,23 @@ def _best_of(jobs, candidates, steps, warmup, graph, per_block):
     return best
 
 
-def batched_record(graph, steps=60, warmup=10):
+def batched_record(graph, steps=60, warmup=10, cpu=True):
     """The 40-block uneven array (5x4x2, slab sizes seeded in [16, 96]) in one launch per
     stage for all blocks, against one launch per block per stage (the reference's launch
-    structure, simulate.cu:605-625), each with its best launch shape of a short list."""
+    structure, simulate.cu:605-625), each with its best launch shape of a short list; and the
+    reference's CPU path on the same array (OpenMP over its 40 blocks, at most 16 threads)."""
     jobs, desc = build_workload("batched", warmup + steps)
     wall, kms, launches, rep, cells, opts = _best_of(jobs, SUB_FUSED, steps, warmup, graph, False)
     rec = roofline_record(jobs, opts, steps, warmup, "batched", kms, launches, cells)
     pw, _, pl, _, _, popts = _best_of(jobs, SUB_PER_BLOCK, steps, warmup, graph, True)
-    return {"workload": desc["workload"], "cells": cells, "steps": steps, "warmup": warmup,
-            "value": cells * steps / wall, "unit": "cell-updates/s", "ms_per_step": wall * 1e3 / steps,
-            "launch": opts, "roofline": rec,
-            "per_block_value": cells * steps / pw, "per_block_launches": pl, "per_block_launch": popts,
-            "fused_over_per_block": pw / wall}
+    out = {"workload": desc["workload"], "cells": cells, "steps": steps, "warmup": warmup,
+           "value": cells * steps / wall, "unit": "cell-updates/s", "ms_per_step": wall * 1e3 / steps,
+           "launch": opts, "roofline": rec,
+           "per_block_value": cells * steps / pw, "per_block_launches": pl, "per_block_launch": popts,
+           "fused_over_per_block": pw / wall}
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(jobs[0], 2)
+    return out
 
 
 def diff_record(graph, steps=100):
