@@ -15,20 +15,34 @@ pass it back through `dts` to replay the same sequence (finite-difference checks
 import numpy as np
 import torch
 
-GAMMA = 1.4
-R_GAS = 287.1
-CV = R_GAS / (GAMMA - 1.0)
-Q_HEAT = 0.0
 WALL_CODES = (0, 1, 2)
 
 
-def inflow_from_mpt(mpt, YB=0.0):
+class Gas:
+    """The gas model's constants as the library applies them (gas.cu:20-48): ratio of specific
+    heats, gas constant, heat of reaction q and Cv = R / (gamma - 1)."""
+
+    def __init__(self, gamma, R, q=0.0):
+        self.gamma, self.R, self.q = float(gamma), float(R), float(q)
+        self.cv = self.R / (self.gamma - 1.0)
+
+    @classmethod
+    def of(cls, job):
+        """The gas model of a job (JobData.gas_model: gamma, R, q, alpha, Ti, Cv)."""
+        gm = job.gas_model
+        return cls(gm[0], gm[1], gm[2])
+
+
+AIR = Gas(1.4, 287.1)
+
+
+def inflow_from_mpt(mpt, YB=0.0, gas=AIR):
     """(Mach, p, T) -> (rho, e, YB, vx, vy, vz) with the operation order of update_from_pT
     (gas.cu:119-126) and velx = Mach * a: the state chkn.cases.freestream describes."""
     M, p, T = mpt[0], mpt[1], mpt[2]
-    e = T * CV - YB * Q_HEAT
-    rho = p / (T * R_GAS)
-    a = torch.sqrt(GAMMA * R_GAS * T)
+    e = T * gas.cv - YB * gas.q
+    rho = p / (T * gas.R)
+    a = torch.sqrt(gas.gamma * gas.R * T)
     z = torch.zeros_like(T)
     return torch.stack([rho, e, z + YB, M * a, z, z])
 
@@ -86,6 +100,7 @@ class WallPressure:
     p(cell at k = 0) * face area: the pressure force on the ramp wall."""
 
     def __init__(self, sim, jobs):
+        self.gas = Gas.of(jobs[0])  # the instances of a handle share the library's gas model
         self.terms = []  # (offset of the block in the flat state, ncells, nic*njc, areas)
         off = 0
         b = 0
@@ -106,8 +121,8 @@ class WallPressure:
             rho = U[0]
             vx, vy, vz = U[1] / rho, U[2] / rho, U[3] / rho
             e = U[4] / rho - 0.5 * (vx * vx + vy * vy + vz * vz)
-            T = (e + (U[5] / rho) * Q_HEAT) / CV
-            p = rho * R_GAS * T
+            T = (e + (U[5] / rho) * self.gas.q) / self.gas.cv
+            p = rho * self.gas.R * T
             out.append(torch.sum(p * areas))
         return out
 

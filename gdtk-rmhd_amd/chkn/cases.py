@@ -384,11 +384,11 @@ def sbp_channel(nics=(10, 8), njc=6, nkc=5, max_step=25, seed=3, flux_calc="sbp_
     return JobData.from_prep()
 
 
-def ab_reacting(nics=(16, 12), njc=6, nkc=5, max_step=30, seed=2, reacting=True):
+def ab_reacting(nics=(16, 12), njc=6, nkc=5, max_step=30, seed=2, reacting=True, k_walls=False):
     """The reacting A->B gas (gas.cu:25-48, examples/chicken/oblique-detonation-wave): a
-    Mach 2.4 stream (p = 86.1 kPa, T = 300 K) into gas at 28.7 kPa holding a hot, partly
+    Mach 3 stream (p = 86.1 kPa, T = 300 K) into gas at 28.7 kPa holding a hot, partly
     burnt slug above the ignition temperature, on two blocks: inflow at iminus, outflow at
-    iplus, slip walls in j, periodic in k."""
+    iplus, slip walls in j, periodic in k (slip walls in k with k_walls)."""
     _common(max_step, title="A->B reacting gas (seed %d)" % seed, cfl=0.5, dt_init=5.0e-7)
     gm = prep.GasModel(gamma=6.0 / 5.0, R=287.0, q=300000.0, alpha=1000.0, Ti=362.58)
     prep.config.gas_model = gm
@@ -413,7 +413,8 @@ def ab_reacting(nics=(16, 12), njc=6, nkc=5, max_step=30, seed=2, reacting=True)
         bcs = {"iminus": prep.InflowBC(inflow) if i == 0 else prep.ExchangeBC(),
                "iplus": prep.OutflowBC() if i == 1 else prep.ExchangeBC(),
                "jminus": prep.WallWithSlipBC(), "jplus": prep.WallWithSlipBC(),
-               "kminus": prep.ExchangeBC(), "kplus": prep.ExchangeBC()}
+               "kminus": prep.WallWithSlipBC() if k_walls else prep.ExchangeBC(),
+               "kplus": prep.WallWithSlipBC() if k_walls else prep.ExchangeBC()}
         sub = g.subgrid(i0=io[i], j0=0, k0=0, niv=nics[i] + 1, njv=njc + 1, nkv=nkc + 1)
         init = {key: np.ascontiguousarray(val[io[i]:io[i + 1]]) for key, val in st.items()}
         prep.FluidBlock(i=i, j=0, k=0, grid=sub, initialState=init, bcs=bcs)

@@ -372,6 +372,38 @@ CHKN_HD void decode_vjp(const Prim& w, const double Wb[6], double Ub[6])
     Ub[0] = Wb[0] - (bx * w.vx + by * w.vy + bz * w.vz + Wb[1] * (w.e + ke) + Wb[2] * w.YB) * dinv;
 }
 
+// Adjoint of encode(w) -> U (flow.cu:227-243) at the primitive state w: Wb[p] = sum_q Ub[q] dU_q/dW_p.
+CHKN_HD void encode_vjp(const Prim& w, const double Ub[6], double Wb[6])
+{
+    const double rho = w.rho;
+    const double ke = 0.5 * (w.vx * w.vx + w.vy * w.vy + w.vz * w.vz);
+    Wb[0] = Ub[0] + Ub[1] * w.vx + Ub[2] * w.vy + Ub[3] * w.vz + Ub[4] * (w.e + ke) + Ub[5] * w.YB;
+    Wb[1] = Ub[4] * rho;
+    Wb[2] = Ub[5] * rho;
+    Wb[3] = (Ub[1] + Ub[4] * w.vx) * rho;
+    Wb[4] = (Ub[2] + Ub[4] * w.vy) * rho;
+    Wb[5] = (Ub[3] + Ub[4] * w.vz) * rho;
+}
+
+// Adjoint of the A->B reaction step U3 -> U = encode(chem(decode(U3))) (block.cu:1174-1184,
+// gas.cu:102-111: where T > Ti, YB' = 1 - (1 - YB) f with f = exp(-alpha dt); rho, e and the
+// velocity are unchanged), from the state after the reaction wr and the adjoint Ub of U; f < 0
+// for an instance without the reaction.  The reaction took place exactly where the state after
+// it is above the ignition temperature (it only raises YB, hence T = (e + YB q)/Cv, and leaves
+// states at or below Ti unchanged), and there YB before it is 1 - (1 - YB')/f.  Returns the
+// adjoint of U3 in Ub.
+CHKN_HD void chem_vjp(const Prim& wr, double f, double Ub[6])
+{
+    double Wb[6];
+    encode_vjp(wr, Ub, Wb);
+    Prim w3 = wr;
+    if (f >= 0.0 && eos_T(wr.e, wr.YB) > kTi) {
+        w3.YB = 1.0 - (1.0 - wr.YB) / f;
+        Wb[2] *= f;
+    }
+    decode_vjp(w3, Wb, Ub);
+}
+
 // Adjoint of the wall ghost state reflect(s, f): the map is v -> F^T diag(-1,1,1) F v,
 // which is symmetric, so its transpose applied to the ghost's velocity adjoint is the same
 // reflection; rho, e and YB pass through.

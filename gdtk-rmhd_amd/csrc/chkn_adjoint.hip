@@ -386,6 +386,34 @@ __global__ __launch_bounds__(256) void adj_seed_kernel(AdjArgs Ain, const BlockD
     }
 }
 
+__global__ __launch_bounds__(256) void adj_chem_seed_kernel(AdjArgs Ain, const BlockDev* __restrict__ blocks,
+                                                            const CellChunk* __restrict__ cells)
+{
+    AdjArgs A = Ain;
+    A.blocks = blocks;
+    A.cells = cells;
+    const CellChunk ch = A.cells[blockIdx.x];
+    const BlockDev* B = A.blocks + ch.blk;
+    const long long n = B->n;
+    const int cidx = ch.c0 + (int)threadIdx.x;
+    if (cidx >= ch.c1) return;
+    const double dt = A.dt[B->inst];
+    double* A0 = B->adj;
+    double* Pp = B->adj + 6 * (long long)n;
+    double* D = B->adj + 12 * (long long)n;
+    double Ub[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Ub[q] = A0[(long long)q * n + cidx];
+    chem_vjp(ld_prim(B->W + A.slot * B->wstride, n, cidx), A.chem[B->inst], Ub);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const long long o = (long long)q * n + cidx;
+        A0[o] = Ub[q];
+        Pp[o] = A.seed_p * dt * Ub[q];
+        D[o] = A.seed_d * dt * Ub[q];
+    }
+}
+
 __global__ __launch_bounds__(64) void flux_vjp_points_kernel(const double* in, double* out, int npts,
                                                             int x_order)
 {
@@ -410,6 +438,13 @@ hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s)
 {
     if (ncell_chunks <= 0) return hipSuccess;
     hipLaunchKernelGGL(adj_seed_kernel, dim3(ncell_chunks), dim3(256), 0, s, a, a.blocks, a.cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_chem_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s)
+{
+    if (ncell_chunks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_chem_seed_kernel, dim3(ncell_chunks), dim3(256), 0, s, a, a.blocks, a.cells);
     return hipGetLastError();
 }
 

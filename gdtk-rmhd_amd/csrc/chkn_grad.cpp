@@ -335,6 +335,7 @@ void queue_recompute(chkn_sim* s, int j, int K, int t, hipStream_t st)
             sg[qi] = ~0ull;
         }
         bad[0] = bad[1] = bad[2] = 0;
+        if (s->any_reacting) fill_chem(s, q);
     }
     HIPCHK(hipMemcpyAsync(s->d_ctl, s->h_ctl, s->ctl_bytes, hipMemcpyHostToDevice, st));
     for (int q = 0; q < K; ++q) {
@@ -348,6 +349,9 @@ void queue_recompute(chkn_sim* s, int j, int K, int t, hipStream_t st)
             a.wout = nst * q + pos;
             launch_stage_form(s, a, (s->time_scheme == CHKN_TIME_PC && pos == 2) ? 4 : pos, false, false, st);
         }
+        // The step's reaction on its last stage state (as the forward step, simulate.cu:296-303).
+        if (s->any_reacting)
+            HIPCHK(launch_chemistry(a.blocks, s->d_split_cells, s->nsplit_cells, a.ctl, a.wout, false, st));
     }
     HIPCHK(hipMemcpyAsync(s->h_ctl, s->d_ctl, s->ctl_bytes, hipMemcpyDeviceToHost, st));
 }
@@ -379,6 +383,7 @@ int fixed_dt_steps(chkn_sim* s, int K, const double* dts)
             sg[qi] = ~0ull;
         }
         bad[0] = bad[1] = bad[2] = 0;
+        if (s->any_reacting) fill_chem(s, q);
     }
     run_segment(s, K, false);
     for (int q = 0; q < K; ++q) {
@@ -416,7 +421,6 @@ int grad_forward(chkn_sim* s, int nsteps, int C, const double* dt_in, double* dt
     if (nsteps <= 0) return set_error(s, CHKN_ERR_ARG, "nsteps must be positive");
     if (C <= 0 || C > kMaxSeg) return set_error(s, CHKN_ERR_ARG, "checkpoint interval must be in 1..64");
     if (s->viscous) return set_error(s, CHKN_ERR_ARG, "the adjoint of the viscous fluxes is not built");
-    if (s->any_reacting) return set_error(s, CHKN_ERR_ARG, "the adjoint of the A->B reaction is not built");
     // The unroll is a function of the conserved state: gas states follow from it, also in
     // the first stage (the recomputation from checkpoints does the same).
     s->fresh = false;
@@ -510,6 +514,16 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     g.d_dts = nullptr;
     HIPCHK(hipMalloc(&g.d_dts, sizeof(double) * g.dts.size()));
     HIPCHK(hipMemcpy(g.d_dts, g.dts.data(), sizeof(double) * g.dts.size(), hipMemcpyHostToDevice));
+    dfree(g.d_chem);
+    g.d_chem = nullptr;
+    if (s->any_reacting) {
+        // exp(-alpha dt) of every step, computed as the forward step computed it (fill_chem).
+        std::vector<double> chem(g.dts.size());
+        for (size_t q = 0; q < chem.size(); ++q)
+            chem[q] = s->inst[q % N].reacting ? (double)std::exp(-kAlpha * (real)g.dts[q]) : -1.0;
+        HIPCHK(hipMalloc(&g.d_chem, sizeof(double) * chem.size()));
+        HIPCHK(hipMemcpy(g.d_chem, chem.data(), sizeof(double) * chem.size(), hipMemcpyHostToDevice));
+    }
     hipEvent_t e0 = s->ev0, e1 = s->ev1;
     HIPCHK(hipEventRecord(e0, s->stream));
     // Park the final state; seed A0 with the final adjoint.
@@ -563,14 +577,24 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
         for (int q = K - 1; q >= 0; --q) {
             const int step = j * C + q;
             A.dt = g.d_dts + (size_t)step * N;
-            A.dt_next = step > 0 ? g.d_dts + (size_t)(step - 1) * N : nullptr;
-            if (step == nsteps - 1) {
-                HIPCHK(launch_adj_seed(A, g.ncell_chunks, s->stream));
+            const int nst = stages_per_step(s);
+            if (s->any_reacting) {
+                // The step's reaction first (its state after the reaction is the next step's
+                // first slot), then the seeds; the stage-1 gathers do not seed.
+                A.dt_next = nullptr;
+                A.chem = g.d_chem + (size_t)step * N;
+                A.slot = nst * (q + 1);
+                HIPCHK(launch_adj_chem_seed(A, g.ncell_chunks, s->stream));
                 launches += 1;
+            } else {
+                A.dt_next = step > 0 ? g.d_dts + (size_t)(step - 1) * N : nullptr;
+                if (step == nsteps - 1) {
+                    HIPCHK(launch_adj_seed(A, g.ncell_chunks, s->stream));
+                    launches += 1;
+                }
             }
             // Stage positions in reverse; the predictor-corrector's corrector (position 2)
             // gathers as the TVD-RK3 stage 2 (D = P + dt Ubar), its predictor as stage 1.
-            const int nst = stages_per_step(s);
             for (int pos = nst; pos >= 1; --pos) {
                 A.slot = nst * q + pos - 1;
                 HIPCHK(launch_adj_faces(A, g.nface_chunks, s->x_order, s->flux_calc, s->stream));
@@ -636,7 +660,7 @@ void free_grad(chkn_sim* s)
     if (s->grad) {
         free_resident(s);
         dfree(s->grad->d_face_chunks); dfree(s->grad->d_cell_chunks);
-        dfree(s->grad->d_inflow_cols); dfree(s->grad->d_inflow_acc); dfree(s->grad->d_dts);
+        dfree(s->grad->d_inflow_cols); dfree(s->grad->d_inflow_acc); dfree(s->grad->d_dts); dfree(s->grad->d_chem);
         dfree(s->grad->d_tab[0]); dfree(s->grad->d_tab[1]);
         if (s->grad->ev_rev[0]) (void)hipEventDestroy(s->grad->ev_rev[0]);
         if (s->grad->ev_rev[1]) (void)hipEventDestroy(s->grad->ev_rev[1]);

@@ -143,6 +143,8 @@ struct AdjArgs {
     const double* dt_next;    // [ninst] time step of the step reversed next, or null (stage 1)
     double seed_p, seed_d;    // a reversed step starts from P = seed_p dt A0, D = seed_d dt A0:
                               // 1/6 and 4/6 (TVD-RK3), 1/2 and 1/2 (predictor-corrector)
+    const double* chem;       // [ninst] exp(-alpha dt) of the step being reversed (< 0: no
+                              // reaction), for the reacting A->B gas
 };
 
 // Adjoint of one stage's residual: face pass (per patch, every face's stencil adjoints summed
@@ -152,6 +154,10 @@ struct AdjArgs {
 // inflow lines; the stage-1 gather also seeds the next reversed step when dt_next is given.
 // seed: P = seed_p dt A0, D = seed_d dt A0 (start of the first reversed step).
 hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s);
+// Reacting jobs: A0 = adjoint of U after the step's reaction becomes the adjoint of U before
+// it (chem_vjp, with the state after the reaction in W slot a.slot), then the seeds of the
+// reversed step as launch_adj_seed.
+hipError_t launch_adj_chem_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s);
 hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, int flux_calc, hipStream_t s);
 hipError_t launch_adj_gather(const AdjArgs& a, int ncell_chunks, int stage, hipStream_t s);
 // U and W = decode(U) of every active block from U (after the conserved state was replaced).

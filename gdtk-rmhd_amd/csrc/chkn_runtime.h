@@ -111,6 +111,8 @@ struct GradState {
     void* d_inflow_cols = nullptr;
     double* d_inflow_acc = nullptr;  // 6 x ninflow_cols
     double* d_dts = nullptr;         // dts on the device, [nsteps][ninst]
+    double* d_chem = nullptr;        // reacting handles: exp(-alpha dt) of every step, [nsteps][ninst]
+                                     // (-1 for instances without the reaction)
     // Two tapes: segment j-1 is recomputed on stream rec into one while segment j is reversed
     // from the other on the handle's stream.
     bool two_tapes = false;

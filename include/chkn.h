@@ -220,7 +220,7 @@ int chkn_boundary_face_areas(chkn_sim* sim, int blk, int side, double* out);
  * and max_time do not stop it).  The time steps follow the CFL schedule (dt_in NULL) or are
  * given, dt_in[step*ninst + inst]; those used are written to dt_out when it is not NULL.
  * Either time scheme (TVD-RK3 or predictor-corrector) and either flux (AUSMDV or SBP-ASF);
- * inviscid jobs without the reaction. */
+ * inviscid jobs, reacting or not (the reaction is reversed between the steps). */
 int chkn_grad_forward(chkn_sim* sim, int nsteps, int checkpoint_every, const double* dt_in,
                       double* dt_out);
 /* Device memory the forward pass may fill with the stage states of its last checkpoint

@@ -115,11 +115,12 @@ class Problem:
         self.nsteps, self.C = nsteps, C
         self.mpt = [np.asarray(m, dtype=np.float64) for m in mpt]
         self.dts = None
+        self.gas = adjoint.Gas.of(jobs[0])
         # Each inflow flow state takes the (Mach, p, T) of its instance.
         self.state_inst = [q for q, j in enumerate(jobs) for _ in j.flow_states]
 
     def inflow(self, mpts):
-        return torch.stack([adjoint.inflow_from_mpt(mpts[q]) for q in self.state_inst])
+        return torch.stack([adjoint.inflow_from_mpt(mpts[q], gas=self.gas) for q in self.state_inst])
 
     def loss(self, u0, mpts, C=None):
         uN = adjoint.unroll(self.sim, u0, self.inflow(mpts), self.nsteps, C or self.C, self.dts)
@@ -154,18 +155,28 @@ def _fd_rows_inflow(prob, grad_mpt, inst=0):
     return rows
 
 
-def _fd_rows_u0(prob, grad_u0):
+def _fd_rows_u0(prob, grad_u0, component=None):
     """Central differences along a smooth direction of U0.  The limited scheme has kinks
     (|ab| in each van Albada limiter, |pL - pR| and the upwind switch in AUSMDV) at which
     its derivative jumps; a perturbation of every cell by 1e-7 relative crosses a few of
     them, which costs central differences up to about 1e-4 of accuracy, while the step 1e-9
-    does not (and roundoff still allows 1e-5 there).  Both are checked."""
+    does not (and roundoff still allows 1e-5 there).  Both are checked.  With `component`
+    the direction moves that conserved quantity only (5: rho YB)."""
     v = _smooth_direction(prob, 7)
+    tag = "U0"
+    if component is not None:
+        keep = []
+        for job in prob.jobs:
+            for b in job.blocks:
+                n = b.nic * b.njc * b.nkc
+                keep.append(np.repeat(np.arange(6) == component, n))
+        v = v * torch.as_tensor(np.concatenate(keep), device="cuda")
+        tag = "U0[%d]" % component
     g = float(torch.dot(grad_u0, v))
     rows = []
     for h, tol in ((FD_REL, 1e-3), (1e-9, FD_TOL)):
         fd = (prob.value(u0=prob.u0 + h * v) - prob.value(u0=prob.u0 - h * v)) / (2 * h)
-        rows.append(("dL/dU0.v (step %.0e)" % h, fd, g, tol))
+        rows.append(("dL/d%s.v (step %.0e)" % (tag, h), fd, g, tol))
     return rows
 
 
@@ -266,6 +277,32 @@ def test_sbp_asf_gradient_matches_finite_differences():
     short = Problem([job], nsteps=4, C=3, mpt=[[0.4, 5.0e4, 300.0]])
     _, gu0s, _ = short.gradient()
     _assert_rows(rows + _fd_rows_u0(short, gu0s))
+
+
+@pytest.mark.parametrize("time_scheme", [0, 1])
+def test_reacting_gradient_matches_finite_differences(time_scheme):
+    # The A->B gas (libchkn_ab) with the reaction after every step: the hot slug above the
+    # ignition temperature reacts, the Mach 3 inflow at 300 K does not.  The reaction's adjoint
+    # (chem_vjp) sits between the reversed steps; a recomputed segment repeats the reaction.
+    # Both time schemes: TVD-RK3 and the predictor-corrector.
+    job = cases.ab_reacting(max_step=40, k_walls=True)
+    job.time_scheme = time_scheme
+    gas = adjoint.Gas.of(job)
+    mpt = [[964.302 / (gas.gamma * gas.R * 300.0) ** 0.5, 86.1e3, 300.0]]
+    prob = Problem([job], nsteps=12, C=5, mpt=mpt)
+    L, g1, gm = prob.gradient()
+    assert L > 0 and abs(prob.value() - L) <= 1e-15 * abs(L)
+    rows = _fd_rows_inflow(prob, gm)
+    short = Problem([job], nsteps=4, C=3, mpt=mpt)
+    _, gu0s, _ = short.gradient()
+    # rho YB is non-zero only in the burning slug: along it the gradient runs through the
+    # reaction's factor f = exp(-alpha dt) once per step (about 2 % over the four steps).
+    _assert_rows(rows + _fd_rows_u0(short, gu0s) + _fd_rows_u0(short, gu0s, component=5))
+    # Every segment recomputed: the same gradient bit for bit.
+    prob.sim.set_tape_budget(0)
+    L2, g2, gm2 = prob.gradient()
+    assert prob.sim.grad_resident_segments == 0
+    assert L2 == L and torch.equal(g1, g2) and torch.equal(gm[0], gm2[0])
 
 
 def _pc(job):
