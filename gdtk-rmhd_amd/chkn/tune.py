@@ -14,6 +14,7 @@ import copy
 import hashlib
 import json
 import os
+import time
 
 from .lib import BadCellError, Sim
 
@@ -67,12 +68,16 @@ def _scratch(jobs):
 
 
 def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=None,
-             per_block_launch=False, verbose=False, refine=8, refine_steps=50):
+             per_block_launch=False, verbose=False, refine=8, refine_steps=50, heat_seconds=0.1):
     """Time every candidate for `steps` steps (after `warmup` untimed steps) on a scratch
     handle, then time the `refine` fastest twice more for `refine_steps` steps each (each run
     on a fresh handle from the initial state), and
     return (best_options, [(ms_per_step, options), ...] of the sweep, fastest first).  The
     short sweep cannot separate shapes a few percent apart; the longer runs pick among them.
+    Building a handle leaves the device idle long enough to lower its clocks: the first 25
+    steps of a fresh 256x128x128 block then run 2.4% slower, more than the shapes being ranked
+    differ.  So the sweep's first candidate and every refine run are timed only after
+    `heat_seconds` of untimed steps.
 
     Timing is the device time of the step kernels (HIP events around the launches).  The
     scratch handle's state advances during the sweep; a candidate that meets a bad cell
@@ -93,12 +98,12 @@ def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=Non
 
     sim = [fresh()]
 
-    def time_steps(opts, n):
+    def time_steps(opts, n, warm=warmup):
         for attempt in range(2):
             try:
                 sim[0].set_launch(**opts)
-                if warmup:
-                    sim[0].step(warmup)
+                if warm:
+                    sim[0].step(warm)
                 sim[0].step(n)
                 break
             except BadCellError:
@@ -109,7 +114,14 @@ def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=Non
         ms, _ = sim[0].last_timing()
         return ms / n
 
+    def heat(opts):
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < heat_seconds:
+            time_steps(opts, 10, 0)
+
     table = []
+    if cands:
+        heat(dict(cands[0], use_graph=True, per_block_launch=per_block_launch))
     for c in cands:
         opts = dict(c, use_graph=True, per_block_launch=per_block_launch)
         table.append((time_steps(opts, steps), opts))
@@ -126,6 +138,7 @@ def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=Non
         for _ in range(2):
             for q, opts in enumerate(top):
                 sim[0] = fresh()
+                heat(opts)
                 best_ms[q] = min(best_ms[q], time_steps(opts, refine_steps))
                 sim[0].close()
         best = top[min(range(len(top)), key=lambda q: best_ms[q])]
