@@ -146,13 +146,6 @@ def test_sbp_asf_matches_oracle_and_launch_options():
         assert np.array_equal(a, b)
 
 
-def test_sbp_asf_gradient_is_rejected():
-    sim = Sim(cases.sbp_channel(max_step=5))
-    with pytest.raises(ChknError, match="adjoint of the SBP-ASF flux"):
-        sim.grad_forward(2, 2)
-    sim.close()
-
-
 def test_reacting_gas_matches_oracle():
     # The A->B reacting gas build (libchkn_ab.so): gas constants and the reaction step.
     compare(cases.ab_reacting(max_step=30), 30)
@@ -169,13 +162,6 @@ def test_reacting_ensemble_and_launch_options():
         for b in range(len(job.blocks)):
             assert np.array_equal(single[b], ens[off + b])
         off += len(job.blocks)
-
-
-def test_reacting_gradient_is_rejected():
-    sim = Sim(cases.ab_reacting(max_step=5))
-    with pytest.raises(ChknError, match="adjoint of the A->B reaction"):
-        sim.grad_forward(2, 2)
-    sim.close()
 
 
 def test_viscous_closed_box_conserves_mass_at_size():
