@@ -92,6 +92,8 @@ EXPORTS = {
     "chkn_set_verbosity": (ctypes.c_int, [ctypes.c_int]),
     "chkn_grad_generation": (ctypes.c_longlong, [ctypes.c_void_p]),
     "chkn_grad_set_tape_budget": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
+    "chkn_grad_set_gather_form": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "chkn_grad_gather_form": (ctypes.c_int, [ctypes.c_void_p]),
     "chkn_grad_resident_segments": (ctypes.c_int, [ctypes.c_void_p]),
     "chkn_check_block_dims": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "chkn_device_bytes": (ctypes.c_longlong, [ctypes.c_void_p]),
@@ -372,6 +374,19 @@ class Sim:
         segments (reversed without recomputation): None or a negative value for half of the
         free device memory (the default), 0 to recompute every segment."""
         self._check(self._lib.chkn_grad_set_tape_budget(self.h, -1 if nbytes is None else int(nbytes)))
+
+    def set_gather_form(self, separate):
+        """separate=False (the default): the k-direction face pass of each reversed stage
+        finishes the cells itself where the blocks allow it; True: a separate per-cell gather
+        launch.  The gradient is the same bit for bit."""
+        self._check(self._lib.chkn_grad_set_gather_form(self.h, 1 if separate else 0))
+
+    @property
+    def gather_form(self):
+        """0: the reversed stages run in the fused form; 1: with a separate gather launch."""
+        rc = self._lib.chkn_grad_gather_form(self.h)
+        self._check(min(rc, 0))
+        return rc
 
     @property
     def grad_resident_segments(self):

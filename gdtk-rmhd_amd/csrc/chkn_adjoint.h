@@ -358,6 +358,7 @@ CHKN_HD void face_flux_vjp(const Prim& L1, const Prim& L0, const Prim& R0, const
 // Ub[q] = sum_p Wb[p] dW_p/dU_q.
 CHKN_HD void decode_vjp(const Prim& w, const double Wb[6], double Ub[6])
 {
+#pragma clang fp contract(off)
     const double rho = w.rho;
     const double dinv = 1.0 / rho;
     const double bx = Wb[3] - Wb[1] * w.vx;  // velocity adjoints including e = U4/rho - ke
@@ -409,6 +410,7 @@ CHKN_HD void chem_vjp(const Prim& wr, double f, double Ub[6])
 // reflection; rho, e and YB pass through.
 CHKN_HD void reflect_vjp(const double Gb[6], const Frame& f, double Sb[6])
 {
+#pragma clang fp contract(off)
     Prim g;
     g.rho = Gb[0]; g.e = Gb[1]; g.YB = Gb[2]; g.vx = Gb[3]; g.vy = Gb[4]; g.vz = Gb[5];
     const Prim r = reflect(g, f);

@@ -16,6 +16,12 @@
 //      transpose of the TVD-RK3 stage combination (block.cu:1090-1160).  A cell next to an
 //      inflow boundary also adds the adjoint of its line of inflow ghosts into that line's
 //      accumulator (the gradient of the inflow state).
+// Fused form (no block exchanges through its k sides): the i and j patches run first (step 1
+// for those directions), then the k patches, whose owned cells end with step 2 for themselves
+// -- the i and j sums from memory, the k sum and the k ghost lines from the patch's own LDS --
+// with the same operations in the same order as the separate gather.  A k patch reads D of the
+// cells its neighbouring patches finish, so D is double-buffered: a stage reads it from one
+// buffer (A.dr) and its cells write the next stage's into the other (A.dw).
 // No atomics: every sum has one writer and a fixed order, so the gradient is deterministic.
 // Gradients need no bit-exact arithmetic, so this file is built with floating-point
 // contraction, reciprocal arithmetic and approximate (Newton-refined reciprocal) division
@@ -31,6 +37,7 @@
 #include <mutex>
 #include <vector>
 
+#include "../../include/chkn.h"
 #include "chkn_adjoint.h"
 #include "chkn_internal.h"
 #include "chkn_physics.h"
@@ -102,7 +109,7 @@ __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, in
     const double* g = geom(B, d);
     const int fi = valid ? face_index(B, d, m, a, c) : 0;
     if (valid) {
-        const double* D = B->adj + 12 * n;
+        const double* D = B->adj + (long long)A.dr * n;
         const double area = g[fi];
         const int cl = (m - 1 >= 0) ? cell_index(nic, njc, d, m - 1, a, c) : -1;
         const int cr = (m < nline) ? cell_index(nic, njc, d, m, a, c) : -1;
@@ -203,11 +210,134 @@ __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, in
     }
 }
 
+// What the ghosts of one side of a line return to the cell next to that side, gb = the sum of
+// both ghost depths: wall and outflow ghosts are copies of the cell (walls through the
+// reflection), inflow ghosts hold the inflow state, whose line accumulator this cell owns.
+__device__ __forceinline__ void own_side_return(const AdjArgs& A, const BlockDev* B, int d, int side, int a,
+                                                int c, int line, const double gb[6], double Wb[6])
+{
+#pragma clang fp contract(off)
+    const int code = B->bc[2 * d + side];
+    if (code == CHKN_BC_INFLOW) {
+        const long long col = B->icol + B->icol_side[2 * d + side] + line;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) A.inflow_acc[(long long)q * A.nicols + col] += gb[q];
+    } else if (code == CHKN_BC_OUTFLOW) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Wb[q] += gb[q];
+    } else {
+        const double* g; int nf, fi;
+        boundary_face(B, d, side, a, c, g, nf, fi);
+        reflect_vjp(gb, ld_frame(g, nf, fi), Wb);
+    }
+}
+
+__device__ __forceinline__ bool returns_to_cell(int code)
+{
+    return code != CHKN_BC_EXCHANGE && code != CHKN_BC_INFLOW_FUNCTION;
+}
+
+// The ghost-line adjoints of direction d that come back to cell ijk, from the ghost buffers:
+// the block's own sides, then the exchange ghosts of other blocks (or this one, periodically)
+// that read the cell -- the plus-side ghosts of P at depth m read cells m = 0, 1; the
+// minus-side ghosts of Q at depth nline-1-m read cells m = nline-1, nline-2.
+__device__ __forceinline__ void line_returns(const AdjArgs& A, const BlockDev* B, int d, const int ijk[3],
+                                             double Wb[6])
+{
+#pragma clang fp contract(off)
+    const int m = ijk[d];
+    const int a = d == 0 ? ijk[1] : ijk[0];
+    const int c = d == 2 ? ijk[1] : ijk[2];
+    const int nline = line_len(B, d);
+    const int line = a + (d == 0 ? B->njc : B->nic) * c;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        if (m != (side ? nline - 1 : 0) || !returns_to_cell(B->bc[2 * d + side])) continue;
+        const double* gbuf = ghost_buffer(B, 2 * d + side);
+        const int nt = side_lines(B, 2 * d + side);
+        double gb[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) gb[q] = gbuf[(long long)q * nt + line] + gbuf[(long long)(6 + q) * nt + line];
+        own_side_return(A, B, d, side, a, c, line, gb, Wb);
+    }
+    const int P = B->rp[2 * d];
+    if (P >= 0 && m <= 1) {
+        const BlockDev* O = A.blocks + P;
+        const double* gbuf = ghost_buffer(O, 2 * d + 1) + (long long)(6 * m) * side_lines(O, 2 * d + 1);
+        const int nt = side_lines(O, 2 * d + 1);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Wb[q] += gbuf[(long long)q * nt + line];
+    }
+    const int Q = B->rp[2 * d + 1];
+    if (Q >= 0 && m >= nline - 2) {
+        const BlockDev* O = A.blocks + Q;
+        const int depth = nline - 1 - m;
+        const double* gbuf = ghost_buffer(O, 2 * d) + (long long)(6 * depth) * side_lines(O, 2 * d);
+        const int nt = side_lines(O, 2 * d);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Wb[q] += gbuf[(long long)q * nt + line];
+    }
+}
+
+// What a cell's stage combination reads besides its adjoint: the stage state and A0, P.
+struct CellIn {
+    Prim w;
+    double a0[6], p[6];
+};
+
+template <int STAGE>
+__device__ __forceinline__ void load_cell(const AdjArgs& A, const BlockDev* B, int cidx, CellIn& c)
+{
+    const long long n = B->n;
+    const unsigned o = boff(cidx);
+    c.w = ld_prim(B->W + A.slot * B->wstride, n, cidx);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        c.a0[q] = gld(B->adj + (long long)q * n, o);
+        c.p[q] = STAGE > 1 ? gld(B->adj + (long long)(6 + q) * n, o) : 0.0;
+    }
+}
+
+// The cell's adjoint of W_s to that of U_s (decode_vjp), then the transposed stage
+// combination (block.cu:1090-1160); D is written to the buffer A.dw.
+template <int STAGE>
+__device__ __forceinline__ void finish_cell(const AdjArgs& A, const BlockDev* B, int cidx, const double Wb[6],
+                                            const CellIn& c)
+{
+#pragma clang fp contract(off)
+    const long long n = B->n;
+    const unsigned o = boff(cidx);
+    double Ub[6];
+    decode_vjp(c.w, Wb, Ub);
+    const double dt = A.dt[B->inst];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        double* A0 = B->adj + (long long)q * n;
+        double* Pp = B->adj + (long long)(6 + q) * n;
+        double* D = B->adj + (long long)(A.dw + q) * n;
+        const double a0 = c.a0[q] + Ub[q];
+        gst(A0, o, a0);
+        if (STAGE == 3) {
+            const double v = c.p[q] + 0.25 * dt * Ub[q];
+            gst(D, o, v);
+            gst(Pp, o, v);
+        } else if (STAGE == 2) {
+            gst(D, o, c.p[q] + dt * Ub[q]);
+        } else if (A.dt_next) {
+            const double dtn = A.dt_next[B->inst];
+            gst(Pp, o, A.seed_p * dtn * a0);
+            gst(D, o, A.seed_d * dtn * a0);
+        }
+    }
+}
+
 // Face pass over one patch (see AdjPatch).  Lanes run along i in every direction so that the
 // stencil, D and geometry loads of a wave are contiguous: in direction 0 a lane is a face
 // position along the line, in directions 1 and 2 a line.
 // Three work-groups per CU fit the LDS (48 KB each), three waves per SIMD.
-template <int XORDER, int FLUX>
+// EPI 0: the owned cells' sums go to Gc[dir] and the ghost lines' to GB.  EPI = s (1..3), k
+// patches of the fused form only: each owned cell finishes stage s itself (see the header).
+template <int XORDER, int FLUX, int EPI>
 __global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjArgs Ain, const BlockDev* __restrict__ blocks,
                                                                           const AdjPatch* __restrict__ patches)
 {
@@ -219,21 +349,62 @@ __global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjA
     const int t = (int)threadIdx.x;
     const AdjPatch P = A.patches[blockIdx.x];
     const BlockDev* B = A.blocks + P.blk;
-    const int d = P.dir;
+    const int d = EPI > 0 ? 2 : P.dir;
     const int tm = d == 0 ? t % kAdjPatchM : t / kAdjPatchT;  // face position in the patch
     const int ta = d == 0 ? t / kAdjPatchM : t % kAdjPatchT;  // line in the patch
     const int a = P.a0 + ta, c = P.c;
     face_vjp<XORDER, FLUX>(A, B, d, P.m0 - 1 + tm, a, c, st, t);
-    __syncthreads();
     // Owned cell m = m0 + tm of line (a, c): faces m-1 .. m+2 sit at positions tm .. tm+3.
     const int nline = line_len(B, d);
     const int na = d == 0 ? B->njc : B->nic;
     const int m = P.m0 + tm;
-    if (tm >= kAdjPatchOwn || m >= nline || a >= na) return;
+    const bool own = tm < kAdjPatchOwn && m < nline && a < na;
+    const long long n = B->n;
+    const int cidx = own ? cell_index(B->nic, B->njc, d, m, a, c) : 0;
+    // Fused form: the owned cell's i and j sums and its stage inputs are loaded before the
+    // barrier, while the registers of the flux adjoint are free and other waves finish theirs.
+    double gij[6];
+    CellIn cin;
+    if constexpr (EPI > 0) {
+        if (own) {
+            const unsigned o = boff(cidx);
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                gij[q] = gld_nt(B->Gc + (long long)q * n, o) + gld_nt(B->Gc + (long long)(6 + q) * n, o);
+            load_cell<EPI>(A, B, cidx, cin);
+        }
+    }
+    __syncthreads();
+    if (!own) return;
     auto lane = [&](int pos) { return d == 0 ? pos + kAdjPatchM * ta : ta + kAdjPatchT * pos; };
     const int l3 = lane(tm), l2 = lane(tm + 1), l1 = lane(tm + 2), l0 = lane(tm + 3);
-    const long long n = B->n;
-    const int cidx = cell_index(B->nic, B->njc, d, m, a, c);
+    const int line = a + na * c;
+    if constexpr (EPI > 0) {
+        // The cell (i, j, k) = (a, c, m): its three direction sums, the ghost lines of i and j
+        // from the buffers, those of k from this patch, then the stage combination.
+        double Wb[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Wb[q] = gij[q] + (((st[18 + q][l3] + st[12 + q][l2]) + st[6 + q][l1]) + st[q][l0]);
+        const int ijk[3] = {a, c, m};
+        line_returns(A, B, 0, ijk, Wb);
+        line_returns(A, B, 1, ijk, Wb);
+        if (m == 0 && returns_to_cell(B->bc[4])) {
+            const int f0 = lane(tm + 1), f1 = lane(tm + 2);
+            double gb[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) gb[q] = (st[6 + q][f0] + st[q][f1]) + st[q][f0];
+            own_side_return(A, B, 2, 0, a, c, line, gb, Wb);
+        }
+        if (m == nline - 1 && returns_to_cell(B->bc[5])) {
+            const int fa = lane(tm + 1), fb = lane(tm + 2);
+            double gb[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) gb[q] = (st[18 + q][fa] + st[12 + q][fb]) + st[18 + q][fb];
+            own_side_return(A, B, 2, 1, a, c, line, gb, Wb);
+        }
+        finish_cell<EPI>(A, B, cidx, Wb, cin);
+        return;
+    }
     double* Gd = B->Gc + 6 * (long long)d * n;
 #pragma unroll
     for (int q = 0; q < 6; ++q)
@@ -241,7 +412,6 @@ __global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjA
     // Ghost lines: what lands on the ghosts at depth 0 and 1 of the line's minus side (faces 0
     // and 1 as L0, L1 and face 0 as L1) and plus side (faces n-1 and n as R1, R0 and face n as
     // R1).
-    const int line = a + na * c;
     if (m == 0) {
         const int f0 = lane(tm + 1), f1 = lane(tm + 2);
         double* gb = ghost_buffer(B, 2 * d);
@@ -285,82 +455,10 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjArgs Ain, const Bloc
         Wb[q] = (gld_nt(B->Gc + (long long)q * n, o) + gld_nt(B->Gc + (long long)(6 + q) * n, o)) +
                 gld_nt(B->Gc + (long long)(12 + q) * n, o);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const int m = ijk[d];
-        const int a = d == 0 ? ijk[1] : ijk[0];
-        const int c = d == 2 ? ijk[1] : ijk[2];
-        const int nline = line_len(B, d);
-        const int line = a + (d == 0 ? njc : nic) * c;
-        // Wall, outflow and inflow ghosts of this block hold copies of (or fixed values next
-        // to) the cell next to the boundary: both depths come back to it.
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            if (m != (side ? nline - 1 : 0)) continue;
-            const int code = B->bc[2 * d + side];
-            if (code == 3 || code == 6) continue;
-            const double* gbuf = ghost_buffer(B, 2 * d + side);
-            const int nt = side_lines(B, 2 * d + side);
-            double gb[6];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) gb[q] = gbuf[(long long)q * nt + line] + gbuf[(long long)(6 + q) * nt + line];
-            if (code == 4) {
-                // Inflow ghosts hold the inflow state: this cell owns its line's accumulator.
-                const long long col = B->icol + B->icol_side[2 * d + side] + line;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) A.inflow_acc[(long long)q * A.nicols + col] += gb[q];
-            } else if (code == 5) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) Wb[q] += gb[q];
-            } else {
-                const double* g; int nf, fi;
-                boundary_face(B, d, side, a, c, g, nf, fi);
-                reflect_vjp(gb, ld_frame(g, nf, fi), Wb);
-            }
-        }
-        // Exchange ghosts of other blocks (or this one, periodically) that read the cell: the
-        // plus-side ghosts of P at depth m read cells m = 0, 1; the minus-side ghosts of Q at
-        // depth nline-1-m read cells m = nline-1, nline-2.
-        const int P = B->rp[2 * d];
-        if (P >= 0 && m <= 1) {
-            const BlockDev* O = A.blocks + P;
-            const double* gbuf = ghost_buffer(O, 2 * d + 1) + (long long)(6 * m) * side_lines(O, 2 * d + 1);
-            const int nt = side_lines(O, 2 * d + 1);
-#pragma unroll
-            for (int q = 0; q < 6; ++q) Wb[q] += gbuf[(long long)q * nt + line];
-        }
-        const int Q = B->rp[2 * d + 1];
-        if (Q >= 0 && m >= nline - 2) {
-            const BlockDev* O = A.blocks + Q;
-            const int depth = nline - 1 - m;
-            const double* gbuf = ghost_buffer(O, 2 * d) + (long long)(6 * depth) * side_lines(O, 2 * d);
-            const int nt = side_lines(O, 2 * d);
-#pragma unroll
-            for (int q = 0; q < 6; ++q) Wb[q] += gbuf[(long long)q * nt + line];
-        }
-    }
-    const Prim w = ld_prim(B->W + A.slot * B->wstride, n, cidx);
-    double Ub[6];
-    decode_vjp(w, Wb, Ub);
-    const double dt = A.dt[B->inst];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        double* A0 = B->adj + (long long)q * n;
-        double* Pp = B->adj + (long long)(6 + q) * n;
-        double* D = B->adj + (long long)(12 + q) * n;
-        const double a0 = gld(A0, o) + Ub[q];
-        gst(A0, o, a0);
-        if (STAGE == 3) {
-            const double v = gld(Pp, o) + 0.25 * dt * Ub[q];
-            gst(D, o, v);
-            gst(Pp, o, v);
-        } else if (STAGE == 2) {
-            gst(D, o, gld(Pp, o) + dt * Ub[q]);
-        } else if (A.dt_next) {
-            const double dtn = A.dt_next[B->inst];
-            gst(Pp, o, A.seed_p * dtn * a0);
-            gst(D, o, A.seed_d * dtn * a0);
-        }
-    }
+    for (int d = 0; d < 3; ++d) line_returns(A, B, d, ijk, Wb);
+    CellIn cin;
+    load_cell<STAGE>(A, B, cidx, cin);
+    finish_cell<STAGE>(A, B, cidx, Wb, cin);
 }
 
 __global__ __launch_bounds__(256) void adj_seed_kernel(AdjArgs Ain, const BlockDev* __restrict__ blocks,
@@ -377,7 +475,7 @@ __global__ __launch_bounds__(256) void adj_seed_kernel(AdjArgs Ain, const BlockD
     const double dt = A.dt[B->inst];
     const double* A0 = B->adj;
     double* Pp = B->adj + 6 * (long long)n;
-    double* D = B->adj + 12 * (long long)n;
+    double* D = B->adj + (long long)A.dr * n;  // read by the next face pass
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const long long o = (long long)q * n + cidx;
@@ -400,7 +498,7 @@ __global__ __launch_bounds__(256) void adj_chem_seed_kernel(AdjArgs Ain, const B
     const double dt = A.dt[B->inst];
     double* A0 = B->adj;
     double* Pp = B->adj + 6 * (long long)n;
-    double* D = B->adj + 12 * (long long)n;
+    double* D = B->adj + (long long)A.dr * n;  // read by the next face pass
     double Ub[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) Ub[q] = A0[(long long)q * n + cidx];
@@ -448,13 +546,24 @@ hipError_t launch_adj_chem_seed(const AdjArgs& a, int ncell_chunks, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, int flux_calc, hipStream_t s)
+namespace {
+template <int XORDER, int FLUX>
+void launch_patches(const AdjArgs& a, int npatches, int epi, hipStream_t s)
+{
+    const dim3 grid(npatches), blk(kAdjPatchM * kAdjPatchT);
+    if (epi == 3) hipLaunchKernelGGL((adj_patch_kernel<XORDER, FLUX, 3>), grid, blk, 0, s, a, a.blocks, a.patches);
+    else if (epi == 2) hipLaunchKernelGGL((adj_patch_kernel<XORDER, FLUX, 2>), grid, blk, 0, s, a, a.blocks, a.patches);
+    else if (epi == 1) hipLaunchKernelGGL((adj_patch_kernel<XORDER, FLUX, 1>), grid, blk, 0, s, a, a.blocks, a.patches);
+    else hipLaunchKernelGGL((adj_patch_kernel<XORDER, FLUX, 0>), grid, blk, 0, s, a, a.blocks, a.patches);
+}
+}  // namespace
+
+hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, int flux_calc, int epi, hipStream_t s)
 {
     if (npatches <= 0) return hipSuccess;
-    const dim3 blk(kAdjPatchM * kAdjPatchT);
-    if (flux_calc == 1) hipLaunchKernelGGL((adj_patch_kernel<2, 1>), dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
-    else if (x_order > 1) hipLaunchKernelGGL((adj_patch_kernel<2, 0>), dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
-    else hipLaunchKernelGGL((adj_patch_kernel<1, 0>), dim3(npatches), blk, 0, s, a, a.blocks, a.patches);
+    if (flux_calc == 1) launch_patches<2, 1>(a, npatches, epi, s);
+    else if (x_order > 1) launch_patches<2, 0>(a, npatches, epi, s);
+    else launch_patches<1, 0>(a, npatches, epi, s);
     return hipGetLastError();
 }
 

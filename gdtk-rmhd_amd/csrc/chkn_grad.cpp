@@ -33,21 +33,24 @@ void build_adj_tables(chkn_sim* s)
 {
     GradState& g = *s->grad;
     if (g.d_face_chunks) return;
-    std::vector<AdjPatch> fc;
+    std::vector<AdjPatch> fc, fck;
     std::vector<CellChunk> cc;
     std::vector<InflowCol> ic;
+    g.fused_ok = true;
     for (size_t ib = 0; ib < s->blocks.size(); ++ib) {
         const HostBlock& b = s->blocks[ib];
         if (!b.active) continue;
+        if (s->bdev[ib].rp[4] >= 0 || s->bdev[ib].rp[5] >= 0) g.fused_ok = false;
         // Face patches per direction: lines of kAdjPatchT first transverse indices, each
-        // covering kAdjPatchOwn cells along the line.
+        // covering kAdjPatchOwn cells along the line; i and j patches of every block first.
         for (int d = 0; d < 3; ++d) {
             const int nline = d == 0 ? b.nic : (d == 1 ? b.njc : b.nkc);
             const int na = d == 0 ? b.njc : b.nic;
             const int nc = d == 2 ? b.njc : b.nkc;
             for (int c = 0; c < nc; ++c)
                 for (int a0 = 0; a0 < na; a0 += kAdjPatchT)
-                    for (int m0 = 0; m0 < nline; m0 += kAdjPatchOwn) fc.push_back(AdjPatch{(int)ib, d, m0, a0, c});
+                    for (int m0 = 0; m0 < nline; m0 += kAdjPatchOwn)
+                        (d == 2 ? fck : fc).push_back(AdjPatch{(int)ib, d, m0, a0, c});
         }
         for (int c0 = 0; c0 < b.n; c0 += 256) cc.push_back(CellChunk{(int)ib, c0, std::min(c0 + 256, b.n)});
         BlockDev& bd = s->bdev[ib];
@@ -64,6 +67,8 @@ void build_adj_tables(chkn_sim* s)
             }
         }
     }
+    g.nface_ij = (int)fc.size();
+    fc.insert(fc.end(), fck.begin(), fck.end());
     g.nface_chunks = (int)fc.size();
     g.ncell_chunks = (int)cc.size();
     g.ninflow_cols = (int)ic.size();
@@ -264,7 +269,7 @@ void alloc_adjoint(chkn_sim* s, int C, bool ring)
         if (!b.adj) {
             // Ghost lines of the six sides, two depths of six values each.
             const size_t lines = 2 * ((size_t)b.njc * b.nkc + (size_t)b.nic * b.nkc + (size_t)b.nic * b.njc);
-            HIPCHK(hipMalloc(&b.adj, sizeof(double) * 18 * (size_t)b.n));
+            HIPCHK(hipMalloc(&b.adj, sizeof(double) * 24 * (size_t)b.n));
             HIPCHK(hipMalloc(&b.Gc, sizeof(double) * 18 * (size_t)b.n));
             HIPCHK(hipMalloc(&b.GB, sizeof(double) * 12 * lines));
             s->bdev[ib].adj = b.adj;
@@ -550,6 +555,12 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     // Seeds of a reversed step: the weights of the stages' residuals in U^{n+1}.
     A.seed_p = s->time_scheme == CHKN_TIME_PC ? 0.5 : kOneSixth;
     A.seed_d = s->time_scheme == CHKN_TIME_PC ? 0.5 : kFourSixth;
+    // D is double-buffered in the fused form (the k patches finish cells whose D neighbouring
+    // patches still read): dcur is the buffer the next face pass reads, which seeds write.
+    const bool fused = g.fused_ok && !g.separate_gather;
+    int dcur = 12;
+    A.dr = A.dw = dcur;
+    const AdjPatch* kpatches = (const AdjPatch*)g.d_face_chunks + g.nface_ij;
     long long launches = 0;
     int rc = CHKN_OK;
     // Segments below first_res are recomputed.  With two tapes, segment j-1 is recomputed on
@@ -578,6 +589,7 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
             const int step = j * C + q;
             A.dt = g.d_dts + (size_t)step * N;
             const int nst = stages_per_step(s);
+            A.dr = A.dw = dcur;
             if (s->any_reacting) {
                 // The step's reaction first (its state after the reaction is the next step's
                 // first slot), then the seeds; the stage-1 gathers do not seed.
@@ -597,8 +609,20 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
             // gathers as the TVD-RK3 stage 2 (D = P + dt Ubar), its predictor as stage 1.
             for (int pos = nst; pos >= 1; --pos) {
                 A.slot = nst * q + pos - 1;
-                HIPCHK(launch_adj_faces(A, g.nface_chunks, s->x_order, s->flux_calc, s->stream));
-                HIPCHK(launch_adj_gather(A, g.ncell_chunks, pos, s->stream));
+                A.dr = dcur;
+                if (fused) {
+                    A.dw = 30 - dcur;  // 12 <-> 18
+                    HIPCHK(launch_adj_faces(A, g.nface_ij, s->x_order, s->flux_calc, 0, s->stream));
+                    AdjArgs Ak = A;
+                    Ak.patches = kpatches;
+                    HIPCHK(launch_adj_faces(Ak, g.nface_chunks - g.nface_ij, s->x_order, s->flux_calc, pos,
+                                            s->stream));
+                } else {
+                    A.dw = dcur;
+                    HIPCHK(launch_adj_faces(A, g.nface_chunks, s->x_order, s->flux_calc, 0, s->stream));
+                    HIPCHK(launch_adj_gather(A, g.ncell_chunks, pos, s->stream));
+                }
+                dcur = A.dw;
                 launches += 2;
             }
         }
@@ -697,6 +721,24 @@ int chkn_grad_set_tape_budget(chkn_sim* sim, long long bytes)
         grad_state(sim).tape_budget = bytes < 0 ? -1 : bytes;
         return CHKN_OK;
     });
+}
+
+int chkn_grad_set_gather_form(chkn_sim* sim, int separate)
+{
+    if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
+    return guarded(sim, [&] {
+        grad_state(sim).separate_gather = separate != 0;
+        return CHKN_OK;
+    });
+}
+
+int chkn_grad_gather_form(chkn_sim* sim)
+{
+    if (!sim) return set_error(nullptr, CHKN_ERR_ARG, "null handle");
+    if (sim->grad && sim->grad->separate_gather) return 1;
+    for (size_t ib = 0; ib < sim->blocks.size(); ++ib)
+        if (sim->blocks[ib].active && (sim->bdev[ib].rp[4] >= 0 || sim->bdev[ib].rp[5] >= 0)) return 1;
+    return 0;
 }
 
 int chkn_grad_resident_segments(chkn_sim* sim)

@@ -41,6 +41,8 @@ long long chkn_grad_generation(chkn_sim* sim)
     return -1;
 }
 int chkn_grad_set_tape_budget(chkn_sim* sim, long long) { return off(sim); }
+int chkn_grad_set_gather_form(chkn_sim* sim, int) { return off(sim); }
+int chkn_grad_gather_form(chkn_sim* sim) { return off(sim); }
 int chkn_grad_resident_segments(chkn_sim* sim)
 {
     off(sim);

@@ -62,7 +62,7 @@ struct BlockDev {
     double* Gc;            // per direction d, the sum over the block's own faces of the adjoint
                            // reaching each cell through their stencils: [3][6][n]
     double* GB;            // adjoint reaching each side's ghost lines: per side [2 depths][6][nt]
-    double* adj;           // A0 (adjoint of U at the step boundary), P, D: [3][6][n]
+    double* adj;           // A0 (adjoint of U at the step boundary), P, D, D': [4][6][n]
     // face-list stage
     real* F;
     real* FS;
@@ -145,6 +145,8 @@ struct AdjArgs {
                               // 1/6 and 4/6 (TVD-RK3), 1/2 and 1/2 (predictor-corrector)
     const double* chem;       // [ninst] exp(-alpha dt) of the step being reversed (< 0: no
                               // reaction), for the reacting A->B gas
+    int dr, dw;               // component offsets in adj of the D this stage reads (faces) and
+                              // writes (cells): 12 or 18; equal in the separate-gather form
 };
 
 // Adjoint of one stage's residual: face pass (per patch, every face's stencil adjoints summed
@@ -158,7 +160,10 @@ hipError_t launch_adj_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s);
 // it (chem_vjp, with the state after the reaction in W slot a.slot), then the seeds of the
 // reversed step as launch_adj_seed.
 hipError_t launch_adj_chem_seed(const AdjArgs& a, int ncell_chunks, hipStream_t s);
-hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, int flux_calc, hipStream_t s);
+// epi 0: the face pass of the separate-gather form (sums to Gc and GB); epi = stage 1..3: k
+// patches of the fused form, whose owned cells finish that stage (the i and j patches of the
+// stage ran before with epi 0).
+hipError_t launch_adj_faces(const AdjArgs& a, int npatches, int x_order, int flux_calc, int epi, hipStream_t s);
 hipError_t launch_adj_gather(const AdjArgs& a, int ncell_chunks, int stage, hipStream_t s);
 // U and W = decode(U) of every active block from U (after the conserved state was replaced).
 hipError_t launch_decode_all(const BlockDev* blocks, const CellChunk* cells, int ncell_chunks,

@@ -106,6 +106,12 @@ struct GradState {
     long long generation = 0;     // number of successful chkn_grad_forward calls on the handle
     // face/cell work tables and inflow patches (device)
     int nface_chunks = 0, ncell_chunks = 0, ninflow_cols = 0;  // nface_chunks: face patches
+    // Patches are ordered i and j first (nface_ij of them), then k.  fused_ok: no active block
+    // exchanges through a k side, so the k patches can finish their cells (the fused form);
+    // separate_gather: the caller asked for the separate gather launch anyway.
+    int nface_ij = 0;
+    bool fused_ok = false;
+    bool separate_gather = false;
     void* d_face_chunks = nullptr;
     void* d_cell_chunks = nullptr;
     void* d_inflow_cols = nullptr;

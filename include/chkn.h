@@ -229,6 +229,14 @@ int chkn_grad_forward(chkn_sim* sim, int nsteps, int checkpoint_every, const dou
  * recomputes every segment.  The gradient does not depend on it.  No counterpart in the
  * reference, which has no adjoint. */
 int chkn_grad_set_tape_budget(chkn_sim* sim, long long bytes);
+/* Form of each reversed stage: separate = 0 (the default) lets the k-direction face pass
+ * finish the cells itself when no block exchanges through a k side; separate = 1 always
+ * launches the face pass of every direction and then a per-cell gather.  Both give the same
+ * gradient bit for bit.  No counterpart in the reference, which has no adjoint. */
+int chkn_grad_set_gather_form(chkn_sim* sim, int separate);
+/* The form the handle's reversed stages run in: 0 fused, 1 separate (asked for, or a block
+ * exchanges through a k side); < 0 on error. */
+int chkn_grad_gather_form(chkn_sim* sim);
 /* Checkpoint segments of the last chkn_grad_forward held on resident tapes. */
 int chkn_grad_resident_segments(chkn_sim* sim);
 /* Number of successful chkn_grad_forward calls on the handle: chkn_grad_backward reverses the
