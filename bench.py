@@ -305,9 +305,9 @@ def main():
     jobs, desc = build_workload(args.workload, args.warmup + steps)
     tuned = None
     # The measuring handle is built before the tuner runs, so that its warmup and timed steps
-    # follow the tuner's launches without the idle gap of building a 4M-cell handle: after a
-    # few seconds idle the device needs some 50 ms of work to return to its full clocks (the
-    # first 25 steps of a fresh 256x128x128 run take 2.4% longer after such a gap).
+    # follow the tuner's launches without the idle gap of building a 4M-cell handle (on some
+    # devices the first 25 steps of a fresh 256x128x128 run take up to 2.4% longer after such
+    # a gap).
     sim = new_sim(jobs) if args.workload not in ("diff", "diff-pc", "ensemble-diff") else None
     if args.tile:
         ti, tj, tk = (int(v) for v in args.tile.split("x"))

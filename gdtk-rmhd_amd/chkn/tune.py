@@ -74,10 +74,11 @@ def autotune(job=None, jobs=None, steps=10, warmup=2, cands=None, cache_path=Non
     on a fresh handle from the initial state), and
     return (best_options, [(ms_per_step, options), ...] of the sweep, fastest first).  The
     short sweep cannot separate shapes a few percent apart; the longer runs pick among them.
-    Building a handle leaves the device idle long enough to lower its clocks: the first 25
-    steps of a fresh 256x128x128 block then run 2.4% slower, more than the shapes being ranked
-    differ.  So the sweep's first candidate and every refine run are timed only after
-    `heat_seconds` of untimed steps.
+    The first steps of a fresh handle run slower than later ones (the flow's first steps cost
+    more, and on some devices the clocks are still low after the idle time of building the
+    handle: up to 2.4% on a 256x128x128 block, more than the shapes being ranked differ), so
+    the sweep's first candidate and every refine run are timed only after `heat_seconds` of
+    untimed steps.
 
     Timing is the device time of the step kernels (HIP events around the launches).  The
     scratch handle's state advances during the sweep; a candidate that meets a bad cell
