@@ -69,127 +69,127 @@ CHKN_HD void ausmdv_vjp(double rhoL, double eL, double pL, double aL, double YBL
     double uL, vL, wL, uR, vR, wR;
     to_local(f, vxL, vyL, vzL, uL, vL, wL);
     to_local(f, vxR, vyR, vzR, uR, vR, wR);
-    const double pLrL = pL / rhoL, pRrR = pR / rhoR;
+    const double prL = pL / rhoL, prR = pR / rhoR;
     const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
     const double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
-    const double HL = eL + pLrL + keL, HR = eR + pRrR + keR;
-    const double S = pLrL + pRrR;
-    const double alphaL = 2.0 * pLrL / S, alphaR = 2.0 * pRrR / S;
-    const double am = fmax(aL, aR);
-    const double ML = uL / am, MR = uR / am;
-    const double dvL = 0.5 * (uL + fabs(uL));
-    const bool subL = fabs(ML) <= 1.0;
-    const double uLplus = subL ? alphaL * ((uL + am) * (uL + am) / (4.0 * am) - dvL) + dvL : dvL;
-    const double dvR = 0.5 * (uR - fabs(uR));
-    const bool subR = fabs(MR) <= 1.0;
-    const double uRminus = subR ? alphaR * (-(uR - am) * (uR - am) / (4.0 * am) - dvR) + dvR : dvR;
-    const double massL = uLplus * rhoL, massR = uRminus * rhoR;
-    const double m = massL + massR;
-    const double g = pL - pR;
+    const double hL = eL + prL + keL, hR = eR + prR + keR;
+    const double S = prL + prR;
+    const double wdL = 2.0 * prL / S, wdR = 2.0 * prR / S;
+    const double cmax = fmax(aL, aR);
+    const double machL = uL / cmax, machR = uR / cmax;
+    const double upL = 0.5 * (uL + fabs(uL));
+    const bool subL = fabs(machL) <= 1.0;
+    const double usL = subL ? wdL * ((uL + cmax) * (uL + cmax) / (4.0 * cmax) - upL) + upL : upL;
+    const double unR = 0.5 * (uR - fabs(uR));
+    const bool subR = fabs(machR) <= 1.0;
+    const double usR = subR ? wdR * (-(uR - cmax) * (uR - cmax) / (4.0 * cmax) - unR) + unR : unR;
+    const double mfL = usL * rhoL, mfR = usR * rhoR;
+    const double mdot = mfL + mfR;
+    const double pdiff = pL - pR;
     const double pmin = fmin(pL, pR);
-    const double dp = 10.0 * fabs(g) / pmin;
-    const double s = 0.5 * fmin(1.0, dp);
-    const double rvV = massL * uL + massR * uR;
-    const double rvD = 0.5 * (m * (uL + uR) - fabs(m) * (uR - uL));
-    const bool up = (m >= 0.0);
-    const double vt = up ? vL : vR, wt = up ? wL : wR;
-    const double Hu = up ? HL : HR, Yu = up ? YBL : YBR;
+    const double pjump = 10.0 * fabs(pdiff) / pmin;
+    const double blend = 0.5 * fmin(1.0, pjump);
+    const double momV = mfL * uL + mfR * uR;
+    const double momD = 0.5 * (mdot * (uL + uR) - fabs(mdot) * (uR - uL));
+    const bool upw = (mdot >= 0.0);
+    const double vt = upw ? vL : vR, wt = upw ? wL : wR;
+    const double hu = upw ? hL : hR, Yu = upw ? YBL : YBR;
 
     // ---- reverse
     double X1, X2, X3;  // adjoint of the local-frame flux (transpose of to_global)
     to_local(f, Fb[1], Fb[2], Fb[3], X1, X2, X3);
-    const double brv2 = X1, bph = X1;
-    double bm = Fb[0] + X2 * vt + X3 * wt + Fb[4] * Hu + Fb[5] * Yu;
-    const double bvt = X2 * m, bwt = X3 * m, bHu = Fb[4] * m, bYu = Fb[5] * m;
+    const double bmom = X1, bpface = X1;
+    double bmdot = Fb[0] + X2 * vt + X3 * wt + Fb[4] * hu + Fb[5] * Yu;
+    const double bvt = X2 * mdot, bwt = X3 * mdot, bhu = Fb[4] * mdot, bYu = Fb[5] * mdot;
     double buL = 0.0, bvL = 0.0, bwL = 0.0, buR = 0.0, bvR = 0.0, bwR = 0.0;
-    double bHL = 0.0, bHR = 0.0;
-    if (up) { bvL += bvt; bwL += bwt; bHL += bHu; bL[4] += bYu; }
-    else    { bvR += bvt; bwR += bwt; bHR += bHu; bR[4] += bYu; }
-    // rv2 = (0.5 + s) rvV + (0.5 - s) rvD
-    const double bs = brv2 * (rvV - rvD);
-    const double brvV = brv2 * (0.5 + s), brvD = brv2 * (0.5 - s);
-    // rvD = 0.5 (m (uL + uR) - |m| (uR - uL))
-    bm += brvD * 0.5 * ((uL + uR) - sgn(m) * (uR - uL));
-    buL += brvD * 0.5 * (m + fabs(m));
-    buR += brvD * 0.5 * (m - fabs(m));
-    // rvV = massL uL + massR uR
-    double bmassL = brvV * uL, bmassR = brvV * uR;
-    buL += brvV * massL;
-    buR += brvV * massR;
-    // s = 0.5 min(1, dp); dp = 10 |g| / pmin
-    if (dp < 1.0) {
-        const double bdp = 0.5 * bs;
-        const double bg = bdp * 10.0 * sgn(g) / pmin;
-        const double bpmin = -bdp * dp / pmin;
-        bL[2] += bg;
-        bR[2] -= bg;
+    double bhL = 0.0, bhR = 0.0;
+    if (upw) { bvL += bvt; bwL += bwt; bhL += bhu; bL[4] += bYu; }
+    else    { bvR += bvt; bwR += bwt; bhR += bhu; bR[4] += bYu; }
+    // mom = (0.5 + blend) momV + (0.5 - blend) momD
+    const double bblend = bmom * (momV - momD);
+    const double bmomV = bmom * (0.5 + blend), bmomD = bmom * (0.5 - blend);
+    // momD = 0.5 (mdot (uL + uR) - |mdot| (uR - uL))
+    bmdot += bmomD * 0.5 * ((uL + uR) - sgn(mdot) * (uR - uL));
+    buL += bmomD * 0.5 * (mdot + fabs(mdot));
+    buR += bmomD * 0.5 * (mdot - fabs(mdot));
+    // momV = mfL uL + mfR uR
+    double bmfL = bmomV * uL, bmfR = bmomV * uR;
+    buL += bmomV * mfL;
+    buR += bmomV * mfR;
+    // blend = 0.5 min(1, pjump); pjump = 10 |pdiff| / pmin
+    if (pjump < 1.0) {
+        const double bpjump = 0.5 * bblend;
+        const double bpdiff = bpjump * 10.0 * sgn(pdiff) / pmin;
+        const double bpmin = -bpjump * pjump / pmin;
+        bL[2] += bpdiff;
+        bR[2] -= bpdiff;
         if (pL <= pR) bL[2] += bpmin;
         else bR[2] += bpmin;
     }
-    // p_half = pLplus + pRminus; m = massL + massR
-    const double bpLplus = bph, bpRminus = bph;
-    bmassL += bm;
-    bmassR += bm;
-    const double buLplus = bmassL * rhoL, buRminus = bmassR * rhoR;
-    bL[0] += bmassL * uLplus;
-    bR[0] += bmassR * uRminus;
-    double bam = 0.0, balphaL = 0.0, balphaR = 0.0, bML = 0.0, bMR = 0.0;
-    double bdvL = 0.0, bdvR = 0.0;
+    // pface = psL + psR; mdot = mfL + mfR
+    const double bpsL = bpface, bpsR = bpface;
+    bmfL += bmdot;
+    bmfR += bmdot;
+    const double busL = bmfL * rhoL, busR = bmfR * rhoR;
+    bL[0] += bmfL * usL;
+    bR[0] += bmfR * usR;
+    double bcmax = 0.0, bwdL = 0.0, bwdR = 0.0, bmachL = 0.0, bmachR = 0.0;
+    double bupL = 0.0, bunR = 0.0;
     if (subL) {
-        const double q1 = (ML + 1.0) * (ML + 1.0);
-        bL[2] += bpLplus * q1 * (2.0 - ML) * 0.25;
-        bML += bpLplus * pL * 0.25 * (2.0 * (ML + 1.0) * (2.0 - ML) - q1);
-        const double Q = (uL + am) * (uL + am) / (4.0 * am);
-        balphaL += buLplus * (Q - dvL);
-        const double bQ = buLplus * alphaL;
-        bdvL += buLplus * (1.0 - alphaL);
-        buL += bQ * 2.0 * (uL + am) / (4.0 * am);
-        bam += bQ * (2.0 * (uL + am) / (4.0 * am) - (uL + am) * (uL + am) / (4.0 * am * am));
+        const double q1 = (machL + 1.0) * (machL + 1.0);
+        bL[2] += bpsL * q1 * (2.0 - machL) * 0.25;
+        bmachL += bpsL * pL * 0.25 * (2.0 * (machL + 1.0) * (2.0 - machL) - q1);
+        const double Q = (uL + cmax) * (uL + cmax) / (4.0 * cmax);
+        bwdL += busL * (Q - upL);
+        const double bQ = busL * wdL;
+        bupL += busL * (1.0 - wdL);
+        buL += bQ * 2.0 * (uL + cmax) / (4.0 * cmax);
+        bcmax += bQ * (2.0 * (uL + cmax) / (4.0 * cmax) - (uL + cmax) * (uL + cmax) / (4.0 * cmax * cmax));
     } else {
-        bL[2] += bpLplus * dvL / uL;
-        bdvL += bpLplus * pL / uL;
-        buL -= bpLplus * pL * dvL / (uL * uL);
-        bdvL += buLplus;
+        bL[2] += bpsL * upL / uL;
+        bupL += bpsL * pL / uL;
+        buL -= bpsL * pL * upL / (uL * uL);
+        bupL += busL;
     }
-    buL += bdvL * 0.5 * (1.0 + sgn(uL));
+    buL += bupL * 0.5 * (1.0 + sgn(uL));
     if (subR) {
-        const double q1 = (MR - 1.0) * (MR - 1.0);
-        bR[2] += bpRminus * q1 * (2.0 + MR) * 0.25;
-        bMR += bpRminus * pR * 0.25 * (2.0 * (MR - 1.0) * (2.0 + MR) + q1);
-        const double Q = -(uR - am) * (uR - am) / (4.0 * am);
-        balphaR += buRminus * (Q - dvR);
-        const double bQ = buRminus * alphaR;
-        bdvR += buRminus * (1.0 - alphaR);
-        buR -= bQ * 2.0 * (uR - am) / (4.0 * am);
-        bam += bQ * (2.0 * (uR - am) / (4.0 * am) + (uR - am) * (uR - am) / (4.0 * am * am));
+        const double q1 = (machR - 1.0) * (machR - 1.0);
+        bR[2] += bpsR * q1 * (2.0 + machR) * 0.25;
+        bmachR += bpsR * pR * 0.25 * (2.0 * (machR - 1.0) * (2.0 + machR) + q1);
+        const double Q = -(uR - cmax) * (uR - cmax) / (4.0 * cmax);
+        bwdR += busR * (Q - unR);
+        const double bQ = busR * wdR;
+        bunR += busR * (1.0 - wdR);
+        buR -= bQ * 2.0 * (uR - cmax) / (4.0 * cmax);
+        bcmax += bQ * (2.0 * (uR - cmax) / (4.0 * cmax) + (uR - cmax) * (uR - cmax) / (4.0 * cmax * cmax));
     } else {
-        bR[2] += bpRminus * dvR / uR;
-        bdvR += bpRminus * pR / uR;
-        buR -= bpRminus * pR * dvR / (uR * uR);
-        bdvR += buRminus;
+        bR[2] += bpsR * unR / uR;
+        bunR += bpsR * pR / uR;
+        buR -= bpsR * pR * unR / (uR * uR);
+        bunR += busR;
     }
-    buR += bdvR * 0.5 * (1.0 - sgn(uR));
-    // ML = uL / am, MR = uR / am
-    buL += bML / am;
-    buR += bMR / am;
-    bam -= (bML * ML + bMR * MR) / am;
-    if (aL >= aR) bL[3] += bam;
-    else bR[3] += bam;
-    // alpha = 2 prho / S, S = pLrL + pRrR
-    double bpLrL = balphaL * 2.0 / S, bpRrR = balphaR * 2.0 / S;
-    const double bS = -(balphaL * alphaL + balphaR * alphaR) / S;
-    bpLrL += bS;
-    bpRrR += bS;
+    buR += bunR * 0.5 * (1.0 - sgn(uR));
+    // machL = uL / cmax, machR = uR / cmax
+    buL += bmachL / cmax;
+    buR += bmachR / cmax;
+    bcmax -= (bmachL * machL + bmachR * machR) / cmax;
+    if (aL >= aR) bL[3] += bcmax;
+    else bR[3] += bcmax;
+    // wd = 2 pr / S, S = prL + prR
+    double bprL = bwdL * 2.0 / S, bprR = bwdR * 2.0 / S;
+    const double bS = -(bwdL * wdL + bwdR * wdR) / S;
+    bprL += bS;
+    bprR += bS;
     // H = e + p/rho + ke
-    bL[1] += bHL; bpLrL += bHL;
-    bR[1] += bHR; bpRrR += bHR;
-    buL += bHL * uL; bvL += bHL * vL; bwL += bHL * wL;
-    buR += bHR * uR; bvR += bHR * vR; bwR += bHR * wR;
+    bL[1] += bhL; bprL += bhL;
+    bR[1] += bhR; bprR += bhR;
+    buL += bhL * uL; bvL += bhL * vL; bwL += bhL * wL;
+    buR += bhR * uR; bvR += bhR * vR; bwR += bhR * wR;
     // p/rho
-    bL[2] += bpLrL / rhoL;
-    bL[0] -= bpLrL * pLrL / rhoL;
-    bR[2] += bpRrR / rhoR;
-    bR[0] -= bpRrR * pRrR / rhoR;
+    bL[2] += bprL / rhoL;
+    bL[0] -= bprL * prL / rhoL;
+    bR[2] += bprR / rhoR;
+    bR[0] -= bprR * prR / rhoR;
     // velocities back to the global frame (transpose of to_local)
     double gx, gy, gz;
     to_global(f, buL, bvL, bwL, gx, gy, gz);
@@ -201,9 +201,9 @@ CHKN_HD void ausmdv_vjp(double rhoL, double eL, double pL, double aL, double YBL
 // Adjoint of the SBP-ASF flux (sbp_asf, face.cu:268-342) with each stencil state's p from the
 // equation of state of its (rho, e, YB): accumulates the adjoints of the four stencil states
 // (rho, e, YB, vx, vy, vz) into b[0..3][6].  The flux is a sum of products of the per-cell
-// factors v_j = (rho, u rho, v rho, w rho, e rho, u^2 rho, v^2 rho, w^2 rho, p, p) and
-// w_j = (u, ..., u, 1) in the face frame, so its adjoint runs back through the central (fc)
-// and product-rule (fe) combinations to the factors, then to each state.
+// factors fac_j = (rho, u rho, v rho, w rho, e rho, u^2 rho, v^2 rho, w^2 rho, p, p) and
+// adv_j = (u, ..., u, 1) in the face frame, so its adjoint runs back through the central (cen)
+// and product-rule (prod) combinations to the factors, then to each state.
 CHKN_HD void sbp_asf_vjp(const Prim s[4], const Frame& f, const double Fb[6], double b[4][6])
 {
     constexpr double c = 1.0 / 12.0;
@@ -215,7 +215,7 @@ CHKN_HD void sbp_asf_vjp(const Prim s[4], const Frame& f, const double Fb[6], do
         p[i] = eos_p(s[i].rho, T[i]);
     }
     // Factors of product j at cell i (forward values).
-    auto fv = [&](int j, int i) -> double {
+    auto fac = [&](int j, int i) -> double {
         const double rho = s[i].rho;
         switch (j) {
             case 0: return rho;
@@ -229,54 +229,54 @@ CHKN_HD void sbp_asf_vjp(const Prim s[4], const Frame& f, const double Fb[6], do
             default: return p[i];
         }
     };
-    auto fw = [&](int j, int i) -> double { return j == 9 ? 1.0 : u[i]; };
-    const double mass = c * (-fv(0, 0) * fw(0, 0) + 7.0 * fv(0, 1) * fw(0, 1) + 7.0 * fv(0, 2) * fw(0, 2) -
-                             fv(0, 3) * fw(0, 3));
-    const bool up = mass >= 0.0;
+    auto adv = [&](int j, int i) -> double { return j == 9 ? 1.0 : u[i]; };
+    const double mass = c * (-fac(0, 0) * adv(0, 0) + 7.0 * fac(0, 1) * adv(0, 1) + 7.0 * fac(0, 2) * adv(0, 2) -
+                             fac(0, 3) * adv(0, 3));
+    const bool upw = mass >= 0.0;
     // Adjoints of the blended sums: mass = fc0; (mx, my, mz) rotated back by to_global (its
     // transpose is to_local); energy and the pressure terms with the alphas of sbp_asf.
     double bmx, bmy, bmz;
     to_local(f, Fb[1], Fb[2], Fb[3], bmx, bmy, bmz);
-    double bfc[10], bfe[10];
+    double bcen[10], bprod[10];
 #pragma unroll
-    for (int j = 0; j < 10; ++j) bfc[j] = bfe[j] = 0.0;
-    bfc[0] = Fb[0] + Fb[5] * (up ? s[1].YB : s[2].YB);
-    bfc[1] = 0.5 * bmx; bfe[1] = 0.5 * bmx; bfe[9] = bmx;
-    bfc[2] = 0.5 * bmy; bfe[2] = 0.5 * bmy;
-    bfc[3] = 0.5 * bmz; bfe[3] = 0.5 * bmz;
-    bfc[4] = 0.5 * Fb[4]; bfe[4] = 0.5 * Fb[4];
-    bfe[5] = 0.5 * Fb[4]; bfe[6] = 0.5 * Fb[4]; bfe[7] = 0.5 * Fb[4];
-    bfe[8] = Fb[4];
-    // Adjoints of the factors: fc_j = c(-v0 w0 + 7 v1 w1 + 7 v2 w2 - v3 w3),
-    // fe_j = c(-v0 w2 - v2 w0 + 8 v1 w2 + 8 v2 w1 - v1 w3 - v3 w1).
-    double bv[10][4], bwu[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 10; ++j) bcen[j] = bprod[j] = 0.0;
+    bcen[0] = Fb[0] + Fb[5] * (upw ? s[1].YB : s[2].YB);
+    bcen[1] = 0.5 * bmx; bprod[1] = 0.5 * bmx; bprod[9] = bmx;
+    bcen[2] = 0.5 * bmy; bprod[2] = 0.5 * bmy;
+    bcen[3] = 0.5 * bmz; bprod[3] = 0.5 * bmz;
+    bcen[4] = 0.5 * Fb[4]; bprod[4] = 0.5 * Fb[4];
+    bprod[5] = 0.5 * Fb[4]; bprod[6] = 0.5 * Fb[4]; bprod[7] = 0.5 * Fb[4];
+    bprod[8] = Fb[4];
+    // Adjoints of the factors: cen_j = c(-f0 a0 + 7 f1 a1 + 7 f2 a2 - f3 a3),
+    // prod_j = c(-f0 a2 - f2 a0 + 8 f1 a2 + 8 f2 a1 - f1 a3 - f3 a1).
+    double bfac[10][4], badv[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < 10; ++j) {
-        const double gc = c * bfc[j], ge = c * bfe[j];
-        const double v0 = fv(j, 0), v1 = fv(j, 1), v2 = fv(j, 2), v3 = fv(j, 3);
-        const double w0 = fw(j, 0), w1 = fw(j, 1), w2 = fw(j, 2), w3 = fw(j, 3);
-        bv[j][0] = -gc * w0 - ge * w2;
-        bv[j][1] = 7.0 * gc * w1 + ge * (8.0 * w2 - w3);
-        bv[j][2] = 7.0 * gc * w2 + ge * (8.0 * w1 - w0);
-        bv[j][3] = -gc * w3 - ge * w1;
-        if (j < 9) {  // w_j = u of each cell (w_9 = 1 is constant)
-            bwu[0] += -gc * v0 - ge * v2;
-            bwu[1] += 7.0 * gc * v1 + ge * (8.0 * v2 - v3);
-            bwu[2] += 7.0 * gc * v2 + ge * (8.0 * v1 - v0);
-            bwu[3] += -gc * v3 - ge * v1;
+        const double gc = c * bcen[j], ge = c * bprod[j];
+        const double f0 = fac(j, 0), f1 = fac(j, 1), f2 = fac(j, 2), f3 = fac(j, 3);
+        const double a0 = adv(j, 0), a1 = adv(j, 1), a2 = adv(j, 2), a3 = adv(j, 3);
+        bfac[j][0] = -gc * a0 - ge * a2;
+        bfac[j][1] = 7.0 * gc * a1 + ge * (8.0 * a2 - a3);
+        bfac[j][2] = 7.0 * gc * a2 + ge * (8.0 * a1 - a0);
+        bfac[j][3] = -gc * a3 - ge * a1;
+        if (j < 9) {  // adv_j = u of each cell (adv_9 = 1 is constant)
+            badv[0] += -gc * f0 - ge * f2;
+            badv[1] += 7.0 * gc * f1 + ge * (8.0 * f2 - f3);
+            badv[2] += 7.0 * gc * f2 + ge * (8.0 * f1 - f0);
+            badv[3] += -gc * f3 - ge * f1;
         }
     }
     // Factors -> (rho, e, p, u, v, w) of each cell -> its primitive state.
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const double rho = s[i].rho;
-        double brho = bv[0][i] + bv[1][i] * u[i] + bv[2][i] * v[i] + bv[3][i] * w[i] + bv[4][i] * s[i].e +
-                      bv[5][i] * u[i] * u[i] + bv[6][i] * v[i] * v[i] + bv[7][i] * w[i] * w[i];
-        const double bu = bv[1][i] * rho + 2.0 * bv[5][i] * u[i] * rho + bwu[i];
-        const double bvv = bv[2][i] * rho + 2.0 * bv[6][i] * v[i] * rho;
-        const double bww = bv[3][i] * rho + 2.0 * bv[7][i] * w[i] * rho;
-        double be = bv[4][i] * rho;
-        const double bp = bv[8][i] + bv[9][i];
+        double brho = bfac[0][i] + bfac[1][i] * u[i] + bfac[2][i] * v[i] + bfac[3][i] * w[i] + bfac[4][i] * s[i].e +
+                      bfac[5][i] * u[i] * u[i] + bfac[6][i] * v[i] * v[i] + bfac[7][i] * w[i] * w[i];
+        const double bu = bfac[1][i] * rho + 2.0 * bfac[5][i] * u[i] * rho + badv[i];
+        const double bvv = bfac[2][i] * rho + 2.0 * bfac[6][i] * v[i] * rho;
+        const double bww = bfac[3][i] * rho + 2.0 * bfac[7][i] * w[i] * rho;
+        double be = bfac[4][i] * rho;
+        const double bp = bfac[8][i] + bfac[9][i];
         // p = rho R T, T = (e + YB Q) / Cv
         brho += bp * kR * T[i];
         const double bT = bp * rho * kR;
@@ -290,7 +290,7 @@ CHKN_HD void sbp_asf_vjp(const Prim s[4], const Frame& f, const double Fb[6], do
         b[i][4] += by;
         b[i][5] += bz;
     }
-    b[up ? 1 : 2][2] += Fb[5] * mass;
+    b[upw ? 1 : 2][2] += Fb[5] * mass;
 }
 
 // Adjoint of face_flux<XORDER>: accumulates the adjoints of the four stencil states

@@ -107,55 +107,55 @@ CHKN_HD void ausmdv(real rhoL, real eL, real pL, real aL, real YBL, real vxL,
                     real YBR, real vxR, real vyR, real vzR, const Frame& f,
                     real F[6])
 {
-    real velxL, velyL, velzL, velxR, velyR, velzR;
-    to_local(f, vxL, vyL, vzL, velxL, velyL, velzL);
-    to_local(f, vxR, vyR, vzR, velxR, velyR, velzR);
-    real pLrL = pL / rhoL;
-    real keL = real(0.5) * (velxL * velxL + velyL * velyL + velzL * velzL);
-    real HL = eL + pLrL + keL;
-    real pRrR = pR / rhoR;
-    real keR = real(0.5) * (velxR * velxR + velyR * velyR + velzR * velzR);
-    real HR = eR + pR / rhoR + keR;
-    real alphaL = real(2.0) * pLrL / (pLrL + pRrR);
-    real alphaR = real(2.0) * pRrR / (pLrL + pRrR);
-    real am = fmax(aL, aR);
-    real ML = velxL / am;
-    real MR = velxR / am;
-    real pLplus, velxLplus;
-    real dvelxL = real(0.5) * (velxL + fabs(velxL));
-    if (fabs(ML) <= real(1.0)) {
-        pLplus = pL * (ML + real(1.0)) * (ML + real(1.0)) * (real(2.0) - ML) * real(0.25);
-        velxLplus = alphaL * ((velxL + am) * (velxL + am) / (real(4.0) * am) - dvelxL) + dvelxL;
+    real uL, vL, wL, uR, vR, wR;
+    to_local(f, vxL, vyL, vzL, uL, vL, wL);
+    to_local(f, vxR, vyR, vzR, uR, vR, wR);
+    real prL = pL / rhoL;
+    real keL = real(0.5) * (uL * uL + vL * vL + wL * wL);
+    real hL = eL + prL + keL;
+    real prR = pR / rhoR;
+    real keR = real(0.5) * (uR * uR + vR * vR + wR * wR);
+    real hR = eR + pR / rhoR + keR;
+    real wdL = real(2.0) * prL / (prL + prR);
+    real wdR = real(2.0) * prR / (prL + prR);
+    real cmax = fmax(aL, aR);
+    real machL = uL / cmax;
+    real machR = uR / cmax;
+    real psL, usL;
+    real upL = real(0.5) * (uL + fabs(uL));
+    if (fabs(machL) <= real(1.0)) {
+        psL = pL * (machL + real(1.0)) * (machL + real(1.0)) * (real(2.0) - machL) * real(0.25);
+        usL = wdL * ((uL + cmax) * (uL + cmax) / (real(4.0) * cmax) - upL) + upL;
     } else {
-        pLplus = pL * dvelxL / velxL;
-        velxLplus = dvelxL;
+        psL = pL * upL / uL;
+        usL = upL;
     }
-    real pRminus, velxRminus;
-    real dvelxR = real(0.5) * (velxR - fabs(velxR));
-    if (fabs(MR) <= real(1.0)) {
-        pRminus = pR * (MR - real(1.0)) * (MR - real(1.0)) * (real(2.0) + MR) * real(0.25);
-        velxRminus = alphaR * (-(velxR - am) * (velxR - am) / (real(4.0) * am) - dvelxR) + dvelxR;
+    real psR, usR;
+    real unR = real(0.5) * (uR - fabs(uR));
+    if (fabs(machR) <= real(1.0)) {
+        psR = pR * (machR - real(1.0)) * (machR - real(1.0)) * (real(2.0) + machR) * real(0.25);
+        usR = wdR * (-(uR - cmax) * (uR - cmax) / (real(4.0) * cmax) - unR) + unR;
     } else {
-        pRminus = pR * dvelxR / velxR;
-        velxRminus = dvelxR;
+        psR = pR * unR / uR;
+        usR = unR;
     }
-    real massL = velxLplus * rhoL;
-    real massR = velxRminus * rhoR;
-    real mass_half = massL + massR;
-    real p_half = pLplus + pRminus;
-    real dp = pL - pR;
-    dp = real(10.0) * fabs(dp) / fmin(pL, pR);
-    real s = real(0.5) * fmin(real(1.0), dp);
-    real rv2_V = massL * velxL + massR * velxR;
-    real rv2_D = real(0.5) * (mass_half * (velxL + velxR) - fabs(mass_half) * (velxR - velxL));
-    real rv2 = (real(0.5) + s) * rv2_V + (real(0.5) - s) * rv2_D;
-    bool up = (mass_half >= real(0.0));
-    real vely = up ? velyL : velyR;
-    real velz = up ? velzL : velzR;
-    F[0] = mass_half;
-    to_global(f, rv2 + p_half, mass_half * vely, mass_half * velz, F[1], F[2], F[3]);
-    F[4] = mass_half * (up ? HL : HR);
-    F[5] = mass_half * (up ? YBL : YBR);
+    real mfL = usL * rhoL;
+    real mfR = usR * rhoR;
+    real mdot = mfL + mfR;
+    real pface = psL + psR;
+    real pjump = pL - pR;
+    pjump = real(10.0) * fabs(pjump) / fmin(pL, pR);
+    real blend = real(0.5) * fmin(real(1.0), pjump);
+    real momV = mfL * uL + mfR * uR;
+    real momD = real(0.5) * (mdot * (uL + uR) - fabs(mdot) * (uR - uL));
+    real mom = (real(0.5) + blend) * momV + (real(0.5) - blend) * momD;
+    bool upw = (mdot >= real(0.0));
+    real vt = upw ? vL : vR;
+    real wt = upw ? wL : wR;
+    F[0] = mdot;
+    to_global(f, mom + pface, mdot * vt, mdot * wt, F[1], F[2], F[3]);
+    F[4] = mdot * (upw ? hL : hR);
+    F[5] = mdot * (upw ? YBL : YBR);
 }
 
 // face.cu:347-371 for flux_calc = ausmdv.  XORDER 2 reconstructs rho, e, YB, then applies
@@ -190,40 +190,40 @@ CHKN_HD void face_flux(const Prim& L1, const Prim& L0, const Prim& R0, const Pri
 CHKN_HD void sbp_asf(const Prim* s, const real* p, const Frame& f, real F[6])
 {
     constexpr real one_twelfth = real(1.0) / real(12.0);
-    real v[10][4], w[10][4];
+    real fac[10][4], adv[10][4];
     for (int i = 0; i < 4; ++i) {
-        real velx, vely, velz;
-        to_local(f, s[i].vx, s[i].vy, s[i].vz, velx, vely, velz);
+        real un, ut1, ut2;
+        to_local(f, s[i].vx, s[i].vy, s[i].vz, un, ut1, ut2);
         const real rho = s[i].rho;
-        v[0][i] = rho;                 w[0][i] = velx;
-        v[1][i] = velx * rho;          w[1][i] = velx;
-        v[2][i] = vely * rho;          w[2][i] = velx;
-        v[3][i] = velz * rho;          w[3][i] = velx;
-        v[4][i] = s[i].e * rho;        w[4][i] = velx;
-        v[5][i] = velx * velx * rho;   w[5][i] = velx;
-        v[6][i] = vely * vely * rho;   w[6][i] = velx;
-        v[7][i] = velz * velz * rho;   w[7][i] = velx;
-        v[8][i] = p[i];                w[8][i] = velx;
-        v[9][i] = p[i];                w[9][i] = real(1.0);
+        fac[0][i] = rho;                 adv[0][i] = un;
+        fac[1][i] = un * rho;          adv[1][i] = un;
+        fac[2][i] = ut1 * rho;          adv[2][i] = un;
+        fac[3][i] = ut2 * rho;          adv[3][i] = un;
+        fac[4][i] = s[i].e * rho;        adv[4][i] = un;
+        fac[5][i] = un * un * rho;   adv[5][i] = un;
+        fac[6][i] = ut1 * ut1 * rho;   adv[6][i] = un;
+        fac[7][i] = ut2 * ut2 * rho;   adv[7][i] = un;
+        fac[8][i] = p[i];                adv[8][i] = un;
+        fac[9][i] = p[i];                adv[9][i] = real(1.0);
     }
-    real fc[10], fe[10];
+    real cen[10], prod[10];
     for (int j = 0; j < 10; ++j) {
-        fc[j] = one_twelfth * (-v[j][0] * w[j][0] + real(7.0) * v[j][1] * w[j][1] + real(7.0) * v[j][2] * w[j][2] -
-                               v[j][3] * w[j][3]);
-        fe[j] = one_twelfth * (-v[j][0] * w[j][2] - v[j][2] * w[j][0] + real(8.0) * v[j][1] * w[j][2] +
-                               real(8.0) * v[j][2] * w[j][1] - v[j][1] * w[j][3] - v[j][3] * w[j][1]);
+        cen[j] = one_twelfth * (-fac[j][0] * adv[j][0] + real(7.0) * fac[j][1] * adv[j][1] + real(7.0) * fac[j][2] * adv[j][2] -
+                               fac[j][3] * adv[j][3]);
+        prod[j] = one_twelfth * (-fac[j][0] * adv[j][2] - fac[j][2] * adv[j][0] + real(8.0) * fac[j][1] * adv[j][2] +
+                               real(8.0) * fac[j][2] * adv[j][1] - fac[j][1] * adv[j][3] - fac[j][3] * adv[j][1]);
     }
-    const real a_mass = real(1.0), a_mom = real(0.5), a_ie = real(0.5), a_ke = real(0.0), a_p = real(0.0);
-    const real mass = a_mass * fc[0] + (real(1.0) - a_mass) * fe[0];
+    const real bl_mass = real(1.0), bl_mom = real(0.5), bl_ie = real(0.5), bl_ke = real(0.0), bl_p = real(0.0);
+    const real mass = bl_mass * cen[0] + (real(1.0) - bl_mass) * prod[0];
     F[0] = mass;
-    const real mx = a_mom * fc[1] + (real(1.0) - a_mom) * fe[1] + (a_p * fc[9] + (real(1.0) - a_p) * fe[9]);
-    const real my = a_mom * fc[2] + (real(1.0) - a_mom) * fe[2];
-    const real mz = a_mom * fc[3] + (real(1.0) - a_mom) * fe[3];
+    const real mx = bl_mom * cen[1] + (real(1.0) - bl_mom) * prod[1] + (bl_p * cen[9] + (real(1.0) - bl_p) * prod[9]);
+    const real my = bl_mom * cen[2] + (real(1.0) - bl_mom) * prod[2];
+    const real mz = bl_mom * cen[3] + (real(1.0) - bl_mom) * prod[3];
     to_global(f, mx, my, mz, F[1], F[2], F[3]);
-    F[4] = a_ie * fc[4] + (real(1.0) - a_ie) * fe[4] +
-           real(0.5) * (a_ke * fc[5] + (real(1.0) - a_ke) * fe[5] + a_ke * fc[6] + (real(1.0) - a_ke) * fe[6] +
-                  a_ke * fc[7] + (real(1.0) - a_ke) * fe[7]) +
-           a_p * fc[8] + (real(1.0) - a_p) * fe[8];
+    F[4] = bl_ie * cen[4] + (real(1.0) - bl_ie) * prod[4] +
+           real(0.5) * (bl_ke * cen[5] + (real(1.0) - bl_ke) * prod[5] + bl_ke * cen[6] + (real(1.0) - bl_ke) * prod[6] +
+                  bl_ke * cen[7] + (real(1.0) - bl_ke) * prod[7]) +
+           bl_p * cen[8] + (real(1.0) - bl_p) * prod[8];
     F[5] = mass * ((mass >= real(0.0)) ? s[1].YB : s[2].YB);
 }
 
