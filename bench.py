@@ -79,7 +79,7 @@ def build_workload(name, steps_total):
         desc = {"workload": "differentiation: one 128x64x64 block, Mach 5 ramp; forward + adjoint "
                             "of the wall-pressure loss w.r.t. U0 and inflow (Mach, p, T), "
                             "checkpoint every 50 steps (segments taped by the forward pass while "
-                            "they fit in half the free device memory, recomputed otherwise), dt "
+                            "they fit in the device's free memory, recomputed otherwise), dt "
                             "sequence frozen, FP64"}
     elif name == "diff-pc":
         jobs = [cases.ramp(128, 64, 64, max_step=1 << 30)]
@@ -87,14 +87,15 @@ def build_workload(name, steps_total):
         desc = {"workload": "differentiation, predictor-corrector: one 128x64x64 block, Mach 5 ramp "
                             "stepped by the two-stage predictor-corrector; forward + adjoint of the "
                             "wall-pressure loss w.r.t. U0 and inflow (Mach, p, T), checkpoint every "
-                            "50 steps (segments taped while they fit in half the free device memory, "
+                            "50 steps (segments taped while they fit in the device's free memory, "
                             "recomputed otherwise), dt sequence frozen, FP64"}
     elif name == "ensemble-diff":
         jobs = cases.ramp_ensemble(seed=11, n=64, max_step=1 << 30)
         desc = {"workload": "batched differentiation: 64 independent Mach 3-8 ramp jobs (seed 11), "
                             "forward + adjoint of each wall-pressure loss w.r.t. its inflow "
                             "(Mach, p, T), checkpoint every 50 steps (segments taped while they fit "
-                            "in half the free device memory, recomputed otherwise), FP64"}
+                            "beside one recompute tape in the device's free memory, recomputed "
+                            "otherwise), FP64"}
     else:
         raise SystemExit("unknown workload " + name)
     stepper = ("predictor-corrector (2 stages)" if jobs[0].time_scheme == 1 else "TVD-RK3 (3 stages)")
@@ -130,16 +131,34 @@ def run_timed(jobs, opts, steps, warmup, sim=None):
     return wall, kernel_ms, launches, rep, cells
 
 
+def dedicated_tape_budget(sim, jobs, steps, checkpoint_every):
+    """Resident-tape budget of a device that runs this unroll alone: every checkpoint segment
+    if they fit in the free memory less 8 GB, else as many as fit beside one segment's
+    recompute tape (the library's default, half of the free memory, leaves room for a
+    caller's other work)."""
+    import torch
+    nst = 2 if jobs[0].time_scheme == 1 else 3
+    seg = 8 * 6 * sim.active_cells * (nst * checkpoint_every + 1)
+    nseg = -(-steps // checkpoint_every)
+    free, _ = torch.cuda.mem_get_info()
+    avail = free - 8e9
+    if nseg * seg <= avail:
+        return nseg * seg
+    return max(0, int((avail - seg) // seg)) * seg
+
+
 def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
     """Forward (checkpointed) + adjoint of the wall-pressure loss against the plain forward
-    of the same steps, with the default tape budget (the forward pass tapes the checkpoint
-    segments that fit in half the free device memory, the backward pass recomputes the rest)
-    and with every segment recomputed.  Returns a dict of timings."""
+    of the same steps, with the tapes of a device that runs the unroll alone (the forward
+    pass tapes the checkpoint segments that fit, the backward pass recomputes the rest) and
+    with every segment recomputed.  Returns a dict of timings."""
     import torch
     from chkn import adjoint
     from chkn.lib import Sim
     sim = Sim(jobs=jobs) if len(jobs) > 1 else Sim(jobs[0])
     sim.set_launch(**opts)
+    budget = dedicated_tape_budget(sim, jobs, steps, checkpoint_every)
+    sim.set_tape_budget(budget)
     loss_fn = adjoint.WallPressure(sim, jobs)
     u0 = adjoint.initial_state(sim)
     mpts = []
@@ -170,7 +189,7 @@ def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
     gradient(steps)
     rc_fwd, rc_adj, rc_L, rc_gm, rc_gu = gradient(steps)
     same = rc_L == L and all(bool(torch.equal(a, b)) for a, b in zip(gm, rc_gm)) and bool(torch.equal(gu, rc_gu))
-    sim.set_tape_budget(None)
+    sim.set_tape_budget(budget)
     # Plain forward of the same steps from the same state (stepping ring, graph replay).
     sim.write_conserved(u0)
     sim.step(1)
@@ -185,7 +204,7 @@ def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
     nseg = -(-steps // checkpoint_every)
     return {"seconds_forward_taped": t_fwd, "seconds_adjoint": t_adj, "seconds_plain_forward": t_plain,
             "fwd_adj_over_fwd": (t_fwd + t_adj) / t_plain,
-            "resident_segments": "%d of %d" % (resident, nseg),
+            "resident_segments": "%d of %d" % (resident, nseg), "tape_budget_bytes": budget,
             "recompute_all": {"seconds_forward": rc_fwd, "seconds_adjoint": rc_adj,
                               "fwd_adj_over_fwd": (rc_fwd + rc_adj) / t_plain,
                               "gradient_identical": same},

@@ -179,11 +179,30 @@ void plan_resident(chkn_sim* s, int C, int nck)
     int want = seg > 0 ? (int)std::min<size_t>((size_t)nck, budget / seg) : 0;
     if (want > g.nres_alloc || (g.tape_budget >= 0 && want < g.nres_alloc)) {
         free_resident(s);
-        for (auto& b : s->blocks) {
-            if (!b.active || want == 0) continue;
-            if (hipMalloc(&b.rtape, sizeof(double) * tape_doubles(s, b, C) * want) != hipSuccess) {
-                (void)hipGetLastError();
-                free_resident(s);
+        // A budget raised on a handle that holds recompute tapes: those are released (and
+        // allocated again, as far as they are still needed) when the resident tapes do not fit
+        // beside them.
+        for (int attempt = 0; attempt < 2 && want > 0; ++attempt) {
+            bool ok = true;
+            for (auto& b : s->blocks) {
+                if (!b.active) continue;
+                if (hipMalloc(&b.rtape, sizeof(double) * tape_doubles(s, b, C) * want) != hipSuccess) {
+                    (void)hipGetLastError();
+                    ok = false;
+                    break;
+                }
+            }
+            if (ok) break;
+            free_resident(s);
+            if (attempt == 0 && g.tape_C > 0) {
+                for (auto& b : s->blocks) {
+                    dfree(b.tape);
+                    dfree(b.tape2);
+                    b.tape = b.tape2 = nullptr;
+                }
+                g.tape_C = 0;
+                g.two_tapes = false;
+            } else {
                 want = 0;
             }
         }
