@@ -446,7 +446,7 @@ def _best_of(jobs, candidates, steps, warmup, graph, per_block):
     return best
 
 
-def batched_record(graph, steps=60, warmup=10, cpu=True):
+def batched_record(graph, steps=WORKLOAD_STEPS["batched"], warmup=10, cpu=True):
     """The 40-block uneven array (5x4x2, slab sizes seeded in [16, 96]) in one launch per
     stage for all blocks, against one launch per block per stage (the reference's launch
     structure, simulate.cu:605-625), each with its best launch shape of a short list; and the
@@ -465,10 +465,11 @@ def batched_record(graph, steps=60, warmup=10, cpu=True):
     return out
 
 
-def diff_record(graph, steps=100):
+def diff_record(graph, steps=WORKLOAD_STEPS["diff"]):
     """Forward (checkpoint every 50 steps) + adjoint of the wall-pressure loss of the
     128x64x64 ramp w.r.t. U0 and the inflow (Mach, p, T), against the plain forward of the
-    same steps."""
+    same steps: the differentiation workload's 500-step unroll (shorter unrolls reverse a
+    larger share of steps that the adjoint has not reached yet, which are cheaper)."""
     jobs, desc = build_workload("diff", steps)
     opts = dict(tile_i=32, tile_j=8, tile_k=0, use_graph=bool(graph), per_block_launch=False)
     r = run_diff(jobs, opts, steps, 5)
