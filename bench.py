@@ -112,12 +112,41 @@ def new_sim(jobs):
     return Sim(jobs=jobs) if len(jobs) > 1 else Sim(jobs[0])
 
 
-def run_timed(jobs, opts, steps, warmup, sim=None):
-    """A handle at the initial state (fresh, or `sim` built beforehand and not yet stepped),
-    warmup steps, then `steps` timed steps.  Returns (wall_s, kernel_ms, launches, report)."""
+def heat_sim(jobs, opts):
+    """A handle of the same jobs that steps without a step limit, for heat_device."""
+    import copy
+    unbounded = []
+    for j in jobs:
+        c = copy.copy(j)
+        c.max_step = 1 << 30
+        c.max_time = 1.0e30
+        unbounded.append(c)
+    h = new_sim(unbounded)
+    h.set_launch(**opts)
+    return h
+
+
+def heat_device(h, seconds=0.25):
+    """Untimed steps on the handle h (not the measured one) for `seconds`, right before a timed
+    run: after even a short idle gap (building or freeing a handle) the device takes some 50 ms
+    of work to return to its full clocks -- the first 25 steps of a 256x128x128 block then run
+    up to 24% slower -- which a few warmup steps of the measured handle do not cover.  The
+    measured handle still starts from the initial state and runs exactly its warmup and timed
+    steps."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        h.step(10)
+
+
+def run_timed(jobs, opts, steps, warmup):
+    """A fresh handle at the initial state, the device heated on a second handle
+    (heat_device), warmup steps, then `steps` timed steps.  Returns (wall_s, kernel_ms,
+    launches, report)."""
     import torch
-    sim = sim if sim is not None else new_sim(jobs)
+    sim = new_sim(jobs)
     sim.set_launch(**opts)
+    h = heat_sim(jobs, opts)
+    heat_device(h)
     if warmup:
         sim.step(warmup)
     torch.cuda.synchronize()
@@ -128,6 +157,7 @@ def run_timed(jobs, opts, steps, warmup, sim=None):
     kernel_ms, launches = sim.last_timing()
     cells = sim.active_cells
     sim.close()
+    h.close()
     return wall, kernel_ms, launches, rep, cells
 
 
@@ -157,6 +187,7 @@ def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
     from chkn.lib import Sim
     sim = Sim(jobs=jobs) if len(jobs) > 1 else Sim(jobs[0])
     sim.set_launch(**opts)
+    h = heat_sim(jobs, opts)  # built before the tapes are sized, which counts its memory
     budget = dedicated_tape_budget(sim, jobs, steps, checkpoint_every)
     sim.set_tape_budget(budget)
     loss_fn = adjoint.WallPressure(sim, jobs)
@@ -183,6 +214,7 @@ def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
         return t1 - t0, t2 - t1, float(L.detach()), [m.grad for m in mp], u.grad
 
     gradient(steps)  # allocates tapes and checkpoints
+    heat_device(h)
     t_fwd, t_adj, L, gm, gu = gradient(steps)
     resident = sim.grad_resident_segments
     sim.set_tape_budget(0)
@@ -194,6 +226,7 @@ def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
     sim.write_conserved(u0)
     sim.step(1)
     sim.write_conserved(u0)
+    heat_device(h)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.step(steps)
@@ -201,6 +234,7 @@ def run_diff(jobs, opts, steps, warmup, checkpoint_every=50):
     t_plain = time.perf_counter() - t0
     cells = sim.active_cells
     sim.close()
+    h.close()
     nseg = -(-steps // checkpoint_every)
     return {"seconds_forward_taped": t_fwd, "seconds_adjoint": t_adj, "seconds_plain_forward": t_plain,
             "fwd_adj_over_fwd": (t_fwd + t_adj) / t_plain,
@@ -323,11 +357,6 @@ def main():
     steps = args.steps or WORKLOAD_STEPS[args.workload]
     jobs, desc = build_workload(args.workload, args.warmup + steps)
     tuned = None
-    # The measuring handle is built before the tuner runs, so that its warmup and timed steps
-    # follow the tuner's launches without the idle gap of building a 4M-cell handle (on some
-    # devices the first 25 steps of a fresh 256x128x128 run take up to 2.4% longer after such
-    # a gap).
-    sim = new_sim(jobs) if args.workload not in ("diff", "diff-pc", "ensemble-diff") else None
     if args.tile:
         ti, tj, tk = (int(v) for v in args.tile.split("x"))
         opts = dict(tile_i=ti, tile_j=tj, tile_k=tk, tile_order=args.order,
@@ -354,17 +383,16 @@ def main():
                                timings=r)}
         print(json.dumps(line))
         return
-    wall, kernel_ms, launches, rep, cells = run_timed(jobs, opts, steps, args.warmup, sim)
+    wall, kernel_ms, launches, rep, cells = run_timed(jobs, opts, steps, args.warmup)
     rec = roofline_record(jobs, opts, steps, args.warmup, args.workload, kernel_ms, launches, cells)
     config = dict(desc, launch=opts, steps_completed=rep.step)
     if tuned:
         config["autotune"] = tuned
     compare = args.compare_per_block if args.compare_per_block >= 0 else int(args.workload == "batched")
     if compare and not args.per_block:
-        pb_sim = new_sim(jobs)  # built before its tuning, as the fused run's handle
         pb_opts, _ = tune.autotune(jobs=jobs, per_block_launch=True)
         pb_opts = dict(pb_opts, use_graph=bool(args.graph), per_block_launch=True)
-        pw, _, pl, _, _ = run_timed(jobs, pb_opts, steps, args.warmup, pb_sim)
+        pw, _, pl, _, _ = run_timed(jobs, pb_opts, steps, args.warmup)
         config["per_block_comparator"] = {
             "value": cells * steps / pw, "launches": pl, "launch": pb_opts,
             "fused_speedup": pw / wall}
