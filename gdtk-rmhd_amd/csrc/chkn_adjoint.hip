@@ -27,7 +27,8 @@
 // contraction, reciprocal arithmetic and approximate (Newton-refined reciprocal) division
 // allowed (the stage kernels are not).
 //
-// Reversed TVD-RK3 step with A0 = adjoint of U^{n+1} on entry (seed: P = dt/6 A0, D = 4dt/6 A0):
+// Reversed TVD-RK3 step with A0 = adjoint of U^{n+1} on entry (seed: P = dt/6 A0, D = 4dt/6 A0;
+// stage 3 forms that P from A0 itself, so the stage-1 seeds store D only):
 //   stage 3 (W2): A0 += U2bar; D = P + dt/4 U2bar; P = D
 //   stage 2 (W1): A0 += U1bar; D = P + dt U1bar
 //   stage 1 (W0): A0 += U0bar        -> A0 = adjoint of U^n; seed of the next reversed step
@@ -93,9 +94,11 @@ __device__ __forceinline__ double* ghost_buffer(const BlockDev* B, int side)
 // (a, c) in direction d, left in column t of st: st[6p+q][t] = adjoint of stencil slot p
 // (L1, L0, R0, R1), quantity q.  Zero for faces outside the block.
 // FLUX 0: AUSMDV on MUSCL (XORDER 2) or adjacent states; FLUX 1: the SBP-ASF flux.
-template <int XORDER, int FLUX>
-__device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, int d, int m, int a,
-                                         int c, double (&st)[24][kAdjPatchM * kAdjPatchT], int t)
+// KEEP: the stencil's R1 state (cell m + 1) is also returned in r1, and the function returns
+// whether it loaded the stencil at all (faces with zero Fbar do not).
+template <int XORDER, int FLUX, bool KEEP = false>
+__device__ __forceinline__ bool face_vjp(const AdjArgs& A, const BlockDev* B, int d, int m, int a,
+                                         int c, double (&st)[24][kAdjPatchM * kAdjPatchT], int t, Prim& r1)
 {
     const int nline = line_len(B, d);
     const int na = d == 0 ? B->njc : B->nic;
@@ -125,7 +128,7 @@ __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, in
     if (!any) {
 #pragma unroll
         for (int k = 0; k < 24; ++k) st[k][t] = 0.0;
-        return;
+        return false;
     }
     const double* W = B->W + A.slot * B->wstride;
 #pragma unroll
@@ -139,6 +142,7 @@ __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, in
         st[6 * p + 3][t] = sp.vx;
         st[6 * p + 4][t] = sp.vy;
         st[6 * p + 5][t] = sp.vz;
+        if (KEEP && p == 3) r1 = sp;
     }
     // Compiler fence: the stencil values are read back from LDS where they are needed rather
     // than forwarded from the stores above and held in registers through the flux adjoint.
@@ -159,7 +163,7 @@ __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, in
         for (int p = 0; p < 4; ++p)
 #pragma unroll
             for (int q = 0; q < 6; ++q) st[6 * p + q][t] = bs[p][q];
-        return;
+        return true;
     }
     // Reconstructed face states (face.cu:347-371).
     double qL[6], qR[6];
@@ -208,6 +212,7 @@ __device__ __forceinline__ void face_vjp(const AdjArgs& A, const BlockDev* B, in
 #pragma unroll
         for (int p = 0; p < 4; ++p) st[6 * p + q][t] = gq[p];
     }
+    return true;
 }
 
 // What the ghosts of one side of a line return to the cell next to that side, gb = the sum of
@@ -285,16 +290,18 @@ struct CellIn {
     double a0[6], p[6];
 };
 
+// P is read at stage 2 only: stage 3 (TVD-RK3's first reversed stage) forms it from A0.
 template <int STAGE>
-__device__ __forceinline__ void load_cell(const AdjArgs& A, const BlockDev* B, int cidx, CellIn& c)
+__device__ __forceinline__ void load_cell(const AdjArgs& A, const BlockDev* B, int cidx, CellIn& c,
+                                          bool load_w = true)
 {
     const long long n = B->n;
     const unsigned o = boff(cidx);
-    c.w = ld_prim(B->W + A.slot * B->wstride, n, cidx);
+    if (load_w) c.w = ld_prim(B->W + A.slot * B->wstride, n, cidx);
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         c.a0[q] = gld(B->adj + (long long)q * n, o);
-        c.p[q] = STAGE > 1 ? gld(B->adj + (long long)(6 + q) * n, o) : 0.0;
+        c.p[q] = STAGE == 2 ? gld(B->adj + (long long)(6 + q) * n, o) : 0.0;
     }
 }
 
@@ -318,14 +325,15 @@ __device__ __forceinline__ void finish_cell(const AdjArgs& A, const BlockDev* B,
         const double a0 = c.a0[q] + Ub[q];
         gst(A0, o, a0);
         if (STAGE == 3) {
-            const double v = c.p[q] + 0.25 * dt * Ub[q];
+            // The step's seed P = seed_p dt A0 (as the seeds form it), A0 before this stage.
+            const double v = A.seed_p * dt * c.a0[q] + 0.25 * dt * Ub[q];
             gst(D, o, v);
             gst(Pp, o, v);
         } else if (STAGE == 2) {
             gst(D, o, c.p[q] + dt * Ub[q]);
         } else if (A.dt_next) {
             const double dtn = A.dt_next[B->inst];
-            gst(Pp, o, A.seed_p * dtn * a0);
+            if (A.seed_p_stored) gst(Pp, o, A.seed_p * dtn * a0);
             gst(D, o, A.seed_d * dtn * a0);
         }
     }
@@ -353,7 +361,10 @@ __global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjA
     const int tm = d == 0 ? t % kAdjPatchM : t / kAdjPatchT;  // face position in the patch
     const int ta = d == 0 ? t / kAdjPatchM : t % kAdjPatchT;  // line in the patch
     const int a = P.a0 + ta, c = P.c;
-    face_vjp<XORDER, FLUX>(A, B, d, P.m0 - 1 + tm, a, c, st, t);
+    // In the fused form the face at position tm (face m - 1) has the owned cell m as its R1
+    // stencil state: kept from the stencil load instead of read again.
+    Prim own_w;
+    const bool have_w = face_vjp<XORDER, FLUX, (EPI > 0)>(A, B, d, P.m0 - 1 + tm, a, c, st, t, own_w);
     // Owned cell m = m0 + tm of line (a, c): faces m-1 .. m+2 sit at positions tm .. tm+3.
     const int nline = line_len(B, d);
     const int na = d == 0 ? B->njc : B->nic;
@@ -371,7 +382,8 @@ __global__ __launch_bounds__(kAdjPatchM * kAdjPatchT) void adj_patch_kernel(AdjA
 #pragma unroll
             for (int q = 0; q < 6; ++q)
                 gij[q] = gld_nt(B->Gc + (long long)q * n, o) + gld_nt(B->Gc + (long long)(6 + q) * n, o);
-            load_cell<EPI>(A, B, cidx, cin);
+            load_cell<EPI>(A, B, cidx, cin, !have_w);
+            if (have_w) cin.w = own_w;
         }
     }
     __syncthreads();

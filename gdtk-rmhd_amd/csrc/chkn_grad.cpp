@@ -574,6 +574,7 @@ int grad_backward(chkn_sim* s, const double* lam, double* grad_u0, double* grad_
     // Seeds of a reversed step: the weights of the stages' residuals in U^{n+1}.
     A.seed_p = s->time_scheme == CHKN_TIME_PC ? 0.5 : kOneSixth;
     A.seed_d = s->time_scheme == CHKN_TIME_PC ? 0.5 : kFourSixth;
+    A.seed_p_stored = s->time_scheme == CHKN_TIME_PC ? 1 : 0;
     // D is double-buffered in the fused form (the k patches finish cells whose D neighbouring
     // patches still read): dcur is the buffer the next face pass reads, which seeds write.
     const bool fused = g.fused_ok && !g.separate_gather;

@@ -143,6 +143,9 @@ struct AdjArgs {
     const double* dt_next;    // [ninst] time step of the step reversed next, or null (stage 1)
     double seed_p, seed_d;    // a reversed step starts from P = seed_p dt A0, D = seed_d dt A0:
                               // 1/6 and 4/6 (TVD-RK3), 1/2 and 1/2 (predictor-corrector)
+    int seed_p_stored;        // 1: the stage-1 seeds store P (the predictor-corrector's first
+                              // reversed stage reads it); 0: TVD-RK3's stage 3 forms
+                              // P = seed_p dt A0 from the A0 it reads anyway
     const double* chem;       // [ninst] exp(-alpha dt) of the step being reversed (< 0: no
                               // reaction), for the reacting A->B gas
     int dr, dw;               // component offsets in adj of the D this stage reads (faces) and
