@@ -384,6 +384,31 @@ def sbp_channel(nics=(10, 8), njc=6, nkc=5, max_step=25, seed=3, flux_calc="sbp_
     return JobData.from_prep()
 
 
+def k_inflow_box(nic=10, njc=6, nkc=14, max_step=40, seed=5):
+    """A Mach 2 stream (p = 50 kPa, T = 300 K) entering a warped box through its kminus side
+    and leaving through kplus; slip walls on the i sides, periodic in j.  The initial state is
+    the stream with a smooth seeded variation of pressure, temperature and velocity."""
+    _common(max_step, title="k inflow box (seed %d)" % seed, dt_init=5.0e-7)
+    g = box_grid(nic, njc, nkc, L=(0.1, 0.06, 0.28))
+    rng = np.random.default_rng(seed)
+    v = g.vertices
+    for a, L in ((v.x, 0.1), (v.y, 0.06), (v.z, 0.28)):
+        a[1:-1, 1:-1, 1:-1] += L * 0.04 / nkc * rng.uniform(-1, 1, size=a[1:-1, 1:-1, 1:-1].shape)
+    speed = 2.0 * math.sqrt(GAMMA * R_GAS * 300.0)
+    inflow = prep.FlowState(p=5.0e4, T=300.0, velz=speed)
+    cc = cell_centres(g)
+    X, Y, Z = cc.x / 0.1, cc.y / 0.06, cc.z / 0.28
+    wave = np.sin(2 * np.pi * (Z + 0.3 * X)) * np.cos(2 * np.pi * Y)
+    st = _state_arrays(5.0e4 * (1.0 + 0.02 * wave), 300.0 * (1.0 + 0.01 * wave),
+                       4.0 * np.sin(2 * np.pi * (X + Z)), 3.0 * np.cos(2 * np.pi * (Y - Z)),
+                       speed * (1.0 + 0.03 * wave))
+    bcs = {"iminus": prep.WallWithSlipBC(), "iplus": prep.WallWithSlipBC(),
+           "jminus": prep.ExchangeBC(), "jplus": prep.ExchangeBC(),
+           "kminus": prep.InflowBC(inflow), "kplus": prep.OutflowBC()}
+    prep.FluidBlock(i=0, j=0, k=0, grid=g, initialState=st, bcs=bcs)
+    return JobData.from_prep()
+
+
 def ab_reacting(nics=(16, 12), njc=6, nkc=5, max_step=30, seed=2, reacting=True, k_walls=False):
     """The reacting A->B gas (gas.cu:25-48, examples/chicken/oblique-detonation-wave): a
     Mach 3 stream (p = 86.1 kPa, T = 300 K) into gas at 28.7 kPa holding a hot, partly

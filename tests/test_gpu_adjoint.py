@@ -292,6 +292,65 @@ def test_fused_gather_matches_the_separate_gather():
     sim.close()
 
 
+def _inflow_z(mpt):
+    """(Mach, p, T) -> the inflow state of a stream along +z."""
+    w = adjoint.inflow_from_mpt(mpt)
+    return torch.stack([w[0], w[1], w[2], w[4], w[5], w[3]])
+
+
+def test_fused_gather_with_inflow_through_a_k_side():
+    # In the fused form the k patches finish their cells, including the accumulator of each
+    # inflow line of a k side (the inflow state's gradient).  A Mach 2 stream entering through
+    # kminus: against the separate gather bit for bit, and dL/d(Mach, p, T) of a linear loss on
+    # the final energy and z momentum against central differences.
+    job = cases.k_inflow_box(max_step=40)
+    sim = Sim(job)
+    assert sim.gather_form == 0
+    u0 = adjoint.initial_state(sim)
+    n = 10 * 6 * 14
+    wc = torch.as_tensor(np.random.default_rng(3).uniform(0.5, 1.5, size=n), device="cuda")
+    mpt0 = [2.0, 5.0e4, 300.0]
+    dts = [None]
+
+    def run(mp, u=None):
+        uN = adjoint.unroll(sim, u0 if u is None else u, _inflow_z(mp)[None], 12, 5, dts[0])
+        if dts[0] is None:
+            dts[0] = uN.grad_fn.dts
+        U = uN.view(6, n)
+        return (U[4] * wc).sum() + 1.0e3 * (U[3] * wc).sum()
+
+    def grad(separate):
+        sim.set_gather_form(separate)
+        mp = torch.tensor(mpt0, dtype=torch.float64, requires_grad=True)
+        u = u0.clone().requires_grad_(True)
+        L = run(mp, u)
+        L.backward()
+        return float(L.detach()), u.grad.clone(), mp.grad.clone()
+
+    L1, gu1, gm1 = grad(False)
+    L2, gu2, gm2 = grad(True)
+    assert sim.gather_form == 1
+    assert L1 == L2 and torch.equal(gu1, gu2) and torch.equal(gm1, gm2)
+    assert bool(torch.all(gm1 != 0))
+    sim.set_gather_form(False)
+    # Two relative steps, as for U0: perturbations of 1e-7 cross a few limiter kinks here
+    # (observed 1.3e-5 for T at 1e-7, 2.6e-6 at 1e-8; steps of 1e-5 and 1e-6 err by 1e-4 to
+    # 1e-3 and 1e-9 by roundoff).
+    rows = []
+    for k in range(3):
+        for rel, tol in ((FD_REL, 1e-4), (1e-8, FD_TOL)):
+            h = rel * mpt0[k]
+            mp_p, mp_m = list(mpt0), list(mpt0)
+            mp_p[k] += h
+            mp_m[k] -= h
+            with torch.no_grad():
+                fd = (float(run(torch.tensor(mp_p, dtype=torch.float64))) -
+                      float(run(torch.tensor(mp_m, dtype=torch.float64)))) / (2 * h)
+            rows.append(("dL/d%s (step %.0e)" % ("MpT"[k], rel), fd, float(gm1[k]), tol))
+    _assert_rows(rows)
+    sim.close()
+
+
 def test_first_order_gradient_matches_finite_differences():
     cases.mixed_bc_array(max_step=40)
     prep.config.x_order = 1
