@@ -33,7 +33,12 @@ BYTES_PER_CELL_STEP_VISCOUS = BYTES_PER_CELL_STEP + 3 * 96
 
 
 WORKLOAD_STEPS = {"scale": 500, "plain": 200, "batched": 300, "ensemble": 300, "diff": 500, "diff-pc": 500,
-                  "ensemble-diff": 300, "viscous": 300, "shard": 500, "pc": 500}
+                  "ensemble-diff": 300, "viscous": 300, "shard": 500, "pc": 500, "diff-scale": 500,
+                  "xl": 100}
+DIFF_WORKLOADS = ("diff", "diff-pc", "ensemble-diff", "diff-scale")
+# Launch shape of workloads too large to sweep in a bench run: the tuner's pick for the
+# 256x128x128 block of the same flow.
+FIXED_LAUNCH = {"xl": dict(tile_i=32, tile_j=8, tile_k=0)}
 # Steps of the reference CPU build timed for cpu_baseline: a sample of about 10-30 s on one
 # core (the viscous step runs about 6x slower there than the inviscid one).
 CPU_STEPS = {"viscous": 1}
@@ -81,6 +86,19 @@ def build_workload(name, steps_total):
                             "checkpoint every 50 steps (segments taped by the forward pass while "
                             "they fit in the device's free memory, recomputed otherwise), dt "
                             "sequence frozen, FP64"}
+    elif name == "diff-scale":
+        jobs = [cases.ramp(256, 128, 128, max_step=1 << 30)]
+        desc = {"workload": "differentiation at scale: the 256x128x128 block of plain-at-scale; "
+                            "forward + adjoint of the wall-pressure loss w.r.t. U0 and inflow "
+                            "(Mach, p, T) over 500 steps, checkpoint every 50 steps (30 GB per "
+                            "segment tape: the segments that fit beside one recompute tape are "
+                            "taped by the forward pass, the rest recomputed), dt sequence frozen, "
+                            "FP64"}
+    elif name == "xl":
+        jobs = [cases.ramp(768, 384, 384, max_step=steps_total)]
+        jobs[0].dt_init = 2.5e-7  # cells 3x smaller than the 256x128x128 block's: CFL-stable start
+        desc = {"workload": "xl: one 768x384x384 block (113M cells) of the Mach 5 ramp flow of "
+                            "plain-at-scale, dt_init 2.5e-7 s, FP64"}
     elif name == "diff-pc":
         jobs = [cases.ramp(128, 64, 64, max_step=1 << 30)]
         jobs[0].time_scheme = 1
@@ -355,12 +373,20 @@ def main():
     from chkn import tune
 
     steps = args.steps or WORKLOAD_STEPS[args.workload]
+    t_start = time.perf_counter()
+
+    def progress(msg):
+        print("[bench %6.1f s] %s" % (time.perf_counter() - t_start, msg), file=sys.stderr, flush=True)
+
     jobs, desc = build_workload(args.workload, args.warmup + steps)
+    progress("workload %s built: %d cells" % (args.workload, sum(j.active_cells for j in jobs)))
     tuned = None
     if args.tile:
         ti, tj, tk = (int(v) for v in args.tile.split("x"))
         opts = dict(tile_i=ti, tile_j=tj, tile_k=tk, tile_order=args.order,
                     persistent=bool(args.persistent), vertex_frames=bool(args.vertex_frames))
+    elif args.workload in FIXED_LAUNCH:
+        opts = dict(FIXED_LAUNCH[args.workload])
     elif args.tune and not jobs[0].viscous:  # the face-list stage has no tile shape
         t0 = time.perf_counter()
         opts, table = tune.autotune(jobs=jobs, per_block_launch=bool(args.per_block))
@@ -370,7 +396,9 @@ def main():
     else:
         opts = {}
     opts = dict(opts, use_graph=bool(args.graph), per_block_launch=bool(args.per_block))
-    if args.workload in ("diff", "diff-pc", "ensemble-diff"):
+    progress("launch %s%s" % ({k: opts.get(k) for k in ("tile_i", "tile_j", "tile_k")},
+                              " (tuned)" if tuned else ""))
+    if args.workload in DIFF_WORKLOADS:
         r = run_diff(jobs, opts, steps, args.warmup)
         wall = r["seconds_forward_taped"] + r["seconds_adjoint"]
         line = {"metric": "cell-updates/s (forward + adjoint)",
@@ -384,6 +412,7 @@ def main():
         print(json.dumps(line))
         return
     wall, kernel_ms, launches, rep, cells = run_timed(jobs, opts, steps, args.warmup)
+    progress("timed %d steps: %.3f ms per step" % (steps, wall * 1e3 / steps))
     rec = roofline_record(jobs, opts, steps, args.warmup, args.workload, kernel_ms, launches, cells)
     config = dict(desc, launch=opts, steps_completed=rep.step)
     if tuned:
