@@ -25,7 +25,8 @@
 // No atomics: every sum has one writer and a fixed order, so the gradient is deterministic.
 // Gradients need no bit-exact arithmetic, so this file is built with floating-point
 // contraction, reciprocal arithmetic and approximate (Newton-refined reciprocal) division
-// allowed (the stage kernels are not).
+// allowed (the stage kernels are not); the helpers the gather and the fused k pass share turn
+// contraction off, so that the two forms round alike and give the same gradient bit for bit.
 //
 // Reversed TVD-RK3 step with A0 = adjoint of U^{n+1} on entry (seed: P = dt/6 A0, D = 4dt/6 A0;
 // stage 3 forms that P from A0 itself, so the stage-1 seeds store D only):
