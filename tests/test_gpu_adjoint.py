@@ -292,6 +292,24 @@ def test_fused_gather_matches_the_separate_gather():
     sim.close()
 
 
+@pytest.mark.parametrize("dims", [(5, 3, 2), (13, 16, 13), (14, 17, 27), (30, 5, 14)])
+def test_fused_gather_on_small_and_ragged_blocks(dims):
+    # Patch edges in the fused form: k lines of 2 cells (both ends in one cell pair), lines
+    # of exactly one patch's 13 owned cells, lines whose last patch owns one cell, i and j
+    # lines shorter or longer than a 16-face patch.  Fused against separate, bit for bit.
+    cases.ramp(*dims, max_step=40, seed=4)
+    prep.config.dt_init = 2.0e-6
+    job = _vary_state(JobData.from_prep(), 4, 1486.8)
+    prob = Problem([job], nsteps=6, C=4, mpt=[[5.0, 2.0e3, 220.0]])
+    assert prob.sim.gather_form == 0
+    L, g, m = prob.gradient()
+    prob.sim.set_gather_form(True)
+    L2, g2, m2 = prob.gradient()
+    assert L == L2 and torch.equal(g, g2) and torch.equal(m[0], m2[0])
+    assert bool(torch.any(g != 0)) and bool(torch.all(m[0] != 0))
+    prob.sim.close()
+
+
 def _inflow_z(mpt):
     """(Mach, p, T) -> the inflow state of a stream along +z."""
     w = adjoint.inflow_from_mpt(mpt)
