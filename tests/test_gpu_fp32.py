@@ -63,6 +63,17 @@ def test_fp32_launch_options_and_split_calls(launch):
                 splits=[1, 9, 14])
 
 
+@pytest.mark.parametrize("name", ["uneven_array", "ramp_mid"])
+def test_fp32_predictor_corrector_matches_oracle(name):
+    # The two-stage predictor-corrector (time_scheme 1) in float numbers.
+    if name == "uneven_array":
+        job = cases.ramp_array(seed=3, lo=4, hi=9, max_step=22, i_walls=True)
+    else:
+        job = cases.ramp(64, 32, 32, max_step=25)
+    job.time_scheme = 1
+    compare_f32(job, job.max_step)
+
+
 def test_fp32_reacting_matches_oracle():
     # The FP32 build's own Cv = R/(g - 1.0) in float g and R; the prep's 1435 is outside its
     # 1e-4 gas-model check (test_abi.py), as it is for the reference's FP32 build.
